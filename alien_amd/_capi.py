@@ -1,0 +1,294 @@
+"""ctypes binding of libalien_hip.so (the C ABI declared in include/alien_hip.h).
+
+The library is built in-tree by ``python -m alien_amd.build`` (or
+``__graft_entry__.build()``). Loading fails loudly when it is missing: there is
+no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG / "libalien_hip.so"
+
+c_int_p = C.POINTER(C.c_int)
+c_double_p = C.POINTER(C.c_double)
+c_float_p = C.POINTER(C.c_float)
+c_u8_p = C.POINTER(C.c_uint8)
+c_void_p = C.c_void_p
+
+# enums (include/alien_hip.h)
+GEN_POISSON7, GEN_CONVDIFF7, GEN_LOGNORMAL7 = 0, 1, 2
+CG, BICGS = 0, 1
+PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE = 0, 1, 2, 3
+LAYOUT_CSR, LAYOUT_BELLPACK, LAYOUT_RBCSR = 0, 1, 2
+MODE_REFERENCE, MODE_FUSED, MODE_GRAPH, MODE_MIXED = 0, 1, 2, 3
+EINVAL = -22
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("solver", C.c_int), ("precond", C.c_int), ("layout", C.c_int), ("mode", C.c_int),
+                ("tol", C.c_double), ("max_iter", C.c_int), ("poly_factor", C.c_double),
+                ("poly_order", C.c_int), ("poly_max_iter", C.c_int), ("ilu_schedule", C.c_int),
+                ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("max_outer", C.c_int),
+                ("poll_every", C.c_int)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("error", C.c_int), ("outer", C.c_int),
+                ("value", C.c_double), ("true_rel_res", C.c_double), ("cheb_alpha", C.c_double),
+                ("cheb_beta", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# (name, restype, argtypes)
+_SIGS = [
+    ("alien_hip_last_error", C.c_char_p, []),
+    ("alien_hip_version", C.c_char_p, []),
+    ("alien_hip_ctx_create", C.c_int, [c_void_p, C.POINTER(c_void_p)]),
+    ("alien_hip_ctx_destroy", C.c_int, [c_void_p]),
+    ("alien_hip_ctx_synchronize", C.c_int, [c_void_p]),
+    ("alien_hip_ctx_stream", c_void_p, [c_void_p]),
+    ("alien_hip_gen_nnz", C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("alien_hip_gen_csr", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, c_double_p, C.c_uint64, c_int_p, c_int_p,
+                                    c_double_p]),
+    ("alien_hip_gen_bsr2", C.c_int, [C.c_int, C.c_int, C.c_int, c_double_p, C.c_uint64, c_int_p, c_int_p,
+                                     c_double_p]),
+    ("alien_hip_fill_uniform01", C.c_int, [C.c_uint64, C.c_int64, c_double_p]),
+    ("alien_hip_host_dcol", C.c_int, [C.c_int, c_int_p, c_int_p, c_int_p]),
+    ("alien_hip_host_bellpack_bro", C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p]),
+    ("alien_hip_host_bellpack_fill", C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_int_p, c_int_p,
+                                               c_double_p, c_u8_p, c_u8_p]),
+    ("alien_hip_host_levels", C.c_int, [C.c_int, c_int_p, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p]),
+    ("alien_hip_host_color", C.c_int, [C.c_int, c_int_p, c_int_p, c_int_p, c_int_p]),
+    ("alien_hip_rbcsr_bundle_offsets", C.c_int, [C.c_int, c_int_p, c_int_p]),
+    ("alien_hip_spmv_csr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_csr_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_bellpack_f64", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p]),
+    ("alien_hip_addmult_bellpack_f64", C.c_int, [c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_addmult_csr_f64", C.c_int, [c_void_p, C.c_int, C.c_double, C.c_int, c_void_p, c_void_p, c_void_p,
+                                            c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_bsr2_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_bsr2_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_invdiag_csr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_invdiag_bellpack_f64", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p,
+                                                 c_void_p]),
+    ("alien_hip_rbcsr_fill_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    ("alien_hip_rbcsr_fill_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    ("alien_hip_spmv_rbcsr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_rbcsr_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_copy_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p]),
+    ("alien_hip_axpy_f64", C.c_int, [c_void_p, C.c_int, C.c_double, c_void_p, c_void_p]),
+    ("alien_hip_scal_f64", C.c_int, [c_void_p, C.c_int, C.c_double, c_void_p]),
+    ("alien_hip_assign_f64", C.c_int, [c_void_p, C.c_int, C.c_double, c_void_p]),
+    ("alien_hip_pointwise_mult_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_dot_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_dot_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_axpy_dot_f64", C.c_int, [c_void_p, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_dot_csr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                             c_void_p]),
+    ("alien_hip_ilu0_factor_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, C.c_int,
+                                            c_int_p, c_void_p]),
+    ("alien_hip_ilu0_solve_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, C.c_int,
+                                           c_int_p, c_void_p, C.c_int, c_int_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    ("alien_hip_solve_csr", C.c_int, [c_void_p, C.POINTER(SolveParams), C.c_int, c_int_p, c_int_p, c_double_p,
+                                      c_double_p, c_double_p, C.POINTER(SolveResult)]),
+]
+
+EXPORTED = [s[0] for s in _SIGS]
+
+_lib = None
+
+
+class AlienHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """Load libalien_hip.so (built in-tree). Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m alien_amd.build` first")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != 0:
+        raise AlienHipError(rc, lib().alien_hip_last_error().decode())
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_int_p)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_double_p)
+
+
+def u8ptr(a: np.ndarray):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_u8_p)
+
+
+# ---- host generators and builders (no GPU) -----------------------------------------
+def gen_nnz(kind: int, nx: int, ny: int | None = None, nz: int | None = None) -> int:
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return int(lib().alien_hip_gen_nnz(kind, nx, ny, nz))
+
+
+def gen_csr(kind: int, nx: int, ny: int | None = None, nz: int | None = None, params=None, seed: int = 42):
+    """Generate one of the synthetic 7-point systems as host CSR (int32/float64)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = nx * ny * nz
+    nnz = gen_nnz(kind, nx, ny, nz)
+    ro = np.empty(n + 1, np.int32)
+    cols = np.empty(nnz, np.int32)
+    vals = np.empty(nnz, np.float64)
+    par = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    check(lib().alien_hip_gen_csr(kind, nx, ny, nz, None if par is None else dptr(par), seed, iptr(ro), iptr(cols),
+                                  dptr(vals)))
+    return ro, cols, vals
+
+
+def gen_bsr2(nx: int, ny: int | None = None, nz: int | None = None, params=None, seed: int = 42):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = nx * ny * nz
+    nnzb = gen_nnz(GEN_POISSON7, nx, ny, nz)
+    ro = np.empty(n + 1, np.int32)
+    cols = np.empty(nnzb, np.int32)
+    vals = np.empty(4 * nnzb, np.float64)
+    par = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    check(lib().alien_hip_gen_bsr2(nx, ny, nz, None if par is None else dptr(par), seed, iptr(ro), iptr(cols),
+                                   dptr(vals)))
+    return ro, cols, vals
+
+
+def uniform01(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    check(lib().alien_hip_fill_uniform01(seed, n, dptr(out)))
+    return out
+
+
+def host_dcol(ro, cols):
+    n = len(ro) - 1
+    d = np.empty(n, np.int32)
+    check(lib().alien_hip_host_dcol(n, iptr(ro), iptr(cols), iptr(d)))
+    return d
+
+
+def bellpack_build(ro, cols, vals=None, slice_=1024, masks=False):
+    n = len(ro) - 1
+    ns = (n + slice_ - 1) // slice_
+    bro = np.empty(ns + 1, np.int32)
+    check(lib().alien_hip_host_bellpack_bro(n, slice_, iptr(ro), iptr(bro)))
+    total = int(bro[ns]) * slice_
+    bc = np.empty(total, np.int32)
+    bv = np.empty(total, np.float64) if vals is not None else None
+    lm = np.empty(total, np.uint8) if masks else None
+    um = np.empty(total, np.uint8) if masks else None
+    check(lib().alien_hip_host_bellpack_fill(n, slice_, iptr(ro), iptr(cols),
+                                             None if vals is None else dptr(vals), iptr(bro), iptr(bc),
+                                             None if bv is None else dptr(bv), None if lm is None else u8ptr(lm),
+                                             None if um is None else u8ptr(um)))
+    return bro, bc, bv, lm, um
+
+
+def host_levels(ro, cols, dcol, upper: bool):
+    n = len(ro) - 1
+    lv = np.empty(n, np.int32)
+    nl = C.c_int(0)
+    check(lib().alien_hip_host_levels(n, iptr(ro), iptr(cols), iptr(dcol), 1 if upper else 0, iptr(lv),
+                                      C.byref(nl)))
+    return lv, nl.value
+
+
+def host_color(ro, cols):
+    n = len(ro) - 1
+    c = np.empty(n, np.int32)
+    nc = C.c_int(0)
+    check(lib().alien_hip_host_color(n, iptr(ro), iptr(cols), iptr(c), C.byref(nc)))
+    return c, nc.value
+
+
+def rbcsr_bundle_offsets(ro):
+    n = len(ro) - 1
+    bo = np.empty((n + 63) // 64 + 1, np.int32)
+    check(lib().alien_hip_rbcsr_bundle_offsets(n, iptr(ro), iptr(bo)))
+    return bo
+
+
+def group_by_level(level: np.ndarray, nlevels: int):
+    """Rows grouped by level, ascending inside a level; returns (ptr, rows)."""
+    order = np.argsort(level, kind="stable").astype(np.int32)
+    counts = np.bincount(level, minlength=nlevels)
+    ptr = np.zeros(nlevels + 1, np.int32)
+    ptr[1:] = np.cumsum(counts)
+    return ptr, order
+
+
+# ---- device context ----------------------------------------------------------------------
+class Context:
+    """A HIP stream + reduction workspace of the backend (alien_hip_ctx)."""
+
+    def __init__(self, stream: int | None = None):
+        h = c_void_p()
+        check(lib().alien_hip_ctx_create(c_void_p(stream) if stream else None, C.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        check(lib().alien_hip_ctx_synchronize(self.h))
+
+    @property
+    def stream(self) -> int:
+        return lib().alien_hip_ctx_stream(self.h)
+
+    def close(self):
+        if self.h:
+            check(lib().alien_hip_ctx_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def solve_csr(ctx: Context, ro, cols, vals, b, x0=None, *, solver=CG, precond=PREC_CHEB, layout=LAYOUT_CSR,
+              mode=MODE_REFERENCE, tol=1e-8, max_iter=1000, poly_factor=0.5, poly_order=3, poly_max_iter=10,
+              ilu_schedule=0, inner_tol=1e-4, inner_max_iter=0, max_outer=50, poll_every=8):
+    """Run one Krylov solve through the hip backend; returns (x, result dict)."""
+    n = len(ro) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    p = SolveParams(solver, precond, layout, mode, tol, max_iter, poly_factor, poly_order, poly_max_iter,
+                    ilu_schedule, inner_tol, inner_max_iter, max_outer, poll_every)
+    r = SolveResult()
+    check(lib().alien_hip_solve_csr(ctx.h, C.byref(p), n, iptr(ro), iptr(cols), dptr(vals), dptr(b), dptr(x),
+                                    C.byref(r)))
+    return x, r.as_dict()

@@ -1,0 +1,400 @@
+// The `hip` backend of Alien: backend tag, AlgebraTraits, vector, matrix and
+// the internal linear algebra whose method set is the duck type that Alien's
+// Krylov templates call (src/core/alien/expression/krylov/*.h).
+//
+// Mirrors:
+//   tag + traits       SYCLBackEnd.h:66-97 / SimpleCSRBackEnd.h:62-94
+//   vector             SYCLVector (sycl/data/SYCLVector.h:47-150), SimpleCSRVector.h:96-163
+//   matrix             SYCLBEllPackMatrix (sycl/data/SYCLBEllPackMatrix.h:63-225)
+//   algebra            SYCLInternalLinearAlgebra (sycl/algebra/SYCLInternalLinearAlgebra.h:46-141)
+//
+// Outside a full Alien build there is no MultiMatrixImpl to own the matrix, so
+// HIPMatrix is built directly from a host CSR view (what the SimpleCSR -> hip
+// converter does inside Alien; the converter code is in INTEGRATION.md).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../alien_hip_internal.h"
+#include "../../kernels/launch.h"
+#include "HIPContext.h"
+
+namespace Alien {
+
+namespace BackEnd {
+namespace tag {
+struct hip {};
+} // namespace tag
+} // namespace BackEnd
+
+template <typename T> class HIPVectorT;
+class HIPMatrix;
+class HIPInternalLinearAlgebra;
+
+#ifndef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+// Alien declares this primary template in alien/core/backend/BackEnd.h.
+template <typename Tag> struct AlgebraTraits;
+#endif
+
+template <>
+struct AlgebraTraits<BackEnd::tag::hip> {
+  typedef HIPMatrix matrix_type;
+  typedef HIPVectorT<double> vector_type;
+  typedef HIPInternalLinearAlgebra algebra_type;
+  typedef double value_type;
+  static const char* name() { return "hip"; }
+};
+
+// "Resource" of a vector: its size and the context that owns its stream
+// (Alien uses the row VectorDistribution, SimpleCSRInternalLinearAlgebra.cc:74-78).
+struct HIPResource {
+  int n = 0;
+  alien_hip_ctx* ctx = nullptr;
+};
+
+template <typename T>
+class HIPVectorT {
+ public:
+  typedef T ValueType;
+  static const bool on_host_only = false;
+
+  HIPVectorT() = default;
+  HIPVectorT(const HIPResource& r) { init(r, true); }
+
+  // zero-filled like SimpleCSRVector::init (SimpleCSRVector.h:111-131)
+  void init(const HIPResource& r, bool need_alloc) {
+    m_ctx = r.ctx;
+    m_n = r.n;
+    if (need_alloc) {
+      m_data.resize((size_t)m_n);
+      if (m_n) ALIEN_HIP_CHECK(hipMemsetAsync(m_data.data(), 0, sizeof(T) * m_n, m_ctx->stream));
+    }
+  }
+  void clear() {
+    m_data.reset();
+    m_n = 0;
+  }
+  int scalarizedLocalSize() const { return m_n; }
+  int getAllocSize() const { return m_n; }
+  T* getDataPtr() { return m_data.data(); }
+  const T* getDataPtr() const { return m_data.data(); }
+  T* data() { return m_data.data(); }
+  const T* data() const { return m_data.data(); }
+  alien_hip_ctx* ctx() const { return m_ctx; }
+  HIPResource resource() const { return HIPResource{m_n, m_ctx}; }
+
+  void setValues(const T* host) {
+    if (m_n) ALIEN_HIP_CHECK(hipMemcpyAsync(m_data.data(), host, sizeof(T) * m_n, hipMemcpyHostToDevice, m_ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
+  }
+  void copyValuesTo(T* host) const {
+    if (m_n) ALIEN_HIP_CHECK(hipMemcpyAsync(host, m_data.data(), sizeof(T) * m_n, hipMemcpyDeviceToHost, m_ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
+  }
+  // y[i] = f(i) evaluated on the host, in index order, then uploaded: the
+  // order of the host-side random stream is the reference's
+  // (SYCLVector::apply, sycl/data/SYCLVector.h:145-150).
+  template <typename Lambda>
+  void apply(const Lambda& f) {
+    std::vector<T> h(m_n);
+    for (int i = 0; i < m_n; ++i) h[i] = f(i);
+    setValues(h.data());
+  }
+
+ private:
+  alien_hip::DeviceArray<T> m_data;
+  int m_n = 0;
+  alien_hip_ctx* m_ctx = nullptr;
+};
+typedef HIPVectorT<double> HIPVector;
+
+// Host CSR input (Alien's SimpleCSR profile + values, ascending columns).
+struct CSRHostView {
+  int nrows = 0;
+  const int* row_offset = nullptr;
+  const int* cols = nullptr;
+  const double* vals = nullptr;
+};
+
+enum HIPLayout { kLayoutCSR = ALIEN_HIP_LAYOUT_CSR, kLayoutBEllPack = ALIEN_HIP_LAYOUT_BELLPACK,
+                 kLayoutRBCSR = ALIEN_HIP_LAYOUT_RBCSR };
+
+class HIPMatrix {
+ public:
+  typedef double ValueType;
+  typedef BackEnd::tag::hip TagType;
+  static const bool on_host_only = false;
+  static constexpr int kBEllPackSlice = 1024; // BEllPackStructInfo<1024,int>
+
+  HIPMatrix(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR) : m_ctx(ctx) {
+    m_n = a.nrows;
+    m_nnz = a.row_offset[m_n];
+    m_kcol.assign(a.row_offset, a.row_offset + m_n + 1);
+    m_cols.assign(a.cols, a.cols + m_nnz);
+    m_vals.assign(a.vals, a.vals + m_nnz);
+    m_dcol.resize(m_n);
+    alien_hip::host_dcol(m_n, m_kcol.data(), m_cols.data(), m_dcol.data());
+    hipStream_t st = m_ctx->stream;
+    d_ro.upload(m_kcol.data(), m_kcol.size(), st);
+    d_cols.upload(m_cols.data(), m_cols.size(), st);
+    d_dcol.upload(m_dcol.data(), m_dcol.size(), st);
+    d_vals.upload(m_vals.data(), m_vals.size(), st);
+    setSpmvLayout(spmv_layout);
+    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+  }
+
+  int getNRows() const { return m_n; }
+  int getNnz() const { return m_nnz; }
+  bool isParallel() const { return false; }
+  HIPResource resource() const { return HIPResource{m_n, m_ctx}; }
+  alien_hip_ctx* ctx() const { return m_ctx; }
+  int spmvLayout() const { return m_layout; }
+
+  // host profile (CSRStructInfo: kcol/cols/dcol)
+  const int* kcol() const { return m_kcol.data(); }
+  const int* cols() const { return m_cols.data(); }
+  const int* dcol() const { return m_dcol.data(); }
+  const double* hostValues() const { return m_vals.data(); }
+
+  // Builds the layout on first use and selects it for mult().
+  void setSpmvLayout(int layout) {
+    if (layout == kLayoutRBCSR) ensureRBCSR();
+    else if (layout == kLayoutBEllPack) ensureBEllPack(false);
+    else if (layout != kLayoutCSR) throw alien_hip::Error(ALIEN_HIP_EINVAL, "unknown SpMV layout");
+    m_layout = layout;
+  }
+
+  void ensureRBCSR() {
+    if (!d_bo.empty()) return;
+    const int nb = (m_n + alien_hip::kBundle - 1) / alien_hip::kBundle;
+    m_bo.resize(nb + 1);
+    if (alien_hip::rbcsr_bundle_offsets(m_n, m_kcol.data(), m_bo.data()) != 0)
+      throw alien_hip::Error(ALIEN_HIP_EINVAL, "RB-CSR storage exceeds int32 indexing");
+    hipStream_t st = m_ctx->stream;
+    d_bo.upload(m_bo.data(), m_bo.size(), st);
+    const size_t slots = (size_t)m_bo[nb] * alien_hip::kBundle;
+    d_bc.resize(slots);
+    d_bv.resize(slots);
+    alien_hip::launch_rbcsr_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
+                                         d_bc.data(), d_bv.data());
+    alien_hip::check_launch("rbcsr_fill");
+  }
+  void ensureRBCSR32() {
+    ensureRBCSR();
+    if (!d_bv32.empty()) return;
+    d_bv32.resize(d_bv.size());
+    // the same fill with float values (columns are rewritten identically)
+    alien_hip::launch_rbcsr_fill<float>(m_ctx->stream, m_n, d_ro.data(), d_cols.data(), d_vals.data(),
+                                        d_bo.data(), d_bc.data(), d_bv32.data());
+    alien_hip::check_launch("rbcsr_fill<float>");
+  }
+  void ensureBEllPack(bool masks) {
+    if (d_bro.empty()) {
+      const int ns = alien_hip::bellpack_nslices(m_n, kBEllPackSlice);
+      m_bro.resize(ns + 1);
+      alien_hip::bellpack_bro(m_n, kBEllPackSlice, m_kcol.data(), m_bro.data());
+      const size_t total = (size_t)m_bro[ns] * kBEllPackSlice;
+      std::vector<int> bc(total);
+      std::vector<double> bv(total);
+      alien_hip::bellpack_fill(m_n, kBEllPackSlice, m_kcol.data(), m_cols.data(), m_vals.data(), m_bro.data(),
+                               bc.data(), bv.data(), nullptr, nullptr);
+      hipStream_t st = m_ctx->stream;
+      d_bro.upload(m_bro.data(), m_bro.size(), st);
+      d_bellc.upload(bc.data(), bc.size(), st);
+      d_bellv.upload(bv.data(), bv.size(), st);
+      ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    if (masks && d_lmask.empty()) {
+      const int ns = alien_hip::bellpack_nslices(m_n, kBEllPackSlice);
+      const size_t total = (size_t)m_bro[ns] * kBEllPackSlice;
+      std::vector<int> bc(total);
+      std::vector<uint8_t> lm(total), um(total);
+      alien_hip::bellpack_fill(m_n, kBEllPackSlice, m_kcol.data(), m_cols.data(), nullptr, m_bro.data(),
+                               bc.data(), nullptr, lm.data(), um.data());
+      d_lmask.upload(lm.data(), lm.size(), m_ctx->stream);
+      d_umask.upload(um.data(), um.size(), m_ctx->stream);
+      ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
+    }
+  }
+
+  // device arrays (public for the kernels' callers inside the backend)
+  alien_hip::DeviceArray<int> d_ro, d_cols, d_dcol;
+  alien_hip::DeviceArray<double> d_vals;
+  alien_hip::DeviceArray<int> d_bo, d_bc;        // RB-CSR
+  alien_hip::DeviceArray<double> d_bv;
+  alien_hip::DeviceArray<float> d_bv32;
+  alien_hip::DeviceArray<int> d_bro, d_bellc;    // BEllPack
+  alien_hip::DeviceArray<double> d_bellv;
+  alien_hip::DeviceArray<uint8_t> d_lmask, d_umask;
+
+ private:
+  alien_hip_ctx* m_ctx;
+  int m_n = 0, m_nnz = 0;
+  int m_layout = kLayoutCSR;
+  std::vector<int> m_kcol, m_cols, m_dcol, m_bo, m_bro;
+  std::vector<double> m_vals;
+};
+
+// Future of a device-resident scalar: dot(x, y, f) leaves the result on the
+// device; get() copies it back and synchronises (SYCLInternal::Future,
+// sycl/algebra/SYCLKernelInternal.h:32-84).
+class HIPFuture {
+ public:
+  explicit HIPFuture(double& v) : m_value(v) {}
+  double& operator()() { return m_value; }
+  double operator()() const { return m_value; }
+  void attach(alien_hip_ctx* c, int slot) {
+    m_ctx = c;
+    m_slot = slot;
+    m_pending = true;
+  }
+  double get() {
+    if (m_pending) {
+      m_value = m_ctx->read_slot(m_slot);
+      m_pending = false;
+    }
+    return m_value;
+  }
+
+ private:
+  double& m_value;
+  alien_hip_ctx* m_ctx = nullptr;
+  int m_slot = 0;
+  bool m_pending = false;
+};
+
+class HIPInternalLinearAlgebra {
+ public:
+  typedef BackEnd::tag::hip BackEndType;
+  typedef HIPMatrix Matrix;
+  typedef HIPVector Vector;
+  typedef HIPResource ResourceType;
+  typedef HIPFuture FutureType;
+
+  class NullValueException : public std::runtime_error {
+   public:
+    explicit NullValueException(const std::string& what) : std::runtime_error(what) {}
+  };
+  class NotImplementedException : public std::runtime_error {
+   public:
+    explicit NotImplementedException(const std::string& what) : std::runtime_error(what) {}
+  };
+
+  explicit HIPInternalLinearAlgebra(alien_hip_ctx* ctx = nullptr) : m_ctx(ctx ? ctx : alien_hip::default_ctx()) {}
+
+  alien_hip_ctx* ctx() const { return m_ctx; }
+  hipStream_t stream() const { return m_ctx->stream; }
+
+  static ResourceType resource(const Matrix& A) { return A.resource(); }
+  void allocate(const ResourceType& r, Vector& v) { v.init(r, true); }
+  template <typename T0, typename... T>
+  void allocate(const ResourceType& r, T0& v0, T&... args) {
+    allocate(r, v0);
+    allocate(r, args...);
+  }
+  void free(Vector& v) { v.clear(); }
+  template <typename T0, typename... T>
+  void free(T0& v0, T&... args) {
+    free(v0);
+    free(args...);
+  }
+
+  // ---- IInternalLinearAlgebra (IInternalLinearAlgebraT.h:69-186) ----
+  double norm0(const Vector&) const { throw NotImplementedException("HIPInternalLinearAlgebra::norm0"); }
+  double norm1(const Vector&) const { throw NotImplementedException("HIPInternalLinearAlgebra::norm1"); }
+  double norm2(const Vector& x) const { return std::sqrt(dot(x, x)); } // sqrt(dot), CBLASMPIKernel.h:114-125
+
+  void mult(const Matrix& A, const Vector& x, Vector& y) const {
+    switch (A.spmvLayout()) {
+    case kLayoutRBCSR:
+      alien_hip::launch_spmv_rbcsr<double>(stream(), A.getNRows(), A.d_bo.data(), A.d_bc.data(), A.d_bv.data(),
+                                           x.data(), y.data());
+      break;
+    case kLayoutBEllPack:
+      alien_hip::launch_spmv_bellpack<double>(stream(), A.getNRows(), HIPMatrix::kBEllPackSlice, A.d_bro.data(),
+                                              A.d_bellc.data(), A.d_bellv.data(), x.data(), y.data());
+      break;
+    default:
+      alien_hip::launch_spmv_csr<double>(stream(), A.getNRows(), A.d_ro.data(), A.d_cols.data(), A.d_vals.data(),
+                                         x.data(), y.data());
+    }
+    alien_hip::check_launch("mult");
+  }
+  void addLMult(double alpha, const Matrix& A, const Vector& x, Vector& y) const {
+    alien_hip::launch_addmult_csr(stream(), A.getNRows(), 0, alpha, A.d_ro.data(), A.d_cols.data(), A.d_dcol.data(),
+                                  A.d_vals.data(), x.data(), y.data());
+    alien_hip::check_launch("addLMult");
+  }
+  void addUMult(double alpha, const Matrix& A, const Vector& x, Vector& y) const {
+    alien_hip::launch_addmult_csr(stream(), A.getNRows(), 1, alpha, A.d_ro.data(), A.d_cols.data(), A.d_dcol.data(),
+                                  A.d_vals.data(), x.data(), y.data());
+    alien_hip::check_launch("addUMult");
+  }
+  // d_i = 1 / a[dcol_i] (SimpleCSRMatrixMultT::computeInvDiag, :494-509)
+  void computeInvDiag(const Matrix& A, Vector& d) const {
+    alien_hip::launch_invdiag_csr(stream(), A.getNRows(), A.d_dcol.data(), A.d_vals.data(), d.data());
+    alien_hip::check_launch("computeInvDiag");
+  }
+  // y_i = y_i / a[dcol_i] (SimpleCSRMatrixMultT::multInvDiag, :478-492)
+  void multInvDiag(const Matrix& A, Vector& y) const {
+    Vector d;
+    d.init(A.resource(), true);
+    computeInvDiag(A, d);
+    pointwiseMult(d, y, y);
+  }
+
+  void axpy(double a, const Vector& x, Vector& y) const {
+    alien_hip::launch_axpy<double>(stream(), x.scalarizedLocalSize(), a, x.data(), y.data());
+    alien_hip::check_launch("axpy");
+  }
+  void aypx(double, Vector&, const Vector&) const { throw NotImplementedException("HIPInternalLinearAlgebra::aypx"); }
+  void copy(const Vector& x, Vector& y) const {
+    alien_hip::launch_copy<double>(stream(), x.scalarizedLocalSize(), x.data(), y.data());
+    alien_hip::check_launch("copy");
+  }
+  double dot(const Vector& x, const Vector& y) const {
+    const int s = m_ctx->take_slot();
+    alien_hip::launch_dot<double>(stream(), m_ctx->red(), x.scalarizedLocalSize(), x.data(), y.data(), m_ctx->dslot(s));
+    alien_hip::check_launch("dot");
+    return m_ctx->read_slot(s);
+  }
+  void dot(const Vector& x, const Vector& y, FutureType& f) const {
+    const int s = m_ctx->take_slot();
+    alien_hip::launch_dot<double>(stream(), m_ctx->red(), x.scalarizedLocalSize(), x.data(), y.data(), m_ctx->dslot(s));
+    alien_hip::check_launch("dot(future)");
+    f.attach(m_ctx, s);
+  }
+  void scal(double a, Vector& x) const {
+    alien_hip::launch_scal<double>(stream(), x.scalarizedLocalSize(), a, x.data());
+    alien_hip::check_launch("scal");
+  }
+  void diagonal(const Matrix&, Vector&) const { throw NotImplementedException("HIPInternalLinearAlgebra::diagonal"); }
+  void reciprocal(Vector&) const { throw NotImplementedException("HIPInternalLinearAlgebra::reciprocal"); }
+  void pointwiseMult(const Vector& x, const Vector& y, Vector& z) const {
+    alien_hip::launch_pmult<double>(stream(), x.scalarizedLocalSize(), x.data(), y.data(), z.data());
+    alien_hip::check_launch("pointwiseMult");
+  }
+  void assign(Vector& x, double a) const {
+    alien_hip::launch_assign<double>(stream(), x.scalarizedLocalSize(), a, x.data());
+    alien_hip::check_launch("assign");
+  }
+  template <typename LambdaT>
+  void assign(Vector& x, const LambdaT& lambda) const {
+    x.apply(lambda);
+  }
+  template <typename PrecondT>
+  void exec(PrecondT& precond, const Vector& x, Vector& y) {
+    precond.solve(*this, x, y);
+  }
+
+ private:
+  alien_hip_ctx* m_ctx;
+};
+
+} // namespace Alien

@@ -1,0 +1,112 @@
+// HIP execution context of the backend: one stream, the workspace of the
+// one-pass reductions, a ring of device-resident scalar slots and a pinned
+// host mirror used to read scalars back. Plays the part of the SYCL queue
+// holder of Alien's SYCL backend (sycl/algebra/SYCLKernelInternal.h:32-84).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "../../kernels/device_common.h"
+
+namespace alien_hip {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define ALIEN_HIP_CHECK(expr)                                                                 \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess)                                                                     \
+      throw ::alien_hip::Error((int)_e, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+  } while (0)
+
+// Check of the most recent launch(es) on this thread.
+inline void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error((int)e, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Owning device array.
+template <typename T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(size_t n) { resize(n); }
+  ~DeviceArray() { reset(); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : m_p(o.m_p), m_n(o.m_n) { o.m_p = nullptr; o.m_n = 0; }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) { reset(); m_p = o.m_p; m_n = o.m_n; o.m_p = nullptr; o.m_n = 0; }
+    return *this;
+  }
+  void resize(size_t n) {
+    reset();
+    if (n) ALIEN_HIP_CHECK(hipMalloc((void**)&m_p, n * sizeof(T)));
+    m_n = n;
+  }
+  void reset() {
+    if (m_p) (void)hipFree(m_p);
+    m_p = nullptr;
+    m_n = 0;
+  }
+  // Synchronous upload: the host buffer may be reused or freed on return.
+  void upload(const T* host, size_t n, hipStream_t st) {
+    if (n != m_n) resize(n);
+    if (n) {
+      ALIEN_HIP_CHECK(hipMemcpyAsync(m_p, host, n * sizeof(T), hipMemcpyHostToDevice, st));
+      ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+    }
+  }
+  T* data() { return m_p; }
+  const T* data() const { return m_p; }
+  size_t size() const { return m_n; }
+  bool empty() const { return m_n == 0; }
+
+ private:
+  T* m_p = nullptr;
+  size_t m_n = 0;
+};
+
+} // namespace alien_hip
+
+// The context object handed through the C ABI.
+struct alien_hip_ctx {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  alien_hip::DeviceArray<double> partials; // kRedBlocks * kMaxSlots
+  alien_hip::DeviceArray<unsigned> ticket;
+  alien_hip::DeviceArray<double> scalars;  // device-resident scalar slots
+  double* host_scalars = nullptr;           // pinned mirror of `scalars`
+  int next_slot = 0;
+  static constexpr int kSlots = 64;
+
+  alien_hip::RedWork red() { return alien_hip::RedWork{partials.data(), ticket.data()}; }
+  // A fresh device scalar slot (ring of kSlots; callers keep at most a few live).
+  int take_slot() {
+    int s = next_slot;
+    next_slot = (next_slot + 1) % kSlots;
+    return s;
+  }
+  double* dslot(int s) { return scalars.data() + s; }
+  // Copy one slot to the host (synchronises the stream).
+  double read_slot(int s) {
+    ALIEN_HIP_CHECK(hipMemcpyAsync(host_scalars + s, scalars.data() + s, sizeof(double),
+                                   hipMemcpyDeviceToHost, stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(stream));
+    return host_scalars[s];
+  }
+};
+
+namespace alien_hip {
+alien_hip_ctx* default_ctx();
+alien_hip_ctx* make_ctx(void* stream);
+void destroy_ctx(alien_hip_ctx* c);
+} // namespace alien_hip

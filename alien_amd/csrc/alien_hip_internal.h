@@ -1,0 +1,41 @@
+// Internal declarations shared by the host code and the HIP sources of
+// libalien_hip.so. Not part of the public C ABI (that is include/alien_hip.h).
+#pragma once
+
+#include "../../include/alien_hip.h"
+
+#include <cstdint>
+#include <string>
+
+namespace alien_hip {
+
+// ---- error reporting (thread-local last error, see alien_hip_last_error) ----
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+// ---- host generators (host/generators.cpp) ----
+uint64_t splitmix64_next(uint64_t& s);
+double splitmix64_u01(uint64_t& s);
+void fill_uniform01(uint64_t seed, int64_t n, double* out);
+void fill_normal(uint64_t seed, int64_t n, double* out);
+int64_t gen_nnz(int kind, int nx, int ny, int nz);
+int gen_csr(int kind, int nx, int ny, int nz, const double* params, uint64_t seed, int* ro,
+            int* cols, double* vals);
+int gen_bsr2(int nx, int ny, int nz, const double* params, uint64_t seed, int* ro, int* cols,
+             double* vals);
+
+// ---- host structure builders (host/structure.cpp) ----
+void host_dcol(int nrows, const int* ro, const int* cols, int* dcol);
+int bellpack_nslices(int nrows, int slice);
+void bellpack_bro(int nrows, int slice, const int* ro, int* bro);
+void bellpack_fill(int nrows, int slice, const int* ro, const int* cols, const double* vals,
+                   const int* bro, int* bcols, double* bvals, uint8_t* lmask, uint8_t* umask);
+int host_levels(int nrows, const int* ro, const int* cols, const int* dcol, int upper, int* level);
+int host_color(int nrows, const int* ro, const int* cols, int* color);
+// rows grouped by level (stable: ascending row inside a level); ptr has nlevels+1 entries
+void group_by_level(int nrows, const int* level, int nlevels, int* ptr, int* rows);
+
+constexpr int kBundle = 64; // rows per RB-CSR bundle = one wavefront
+int rbcsr_bundle_offsets(int nrows, const int* ro, int* bundle_off);
+
+} // namespace alien_hip

@@ -1,0 +1,434 @@
+// C ABI of libalien_hip.so (declarations and reference citations in
+// include/alien_hip.h). Each entry point validates what the host can see,
+// launches on the context's stream and converts C++ exceptions into status
+// codes plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "alien/hip/HIPBackEnd.h"
+#include "alien_hip_internal.h"
+#include "kernels/device_common.h"
+#include "kernels/launch.h"
+
+namespace alien_hip {
+
+static thread_local std::string t_last_error;
+
+void set_error(const std::string& msg) { t_last_error = msg; }
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+alien_hip_ctx* make_ctx(void* stream) {
+  auto* c = new alien_hip_ctx();
+  try {
+    if (stream) {
+      c->stream = (hipStream_t)stream;
+    } else {
+      ALIEN_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      c->own_stream = true;
+    }
+    c->partials.resize((size_t)kRedBlocks * kMaxSlots);
+    c->ticket.resize(1);
+    c->scalars.resize(alien_hip_ctx::kSlots);
+    ALIEN_HIP_CHECK(hipMemsetAsync(c->ticket.data(), 0, sizeof(unsigned), c->stream));
+    ALIEN_HIP_CHECK(hipMemsetAsync(c->scalars.data(), 0, sizeof(double) * alien_hip_ctx::kSlots, c->stream));
+    ALIEN_HIP_CHECK(hipHostMalloc((void**)&c->host_scalars, sizeof(double) * alien_hip_ctx::kSlots,
+                                  hipHostMallocDefault));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    destroy_ctx(c);
+    throw;
+  }
+  return c;
+}
+
+void destroy_ctx(alien_hip_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->partials.reset();
+  c->ticket.reset();
+  c->scalars.reset();
+  if (c->host_scalars) (void)hipHostFree(c->host_scalars);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+alien_hip_ctx* default_ctx() {
+  static alien_hip_ctx* c = nullptr;
+  static std::once_flag once;
+  std::call_once(once, [] { c = make_ctx(nullptr); });
+  return c;
+}
+
+} // namespace alien_hip
+
+using namespace alien_hip;
+
+#define API_BEGIN try {
+#define API_END                                          \
+  }                                                      \
+  catch (const alien_hip::Error& e) {                    \
+    return fail(e.code, e.what());                       \
+  }                                                      \
+  catch (const std::exception& e) {                      \
+    return fail(ALIEN_HIP_EINVAL, e.what());             \
+  }                                                      \
+  catch (...) {                                          \
+    return fail(ALIEN_HIP_EINVAL, "unknown exception");  \
+  }                                                      \
+  return 0;
+
+#define REQUIRE(cond, msg)                            \
+  do {                                                \
+    if (!(cond)) return fail(ALIEN_HIP_EINVAL, msg);  \
+  } while (0)
+
+#define LAUNCHED(what) check_launch(what)
+
+extern "C" {
+
+const char* alien_hip_last_error(void) { return t_last_error.c_str(); }
+const char* alien_hip_version(void) { return "alien_hip 0.1.0 (gfx950)"; }
+
+int alien_hip_ctx_create(void* stream, alien_hip_ctx** out) {
+  REQUIRE(out, "ctx_create: out is NULL");
+  API_BEGIN
+  *out = make_ctx(stream);
+  API_END
+}
+
+int alien_hip_ctx_destroy(alien_hip_ctx* ctx) {
+  API_BEGIN
+  destroy_ctx(ctx);
+  API_END
+}
+
+int alien_hip_ctx_synchronize(alien_hip_ctx* ctx) {
+  REQUIRE(ctx, "ctx is NULL");
+  API_BEGIN
+  ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  API_END
+}
+
+void* alien_hip_ctx_stream(alien_hip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+// ---- host generators / builders ------------------------------------------------
+int64_t alien_hip_gen_nnz(int kind, int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  (void)kind;
+  return gen_nnz(kind, nx, ny, nz);
+}
+
+int alien_hip_gen_csr(int kind, int nx, int ny, int nz, const double* params, uint64_t seed, int* ro,
+                      int* cols, double* vals) {
+  REQUIRE(ro && cols && vals, "gen_csr: NULL output");
+  REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "gen_csr: grid sizes must be >= 1");
+  REQUIRE(gen_nnz(kind, nx, ny, nz) <= 0x7fffffffLL, "gen_csr: nnz exceeds int32");
+  API_BEGIN
+  if (gen_csr(kind, nx, ny, nz, params, seed, ro, cols, vals) != 0) return fail(ALIEN_HIP_EINVAL, "gen_csr: bad kind or size");
+  API_END
+}
+
+int alien_hip_gen_bsr2(int nx, int ny, int nz, const double* params, uint64_t seed, int* ro, int* cols,
+                       double* vals) {
+  REQUIRE(ro && cols && vals, "gen_bsr2: NULL output");
+  REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "gen_bsr2: grid sizes must be >= 1");
+  API_BEGIN
+  if (gen_bsr2(nx, ny, nz, params, seed, ro, cols, vals) != 0) return fail(ALIEN_HIP_EINVAL, "gen_bsr2: bad size");
+  API_END
+}
+
+int alien_hip_fill_uniform01(uint64_t seed, int64_t n, double* out) {
+  REQUIRE(n >= 0 && (n == 0 || out), "fill_uniform01: bad arguments");
+  fill_uniform01(seed, n, out);
+  return 0;
+}
+
+int alien_hip_host_dcol(int nrows, const int* ro, const int* cols, int* dcol) {
+  REQUIRE(nrows >= 0 && ro && (nrows == 0 || (cols && dcol)), "host_dcol: bad arguments");
+  host_dcol(nrows, ro, cols, dcol);
+  return 0;
+}
+
+int alien_hip_host_bellpack_bro(int nrows, int slice, const int* ro, int* bro) {
+  REQUIRE(nrows >= 0 && slice > 0 && ro && bro, "host_bellpack_bro: bad arguments");
+  bellpack_bro(nrows, slice, ro, bro);
+  return 0;
+}
+
+int alien_hip_host_bellpack_fill(int nrows, int slice, const int* ro, const int* cols, const double* vals,
+                                 const int* bro, int* bcols, double* bvals, uint8_t* lmask, uint8_t* umask) {
+  REQUIRE(nrows >= 0 && slice > 0 && ro && bro && bcols, "host_bellpack_fill: bad arguments");
+  REQUIRE(!bvals || vals, "host_bellpack_fill: bvals requested without vals");
+  bellpack_fill(nrows, slice, ro, cols, vals, bro, bcols, bvals, lmask, umask);
+  return 0;
+}
+
+int alien_hip_host_levels(int nrows, const int* ro, const int* cols, const int* dcol, int upper, int* level,
+                          int* nlevels) {
+  REQUIRE(nrows >= 0 && ro && level && nlevels && (nrows == 0 || (cols && dcol)), "host_levels: bad arguments");
+  *nlevels = host_levels(nrows, ro, cols, dcol, upper, level);
+  return 0;
+}
+
+int alien_hip_host_color(int nrows, const int* ro, const int* cols, int* color, int* ncolors) {
+  REQUIRE(nrows >= 0 && ro && color && ncolors, "host_color: bad arguments");
+  API_BEGIN
+  *ncolors = host_color(nrows, ro, cols, color);
+  API_END
+}
+
+int alien_hip_rbcsr_bundle_offsets(int nrows, const int* ro, int* bo) {
+  REQUIRE(nrows >= 0 && ro && bo, "rbcsr_bundle_offsets: bad arguments");
+  if (rbcsr_bundle_offsets(nrows, ro, bo) != 0) return fail(ALIEN_HIP_EINVAL, "RB-CSR storage exceeds int32");
+  return 0;
+}
+
+// ---- device kernels -------------------------------------------------------------------
+#define CTX_OK(ctx) REQUIRE(ctx, "ctx is NULL")
+
+int alien_hip_spmv_csr_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                           const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && vals && x && y)), "spmv_csr_f64: bad arguments");
+  API_BEGIN
+  launch_spmv_csr<double>(ctx->stream, n, ro, cols, vals, x, y);
+  LAUNCHED("spmv_csr_f64");
+  API_END
+}
+
+int alien_hip_spmv_csr_f32(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const float* vals,
+                           const float* x, float* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && vals && x && y)), "spmv_csr_f32: bad arguments");
+  API_BEGIN
+  launch_spmv_csr<float>(ctx->stream, n, ro, cols, vals, x, y);
+  LAUNCHED("spmv_csr_f32");
+  API_END
+}
+
+int alien_hip_spmv_bellpack_f64(alien_hip_ctx* ctx, int n, int slice, const int* bro, const int* bc,
+                                const double* bv, const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && slice > 0 && (n == 0 || (bro && bc && bv && x && y)), "spmv_bellpack_f64: bad arguments");
+  API_BEGIN
+  launch_spmv_bellpack<double>(ctx->stream, n, slice, bro, bc, bv, x, y);
+  LAUNCHED("spmv_bellpack_f64");
+  API_END
+}
+
+int alien_hip_addmult_bellpack_f64(alien_hip_ctx* ctx, int upper, double alpha, int n, int slice, const int* bro,
+                                   const int* bc, const double* bv, const uint8_t* mask, const double* x,
+                                   double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && slice > 0 && (n == 0 || (bro && bc && bv && mask && x && y)),
+          "addmult_bellpack_f64: bad arguments");
+  (void)upper; // the mask selects lower or upper
+  API_BEGIN
+  launch_addmult_bellpack(ctx->stream, n, slice, alpha, bro, bc, bv, mask, x, y);
+  LAUNCHED("addmult_bellpack_f64");
+  API_END
+}
+
+int alien_hip_addmult_csr_f64(alien_hip_ctx* ctx, int upper, double alpha, int n, const int* ro, const int* cols,
+                              const int* dcol, const double* vals, const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && dcol && vals && x && y)), "addmult_csr_f64: bad arguments");
+  API_BEGIN
+  launch_addmult_csr(ctx->stream, n, upper ? 1 : 0, alpha, ro, cols, dcol, vals, x, y);
+  LAUNCHED("addmult_csr_f64");
+  API_END
+}
+
+static bool aligned(const void* p, size_t a) { return ((uintptr_t)p % a) == 0; }
+
+int alien_hip_spmv_bsr2_f64(alien_hip_ctx* ctx, int nb, const int* ro, const int* cols, const double* vals,
+                            const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(nb >= 0 && (nb == 0 || (ro && cols && vals && x && y)), "spmv_bsr2_f64: bad arguments");
+  REQUIRE(aligned(vals, 32) && aligned(x, 16) && aligned(y, 16), "spmv_bsr2_f64: misaligned block arrays");
+  API_BEGIN
+  launch_spmv_bsr2<double>(ctx->stream, nb, ro, cols, vals, x, y);
+  LAUNCHED("spmv_bsr2_f64");
+  API_END
+}
+
+int alien_hip_spmv_bsr2_f32(alien_hip_ctx* ctx, int nb, const int* ro, const int* cols, const float* vals,
+                            const float* x, float* y) {
+  CTX_OK(ctx);
+  REQUIRE(nb >= 0 && (nb == 0 || (ro && cols && vals && x && y)), "spmv_bsr2_f32: bad arguments");
+  REQUIRE(aligned(vals, 16) && aligned(x, 8) && aligned(y, 8), "spmv_bsr2_f32: misaligned block arrays");
+  API_BEGIN
+  launch_spmv_bsr2<float>(ctx->stream, nb, ro, cols, vals, x, y);
+  LAUNCHED("spmv_bsr2_f32");
+  API_END
+}
+
+int alien_hip_invdiag_csr_f64(alien_hip_ctx* ctx, int n, const int* dcol, const double* vals, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (dcol && vals && y)), "invdiag_csr_f64: bad arguments");
+  API_BEGIN
+  launch_invdiag_csr(ctx->stream, n, dcol, vals, y);
+  LAUNCHED("invdiag_csr_f64");
+  API_END
+}
+
+int alien_hip_invdiag_bellpack_f64(alien_hip_ctx* ctx, int n, int slice, const int* bro, const int* bc,
+                                   const double* bv, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && slice > 0 && (n == 0 || (bro && bc && bv && y)), "invdiag_bellpack_f64: bad arguments");
+  API_BEGIN
+  launch_invdiag_bellpack(ctx->stream, n, slice, bro, bc, bv, y);
+  LAUNCHED("invdiag_bellpack_f64");
+  API_END
+}
+
+int alien_hip_rbcsr_fill_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                             const int* bo, int* bc, double* bv) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && vals && bo && bc && bv)), "rbcsr_fill_f64: bad arguments");
+  API_BEGIN
+  launch_rbcsr_fill<double>(ctx->stream, n, ro, cols, vals, bo, bc, bv);
+  LAUNCHED("rbcsr_fill_f64");
+  API_END
+}
+
+int alien_hip_rbcsr_fill_f32(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                             const int* bo, int* bc, float* bv) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && vals && bo && bc && bv)), "rbcsr_fill_f32: bad arguments");
+  API_BEGIN
+  launch_rbcsr_fill<float>(ctx->stream, n, ro, cols, vals, bo, bc, bv);
+  LAUNCHED("rbcsr_fill_f32");
+  API_END
+}
+
+int alien_hip_spmv_rbcsr_f64(alien_hip_ctx* ctx, int n, const int* bo, const int* bc, const double* bv,
+                             const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (bo && bc && bv && x && y)), "spmv_rbcsr_f64: bad arguments");
+  API_BEGIN
+  launch_spmv_rbcsr<double>(ctx->stream, n, bo, bc, bv, x, y);
+  LAUNCHED("spmv_rbcsr_f64");
+  API_END
+}
+
+int alien_hip_spmv_rbcsr_f32(alien_hip_ctx* ctx, int n, const int* bo, const int* bc, const float* bv,
+                             const float* x, float* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (bo && bc && bv && x && y)), "spmv_rbcsr_f32: bad arguments");
+  API_BEGIN
+  launch_spmv_rbcsr<float>(ctx->stream, n, bo, bc, bv, x, y);
+  LAUNCHED("spmv_rbcsr_f32");
+  API_END
+}
+
+// ---- BLAS-1 ----------------------------------------------------------------------------------
+int alien_hip_copy_f64(alien_hip_ctx* ctx, int n, const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (x && y)), "copy_f64: bad arguments");
+  API_BEGIN
+  launch_copy<double>(ctx->stream, n, x, y);
+  LAUNCHED("copy_f64");
+  API_END
+}
+int alien_hip_axpy_f64(alien_hip_ctx* ctx, int n, double a, const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (x && y)), "axpy_f64: bad arguments");
+  API_BEGIN
+  launch_axpy<double>(ctx->stream, n, a, x, y);
+  LAUNCHED("axpy_f64");
+  API_END
+}
+int alien_hip_scal_f64(alien_hip_ctx* ctx, int n, double a, double* x) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || x), "scal_f64: bad arguments");
+  API_BEGIN
+  launch_scal<double>(ctx->stream, n, a, x);
+  LAUNCHED("scal_f64");
+  API_END
+}
+int alien_hip_assign_f64(alien_hip_ctx* ctx, int n, double a, double* x) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || x), "assign_f64: bad arguments");
+  API_BEGIN
+  launch_assign<double>(ctx->stream, n, a, x);
+  LAUNCHED("assign_f64");
+  API_END
+}
+int alien_hip_pointwise_mult_f64(alien_hip_ctx* ctx, int n, const double* x, const double* y, double* z) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (x && y && z)), "pointwise_mult_f64: bad arguments");
+  API_BEGIN
+  launch_pmult<double>(ctx->stream, n, x, y, z);
+  LAUNCHED("pointwise_mult_f64");
+  API_END
+}
+int alien_hip_dot_f64(alien_hip_ctx* ctx, int n, const double* x, const double* y, double* result) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && result && (n == 0 || (x && y)), "dot_f64: bad arguments");
+  API_BEGIN
+  launch_dot<double>(ctx->stream, ctx->red(), n, x, y, result);
+  LAUNCHED("dot_f64");
+  API_END
+}
+int alien_hip_dot_f32(alien_hip_ctx* ctx, int n, const float* x, const float* y, double* result) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && result && (n == 0 || (x && y)), "dot_f32: bad arguments");
+  API_BEGIN
+  launch_dot<float>(ctx->stream, ctx->red(), n, x, y, result);
+  LAUNCHED("dot_f32");
+  API_END
+}
+int alien_hip_axpy_dot_f64(alien_hip_ctx* ctx, int n, double a, const double* x, double* y, double* result) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && result && (n == 0 || (x && y)), "axpy_dot_f64: bad arguments");
+  API_BEGIN
+  launch_axpy_dot<double>(ctx->stream, ctx->red(), n, a, x, y, result);
+  LAUNCHED("axpy_dot_f64");
+  API_END
+}
+int alien_hip_spmv_dot_csr_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                               const double* x, double* y, double* result) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && result && (n == 0 || (ro && cols && vals && x && y)), "spmv_dot_csr_f64: bad arguments");
+  API_BEGIN
+  launch_spmv_dot_csr<double>(ctx->stream, ctx->red(), n, ro, cols, vals, x, y, result);
+  LAUNCHED("spmv_dot_csr_f64");
+  API_END
+}
+
+// ---- ILU0 --------------------------------------------------------------------------------------
+int alien_hip_ilu0_factor_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const int* dcol,
+                              double* lu, int nlevels, const int* lptr_host, const int* lrows) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && nlevels >= 0 && (n == 0 || (ro && cols && dcol && lu && lptr_host && lrows)),
+          "ilu0_factor_f64: bad arguments");
+  REQUIRE(nlevels == 0 || (lptr_host[0] == 0 && lptr_host[nlevels] == n), "ilu0_factor_f64: schedule must cover the rows");
+  API_BEGIN
+  launch_ilu0_factor<double>(ctx->stream, nlevels, lptr_host, lrows, ro, cols, dcol, lu);
+  LAUNCHED("ilu0_factor_f64");
+  API_END
+}
+
+int alien_hip_ilu0_solve_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const int* dcol,
+                             const double* lu, int nl, const int* lptr, const int* lrows, int nu, const int* uptr,
+                             const int* urows, const double* y, double* t, double* x) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && nl >= 0 && nu >= 0 &&
+              (n == 0 || (ro && cols && dcol && lu && lptr && lrows && uptr && urows && y && t && x)),
+          "ilu0_solve_f64: bad arguments");
+  REQUIRE(nl == 0 || (lptr[0] == 0 && lptr[nl] == n), "ilu0_solve_f64: lower schedule must cover the rows");
+  REQUIRE(nu == 0 || (uptr[0] == 0 && uptr[nu] == n), "ilu0_solve_f64: upper schedule must cover the rows");
+  API_BEGIN
+  launch_ilu0_solve<double>(ctx->stream, nl, lptr, lrows, nu, uptr, urows, ro, cols, dcol, lu, y, t, x);
+  LAUNCHED("ilu0_solve_f64");
+  API_END
+}
+
+} // extern "C"

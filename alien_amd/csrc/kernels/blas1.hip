@@ -1,0 +1,109 @@
+// BLAS-1 kernels of the HIP backend: the vector operations that
+// CBLASMPIKernel (simple_csr/algebra/CBLASMPIKernel.h:37-125) and the SYCL
+// KernelInternal (sycl/algebra/SYCLKernelInternal.h:110-246) provide.
+//
+// All element-wise operations round exactly like the sequential reference
+// loops (built with -ffp-contract=off: no fused multiply-add). Dot products
+// accumulate in FP64 in one launch: each workgroup reduces its grid-stride
+// share, the last workgroup combines the partials in workgroup order and
+// writes the scalar to device memory.
+#include "../alien_hip_internal.h"
+#include "launch.h"
+#include "device_common.h"
+
+namespace alien_hip {
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_copy(int n, const T* __restrict__ x, T* __restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = x[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_axpy(int n, T a, const T* __restrict__ x, T* __restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = y[i] + a * x[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_scal(int n, T a, T* __restrict__ x) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = a * x[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_assign(int n, T a, T* __restrict__ x) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = a;
+}
+
+// z may alias x or y (pointwiseMult(invD, y, y) in the Chebyshev power method)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pmult(int n, const T* x, const T* y, T* z) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) z[i] = x[i] * y[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_dot(int n, const T* __restrict__ x, const T* __restrict__ y,
+                                                 RedWork w, double* __restrict__ out) {
+  __shared__ double lds4[4];
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    s += (double)x[i] * (double)y[i];
+  double v[1] = {block_sum(s, lds4)};
+  grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
+}
+
+// y += a x ; out = y . y
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_axpy_dot(int n, T a, const T* __restrict__ x, T* __restrict__ y,
+                                                      RedWork w, double* __restrict__ out) {
+  __shared__ double lds4[4];
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    T v = y[i] + a * x[i];
+    y[i] = v;
+    s += (double)v * (double)v;
+  }
+  double v[1] = {block_sum(s, lds4)};
+  grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
+}
+
+// ---- launchers --------------------------------------------------------------
+template <typename T>
+void launch_copy(hipStream_t st, int n, const T* x, T* y) {
+  if (n > 0) hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, x, y);
+}
+template <typename T>
+void launch_axpy(hipStream_t st, int n, T a, const T* x, T* y) {
+  if (n > 0) hipLaunchKernelGGL(k_axpy<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, a, x, y);
+}
+template <typename T>
+void launch_scal(hipStream_t st, int n, T a, T* x) {
+  if (n > 0) hipLaunchKernelGGL(k_scal<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, a, x);
+}
+template <typename T>
+void launch_assign(hipStream_t st, int n, T a, T* x) {
+  if (n > 0) hipLaunchKernelGGL(k_assign<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, a, x);
+}
+template <typename T>
+void launch_pmult(hipStream_t st, int n, const T* x, const T* y, T* z) {
+  if (n > 0) hipLaunchKernelGGL(k_pmult<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, x, y, z);
+}
+template <typename T>
+void launch_dot(hipStream_t st, RedWork w, int n, const T* x, const T* y, double* out) {
+  hipLaunchKernelGGL(k_dot<T>, dim3(red_grid(n)), dim3(kBlock), 0, st, n, x, y, w, out);
+}
+template <typename T>
+void launch_axpy_dot(hipStream_t st, RedWork w, int n, T a, const T* x, T* y, double* out) {
+  hipLaunchKernelGGL(k_axpy_dot<T>, dim3(red_grid(n)), dim3(kBlock), 0, st, n, a, x, y, w, out);
+}
+
+#define ALIEN_HIP_INST_BLAS1(T)                                                         \
+  template void launch_copy<T>(hipStream_t, int, const T*, T*);                         \
+  template void launch_axpy<T>(hipStream_t, int, T, const T*, T*);                      \
+  template void launch_scal<T>(hipStream_t, int, T, T*);                                \
+  template void launch_assign<T>(hipStream_t, int, T, T*);                              \
+  template void launch_pmult<T>(hipStream_t, int, const T*, const T*, T*);              \
+  template void launch_dot<T>(hipStream_t, RedWork, int, const T*, const T*, double*);  \
+  template void launch_axpy_dot<T>(hipStream_t, RedWork, int, T, const T*, T*, double*);
+ALIEN_HIP_INST_BLAS1(double)
+ALIEN_HIP_INST_BLAS1(float)
+
+} // namespace alien_hip

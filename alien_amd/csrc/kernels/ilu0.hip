@@ -1,0 +1,122 @@
+// ILU(0) on level-set schedules.
+//
+// LUFactorisationAlgo (src/core/alien/expression/krylov/ILU0Preconditioner.h:71-231)
+// factorises and solves row after row on the host. Row i only depends on the
+// rows named by its strictly-lower entries (factorisation and L sweep) or its
+// strictly-upper entries (U sweep), so rows of one level of that DAG run in
+// parallel, one lane per row, and levels run in order (one launch each, all
+// on one stream, capturable in a hipGraph). Inside a row the operations are
+// the reference's, in the reference's order, with separate multiply and add,
+// so the factors and the sweeps are bit-exact with the sequential host code
+// for any schedule that respects the DAG (level sets of the natural order,
+// or the colour classes of a multicolour permutation).
+#include "../alien_hip_internal.h"
+#include "device_common.h"
+#include "launch.h"
+
+namespace alien_hip {
+
+// IKJ update of row i. For each strictly-lower entry a_ik (ascending k):
+// a_ik /= a_kk, then a_ij -= a_ik * a_kj for every later entry j of row i
+// whose column also appears in row k. The reference finds matches with a
+// scatter map of row k's columns; both rows are sorted, so a merge finds the
+// same matches in the same order.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ilu0_factor_level(int count, const int* __restrict__ rows,
+                                                               const int* __restrict__ ro,
+                                                               const int* __restrict__ cols,
+                                                               const int* __restrict__ dcol,
+                                                               T* __restrict__ lu) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int i = rows[t];
+  const int rb = ro[i], re = ro[i + 1], di = dcol[i];
+  for (int k = rb; k < di; ++k) {
+    const int kr = cols[k];
+    const T aik = lu[k] / lu[dcol[kr]];
+    lu[k] = aik;
+    int p = ro[kr];
+    const int pe = ro[kr + 1];
+    for (int j = k + 1; j < re; ++j) {
+      const int cj = cols[j];
+      while (p < pe && cols[p] < cj) ++p;
+      if (p == pe) break;
+      if (cols[p] == cj) lu[j] = lu[j] - aik * lu[p];
+    }
+  }
+}
+
+// L sweep of one level: t_i = y_i - sum_{k<dcol} l_ik t_k (solveL, :181-198)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ilu0_lsolve_level(int count, const int* __restrict__ rows,
+                                                               const int* __restrict__ ro,
+                                                               const int* __restrict__ cols,
+                                                               const int* __restrict__ dcol,
+                                                               const T* __restrict__ lu,
+                                                               const T* __restrict__ y, T* __restrict__ t) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= count) return;
+  const int i = rows[q];
+  T v = y[i];
+  const int e = dcol[i];
+  for (int k = ro[i]; k < e; ++k) v = v - lu[k] * t[cols[k]];
+  t[i] = v;
+}
+
+// U sweep of one level: x_i = (t_i - sum_{k>dcol} u_ik x_k) / u_ii (solveU, :200-231)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ilu0_usolve_level(int count, const int* __restrict__ rows,
+                                                               const int* __restrict__ ro,
+                                                               const int* __restrict__ cols,
+                                                               const int* __restrict__ dcol,
+                                                               const T* __restrict__ lu,
+                                                               const T* __restrict__ t, T* __restrict__ x) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= count) return;
+  const int i = rows[q];
+  const int dk = dcol[i];
+  T v = t[i];
+  const int e = ro[i + 1];
+  for (int k = dk + 1; k < e; ++k) v = v - lu[k] * x[cols[k]];
+  x[i] = v / lu[dk];
+}
+
+template <typename T>
+void launch_ilu0_factor(hipStream_t st, int nlevels, const int* ptr_host, const int* rows, const int* ro,
+                        const int* cols, const int* dcol, T* lu) {
+  for (int l = 0; l < nlevels; ++l) {
+    const int c = ptr_host[l + 1] - ptr_host[l];
+    if (c > 0)
+      hipLaunchKernelGGL(k_ilu0_factor_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
+                         rows + ptr_host[l], ro, cols, dcol, lu);
+  }
+}
+
+template <typename T>
+void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* lrows, int nu,
+                       const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,
+                       const T* lu, const T* y, T* t, T* x) {
+  for (int l = 0; l < nl; ++l) {
+    const int c = lptr_host[l + 1] - lptr_host[l];
+    if (c > 0)
+      hipLaunchKernelGGL(k_ilu0_lsolve_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
+                         lrows + lptr_host[l], ro, cols, dcol, lu, y, t);
+  }
+  for (int l = 0; l < nu; ++l) {
+    const int c = uptr_host[l + 1] - uptr_host[l];
+    if (c > 0)
+      hipLaunchKernelGGL(k_ilu0_usolve_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
+                         urows + uptr_host[l], ro, cols, dcol, lu, t, x);
+  }
+}
+
+#define ALIEN_HIP_INST_ILU0(T)                                                                            \
+  template void launch_ilu0_factor<T>(hipStream_t, int, const int*, const int*, const int*, const int*,   \
+                                      const int*, T*);                                                    \
+  template void launch_ilu0_solve<T>(hipStream_t, int, const int*, const int*, int, const int*,           \
+                                     const int*, const int*, const int*, const int*, const T*, const T*,  \
+                                     T*, T*);
+ALIEN_HIP_INST_ILU0(double)
+ALIEN_HIP_INST_ILU0(float)
+
+} // namespace alien_hip

@@ -1,0 +1,53 @@
+// Device-resident state of the fused Krylov iterations and the launchers of
+// krylov.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace alien_hip {
+
+struct RedWork;
+
+// Row-bundled CSR operand (see spmv.hip)
+template <typename T>
+struct RBMat {
+  const int* bo; // bundle offsets in 64-entry slots, nbundles + 1
+  const int* bc; // columns, -1 = padding
+  const T* bv;   // values
+};
+
+// Scalars of one CG solve; lives in device memory, read by the host only to
+// poll `done`. Mirrors the locals of CG::solve (rho, rho1, alpha) and the
+// fields of Iteration (iteration count, criteria value, status).
+struct CGState {
+  double rho;   // previous r.z
+  double rho1;  // current r.z
+  double pq;    // p.q
+  double rr;    // last computed r.r (Iteration's m_value)
+  double crit;  // tol^2 (b,b)
+  int it;       // iterations done
+  int max_it;
+  int done;     // set when the stop test passes, max_it is reached or on breakdown
+  int status;   // Iteration::getStatus()
+  int breakdown;// p.q == 0
+  int pad[3];
+};
+
+template <typename T>
+void launch_residual(hipStream_t, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out);
+template <typename T>
+void launch_cheb_step(hipStream_t, RedWork w, int n, RBMat<T> A, const T* xin, const T* yin, T* yout, T* p,
+                      const T* invd, double a, double b, bool first, bool last, CGState* st);
+template <typename T>
+void launch_diag_apply(hipStream_t, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st);
+template <typename T>
+void launch_cg_p(hipStream_t, int n, const T* z, T* p, CGState* st, int first);
+template <typename T>
+void launch_cg_q(hipStream_t, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st);
+template <typename T, typename TX>
+void launch_cg_xr(hipStream_t, RedWork w, int n, const T* p, const T* q, TX* x, T* r, CGState* st);
+void launch_demote_scaled(hipStream_t, int n, const double* r, const double* rr, float* r32);
+void launch_promote_add(hipStream_t, int n, const float* d, const double* rr, double* x);
+void launch_demote(hipStream_t, int n, const double* a, float* b);
+
+} // namespace alien_hip

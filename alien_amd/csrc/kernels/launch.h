@@ -1,0 +1,55 @@
+// Host-side launchers of the gfx950 kernels (definitions in the .hip files).
+// All launches are asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace alien_hip {
+
+struct RedWork;
+
+// ---- BLAS-1 (blas1.hip) ----
+template <typename T> void launch_copy(hipStream_t, int n, const T* x, T* y);
+template <typename T> void launch_axpy(hipStream_t, int n, T a, const T* x, T* y);
+template <typename T> void launch_scal(hipStream_t, int n, T a, T* x);
+template <typename T> void launch_assign(hipStream_t, int n, T a, T* x);
+template <typename T> void launch_pmult(hipStream_t, int n, const T* x, const T* y, T* z);
+template <typename T> void launch_dot(hipStream_t, RedWork w, int n, const T* x, const T* y, double* out);
+template <typename T> void launch_axpy_dot(hipStream_t, RedWork w, int n, T a, const T* x, T* y, double* out);
+
+// ---- SpMV (spmv.hip) ----
+template <typename T>
+void launch_spmv_csr(hipStream_t, int n, const int* ro, const int* cols, const T* vals, const T* x, T* y);
+template <typename T>
+void launch_spmv_dot_csr(hipStream_t, RedWork w, int n, const int* ro, const int* cols, const T* vals,
+                         const T* x, T* y, double* out);
+template <typename T>
+void launch_spmv_rbcsr(hipStream_t, int n, const int* bo, const int* bc, const T* bv, const T* x, T* y);
+template <typename T>
+void launch_rbcsr_fill(hipStream_t, int n, const int* ro, const int* cols, const double* vals, const int* bo,
+                       int* bc, T* bv);
+template <typename T>
+void launch_spmv_bellpack(hipStream_t, int n, int slice, const int* bro, const int* bc, const T* bv,
+                          const T* x, T* y);
+void launch_addmult_bellpack(hipStream_t, int n, int slice, double alpha, const int* bro, const int* bc,
+                             const double* bv, const uint8_t* mask, const double* x, double* y);
+void launch_addmult_csr(hipStream_t, int n, int upper, double alpha, const int* ro, const int* cols,
+                        const int* dcol, const double* vals, const double* x, double* y);
+template <typename T>
+void launch_spmv_bsr2(hipStream_t, int nb, const int* ro, const int* cols, const T* vals, const T* x, T* y);
+void launch_invdiag_csr(hipStream_t, int n, const int* dcol, const double* vals, double* y);
+void launch_invdiag_bellpack(hipStream_t, int n, int slice, const int* bro, const int* bc, const double* bv,
+                             double* y);
+
+// ---- ILU0 (ilu0.hip) ----
+template <typename T>
+void launch_ilu0_factor(hipStream_t, int nlevels, const int* ptr_host, const int* rows, const int* ro,
+                        const int* cols, const int* dcol, T* lu);
+template <typename T>
+void launch_ilu0_solve(hipStream_t, int nl, const int* lptr_host, const int* lrows, int nu,
+                       const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,
+                       const T* lu, const T* y, T* t, T* x);
+
+} // namespace alien_hip

@@ -1,0 +1,325 @@
+// SpMV kernels for gfx950.
+//
+// Every kernel keeps the reference's per-row summation order (the entries of
+// a row in ascending column order, accumulated left to right, separate
+// multiply and add), so results are bit-exact with
+// SimpleCSRMatrixMultT::_seqMult (simple_csr/algebra/SimpleCSRMatrixMultT.h:172-191);
+// what changes between layouts is only how the value and column streams are
+// laid out in HBM:
+//
+//  * CSR: one lane per row over Alien's own arrays (no conversion).
+//  * RB-CSR (row-bundled CSR): 64 rows per bundle = one wavefront, entry j of
+//    the bundle's rows stored contiguously, so each of the width loads of a
+//    wavefront is one 512-byte (FP64) / 256-byte (FP32, int32) coalesced
+//    access. Padding slots carry column -1 and are skipped.
+//  * BEllPack: Alien's sliced-ELL layout (slice height 1024, padding column 0
+//    value 0), computed exactly as MatrixInternal::mult
+//    (sycl/data/SYCLBEllPackMatrix.cc:623-688), padding products included.
+//
+// The matrix streams are read once per SpMV and loaded non-temporally so that
+// L2 / Infinity Cache capacity stays with the gathered x vector.
+#include "../alien_hip_internal.h"
+#include "device_common.h"
+#include "launch.h"
+
+namespace alien_hip {
+
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// ---- CSR, one lane per row ------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_csr(int n, const int* __restrict__ ro,
+                                                      const int* __restrict__ cols,
+                                                      const T* __restrict__ vals,
+                                                      const T* __restrict__ x, T* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  T acc = 0;
+  const int e = ro[i + 1];
+  for (int k = ro[i]; k < e; ++k) acc = acc + vals[k] * x[cols[k]];
+  y[i] = acc;
+}
+
+// y = A x and out = y . x
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_dot_csr(int n, const int* __restrict__ ro,
+                                                          const int* __restrict__ cols,
+                                                          const T* __restrict__ vals,
+                                                          const T* __restrict__ x, T* __restrict__ y,
+                                                          RedWork w, double* __restrict__ out) {
+  __shared__ double lds4[4];
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    T acc = 0;
+    const int e = ro[i + 1];
+    for (int k = ro[i]; k < e; ++k) acc = acc + vals[k] * x[cols[k]];
+    y[i] = acc;
+    s += (double)acc * (double)x[i];
+  }
+  double v[1] = {block_sum(s, lds4)};
+  grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
+}
+
+// ---- RB-CSR -------------------------------------------------------------------------
+// One wavefront per bundle; U entries of a row are loaded before any is used,
+// so a wave keeps 2U independent streaming loads and U gathers in flight.
+template <typename T, int U>
+__device__ __forceinline__ T rb_row_sum(long base, int w, const int* __restrict__ bc,
+                                        const T* __restrict__ bv, const T* __restrict__ x) {
+  T acc = 0;
+  int j = 0;
+  for (; j + U <= w; j += U) {
+    int c[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      c[u] = stream_load(bc + base + (long)(j + u) * kBundle);
+      v[u] = stream_load(bv + base + (long)(j + u) * kBundle);
+    }
+    T xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = (c[u] >= 0) ? x[c[u]] : T(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (c[u] >= 0) acc = acc + v[u] * xv[u];
+  }
+  for (; j < w; ++j) {
+    const int c = stream_load(bc + base + (long)j * kBundle);
+    const T v = stream_load(bv + base + (long)j * kBundle);
+    if (c >= 0) acc = acc + v * x[c];
+  }
+  return acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_rbcsr(int n, const int* __restrict__ bo,
+                                                        const int* __restrict__ bc,
+                                                        const T* __restrict__ bv,
+                                                        const T* __restrict__ x, T* __restrict__ y) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int b = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) / kWave));
+  const int nb = (n + kBundle - 1) / kBundle;
+  if (b >= nb) return;
+  const int s0 = bo[b];
+  const int w = bo[b + 1] - s0;
+  const long base = (long)s0 * kBundle + lane;
+  const T acc = rb_row_sum<T, 7>(base, w, bc, bv, x);
+  const int row = b * kBundle + lane;
+  if (row < n) y[row] = acc;
+}
+
+// Device conversion CSR -> RB-CSR (one lane per row writes its slots).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_rbcsr_fill(int n, const int* __restrict__ ro,
+                                                        const int* __restrict__ cols,
+                                                        const double* __restrict__ vals,
+                                                        const int* __restrict__ bo, int* __restrict__ bc,
+                                                        T* __restrict__ bv) {
+  const int nb = (n + kBundle - 1) / kBundle;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int b = t / kBundle, lane = t % kBundle;
+  if (b >= nb) return;
+  const int row = t;
+  const int s0 = bo[b], w = bo[b + 1] - s0;
+  const long base = (long)s0 * kBundle + lane;
+  int len = 0, r0 = 0;
+  if (row < n) {
+    r0 = ro[row];
+    len = ro[row + 1] - r0;
+  }
+  for (int j = 0; j < w; ++j) {
+    const long e = base + (long)j * kBundle;
+    if (j < len) {
+      bc[e] = cols[r0 + j];
+      bv[e] = (T)vals[r0 + j];
+    } else {
+      bc[e] = -1;
+      bv[e] = T(0);
+    }
+  }
+}
+
+// ---- BEllPack (Alien's sliced ELL) ------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_bellpack(int n, int slice, const int* __restrict__ bro,
+                                                           const int* __restrict__ bc,
+                                                           const T* __restrict__ bv,
+                                                           const T* __restrict__ x, T* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int s = i / slice, l = i - s * slice;
+  const int w = bro[s + 1] - bro[s];
+  const long base = (long)bro[s] * slice + l;
+  T acc = 0;
+  for (int j = 0; j < w; ++j) {
+    const long e = base + (long)j * slice;
+    acc = acc + stream_load(bv + e) * x[stream_load(bc + e)];
+  }
+  y[i] = acc;
+}
+
+// y_i += alpha * sum_j mask * a * x  (SYCLBEllPackMatrix.cc:796-835 / 872-908):
+// the product is formed as ((alpha * mask) * a) * x, as there.
+__global__ void __launch_bounds__(kBlock) k_addmult_bellpack(int n, int slice, double alpha,
+                                                              const int* __restrict__ bro,
+                                                              const int* __restrict__ bc,
+                                                              const double* __restrict__ bv,
+                                                              const uint8_t* __restrict__ mask,
+                                                              const double* __restrict__ x,
+                                                              double* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int s = i / slice, l = i - s * slice;
+  const int w = bro[s + 1] - bro[s];
+  const long base = (long)bro[s] * slice + l;
+  double acc = y[i];
+  for (int j = 0; j < w; ++j) {
+    const long e = base + (long)j * slice;
+    acc = acc + ((alpha * (double)mask[e]) * bv[e]) * x[bc[e]];
+  }
+  y[i] = acc;
+}
+
+// CSR addLMult / addUMult: tmpy += (alpha * a) * x over k < dcol / k > dcol
+__global__ void __launch_bounds__(kBlock) k_addmult_csr(int n, int upper, double alpha,
+                                                         const int* __restrict__ ro,
+                                                         const int* __restrict__ cols,
+                                                         const int* __restrict__ dcol,
+                                                         const double* __restrict__ vals,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int b = upper ? dcol[i] + 1 : ro[i];
+  const int e = upper ? ro[i + 1] : dcol[i];
+  double acc = y[i];
+  for (int k = b; k < e; ++k) acc = acc + (alpha * vals[k]) * x[cols[k]];
+  y[i] = acc;
+}
+
+// ---- 2x2 block CSR ---------------------------------------------------------------------
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> { typedef double4 type; typedef double2 half; };
+template <>
+struct Vec4<float> { typedef float4 type; typedef float2 half; };
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_bsr2(int nb, const int* __restrict__ ro,
+                                                       const int* __restrict__ cols,
+                                                       const T* __restrict__ vals,
+                                                       const T* __restrict__ x, T* __restrict__ y) {
+  typedef typename Vec4<T>::type V4;
+  typedef typename Vec4<T>::half V2;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nb) return;
+  T y0 = 0, y1 = 0;
+  const int e = ro[i + 1];
+  for (int k = ro[i]; k < e; ++k) {
+    const V4 m = reinterpret_cast<const V4*>(vals)[k];
+    const V2 xv = reinterpret_cast<const V2*>(x)[cols[k]];
+    // row ieq = 0 then 1, columns iu = 0 then 1: m[ieq + 2 iu]
+    y0 = y0 + m.x * xv.x;
+    y0 = y0 + m.z * xv.y;
+    y1 = y1 + m.y * xv.x;
+    y1 = y1 + m.w * xv.y;
+  }
+  V2 out;
+  out.x = y0;
+  out.y = y1;
+  reinterpret_cast<V2*>(y)[i] = out;
+}
+
+// ---- inverse diagonal -------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_invdiag_csr(int n, const int* __restrict__ dcol,
+                                                         const double* __restrict__ vals,
+                                                         double* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) y[i] = 1. / vals[dcol[i]];
+}
+
+__global__ void __launch_bounds__(kBlock) k_invdiag_bellpack(int n, int slice, const int* __restrict__ bro,
+                                                              const int* __restrict__ bc,
+                                                              const double* __restrict__ bv,
+                                                              double* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int s = i / slice, l = i - s * slice;
+  const int w = bro[s + 1] - bro[s];
+  const long base = (long)bro[s] * slice + l;
+  for (int j = 0; j < w; ++j) {
+    const long e = base + (long)j * slice;
+    if (bc[e] == i && bv[e] != 0) y[i] = 1. / bv[e];
+  }
+}
+
+// ---- launchers ---------------------------------------------------------------------------
+static inline dim3 rows_grid(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+template <typename T>
+void launch_spmv_csr(hipStream_t st, int n, const int* ro, const int* cols, const T* vals, const T* x, T* y) {
+  if (n > 0) hipLaunchKernelGGL(k_spmv_csr<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, x, y);
+}
+template <typename T>
+void launch_spmv_dot_csr(hipStream_t st, RedWork w, int n, const int* ro, const int* cols, const T* vals,
+                         const T* x, T* y, double* out) {
+  hipLaunchKernelGGL(k_spmv_dot_csr<T>, dim3(red_grid(n)), dim3(kBlock), 0, st, n, ro, cols, vals, x, y, w, out);
+}
+template <typename T>
+void launch_spmv_rbcsr(hipStream_t st, int n, const int* bo, const int* bc, const T* bv, const T* x, T* y) {
+  if (n > 0) hipLaunchKernelGGL(k_spmv_rbcsr<T>, rows_grid(n), dim3(kBlock), 0, st, n, bo, bc, bv, x, y);
+}
+template <typename T>
+void launch_rbcsr_fill(hipStream_t st, int n, const int* ro, const int* cols, const double* vals,
+                       const int* bo, int* bc, T* bv) {
+  if (n > 0) hipLaunchKernelGGL(k_rbcsr_fill<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, bo, bc, bv);
+}
+template <typename T>
+void launch_spmv_bellpack(hipStream_t st, int n, int slice, const int* bro, const int* bc, const T* bv,
+                          const T* x, T* y) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_spmv_bellpack<T>, rows_grid(n), dim3(kBlock), 0, st, n, slice, bro, bc, bv, x, y);
+}
+void launch_addmult_bellpack(hipStream_t st, int n, int slice, double alpha, const int* bro, const int* bc,
+                             const double* bv, const uint8_t* mask, const double* x, double* y) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_addmult_bellpack, rows_grid(n), dim3(kBlock), 0, st, n, slice, alpha, bro, bc, bv,
+                       mask, x, y);
+}
+void launch_addmult_csr(hipStream_t st, int n, int upper, double alpha, const int* ro, const int* cols,
+                        const int* dcol, const double* vals, const double* x, double* y) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_addmult_csr, rows_grid(n), dim3(kBlock), 0, st, n, upper, alpha, ro, cols, dcol,
+                       vals, x, y);
+}
+template <typename T>
+void launch_spmv_bsr2(hipStream_t st, int nb, const int* ro, const int* cols, const T* vals, const T* x, T* y) {
+  if (nb > 0) hipLaunchKernelGGL(k_spmv_bsr2<T>, rows_grid(nb), dim3(kBlock), 0, st, nb, ro, cols, vals, x, y);
+}
+void launch_invdiag_csr(hipStream_t st, int n, const int* dcol, const double* vals, double* y) {
+  if (n > 0) hipLaunchKernelGGL(k_invdiag_csr, rows_grid(n), dim3(kBlock), 0, st, n, dcol, vals, y);
+}
+void launch_invdiag_bellpack(hipStream_t st, int n, int slice, const int* bro, const int* bc, const double* bv,
+                             double* y) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_invdiag_bellpack, rows_grid(n), dim3(kBlock), 0, st, n, slice, bro, bc, bv, y);
+}
+
+#define ALIEN_HIP_INST_SPMV(T)                                                                          \
+  template void launch_spmv_csr<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*);   \
+  template void launch_spmv_dot_csr<T>(hipStream_t, RedWork, int, const int*, const int*, const T*,     \
+                                       const T*, T*, double*);                                          \
+  template void launch_spmv_rbcsr<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*); \
+  template void launch_rbcsr_fill<T>(hipStream_t, int, const int*, const int*, const double*,           \
+                                     const int*, int*, T*);                                             \
+  template void launch_spmv_bellpack<T>(hipStream_t, int, int, const int*, const int*, const T*,        \
+                                        const T*, T*);                                                  \
+  template void launch_spmv_bsr2<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*);
+ALIEN_HIP_INST_SPMV(double)
+ALIEN_HIP_INST_SPMV(float)
+
+} // namespace alien_hip

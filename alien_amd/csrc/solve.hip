@@ -1,0 +1,457 @@
+// Whole-solve driver of the hip backend: test_krylov's run(alg)
+// (examples/test_krylov.cpp:500-665) for --kernel hip, plus the fused,
+// graph-captured and mixed-precision variants of the CG path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "alien/hip/HIPBackEnd.h"
+#include "alien/hip/krylov/HIPILU0Preconditioner.h"
+#include "alien/hip/krylov/Krylov.h"
+#include "alien_hip_internal.h"
+#include "kernels/device_common.h"
+#include "kernels/krylov_state.h"
+#include "kernels/launch.h"
+
+namespace alien_hip {
+
+namespace {
+
+// Identity preconditioner (test_krylov has none; used for unpreconditioned runs).
+template <typename AlgebraT>
+struct IdentityPreconditioner {
+  void init() {}
+  template <typename Alg>
+  void solve(Alg& alg, const typename AlgebraT::Vector& x, typename AlgebraT::Vector& y) const {
+    alg.copy(x, y);
+  }
+};
+
+struct EventTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s;
+  explicit EventTimer(hipStream_t st) : s(st) {
+    ALIEN_HIP_CHECK(hipEventCreate(&a));
+    ALIEN_HIP_CHECK(hipEventCreate(&b));
+  }
+  ~EventTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  void start() { ALIEN_HIP_CHECK(hipEventRecord(a, s)); }
+  double stop() {
+    ALIEN_HIP_CHECK(hipEventRecord(b, s));
+    ALIEN_HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    ALIEN_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms * 1e-3;
+  }
+};
+
+// Host-side validation of an input CSR before anything is uploaded: offsets
+// monotone, columns in range and strictly ascending inside each row, and a
+// stored diagonal in every row (dcol must point at a_ii for the inverse
+// diagonal and the ILU0 pivots).
+void validate_csr(int n, const int* ro, const int* cols, bool need_diag) {
+  if (n < 0 || !ro || (n > 0 && !cols)) throw Error(ALIEN_HIP_EINVAL, "CSR: NULL arrays");
+  if (ro[0] != 0) throw Error(ALIEN_HIP_EINVAL, "CSR: row_offset[0] != 0");
+  for (int i = 0; i < n; ++i) {
+    if (ro[i + 1] < ro[i]) throw Error(ALIEN_HIP_EINVAL, "CSR: row_offset not monotone");
+    bool diag = false;
+    for (int k = ro[i]; k < ro[i + 1]; ++k) {
+      if (cols[k] < 0 || cols[k] >= n) throw Error(ALIEN_HIP_EINVAL, "CSR: column out of range");
+      if (k > ro[i] && cols[k] <= cols[k - 1]) throw Error(ALIEN_HIP_EINVAL, "CSR: columns not strictly ascending");
+      diag |= (cols[k] == i);
+    }
+    if (need_diag && !diag) throw Error(ALIEN_HIP_EINVAL, "CSR: missing diagonal entry");
+  }
+}
+
+// Symmetric permutation P A P^T with rows (and columns) renumbered so that
+// new row r is old row perm[r]; columns re-sorted ascending inside each row.
+void permute_csr(int n, const int* ro, const int* cols, const double* vals, const std::vector<int>& perm,
+                 std::vector<int>& pro, std::vector<int>& pcols, std::vector<double>& pvals) {
+  std::vector<int> inv(n);
+  for (int r = 0; r < n; ++r) inv[perm[r]] = r;
+  pro.assign(n + 1, 0);
+  for (int r = 0; r < n; ++r) pro[r + 1] = pro[r] + (ro[perm[r] + 1] - ro[perm[r]]);
+  pcols.resize(ro[n]);
+  pvals.resize(ro[n]);
+  std::vector<std::pair<int, double>> row;
+  for (int r = 0; r < n; ++r) {
+    const int o = perm[r];
+    row.clear();
+    for (int k = ro[o]; k < ro[o + 1]; ++k) row.emplace_back(inv[cols[k]], vals[k]);
+    std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (size_t q = 0; q < row.size(); ++q) {
+      pcols[pro[r] + q] = row[q].first;
+      pvals[pro[r] + q] = row[q].second;
+    }
+  }
+}
+
+// rows sorted by colour, natural order inside a colour
+std::vector<int> colour_permutation(int n, const int* ro, const int* cols) {
+  std::vector<int> color(n);
+  int nc = host_color(n, ro, cols, color.data());
+  std::vector<int> perm(n);
+  std::vector<int> cnt(nc + 1, 0);
+  for (int i = 0; i < n; ++i) ++cnt[color[i] + 1];
+  for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+  for (int i = 0; i < n; ++i) perm[cnt[color[i]]++] = i;
+  return perm;
+}
+
+using Alien::HIPInternalLinearAlgebra;
+using Alien::HIPMatrix;
+using Alien::HIPVector;
+typedef HIPInternalLinearAlgebra Alg;
+
+// ---- reference mode: the Krylov templates over the hip algebra ----------------------------
+template <typename Precond>
+void run_reference(Alg& alg, const alien_hip_solve_params* p, Precond& P, const HIPMatrix& A, const HIPVector& b,
+                   HIPVector& x, alien_hip_solve_result* res, EventTimer& tm) {
+  Alien::Iteration<Alg> it(alg, b, p->tol, p->max_iter);
+  tm.start();
+  try {
+    if (p->solver == ALIEN_HIP_CG) {
+      Alien::CG<Alg> s(alg);
+      s.solve(P, it, A, b, x);
+    } else {
+      Alien::BiCGStab<Alg> s(alg);
+      s.solve(P, it, A, b, x);
+    }
+  } catch (const Alg::NullValueException&) {
+    res->error = 1;
+  }
+  res->solve_s = tm.stop();
+  res->iterations = it();
+  res->status = it.getStatus() ? 1 : 0;
+  res->value = it.getValue();
+}
+
+// ---- fused CG ------------------------------------------------------------------------------
+struct PrecondSpec {
+  bool cheb = false;
+  int order = 0;
+  std::vector<double> a, b; // Chebyshev step coefficients
+};
+
+template <typename T>
+struct FusedCG {
+  alien_hip_ctx* ctx;
+  int n;
+  RBMat<T> A;
+  const T* invd;
+  PrecondSpec P;
+  DeviceArray<T> r, z, pp, q, y0, y1;
+  DeviceArray<CGState> st;
+  CGState* h_st = nullptr; // pinned
+
+  FusedCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
+  : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
+    r.resize(n); z.resize(n); pp.resize(n); q.resize(n);
+    if (P.cheb && P.order > 1) { y0.resize(n); y1.resize(n); }
+    st.resize(1);
+    ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
+  }
+  ~FusedCG() {
+    if (h_st) (void)hipHostFree(h_st);
+  }
+
+  void precond(const T* rin, T* zout) {
+    hipStream_t s = ctx->stream;
+    if (!P.cheb) {
+      launch_diag_apply<T>(s, ctx->red(), n, rin, invd, zout, st.data());
+      return;
+    }
+    const int m = P.order;
+    T* bufs[2] = {y0.data(), y1.data()};
+    const T* yin = rin;
+    for (int i = 0; i < m; ++i) {
+      T* yout = (i == m - 1) ? zout : bufs[i % 2];
+      launch_cheb_step<T>(s, ctx->red(), n, A, rin, yin, yout, pp.data(), invd, P.a[i], P.b[i], i == 0,
+                          i == m - 1, st.data());
+      yin = yout;
+    }
+  }
+  // One CG update: z = M r, p = z (+ beta p), q = A p, x, r, stop test.
+  void iteration(T* x, bool first) {
+    hipStream_t s = ctx->stream;
+    precond(r.data(), z.data());
+    launch_cg_p<T>(s, n, z.data(), pp.data(), st.data(), first ? 1 : 0);
+    launch_cg_q<T>(s, ctx->red(), n, A, pp.data(), q.data(), st.data());
+    launch_cg_xr<T, T>(s, ctx->red(), n, pp.data(), q.data(), x, r.data(), st.data());
+  }
+  CGState read_state() {
+    ALIEN_HIP_CHECK(hipMemcpyAsync(h_st, st.data(), sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return *h_st;
+  }
+
+  // Solve A x = b from the given x (x_is_zero: skip the initial residual SpMV).
+  // Returns the final state; bb = (b,b).
+  CGState solve(const T* b, T* x, bool x_is_zero, double tol, int max_it, bool graph, int poll_every,
+                double* bb_out) {
+    hipStream_t s = ctx->stream;
+    const int slot = ctx->take_slot();
+    // Iteration ctor: (b,b) and the criteria
+    launch_dot<T>(s, ctx->red(), n, b, b, ctx->dslot(slot));
+    const double bb = ctx->read_slot(slot);
+    if (bb_out) *bb_out = bb;
+    CGState h{};
+    h.crit = tol * tol * bb;
+    h.max_it = max_it;
+    h.rr = h.crit + 1;
+    h.status = (bb == 0) ? 1 : 0;
+    if (bb == 0) return h; // nullRhs
+    ALIEN_HIP_CHECK(hipMemcpyAsync(st.data(), &h, sizeof(CGState), hipMemcpyHostToDevice, s));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+    if (x_is_zero) launch_copy<T>(s, n, b, r.data());
+    else launch_residual<T>(s, ctx->red(), n, A, x, b, r.data(), ctx->dslot(slot));
+    iteration(x, true);
+    check_launch("fused CG first iteration");
+    CGState cur = read_state();
+    if (graph && !cur.done) {
+      const int k = std::max(1, poll_every);
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ge = nullptr;
+      ALIEN_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < k; ++i) iteration(x, false);
+      ALIEN_HIP_CHECK(hipStreamEndCapture(s, &g));
+      ALIEN_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      while (!cur.done) {
+        ALIEN_HIP_CHECK(hipGraphLaunch(ge, s));
+        cur = read_state();
+      }
+      (void)hipGraphExecDestroy(ge);
+      (void)hipGraphDestroy(g);
+    } else {
+      while (!cur.done) {
+        iteration(x, false);
+        check_launch("fused CG iteration");
+        cur = read_state();
+      }
+    }
+    return cur;
+  }
+};
+
+PrecondSpec cheb_spec(const Alien::ChebyshevPreconditioner<Alg>& C) {
+  PrecondSpec ps;
+  ps.cheb = true;
+  ps.order = C.order();
+  ps.a.resize(ps.order);
+  ps.b.resize(ps.order);
+  C.coefficients(ps.a.data(), ps.b.data());
+  return ps;
+}
+
+// CG breakdown (p.q == 0): Iteration::stop(r) decides between a clean exit
+// with one more iteration counted and NullValueException (CG.h:96-104).
+template <typename T>
+void finish_breakdown(FusedCG<T>& F, CGState& s, int* error) {
+  if (!s.breakdown) return;
+  if (s.it >= s.max_it) {
+    s.it += 1;
+    return;
+  }
+  const int slot = F.ctx->take_slot();
+  launch_dot<T>(F.ctx->stream, F.ctx->red(), F.n, F.r.data(), F.r.data(), F.ctx->dslot(slot));
+  const double rr = F.ctx->read_slot(slot);
+  s.rr = rr;
+  s.status = rr < s.crit;
+  if (s.status) s.it += 1;
+  else *error = 1;
+}
+
+double true_residual(Alg& alg, const HIPMatrix& A, const HIPVector& b, const HIPVector& x) {
+  // CSR SpMV (bit-exact with _seqMult), FP64
+  HIPVector r;
+  r.init(A.resource(), true);
+  alien_hip::launch_spmv_csr<double>(alg.stream(), A.getNRows(), A.d_ro.data(), A.d_cols.data(), A.d_vals.data(),
+                                     x.data(), r.data());
+  alg.scal(-1., r);
+  alg.axpy(1., b, r);
+  const double rr = alg.dot(r, r), bb = alg.dot(b, b);
+  return bb > 0 ? std::sqrt(rr / bb) : 0.0;
+}
+
+} // namespace
+
+} // namespace alien_hip
+
+using namespace alien_hip;
+
+extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_params* p, int n, const int* ro,
+                                   const int* cols, const double* vals, const double* bh, double* xh,
+                                   alien_hip_solve_result* res) {
+  if (!ctx || !p || !res || !bh || !xh || !vals) return fail(ALIEN_HIP_EINVAL, "solve_csr: NULL argument");
+  std::memset(res, 0, sizeof(*res));
+  try {
+    const bool need_diag = p->precond != ALIEN_HIP_PREC_NONE;
+    validate_csr(n, ro, cols, need_diag);
+    if (p->max_iter < 0 || !(p->tol >= 0)) throw Error(ALIEN_HIP_EINVAL, "solve_csr: bad tol / max_iter");
+    const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
+    const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
+    if ((fused || mixed) && (p->solver != ALIEN_HIP_CG ||
+                             (p->precond != ALIEN_HIP_PREC_CHEB && p->precond != ALIEN_HIP_PREC_DIAG)))
+      throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement CG with the Diag or Chebyshev preconditioner");
+    if (p->precond == ALIEN_HIP_PREC_CHEB && p->poly_order < 1)
+      throw Error(ALIEN_HIP_EINVAL, "Chebyshev order must be >= 1");
+
+    EventTimer tm(ctx->stream);
+    Alg alg(ctx);
+
+    // multicolour ILU0: solve the permuted system P A P^T (P x) = P b
+    std::vector<int> perm;
+    std::vector<int> pro, pcols;
+    std::vector<double> pvals, pb(n), px(n);
+    const int* Aro = ro;
+    const int* Acols = cols;
+    const double* Avals = vals;
+    const double* b_in = bh;
+    double* x_io = xh;
+    if (p->precond == ALIEN_HIP_PREC_ILU0 && p->ilu_schedule == 1) {
+      perm = colour_permutation(n, ro, cols);
+      permute_csr(n, ro, cols, vals, perm, pro, pcols, pvals);
+      for (int r = 0; r < n; ++r) { pb[r] = bh[perm[r]]; px[r] = xh[perm[r]]; }
+      Aro = pro.data(); Acols = pcols.data(); Avals = pvals.data();
+      b_in = pb.data(); x_io = px.data();
+    }
+
+    tm.start();
+    Alien::CSRHostView view{n, Aro, Acols, Avals};
+    int layout = (fused || mixed) ? ALIEN_HIP_LAYOUT_RBCSR : p->layout;
+    HIPMatrix A(ctx, view, layout);
+    HIPVector b, x;
+    alg.allocate(A.resource(), b, x);
+    b.setValues(b_in);
+    x.setValues(x_io);
+
+    if (!fused && !mixed) {
+      switch (p->precond) {
+      case ALIEN_HIP_PREC_DIAG: {
+        Alien::DiagPreconditioner<Alg> P(alg, A);
+        P.init();
+        res->setup_s = tm.stop();
+        run_reference(alg, p, P, A, b, x, res, tm);
+        break;
+      }
+      case ALIEN_HIP_PREC_CHEB: {
+        Alien::ChebyshevPreconditioner<Alg> P(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter);
+        P.init();
+        res->cheb_alpha = P.alpha();
+        res->cheb_beta = P.beta();
+        res->setup_s = tm.stop();
+        run_reference(alg, p, P, A, b, x, res, tm);
+        break;
+      }
+      case ALIEN_HIP_PREC_ILU0: {
+        Alien::ILU0Preconditioner<Alg> P(alg, A);
+        P.init();
+        res->setup_s = tm.stop();
+        run_reference(alg, p, P, A, b, x, res, tm);
+        break;
+      }
+      default: {
+        IdentityPreconditioner<Alg> P;
+        res->setup_s = tm.stop();
+        run_reference(alg, p, P, A, b, x, res, tm);
+      }
+      }
+    } else {
+      // preconditioner set-up shared by the fused paths (FP64, as the reference)
+      std::unique_ptr<Alien::ChebyshevPreconditioner<Alg>> cheb;
+      std::unique_ptr<Alien::DiagPreconditioner<Alg>> diag;
+      PrecondSpec ps;
+      const double* invd64;
+      if (p->precond == ALIEN_HIP_PREC_CHEB) {
+        cheb.reset(new Alien::ChebyshevPreconditioner<Alg>(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter));
+        cheb->init();
+        res->cheb_alpha = cheb->alpha();
+        res->cheb_beta = cheb->beta();
+        ps = cheb_spec(*cheb);
+        invd64 = cheb->invDiag().data();
+      } else {
+        diag.reset(new Alien::DiagPreconditioner<Alg>(alg, A));
+        diag->init();
+        invd64 = diag->invDiag().data();
+      }
+      if (!mixed) {
+        RBMat<double> rb{A.d_bo.data(), A.d_bc.data(), A.d_bv.data()};
+        FusedCG<double> F(ctx, n, rb, invd64, ps);
+        res->setup_s = tm.stop();
+        tm.start();
+        CGState s = F.solve(b.data(), x.data(), false, p->tol, p->max_iter, p->mode == ALIEN_HIP_MODE_GRAPH,
+                            p->poll_every, nullptr);
+        finish_breakdown(F, s, &res->error);
+        res->solve_s = tm.stop();
+        res->iterations = s.it;
+        res->status = s.status;
+        double bb = alg.dot(b, b);
+        res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
+      } else {
+        // FP32 matrix, FP32 preconditioner and vectors, FP64 dots; FP64 outer defect correction
+        A.ensureRBCSR32();
+        DeviceArray<float> invd32(n), r32(n), d32(n);
+        launch_demote(ctx->stream, n, invd64, invd32.data());
+        RBMat<float> rb32{A.d_bo.data(), A.d_bc.data(), A.d_bv32.data()};
+        RBMat<double> rb64{A.d_bo.data(), A.d_bc.data(), A.d_bv.data()};
+        FusedCG<float> F(ctx, n, rb32, invd32.data(), ps);
+        DeviceArray<double> r64(n);
+        res->setup_s = tm.stop();
+        tm.start();
+        const int slot = ctx->take_slot();
+        launch_dot<double>(ctx->stream, ctx->red(), n, b.data(), b.data(), ctx->dslot(slot));
+        const double bb = ctx->read_slot(slot);
+        const int rslot = ctx->take_slot();
+        int total = 0, outer = 0;
+        bool conv = (bb == 0);
+        double rel = 0.0;
+        const int max_outer = p->max_outer > 0 ? p->max_outer : 50;
+        const double inner_tol = p->inner_tol > 0 ? p->inner_tol : 1e-4;
+        const int inner_max = p->inner_max_iter > 0 ? p->inner_max_iter : p->max_iter;
+        while (!conv) {
+          launch_residual<double>(ctx->stream, ctx->red(), n, rb64, x.data(), b.data(), r64.data(),
+                                  ctx->dslot(rslot));
+          const double rr = ctx->read_slot(rslot);
+          rel = std::sqrt(rr / bb);
+          if (rel < p->tol) { conv = true; break; }
+          if (outer >= max_outer || total >= p->max_iter || !(rr > 0)) break;
+          launch_demote_scaled(ctx->stream, n, r64.data(), ctx->dslot(rslot), r32.data());
+          ALIEN_HIP_CHECK(hipMemsetAsync(d32.data(), 0, sizeof(float) * n, ctx->stream));
+          CGState s = F.solve(r32.data(), d32.data(), true, inner_tol, std::min(inner_max, p->max_iter - total),
+                              true, p->poll_every, nullptr);
+          int err = 0;
+          finish_breakdown(F, s, &err);
+          total += s.it;
+          launch_promote_add(ctx->stream, n, d32.data(), ctx->dslot(rslot), x.data());
+          check_launch("mixed refinement");
+          ++outer;
+          if (err) { res->error = 1; break; }
+        }
+        res->solve_s = tm.stop();
+        res->iterations = total;
+        res->outer = outer;
+        res->status = conv ? 1 : 0;
+        res->value = rel;
+      }
+    }
+    res->true_rel_res = true_residual(alg, A, b, x);
+    x.copyValuesTo(x_io);
+    if (!perm.empty())
+      for (int r = 0; r < n; ++r) xh[perm[r]] = px[r];
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what());
+  }
+  return 0;
+}

@@ -1,0 +1,240 @@
+/*
+ * alien_hip.h -- C ABI of libalien_hip.so, the MI355X (gfx950) Krylov backend
+ * that registers beside Alien's `simplecsr` and `sycl` kernels.
+ *
+ * Conventions
+ *  - Every function returns int: 0 = success, otherwise a hipError_t value or
+ *    ALIEN_HIP_EINVAL (-22). alien_hip_last_error() gives the message.
+ *  - Device-side entry points take DEVICE pointers they do not own, sizes, and
+ *    a context handle whose stream orders the work. Nothing here synchronises
+ *    the host unless the comment says so.
+ *  - Integers are int32 row/column indices (Alien's `Integer`,
+ *    src/core/alien/kernels/simple_csr/CSRStructInfo.h:44); sizes are int.
+ *  - Reductions write their result to device memory (device-resident scalar)
+ *    and are deterministic: a fixed grid and a fixed combination order.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the arcaneframework/alien source tree).
+ */
+#ifndef ALIEN_HIP_H
+#define ALIEN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALIEN_HIP_EINVAL (-22)
+
+typedef struct alien_hip_ctx alien_hip_ctx;       /* stream + reduction workspace */
+typedef struct alien_hip_matrix alien_hip_matrix; /* device matrix in all layouts */
+
+/* ------------------------------------------------------------------------ */
+/* Runtime / context                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failing call on this thread (never NULL). */
+const char* alien_hip_last_error(void);
+/* Library version string. Does not touch the GPU. */
+const char* alien_hip_version(void);
+
+/* Context = one HIP stream (NULL = create a private one) plus the workspace
+ * used by the reductions. Replaces SYCLInternal::KernelInternal, the queue
+ * holder of SYCLInternalLinearAlgebra (sycl/algebra/SYCLInternalLinearAlgebra.h:137). */
+int alien_hip_ctx_create(void* stream, alien_hip_ctx** out);
+int alien_hip_ctx_destroy(alien_hip_ctx* ctx);
+int alien_hip_ctx_synchronize(alien_hip_ctx* ctx);
+void* alien_hip_ctx_stream(alien_hip_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side system generators and structure builders (no GPU involved)     */
+/* ------------------------------------------------------------------------ */
+
+enum {
+  ALIEN_HIP_GEN_POISSON7 = 0,  /* 7-point Laplacian [6,-1x6], Dirichlet eliminated */
+  ALIEN_HIP_GEN_CONVDIFF7 = 1, /* upwind convection-diffusion; params {D, Pe, vx, vy, vz} */
+  ALIEN_HIP_GEN_LOGNORMAL7 = 2 /* log-normal diffusion; params {sigma, ax, ay, az} */
+};
+int64_t alien_hip_gen_nnz(int kind, int nx, int ny, int nz);
+/* Fills row_offset[N+1], cols[nnz], vals[nnz] (ascending columns per row).
+ * Stands in for the assembly done by examples/test_krylov.cpp:291-437. */
+int alien_hip_gen_csr(int kind, int nx, int ny, int nz, const double* params, uint64_t seed,
+                      int* row_offset, int* cols, double* vals);
+/* 2 unknowns per cell, 2x2 column-major blocks (SimpleCSRMatrixMultT.h:317-342 layout);
+ * params {sigma, coupling}; nnz blocks = alien_hip_gen_nnz(...); vals has 4*nnzb entries */
+int alien_hip_gen_bsr2(int nx, int ny, int nz, const double* params, uint64_t seed,
+                       int* row_offset, int* cols, double* vals);
+/* splitmix64 stream -> U[0,1) (x_true of the workloads) */
+int alien_hip_fill_uniform01(uint64_t seed, int64_t n, double* out);
+
+/* CSRStructInfo::computeUpperDiagOffset (simple_csr/CSRStructInfo.h:291-309) */
+int alien_hip_host_dcol(int nrows, const int* row_offset, const int* cols, int* dcol);
+/* BEllPackStructInfo::computeBlockRowOffset (sycl/data/SYCLBEllPackMatrix.cc:69-88);
+ * bro has nslices+1 entries, nslices = ceil(nrows/slice) */
+int alien_hip_host_bellpack_bro(int nrows, int slice, const int* row_offset, int* bro);
+/* block_cols / values scatter and L/U masks (SYCLBEllPackMatrix.cc:127-170, 235-315, 422-460);
+ * arrays have bro[nslices]*slice entries; vals/bvals and masks may be NULL */
+int alien_hip_host_bellpack_fill(int nrows, int slice, const int* row_offset, const int* cols,
+                                 const double* vals, const int* bro, int* bcols, double* bvals,
+                                 uint8_t* lmask, uint8_t* umask);
+/* Level sets of the ILU0 triangular sweeps (replace the strictly sequential row
+ * recurrences of LUFactorisationAlgo::solveL/solveU, ILU0Preconditioner.h:181-231).
+ * upper = 0: lower DAG (row i after rows cols[k], k < dcol[i]); upper = 1: upper DAG.
+ * Returns the number of levels in *nlevels. */
+int alien_hip_host_levels(int nrows, const int* row_offset, const int* cols, const int* dcol,
+                          int upper, int* level, int* nlevels);
+/* Greedy first-fit colouring over the symmetrised pattern, natural row order.
+ * Returns the number of colours in *ncolors. */
+int alien_hip_host_color(int nrows, const int* row_offset, const int* cols, int* color, int* ncolors);
+
+/* ------------------------------------------------------------------------ */
+/* Device kernels: SpMV                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* y = A x, CSR, one lane per row, sequential column order, separate multiply
+ * and add: bit-exact with SimpleCSRMatrixMultT::_seqMult (SimpleCSRMatrixMultT.h:172-191). */
+int alien_hip_spmv_csr_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                           const double* vals, const double* x, double* y);
+int alien_hip_spmv_csr_f32(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                           const float* vals, const float* x, float* y);
+/* y = A x on Alien's sliced-ELL layout (slice height `slice`, padding col 0 / value 0):
+ * MatrixInternal::mult (sycl/data/SYCLBEllPackMatrix.cc:623-688) */
+int alien_hip_spmv_bellpack_f64(alien_hip_ctx* ctx, int nrows, int slice, const int* bro,
+                                const int* bcols, const double* bvals, const double* x, double* y);
+/* y += alpha * (strict lower | strict upper) A x through the BEllPack masks:
+ * MatrixInternal::addLMult / addUMult (SYCLBEllPackMatrix.cc:769-911) */
+int alien_hip_addmult_bellpack_f64(alien_hip_ctx* ctx, int upper, double alpha, int nrows, int slice,
+                                   const int* bro, const int* bcols, const double* bvals,
+                                   const uint8_t* mask, const double* x, double* y);
+/* CSR addLMult / addUMult (SimpleCSRMatrixMultT.h:193-233) */
+int alien_hip_addmult_csr_f64(alien_hip_ctx* ctx, int upper, double alpha, int nrows,
+                              const int* row_offset, const int* cols, const int* dcol,
+                              const double* vals, const double* x, double* y);
+/* y_blk(i) = sum_j B_ij x_blk(j), 2x2 column-major blocks:
+ * SimpleCSRMatrixMultT::_seqMultBlock (SimpleCSRMatrixMultT.h:317-342) */
+int alien_hip_spmv_bsr2_f64(alien_hip_ctx* ctx, int nbrows, const int* row_offset, const int* cols,
+                            const double* vals, const double* x, double* y);
+int alien_hip_spmv_bsr2_f32(alien_hip_ctx* ctx, int nbrows, const int* row_offset, const int* cols,
+                            const float* vals, const float* x, float* y);
+/* inv_diag_i = 1 / a[dcol_i]: SimpleCSRMatrixMultT::computeInvDiag (:494-509).
+ * Every row must store its diagonal (dcol_i then points at a_ii). */
+int alien_hip_invdiag_csr_f64(alien_hip_ctx* ctx, int nrows, const int* dcol, const double* vals,
+                              double* y);
+/* inv_diag_i = 1/a_ii only where a_ii is stored and nonzero (y keeps its value elsewhere):
+ * MatrixInternal::computeInvDiag (SYCLBEllPackMatrix.cc:930-986) */
+int alien_hip_invdiag_bellpack_f64(alien_hip_ctx* ctx, int nrows, int slice, const int* bro,
+                                   const int* bcols, const double* bvals, double* y);
+
+/* Row-bundled CSR ("RB-CSR"): rows in bundles of 64 (one wavefront), entries
+ * interleaved by lane so each value/column load is one coalesced 512/256-byte
+ * access; padding slots hold column -1 and are skipped, so the per-row sum is
+ * the sequential CSR sum, bit-exact with _seqMult. Built on the device from CSR.
+ * nbundles = ceil(nrows/64); bundle_off has nbundles+1 entries (in slots of 64). */
+int alien_hip_rbcsr_bundle_offsets(int nrows, const int* row_offset_host, int* bundle_off_host);
+int alien_hip_rbcsr_fill_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                             const double* vals, const int* bundle_off, int* bcols, double* bvals);
+int alien_hip_rbcsr_fill_f32(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                             const double* vals, const int* bundle_off, int* bcols, float* bvals);
+int alien_hip_spmv_rbcsr_f64(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* bcols,
+                             const double* bvals, const double* x, double* y);
+int alien_hip_spmv_rbcsr_f32(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* bcols,
+                             const float* bvals, const float* x, float* y);
+
+/* ------------------------------------------------------------------------ */
+/* Device kernels: BLAS-1 (CBLASMPIKernel.h:37-125 / SYCLKernelInternal.h:110-246) */
+/* ------------------------------------------------------------------------ */
+int alien_hip_copy_f64(alien_hip_ctx* ctx, int n, const double* x, double* y);          /* y = x */
+int alien_hip_axpy_f64(alien_hip_ctx* ctx, int n, double a, const double* x, double* y); /* y += a x */
+int alien_hip_scal_f64(alien_hip_ctx* ctx, int n, double a, double* x);                  /* x = a x */
+int alien_hip_assign_f64(alien_hip_ctx* ctx, int n, double a, double* x);                /* x = a */
+int alien_hip_pointwise_mult_f64(alien_hip_ctx* ctx, int n, const double* x, const double* y,
+                                 double* z); /* z = x .* y */
+/* *result (device) = x . y, FP64 accumulation, one pass. Replaces the six SYCL
+ * dot variants (SYCLKernelInternal.h:261-1085) and CBLAS ddot. */
+int alien_hip_dot_f64(alien_hip_ctx* ctx, int n, const double* x, const double* y, double* result);
+/* same for FP32 vectors, accumulated in FP64 */
+int alien_hip_dot_f32(alien_hip_ctx* ctx, int n, const float* x, const float* y, double* result);
+/* fused: y += a x, then *result = y . y (axpy + stop-test dot in one pass) */
+int alien_hip_axpy_dot_f64(alien_hip_ctx* ctx, int n, double a, const double* x, double* y,
+                           double* result);
+/* fused: y = A x (CSR, bit-exact with _seqMult), *result = y . x */
+int alien_hip_spmv_dot_csr_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                               const double* vals, const double* x, double* y, double* result);
+
+/* ------------------------------------------------------------------------ */
+/* Device kernels: ILU0 on level-set schedules                               */
+/* ------------------------------------------------------------------------ */
+/* In-place ILU(0) factorisation, IKJ per row, rows of one level in parallel,
+ * (every row must store its diagonal: the pivots are read at dcol),
+ * levels in order: bit-exact with LUFactorisationAlgo::factorize, sequential
+ * branch (ILU0Preconditioner.h:71-179, 156-178). level_rows lists the rows of
+ * level l in level_ptr[l]..level_ptr[l+1] (built on the host from
+ * alien_hip_host_levels with upper = 0). */
+int alien_hip_ilu0_factor_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                              const int* dcol, double* lu, int nlevels, const int* level_ptr_host,
+                              const int* level_rows);
+/* L x1 = y then U x = x1 (solveL/solveU, ILU0Preconditioner.h:181-231), level by
+ * level; t is an n-vector of workspace. lower/upper schedules as above. */
+int alien_hip_ilu0_solve_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                             const int* dcol, const double* lu, int nlev_l, const int* lptr_host,
+                             const int* lrows, int nlev_u, const int* uptr_host, const int* urows,
+                             const double* y, double* t, double* x);
+
+/* ------------------------------------------------------------------------ */
+/* Whole solves through the C++ backend (test_krylov's run(alg) with         */
+/* --kernel hip): examples/test_krylov.cpp:500-665                           */
+/* ------------------------------------------------------------------------ */
+enum { ALIEN_HIP_CG = 0, ALIEN_HIP_BICGS = 1 };
+enum { ALIEN_HIP_PREC_DIAG = 0, ALIEN_HIP_PREC_CHEB = 1, ALIEN_HIP_PREC_ILU0 = 2,
+       ALIEN_HIP_PREC_NONE = 3 };
+enum { ALIEN_HIP_LAYOUT_CSR = 0, ALIEN_HIP_LAYOUT_BELLPACK = 1, ALIEN_HIP_LAYOUT_RBCSR = 2 };
+enum {
+  ALIEN_HIP_MODE_REFERENCE = 0, /* the reference call sequence, one launch per algebra call */
+  ALIEN_HIP_MODE_FUSED = 1,     /* fused kernels, device-resident scalars, host polls stop test */
+  ALIEN_HIP_MODE_GRAPH = 2,     /* fused iteration captured in a hipGraph, batched stop polls */
+  ALIEN_HIP_MODE_MIXED = 3      /* FP32 inner solver inside FP64 iterative refinement */
+};
+
+typedef struct {
+  int    solver;        /* ALIEN_HIP_CG / ALIEN_HIP_BICGS */
+  int    precond;       /* ALIEN_HIP_PREC_* */
+  int    layout;        /* ALIEN_HIP_LAYOUT_* for the SpMV of the solve */
+  int    mode;          /* ALIEN_HIP_MODE_* */
+  double tol;           /* relative residual (Iteration: ||r||^2 < tol^2 ||b||^2) */
+  int    max_iter;
+  double poly_factor;   /* Chebyshev: test_krylov --poly-factor */
+  int    poly_order;    /* --poly-order */
+  int    poly_max_iter; /* --poly-factor-max-iter */
+  int    ilu_schedule;  /* 0 = level sets (natural order, bit-exact), 1 = multicolour */
+  double inner_tol;     /* MIXED: inner relative tolerance */
+  int    inner_max_iter;/* MIXED: inner iteration cap */
+  int    max_outer;     /* MIXED: refinement steps cap */
+  int    poll_every;    /* GRAPH: iterations per graph launch / stop-flag poll */
+} alien_hip_solve_params;
+
+typedef struct {
+  int    iterations;    /* Iteration::operator()() (MIXED: total inner iterations) */
+  int    status;        /* Iteration::getStatus() */
+  int    error;         /* 0, or 1 = NullValueException */
+  int    outer;         /* MIXED: refinement steps */
+  double value;         /* Iteration::getValue() */
+  double true_rel_res;  /* ||b - A x||_2 / ||b||_2, recomputed in FP64 on the GPU */
+  double cheb_alpha;
+  double cheb_beta;
+  double setup_s;       /* conversion + preconditioner init, hipEvent-timed */
+  double solve_s;       /* the Krylov solve, hipEvent-timed */
+} alien_hip_solve_result;
+
+/* Host CSR in, host x out (x holds the initial guess on entry). Uploads,
+ * converts to the requested layout, runs, downloads x. Synchronous. */
+int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_params* p, int nrows,
+                        const int* row_offset, const int* cols, const double* vals,
+                        const double* b, double* x, alien_hip_solve_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALIEN_HIP_H */

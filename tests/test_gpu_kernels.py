@@ -1,0 +1,269 @@
+"""GPU parity of the gfx950 kernels, called through the C ABI, against the CPU
+oracle on the same seeded inputs. Integer/index work and every kernel that
+keeps the reference's summation order must be bit-exact; dot products are
+compared with a tolerance stated in each test."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import alien_amd as aa
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+from gpu_util import call, dev, empty, host, p, row_scale  # noqa: E402
+
+
+def systems():
+    yield "poisson16", aa.gen_csr(aa.GEN_POISSON7, 16)
+    yield "convdiff 13x7x9", aa.gen_csr(aa.GEN_CONVDIFF7, 13, 7, 9)
+    yield "lognormal 20", aa.gen_csr(aa.GEN_LOGNORMAL7, 20, params=[2.0, 1, 1, 1], seed=3)
+    rng = np.random.default_rng(5)
+    A = (sp.random(3001, 3001, density=0.003, random_state=rng, format="csr") + sp.identity(3001)).tocsr()
+    A = A.tolil()
+    for i in rng.choice(3001, 50, replace=False):  # some empty rows, some very long ones
+        A.rows[i] = []
+        A.data[i] = []
+    for i in rng.choice(3001, 5, replace=False):
+        A[i, rng.choice(3001, 200, replace=False)] = 1.5
+    A = A.tocsr()
+    A.sort_indices()
+    yield "ragged 3001", (A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+SYSTEMS = list(systems())
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("name,A", SYSTEMS, ids=[s[0] for s in SYSTEMS])
+def test_spmv_csr_and_rbcsr_bit_exact(ctx, name, A):
+    ro, cols, vals = A
+    n = len(ro) - 1
+    x = np.random.default_rng(1).standard_normal(n)
+    ref = orc.spmv(ro, cols, vals, x)
+    say("case spmv csr/rbcsr", name, n)
+    dro, dc, dv, dx = dev(ro), dev(cols), dev(vals), dev(x)
+    y = empty(n)
+    call("alien_hip_spmv_csr_f64", ctx.h, n, p(dro), p(dc), p(dv), p(dx), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:n], ref)
+    bo = aa.rbcsr_bundle_offsets(ro)
+    slots = int(bo[-1]) * 64
+    dbo = dev(bo)
+    bc = empty(slots, torch.int32)
+    bv = empty(slots)
+    call("alien_hip_rbcsr_fill_f64", ctx.h, n, p(dro), p(dc), p(dv), p(dbo), p(bc), p(bv))
+    y2 = empty(n)
+    call("alien_hip_spmv_rbcsr_f64", ctx.h, n, p(dbo), p(bc), p(bv), p(dx), p(y2))
+    np.testing.assert_array_equal(host(y2, ctx)[:n], ref)
+    # FP32 matrix and vectors: within FP32 rounding of the per-row scale
+    bv32 = empty(slots, torch.float32)
+    call("alien_hip_rbcsr_fill_f32", ctx.h, n, p(dro), p(dc), p(dv), p(dbo), p(bc), p(bv32))
+    dx32 = dev(x.astype(np.float32))
+    y3 = empty(n, torch.float32)
+    call("alien_hip_spmv_rbcsr_f32", ctx.h, n, p(dbo), p(bc), p(bv32), p(dx32), p(y3))
+    scale = row_scale(ro, cols, vals, x)
+    err = np.abs(host(y3, ctx)[:n].astype(np.float64) - ref)
+    assert np.all(err <= 1e-5 * scale + 1e-30)
+    y4 = empty(n, torch.float32)
+    dv32 = dev(vals.astype(np.float32))
+    call("alien_hip_spmv_csr_f32", ctx.h, n, p(dro), p(dc), p(dv32), p(dx32), p(y4))
+    np.testing.assert_array_equal(host(y4, ctx)[:n], host(y3, ctx)[:n])
+
+
+@pytest.mark.parametrize("name,A", SYSTEMS[:3], ids=[s[0] for s in SYSTEMS[:3]])
+def test_spmv_bellpack_bit_exact(ctx, name, A):
+    ro, cols, vals = A
+    n = len(ro) - 1
+    say("case spmv bellpack", name, n)
+    bro, bc, bv, lm, um = aa.bellpack_build(ro, cols, vals, 1024, masks=True)
+    x = np.random.default_rng(2).standard_normal(n)
+    ref = orc.bellpack_spmv(n, bro, bc, bv, x)
+    np.testing.assert_array_equal(ref, orc.spmv(ro, cols, vals, x))
+    dbro, dbc, dbv, dx = dev(bro), dev(bc), dev(bv), dev(x)
+    y = empty(n)
+    call("alien_hip_spmv_bellpack_f64", ctx.h, n, 1024, p(dbro), p(dbc), p(dbv), p(dx), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:n], ref)
+    # masked lower / upper products (addLMult / addUMult)
+    y0 = np.random.default_rng(3).standard_normal(n)
+    for upper, mask in ((0, lm), (1, um)):
+        dy = dev(y0)
+        call("alien_hip_addmult_bellpack_f64", ctx.h, upper, C.c_double(0.75), n, 1024, p(dbro), p(dbc), p(dbv),
+             p(dev(mask)), p(dx), p(dy))
+        got = host(dy, ctx)
+        exp = orc.addmult(ro, cols, vals, 0.75, x, y0, bool(upper))
+        np.testing.assert_allclose(got, exp, rtol=1e-14, atol=1e-14)
+    # inverse diagonal, BEllPack rule (preset kept where a_ii is absent or zero)
+    d = empty(n)
+    call("alien_hip_assign_f64", ctx.h, n, C.c_double(1.0), p(d))
+    call("alien_hip_invdiag_bellpack_f64", ctx.h, n, 1024, p(dbro), p(dbc), p(dbv), p(d))
+    np.testing.assert_array_equal(host(d, ctx)[:n], orc.bellpack_invdiag(n, bro, bc, bv))
+
+
+@pytest.mark.parametrize("name,A", SYSTEMS, ids=[s[0] for s in SYSTEMS])
+def test_addmult_and_invdiag_csr_bit_exact(ctx, name, A):
+    ro, cols, vals = A
+    n = len(ro) - 1
+    say("case addmult/invdiag csr", name, n)
+    d = aa.host_dcol(ro, cols)
+    x = np.random.default_rng(4).standard_normal(n)
+    y0 = np.random.default_rng(5).standard_normal(n)
+    dro, dc, dd, dv, dx = dev(ro), dev(cols), dev(d), dev(vals), dev(x)
+    for upper in (0, 1):
+        dy = dev(y0)
+        call("alien_hip_addmult_csr_f64", ctx.h, upper, C.c_double(-1.25), n, p(dro), p(dc), p(dd), p(dv), p(dx),
+             p(dy))
+        np.testing.assert_array_equal(host(dy, ctx), orc.addmult(ro, cols, vals, -1.25, x, y0, bool(upper)))
+    has_diag = np.array([i in cols[ro[i]:ro[i + 1]] for i in range(n)])
+    if has_diag.all():
+        y = empty(n)
+        call("alien_hip_invdiag_csr_f64", ctx.h, n, p(dd), p(dv), p(y))
+        np.testing.assert_array_equal(host(y, ctx)[:n], orc.invdiag_csr(ro, cols, vals))
+
+
+def test_spmv_bsr2_bit_exact(ctx):
+    ro, cols, vals = aa.gen_bsr2(11, 9, 7, params=[2.0, 0.3], seed=9)
+    nb = len(ro) - 1
+    say("case bsr2", nb)
+    x = np.random.default_rng(6).standard_normal(2 * nb)
+    ref = orc.spmv_bsr(ro, cols, vals, x)
+    y = empty(2 * nb)
+    call("alien_hip_spmv_bsr2_f64", ctx.h, nb, p(dev(ro)), p(dev(cols)), p(dev(vals)), p(dev(x)), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:2 * nb], ref)
+    y32 = empty(2 * nb, torch.float32)
+    call("alien_hip_spmv_bsr2_f32", ctx.h, nb, p(dev(ro)), p(dev(cols)), p(dev(vals.astype(np.float32))),
+         p(dev(x.astype(np.float32))), p(y32))
+    np.testing.assert_allclose(host(y32, ctx)[:2 * nb], ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 1000, 1 << 20])
+def test_blas1_bit_exact_and_dot_tolerance(ctx, n):
+    say("case blas1", n)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    dx, dy = dev(x), dev(y)
+    out = empty(n)
+    call("alien_hip_copy_f64", ctx.h, n, p(dx), p(out))
+    np.testing.assert_array_equal(host(out, ctx)[:n], x)
+    call("alien_hip_axpy_f64", ctx.h, n, C.c_double(-0.3), p(dx), p(dy))
+    np.testing.assert_array_equal(host(dy, ctx)[:n], y + (-0.3) * x)
+    y1 = y + (-0.3) * x
+    call("alien_hip_scal_f64", ctx.h, n, C.c_double(1.7), p(dy))
+    np.testing.assert_array_equal(host(dy, ctx)[:n], 1.7 * y1)
+    call("alien_hip_pointwise_mult_f64", ctx.h, n, p(dx), p(dy), p(out))
+    np.testing.assert_array_equal(host(out, ctx)[:n], x * (1.7 * y1))
+    call("alien_hip_assign_f64", ctx.h, n, C.c_double(2.5), p(out))
+    np.testing.assert_array_equal(host(out, ctx)[:n], np.full(n, 2.5))
+    res = empty(1)
+    call("alien_hip_dot_f64", ctx.h, n, p(dx), p(dy), p(res))
+    exact = float(np.dot(x.astype(np.longdouble), (1.7 * y1).astype(np.longdouble)))
+    tol = 1e-15 * max(1, n) ** 0.5 * float(np.abs(x * (1.7 * y1)).sum()) + 1e-300
+    assert abs(host(res, ctx)[0] - exact) <= tol
+    # determinism: same inputs, same bits
+    res2 = empty(1)
+    call("alien_hip_dot_f64", ctx.h, n, p(dx), p(dy), p(res2))
+    assert host(res2, ctx)[0] == host(res, ctx)[0]
+    # FP32 vectors, FP64 accumulation
+    dx32, dy32 = dev(x.astype(np.float32)), dev(y.astype(np.float32))
+    call("alien_hip_dot_f32", ctx.h, n, p(dx32), p(dy32), p(res))
+    e32 = float(np.dot(x.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64)))
+    assert abs(host(res, ctx)[0] - e32) <= 1e-14 * float(np.abs(x * y).sum()) + 1e-300
+    # fused axpy + dot
+    dy = dev(y)
+    call("alien_hip_axpy_dot_f64", ctx.h, n, C.c_double(0.5), p(dx), p(dy), p(res))
+    yn = y + 0.5 * x
+    np.testing.assert_array_equal(host(dy, ctx)[:n], yn)
+    assert abs(host(res, ctx)[0] - float(np.dot(yn, yn))) <= 1e-13 * float(np.dot(yn, yn)) + 1e-300
+
+
+def test_spmv_dot_fused(ctx):
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 24)
+    n = len(ro) - 1
+    say("case spmv+dot", n)
+    x = np.random.default_rng(8).standard_normal(n)
+    y = empty(n)
+    res = empty(1)
+    call("alien_hip_spmv_dot_csr_f64", ctx.h, n, p(dev(ro)), p(dev(cols)), p(dev(vals)), p(dev(x)), p(y), p(res))
+    ref = orc.spmv(ro, cols, vals, x)
+    np.testing.assert_array_equal(host(y, ctx)[:n], ref)
+    assert abs(host(res, ctx)[0] - np.dot(ref, x)) <= 1e-13 * np.abs(ref * x).sum()
+
+
+def level_schedule(ro, cols, d, upper):
+    lv, nl = aa.host_levels(ro, cols, d, upper)
+    ptr, rows = aa.group_by_level(lv, nl)
+    return nl, ptr, rows
+
+
+@pytest.mark.parametrize("name,A", SYSTEMS[:3], ids=[s[0] for s in SYSTEMS[:3]])
+def test_ilu0_factor_and_solve_bit_exact(ctx, name, A):
+    ro, cols, vals = A
+    n = len(ro) - 1
+    say("case ilu0", name, n)
+    d = aa.host_dcol(ro, cols)
+    nl, lptr, lrows = level_schedule(ro, cols, d, False)
+    nu, uptr, urows = level_schedule(ro, cols, d, True)
+    dro, dc, dd = dev(ro), dev(cols), dev(d)
+    lu = dev(vals)
+    call("alien_hip_ilu0_factor_f64", ctx.h, n, p(dro), p(dc), p(dd), p(lu), nl, aa._capi.iptr(lptr),
+         p(dev(lrows)))
+    ref_lu = orc.ilu0_factor(ro, cols, vals)
+    np.testing.assert_array_equal(host(lu, ctx), ref_lu)
+    yv = np.random.default_rng(9).standard_normal(n)
+    t = empty(n)
+    x = empty(n)
+    call("alien_hip_ilu0_solve_f64", ctx.h, n, p(dro), p(dc), p(dd), p(lu), nl, aa._capi.iptr(lptr), p(dev(lrows)),
+         nu, aa._capi.iptr(uptr), p(dev(urows)), p(dev(yv)), p(t), p(x))
+    np.testing.assert_array_equal(host(x, ctx)[:n], orc.ilu0_solve(ro, cols, ref_lu, yv))
+
+
+def test_ilu0_multicolour_permuted_bit_exact(ctx):
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 12)
+    n = len(ro) - 1
+    say("case ilu0 multicolour", n)
+    c, nc = aa.host_color(ro, cols)
+    perm = np.argsort(c, kind="stable")
+    A = sp.csr_matrix((vals, cols, ro), shape=(n, n))[perm][:, perm].tocsr()
+    A.sort_indices()
+    pro, pcols, pvals = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+    d = aa.host_dcol(pro, pcols)
+    nl, lptr, lrows = level_schedule(pro, pcols, d, False)
+    nu, uptr, urows = level_schedule(pro, pcols, d, True)
+    assert nl <= nc and nu <= nc
+    lu = dev(pvals)
+    call("alien_hip_ilu0_factor_f64", ctx.h, n, p(dev(pro)), p(dev(pcols)), p(dev(d)), p(lu), nl,
+         aa._capi.iptr(lptr), p(dev(lrows)))
+    np.testing.assert_array_equal(host(lu, ctx), orc.ilu0_factor(pro, pcols, pvals))
+
+
+def test_full_size_spmv_properties_256(ctx):
+    """256^3 log-normal system (N = 16.8M, nnz = 117M): RB-CSR and CSR kernels
+    agree bit for bit, and A.1 equals the row sums computed on the host."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 256, params=[2.0, 1, 1, 1], seed=42)
+    n = len(ro) - 1
+    say("case full-size 256^3 spmv", n, len(vals))
+    dro, dc, dv = dev(ro), dev(cols), dev(vals)
+    ones = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = empty(n)
+    call("alien_hip_spmv_csr_f64", ctx.h, n, p(dro), p(dc), p(dv), p(ones), p(y))
+    rs = np.add.reduceat(vals, ro[:-1])  # sequential left-to-right per row == _seqMult order on ones
+    yh = host(y, ctx)
+    np.testing.assert_allclose(yh, rs, rtol=1e-12, atol=1e-12 * np.abs(vals).max())
+    bo = aa.rbcsr_bundle_offsets(ro)
+    dbo = dev(bo)
+    bc = empty(int(bo[-1]) * 64, torch.int32)
+    bv = empty(int(bo[-1]) * 64)
+    call("alien_hip_rbcsr_fill_f64", ctx.h, n, p(dro), p(dc), p(dv), p(dbo), p(bc), p(bv))
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y1 = empty(n)
+    y2 = empty(n)
+    call("alien_hip_spmv_csr_f64", ctx.h, n, p(dro), p(dc), p(dv), p(x), p(y1))
+    call("alien_hip_spmv_rbcsr_f64", ctx.h, n, p(dbo), p(bc), p(bv), p(x), p(y2))
+    ctx.synchronize()
+    assert torch.equal(y1, y2)

@@ -1,0 +1,122 @@
+"""GPU Krylov solves through the C ABI (test_krylov's run(alg) with --kernel hip)
+against the CPU oracle on the same inputs: same relative residual target,
+iteration counts within +-2, true residual recomputed in FP64."""
+import numpy as np
+import pytest
+
+import alien_amd as aa
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def c0():
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 32)
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, 32 ** 3))
+    return ro, cols, vals, b
+
+
+ORC_PREC = {aa.PREC_DIAG: orc.PREC_DIAG, aa.PREC_CHEB: orc.PREC_CHEB, aa.PREC_ILU0: orc.PREC_ILU0,
+            aa.PREC_NONE: orc.PREC_NONE}
+
+
+@pytest.mark.parametrize("layout", [aa.LAYOUT_CSR, aa.LAYOUT_BELLPACK, aa.LAYOUT_RBCSR])
+@pytest.mark.parametrize("precond", [aa.PREC_CHEB, aa.PREC_DIAG])
+def test_c0_cg_reference_mode(ctx, layout, precond):
+    ro, cols, vals, b = c0()
+    say("case C0 CG reference", layout, precond)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=precond, layout=layout, tol=1e-8)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=ORC_PREC[precond], tol=1e-8)
+    assert r["status"] == 1 and r["error"] == 0
+    assert abs(r["iterations"] - o["iterations"]) <= 2
+    assert r["true_rel_res"] < 1e-8
+    assert r["true_rel_res"] == pytest.approx(o["true_rel_res"], rel=0.05)
+    if precond == aa.PREC_CHEB:
+        assert r["cheb_alpha"] == pytest.approx(o["cheb_alpha"], rel=1e-12)
+        assert r["cheb_beta"] == pytest.approx(o["cheb_beta"], rel=1e-12)
+    xo, _ = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=ORC_PREC[precond], tol=1e-8)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH])
+@pytest.mark.parametrize("precond", [aa.PREC_CHEB, aa.PREC_DIAG])
+def test_c0_cg_fused_and_graph(ctx, mode, precond):
+    ro, cols, vals, b = c0()
+    say("case C0 CG fused", mode, precond)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=precond, mode=mode, tol=1e-8, poll_every=5)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=ORC_PREC[precond], tol=1e-8)
+    assert r["status"] == 1 and r["error"] == 0
+    assert abs(r["iterations"] - o["iterations"]) <= 2
+    assert r["true_rel_res"] < 1e-8
+
+
+def test_c0_cg_graph_matches_fused_exactly(ctx):
+    ro, cols, vals, b = c0()
+    say("case C0 graph vs fused determinism")
+    x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_FUSED)
+    x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_GRAPH, poll_every=7)
+    assert r1["iterations"] == r2["iterations"]
+    np.testing.assert_array_equal(x1, x2)
+
+
+def test_c0_mixed_precision_reaches_fp64_tolerance(ctx):
+    ro, cols, vals, b = c0()
+    say("case C0 mixed")
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_MIXED, tol=1e-8, inner_tol=1e-4, max_iter=2000)
+    assert r["status"] == 1
+    assert r["true_rel_res"] < 1e-8
+    assert r["outer"] >= 2
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_convdiff_bicgstab_ilu0(ctx, schedule):
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 24)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case conv-diff BiCGStab ILU0", schedule)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_ILU0, layout=aa.LAYOUT_CSR,
+                        tol=1e-8, ilu_schedule=schedule)
+    assert r["status"] == 1 and r["true_rel_res"] < 1e-8
+    if schedule == 0:  # natural-order level sets: the reference preconditioner exactly
+        _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-8)
+        assert abs(r["iterations"] - o["iterations"]) <= 2
+
+
+@pytest.mark.parametrize("precond", [aa.PREC_DIAG, aa.PREC_CHEB, aa.PREC_NONE])
+def test_bicgstab_other_preconditioners(ctx, precond):
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 16)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(7, n))
+    say("case BiCGStab", precond)
+    _, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=precond, layout=aa.LAYOUT_BELLPACK,
+                        tol=1e-8, max_iter=3000)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=ORC_PREC[precond], tol=1e-8, max_iter=3000)
+    assert r["status"] == o["status"]
+    assert abs(r["iterations"] - o["iterations"]) <= 2
+
+
+def test_null_rhs_and_max_iter(ctx):
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 8)
+    n = len(ro) - 1
+    say("case null rhs / max_iter")
+    x, r = aa.solve_csr(ctx, ro, cols, vals, np.zeros(n))
+    assert r["iterations"] == 0 and r["status"] == 1
+    for mode in (aa.MODE_REFERENCE, aa.MODE_FUSED, aa.MODE_GRAPH):
+        _, r = aa.solve_csr(ctx, ro, cols, vals, np.ones(n), precond=aa.PREC_DIAG, max_iter=1, mode=mode)
+        assert r["iterations"] == 1 and r["status"] == 0
+
+
+def test_lognormal_small_cg_chebyshev_and_diag(ctx):
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 20, params=[2.0, 1, 1, 1], seed=42)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case log-normal 20^3")
+    for precond in (aa.PREC_CHEB, aa.PREC_DIAG):
+        _, r = aa.solve_csr(ctx, ro, cols, vals, b, precond=precond, mode=aa.MODE_GRAPH, tol=1e-8, max_iter=3000)
+        _, o = orc.solve(ro, cols, vals, b, precond=ORC_PREC[precond], tol=1e-8, max_iter=3000)
+        assert r["status"] == o["status"]
+        assert abs(r["iterations"] - o["iterations"]) <= max(2, 0.02 * o["iterations"])

@@ -1,0 +1,164 @@
+"""Host-side (no GPU) parts of libalien_hip.so: the system generators, the
+BEllPack / RB-CSR builders and the ILU0 schedules, checked bit-exactly against
+the oracle and against independent numpy restatements."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import alien_amd as aa
+import oracle as orc
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64_u01_numpy(seed, n):
+    """splitmix64 (Steele, Lea & Flood 2014) in Python integers -> U[0,1)."""
+    s = seed
+    out = np.empty(n)
+    for i in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & MASK64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        z = z ^ (z >> 31)
+        out[i] = (z >> 11) * 2.0 ** -53
+    return out
+
+
+def test_uniform_stream_bit_exact():
+    np.testing.assert_array_equal(aa.uniform01(42, 1000), splitmix64_u01_numpy(42, 1000))
+    # the well-known first output of splitmix64 seeded with 0
+    assert aa.uniform01(0, 1)[0] == (0xE220A8397B1DCDAF >> 11) * 2.0 ** -53
+
+
+def laplacian3d(nx, ny, nz):
+    def t(n):
+        return sp.diags([-np.ones(n - 1), -np.ones(n - 1)], [-1, 1], shape=(n, n))
+    Ix, Iy, Iz = sp.identity(nx), sp.identity(ny), sp.identity(nz)
+    off = sp.kron(Iz, sp.kron(Iy, t(nx))) + sp.kron(Iz, sp.kron(t(ny), Ix)) + sp.kron(t(nz), sp.kron(Iy, Ix))
+    return (off + 6 * sp.identity(nx * ny * nz)).tocsr()
+
+
+@pytest.mark.parametrize("dims", [(5, 4, 3), (1, 1, 7), (8, 8, 8)])
+def test_poisson_generator_is_kron_laplacian(dims):
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, *dims)
+    A = sp.csr_matrix((vals, cols, ro), shape=(np.prod(dims),) * 2)
+    L = laplacian3d(*dims)
+    assert (A != L).nnz == 0
+    assert len(vals) == aa.gen_nnz(aa.GEN_POISSON7, *dims)
+    for i in range(len(ro) - 1):
+        assert np.all(np.diff(cols[ro[i]:ro[i + 1]]) > 0)
+
+
+def test_convdiff_generator_coefficients():
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 6)
+    A = sp.csr_matrix((vals, cols, ro), shape=(216, 216)).toarray()
+    c = 2 + 6 * (2 + 6 * 2)  # interior cell (2,2,2)
+    assert A[c, c] == 93.5
+    assert A[c, c - 1] == -51 and A[c, c + 1] == -1
+    assert A[c, c - 6] == -26 and A[c, c + 6] == -1
+    assert A[c, c - 36] == -13.5 and A[c, c + 36] == -1
+
+
+def lognormal_numpy(n, sigma, aniso, seed):
+    """Independent restatement of the heterogeneous-diffusion generator."""
+    u = splitmix64_u01_numpy(seed ^ 0xA5A5A5A5A5A5A5A5, n ** 3 + (n ** 3) % 2)
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log(1.0 - u1))
+    g = np.empty(len(u))
+    g[0::2] = r * np.cos(2 * np.pi * u2)
+    g[1::2] = r * np.sin(2 * np.pi * u2)
+    k = np.exp(sigma * g[:n ** 3]).reshape(n, n, n)  # [k][j][i]
+    N = n ** 3
+    A = sp.lil_matrix((N, N))
+    idx = np.arange(N).reshape(n, n, n)
+    for axis, a in zip((2, 1, 0), aniso):  # numpy axis 2 = i (x)
+        sl0 = [slice(None)] * 3
+        sl1 = [slice(None)] * 3
+        sl0[axis] = slice(0, n - 1)
+        sl1[axis] = slice(1, n)
+        ka, kb = k[tuple(sl0)].ravel(), k[tuple(sl1)].ravel()
+        T = a * (2.0 * ka * kb) / (ka + kb)
+        ia, ib = idx[tuple(sl0)].ravel(), idx[tuple(sl1)].ravel()
+        A[ia, ib] = -T
+        A[ib, ia] = -T
+    A = A.tocsr()
+    diag = -np.asarray(A.sum(axis=1)).ravel()
+    diag[idx[:, :, 0].ravel()] += aniso[0] * 2.0 * k[:, :, 0].ravel()
+    return (A + sp.diags(diag)).tocsr()
+
+
+def test_lognormal_generator_matches_numpy_and_is_symmetric():
+    n, sigma, aniso = 6, 2.0, (1.0, 3.0, 0.25)
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=[sigma, *aniso], seed=7)
+    A = sp.csr_matrix((vals, cols, ro), shape=(n ** 3,) * 2)
+    assert (A != A.T).nnz == 0  # bit-exact symmetry
+    B = lognormal_numpy(n, sigma, aniso, 7)
+    np.testing.assert_allclose(A.toarray(), B.toarray(), rtol=1e-13, atol=0)
+    # SPD: Dirichlet on one face only
+    assert np.linalg.eigvalsh(A.toarray()).min() > 0
+
+
+def test_bsr2_generator_structure():
+    ro, cols, vals = aa.gen_bsr2(4, 4, 4, params=[2.0, 0.3], seed=3)
+    nb = len(ro) - 1
+    for i in range(nb):
+        for k in range(ro[i], ro[i + 1]):
+            m = vals[4 * k:4 * k + 4].reshape(2, 2, order="F")
+            # off-diagonal coupling inside each block row is 0.3 x the block's diagonal entry
+            assert m[0, 1] == pytest.approx(0.3 * m[0, 0], rel=1e-15)
+            assert m[1, 0] == pytest.approx(0.3 * m[1, 1], rel=1e-15)
+
+
+def test_dcol_matches_oracle():
+    ro, cols, _ = aa.gen_csr(aa.GEN_POISSON7, 9, 7, 5)
+    np.testing.assert_array_equal(aa.host_dcol(ro, cols), orc.dcol(ro, cols))
+
+
+@pytest.mark.parametrize("n", [5, 17, 33])
+def test_bellpack_builder_bit_exact_vs_oracle(n):
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=[2.0, 1, 1, 1], seed=n)
+    for S in (64, 1024):
+        bro, bc, bv, lm, um = aa.bellpack_build(ro, cols, vals, S, masks=True)
+        obro, obc, obv = orc.bellpack(ro, cols, vals, S)
+        olm, oum = orc.bellpack_masks(ro, cols, obro, S)
+        np.testing.assert_array_equal(bro, obro)
+        np.testing.assert_array_equal(bc, obc)
+        np.testing.assert_array_equal(bv, obv)
+        np.testing.assert_array_equal(lm, olm)
+        np.testing.assert_array_equal(um, oum)
+
+
+def test_bellpack_builder_ragged_rows():
+    rng = np.random.default_rng(4)
+    A = sp.random(1500, 1500, density=0.004, random_state=rng, format="csr") + sp.identity(1500)
+    A = A.tocsr()
+    A.sort_indices()
+    ro, cols, vals = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    bro, bc, bv, _, _ = aa.bellpack_build(ro, cols, vals, 1024)
+    obro, obc, obv = orc.bellpack(ro, cols, vals, 1024)
+    np.testing.assert_array_equal(bro, obro)
+    np.testing.assert_array_equal(bc, obc)
+    np.testing.assert_array_equal(bv, obv)
+
+
+def test_levels_and_colors_match_oracle():
+    ro, cols, _ = aa.gen_csr(aa.GEN_CONVDIFF7, 11, 9, 6)
+    d = aa.host_dcol(ro, cols)
+    for upper in (False, True):
+        lv, nl = aa.host_levels(ro, cols, d, upper)
+        olv, onl = orc.levels(ro, cols, upper)
+        np.testing.assert_array_equal(lv, olv)
+        assert nl == onl
+    c, nc = aa.host_color(ro, cols)
+    oc, onc = orc.color(ro, cols)
+    np.testing.assert_array_equal(c, oc)
+    assert nc == onc == 2
+
+
+def test_rbcsr_bundle_offsets():
+    ro, cols, _ = aa.gen_csr(aa.GEN_POISSON7, 10)
+    bo = aa.rbcsr_bundle_offsets(ro)
+    n = len(ro) - 1
+    widths = [int(np.max(np.diff(ro[b * 64:min(n, b * 64 + 64) + 1]))) for b in range((n + 63) // 64)]
+    np.testing.assert_array_equal(bo, np.concatenate([[0], np.cumsum(widths)]))

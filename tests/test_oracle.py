@@ -1,0 +1,318 @@
+"""Pins the CPU oracle (oracle/alien_oracle.cpp) against what it must reproduce:
+
+* golden runs of the reference's own Krylov templates on the 32^3 Poisson
+  system (recorded in the build's survey: Chebyshev-CG 38 iterations,
+  relative residual 7.03e-9, ALPHA 3.6392 / BETA 1.8196; Diag-CG 108
+  iterations; Chebyshev factor 30 not converged after 1000 iterations,
+  relative residual 3.9e-2);
+* the known-answer tests of src/refsemantic/tests/TestMVExpr.cc:105-171;
+* closed forms and brute force (dense matvec, dense ILU(0) by Saad's IKJ
+  algorithm on a pattern mask, the ILU(0) identity (LU)_ij = a_ij on the
+  pattern, exact LU on tridiagonal matrices, scipy CSR products, level and
+  colouring invariants).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import alien_amd as aa
+import oracle as orc
+
+
+def poisson(n):
+    return aa.gen_csr(aa.GEN_POISSON7, n)
+
+
+def c0_system():
+    ro, cols, vals = poisson(32)
+    xt = aa.uniform01(42, 32 ** 3)
+    b = orc.spmv(ro, cols, vals, xt)
+    return ro, cols, vals, b
+
+
+def random_csr(n, density, seed, diag=True, empty_rows=False):
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=density, random_state=rng, format="csr")
+    if diag:
+        A = A + sp.diags(rng.uniform(4, 8, n))
+    A = A.tocsr()
+    if empty_rows:
+        A = A.tolil()
+        for i in rng.choice(n, size=max(1, n // 10), replace=False):
+            A.rows[i] = []
+            A.data[i] = []
+        A = A.tocsr()
+    A.sort_indices()
+    return (A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+# ---- golden runs of the reference templates ------------------------------------------------
+def test_golden_c0_chebyshev_cg():
+    ro, cols, vals, b = c0_system()
+    x, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_CHEB, tol=1e-8, max_iter=1000,
+                     poly_factor=0.5, poly_order=3, poly_max_iter=10)
+    assert r["iterations"] == 38 and r["status"] == 1
+    assert r["true_rel_res"] == pytest.approx(7.03e-9, rel=2e-3)
+    assert r["cheb_alpha"] == pytest.approx(3.6392, abs=1e-4)
+    assert r["cheb_beta"] == pytest.approx(1.8196, abs=1e-4)
+
+
+def test_golden_c0_diag_cg():
+    ro, cols, vals, b = c0_system()
+    _, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_DIAG)
+    assert r["iterations"] == 108 and r["status"] == 1
+
+
+def test_golden_c0_chebyshev_factor30_stalls():
+    ro, cols, vals, b = c0_system()
+    _, r = orc.solve(ro, cols, vals, b, precond=orc.PREC_CHEB, poly_factor=30.0)
+    assert r["iterations"] == 1000 and r["status"] == 0
+    assert r["true_rel_res"] == pytest.approx(3.9e-2, rel=2e-2)
+
+
+# ---- TestMVExpr.cc known answers (10x10 tridiag(-1,2,-1)) -----------------------------------
+def tridiag10():
+    n = 10
+    A = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
+def test_testmvexpr_known_answers():
+    ro, cols, vals = tridiag10()
+    x = np.ones(10)
+    assert np.sqrt(orc.dot(x, x)) == np.sqrt(10)                        # :105
+    y = orc.spmv(ro, cols, vals, x)
+    assert np.sqrt(orc.dot(y, y)) == np.sqrt(2)                         # :123
+    r = y - orc.spmv(ro, cols, vals, x)
+    assert np.sqrt(orc.dot(r, r)) == 0.0                                # :128
+    yi = np.arange(10, dtype=np.float64)
+    assert orc.dot(x, yi) == 45                                         # :148
+    assert orc.dot(0.5 * x, yi) == 22.5                                 # :153
+    assert orc.dot(x, orc.spmv(ro, cols, vals, yi)) == 9                # :160
+    y2 = orc.spmv(ro, cols, vals, 0.5 * x)
+    assert np.sqrt(orc.dot(y2, y2)) == np.sqrt(0.5)                     # :141
+
+
+# ---- SpMV --------------------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_spmv_matches_scipy_and_dense(seed):
+    ro, cols, vals = random_csr(300, 0.02, seed, empty_rows=True)
+    A = sp.csr_matrix((vals, cols, ro), shape=(300, 300))
+    x = np.random.default_rng(seed).standard_normal(300)
+    y = orc.spmv(ro, cols, vals, x)
+    np.testing.assert_allclose(y, A @ x, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(y, A.toarray() @ x, rtol=1e-13, atol=1e-13)
+
+
+def test_spmv_poisson_row_sums_closed_form():
+    n = 9
+    ro, cols, vals = poisson(n)
+    y = orc.spmv(ro, cols, vals, np.ones(n ** 3))
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    faces = sum((c == e).astype(int) for c in (i, j, k) for e in (0, n - 1)).transpose(2, 1, 0).ravel()
+    np.testing.assert_array_equal(y, faces.astype(np.float64))
+
+
+def test_bsr_matches_dense_blocks():
+    ro, cols, vals = aa.gen_bsr2(4, 3, 2, params=[1.0, 0.3], seed=5)
+    nb = len(ro) - 1
+    D = np.zeros((2 * nb, 2 * nb))
+    for i in range(nb):
+        for k in range(ro[i], ro[i + 1]):
+            m = vals[4 * k:4 * k + 4].reshape(2, 2, order="F")  # m[ieq + 2 iu]
+            D[2 * i:2 * i + 2, 2 * cols[k]:2 * cols[k] + 2] = m
+    x = np.random.default_rng(1).standard_normal(2 * nb)
+    np.testing.assert_allclose(orc.spmv_bsr(ro, cols, vals, x), D @ x, rtol=1e-13, atol=1e-12)
+
+
+def test_addlmult_addumult_brute_force():
+    ro, cols, vals = random_csr(200, 0.03, 7)
+    A = sp.csr_matrix((vals, cols, ro), shape=(200, 200)).toarray()
+    x = np.random.default_rng(3).standard_normal(200)
+    y0 = np.random.default_rng(4).standard_normal(200)
+    np.testing.assert_allclose(orc.addmult(ro, cols, vals, 0.7, x, y0, False), y0 + 0.7 * np.tril(A, -1) @ x,
+                               rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(orc.addmult(ro, cols, vals, -1.3, x, y0, True), y0 - 1.3 * np.triu(A, 1) @ x,
+                               rtol=1e-13, atol=1e-13)
+
+
+def test_dcol_brute_force():
+    ro, cols, vals = random_csr(150, 0.05, 9, diag=False, empty_rows=True)
+    d = orc.dcol(ro, cols)
+    for i in range(150):
+        row = cols[ro[i]:ro[i + 1]]
+        assert d[i] == ro[i] + int(np.sum(row < i))
+
+
+# ---- BEllPack (SYCLBEllPackMatrix.cc:69-88,127-170,235-315,422-460) --------------------------------
+def brute_bellpack(ro, cols, vals, S):
+    n = len(ro) - 1
+    ns = (n + S - 1) // S
+    widths = [max([ro[i + 1] - ro[i] for i in range(s * S, min(n, s * S + S))] + [0]) for s in range(ns)]
+    bro = np.concatenate([[0], np.cumsum(widths)]).astype(np.int32)
+    bc = np.zeros(bro[-1] * S, np.int32)
+    bv = np.zeros(bro[-1] * S)
+    for i in range(n):
+        s, l = divmod(i, S)
+        for j in range(ro[i + 1] - ro[i]):
+            bc[bro[s] * S + j * S + l] = cols[ro[i] + j]
+            bv[bro[s] * S + j * S + l] = vals[ro[i] + j]
+    return bro, bc, bv
+
+
+@pytest.mark.parametrize("S", [4, 16, 1024])
+def test_bellpack_layout_brute_force(S):
+    ro, cols, vals = random_csr(257, 0.03, 11, empty_rows=True)
+    bro, bc, bv = orc.bellpack(ro, cols, vals, S)
+    ebro, ebc, ebv = brute_bellpack(ro, cols, vals, S)
+    np.testing.assert_array_equal(bro, ebro)
+    np.testing.assert_array_equal(bc, ebc)
+    np.testing.assert_array_equal(bv, ebv)
+    x = np.random.default_rng(0).standard_normal(257)
+    np.testing.assert_array_equal(orc.bellpack_spmv(257, bro, bc, bv, x, S), orc.spmv(ro, cols, vals, x))
+    lm, um = orc.bellpack_masks(ro, cols, bro, S)
+    d = orc.dcol(ro, cols)
+    for i in range(257):
+        s, l = divmod(i, S)
+        for j in range(ro[i + 1] - ro[i]):
+            e = bro[s] * S + j * S + l
+            assert lm[e] == (ro[i] + j < d[i]) and um[e] == (ro[i] + j > d[i])
+
+
+def test_bellpack_invdiag_rule():
+    ro, cols, vals = random_csr(100, 0.05, 12)
+    vals = vals.copy()
+    d = orc.dcol(ro, cols)
+    vals[d[5]] = 0.0  # zero diagonal: keeps the preset
+    bro, bc, bv = orc.bellpack(ro, cols, vals, 16)
+    y = orc.bellpack_invdiag(100, bro, bc, bv, preset=1.0, slice_=16)
+    assert y[5] == 1.0
+    mask = np.arange(100) != 5
+    np.testing.assert_array_equal(y[mask], (1.0 / vals[d])[mask])
+    np.testing.assert_array_equal(orc.invdiag_csr(ro, cols, vals)[mask], (1.0 / vals[d])[mask])
+
+
+# ---- ILU(0) ----------------------------------------------------------------------------------------
+def dense_ilu0_ikj(A, pattern):
+    """Saad, Iterative Methods for Sparse Linear Systems, Alg. 10.4 (IKJ) on a dense copy."""
+    a = A.copy()
+    n = a.shape[0]
+    for i in range(1, n):
+        for k in range(i):
+            if not pattern[i, k]:
+                continue
+            a[i, k] = a[i, k] / a[k, k]
+            for j in range(k + 1, n):
+                if pattern[i, j]:
+                    a[i, j] = a[i, j] - a[i, k] * a[k, j]
+    return a
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_ilu0_matches_textbook_and_pattern_identity(seed):
+    ro, cols, vals = random_csr(60, 0.08, seed)
+    A = sp.csr_matrix((vals, cols, ro), shape=(60, 60)).toarray()
+    pattern = A != 0
+    lu = orc.ilu0_factor(ro, cols, vals)
+    F = sp.csr_matrix((lu, cols, ro), shape=(60, 60)).toarray()
+    np.testing.assert_allclose(F[pattern], dense_ilu0_ikj(A, pattern)[pattern], rtol=1e-12, atol=1e-12)
+    L = np.tril(F, -1) + np.eye(60)
+    U = np.triu(F)
+    np.testing.assert_allclose((L @ U)[pattern], A[pattern], rtol=1e-11, atol=1e-11)
+    y = np.random.default_rng(seed).standard_normal(60)
+    np.testing.assert_allclose(orc.ilu0_solve(ro, cols, lu, y), np.linalg.solve(L @ U, y), rtol=1e-10, atol=1e-10)
+
+
+def test_ilu0_tridiagonal_is_exact_lu():
+    ro, cols, vals = tridiag10()
+    lu = orc.ilu0_factor(ro, cols, vals)
+    y = np.arange(1.0, 11.0)
+    A = sp.csr_matrix((vals, cols, ro), shape=(10, 10)).toarray()
+    np.testing.assert_allclose(orc.ilu0_solve(ro, cols, lu, y), np.linalg.solve(A, y), rtol=1e-13)
+
+
+# ---- schedules ---------------------------------------------------------------------------------------
+def test_levels_poisson_closed_form():
+    n = 7
+    ro, cols, _ = poisson(n)
+    lv, nl = orc.levels(ro, cols, upper=False)
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    np.testing.assert_array_equal(lv, (i + j + k).transpose(2, 1, 0).ravel())
+    assert nl == 3 * n - 2
+    lu, nu = orc.levels(ro, cols, upper=True)
+    np.testing.assert_array_equal(lu, (3 * (n - 1) - (i + j + k)).transpose(2, 1, 0).ravel())
+    assert nu == 3 * n - 2
+
+
+def test_levels_are_longest_paths():
+    ro, cols, _ = random_csr(200, 0.03, 5)
+    d = orc.dcol(ro, cols)
+    lv, nl = orc.levels(ro, cols)
+    for i in range(200):
+        deps = cols[ro[i]:d[i]]
+        assert lv[i] == (0 if len(deps) == 0 else 1 + lv[deps].max())
+    assert nl == lv.max() + 1
+
+
+def test_color_poisson_is_red_black_and_valid():
+    n = 6
+    ro, cols, _ = poisson(n)
+    c, nc = orc.color(ro, cols)
+    assert nc == 2
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    np.testing.assert_array_equal(c, ((i + j + k) % 2).transpose(2, 1, 0).ravel())
+    ro, cols, _ = random_csr(300, 0.02, 8)
+    c, nc = orc.color(ro, cols)
+    for r in range(300):
+        for q in cols[ro[r]:ro[r + 1]]:
+            if q != r:
+                assert c[q] != c[r]
+
+
+# ---- solvers: textbook behaviour ---------------------------------------------------------------------------
+def test_cg_and_bicgstab_reach_numpy_solution():
+    ro, cols, vals = poisson(6)
+    A = sp.csr_matrix((vals, cols, ro), shape=(216, 216)).toarray()
+    b = np.random.default_rng(2).uniform(size=216)
+    for prec in (orc.PREC_DIAG, orc.PREC_CHEB, orc.PREC_ILU0, orc.PREC_NONE):
+        for solver in (orc.ORC_CG, orc.ORC_BICGS):
+            x, r = orc.solve(ro, cols, vals, b, solver=solver, precond=prec, tol=1e-10)
+            assert r["status"] == 1, (solver, prec)
+            np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-8, atol=1e-9)
+
+
+def test_bicgstab_nonsymmetric_convdiff():
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 8)
+    A = sp.csr_matrix((vals, cols, ro), shape=(512, 512)).toarray()
+    assert not np.allclose(A, A.T)
+    b = A @ aa.uniform01(42, 512)
+    x, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-10)
+    assert r["status"] == 1
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-7)
+
+
+def test_null_rhs_and_max_iter():
+    ro, cols, vals = poisson(5)
+    x, r = orc.solve(ro, cols, vals, np.zeros(125))
+    assert r["iterations"] == 0 and r["status"] == 1
+    b = np.ones(125)
+    _, r = orc.solve(ro, cols, vals, b, max_iter=1, precond=orc.PREC_DIAG)
+    assert r["iterations"] == 1 and r["status"] == 0
+
+
+def test_cg_residual_orthogonality_small_spd():
+    # CG's Krylov subspace property: on an n x n SPD system CG is exact after n steps
+    ro, cols, vals = tridiag10()
+    b = np.arange(1.0, 11.0)
+    x, r = orc.solve(ro, cols, vals, b, precond=orc.PREC_NONE, tol=1e-14, max_iter=10)
+    A = sp.csr_matrix((vals, cols, ro), shape=(10, 10)).toarray()
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10)
+    assert r["iterations"] <= 10
+
+
+def test_threaded_oracle_matches_sequential_iterations():
+    ro, cols, vals, b = c0_system()
+    _, r1 = orc.solve(ro, cols, vals, b, precond=orc.PREC_CHEB)
+    _, r4 = orc.solve(ro, cols, vals, b, precond=orc.PREC_CHEB, nthreads=4)
+    assert abs(r1["iterations"] - r4["iterations"]) <= 2 and r4["status"] == 1
