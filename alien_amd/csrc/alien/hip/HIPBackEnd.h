@@ -135,6 +135,7 @@ class HIPMatrix {
   HIPMatrix(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR) : m_ctx(ctx) {
     m_n = a.nrows;
     m_nnz = a.row_offset[m_n];
+    m_ctx->reserve_for(m_n);
     m_kcol.assign(a.row_offset, a.row_offset + m_n + 1);
     m_cols.assign(a.cols, a.cols + m_nnz);
     m_vals.assign(a.vals, a.vals + m_nnz);
@@ -292,7 +293,10 @@ class HIPInternalLinearAlgebra {
   hipStream_t stream() const { return m_ctx->stream; }
 
   static ResourceType resource(const Matrix& A) { return A.resource(); }
-  void allocate(const ResourceType& r, Vector& v) { v.init(r, true); }
+  void allocate(const ResourceType& r, Vector& v) {
+    m_ctx->reserve_for(r.n);
+    v.init(r, true);
+  }
   template <typename T0, typename... T>
   void allocate(const ResourceType& r, T0& v0, T&... args) {
     allocate(r, v0);

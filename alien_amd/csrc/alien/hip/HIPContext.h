@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -81,14 +82,40 @@ class DeviceArray {
 struct alien_hip_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  alien_hip::DeviceArray<double> partials; // kRedBlocks * kMaxSlots
-  alien_hip::DeviceArray<unsigned> ticket;
-  alien_hip::DeviceArray<double> scalars;  // device-resident scalar slots
-  double* host_scalars = nullptr;           // pinned mirror of `scalars`
+  // reduction workspace (see RedWork); grows on demand, never while a graph is captured
+  alien_hip::DeviceArray<double> partials, gpartials;
+  alien_hip::DeviceArray<unsigned> tickets;
+  int red_cap = 0;
+  alien_hip::DeviceArray<double> scalars; // device-resident scalar slots
+  double* host_scalars = nullptr;          // pinned mirror of `scalars`
   int next_slot = 0;
   static constexpr int kSlots = 64;
 
-  alien_hip::RedWork red() { return alien_hip::RedWork{partials.data(), ticket.data()}; }
+  // Make room for reductions launched with up to `nblocks` workgroups.
+  void reserve_red(long nblocks) {
+    if (nblocks <= red_cap) return;
+    long cap = red_cap ? red_cap : 4096;
+    while (cap < nblocks) cap *= 2;
+    const long gcap = (cap + alien_hip::kGroup - 1) / alien_hip::kGroup;
+    ALIEN_HIP_CHECK(hipStreamSynchronize(stream));
+    partials.resize((size_t)alien_hip::kMaxSlots * cap);
+    gpartials.resize((size_t)alien_hip::kMaxSlots * gcap);
+    tickets.resize((size_t)gcap + 1);
+    ALIEN_HIP_CHECK(hipMemsetAsync(tickets.data(), 0, sizeof(unsigned) * (gcap + 1), stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(stream));
+    red_cap = (int)cap;
+  }
+  // Room for every kernel of this library on vectors of n rows.
+  void reserve_for(long n) {
+    long b = alien_hip::bundle_blocks(n);
+    long e = alien_hip::ept_blocks(n);
+    long r = (n + alien_hip::kBlock - 1) / alien_hip::kBlock;
+    reserve_red(std::max(b, std::max(e, r)));
+  }
+  alien_hip::RedWork red() {
+    return alien_hip::RedWork{partials.data(), gpartials.data(), tickets.data(), red_cap,
+                              (red_cap + alien_hip::kGroup - 1) / alien_hip::kGroup};
+  }
   // A fresh device scalar slot (ring of kSlots; callers keep at most a few live).
   int take_slot() {
     int s = next_slot;

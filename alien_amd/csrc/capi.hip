@@ -32,10 +32,8 @@ alien_hip_ctx* make_ctx(void* stream) {
       ALIEN_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       c->own_stream = true;
     }
-    c->partials.resize((size_t)kRedBlocks * kMaxSlots);
-    c->ticket.resize(1);
+    c->reserve_red(4096);
     c->scalars.resize(alien_hip_ctx::kSlots);
-    ALIEN_HIP_CHECK(hipMemsetAsync(c->ticket.data(), 0, sizeof(unsigned), c->stream));
     ALIEN_HIP_CHECK(hipMemsetAsync(c->scalars.data(), 0, sizeof(double) * alien_hip_ctx::kSlots, c->stream));
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&c->host_scalars, sizeof(double) * alien_hip_ctx::kSlots,
                                   hipHostMallocDefault));
@@ -51,7 +49,8 @@ void destroy_ctx(alien_hip_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->partials.reset();
-  c->ticket.reset();
+  c->gpartials.reset();
+  c->tickets.reset();
   c->scalars.reset();
   if (c->host_scalars) (void)hipHostFree(c->host_scalars);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -373,6 +372,7 @@ int alien_hip_dot_f64(alien_hip_ctx* ctx, int n, const double* x, const double* 
   CTX_OK(ctx);
   REQUIRE(n >= 0 && result && (n == 0 || (x && y)), "dot_f64: bad arguments");
   API_BEGIN
+  ctx->reserve_for(n);
   launch_dot<double>(ctx->stream, ctx->red(), n, x, y, result);
   LAUNCHED("dot_f64");
   API_END
@@ -381,6 +381,7 @@ int alien_hip_dot_f32(alien_hip_ctx* ctx, int n, const float* x, const float* y,
   CTX_OK(ctx);
   REQUIRE(n >= 0 && result && (n == 0 || (x && y)), "dot_f32: bad arguments");
   API_BEGIN
+  ctx->reserve_for(n);
   launch_dot<float>(ctx->stream, ctx->red(), n, x, y, result);
   LAUNCHED("dot_f32");
   API_END
@@ -389,6 +390,7 @@ int alien_hip_axpy_dot_f64(alien_hip_ctx* ctx, int n, double a, const double* x,
   CTX_OK(ctx);
   REQUIRE(n >= 0 && result && (n == 0 || (x && y)), "axpy_dot_f64: bad arguments");
   API_BEGIN
+  ctx->reserve_for(n);
   launch_axpy_dot<double>(ctx->stream, ctx->red(), n, a, x, y, result);
   LAUNCHED("axpy_dot_f64");
   API_END
@@ -398,6 +400,7 @@ int alien_hip_spmv_dot_csr_f64(alien_hip_ctx* ctx, int n, const int* ro, const i
   CTX_OK(ctx);
   REQUIRE(n >= 0 && result && (n == 0 || (ro && cols && vals && x && y)), "spmv_dot_csr_f64: bad arguments");
   API_BEGIN
+  ctx->reserve_for(n);
   launch_spmv_dot_csr<double>(ctx->stream, ctx->red(), n, ro, cols, vals, x, y, result);
   LAUNCHED("spmv_dot_csr_f64");
   API_END

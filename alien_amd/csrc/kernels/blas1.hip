@@ -3,49 +3,79 @@
 // KernelInternal (sycl/algebra/SYCLKernelInternal.h:110-246) provide.
 //
 // All element-wise operations round exactly like the sequential reference
-// loops (built with -ffp-contract=off: no fused multiply-add). Dot products
-// accumulate in FP64 in one launch: each workgroup reduces its grid-stride
-// share, the last workgroup combines the partials in workgroup order and
-// writes the scalar to device memory.
+// loops (built with -ffp-contract=off: no fused multiply-add). Each lane
+// handles kEPT elements spaced one workgroup apart (coalesced), loading all of
+// them before using any. Dot products accumulate in FP64 in one launch and
+// leave the scalar in device memory (see grid_reduce_finish).
 #include "../alien_hip_internal.h"
-#include "launch.h"
 #include "device_common.h"
+#include "launch.h"
 
 namespace alien_hip {
 
+#define EPT_INDEX(k) (blockIdx.x * (long)(kBlock * kEPT) + (k) * kBlock + threadIdx.x)
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_copy(int n, const T* __restrict__ x, T* __restrict__ y) {
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = x[i];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) y[i] = x[i];
+  }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_axpy(int n, T a, const T* __restrict__ x, T* __restrict__ y) {
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = y[i] + a * x[i];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) y[i] = y[i] + a * x[i];
+  }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_scal(int n, T a, T* __restrict__ x) {
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = a * x[i];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) x[i] = a * x[i];
+  }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_assign(int n, T a, T* __restrict__ x) {
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = a;
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) x[i] = a;
+  }
 }
 
-// z may alias x or y (pointwiseMult(invD, y, y) in the Chebyshev power method)
+// z may alias x or y (pointwiseMult(invD, y, y) in the Chebyshev power method):
+// each element is read and written by the same lane, so aliasing is harmless.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_pmult(int n, const T* x, const T* y, T* z) {
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) z[i] = x[i] * y[i];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) z[i] = x[i] * y[i];
+  }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_dot(int n, const T* __restrict__ x, const T* __restrict__ y,
                                                  RedWork w, double* __restrict__ out) {
   __shared__ double lds4[4];
+  T xv[kEPT], yv[kEPT];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    xv[k] = (i < n) ? x[i] : T(0);
+    yv[k] = (i < n) ? y[i] : T(0);
+  }
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-    s += (double)x[i] * (double)y[i];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) s += (double)xv[k] * (double)yv[k];
   double v[1] = {block_sum(s, lds4)};
   grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
 }
@@ -55,44 +85,57 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_axpy_dot(int n, T a, const T* __restrict__ x, T* __restrict__ y,
                                                       RedWork w, double* __restrict__ out) {
   __shared__ double lds4[4];
+  T xv[kEPT], yv[kEPT];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    xv[k] = (i < n) ? x[i] : T(0);
+    yv[k] = (i < n) ? y[i] : T(0);
+  }
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    T v = y[i] + a * x[i];
-    y[i] = v;
-    s += (double)v * (double)v;
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    const T v = yv[k] + a * xv[k];
+    if (i < n) {
+      y[i] = v;
+      s += (double)v * (double)v;
+    }
   }
   double v[1] = {block_sum(s, lds4)};
   grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
 }
 
 // ---- launchers --------------------------------------------------------------
+static inline dim3 ept_grid(long n) { return dim3((unsigned)ept_blocks(n)); }
+
 template <typename T>
 void launch_copy(hipStream_t st, int n, const T* x, T* y) {
-  if (n > 0) hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, x, y);
+  if (n > 0) hipLaunchKernelGGL(k_copy<T>, ept_grid(n), dim3(kBlock), 0, st, n, x, y);
 }
 template <typename T>
 void launch_axpy(hipStream_t st, int n, T a, const T* x, T* y) {
-  if (n > 0) hipLaunchKernelGGL(k_axpy<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, a, x, y);
+  if (n > 0) hipLaunchKernelGGL(k_axpy<T>, ept_grid(n), dim3(kBlock), 0, st, n, a, x, y);
 }
 template <typename T>
 void launch_scal(hipStream_t st, int n, T a, T* x) {
-  if (n > 0) hipLaunchKernelGGL(k_scal<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, a, x);
+  if (n > 0) hipLaunchKernelGGL(k_scal<T>, ept_grid(n), dim3(kBlock), 0, st, n, a, x);
 }
 template <typename T>
 void launch_assign(hipStream_t st, int n, T a, T* x) {
-  if (n > 0) hipLaunchKernelGGL(k_assign<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, a, x);
+  if (n > 0) hipLaunchKernelGGL(k_assign<T>, ept_grid(n), dim3(kBlock), 0, st, n, a, x);
 }
 template <typename T>
 void launch_pmult(hipStream_t st, int n, const T* x, const T* y, T* z) {
-  if (n > 0) hipLaunchKernelGGL(k_pmult<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, x, y, z);
+  if (n > 0) hipLaunchKernelGGL(k_pmult<T>, ept_grid(n), dim3(kBlock), 0, st, n, x, y, z);
 }
 template <typename T>
 void launch_dot(hipStream_t st, RedWork w, int n, const T* x, const T* y, double* out) {
-  hipLaunchKernelGGL(k_dot<T>, dim3(red_grid(n)), dim3(kBlock), 0, st, n, x, y, w, out);
+  hipLaunchKernelGGL(k_dot<T>, ept_grid(n), dim3(kBlock), 0, st, n, x, y, w, out);
 }
 template <typename T>
 void launch_axpy_dot(hipStream_t st, RedWork w, int n, T a, const T* x, T* y, double* out) {
-  hipLaunchKernelGGL(k_axpy_dot<T>, dim3(red_grid(n)), dim3(kBlock), 0, st, n, a, x, y, w, out);
+  hipLaunchKernelGGL(k_axpy_dot<T>, ept_grid(n), dim3(kBlock), 0, st, n, a, x, y, w, out);
 }
 
 #define ALIEN_HIP_INST_BLAS1(T)                                                         \

@@ -1,5 +1,5 @@
 // Device helpers shared by the gfx950 kernels: launch shapes, wavefront and
-// workgroup reductions, and the one-pass deterministic reduction epilogue.
+// workgroup reductions, and the one-pass deterministic grid reduction.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,35 +8,30 @@
 
 namespace alien_hip {
 
-constexpr int kWave = 64;       // CDNA wavefront
-constexpr int kBlock = 256;     // 4 waves per workgroup
-constexpr int kRedBlocks = 2048; // largest reduction grid: 8 workgroups per CU on 256 CUs
+constexpr int kWave = 64;    // CDNA wavefront
+constexpr int kBlock = 256;  // 4 waves per workgroup
+constexpr int kEPT = 4;      // elements per thread of the streaming kernels
+constexpr int kGroup = 256;  // workgroups per first-level reduction group
+constexpr int kMaxSlots = 4; // up to 4 fused reductions per launch
 
-// Fixed launch shape for grid-stride vector kernels: enough workgroups to fill
-// the 256 CUs several times over, never more than the work needs.
+// Streaming kernels: one workgroup per kBlock*kEPT elements, each lane issues
+// kEPT independent loads per array before using any (memory-level
+// parallelism), no grid-stride loop.
+inline long ept_blocks(long n) {
+  long g = (n + (long)kBlock * kEPT - 1) / ((long)kBlock * kEPT);
+  return g < 1 ? 1 : g;
+}
+// Row kernels on the RB-CSR layout: one wavefront per 64-row bundle.
+inline long bundle_blocks(long n) {
+  long nb = (n + 63) / 64;
+  long g = (nb + (kBlock / kWave) - 1) / (kBlock / kWave);
+  return g < 1 ? 1 : g;
+}
+// Shape of the simple element-wise kernels (grid-stride).
 inline int grid_for(long n, int per_thread = 1) {
   long g = (n + (long)kBlock * per_thread - 1) / ((long)kBlock * per_thread);
   if (g < 1) g = 1;
   if (g > 8192) g = 8192;
-  return (int)g;
-}
-
-// Reduction grid: a function of n only, so the combination order (and the
-// rounding) is the same on every call.
-inline int red_grid(long n) {
-  long g = (n + kBlock * 4 - 1) / (kBlock * 4);
-  if (g < 1) g = 1;
-  if (g > kRedBlocks) g = kRedBlocks;
-  return (int)g;
-}
-
-// Grid of the fused row kernels (one wavefront per 64-row bundle, grid-stride
-// over bundles): a function of n only, at most kRedBlocks workgroups.
-inline int bundle_grid(long n) {
-  long nb = (n + 63) / 64;
-  long g = (nb + 3) / 4;
-  if (g < 1) g = 1;
-  if (g > kRedBlocks) g = kRedBlocks;
   return (int)g;
 }
 
@@ -58,48 +53,76 @@ __device__ inline double block_sum(double v, double* lds4) {
   return r;
 }
 
-// Workspace of the one-pass reductions: per-workgroup partials and a ticket
-// counter that the last workgroup resets, so no memset is ever needed between
-// launches (graph-capture friendly).
+// Workspace of the one-pass reductions. Level 1: one partial per workgroup;
+// the last workgroup of each group of kGroup workgroups sums its group's
+// partials (level 2); the last group to finish sums the group partials and
+// hands the totals to the caller. Every sum runs in a fixed order, so the
+// result depends only on the launch shape (a function of n): deterministic.
+// Tickets are reset by the workgroups that consume them, so no memset is
+// ever needed between launches (graph-capture friendly).
 struct RedWork {
-  double* partial;    // kRedBlocks * kMaxSlots doubles
-  unsigned* ticket;   // 1 counter
+  double* partial;   // kMaxSlots * cap
+  double* gpartial;  // kMaxSlots * gcap
+  unsigned* tickets; // 1 + gcap
+  int cap;           // workgroups per launch the buffers can hold
+  int gcap;          // = ceil(cap / kGroup)
 };
-constexpr int kMaxSlots = 4; // up to 4 fused dot products per launch
 
-// Epilogue: thread 0 of each workgroup holds `nslot` partial sums in vals[].
-// The last workgroup to finish combines all partials in workgroup order and
-// hands the totals to `fin(slot, total)` (called by thread 0 only).
 template <int NSLOT, typename Fin>
 __device__ inline void grid_reduce_finish(const double (&vals)[NSLOT], RedWork w, Fin fin) {
   __shared__ bool is_last;
   __shared__ double lds[4 * NSLOT];
-  if (threadIdx.x == 0) {
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  const int nblk = gridDim.x;
+  const int grp = blockIdx.x / kGroup;
+  const int ngroups = (nblk + kGroup - 1) / kGroup;
+  const int gsize = min(kGroup, nblk - grp * kGroup);
+  if (tid == 0) {
 #pragma unroll
-    for (int s = 0; s < NSLOT; ++s) w.partial[s * kRedBlocks + blockIdx.x] = vals[s];
+    for (int s = 0; s < NSLOT; ++s) w.partial[(long)s * w.cap + blockIdx.x] = vals[s];
     __threadfence();
-    unsigned t = atomicAdd(w.ticket, 1u);
-    is_last = (t == gridDim.x - 1);
+    const unsigned t = atomicAdd(&w.tickets[1 + grp], 1u);
+    is_last = (t == (unsigned)gsize - 1);
   }
   __syncthreads();
   if (!is_last) return;
   __threadfence();
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wid = threadIdx.x / kWave;
+  // level 2: this group's partials, one per lane
 #pragma unroll
   for (int s = 0; s < NSLOT; ++s) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
-      acc += w.partial[s * kRedBlocks + b];
+    double acc = (tid < gsize) ? w.partial[(long)s * w.cap + (long)grp * kGroup + tid] : 0.0;
     acc = wave_sum(acc);
     if (lane == 0) lds[s * 4 + wid] = acc;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s)
+      w.gpartial[(long)s * w.gcap + grp] = (lds[s * 4 + 0] + lds[s * 4 + 1]) + (lds[s * 4 + 2] + lds[s * 4 + 3]);
+    w.tickets[1 + grp] = 0u;
+    __threadfence();
+    const unsigned t = atomicAdd(&w.tickets[0], 1u);
+    is_last = (t == (unsigned)ngroups - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  // level 3: the group partials, in group order
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    double acc = 0.0;
+    for (int q = tid; q < ngroups; q += kBlock) acc += w.gpartial[(long)s * w.gcap + q];
+    acc = wave_sum(acc);
+    if (lane == 0) lds[s * 4 + wid] = acc;
+  }
+  __syncthreads();
+  if (tid == 0) {
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s)
       fin(s, (lds[s * 4 + 0] + lds[s * 4 + 1]) + (lds[s * 4 + 2] + lds[s * 4 + 3]));
-    *w.ticket = 0u;
+    w.tickets[0] = 0u;
   }
 }
 

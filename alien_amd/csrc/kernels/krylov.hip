@@ -1,28 +1,34 @@
-// Fused Krylov kernels: one Chebyshev-preconditioned CG iteration in six
-// launches, with every step coefficient kept on the device.
+// Fused Krylov kernels: a preconditioned CG iteration in a few launches, with
+// every step coefficient kept on the device.
 //
 // The reference (CG.h:111-145 with ChebyshevPreconditioner::_algo2,
-// ChebyshevPreconditioner.h:392-434) issues, per iteration, 4 SpMVs, 3 dot
-// products read back by the host and 19 BLAS-1 launches. Here:
+// ChebyshevPreconditioner.h:392-434, or DiagPreconditioner.h:61-67) issues
+// per iteration one SpMV, three dot products read back by the host and a
+// string of BLAS-1 launches, plus the preconditioner's own. Here:
 //
-//   cheb_step  x3  row kernel: t = (A y)_i, r = x_i - t, z = invD_i r,
+//   cheb_step x m  row kernel: t = (A y)_i, r = x_i - t, z = invD_i r,
 //                  p_i = b p_i + z, y'_i = y_i + a p_i  (+ rho1 = r.z on the last)
-//   cg_p           p = z + (rho1/rho) p        (p = z on the first iteration)
-//   cg_q           q = A p, pq = p.q
-//   cg_xr          x += alpha p, r -= alpha q, rr = r.r, stop test, ++iter
+//   cg_pq          p' = z + (rho1/rho) p (p' = z on the first step) and q = A p',
+//                  pq = p'.q; the p' of the neighbours is formed on the fly from
+//                  z and p (same operands, same rounding as the stored p'), so
+//                  the update and the SpMV need one pass; p and p' alternate
+//                  between two buffers, selected by a parity bit of the state
+//   cg_xr          x += alpha p', r -= alpha q, rr = r.r, the stop test, ++iter;
+//                  with the Diag preconditioner also z = invD r and rho1 = r.z
+//                  for the next iteration
 //
 // Each element-wise step rounds exactly like the reference's separate BLAS-1
 // calls (same operands, same order, no FMA), so apart from the summation
-// order of the three dot products the iteration is the reference's.
+// order of the dot products the iteration is the reference's.
 //
 // The stop test of Iteration::stop (Iteration.h:83-90) runs in cg_xr's last
 // workgroup, which also counts the iteration; once `done` is set every later
 // kernel returns at entry, so a hipGraph holding several iterations can be
 // replayed without overshooting and the iteration count stays exact.
 //
-// Rows are processed in RB-CSR bundles: one wavefront per 64-row bundle,
-// grid-stride over bundles with a grid that depends on n only (deterministic
-// reductions).
+// Row kernels process RB-CSR bundles, one wavefront per 64-row bundle;
+// streaming kernels handle kEPT elements per lane. Launch shapes depend on n
+// only, so the reductions are deterministic.
 #include "../alien_hip_internal.h"
 #include "device_common.h"
 #include "krylov_state.h"
@@ -30,9 +36,21 @@
 
 namespace alien_hip {
 
-template <typename T>
-__device__ __forceinline__ T rb_sum(long base, int w, const int* __restrict__ bc, const T* __restrict__ bv,
-                                    const T* __restrict__ x) {
+#define EPT_INDEX(k) (blockIdx.x * (long)(kBlock * kEPT) + (k) * kBlock + threadIdx.x)
+
+// Wave-uniform bundle of this wavefront, or -1 past the last bundle.
+__device__ __forceinline__ int my_bundle(int n) {
+  const int b = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) / kWave));
+  return b < (n + kBundle - 1) / kBundle ? b : -1;
+}
+
+// Row sum of the RB-CSR row of this lane over the vector produced by `xs(j)`.
+template <typename T, typename XS>
+__device__ __forceinline__ T rb_sum(RBMat<T> A, int b, XS xs) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int s0 = A.bo[b];
+  const int w = A.bo[b + 1] - s0;
+  const long base = (long)s0 * kBundle + lane;
   T acc = 0;
   int j = 0;
   constexpr int U = 7;
@@ -41,30 +59,23 @@ __device__ __forceinline__ T rb_sum(long base, int w, const int* __restrict__ bc
     T v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      c[u] = __builtin_nontemporal_load(bc + base + (long)(j + u) * kBundle);
-      v[u] = __builtin_nontemporal_load(bv + base + (long)(j + u) * kBundle);
+      c[u] = __builtin_nontemporal_load(A.bc + base + (long)(j + u) * kBundle);
+      v[u] = __builtin_nontemporal_load(A.bv + base + (long)(j + u) * kBundle);
     }
     T xv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) xv[u] = (c[u] >= 0) ? x[c[u]] : T(0);
+    for (int u = 0; u < U; ++u) xv[u] = (c[u] >= 0) ? xs(c[u]) : T(0);
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (c[u] >= 0) acc = acc + v[u] * xv[u];
   }
   for (; j < w; ++j) {
-    const int c = __builtin_nontemporal_load(bc + base + (long)j * kBundle);
-    const T v = __builtin_nontemporal_load(bv + base + (long)j * kBundle);
-    if (c >= 0) acc = acc + v * x[c];
+    const int c = __builtin_nontemporal_load(A.bc + base + (long)j * kBundle);
+    const T v = __builtin_nontemporal_load(A.bv + base + (long)j * kBundle);
+    if (c >= 0) acc = acc + v * xs(c);
   }
   return acc;
 }
-
-#define BUNDLE_LOOP(n)                                                                        \
-  const int lane = threadIdx.x & (kWave - 1);                                                 \
-  const int nb = ((n) + kBundle - 1) / kBundle;                                               \
-  const int wave0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) / kWave)); \
-  const int nwaves = gridDim.x * (kBlock / kWave);                                            \
-  for (int b = wave0; b < nb; b += nwaves)
 
 // r = b - A x  (copy(b,r); mult(A,x,p); axpy(-1,p,r)); out = r.r
 template <typename T>
@@ -73,15 +84,14 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, RBMat<T> A, const T*
                                                       RedWork w, double* __restrict__ out) {
   __shared__ double lds4[4];
   double s = 0.0;
-  BUNDLE_LOOP(n) {
-    const int s0 = A.bo[b];
-    const int wdt = A.bo[b + 1] - s0;
-    const T t = rb_sum<T>((long)s0 * kBundle + lane, wdt, A.bc, A.bv, x);
-    const int row = b * kBundle + lane;
+  const int b = my_bundle(n);
+  if (b >= 0) {
+    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       const T v = rhs[row] + T(-1) * t;
       r[row] = v;
-      s += (double)v * (double)v;
+      s = (double)v * (double)v;
     }
   }
   double v[1] = {block_sum(s, lds4)};
@@ -96,21 +106,19 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, RBMat<T> A, const T
                                                        const T* __restrict__ yin, T* __restrict__ yout,
                                                        T* __restrict__ p, const T* __restrict__ invd, T a, T bcoef,
                                                        CGState* st, RedWork w) {
-  if (st && st->done) return;
+  if (st->done) return;
   __shared__ double lds4[4];
   double s = 0.0;
   const T* src = FIRST ? xin : yin;
-  BUNDLE_LOOP(n) {
-    const int s0 = A.bo[b];
-    const int wdt = A.bo[b + 1] - s0;
-    const T t = rb_sum<T>((long)s0 * kBundle + lane, wdt, A.bc, A.bv, src);
-    const int row = b * kBundle + lane;
+  const int b = my_bundle(n);
+  if (b >= 0) {
+    const T t = rb_sum<T>(A, b, [&](int c) { return src[c]; });
+    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       const T xi = xin[row];
       const T rr = xi + T(-1) * t;
       const T z = invd[row] * rr;
-      T pn;
-      T yo;
+      T pn, yo;
       if (FIRST) {
         pn = z;
         yo = xi + a * pn;
@@ -121,7 +129,7 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, RBMat<T> A, const T
       }
       p[row] = pn;
       yout[row] = yo;
-      if (LAST) s += (double)xi * (double)yo;
+      if (LAST) s = (double)xi * (double)yo;
     }
   }
   if (LAST) {
@@ -136,44 +144,49 @@ __global__ void __launch_bounds__(kBlock) k_diag_apply(int n, const T* __restric
                                                         T* __restrict__ z, CGState* st, RedWork w) {
   if (st->done) return;
   __shared__ double lds4[4];
+  T rv[kEPT], dv[kEPT];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    rv[k] = (i < n) ? r[i] : T(0);
+    dv[k] = (i < n) ? invd[i] : T(0);
+  }
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const T v = invd[i] * r[i];
-    z[i] = v;
-    s += (double)r[i] * (double)v;
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    const T v = dv[k] * rv[k];
+    if (i < n) {
+      z[i] = v;
+      s += (double)rv[k] * (double)v;
+    }
   }
   double v[1] = {block_sum(s, lds4)};
   grid_reduce_finish<1>(v, w, [&](int, double tot) { st->rho1 = tot; });
 }
 
-// p = z + (rho1/rho) p  (axpy(alpha,p,z); copy(z,p)), or p = z on the first step
+// p' = z + beta p (or z), q = A p', pq = p'.q. Buffers: p = P[parity], p' = P[parity ^ 1].
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_cg_p(int n, const T* __restrict__ z, T* __restrict__ p, CGState* st,
-                                                  int first) {
-  if (st->done) return;
-  if (first) {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = z[i];
-    return;
-  }
-  const T beta = (T)(st->rho1 / st->rho);
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = z[i] + beta * p[i];
-}
-
-// q = A p ; pq = p . q ; a zero pq raises the breakdown flag
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_cg_q(int n, RBMat<T> A, const T* __restrict__ p, T* __restrict__ q,
-                                                  CGState* st, RedWork w) {
+__global__ void __launch_bounds__(kBlock) k_cg_pq(int n, RBMat<T> A, const T* __restrict__ z, T* pA, T* pB,
+                                                   T* __restrict__ q, CGState* st, RedWork w, int first) {
   if (st->done) return;
   __shared__ double lds4[4];
+  const int par = st->parity;
+  const T* __restrict__ pold = par ? pB : pA;
+  T* __restrict__ pnew = par ? pA : pB;
+  const T beta = first ? T(0) : (T)(st->rho1 / st->rho);
   double s = 0.0;
-  BUNDLE_LOOP(n) {
-    const int s0 = A.bo[b];
-    const int wdt = A.bo[b + 1] - s0;
-    const T t = rb_sum<T>((long)s0 * kBundle + lane, wdt, A.bc, A.bv, p);
-    const int row = b * kBundle + lane;
+  const int b = my_bundle(n);
+  if (b >= 0) {
+    T t;
+    if (first) t = rb_sum<T>(A, b, [&](int c) { return z[c]; });
+    else t = rb_sum<T>(A, b, [&](int c) { return z[c] + beta * pold[c]; });
+    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
+      const T pi = first ? z[row] : z[row] + beta * pold[row];
+      pnew[row] = pi;
       q[row] = t;
-      s += (double)p[row] * (double)t;
+      s = (double)pi * (double)t;
     }
   }
   double v[1] = {block_sum(s, lds4)};
@@ -186,63 +199,100 @@ __global__ void __launch_bounds__(kBlock) k_cg_q(int n, RBMat<T> A, const T* __r
   });
 }
 
-// alpha = rho1 / pq ; x += alpha p ; r += (-alpha) q ; rr = r.r ; stop test
-template <typename T, typename TX>
-__global__ void __launch_bounds__(kBlock) k_cg_xr(int n, const T* __restrict__ p, const T* __restrict__ q,
-                                                   TX* __restrict__ x, T* __restrict__ r, CGState* st, RedWork w) {
+// alpha = rho1 / pq ; x += alpha p' ; r += (-alpha) q ; rr = r.r ; stop test.
+// DIAG: also z = invD r and rho1' = r.z for the next iteration.
+template <typename T, bool DIAG>
+__global__ void __launch_bounds__(kBlock) k_cg_xr(int n, const T* pA, const T* pB, const T* __restrict__ q,
+                                                   T* __restrict__ x, T* __restrict__ r, const T* __restrict__ invd,
+                                                   T* __restrict__ z, CGState* st, RedWork w) {
   if (st->done) return;
   __shared__ double lds4[4];
   const double alpha = st->rho1 / st->pq;
   const T a = (T)alpha;
-  double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    x[i] = x[i] + (TX)a * (TX)p[i];
-    const T v = r[i] + (-a) * q[i];
-    r[i] = v;
-    s += (double)v * (double)v;
+  const T* __restrict__ p = st->parity ? pA : pB; // the p' written by cg_pq
+  T pv[kEPT], qv[kEPT], xv[kEPT], rv[kEPT], dv[kEPT];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    const bool ok = i < n;
+    pv[k] = ok ? p[i] : T(0);
+    qv[k] = ok ? q[i] : T(0);
+    xv[k] = ok ? x[i] : T(0);
+    rv[k] = ok ? r[i] : T(0);
+    if (DIAG) dv[k] = ok ? invd[i] : T(0);
   }
-  double v[1] = {block_sum(s, lds4)};
-  grid_reduce_finish<1>(v, w, [&](int, double tot) {
-    st->rho = st->rho1;
-    st->it += 1;
-    if (st->it >= st->max_it) {
-      st->done = 1; // Iteration::stop returns before the dot; status unchanged
-    } else {
-      st->rr = tot;
-      st->status = (tot < st->crit) ? 1 : 0;
-      st->done = st->status;
+  double srr = 0.0, srz = 0.0;
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) {
+      x[i] = xv[k] + a * pv[k];
+      const T rn = rv[k] + (-a) * qv[k];
+      r[i] = rn;
+      srr += (double)rn * (double)rn;
+      if (DIAG) {
+        const T zn = dv[k] * rn;
+        z[i] = zn;
+        srz += (double)rn * (double)zn;
+      }
+    }
+  }
+  double v[2];
+  v[0] = block_sum(srr, lds4);
+  __syncthreads();
+  v[1] = DIAG ? block_sum(srz, lds4) : 0.0;
+  grid_reduce_finish<2>(v, w, [&](int slot, double tot) {
+    if (slot == 0) {
+      st->rho = st->rho1;
+      st->it += 1;
+      st->parity ^= 1;
+      if (st->it >= st->max_it) {
+        st->done = 1; // Iteration::stop returns before the dot; status unchanged
+      } else {
+        st->rr = tot;
+        st->status = (tot < st->crit) ? 1 : 0;
+        st->done = st->status;
+      }
+    } else if (DIAG) {
+      st->rho1 = tot;
     }
   });
 }
 
 // ---- mixed precision helpers ------------------------------------------------------------
-// r32 = (float)(r64 * inv_scale) with inv_scale = 1/sqrt(rr) read from the device
-__global__ void __launch_bounds__(kBlock) k_demote_scaled(int n, const double* __restrict__ r,
-                                                           const double* __restrict__ rr, float* __restrict__ r32) {
-  const double inv = 1.0 / sqrt(*rr);
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) r32[i] = (float)(r[i] * inv);
+// x64 += (double) x32 ; x32 = 0  (fold the FP32 increment into the FP64 solution)
+__global__ void __launch_bounds__(kBlock) k_fold_increment(int n, float* __restrict__ x32, double* __restrict__ x) {
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) {
+      x[i] = x[i] + (double)x32[i];
+      x32[i] = 0.0f;
+    }
+  }
 }
-// x64 += scale * (double)d32 with scale = sqrt(rr)
-__global__ void __launch_bounds__(kBlock) k_promote_add(int n, const float* __restrict__ d,
-                                                         const double* __restrict__ rr, double* __restrict__ x) {
-  const double sc = sqrt(*rr);
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = x[i] + sc * (double)d[i];
-}
-// narrow copy (FP64 -> FP32)
+// narrow copy (FP64 -> FP32, round to nearest)
 __global__ void __launch_bounds__(kBlock) k_demote(int n, const double* __restrict__ a, float* __restrict__ b) {
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) b[i] = (float)a[i];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) b[i] = (float)a[i];
+  }
 }
 
 // ---- launchers -------------------------------------------------------------------------------
+static inline dim3 bgrid(long n) { return dim3((unsigned)bundle_blocks(n)); }
+static inline dim3 egrid(long n) { return dim3((unsigned)ept_blocks(n)); }
+
 template <typename T>
 void launch_residual(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out) {
-  hipLaunchKernelGGL(k_residual<T>, dim3(bundle_grid(n)), dim3(kBlock), 0, s, n, A, x, b, r, w, out);
+  hipLaunchKernelGGL(k_residual<T>, bgrid(n), dim3(kBlock), 0, s, n, A, x, b, r, w, out);
 }
 
 template <typename T>
 void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin, const T* yin, T* yout, T* p,
                       const T* invd, double a, double b, bool first, bool last, CGState* st) {
-  dim3 g(bundle_grid(n)), blk(kBlock);
+  dim3 g = bgrid(n), blk(kBlock);
   if (first && last)
     hipLaunchKernelGGL((k_cheb_step<T, true, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
   else if (first)
@@ -255,46 +305,42 @@ void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin,
 
 template <typename T>
 void launch_diag_apply(hipStream_t s, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st) {
-  hipLaunchKernelGGL(k_diag_apply<T>, dim3(red_grid(n)), dim3(kBlock), 0, s, n, r, invd, z, st, w);
+  hipLaunchKernelGGL(k_diag_apply<T>, egrid(n), dim3(kBlock), 0, s, n, r, invd, z, st, w);
 }
 
 template <typename T>
-void launch_cg_p(hipStream_t s, int n, const T* z, T* p, CGState* st, int first) {
-  hipLaunchKernelGGL(k_cg_p<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, st, first);
+void launch_cg_pq(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* z, T* pA, T* pB, T* q, CGState* st,
+                  int first) {
+  hipLaunchKernelGGL(k_cg_pq<T>, bgrid(n), dim3(kBlock), 0, s, n, A, z, pA, pB, q, st, w, first);
 }
 
 template <typename T>
-void launch_cg_q(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st) {
-  hipLaunchKernelGGL(k_cg_q<T>, dim3(bundle_grid(n)), dim3(kBlock), 0, s, n, A, p, q, st, w);
+void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* pA, const T* pB, const T* q, T* x, T* r,
+                  const T* invd_or_null, T* z_or_null, CGState* st) {
+  if (invd_or_null)
+    hipLaunchKernelGGL((k_cg_xr<T, true>), egrid(n), dim3(kBlock), 0, s, n, pA, pB, q, x, r, invd_or_null,
+                       z_or_null, st, w);
+  else
+    hipLaunchKernelGGL((k_cg_xr<T, false>), egrid(n), dim3(kBlock), 0, s, n, pA, pB, q, x, r, invd_or_null,
+                       z_or_null, st, w);
 }
 
-template <typename T, typename TX>
-void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* p, const T* q, TX* x, T* r, CGState* st) {
-  hipLaunchKernelGGL((k_cg_xr<T, TX>), dim3(red_grid(n)), dim3(kBlock), 0, s, n, p, q, x, r, st, w);
-}
-
-void launch_demote_scaled(hipStream_t s, int n, const double* r, const double* rr, float* r32) {
-  hipLaunchKernelGGL(k_demote_scaled, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, rr, r32);
-}
-void launch_promote_add(hipStream_t s, int n, const float* d, const double* rr, double* x) {
-  hipLaunchKernelGGL(k_promote_add, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, rr, x);
+void launch_fold_increment(hipStream_t s, int n, float* x32, double* x) {
+  hipLaunchKernelGGL(k_fold_increment, egrid(n), dim3(kBlock), 0, s, n, x32, x);
 }
 void launch_demote(hipStream_t s, int n, const double* a, float* b) {
-  hipLaunchKernelGGL(k_demote, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, b);
+  hipLaunchKernelGGL(k_demote, egrid(n), dim3(kBlock), 0, s, n, a, b);
 }
 
-#define ALIEN_HIP_INST_KRYLOV(T)                                                                       \
-  template void launch_residual<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, double*); \
-  template void launch_cheb_step<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, T*,     \
-                                    const T*, double, double, bool, bool, CGState*);                   \
-  template void launch_diag_apply<T>(hipStream_t, RedWork, int, const T*, const T*, T*, CGState*);     \
-  template void launch_cg_p<T>(hipStream_t, int, const T*, T*, CGState*, int);                         \
-  template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*);
+#define ALIEN_HIP_INST_KRYLOV(T)                                                                           \
+  template void launch_residual<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, double*);  \
+  template void launch_cheb_step<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, T*,      \
+                                    const T*, double, double, bool, bool, CGState*);                    \
+  template void launch_diag_apply<T>(hipStream_t, RedWork, int, const T*, const T*, T*, CGState*);      \
+  template void launch_cg_pq<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, T*, T*, CGState*, int); \
+  template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, const T*, T*, T*, const T*, \
+                                T*, CGState*);
 ALIEN_HIP_INST_KRYLOV(double)
 ALIEN_HIP_INST_KRYLOV(float)
-template void launch_cg_xr<double, double>(hipStream_t, RedWork, int, const double*, const double*, double*,
-                                           double*, CGState*);
-template void launch_cg_xr<float, float>(hipStream_t, RedWork, int, const float*, const float*, float*, float*,
-                                         CGState*);
 
 } // namespace alien_hip

@@ -52,12 +52,13 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot_csr(int n, const int* __res
                                                           RedWork w, double* __restrict__ out) {
   __shared__ double lds4[4];
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
     T acc = 0;
     const int e = ro[i + 1];
     for (int k = ro[i]; k < e; ++k) acc = acc + vals[k] * x[cols[k]];
     y[i] = acc;
-    s += (double)acc * (double)x[i];
+    s = (double)acc * (double)x[i];
   }
   double v[1] = {block_sum(s, lds4)};
   grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
@@ -267,7 +268,7 @@ void launch_spmv_csr(hipStream_t st, int n, const int* ro, const int* cols, cons
 template <typename T>
 void launch_spmv_dot_csr(hipStream_t st, RedWork w, int n, const int* ro, const int* cols, const T* vals,
                          const T* x, T* y, double* out) {
-  hipLaunchKernelGGL(k_spmv_dot_csr<T>, dim3(red_grid(n)), dim3(kBlock), 0, st, n, ro, cols, vals, x, y, w, out);
+  hipLaunchKernelGGL(k_spmv_dot_csr<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, x, y, w, out);
 }
 template <typename T>
 void launch_spmv_rbcsr(hipStream_t st, int n, const int* bo, const int* bc, const T* bv, const T* x, T* y) {

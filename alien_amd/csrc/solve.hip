@@ -149,21 +149,28 @@ struct FusedCG {
   RBMat<T> A;
   const T* invd;
   PrecondSpec P;
-  DeviceArray<T> r, z, pp, q, y0, y1;
+  // r, z: residual and preconditioned residual; pA/pB: the two direction
+  // buffers (the state's parity bit says which is current); cp: the Chebyshev
+  // sweep's own direction vector
+  DeviceArray<T> r, z, pA, pB, q, y0, y1, cp;
   DeviceArray<CGState> st;
   CGState* h_st = nullptr; // pinned
 
   FusedCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
   : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
-    r.resize(n); z.resize(n); pp.resize(n); q.resize(n);
+    ctx->reserve_for(n);
+    r.resize(n); z.resize(n); pA.resize(n); pB.resize(n); q.resize(n);
+    if (P.cheb) cp.resize(n);
     if (P.cheb && P.order > 1) { y0.resize(n); y1.resize(n); }
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
   }
   ~FusedCG() {
+    release_graph();
     if (h_st) (void)hipHostFree(h_st);
   }
 
+  // z = M r and rho1 = r.z into the state
   void precond(const T* rin, T* zout) {
     hipStream_t s = ctx->stream;
     if (!P.cheb) {
@@ -175,23 +182,69 @@ struct FusedCG {
     const T* yin = rin;
     for (int i = 0; i < m; ++i) {
       T* yout = (i == m - 1) ? zout : bufs[i % 2];
-      launch_cheb_step<T>(s, ctx->red(), n, A, rin, yin, yout, pp.data(), invd, P.a[i], P.b[i], i == 0,
+      launch_cheb_step<T>(s, ctx->red(), n, A, rin, yin, yout, cp.data(), invd, P.a[i], P.b[i], i == 0,
                           i == m - 1, st.data());
       yin = yout;
     }
   }
-  // One CG update: z = M r, p = z (+ beta p), q = A p, x, r, stop test.
+  // One CG update: z = M r, p = z (+ beta p), q = A p, x, r, stop test. With
+  // the Diag preconditioner cg_xr already produced z and rho1 for this
+  // iteration, except on the first one.
   void iteration(T* x, bool first) {
     hipStream_t s = ctx->stream;
-    precond(r.data(), z.data());
-    launch_cg_p<T>(s, n, z.data(), pp.data(), st.data(), first ? 1 : 0);
-    launch_cg_q<T>(s, ctx->red(), n, A, pp.data(), q.data(), st.data());
-    launch_cg_xr<T, T>(s, ctx->red(), n, pp.data(), q.data(), x, r.data(), st.data());
+    if (P.cheb || first) precond(r.data(), z.data());
+    launch_cg_pq<T>(s, ctx->red(), n, A, z.data(), pA.data(), pB.data(), q.data(), st.data(), first ? 1 : 0);
+    launch_cg_xr<T>(s, ctx->red(), n, pA.data(), pB.data(), q.data(), x, r.data(), P.cheb ? nullptr : invd,
+                    P.cheb ? nullptr : z.data(), st.data());
+  }
+  // After r has been replaced from outside (reliable update): refresh z and rho1.
+  void residual_replaced() {
+    if (!P.cheb) precond(r.data(), z.data());
   }
   CGState read_state() {
     ALIEN_HIP_CHECK(hipMemcpyAsync(h_st, st.data(), sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
     ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return *h_st;
+  }
+
+  void write_state(const CGState& h) {
+    *h_st = h;
+    ALIEN_HIP_CHECK(hipMemcpyAsync(st.data(), h_st, sizeof(CGState), hipMemcpyHostToDevice, ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+
+  // k loop iterations captured once in a hipGraph (re-captured if x changes)
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  T* g_x = nullptr;
+  int g_k = 0;
+  void release_graph() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    ge = nullptr;
+    g = nullptr;
+  }
+  void launch_graph(T* x, int k) {
+    if (!ge || g_x != x || g_k != k) {
+      release_graph();
+      ALIEN_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < k; ++i) iteration(x, false);
+      ALIEN_HIP_CHECK(hipStreamEndCapture(ctx->stream, &g));
+      ALIEN_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      g_x = x;
+      g_k = k;
+    }
+    ALIEN_HIP_CHECK(hipGraphLaunch(ge, ctx->stream));
+  }
+  // k more iterations (fewer if the device stop flag is raised), then the state
+  CGState advance(T* x, bool graph, int k) {
+    if (graph) {
+      launch_graph(x, k);
+    } else {
+      for (int i = 0; i < k; ++i) iteration(x, false);
+      check_launch("fused CG iteration");
+    }
+    return read_state();
   }
 
   // Solve A x = b from the given x (x_is_zero: skip the initial residual SpMV).
@@ -210,34 +263,15 @@ struct FusedCG {
     h.rr = h.crit + 1;
     h.status = (bb == 0) ? 1 : 0;
     if (bb == 0) return h; // nullRhs
-    ALIEN_HIP_CHECK(hipMemcpyAsync(st.data(), &h, sizeof(CGState), hipMemcpyHostToDevice, s));
-    ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+    write_state(h);
     if (x_is_zero) launch_copy<T>(s, n, b, r.data());
     else launch_residual<T>(s, ctx->red(), n, A, x, b, r.data(), ctx->dslot(slot));
     iteration(x, true);
     check_launch("fused CG first iteration");
     CGState cur = read_state();
-    if (graph && !cur.done) {
-      const int k = std::max(1, poll_every);
-      hipGraph_t g = nullptr;
-      hipGraphExec_t ge = nullptr;
-      ALIEN_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < k; ++i) iteration(x, false);
-      ALIEN_HIP_CHECK(hipStreamEndCapture(s, &g));
-      ALIEN_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-      while (!cur.done) {
-        ALIEN_HIP_CHECK(hipGraphLaunch(ge, s));
-        cur = read_state();
-      }
-      (void)hipGraphExecDestroy(ge);
-      (void)hipGraphDestroy(g);
-    } else {
-      while (!cur.done) {
-        iteration(x, false);
-        check_launch("fused CG iteration");
-        cur = read_state();
-      }
-    }
+    // without a graph, poll after every iteration (the reference's host-side stop test)
+    const int k = graph ? std::max(1, poll_every) : 1;
+    while (!cur.done) cur = advance(x, graph, k);
     return cur;
   }
 };
@@ -268,6 +302,93 @@ void finish_breakdown(FusedCG<T>& F, CGState& s, int* error) {
   s.status = rr < s.crit;
   if (s.status) s.it += 1;
   else *error = 1;
+}
+
+// Mixed-precision CG with reliable updates (Sleijpen & van der Vorst, "Reliable
+// updated residuals in hybrid Bi-CG methods", Computing 56 (1996); the
+// mixed-precision form of Clark et al., "Solving lattice QCD systems of
+// equations using mixed precision solvers on GPUs", CPC 181 (2010)).
+// The CG recurrence runs in FP32 (matrix, preconditioner, vectors; dots and the
+// step coefficients in FP64) on the increment x32 of the FP64 solution. Each
+// time the recursive residual has dropped by delta since the last update (or
+// the FP32 stop test fires), the increment is folded into x64, the true
+// residual b - A x64 is recomputed in FP64 and replaces the FP32 residual, and
+// CG continues with its search direction: no restart, so the Krylov space
+// built so far is kept. Convergence is judged on the FP64 true residual only.
+struct MixedResult {
+  int it = 0, updates = 0, status = 0, error = 0;
+  double rel = 0;
+};
+
+MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMat<float> A32, const float* invd32,
+                              const PrecondSpec& ps, const double* b, double* x, double tol, int max_it,
+                              double delta, int poll_every, bool graph) {
+  MixedResult out;
+  hipStream_t s = ctx->stream;
+  const int slot = ctx->take_slot();
+  launch_dot<double>(s, ctx->red(), n, b, b, ctx->dslot(slot));
+  const double bb = ctx->read_slot(slot);
+  if (bb == 0) {
+    out.status = 1;
+    return out;
+  }
+  const double crit = tol * tol * bb;
+  DeviceArray<double> r64(n);
+  DeviceArray<float> x32(n);
+  ALIEN_HIP_CHECK(hipMemsetAsync(x32.data(), 0, sizeof(float) * n, s));
+  FusedCG<float> F(ctx, n, A32, invd32, ps);
+  const int rslot = ctx->take_slot();
+  launch_residual<double>(s, ctx->red(), n, A64, x, b, r64.data(), ctx->dslot(rslot));
+  double rr64 = ctx->read_slot(rslot);
+  out.rel = std::sqrt(rr64 / bb);
+  if (rr64 < crit) {
+    out.status = 1;
+    return out;
+  }
+  launch_demote(s, n, r64.data(), F.r.data());
+  CGState h{};
+  h.crit = crit;
+  h.max_it = max_it;
+  h.rr = rr64;
+  F.write_state(h);
+  F.iteration(x32.data(), true);
+  check_launch("mixed CG first iteration");
+  CGState cur = F.read_state();
+  double rr_last = rr64;
+  const double d2 = delta * delta;
+  const int k = graph ? std::max(1, poll_every) : 1;
+  for (;;) {
+    const bool update = cur.done || cur.rr < d2 * rr_last;
+    if (!update) {
+      cur = F.advance(x32.data(), graph, k);
+      continue;
+    }
+    // reliable update: x64 += x32, x32 = 0, r64 = b - A x64 in FP64
+    launch_fold_increment(s, n, x32.data(), x);
+    launch_residual<double>(s, ctx->red(), n, A64, x, b, r64.data(), ctx->dslot(rslot));
+    rr64 = ctx->read_slot(rslot);
+    ++out.updates;
+    out.rel = std::sqrt(rr64 / bb);
+    if (rr64 < crit) {
+      out.status = 1;
+      break;
+    }
+    if (cur.breakdown) {
+      out.error = 1;
+      break;
+    }
+    if (cur.it >= max_it || !(rr64 == rr64)) break;
+    launch_demote(s, n, r64.data(), F.r.data());
+    rr_last = rr64;
+    cur.done = 0;
+    cur.status = 0;
+    cur.rr = rr64;
+    F.write_state(cur);
+    F.residual_replaced();
+    cur = F.advance(x32.data(), graph, k);
+  }
+  out.it = cur.it;
+  return out;
 }
 
 double true_residual(Alg& alg, const HIPMatrix& A, const HIPVector& b, const HIPVector& x) {
@@ -397,50 +518,23 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         double bb = alg.dot(b, b);
         res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
       } else {
-        // FP32 matrix, FP32 preconditioner and vectors, FP64 dots; FP64 outer defect correction
+        // FP32 matrix, preconditioner and Krylov vectors, FP64 dots; FP64 defect correction
         A.ensureRBCSR32();
-        DeviceArray<float> invd32(n), r32(n), d32(n);
+        DeviceArray<float> invd32(n);
         launch_demote(ctx->stream, n, invd64, invd32.data());
         RBMat<float> rb32{A.d_bo.data(), A.d_bc.data(), A.d_bv32.data()};
         RBMat<double> rb64{A.d_bo.data(), A.d_bc.data(), A.d_bv.data()};
-        FusedCG<float> F(ctx, n, rb32, invd32.data(), ps);
-        DeviceArray<double> r64(n);
         res->setup_s = tm.stop();
         tm.start();
-        const int slot = ctx->take_slot();
-        launch_dot<double>(ctx->stream, ctx->red(), n, b.data(), b.data(), ctx->dslot(slot));
-        const double bb = ctx->read_slot(slot);
-        const int rslot = ctx->take_slot();
-        int total = 0, outer = 0;
-        bool conv = (bb == 0);
-        double rel = 0.0;
-        const int max_outer = p->max_outer > 0 ? p->max_outer : 50;
-        const double inner_tol = p->inner_tol > 0 ? p->inner_tol : 1e-4;
-        const int inner_max = p->inner_max_iter > 0 ? p->inner_max_iter : p->max_iter;
-        while (!conv) {
-          launch_residual<double>(ctx->stream, ctx->red(), n, rb64, x.data(), b.data(), r64.data(),
-                                  ctx->dslot(rslot));
-          const double rr = ctx->read_slot(rslot);
-          rel = std::sqrt(rr / bb);
-          if (rel < p->tol) { conv = true; break; }
-          if (outer >= max_outer || total >= p->max_iter || !(rr > 0)) break;
-          launch_demote_scaled(ctx->stream, n, r64.data(), ctx->dslot(rslot), r32.data());
-          ALIEN_HIP_CHECK(hipMemsetAsync(d32.data(), 0, sizeof(float) * n, ctx->stream));
-          CGState s = F.solve(r32.data(), d32.data(), true, inner_tol, std::min(inner_max, p->max_iter - total),
-                              true, p->poll_every, nullptr);
-          int err = 0;
-          finish_breakdown(F, s, &err);
-          total += s.it;
-          launch_promote_add(ctx->stream, n, d32.data(), ctx->dslot(rslot), x.data());
-          check_launch("mixed refinement");
-          ++outer;
-          if (err) { res->error = 1; break; }
-        }
+        MixedResult m = mixed_reliable_cg(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol,
+                                          p->max_iter, p->inner_tol > 0 ? p->inner_tol : 0.1, p->poll_every,
+                                          true);
         res->solve_s = tm.stop();
-        res->iterations = total;
-        res->outer = outer;
-        res->status = conv ? 1 : 0;
-        res->value = rel;
+        res->iterations = m.it;
+        res->outer = m.updates;
+        res->status = m.status;
+        res->error = m.error;
+        res->value = m.rel;
       }
     }
     res->true_rel_res = true_residual(alg, A, b, x);

@@ -96,7 +96,14 @@ def test_bicgstab_other_preconditioners(ctx, precond):
                         tol=1e-8, max_iter=3000)
     _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=ORC_PREC[precond], tol=1e-8, max_iter=3000)
     assert r["status"] == o["status"]
-    assert abs(r["iterations"] - o["iterations"]) <= 2
+    if o["status"]:
+        assert abs(r["iterations"] - o["iterations"]) <= 2
+    else:
+        # The reference Chebyshev (start y = x, alpha = beta / factor) diverges under
+        # BiCGStab on this nonsymmetric operator; where the run stops is decided by
+        # rounding (the oracle itself stops at 368, 421, 245, 205 or 117 iterations
+        # with 1, 2, 3, 4 or 8 row blocks), so only the failure is compared.
+        assert r["true_rel_res"] > 1e-2 and o["true_rel_res"] > 1e-2
 
 
 def test_null_rhs_and_max_iter(ctx):
