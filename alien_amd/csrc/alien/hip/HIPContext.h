@@ -83,8 +83,7 @@ struct alien_hip_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   // reduction workspace (see RedWork); grows on demand, never while a graph is captured
-  alien_hip::DeviceArray<double> partials, gpartials;
-  alien_hip::DeviceArray<unsigned> tickets;
+  alien_hip::DeviceArray<double> partials;
   int red_cap = 0;
   alien_hip::DeviceArray<double> scalars; // device-resident scalar slots
   double* host_scalars = nullptr;          // pinned mirror of `scalars`
@@ -96,13 +95,8 @@ struct alien_hip_ctx {
     if (nblocks <= red_cap) return;
     long cap = red_cap ? red_cap : 4096;
     while (cap < nblocks) cap *= 2;
-    const long gcap = (cap + alien_hip::kGroup - 1) / alien_hip::kGroup;
     ALIEN_HIP_CHECK(hipStreamSynchronize(stream));
     partials.resize((size_t)alien_hip::kMaxSlots * cap);
-    gpartials.resize((size_t)alien_hip::kMaxSlots * gcap);
-    tickets.resize((size_t)gcap + 1);
-    ALIEN_HIP_CHECK(hipMemsetAsync(tickets.data(), 0, sizeof(unsigned) * (gcap + 1), stream));
-    ALIEN_HIP_CHECK(hipStreamSynchronize(stream));
     red_cap = (int)cap;
   }
   // Room for every kernel of this library on vectors of n rows.
@@ -113,8 +107,7 @@ struct alien_hip_ctx {
     reserve_red(std::max(b, std::max(e, r)));
   }
   alien_hip::RedWork red() {
-    return alien_hip::RedWork{partials.data(), gpartials.data(), tickets.data(), red_cap,
-                              (red_cap + alien_hip::kGroup - 1) / alien_hip::kGroup};
+    return alien_hip::RedWork{partials.data(), red_cap};
   }
   // A fresh device scalar slot (ring of kSlots; callers keep at most a few live).
   int take_slot() {

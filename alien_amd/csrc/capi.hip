@@ -49,8 +49,6 @@ void destroy_ctx(alien_hip_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->partials.reset();
-  c->gpartials.reset();
-  c->tickets.reset();
   c->scalars.reset();
   if (c->host_scalars) (void)hipHostFree(c->host_scalars);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

@@ -6,7 +6,7 @@
 // loops (built with -ffp-contract=off: no fused multiply-add). Each lane
 // handles kEPT elements spaced one workgroup apart (coalesced), loading all of
 // them before using any. Dot products accumulate in FP64 in one launch and
-// leave the scalar in device memory (see grid_reduce_finish).
+// leave the scalar in device memory (partials + launch_finalize).
 #include "../alien_hip_internal.h"
 #include "device_common.h"
 #include "launch.h"
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(kBlock) k_pmult(int n, const T* x, const T* y,
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_dot(int n, const T* __restrict__ x, const T* __restrict__ y,
-                                                 RedWork w, double* __restrict__ out) {
+                                                 RedWork w) {
   __shared__ double lds4[4];
   T xv[kEPT], yv[kEPT];
 #pragma unroll
@@ -77,13 +77,13 @@ __global__ void __launch_bounds__(kBlock) k_dot(int n, const T* __restrict__ x, 
 #pragma unroll
   for (int k = 0; k < kEPT; ++k) s += (double)xv[k] * (double)yv[k];
   double v[1] = {block_sum(s, lds4)};
-  grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
+  store_partials<1>(v, w);
 }
 
 // y += a x ; out = y . y
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_axpy_dot(int n, T a, const T* __restrict__ x, T* __restrict__ y,
-                                                      RedWork w, double* __restrict__ out) {
+                                                      RedWork w) {
   __shared__ double lds4[4];
   T xv[kEPT], yv[kEPT];
 #pragma unroll
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(kBlock) k_axpy_dot(int n, T a, const T* __rest
     }
   }
   double v[1] = {block_sum(s, lds4)};
-  grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
+  store_partials<1>(v, w);
 }
 
 // ---- launchers --------------------------------------------------------------
@@ -131,11 +131,13 @@ void launch_pmult(hipStream_t st, int n, const T* x, const T* y, T* z) {
 }
 template <typename T>
 void launch_dot(hipStream_t st, RedWork w, int n, const T* x, const T* y, double* out) {
-  hipLaunchKernelGGL(k_dot<T>, ept_grid(n), dim3(kBlock), 0, st, n, x, y, w, out);
+  hipLaunchKernelGGL(k_dot<T>, ept_grid(n), dim3(kBlock), 0, st, n, x, y, w);
+  launch_finalize(st, w, ept_blocks(n), 1, kFinStore, out, nullptr);
 }
 template <typename T>
 void launch_axpy_dot(hipStream_t st, RedWork w, int n, T a, const T* x, T* y, double* out) {
-  hipLaunchKernelGGL(k_axpy_dot<T>, ept_grid(n), dim3(kBlock), 0, st, n, a, x, y, w, out);
+  hipLaunchKernelGGL(k_axpy_dot<T>, ept_grid(n), dim3(kBlock), 0, st, n, a, x, y, w);
+  launch_finalize(st, w, ept_blocks(n), 1, kFinStore, out, nullptr);
 }
 
 #define ALIEN_HIP_INST_BLAS1(T)                                                         \

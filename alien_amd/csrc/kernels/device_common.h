@@ -1,5 +1,5 @@
 // Device helpers shared by the gfx950 kernels: launch shapes, wavefront and
-// workgroup reductions, and the one-pass deterministic grid reduction.
+// workgroup reductions, and the per-workgroup partials of the grid reductions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,7 +11,6 @@ namespace alien_hip {
 constexpr int kWave = 64;    // CDNA wavefront
 constexpr int kBlock = 256;  // 4 waves per workgroup
 constexpr int kEPT = 4;      // elements per thread of the streaming kernels
-constexpr int kGroup = 256;  // workgroups per first-level reduction group
 constexpr int kMaxSlots = 4; // up to 4 fused reductions per launch
 
 // Streaming kernels: one workgroup per kBlock*kEPT elements, each lane issues
@@ -53,76 +52,22 @@ __device__ inline double block_sum(double v, double* lds4) {
   return r;
 }
 
-// Workspace of the one-pass reductions. Level 1: one partial per workgroup;
-// the last workgroup of each group of kGroup workgroups sums its group's
-// partials (level 2); the last group to finish sums the group partials and
-// hands the totals to the caller. Every sum runs in a fixed order, so the
-// result depends only on the launch shape (a function of n): deterministic.
-// Tickets are reset by the workgroups that consume them, so no memset is
-// ever needed between launches (graph-capture friendly).
+// Workspace of the deterministic reductions. A reducing kernel stores one
+// partial per workgroup (store_partials); a second, single-workgroup kernel
+// (launch_finalize, reduce.hip) sums them in a fixed order and consumes the
+// totals. The kernel boundary between the two makes the partials visible on
+// every XCD, so no grid-wide fence or atomic is needed, and the result depends
+// only on the launch shape (a function of n): deterministic.
 struct RedWork {
-  double* partial;   // kMaxSlots * cap
-  double* gpartial;  // kMaxSlots * gcap
-  unsigned* tickets; // 1 + gcap
-  int cap;           // workgroups per launch the buffers can hold
-  int gcap;          // = ceil(cap / kGroup)
+  double* partial; // kMaxSlots * cap
+  int cap;         // workgroups per launch the buffer can hold
 };
 
-template <int NSLOT, typename Fin>
-__device__ inline void grid_reduce_finish(const double (&vals)[NSLOT], RedWork w, Fin fin) {
-  __shared__ bool is_last;
-  __shared__ double lds[4 * NSLOT];
-  const int tid = threadIdx.x;
-  const int lane = tid & (kWave - 1);
-  const int wid = tid / kWave;
-  const int nblk = gridDim.x;
-  const int grp = blockIdx.x / kGroup;
-  const int ngroups = (nblk + kGroup - 1) / kGroup;
-  const int gsize = min(kGroup, nblk - grp * kGroup);
-  if (tid == 0) {
+template <int NSLOT>
+__device__ inline void store_partials(const double (&vals)[NSLOT], RedWork w) {
+  if (threadIdx.x == 0) {
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) w.partial[(long)s * w.cap + blockIdx.x] = vals[s];
-    __threadfence();
-    const unsigned t = atomicAdd(&w.tickets[1 + grp], 1u);
-    is_last = (t == (unsigned)gsize - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  // level 2: this group's partials, one per lane
-#pragma unroll
-  for (int s = 0; s < NSLOT; ++s) {
-    double acc = (tid < gsize) ? w.partial[(long)s * w.cap + (long)grp * kGroup + tid] : 0.0;
-    acc = wave_sum(acc);
-    if (lane == 0) lds[s * 4 + wid] = acc;
-  }
-  __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int s = 0; s < NSLOT; ++s)
-      w.gpartial[(long)s * w.gcap + grp] = (lds[s * 4 + 0] + lds[s * 4 + 1]) + (lds[s * 4 + 2] + lds[s * 4 + 3]);
-    w.tickets[1 + grp] = 0u;
-    __threadfence();
-    const unsigned t = atomicAdd(&w.tickets[0], 1u);
-    is_last = (t == (unsigned)ngroups - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  // level 3: the group partials, in group order
-#pragma unroll
-  for (int s = 0; s < NSLOT; ++s) {
-    double acc = 0.0;
-    for (int q = tid; q < ngroups; q += kBlock) acc += w.gpartial[(long)s * w.gcap + q];
-    acc = wave_sum(acc);
-    if (lane == 0) lds[s * 4 + wid] = acc;
-  }
-  __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int s = 0; s < NSLOT; ++s)
-      fin(s, (lds[s * 4 + 0] + lds[s * 4 + 1]) + (lds[s * 4 + 2] + lds[s * 4 + 3]));
-    w.tickets[0] = 0u;
   }
 }
 

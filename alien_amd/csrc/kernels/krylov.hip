@@ -21,10 +21,11 @@
 // calls (same operands, same order, no FMA), so apart from the summation
 // order of the dot products the iteration is the reference's.
 //
-// The stop test of Iteration::stop (Iteration.h:83-90) runs in cg_xr's last
-// workgroup, which also counts the iteration; once `done` is set every later
-// kernel returns at entry, so a hipGraph holding several iterations can be
-// replayed without overshooting and the iteration count stays exact.
+// Each reduction ends with launch_finalize (reduce.hip), which also applies
+// the stop test of Iteration::stop (Iteration.h:83-90) after cg_xr and counts
+// the iteration; once `done` is set every later kernel returns at entry, so a
+// hipGraph holding several iterations can be replayed without overshooting
+// and the iteration count stays exact.
 //
 // Row kernels process RB-CSR bundles, one wavefront per 64-row bundle;
 // streaming kernels handle kEPT elements per lane. Launch shapes depend on n
@@ -81,7 +82,7 @@ __device__ __forceinline__ T rb_sum(RBMat<T> A, int b, XS xs) {
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_residual(int n, RBMat<T> A, const T* __restrict__ x,
                                                       const T* __restrict__ rhs, T* __restrict__ r,
-                                                      RedWork w, double* __restrict__ out) {
+                                                      RedWork w) {
   __shared__ double lds4[4];
   double s = 0.0;
   const int b = my_bundle(n);
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, RBMat<T> A, const T*
     }
   }
   double v[1] = {block_sum(s, lds4)};
-  grid_reduce_finish<1>(v, w, [&](int, double tot) { *out = tot; });
+  store_partials<1>(v, w);
 }
 
 // One Chebyshev step (ChebyshevPreconditioner::_algo2). FIRST: y_in = x,
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, RBMat<T> A, const T
   }
   if (LAST) {
     double v[1] = {block_sum(s, lds4)};
-    grid_reduce_finish<1>(v, w, [&](int, double tot) { st->rho1 = tot; });
+    store_partials<1>(v, w);
   }
 }
 
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(kBlock) k_diag_apply(int n, const T* __restric
     }
   }
   double v[1] = {block_sum(s, lds4)};
-  grid_reduce_finish<1>(v, w, [&](int, double tot) { st->rho1 = tot; });
+  store_partials<1>(v, w);
 }
 
 // p' = z + beta p (or z), q = A p', pq = p'.q. Buffers: p = P[parity], p' = P[parity ^ 1].
@@ -190,13 +191,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_pq(int n, RBMat<T> A, const T* __
     }
   }
   double v[1] = {block_sum(s, lds4)};
-  grid_reduce_finish<1>(v, w, [&](int, double tot) {
-    st->pq = tot;
-    if (tot == 0) {
-      st->breakdown = 1;
-      st->done = 1;
-    }
-  });
+  store_partials<1>(v, w);
 }
 
 // alpha = rho1 / pq ; x += alpha p' ; r += (-alpha) q ; rr = r.r ; stop test.
@@ -241,22 +236,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_xr(int n, const T* pA, const T* p
   v[0] = block_sum(srr, lds4);
   __syncthreads();
   v[1] = DIAG ? block_sum(srz, lds4) : 0.0;
-  grid_reduce_finish<2>(v, w, [&](int slot, double tot) {
-    if (slot == 0) {
-      st->rho = st->rho1;
-      st->it += 1;
-      st->parity ^= 1;
-      if (st->it >= st->max_it) {
-        st->done = 1; // Iteration::stop returns before the dot; status unchanged
-      } else {
-        st->rr = tot;
-        st->status = (tot < st->crit) ? 1 : 0;
-        st->done = st->status;
-      }
-    } else if (DIAG) {
-      st->rho1 = tot;
-    }
-  });
+  store_partials<2>(v, w);
 }
 
 // ---- mixed precision helpers ------------------------------------------------------------
@@ -286,7 +266,8 @@ static inline dim3 egrid(long n) { return dim3((unsigned)ept_blocks(n)); }
 
 template <typename T>
 void launch_residual(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out) {
-  hipLaunchKernelGGL(k_residual<T>, bgrid(n), dim3(kBlock), 0, s, n, A, x, b, r, w, out);
+  hipLaunchKernelGGL(k_residual<T>, bgrid(n), dim3(kBlock), 0, s, n, A, x, b, r, w);
+  launch_finalize(s, w, bundle_blocks(n), 1, kFinStore, out, nullptr);
 }
 
 template <typename T>
@@ -301,17 +282,20 @@ void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin,
     hipLaunchKernelGGL((k_cheb_step<T, false, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
   else
     hipLaunchKernelGGL((k_cheb_step<T, false, false>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
+  if (last) launch_finalize(s, w, bundle_blocks(n), 1, kFinRho1, nullptr, st);
 }
 
 template <typename T>
 void launch_diag_apply(hipStream_t s, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st) {
   hipLaunchKernelGGL(k_diag_apply<T>, egrid(n), dim3(kBlock), 0, s, n, r, invd, z, st, w);
+  launch_finalize(s, w, ept_blocks(n), 1, kFinRho1, nullptr, st);
 }
 
 template <typename T>
 void launch_cg_pq(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* z, T* pA, T* pB, T* q, CGState* st,
                   int first) {
   hipLaunchKernelGGL(k_cg_pq<T>, bgrid(n), dim3(kBlock), 0, s, n, A, z, pA, pB, q, st, w, first);
+  launch_finalize(s, w, bundle_blocks(n), 1, kFinPQ, nullptr, st);
 }
 
 template <typename T>
@@ -323,6 +307,7 @@ void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* pA, const T* pB, con
   else
     hipLaunchKernelGGL((k_cg_xr<T, false>), egrid(n), dim3(kBlock), 0, s, n, pA, pB, q, x, r, invd_or_null,
                        z_or_null, st, w);
+  launch_finalize(s, w, ept_blocks(n), 2, invd_or_null ? kFinXRDiag : kFinXR, nullptr, st);
 }
 
 void launch_fold_increment(hipStream_t s, int n, float* x32, double* x) {

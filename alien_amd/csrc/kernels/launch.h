@@ -9,6 +9,11 @@
 namespace alien_hip {
 
 struct RedWork;
+struct CGState;
+
+// what the second stage of a reduction does with the totals (reduce.hip)
+enum FinOp { kFinStore = 0, kFinRho1 = 1, kFinPQ = 2, kFinXR = 3, kFinXRDiag = 4 };
+void launch_finalize(hipStream_t, RedWork w, long nblk, int nslot, int op, double* out, CGState* st);
 
 // ---- BLAS-1 (blas1.hip) ----
 template <typename T> void launch_copy(hipStream_t, int n, const T* x, T* y);

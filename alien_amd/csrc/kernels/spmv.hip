@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot_csr(int n, const int* __res
                                                           const int* __restrict__ cols,
                                                           const T* __restrict__ vals,
                                                           const T* __restrict__ x, T* __restrict__ y,
-                                                          RedWork w, double* __restrict__ out) {
+                                                          RedWork w) {
   __shared__ double lds4[4];
   double s = 0.0;
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot_csr(int n, const int* __res
     s = (double)acc * (double)x[i];
   }
   double v[1] = {block_sum(s, lds4)};
-  grid_reduce_finish<1>(v, w, [&](int, double t) { *out = t; });
+  store_partials<1>(v, w);
 }
 
 // ---- RB-CSR -------------------------------------------------------------------------
@@ -268,7 +268,12 @@ void launch_spmv_csr(hipStream_t st, int n, const int* ro, const int* cols, cons
 template <typename T>
 void launch_spmv_dot_csr(hipStream_t st, RedWork w, int n, const int* ro, const int* cols, const T* vals,
                          const T* x, T* y, double* out) {
-  hipLaunchKernelGGL(k_spmv_dot_csr<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, x, y, w, out);
+  if (n <= 0) {
+    launch_finalize(st, w, 0, 1, kFinStore, out, nullptr);
+    return;
+  }
+  hipLaunchKernelGGL(k_spmv_dot_csr<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, x, y, w);
+  launch_finalize(st, w, rows_grid(n).x, 1, kFinStore, out, nullptr);
 }
 template <typename T>
 void launch_spmv_rbcsr(hipStream_t st, int n, const int* bo, const int* bc, const T* bv, const T* x, T* y) {
