@@ -9,8 +9,9 @@
 namespace alien_hip {
 
 constexpr int kWave = 64;    // CDNA wavefront
-constexpr int kBlock = 256;  // 4 waves per workgroup
-constexpr int kEPT = 4;      // elements per thread of the streaming kernels
+constexpr int kBlock = 256;    // 4 waves per workgroup (streaming kernels)
+constexpr int kRowBlock = 512; // 8 waves per workgroup (fused row kernels: one bundle per wave)
+constexpr int kEPT = 8;        // elements per thread of the streaming kernels
 constexpr int kMaxSlots = 4; // up to 4 fused reductions per launch
 
 // Streaming kernels: one workgroup per kBlock*kEPT elements, each lane issues
@@ -20,10 +21,11 @@ inline long ept_blocks(long n) {
   long g = (n + (long)kBlock * kEPT - 1) / ((long)kBlock * kEPT);
   return g < 1 ? 1 : g;
 }
-// Row kernels on the RB-CSR layout: one wavefront per 64-row bundle.
+// Fused row kernels on the RB-CSR layout: one wavefront per 64-row bundle,
+// kRowBlock/64 bundles per workgroup.
 inline long bundle_blocks(long n) {
   long nb = (n + 63) / 64;
-  long g = (nb + (kBlock / kWave) - 1) / (kBlock / kWave);
+  long g = (nb + (kRowBlock / kWave) - 1) / (kRowBlock / kWave);
   return g < 1 ? 1 : g;
 }
 // Shape of the simple element-wise kernels (grid-stride).
@@ -40,15 +42,20 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Sum over a 256-thread workgroup; the result is valid in thread 0.
-__device__ inline double block_sum(double v, double* lds4) {
+// Sum over an NT-thread workgroup (lds holds NT/64 doubles); the result is
+// valid in thread 0. Waves are combined in a fixed order.
+template <int NT = kBlock>
+__device__ inline double block_sum(double v, double* lds) {
   v = wave_sum(v);
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
-  if (lane == 0) lds4[wid] = v;
+  if (lane == 0) lds[wid] = v;
   __syncthreads();
   double r = 0.0;
-  if (threadIdx.x == 0) r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NT / kWave; ++q) r += lds[q];
+  }
   return r;
 }
 

@@ -41,7 +41,7 @@ namespace alien_hip {
 
 // Wave-uniform bundle of this wavefront, or -1 past the last bundle.
 __device__ __forceinline__ int my_bundle(int n) {
-  const int b = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) / kWave));
+  const int b = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kRowBlock + threadIdx.x) / kWave));
   return b < (n + kBundle - 1) / kBundle ? b : -1;
 }
 
@@ -80,10 +80,10 @@ __device__ __forceinline__ T rb_sum(RBMat<T> A, int b, XS xs) {
 
 // r = b - A x  (copy(b,r); mult(A,x,p); axpy(-1,p,r)); out = r.r
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_residual(int n, RBMat<T> A, const T* __restrict__ x,
+__global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const T* __restrict__ x,
                                                       const T* __restrict__ rhs, T* __restrict__ r,
                                                       RedWork w) {
-  __shared__ double lds4[4];
+  __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
   const int b = my_bundle(n);
   if (b >= 0) {
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, RBMat<T> A, const T*
       s = (double)v * (double)v;
     }
   }
-  double v[1] = {block_sum(s, lds4)};
+  double v[1] = {block_sum<kRowBlock>(s, lds4)};
   store_partials<1>(v, w);
 }
 
@@ -103,12 +103,12 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, RBMat<T> A, const T*
 // p = z, y' = x + a z. Otherwise p = b p + z, y' = y + a p. LAST also
 // accumulates rho1 = x . y' into the CG state.
 template <typename T, bool FIRST, bool LAST>
-__global__ void __launch_bounds__(kBlock) k_cheb_step(int n, RBMat<T> A, const T* __restrict__ xin,
+__global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, const T* __restrict__ xin,
                                                        const T* __restrict__ yin, T* __restrict__ yout,
                                                        T* __restrict__ p, const T* __restrict__ invd, T a, T bcoef,
                                                        CGState* st, RedWork w) {
   if (st->done) return;
-  __shared__ double lds4[4];
+  __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
   const T* src = FIRST ? xin : yin;
   const int b = my_bundle(n);
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, RBMat<T> A, const T
     }
   }
   if (LAST) {
-    double v[1] = {block_sum(s, lds4)};
+    double v[1] = {block_sum<kRowBlock>(s, lds4)};
     store_partials<1>(v, w);
   }
 }
@@ -168,10 +168,10 @@ __global__ void __launch_bounds__(kBlock) k_diag_apply(int n, const T* __restric
 
 // p' = z + beta p (or z), q = A p', pq = p'.q. Buffers: p = P[parity], p' = P[parity ^ 1].
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_cg_pq(int n, RBMat<T> A, const T* __restrict__ z, T* pA, T* pB,
+__global__ void __launch_bounds__(kRowBlock) k_cg_pq(int n, RBMat<T> A, const T* __restrict__ z, T* pA, T* pB,
                                                    T* __restrict__ q, CGState* st, RedWork w, int first) {
   if (st->done) return;
-  __shared__ double lds4[4];
+  __shared__ double lds4[kRowBlock / kWave];
   const int par = st->parity;
   const T* __restrict__ pold = par ? pB : pA;
   T* __restrict__ pnew = par ? pA : pB;
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_pq(int n, RBMat<T> A, const T* __
       s = (double)pi * (double)t;
     }
   }
-  double v[1] = {block_sum(s, lds4)};
+  double v[1] = {block_sum<kRowBlock>(s, lds4)};
   store_partials<1>(v, w);
 }
 
@@ -266,14 +266,14 @@ static inline dim3 egrid(long n) { return dim3((unsigned)ept_blocks(n)); }
 
 template <typename T>
 void launch_residual(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out) {
-  hipLaunchKernelGGL(k_residual<T>, bgrid(n), dim3(kBlock), 0, s, n, A, x, b, r, w);
+  hipLaunchKernelGGL(k_residual<T>, bgrid(n), dim3(kRowBlock), 0, s, n, A, x, b, r, w);
   launch_finalize(s, w, bundle_blocks(n), 1, kFinStore, out, nullptr);
 }
 
 template <typename T>
 void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin, const T* yin, T* yout, T* p,
                       const T* invd, double a, double b, bool first, bool last, CGState* st) {
-  dim3 g = bgrid(n), blk(kBlock);
+  dim3 g = bgrid(n), blk(kRowBlock);
   if (first && last)
     hipLaunchKernelGGL((k_cheb_step<T, true, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
   else if (first)
@@ -294,7 +294,7 @@ void launch_diag_apply(hipStream_t s, RedWork w, int n, const T* r, const T* inv
 template <typename T>
 void launch_cg_pq(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* z, T* pA, T* pB, T* q, CGState* st,
                   int first) {
-  hipLaunchKernelGGL(k_cg_pq<T>, bgrid(n), dim3(kBlock), 0, s, n, A, z, pA, pB, q, st, w, first);
+  hipLaunchKernelGGL(k_cg_pq<T>, bgrid(n), dim3(kRowBlock), 0, s, n, A, z, pA, pB, q, st, w, first);
   launch_finalize(s, w, bundle_blocks(n), 1, kFinPQ, nullptr, st);
 }
 

@@ -17,9 +17,21 @@ __global__ void __launch_bounds__(kFin) k_finalize(RedWork w, int nblk, int nslo
   __shared__ double lds[kMaxSlots][kFin / kWave];
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wid = tid / kWave;
+  // lane t sums partials t, t + kFin, t + 2 kFin, ... (fixed order); 8 loads
+  // are issued before they are added
+  constexpr int U = 8;
   for (int s = 0; s < nslot; ++s) {
+    const double* p = w.partial + (long)s * w.cap;
     double acc = 0.0;
-    for (int b = tid; b < nblk; b += kFin) acc += w.partial[(long)s * w.cap + b];
+    int b = tid;
+    for (; b + (U - 1) * kFin < nblk; b += U * kFin) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = p[b + u * kFin];
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += v[u];
+    }
+    for (; b < nblk; b += kFin) acc += p[b];
     acc = wave_sum(acc);
     if (lane == 0) lds[s][wid] = acc;
   }
