@@ -10,8 +10,8 @@ namespace alien_hip {
 
 constexpr int kWave = 64;    // CDNA wavefront
 constexpr int kBlock = 256;    // 4 waves per workgroup (streaming kernels)
-constexpr int kRowBlock = 512; // 8 waves per workgroup (fused row kernels: one bundle per wave)
-constexpr int kEPT = 8;        // elements per thread of the streaming kernels
+constexpr int kRowBlock = 256; // 4 waves per workgroup (fused row kernels: one bundle per wave)
+constexpr int kEPT = 4;        // elements per thread of the streaming kernels
 constexpr int kMaxSlots = 4; // up to 4 fused reductions per launch
 
 // Streaming kernels: one workgroup per kBlock*kEPT elements, each lane issues

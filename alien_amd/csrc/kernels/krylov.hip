@@ -8,12 +8,9 @@
 //
 //   cheb_step x m  row kernel: t = (A y)_i, r = x_i - t, z = invD_i r,
 //                  p_i = b p_i + z, y'_i = y_i + a p_i  (+ rho1 = r.z on the last)
-//   cg_pq          p' = z + (rho1/rho) p (p' = z on the first step) and q = A p',
-//                  pq = p'.q; the p' of the neighbours is formed on the fly from
-//                  z and p (same operands, same rounding as the stored p'), so
-//                  the update and the SpMV need one pass; p and p' alternate
-//                  between two buffers, selected by a parity bit of the state
-//   cg_xr          x += alpha p', r -= alpha q, rr = r.r, the stop test, ++iter;
+//   cg_p           p = z + (rho1/rho) p (p = z on the first step), streaming
+//   cg_q           q = A p, pq = p.q
+//   cg_xr          x += alpha p, r -= alpha q, rr = r.r, the stop test, ++iter;
 //                  with the Diag preconditioner also z = invD r and rho1 = r.z
 //                  for the next iteration
 //
@@ -166,45 +163,57 @@ __global__ void __launch_bounds__(kBlock) k_diag_apply(int n, const T* __restric
   store_partials<1>(v, w);
 }
 
-// p' = z + beta p (or z), q = A p', pq = p'.q. Buffers: p = P[parity], p' = P[parity ^ 1].
+// p = z + (rho1/rho) p  (axpy(alpha,p,z); copy(z,p)), or p = z on the first step.
+// Streaming, in place: each element is read and written by the same lane.
 template <typename T>
-__global__ void __launch_bounds__(kRowBlock) k_cg_pq(int n, RBMat<T> A, const T* __restrict__ z, T* pA, T* pB,
-                                                   T* __restrict__ q, CGState* st, RedWork w, int first) {
+__global__ void __launch_bounds__(kBlock) k_cg_p(int n, const T* __restrict__ z, T* __restrict__ p, CGState* st,
+                                                  int first) {
+  if (st->done) return;
+  const T beta = first ? T(0) : (T)(st->rho1 / st->rho);
+  T zv[kEPT], pv[kEPT];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    zv[k] = (i < n) ? z[i] : T(0);
+    pv[k] = (i < n && !first) ? p[i] : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) p[i] = first ? zv[k] : zv[k] + beta * pv[k];
+  }
+}
+
+// q = A p ; pq = p . q (a zero pq raises the breakdown flag in the finalize)
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* __restrict__ p, T* __restrict__ q,
+                                                     CGState* st, RedWork w) {
   if (st->done) return;
   __shared__ double lds4[kRowBlock / kWave];
-  const int par = st->parity;
-  const T* __restrict__ pold = par ? pB : pA;
-  T* __restrict__ pnew = par ? pA : pB;
-  const T beta = first ? T(0) : (T)(st->rho1 / st->rho);
   double s = 0.0;
   const int b = my_bundle(n);
   if (b >= 0) {
-    T t;
-    if (first) t = rb_sum<T>(A, b, [&](int c) { return z[c]; });
-    else t = rb_sum<T>(A, b, [&](int c) { return z[c] + beta * pold[c]; });
+    const T t = rb_sum<T>(A, b, [&](int c) { return p[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
-      const T pi = first ? z[row] : z[row] + beta * pold[row];
-      pnew[row] = pi;
       q[row] = t;
-      s = (double)pi * (double)t;
+      s = (double)p[row] * (double)t;
     }
   }
   double v[1] = {block_sum<kRowBlock>(s, lds4)};
   store_partials<1>(v, w);
 }
 
-// alpha = rho1 / pq ; x += alpha p' ; r += (-alpha) q ; rr = r.r ; stop test.
+// alpha = rho1 / pq ; x += alpha p ; r += (-alpha) q ; rr = r.r ; stop test.
 // DIAG: also z = invD r and rho1' = r.z for the next iteration.
 template <typename T, bool DIAG>
-__global__ void __launch_bounds__(kBlock) k_cg_xr(int n, const T* pA, const T* pB, const T* __restrict__ q,
+__global__ void __launch_bounds__(kBlock) k_cg_xr(int n, const T* __restrict__ p, const T* __restrict__ q,
                                                    T* __restrict__ x, T* __restrict__ r, const T* __restrict__ invd,
                                                    T* __restrict__ z, CGState* st, RedWork w) {
   if (st->done) return;
   __shared__ double lds4[4];
   const double alpha = st->rho1 / st->pq;
   const T a = (T)alpha;
-  const T* __restrict__ p = st->parity ? pA : pB; // the p' written by cg_pq
   T pv[kEPT], qv[kEPT], xv[kEPT], rv[kEPT], dv[kEPT];
 #pragma unroll
   for (int k = 0; k < kEPT; ++k) {
@@ -292,21 +301,25 @@ void launch_diag_apply(hipStream_t s, RedWork w, int n, const T* r, const T* inv
 }
 
 template <typename T>
-void launch_cg_pq(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* z, T* pA, T* pB, T* q, CGState* st,
-                  int first) {
-  hipLaunchKernelGGL(k_cg_pq<T>, bgrid(n), dim3(kRowBlock), 0, s, n, A, z, pA, pB, q, st, w, first);
+void launch_cg_p(hipStream_t s, int n, const T* z, T* p, CGState* st, int first) {
+  hipLaunchKernelGGL(k_cg_p<T>, egrid(n), dim3(kBlock), 0, s, n, z, p, st, first);
+}
+
+template <typename T>
+void launch_cg_q(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st) {
+  hipLaunchKernelGGL(k_cg_q<T>, bgrid(n), dim3(kRowBlock), 0, s, n, A, p, q, st, w);
   launch_finalize(s, w, bundle_blocks(n), 1, kFinPQ, nullptr, st);
 }
 
 template <typename T>
-void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* pA, const T* pB, const T* q, T* x, T* r,
-                  const T* invd_or_null, T* z_or_null, CGState* st) {
+void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* p, const T* q, T* x, T* r, const T* invd_or_null,
+                  T* z_or_null, CGState* st) {
   if (invd_or_null)
-    hipLaunchKernelGGL((k_cg_xr<T, true>), egrid(n), dim3(kBlock), 0, s, n, pA, pB, q, x, r, invd_or_null,
-                       z_or_null, st, w);
+    hipLaunchKernelGGL((k_cg_xr<T, true>), egrid(n), dim3(kBlock), 0, s, n, p, q, x, r, invd_or_null, z_or_null,
+                       st, w);
   else
-    hipLaunchKernelGGL((k_cg_xr<T, false>), egrid(n), dim3(kBlock), 0, s, n, pA, pB, q, x, r, invd_or_null,
-                       z_or_null, st, w);
+    hipLaunchKernelGGL((k_cg_xr<T, false>), egrid(n), dim3(kBlock), 0, s, n, p, q, x, r, invd_or_null, z_or_null,
+                       st, w);
   launch_finalize(s, w, ept_blocks(n), 2, invd_or_null ? kFinXRDiag : kFinXR, nullptr, st);
 }
 
@@ -322,9 +335,10 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
   template void launch_cheb_step<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, T*,      \
                                     const T*, double, double, bool, bool, CGState*);                    \
   template void launch_diag_apply<T>(hipStream_t, RedWork, int, const T*, const T*, T*, CGState*);      \
-  template void launch_cg_pq<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, T*, T*, CGState*, int); \
-  template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, const T*, T*, T*, const T*, \
-                                T*, CGState*);
+  template void launch_cg_p<T>(hipStream_t, int, const T*, T*, CGState*, int);                       \
+  template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*);          \
+  template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, T*, T*, const T*, T*,     \
+                                CGState*);
 ALIEN_HIP_INST_KRYLOV(double)
 ALIEN_HIP_INST_KRYLOV(float)
 

@@ -30,8 +30,7 @@ struct CGState {
   int done;     // set when the stop test passes, max_it is reached or on breakdown
   int status;   // Iteration::getStatus()
   int breakdown;// p.q == 0
-  int parity;   // which of the two direction buffers holds the current p
-  int pad[2];
+  int pad[3];
 };
 
 template <typename T>
@@ -42,12 +41,13 @@ void launch_cheb_step(hipStream_t, RedWork w, int n, RBMat<T> A, const T* xin, c
 template <typename T>
 void launch_diag_apply(hipStream_t, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st);
 template <typename T>
-void launch_cg_pq(hipStream_t, RedWork w, int n, RBMat<T> A, const T* z, T* pA, T* pB, T* q, CGState* st,
-                  int first);
+void launch_cg_p(hipStream_t, int n, const T* z, T* p, CGState* st, int first);
+template <typename T>
+void launch_cg_q(hipStream_t, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st);
 // invd_or_null != NULL fuses the Diag preconditioner (z = invD r, rho1 = r.z)
 template <typename T>
-void launch_cg_xr(hipStream_t, RedWork w, int n, const T* pA, const T* pB, const T* q, T* x, T* r,
-                  const T* invd_or_null, T* z_or_null, CGState* st);
+void launch_cg_xr(hipStream_t, RedWork w, int n, const T* p, const T* q, T* x, T* r, const T* invd_or_null,
+                  T* z_or_null, CGState* st);
 void launch_fold_increment(hipStream_t, int n, float* x32, double* x);
 void launch_demote(hipStream_t, int n, const double* a, float* b);
 

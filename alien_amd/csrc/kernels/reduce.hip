@@ -62,7 +62,6 @@ __global__ void __launch_bounds__(kFin) k_finalize(RedWork w, int nblk, int nslo
     // end of a CG update (CG.h:140-144) and Iteration::stop (Iteration.h:83-90)
     st->rho = st->rho1;
     st->it += 1;
-    st->parity ^= 1;
     if (st->it >= st->max_it) {
       st->done = 1; // stop() returns before computing the dot; status unchanged
     } else {

@@ -149,17 +149,16 @@ struct FusedCG {
   RBMat<T> A;
   const T* invd;
   PrecondSpec P;
-  // r, z: residual and preconditioned residual; pA/pB: the two direction
-  // buffers (the state's parity bit says which is current); cp: the Chebyshev
-  // sweep's own direction vector
-  DeviceArray<T> r, z, pA, pB, q, y0, y1, cp;
+  // r, z: residual and preconditioned residual; p, q: direction and A p;
+  // cp: the Chebyshev sweep's own direction vector
+  DeviceArray<T> r, z, p, q, y0, y1, cp;
   DeviceArray<CGState> st;
   CGState* h_st = nullptr; // pinned
 
   FusedCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
   : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
     ctx->reserve_for(n);
-    r.resize(n); z.resize(n); pA.resize(n); pB.resize(n); q.resize(n);
+    r.resize(n); z.resize(n); p.resize(n); q.resize(n);
     if (P.cheb) cp.resize(n);
     if (P.cheb && P.order > 1) { y0.resize(n); y1.resize(n); }
     st.resize(1);
@@ -193,8 +192,9 @@ struct FusedCG {
   void iteration(T* x, bool first) {
     hipStream_t s = ctx->stream;
     if (P.cheb || first) precond(r.data(), z.data());
-    launch_cg_pq<T>(s, ctx->red(), n, A, z.data(), pA.data(), pB.data(), q.data(), st.data(), first ? 1 : 0);
-    launch_cg_xr<T>(s, ctx->red(), n, pA.data(), pB.data(), q.data(), x, r.data(), P.cheb ? nullptr : invd,
+    launch_cg_p<T>(s, n, z.data(), p.data(), st.data(), first ? 1 : 0);
+    launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
+    launch_cg_xr<T>(s, ctx->red(), n, p.data(), q.data(), x, r.data(), P.cheb ? nullptr : invd,
                     P.cheb ? nullptr : z.data(), st.data());
   }
   // After r has been replaced from outside (reliable update): refresh z and rho1.
