@@ -132,7 +132,13 @@ class HIPMatrix {
   static const bool on_host_only = false;
   static constexpr int kBEllPackSlice = 1024; // BEllPackStructInfo<1024,int>
 
-  HIPMatrix(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR) : m_ctx(ctx) {
+  HIPMatrix() = default;
+  HIPMatrix(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR) { build(ctx, a, spmv_layout); }
+
+  // Upload a host CSR (ascending columns) and build the requested SpMV layout:
+  // the body of the SimpleCSR -> hip matrix converter.
+  void build(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR) {
+    m_ctx = ctx;
     m_n = a.nrows;
     m_nnz = a.row_offset[m_n];
     m_ctx->reserve_for(m_n);
@@ -235,7 +241,7 @@ class HIPMatrix {
   alien_hip::DeviceArray<uint8_t> d_lmask, d_umask;
 
  private:
-  alien_hip_ctx* m_ctx;
+  alien_hip_ctx* m_ctx = nullptr;
   int m_n = 0, m_nnz = 0;
   int m_layout = kLayoutCSR;
   std::vector<int> m_kcol, m_cols, m_dcol, m_bo, m_bro;
@@ -398,7 +404,7 @@ class HIPInternalLinearAlgebra {
   }
 
  private:
-  alien_hip_ctx* m_ctx;
+  alien_hip_ctx* m_ctx = nullptr;
 };
 
 } // namespace Alien

@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV kernels and a reference copy kernel (for rocprofv3 --pmc runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported in this build "
@@ -147,6 +149,20 @@ def main():
     b = db.cpu().numpy()
     del dro, dc, dv, dx, db
     torch.cuda.empty_cache()
+
+    if a.spmv_only:
+        sp = time_spmv(aa, torch, ctx, ro, cols, vals, a.spmv_reps)
+        # calibration kernel for the HBM counters: a copy of N doubles reads 8N and writes 8N bytes
+        src = torch.rand(N, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(a.spmv_reps):
+            aa.check(aa.lib().alien_hip_copy_f64(ctx.h, N, P(src), P(dst)))
+        ctx.synchronize()
+        print(json.dumps({"spmv_us": {k: v * 1e6 for k, v in sp.items()}, "copy_bytes_each_way": 8 * N,
+                          "spmv_fp64_algorithmic_bytes": spmv_bytes_fp64(N, nnz)}), flush=True)
+        ctx.close()
+        return
 
     prec = aa.PREC_DIAG if a.precond == "diag" else aa.PREC_CHEB
     common = dict(solver=aa.CG, precond=prec, tol=1e-8, max_iter=20000, poll_every=16)
