@@ -24,7 +24,7 @@ c_void_p = C.c_void_p
 # enums (include/alien_hip.h)
 GEN_POISSON7, GEN_CONVDIFF7, GEN_LOGNORMAL7 = 0, 1, 2
 CG, BICGS = 0, 1
-PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE = 0, 1, 2, 3
+PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_LEARNED = 0, 1, 2, 3, 4
 LAYOUT_CSR, LAYOUT_BELLPACK, LAYOUT_RBCSR = 0, 1, 2
 MODE_REFERENCE, MODE_FUSED, MODE_GRAPH, MODE_MIXED = 0, 1, 2, 3
 EINVAL = -22
@@ -35,16 +35,23 @@ class SolveParams(C.Structure):
                 ("tol", C.c_double), ("max_iter", C.c_int), ("poly_factor", C.c_double),
                 ("poly_order", C.c_int), ("poly_max_iter", C.c_int), ("ilu_schedule", C.c_int),
                 ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("max_outer", C.c_int),
-                ("poll_every", C.c_int)]
+                ("poll_every", C.c_int), ("gnn_theta", c_void_p), ("stall_window", C.c_int)]
 
 
 class SolveResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("error", C.c_int), ("outer", C.c_int),
                 ("value", C.c_double), ("true_rel_res", C.c_double), ("cheb_alpha", C.c_double),
-                ("cheb_beta", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double)]
+                ("cheb_beta", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double),
+                ("fallback", C.c_int), ("pad_", C.c_int)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
+class GnnTrainParams(C.Structure):
+    _fields_ = [("steps", C.c_int), ("steps_per_system", C.c_int), ("n_min", C.c_int), ("n_max", C.c_int),
+                ("sigma_min", C.c_double), ("sigma_max", C.c_double), ("aniso_max", C.c_double),
+                ("lr", C.c_double), ("beta", C.c_double), ("seed", C.c_uint64)]
 
 
 # (name, restype, argtypes)
@@ -104,6 +111,11 @@ _SIGS = [
                                            c_void_p]),
     ("alien_hip_solve_csr", C.c_int, [c_void_p, C.POINTER(SolveParams), C.c_int, c_int_p, c_int_p, c_double_p,
                                       c_double_p, c_double_p, C.POINTER(SolveResult)]),
+    ("alien_hip_gnn_nparams", C.c_int, []),
+    ("alien_hip_gnn_forward", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, c_float_p, c_float_p]),
+    ("alien_hip_gnn_loss_grad", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, c_float_p, c_float_p,
+                                          C.c_double, c_double_p, c_double_p, c_float_p]),
+    ("alien_hip_gnn_train", C.c_int, [c_void_p, C.POINTER(GnnTrainParams), c_float_p, c_double_p]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]
@@ -281,14 +293,70 @@ class Context:
 
 def solve_csr(ctx: Context, ro, cols, vals, b, x0=None, *, solver=CG, precond=PREC_CHEB, layout=LAYOUT_CSR,
               mode=MODE_REFERENCE, tol=1e-8, max_iter=1000, poly_factor=0.5, poly_order=3, poly_max_iter=10,
-              ilu_schedule=0, inner_tol=1e-4, inner_max_iter=0, max_outer=50, poll_every=8):
+              ilu_schedule=0, inner_tol=1e-4, inner_max_iter=0, max_outer=50, poll_every=8, gnn_theta=None,
+              stall_window=0):
     """Run one Krylov solve through the hip backend; returns (x, result dict)."""
     n = len(ro) - 1
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     b = np.ascontiguousarray(b, dtype=np.float64)
+    theta = None if gnn_theta is None else np.ascontiguousarray(gnn_theta, dtype=np.float32)
     p = SolveParams(solver, precond, layout, mode, tol, max_iter, poly_factor, poly_order, poly_max_iter,
-                    ilu_schedule, inner_tol, inner_max_iter, max_outer, poll_every)
+                    ilu_schedule, inner_tol, inner_max_iter, max_outer, poll_every,
+                    None if theta is None else theta.ctypes.data, stall_window)
     r = SolveResult()
     check(lib().alien_hip_solve_csr(ctx.h, C.byref(p), n, iptr(ro), iptr(cols), dptr(vals), dptr(b), dptr(x),
                                     C.byref(r)))
     return x, r.as_dict()
+
+
+# --------------------------------------------------------------------------
+# learned preconditioner M = G G^T
+# --------------------------------------------------------------------------
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_float_p)
+
+
+def gnn_nparams() -> int:
+    return lib().alien_hip_gnn_nparams()
+
+
+def _csr_args(ro, cols, vals):
+    ro = np.ascontiguousarray(ro, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    return ro, cols, vals
+
+
+def gnn_forward(ctx: Context, ro, cols, vals, theta) -> np.ndarray:
+    """Ghat per CSR entry of A (FP32; strictly-upper entries 0)."""
+    ro, cols, vals = _csr_args(ro, cols, vals)
+    theta = np.ascontiguousarray(theta, dtype=np.float32)
+    out = np.zeros(len(cols), dtype=np.float32)
+    check(lib().alien_hip_gnn_forward(ctx.h, len(ro) - 1, iptr(ro), iptr(cols), dptr(vals), fptr(theta),
+                                      fptr(out)))
+    return out
+
+
+def gnn_loss_grad(ctx: Context, ro, cols, vals, theta, probes, beta=0.1):
+    """(loss, (kaporin, probe), grad) of the training objective on 4 probes (N x 4)."""
+    ro, cols, vals = _csr_args(ro, cols, vals)
+    theta = np.ascontiguousarray(theta, dtype=np.float32)
+    probes = np.ascontiguousarray(probes, dtype=np.float32)
+    assert probes.shape == (len(ro) - 1, 4)
+    loss = C.c_double()
+    parts = np.zeros(2)
+    grad = np.zeros(gnn_nparams(), dtype=np.float32)
+    check(lib().alien_hip_gnn_loss_grad(ctx.h, len(ro) - 1, iptr(ro), iptr(cols), dptr(vals), fptr(theta),
+                                        fptr(probes), beta, C.byref(loss), dptr(parts), fptr(grad)))
+    return loss.value, (parts[0], parts[1]), grad
+
+
+def gnn_train(ctx: Context, theta, *, steps=200, steps_per_system=4, n_min=8, n_max=16, sigma_min=0.5,
+              sigma_max=2.0, aniso_max=4.0, lr=2e-3, beta=0.1, seed=1):
+    """Adam on generated log-normal diffusion systems; returns (theta, loss history)."""
+    th = np.array(theta, dtype=np.float32, copy=True)
+    hist = np.zeros(steps)
+    p = GnnTrainParams(steps, steps_per_system, n_min, n_max, sigma_min, sigma_max, aniso_max, lr, beta, seed)
+    check(lib().alien_hip_gnn_train(ctx.h, C.byref(p), fptr(th), dptr(hist)))
+    return th, hist

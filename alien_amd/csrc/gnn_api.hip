@@ -1,0 +1,133 @@
+// C ABI of the learned preconditioner: forward, loss + gradient, and the
+// training loop over systems drawn from the heterogeneous-diffusion generator.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "alien/hip/HIPContext.h"
+#include "alien_hip_internal.h"
+#include "kernels/gnn.h"
+#include "kernels/launch.h"
+
+using namespace alien_hip;
+
+namespace {
+
+// structural checks before anything reaches the GPU
+void check_csr(int n, const int* ro, const int* cols, const double* vals) {
+  if (n < 1 || !ro || !cols || !vals) throw Error(ALIEN_HIP_EINVAL, "gnn: empty or NULL matrix");
+  if (ro[0] != 0) throw Error(ALIEN_HIP_EINVAL, "gnn: row_offset[0] != 0");
+  for (int i = 0; i < n; ++i) {
+    if (ro[i + 1] < ro[i]) throw Error(ALIEN_HIP_EINVAL, "gnn: row_offset not monotone");
+    for (int k = ro[i]; k < ro[i + 1]; ++k) {
+      if (cols[k] < 0 || cols[k] >= n) throw Error(ALIEN_HIP_EINVAL, "gnn: column out of range");
+      if (k > ro[i] && cols[k] <= cols[k - 1]) throw Error(ALIEN_HIP_EINVAL, "gnn: columns not ascending");
+    }
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int alien_hip_gnn_nparams(void) { return gnn::kNParam; }
+
+int alien_hip_gnn_forward(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                          const float* theta, float* ghat) {
+  if (!ctx || !theta || !ghat) return fail(ALIEN_HIP_EINVAL, "gnn_forward: NULL argument");
+  try {
+    check_csr(n, ro, cols, vals);
+    gnn::Engine E(ctx, n, ro, cols, vals);
+    DeviceArray<float> P;
+    P.upload(theta, gnn::kNParam, ctx->stream);
+    E.forward(P.data());
+    ALIEN_HIP_CHECK(hipMemcpyAsync(ghat, E.ghat(), sizeof(float) * E.nnz(), hipMemcpyDeviceToHost, ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what());
+  }
+  return 0;
+}
+
+int alien_hip_gnn_loss_grad(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                            const float* theta, const float* probes, double beta, double* loss, double* parts,
+                            float* grad) {
+  if (!ctx || !theta || !probes || !loss || !grad) return fail(ALIEN_HIP_EINVAL, "gnn_loss_grad: NULL argument");
+  try {
+    check_csr(n, ro, cols, vals);
+    gnn::Engine E(ctx, n, ro, cols, vals);
+    DeviceArray<float> P, V, G(gnn::kNParam);
+    P.upload(theta, gnn::kNParam, ctx->stream);
+    V.upload(probes, (size_t)n * 4, ctx->stream);
+    double pp[2];
+    *loss = E.loss_grad(P.data(), V.data(), 4, (float)beta, G.data(), pp);
+    if (parts) {
+      parts[0] = pp[0];
+      parts[1] = pp[1];
+    }
+    ALIEN_HIP_CHECK(hipMemcpyAsync(grad, G.data(), sizeof(float) * gnn::kNParam, hipMemcpyDeviceToHost, ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what());
+  }
+  return 0;
+}
+
+int alien_hip_gnn_train(alien_hip_ctx* ctx, const alien_hip_gnn_train_params* p, float* theta, double* hist) {
+  if (!ctx || !p || !theta) return fail(ALIEN_HIP_EINVAL, "gnn_train: NULL argument");
+  if (p->steps < 0 || p->steps_per_system < 1 || p->n_min < 2 || p->n_max < p->n_min || p->n_max > 128 ||
+      !(p->sigma_min >= 0) || !(p->sigma_max >= p->sigma_min) || !(p->aniso_max >= 1) || !(p->lr > 0))
+    return fail(ALIEN_HIP_EINVAL, "gnn_train: bad parameters");
+  try {
+    hipStream_t s = ctx->stream;
+    DeviceArray<float> P, G(gnn::kNParam), M(gnn::kNParam), Vv(gnn::kNParam), V;
+    P.upload(theta, gnn::kNParam, s);
+    ALIEN_HIP_CHECK(hipMemsetAsync(M.data(), 0, sizeof(float) * gnn::kNParam, s));
+    ALIEN_HIP_CHECK(hipMemsetAsync(Vv.data(), 0, sizeof(float) * gnn::kNParam, s));
+    uint64_t rs = p->seed;
+    std::vector<int> ro, cols;
+    std::vector<double> vals;
+    std::vector<float> probes;
+    int step = 0;
+    while (step < p->steps) {
+      // draw a system: n^3 grid, sigma ~ U[sigma_min, sigma_max], anisotropy log-uniform per axis
+      const int n = p->n_min + (int)(splitmix64_next(rs) % (uint64_t)(p->n_max - p->n_min + 1));
+      double par[4];
+      par[0] = p->sigma_min + (p->sigma_max - p->sigma_min) * splitmix64_u01(rs);
+      for (int a = 0; a < 3; ++a) par[1 + a] = std::exp(std::log(p->aniso_max) * splitmix64_u01(rs));
+      const uint64_t sys_seed = splitmix64_next(rs);
+      const int N = n * n * n;
+      const int64_t nnz = gen_nnz(ALIEN_HIP_GEN_LOGNORMAL7, n, n, n);
+      ro.resize(N + 1);
+      cols.resize(nnz);
+      vals.resize(nnz);
+      if (gen_csr(ALIEN_HIP_GEN_LOGNORMAL7, n, n, n, par, sys_seed, ro.data(), cols.data(), vals.data()) != 0)
+        throw Error(ALIEN_HIP_EINVAL, "gnn_train: generator failed");
+      gnn::Engine E(ctx, N, ro.data(), cols.data(), vals.data());
+      probes.resize((size_t)N * 4);
+      for (int t = 0; t < p->steps_per_system && step < p->steps; ++t, ++step) {
+        for (auto& v : probes) v = (splitmix64_next(rs) >> 63) ? 1.f : -1.f; // Rademacher
+        V.upload(probes.data(), probes.size(), s);
+        const double L = E.loss_grad(P.data(), V.data(), 4, (float)p->beta, G.data(), nullptr);
+        if (hist) hist[step] = L;
+        gnn::launch_adam(s, gnn::kNParam, P.data(), G.data(), M.data(), Vv.data(), (float)p->lr, step + 1);
+        check_launch("gnn adam");
+      }
+    }
+    ALIEN_HIP_CHECK(hipMemcpyAsync(theta, P.data(), sizeof(float) * gnn::kNParam, hipMemcpyDeviceToHost, s));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what());
+  }
+  return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,73 @@
+// Parameter layout and host engine of the learned preconditioner (gnn.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace alien_hip {
+namespace gnn {
+
+constexpr int kHidden = 32;
+constexpr int kFeat = 4;
+// flat FP32 parameter vector: W_in[4x32] b_in[32] W1[64x32] b1[32] W2[64x32] b2[32]
+// W3[64x32] b3[32] w_d[32] c_d w_a[32] w_b[32] c_e   (row-major matrices)
+constexpr int kOffWin = 0;
+constexpr int kOffBin = kOffWin + kFeat * kHidden;
+constexpr int kOffW1 = kOffBin + kHidden;
+constexpr int kLayerSize = 2 * kHidden * kHidden + kHidden; // W_l and b_l
+constexpr int kOffW2 = kOffW1 + kLayerSize;
+constexpr int kOffW3 = kOffW2 + kLayerSize;
+constexpr int kOffWd = kOffW3 + kLayerSize;
+constexpr int kOffCd = kOffWd + kHidden;
+constexpr int kOffWa = kOffCd + 1;
+constexpr int kOffWb = kOffWa + kHidden;
+constexpr int kOffCe = kOffWb + kHidden;
+constexpr int kNParam = kOffCe + 1; // 6498
+constexpr int kHeadGrads = kNParam - kOffWd; // w_d, c_d, w_a, w_b, c_e = 98
+constexpr int kMaxProbes = 8;
+
+} // namespace gnn
+} // namespace alien_hip
+
+#include "../alien/hip/HIPContext.h"
+
+namespace alien_hip {
+namespace gnn {
+
+// Device state of the network applied to one matrix (SPD, symmetric pattern,
+// stored positive diagonal): scaled couplings, features, activations, Ghat.
+class Engine {
+ public:
+  Engine(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals);
+  int n() const { return m_n; }
+  int nnz() const { return m_nnz; }
+  // activations and Ghat (per CSR entry; upper entries 0) for the parameters P (device)
+  void forward(const float* P);
+  // loss = Kaporin term + beta * probe term (4 probes V, device, N x 4); grad (device, kNParam)
+  double loss_grad(const float* P, const float* V, int K, float beta, float* grad, double* parts);
+  const float* ghat() const { return d_ghat.data(); }
+  // RB-CSR layouts of G and G^T and their values (after forward)
+  void build_factor_layout();
+  template <typename T> void fill_factor(T* g_vals, T* gt_vals);
+  const DeviceArray<int>& g_bo() const { return d_g_bo; }
+  const DeviceArray<int>& g_bc() const { return d_g_bc; }
+  const DeviceArray<int>& gt_bo() const { return d_gt_bo; }
+  const DeviceArray<int>& gt_bc() const { return d_gt_bc; }
+
+ private:
+  alien_hip_ctx* m_ctx;
+  int m_n, m_nnz = 0;
+  std::vector<int> h_ro, h_cols, h_dcol, h_tpos, m_gbo, m_gtbo;
+  DeviceArray<int> d_ro, d_cols, d_dcol, d_tpos;
+  DeviceArray<double> d_vals;
+  DeviceArray<float> d_dis, d_e, d_f, d_ghat;
+  DeviceArray<float> d_h[4], d_z[4], d_m[4];
+  DeviceArray<int> d_g_bo, d_g_bc, d_g_sm, d_g_sr, d_gt_bo, d_gt_bc, d_gt_sm, d_gt_sr;
+};
+
+void launch_adam(hipStream_t s, int np, float* P, const float* g, float* m, float* v, float lr, int step);
+
+} // namespace gnn
+} // namespace alien_hip

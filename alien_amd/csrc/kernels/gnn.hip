@@ -1,0 +1,675 @@
+// Learned preconditioner M = G G^T: graph network forward, backward, losses and
+// optimiser on gfx950 (no counterpart in the reference; see the oracle's
+// statement of the model in oracle/gnn_oracle.py for the mathematics).
+//
+// Model, for an SPD matrix A with a symmetric pattern (d_i = a_ii):
+//   e_ij = -a_ij / sqrt(d_i d_j),  f_i = [1, s_i, 1 - s_i, max_j e_ij],  s_i = sum_j e_ij
+//   h0 = relu(f W_in + b_in)                                      (N x 32)
+//   h_l = h_{l-1} + relu([h_{l-1}, E h_{l-1}] W_l + b_l)   l = 1..3
+//   Ghat_ii = exp(h3_i . w_d + c_d),  Ghat_ij = e_ij (h3_i . w_a + h3_j . w_b + c_e)  (j < i)
+//   G = D^-1/2 Ghat,  M = G G^T
+// Dense layers run on the matrix cores: v_mfma_f32_32x32x8_bf16 with bf16
+// weights and bf16-rounded inputs, FP32 accumulation. Every reduction over
+// rows (losses, weight gradients) uses per-workgroup partials summed in a fixed
+// order: the results are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "../alien_hip_internal.h"
+#include "device_common.h"
+#include "gnn.h"
+#include "launch.h"
+
+namespace alien_hip {
+namespace gnn {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// float -> bfloat16 bits, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ short to_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_round(float x) {
+  return __uint_as_float(((unsigned)(unsigned short)to_bf16(x)) << 16);
+}
+
+// ---- graph preparation ----------------------------------------------------------------
+__global__ void k_prep_dis(int n, const int* __restrict__ dcol, const double* __restrict__ vals,
+                           float* __restrict__ dis) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dis[i] = (float)(1.0 / sqrt(vals[dcol[i]]));
+}
+// e per CSR entry (0 on the diagonal) and the node features
+__global__ void k_prep_feat(int n, const int* __restrict__ ro, const int* __restrict__ cols,
+                            const double* __restrict__ vals, const float* __restrict__ dis, float* __restrict__ e,
+                            float* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f, mx = 0.f;
+  bool any = false;
+  for (int k = ro[i]; k < ro[i + 1]; ++k) {
+    const int j = cols[k];
+    float v = 0.f;
+    if (j != i) {
+      v = (float)(-vals[k] * (double)dis[i] * (double)dis[j]);
+      s += v;
+      mx = any ? fmaxf(mx, v) : v;
+      any = true;
+    }
+    e[k] = v;
+  }
+  f[i * 4 + 0] = 1.f;
+  f[i * 4 + 1] = s;
+  f[i * 4 + 2] = 1.f - s;
+  f[i * 4 + 3] = mx;
+}
+
+// ---- dense layer on MFMA --------------------------------------------------------------------
+// Z = [X0 | X1] W + b ; H = Hres + relu(Z)   (Hres may be NULL: H = relu(Z))
+// X0 holds input columns [0, c0), X1 columns [c0, kin); kin <= KIN (multiple of 8).
+// One wavefront computes 32 rows x 32 outputs: KIN/8 mfma 32x32x8 steps.
+// Lane l holds A[row = l%32][k = 4(l/32)..+3], B[k][col = l%32], and the result
+// D[8 (r/4) + 4 (l/32) + r%4][l%32] in accumulator register r.
+template <int KIN>
+__global__ void __launch_bounds__(256) k_dense(int n, const float* __restrict__ X0, int ld0, int c0,
+                                               const float* __restrict__ X1, int ld1, int kin,
+                                               const float* __restrict__ W, const float* __restrict__ b,
+                                               const float* __restrict__ Hres, float* __restrict__ Z,
+                                               float* __restrict__ Hout) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int r0 = wave * 32;
+  if (r0 >= n) return;
+  const int arow = r0 + (lane & 31);
+  const int kb = lane >> 5;
+  const int col = lane & 31;
+  f32x16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < KIN / 8; ++ks) {
+    bf16x4 a, bb;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int k = ks * 8 + kb * 4 + v;
+      float xa = 0.f, wb = 0.f;
+      if (k < kin) {
+        if (arow < n) xa = (k < c0) ? X0[(long)arow * ld0 + k] : X1[(long)arow * ld1 + (k - c0)];
+        wb = W[k * kHidden + col];
+      }
+      a[v] = to_bf16(xa);
+      bb[v] = to_bf16(wb);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bb, acc, 0, 0, 0);
+  }
+  const float bias = b[col];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = r0 + 8 * (r >> 2) + 4 * kb + (r & 3);
+    if (row < n) {
+      const float z = acc[r] + bias;
+      const long o = (long)row * kHidden + col;
+      Z[o] = z;
+      Hout[o] = (Hres ? Hres[o] : 0.f) + fmaxf(z, 0.f);
+    }
+  }
+}
+
+// m_i = sum_{j != i} e_ij h_j   (32 lanes per row, one feature per lane)
+__global__ void __launch_bounds__(256) k_aggregate(int n, const int* __restrict__ ro, const int* __restrict__ cols,
+                                                   const float* __restrict__ e, const float* __restrict__ h,
+                                                   float* __restrict__ m) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const int i = (int)(t >> 5);
+  const int c = (int)(t & 31);
+  if (i >= n) return;
+  float acc = 0.f;
+  for (int k = ro[i]; k < ro[i + 1]; ++k) {
+    const float w = e[k];
+    if (w != 0.f) acc += w * h[(long)cols[k] * kHidden + c];
+  }
+  m[(long)i * kHidden + c] = acc;
+}
+
+// ---- output head ------------------------------------------------------------------------------
+__device__ __forceinline__ float dot32(const float* __restrict__ a, const float* __restrict__ w) {
+  float s = 0.f;
+#pragma unroll
+  for (int q = 0; q < kHidden; ++q) s += a[q] * w[q];
+  return s;
+}
+
+// ghat per CSR entry: diagonal exp(h_i.w_d + c_d), strictly lower e_ij (h_i.w_a + h_j.w_b + c_e),
+// upper entries 0 (G is lower triangular)
+__global__ void k_head(int n, const int* __restrict__ ro, const int* __restrict__ cols, const int* __restrict__ dcol,
+                       const float* __restrict__ e, const float* __restrict__ h, const float* __restrict__ P,
+                       float* __restrict__ ghat) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* hi = h + (long)i * kHidden;
+  const float ai = dot32(hi, P + kOffWa);
+  ghat[dcol[i]] = expf(dot32(hi, P + kOffWd) + P[kOffCd]);
+  for (int k = ro[i]; k < dcol[i]; ++k) {
+    const float* hj = h + (long)cols[k] * kHidden;
+    ghat[k] = e[k] * (ai + dot32(hj, P + kOffWb) + P[kOffCe]);
+  }
+  for (int k = dcol[i] + 1; k < ro[i + 1]; ++k) ghat[k] = 0.f;
+}
+
+// ---- losses ------------------------------------------------------------------------------------
+// Kaporin term. For each lower entry (i,j) of Ghat: AG_ij = sum_k Ahat_ik Ghat_kj,
+// Ahat_ik = 1 (k = i) or -e_ik; Ghat_kj read at the position of column j in row k.
+__device__ __forceinline__ int find_col(const int* __restrict__ cols, int b, int e, int j) {
+  for (int k = b; k < e; ++k)
+    if (cols[k] == j) return k;
+  return -1;
+}
+__global__ void __launch_bounds__(256) k_kcond(int n, const int* __restrict__ ro, const int* __restrict__ cols,
+                                               const int* __restrict__ dcol, const float* __restrict__ e,
+                                               const float* __restrict__ ghat, float* __restrict__ ag, RedWork w) {
+  __shared__ double lds[4];
+  double tr = 0.0, ld = 0.0;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    for (int p = ro[i]; p <= dcol[i]; ++p) { // lower entries and the diagonal of row i
+      const int j = cols[p];
+      float s = 0.f;
+      for (int k2 = ro[i]; k2 < ro[i + 1]; ++k2) {
+        const int c = cols[k2];
+        if (c < j) continue;                       // Ghat_cj = 0 above the diagonal
+        const int q = find_col(cols, ro[c], dcol[c] + 1, j); // (c, j) in c's lower part
+        if (q < 0) continue;
+        const float a = (c == i) ? 1.f : -e[k2];
+        s += a * ghat[q];
+      }
+      ag[p] = s;
+      tr += (double)ghat[p] * (double)s;
+    }
+    ld = log((double)ghat[dcol[i]]);
+  }
+  double v[2];
+  v[0] = block_sum(tr, lds);
+  __syncthreads();
+  v[1] = block_sum(ld, lds);
+  store_partials<2>(v, w);
+}
+
+// probe term, K probes stored row-major (N x K)
+template <int K>
+__global__ void k_ahat_mult(int n, const int* __restrict__ ro, const int* __restrict__ cols, const float* __restrict__ e,
+                            const float* __restrict__ V, float* __restrict__ W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = 0.f;
+  for (int k = ro[i]; k < ro[i + 1]; ++k) {
+    const int j = cols[k];
+    const float a = (j == i) ? 1.f : -e[k];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] += a * V[(long)j * K + c];
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c) W[(long)i * K + c] = acc[c];
+}
+// Y = Ghat X (lower rows) or Y = Ghat^T X (upper rows through the transpose positions)
+template <int K, bool TRANS>
+__global__ void k_ghat_mult(int n, const int* __restrict__ ro, const int* __restrict__ cols, const int* __restrict__ dcol,
+                            const int* __restrict__ tpos, const float* __restrict__ ghat, const float* __restrict__ X,
+                            float* __restrict__ Y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = 0.f;
+  const int b = TRANS ? dcol[i] : ro[i];
+  const int en = TRANS ? ro[i + 1] : dcol[i] + 1;
+  for (int k = b; k < en; ++k) {
+    const int j = cols[k];
+    const float g = TRANS ? ghat[tpos[k]] : ghat[k];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] += g * X[(long)j * K + c];
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c) Y[(long)i * K + c] = acc[c];
+}
+// R = V - Y, dY = -2 R / (N K), loss partial sum R^2
+template <int K>
+__global__ void __launch_bounds__(256) k_probe_resid(int n, const float* __restrict__ V, const float* __restrict__ Y,
+                                                     float* __restrict__ dY, float scale, RedWork w) {
+  __shared__ double lds[4];
+  double s = 0.0;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const float r = V[(long)i * K + c] - Y[(long)i * K + c];
+      dY[(long)i * K + c] = -2.f * r * scale;
+      s += (double)r * (double)r;
+    }
+  }
+  double v[1] = {block_sum(s, lds)};
+  store_partials<1>(v, w);
+}
+
+// ---- dL/dGhat -> dL/d(head inputs) --------------------------------------------------------------
+// g_ij (per lower CSR entry) = 2 AG_ij / tr - (2/N) [i==j] / Ghat_ii  + beta (sum_c dY_ic U_jc + W_ic Q_jc)
+template <int K>
+__global__ void k_grad_ghat(int n, const int* __restrict__ ro, const int* __restrict__ cols, const int* __restrict__ dcol,
+                            const float* __restrict__ ghat, const float* __restrict__ ag, const double* __restrict__ sums,
+                            float beta, const float* __restrict__ dY, const float* __restrict__ U,
+                            const float* __restrict__ Wp, const float* __restrict__ Q, float* __restrict__ gg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float tr = (float)sums[0];
+  for (int k = ro[i]; k <= dcol[i]; ++k) {
+    const int j = cols[k];
+    float g = 2.f * ag[k] / tr;
+    if (k == dcol[i]) g -= 2.f / ((float)n * ghat[k]);
+    if (beta != 0.f) {
+      float p = 0.f;
+#pragma unroll
+      for (int c = 0; c < K; ++c) p += dY[(long)i * K + c] * U[(long)j * K + c] + Wp[(long)i * K + c] * Q[(long)j * K + c];
+      g += beta * p;
+    }
+    gg[k] = g;
+  }
+}
+
+// head backward: do_i = g_ii ghat_ii, dc_ij = g_ij e_ij;
+// dh_i = do_i w_d + sum_{j<i} dc_ij w_a + sum_{m>i} dc_mi w_b ; per-row parameter-gradient terms
+// for (w_d, c_d, w_a, w_b, c_e) go to `prow` (N x kHeadGrads) and are summed by k_colsum.
+__global__ void k_head_bwd(int n, const int* __restrict__ ro, const int* __restrict__ cols, const int* __restrict__ dcol,
+                           const int* __restrict__ tpos, const float* __restrict__ e, const float* __restrict__ ghat,
+                           const float* __restrict__ gg, const float* __restrict__ h, const float* __restrict__ P,
+                           float* __restrict__ dh, float* __restrict__ prow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* hi = h + (long)i * kHidden;
+  const float dO = gg[dcol[i]] * ghat[dcol[i]];
+  float sa = 0.f, sb = 0.f;      // sum of dc over row i's lower edges / over edges (m, i), m > i
+  float wb_acc[kHidden];
+#pragma unroll
+  for (int q = 0; q < kHidden; ++q) wb_acc[q] = 0.f;
+  for (int k = ro[i]; k < dcol[i]; ++k) sa += gg[k] * e[k];
+  for (int k = dcol[i] + 1; k < ro[i + 1]; ++k) {
+    const int t = tpos[k]; // position of (m, i) in row m, m = cols[k] > i
+    const float dc = gg[t] * e[t];
+    sb += dc;
+    const float* hm = h + (long)cols[k] * kHidden;
+#pragma unroll
+    for (int q = 0; q < kHidden; ++q) wb_acc[q] += dc * hi[q];
+    (void)hm;
+  }
+  float* pr = prow + (long)i * kHeadGrads;
+#pragma unroll
+  for (int q = 0; q < kHidden; ++q) {
+    dh[(long)i * kHidden + q] = dO * P[kOffWd + q] + sa * P[kOffWa + q] + sb * P[kOffWb + q];
+    pr[q] = dO * hi[q];               // w_d
+    pr[kHidden + 1 + q] = sa * hi[q]; // w_a: sum over lower edges (i,j) of dc_ij h_i
+    pr[2 * kHidden + 1 + q] = wb_acc[q]; // w_b: sum over edges (m,i) of dc_mi h_i
+  }
+  pr[kHidden] = dO;                    // c_d
+  pr[3 * kHidden + 1] = sa;            // c_e (each lower edge counted once, in its row)
+}
+
+// dense layer backward: dz = dh * (z > 0) ; du = dz W^T (W bf16-rounded) ;
+// dh_prev = dh + du[:, :32] (the E du[:, 32:] part is added by k_aggregate_add) ; dmsg = du[:, 32:]
+template <int KIN>
+__global__ void k_dense_bwd(int n, const float* __restrict__ dh, const float* __restrict__ Z, const float* __restrict__ W,
+                            int kin, float* __restrict__ dz, float* __restrict__ dh_prev, float* __restrict__ dmsg) {
+  __shared__ float Ws[KIN * kHidden];
+  for (int t = threadIdx.x; t < KIN * kHidden; t += blockDim.x)
+    Ws[t] = (t / kHidden < kin) ? bf16_round(W[t]) : 0.f;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float d[kHidden];
+#pragma unroll
+  for (int q = 0; q < kHidden; ++q) {
+    const long o = (long)i * kHidden + q;
+    d[q] = Z[o] > 0.f ? dh[o] : 0.f;
+    dz[o] = d[q];
+  }
+  if (!dh_prev) return;
+  for (int k = 0; k < kin; ++k) {
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < kHidden; ++q) s += d[q] * Ws[k * kHidden + q];
+    if (k < kHidden) dh_prev[(long)i * kHidden + k] = dh[(long)i * kHidden + k] + s;
+    else dmsg[(long)i * kHidden + (k - kHidden)] = s;
+  }
+}
+
+// dh[i] += sum_{j != i} e_ij dmsg[j]   (E is symmetric)
+__global__ void __launch_bounds__(256) k_aggregate_add(int n, const int* __restrict__ ro, const int* __restrict__ cols,
+                                                       const float* __restrict__ e, const float* __restrict__ dmsg,
+                                                       float* __restrict__ dh) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const int i = (int)(t >> 5);
+  const int c = (int)(t & 31);
+  if (i >= n) return;
+  float acc = 0.f;
+  for (int k = ro[i]; k < ro[i + 1]; ++k) {
+    const float w = e[k];
+    if (w != 0.f) acc += w * dmsg[(long)cols[k] * kHidden + c];
+  }
+  dh[(long)i * kHidden + c] += acc;
+}
+
+// weight-gradient partials: per workgroup of 256 rows, sum_r bf16(X_r) (x) dz_r  (kin x 32) and sum_r dz_r (32)
+template <int KIN>
+__global__ void __launch_bounds__(256) k_wgrad_partials(int n, const float* __restrict__ X0, int ld0, int c0,
+                                                        const float* __restrict__ X1, int ld1, int kin,
+                                                        const float* __restrict__ dz, double* __restrict__ part,
+                                                        int cap) {
+  constexpr int NOUT = KIN * kHidden + kHidden;
+  constexpr int CH = 32; // rows staged per chunk
+  __shared__ float xs[CH][KIN];
+  __shared__ float ds[CH][kHidden];
+  const int r0 = blockIdx.x * 256;
+  double acc[(NOUT + 255) / 256];
+#pragma unroll
+  for (int q = 0; q < (NOUT + 255) / 256; ++q) acc[q] = 0.0;
+  for (int c = 0; c < 256; c += CH) {
+    for (int t = threadIdx.x; t < CH * KIN; t += 256) {
+      const int rr = t / KIN, k = t % KIN, row = r0 + c + rr;
+      float v = 0.f;
+      if (row < n && k < kin) v = bf16_round((k < c0) ? X0[(long)row * ld0 + k] : X1[(long)row * ld1 + (k - c0)]);
+      xs[rr][k] = v;
+    }
+    for (int t = threadIdx.x; t < CH * kHidden; t += 256) {
+      const int rr = t / kHidden, q = t % kHidden, row = r0 + c + rr;
+      ds[rr][q] = (row < n) ? dz[(long)row * kHidden + q] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < (NOUT + 255) / 256; ++q) {
+      const int o = threadIdx.x + q * 256;
+      if (o < NOUT) {
+        float s = 0.f;
+        if (o < KIN * kHidden) {
+          const int k = o / kHidden, j = o % kHidden;
+          for (int rr = 0; rr < CH; ++rr) s += xs[rr][k] * ds[rr][j];
+        } else {
+          const int j = o - KIN * kHidden;
+          for (int rr = 0; rr < CH; ++rr) s += ds[rr][j];
+        }
+        acc[q] += (double)s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < (NOUT + 255) / 256; ++q) {
+    const int o = threadIdx.x + q * 256;
+    if (o < NOUT) part[(long)o * cap + blockIdx.x] = acc[q];
+  }
+}
+
+// out[o] (+)= sum_b part[o * cap + b] in block order (one lane per output)
+__global__ void k_colsum(int nout, int nblk, const double* __restrict__ part, int cap, float* __restrict__ out,
+                         int accumulate) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(long)o * cap + b];
+  out[o] = accumulate ? out[o] + (float)s : (float)s;
+}
+// per-row head gradient terms (N x kHeadGrads) -> partials per 256 rows
+__global__ void __launch_bounds__(256) k_rowsum_partials(int n, int ncol, const float* __restrict__ prow,
+                                                         double* __restrict__ part, int cap) {
+  const int r0 = blockIdx.x * 256;
+  for (int o = threadIdx.x; o < ncol; o += 256) {
+    double s = 0.0;
+    for (int r = r0; r < min(n, r0 + 256); ++r) s += (double)prow[(long)r * ncol + o];
+    part[(long)o * cap + blockIdx.x] = s;
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights
+__global__ void k_adam(int np, float* __restrict__ P, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// RB-CSR values of G (rows: lower + diagonal of A) or G^T (rows: diagonal + upper),
+// slot -> CSR position map (-1 = padding); G_ij = dis_i Ghat_ij
+template <typename T>
+__global__ void k_fill_g(long nslots, const int* __restrict__ smap, const int* __restrict__ srow,
+                         const float* __restrict__ ghat, const float* __restrict__ dis, T* __restrict__ bv) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int p = smap[s];
+  bv[s] = p >= 0 ? (T)((double)dis[srow[s]] * (double)ghat[p]) : T(0);
+}
+
+} // namespace gnn
+} // namespace alien_hip
+
+// =====================================================================================
+// Host engine: device buffers of one system and the launch sequences
+// =====================================================================================
+#include "../alien/hip/HIPContext.h"
+
+namespace alien_hip {
+namespace gnn {
+
+namespace {
+inline dim3 g1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+} // namespace
+
+Engine::Engine(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals) : m_ctx(ctx), m_n(n) {
+  const int nnz = ro[n];
+  m_nnz = nnz;
+  // host structure: diagonal positions, transpose positions (symmetric pattern required)
+  h_ro.assign(ro, ro + n + 1);
+  h_cols.assign(cols, cols + nnz);
+  h_dcol.resize(n);
+  host_dcol(n, ro, cols, h_dcol.data());
+  h_tpos.assign(nnz, -1);
+  for (int i = 0; i < n; ++i) {
+    if (h_dcol[i] >= ro[i + 1] || cols[h_dcol[i]] != i) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: missing diagonal");
+    if (!(vals[h_dcol[i]] > 0)) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: non-positive diagonal");
+    for (int k = ro[i]; k < ro[i + 1]; ++k) {
+      const int j = cols[k];
+      int q = -1;
+      for (int t = ro[j]; t < ro[j + 1]; ++t)
+        if (cols[t] == i) { q = t; break; }
+      if (q < 0) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: the pattern must be symmetric");
+      h_tpos[k] = q;
+    }
+  }
+  hipStream_t s = m_ctx->stream;
+  d_ro.upload(h_ro.data(), h_ro.size(), s);
+  d_cols.upload(h_cols.data(), h_cols.size(), s);
+  d_dcol.upload(h_dcol.data(), h_dcol.size(), s);
+  d_tpos.upload(h_tpos.data(), h_tpos.size(), s);
+  d_vals.upload(vals, nnz, s);
+  d_dis.resize(n);
+  d_e.resize(nnz);
+  d_f.resize((size_t)n * kFeat);
+  for (auto& a : d_h) a.resize((size_t)n * kHidden);
+  for (auto& a : d_z) a.resize((size_t)n * kHidden);
+  for (auto& a : d_m) a.resize((size_t)n * kHidden);
+  d_ghat.resize(nnz);
+  ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void Engine::forward(const float* P) {
+  hipStream_t s = m_ctx->stream;
+  const int n = m_n;
+  // scaled couplings and node features from A's values
+  hipLaunchKernelGGL(k_prep_dis, g1(n), dim3(256), 0, s, n, d_dcol.data(), d_vals.data(), d_dis.data());
+  hipLaunchKernelGGL(k_prep_feat, g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_vals.data(), d_dis.data(),
+                     d_e.data(), d_f.data());
+  const dim3 gd((unsigned)(((n + 31) / 32 + 3) / 4)); // 32 rows per wavefront, 4 wavefronts per workgroup
+  hipLaunchKernelGGL(k_dense<8>, gd, dim3(256), 0, s, n, d_f.data(), kFeat, kFeat, (const float*)nullptr, 0, kFeat,
+                     P + kOffWin, P + kOffBin, (const float*)nullptr, d_z[0].data(), d_h[0].data());
+  const int offs[3] = {kOffW1, kOffW2, kOffW3};
+  for (int l = 1; l <= 3; ++l) {
+    hipLaunchKernelGGL(k_aggregate, g1((long)n * 32), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_e.data(),
+                       d_h[l - 1].data(), d_m[l].data());
+    hipLaunchKernelGGL(k_dense<64>, gd, dim3(256), 0, s, n, d_h[l - 1].data(), kHidden, kHidden, d_m[l].data(), kHidden,
+                       2 * kHidden, P + offs[l - 1], P + offs[l - 1] + 2 * kHidden * kHidden, d_h[l - 1].data(),
+                       d_z[l].data(), d_h[l].data());
+  }
+  hipLaunchKernelGGL(k_head, g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(), d_e.data(),
+                     d_h[3].data(), P, d_ghat.data());
+  check_launch("gnn forward");
+}
+
+double Engine::loss_grad(const float* P, const float* V, int K, float beta, float* grad, double* parts) {
+  if (K != 4) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: 4 probe vectors are supported");
+  hipStream_t s = m_ctx->stream;
+  const int n = m_n;
+  forward(P);
+  // ---- Kaporin term
+  m_ctx->reserve_for(n);
+  DeviceArray<float> ag(m_nnz), gg(m_nnz);
+  DeviceArray<double> sums(4);
+  const long nb = (n + 255) / 256;
+  RedWork w = m_ctx->red();
+  hipLaunchKernelGGL(k_kcond, dim3((unsigned)nb), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+                     d_e.data(), d_ghat.data(), ag.data(), w);
+  launch_finalize(s, w, nb, 2, kFinStore, sums.data(), nullptr);
+  // ---- probe term
+  constexpr int KP = 4;
+  DeviceArray<float> W_((size_t)n * KP), U((size_t)n * KP), Y((size_t)n * KP), dY((size_t)n * KP), Q((size_t)n * KP);
+  hipLaunchKernelGGL((k_ahat_mult<KP>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_e.data(), V, W_.data());
+  hipLaunchKernelGGL((k_ghat_mult<KP, true>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+                     d_tpos.data(), d_ghat.data(), W_.data(), U.data());
+  hipLaunchKernelGGL((k_ghat_mult<KP, false>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+                     d_tpos.data(), d_ghat.data(), U.data(), Y.data());
+  hipLaunchKernelGGL((k_probe_resid<KP>), dim3((unsigned)nb), dim3(256), 0, s, n, V, Y.data(), dY.data(),
+                     (float)(1.0 / ((double)n * KP)), w);
+  launch_finalize(s, w, nb, 1, kFinStore, sums.data() + 2, nullptr);
+  hipLaunchKernelGGL((k_ghat_mult<KP, true>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+                     d_tpos.data(), d_ghat.data(), dY.data(), Q.data());
+  hipLaunchKernelGGL((k_grad_ghat<KP>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+                     d_ghat.data(), ag.data(), sums.data(), beta, dY.data(), U.data(), W_.data(), Q.data(), gg.data());
+  // ---- head backward
+  DeviceArray<float> dh[2], dz((size_t)n * kHidden), dmsg((size_t)n * kHidden), prow((size_t)n * kHeadGrads);
+  dh[0].resize((size_t)n * kHidden);
+  dh[1].resize((size_t)n * kHidden);
+  hipLaunchKernelGGL(k_head_bwd, g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(), d_tpos.data(),
+                     d_e.data(), d_ghat.data(), gg.data(), d_h[3].data(), P, dh[0].data(), prow.data());
+  const int cap = (int)nb;
+  DeviceArray<double> part((size_t)(2 * kHidden * kHidden + kHidden) * cap);
+  hipLaunchKernelGGL(k_rowsum_partials, dim3((unsigned)nb), dim3(256), 0, s, n, kHeadGrads, prow.data(), part.data(), cap);
+  hipLaunchKernelGGL(k_colsum, g1(kHeadGrads), dim3(256), 0, s, kHeadGrads, (int)nb, part.data(), cap, grad + kOffWd, 0);
+  // ---- message-passing layers, last to first
+  const int offs[3] = {kOffW1, kOffW2, kOffW3};
+  int cur = 0;
+  for (int l = 3; l >= 1; --l) {
+    const float* Wl = P + offs[l - 1];
+    hipLaunchKernelGGL(k_dense_bwd<64>, g1(n), dim3(256), 0, s, n, dh[cur].data(), d_z[l].data(), Wl, 2 * kHidden,
+                       dz.data(), dh[cur ^ 1].data(), dmsg.data());
+    hipLaunchKernelGGL(k_aggregate_add, g1((long)n * 32), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_e.data(),
+                       dmsg.data(), dh[cur ^ 1].data());
+    hipLaunchKernelGGL(k_wgrad_partials<64>, dim3((unsigned)nb), dim3(256), 0, s, n, d_h[l - 1].data(), kHidden, kHidden,
+                       d_m[l].data(), kHidden, 2 * kHidden, dz.data(), part.data(), cap);
+    hipLaunchKernelGGL(k_colsum, g1(kLayerSize), dim3(256), 0, s, kLayerSize, (int)nb, part.data(), cap,
+                       grad + offs[l - 1], 0);
+    cur ^= 1;
+  }
+  // ---- input layer
+  hipLaunchKernelGGL(k_dense_bwd<4>, g1(n), dim3(256), 0, s, n, dh[cur].data(), d_z[0].data(), P + kOffWin, kFeat,
+                     dz.data(), (float*)nullptr, (float*)nullptr);
+  hipLaunchKernelGGL(k_wgrad_partials<4>, dim3((unsigned)nb), dim3(256), 0, s, n, d_f.data(), kFeat, kFeat,
+                     (const float*)nullptr, 0, kFeat, dz.data(), part.data(), cap);
+  hipLaunchKernelGGL(k_colsum, g1(kFeat * kHidden + kHidden), dim3(256), 0, s, kFeat * kHidden + kHidden, (int)nb,
+                     part.data(), cap, grad + kOffWin, 0);
+  check_launch("gnn backward");
+  double hs[4];
+  ALIEN_HIP_CHECK(hipMemcpyAsync(hs, sums.data(), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+  const double lk = std::log(hs[0] / n) - 2.0 * hs[1] / n;
+  const double lp = hs[2] / ((double)n * KP);
+  if (parts) {
+    parts[0] = lk;
+    parts[1] = lp;
+  }
+  return lk + beta * lp;
+}
+
+// RB-CSR of G (rows = lower + diagonal of A) and of G^T (rows = diagonal + upper of A)
+void Engine::build_factor_layout() {
+  if (!m_gbo.empty()) return;
+  const int n = m_n;
+  const int nb = (n + kBundle - 1) / kBundle;
+  for (int t = 0; t < 2; ++t) {
+    std::vector<int>& bo = t == 0 ? m_gbo : m_gtbo;
+    bo.assign(nb + 1, 0);
+    long acc = 0;
+    for (int b = 0; b < nb; ++b) {
+      bo[b] = (int)acc;
+      int w = 0;
+      for (int r = b * kBundle; r < std::min(n, (b + 1) * kBundle); ++r) {
+        const int len = t == 0 ? h_dcol[r] - h_ro[r] + 1 : h_ro[r + 1] - h_dcol[r];
+        w = std::max(w, len);
+      }
+      acc += w;
+    }
+    bo[nb] = (int)acc;
+    const size_t slots = (size_t)acc * kBundle;
+    std::vector<int> bc(slots, -1), sm(slots, -1), sr(slots, 0);
+    for (int r = 0; r < n; ++r) {
+      const int b = r / kBundle, lane = r % kBundle;
+      const int k0 = t == 0 ? h_ro[r] : h_dcol[r];
+      const int k1 = t == 0 ? h_dcol[r] + 1 : h_ro[r + 1];
+      for (int k = k0; k < k1; ++k) {
+        const size_t sl = ((size_t)bo[b] + (k - k0)) * kBundle + lane;
+        bc[sl] = h_cols[k];
+        sm[sl] = t == 0 ? k : h_tpos[k];
+        sr[sl] = t == 0 ? r : h_cols[k]; // G_ij = dis_i Ghat_ij: scale by the row of G
+      }
+    }
+    hipStream_t s = m_ctx->stream;
+    if (t == 0) {
+      d_g_bo.upload(bo.data(), bo.size(), s);
+      d_g_bc.upload(bc.data(), slots, s);
+      d_g_sm.upload(sm.data(), slots, s);
+      d_g_sr.upload(sr.data(), slots, s);
+    } else {
+      d_gt_bo.upload(bo.data(), bo.size(), s);
+      d_gt_bc.upload(bc.data(), slots, s);
+      d_gt_sm.upload(sm.data(), slots, s);
+      d_gt_sr.upload(sr.data(), slots, s);
+    }
+  }
+}
+
+template <typename T>
+void Engine::fill_factor(T* g_vals, T* gt_vals) {
+  build_factor_layout();
+  hipStream_t s = m_ctx->stream;
+  const long ng = (long)d_g_bc.size(), ngt = (long)d_gt_bc.size();
+  hipLaunchKernelGGL(k_fill_g<T>, g1(ng), dim3(256), 0, s, ng, d_g_sm.data(), d_g_sr.data(), d_ghat.data(),
+                     d_dis.data(), g_vals);
+  hipLaunchKernelGGL(k_fill_g<T>, g1(ngt), dim3(256), 0, s, ngt, d_gt_sm.data(), d_gt_sr.data(), d_ghat.data(),
+                     d_dis.data(), gt_vals);
+  check_launch("gnn fill factor");
+}
+template void Engine::fill_factor<float>(float*, float*);
+template void Engine::fill_factor<double>(double*, double*);
+
+void launch_adam(hipStream_t s, int np, float* P, const float* g, float* m, float* v, float lr, int step) {
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, g1(np), dim3(256), 0, s, np, P, g, m, v, lr, b1, b2, 1e-8f, c1, c2);
+}
+
+} // namespace gnn
+} // namespace alien_hip

@@ -204,6 +204,40 @@ __global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* 
   store_partials<1>(v, w);
 }
 
+// y = A x with rho1 = w . y into the CG state (second SpMV of a factored
+// preconditioner z = G (G^T r): x = G^T r, w = r)
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock) k_rb_spmv_dot(int n, RBMat<T> A, const T* __restrict__ x,
+                                                            const T* __restrict__ wv, T* __restrict__ y, CGState* st,
+                                                            RedWork w) {
+  if (st->done) return;
+  __shared__ double lds4[kRowBlock / kWave];
+  double s = 0.0;
+  const int b = my_bundle(n);
+  if (b >= 0) {
+    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    if (row < n) {
+      y[row] = t;
+      s = (double)wv[row] * (double)t;
+    }
+  }
+  double v[1] = {block_sum<kRowBlock>(s, lds4)};
+  store_partials<1>(v, w);
+}
+
+// y = A x (RB-CSR), returning early once the CG state is done
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock) k_rb_spmv_guarded(int n, RBMat<T> A, const T* __restrict__ x,
+                                                                T* __restrict__ y, const CGState* st) {
+  if (st->done) return;
+  const int b = my_bundle(n);
+  if (b < 0) return;
+  const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+  const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+  if (row < n) y[row] = t;
+}
+
 // alpha = rho1 / pq ; x += alpha p ; r += (-alpha) q ; rr = r.r ; stop test.
 // DIAG: also z = invD r and rho1' = r.z for the next iteration.
 template <typename T, bool DIAG>
@@ -323,6 +357,14 @@ void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* p, const T* q, T* x,
   launch_finalize(s, w, ept_blocks(n), 2, invd_or_null ? kFinXRDiag : kFinXR, nullptr, st);
 }
 
+template <typename T>
+void launch_factor_apply(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* tmp, T* z,
+                         CGState* st) {
+  hipLaunchKernelGGL(k_rb_spmv_guarded<T>, bgrid(n), dim3(kRowBlock), 0, s, n, GT, r, tmp, (const CGState*)st);
+  hipLaunchKernelGGL(k_rb_spmv_dot<T>, bgrid(n), dim3(kRowBlock), 0, s, n, G, tmp, r, z, st, w);
+  launch_finalize(s, w, bundle_blocks(n), 1, kFinRho1, nullptr, st);
+}
+
 void launch_fold_increment(hipStream_t s, int n, float* x32, double* x) {
   hipLaunchKernelGGL(k_fold_increment, egrid(n), dim3(kBlock), 0, s, n, x32, x);
 }
@@ -336,6 +378,8 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
                                     const T*, double, double, bool, bool, CGState*);                    \
   template void launch_diag_apply<T>(hipStream_t, RedWork, int, const T*, const T*, T*, CGState*);      \
   template void launch_cg_p<T>(hipStream_t, int, const T*, T*, CGState*, int);                       \
+  template void launch_factor_apply<T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,  \
+                                       CGState*);                                                        \
   template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*);          \
   template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, T*, T*, const T*, T*,     \
                                 CGState*);

@@ -48,6 +48,10 @@ void launch_cg_q(hipStream_t, RedWork w, int n, RBMat<T> A, const T* p, T* q, CG
 template <typename T>
 void launch_cg_xr(hipStream_t, RedWork w, int n, const T* p, const T* q, T* x, T* r, const T* invd_or_null,
                   T* z_or_null, CGState* st);
+// z = G (G^T r) and rho1 = r.z (factored approximate inverse M = G G^T)
+template <typename T>
+void launch_factor_apply(hipStream_t, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* tmp, T* z,
+                         CGState* st);
 void launch_fold_increment(hipStream_t, int n, float* x32, double* x);
 void launch_demote(hipStream_t, int n, const double* a, float* b);
 

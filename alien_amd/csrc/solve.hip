@@ -15,6 +15,7 @@
 #include "alien/hip/krylov/Krylov.h"
 #include "alien_hip_internal.h"
 #include "kernels/device_common.h"
+#include "kernels/gnn.h"
 #include "kernels/krylov_state.h"
 #include "kernels/launch.h"
 
@@ -136,10 +137,15 @@ void run_reference(Alg& alg, const alien_hip_solve_params* p, Precond& P, const 
 }
 
 // ---- fused CG ------------------------------------------------------------------------------
+enum PrecKind { kPrecDiag = 0, kPrecCheb = 1, kPrecFactor = 2 };
 struct PrecondSpec {
-  bool cheb = false;
+  PrecKind kind = kPrecDiag;
   int order = 0;
   std::vector<double> a, b; // Chebyshev step coefficients
+  // factored approximate inverse M = G G^T: RB-CSR structure of G and G^T and
+  // their values, in the precision of the solver using them
+  const int *g_bo = nullptr, *g_bc = nullptr, *gt_bo = nullptr, *gt_bc = nullptr;
+  const void *g_v = nullptr, *gt_v = nullptr;
 };
 
 template <typename T>
@@ -151,7 +157,7 @@ struct FusedCG {
   PrecondSpec P;
   // r, z: residual and preconditioned residual; p, q: direction and A p;
   // cp: the Chebyshev sweep's own direction vector
-  DeviceArray<T> r, z, p, q, y0, y1, cp;
+  DeviceArray<T> r, z, p, q, y0, y1, cp, tmp;
   DeviceArray<CGState> st;
   CGState* h_st = nullptr; // pinned
 
@@ -159,8 +165,9 @@ struct FusedCG {
   : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
     ctx->reserve_for(n);
     r.resize(n); z.resize(n); p.resize(n); q.resize(n);
-    if (P.cheb) cp.resize(n);
-    if (P.cheb && P.order > 1) { y0.resize(n); y1.resize(n); }
+    if (P.kind == kPrecCheb) cp.resize(n);
+    if (P.kind == kPrecCheb && P.order > 1) { y0.resize(n); y1.resize(n); }
+    if (P.kind == kPrecFactor) tmp.resize(n);
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
   }
@@ -172,8 +179,13 @@ struct FusedCG {
   // z = M r and rho1 = r.z into the state
   void precond(const T* rin, T* zout) {
     hipStream_t s = ctx->stream;
-    if (!P.cheb) {
+    if (P.kind == kPrecDiag) {
       launch_diag_apply<T>(s, ctx->red(), n, rin, invd, zout, st.data());
+      return;
+    }
+    if (P.kind == kPrecFactor) {
+      RBMat<T> G{P.g_bo, P.g_bc, (const T*)P.g_v}, GT{P.gt_bo, P.gt_bc, (const T*)P.gt_v};
+      launch_factor_apply<T>(s, ctx->red(), n, G, GT, rin, tmp.data(), zout, st.data());
       return;
     }
     const int m = P.order;
@@ -191,15 +203,16 @@ struct FusedCG {
   // iteration, except on the first one.
   void iteration(T* x, bool first) {
     hipStream_t s = ctx->stream;
-    if (P.cheb || first) precond(r.data(), z.data());
+    const bool diag = P.kind == kPrecDiag;
+    if (!diag || first) precond(r.data(), z.data());
     launch_cg_p<T>(s, n, z.data(), p.data(), st.data(), first ? 1 : 0);
     launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
-    launch_cg_xr<T>(s, ctx->red(), n, p.data(), q.data(), x, r.data(), P.cheb ? nullptr : invd,
-                    P.cheb ? nullptr : z.data(), st.data());
+    launch_cg_xr<T>(s, ctx->red(), n, p.data(), q.data(), x, r.data(), diag ? invd : nullptr,
+                    diag ? z.data() : nullptr, st.data());
   }
   // After r has been replaced from outside (reliable update): refresh z and rho1.
   void residual_replaced() {
-    if (!P.cheb) precond(r.data(), z.data());
+    if (P.kind == kPrecDiag) precond(r.data(), z.data());
   }
   CGState read_state() {
     ALIEN_HIP_CHECK(hipMemcpyAsync(h_st, st.data(), sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
@@ -248,9 +261,11 @@ struct FusedCG {
   }
 
   // Solve A x = b from the given x (x_is_zero: skip the initial residual SpMV).
-  // Returns the final state; bb = (b,b).
+  // Returns the final state; bb = (b,b). With stall_window > 0 the solve also
+  // stops (*stalled = true) once the residual is not finite, or has not dropped
+  // by a factor 10 within stall_window iterations.
   CGState solve(const T* b, T* x, bool x_is_zero, double tol, int max_it, bool graph, int poll_every,
-                double* bb_out) {
+                double* bb_out, int stall_window = 0, bool* stalled = nullptr) {
     hipStream_t s = ctx->stream;
     const int slot = ctx->take_slot();
     // Iteration ctor: (b,b) and the criteria
@@ -271,14 +286,30 @@ struct FusedCG {
     CGState cur = read_state();
     // without a graph, poll after every iteration (the reference's host-side stop test)
     const int k = graph ? std::max(1, poll_every) : 1;
-    while (!cur.done) cur = advance(x, graph, k);
+    double rr_ref = cur.rr;
+    int it_ref = cur.it;
+    while (!cur.done) {
+      cur = advance(x, graph, k);
+      if (stall_window <= 0) continue;
+      if (!std::isfinite(cur.rr)) {
+        *stalled = true;
+        break;
+      }
+      if (cur.rr < 0.01 * rr_ref) {
+        rr_ref = cur.rr;
+        it_ref = cur.it;
+      } else if (cur.it - it_ref >= stall_window && !cur.done) {
+        *stalled = true;
+        break;
+      }
+    }
     return cur;
   }
 };
 
 PrecondSpec cheb_spec(const Alien::ChebyshevPreconditioner<Alg>& C) {
   PrecondSpec ps;
-  ps.cheb = true;
+  ps.kind = kPrecCheb;
   ps.order = C.order();
   ps.a.resize(ps.order);
   ps.b.resize(ps.order);
@@ -315,14 +346,18 @@ void finish_breakdown(FusedCG<T>& F, CGState& s, int* error) {
 // residual b - A x64 is recomputed in FP64 and replaces the FP32 residual, and
 // CG continues with its search direction: no restart, so the Krylov space
 // built so far is kept. Convergence is judged on the FP64 true residual only.
+// With stall_window > 0 the solve gives up (stalled = true, x = the last
+// reliable update's solution) once the FP32 residual is not finite, or no
+// reliable update has come within stall_window iterations.
 struct MixedResult {
   int it = 0, updates = 0, status = 0, error = 0;
+  bool stalled = false;
   double rel = 0;
 };
 
 MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMat<float> A32, const float* invd32,
                               const PrecondSpec& ps, const double* b, double* x, double tol, int max_it,
-                              double delta, int poll_every, bool graph) {
+                              double delta, int poll_every, bool graph, int stall_window = 0) {
   MixedResult out;
   hipStream_t s = ctx->stream;
   const int slot = ctx->take_slot();
@@ -355,10 +390,17 @@ MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMa
   check_launch("mixed CG first iteration");
   CGState cur = F.read_state();
   double rr_last = rr64;
+  int it_last = 0;
   const double d2 = delta * delta;
   const int k = graph ? std::max(1, poll_every) : 1;
   for (;;) {
-    const bool update = cur.done || cur.rr < d2 * rr_last;
+    if (stall_window > 0 && !std::isfinite(cur.rr)) {
+      // the increment is lost; x holds the last reliable update
+      out.stalled = true;
+      break;
+    }
+    const bool stall = stall_window > 0 && cur.it - it_last >= stall_window;
+    const bool update = cur.done || cur.rr < d2 * rr_last || stall;
     if (!update) {
       cur = F.advance(x32.data(), graph, k);
       continue;
@@ -378,8 +420,13 @@ MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMa
       break;
     }
     if (cur.it >= max_it || !(rr64 == rr64)) break;
+    if (stall && !(rr64 < d2 * rr_last)) {
+      out.stalled = true;
+      break;
+    }
     launch_demote(s, n, r64.data(), F.r.data());
     rr_last = rr64;
+    it_last = cur.it;
     cur.done = 0;
     cur.status = 0;
     cur.rr = rr64;
@@ -420,9 +467,15 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     if (p->max_iter < 0 || !(p->tol >= 0)) throw Error(ALIEN_HIP_EINVAL, "solve_csr: bad tol / max_iter");
     const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
     const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
-    if ((fused || mixed) && (p->solver != ALIEN_HIP_CG ||
-                             (p->precond != ALIEN_HIP_PREC_CHEB && p->precond != ALIEN_HIP_PREC_DIAG)))
-      throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement CG with the Diag or Chebyshev preconditioner");
+    const bool learned = p->precond == ALIEN_HIP_PREC_LEARNED;
+    if ((fused || mixed) && (p->solver != ALIEN_HIP_CG || (p->precond != ALIEN_HIP_PREC_CHEB &&
+                                                           p->precond != ALIEN_HIP_PREC_DIAG && !learned)))
+      throw Error(ALIEN_HIP_EINVAL,
+                  "fused/graph/mixed modes implement CG with the Diag, Chebyshev or learned preconditioner");
+    if (learned && !(fused || mixed))
+      throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed CG modes");
+    if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
+    if (p->stall_window < 0) throw Error(ALIEN_HIP_EINVAL, "solve_csr: stall_window < 0");
     if (p->precond == ALIEN_HIP_PREC_CHEB && p->poly_order < 1)
       throw Error(ALIEN_HIP_EINVAL, "Chebyshev order must be >= 1");
 
@@ -500,18 +553,72 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         ps = cheb_spec(*cheb);
         invd64 = cheb->invDiag().data();
       } else {
+        // Diag, and the learned preconditioner's fallback
         diag.reset(new Alien::DiagPreconditioner<Alg>(alg, A));
         diag->init();
         invd64 = diag->invDiag().data();
       }
+      // learned preconditioner: network forward on A, then G and G^T as RB-CSR
+      // in the solver's precision
+      std::unique_ptr<gnn::Engine> net;
+      DeviceArray<float> theta, gv32, gtv32;
+      DeviceArray<double> gv64, gtv64;
+      int stall_window = 0;
+      if (learned) {
+        net.reset(new gnn::Engine(ctx, n, Aro, Acols, Avals));
+        theta.upload(p->gnn_theta, gnn::kNParam, ctx->stream);
+        net->forward(theta.data());
+        net->build_factor_layout();
+        ps.kind = kPrecFactor;
+        ps.g_bo = net->g_bo().data();
+        ps.g_bc = net->g_bc().data();
+        ps.gt_bo = net->gt_bo().data();
+        ps.gt_bc = net->gt_bc().data();
+        const size_t ng = net->g_bc().size(), ngt = net->gt_bc().size();
+        if (mixed) {
+          gv32.resize(ng);
+          gtv32.resize(ngt);
+          net->fill_factor<float>(gv32.data(), gtv32.data());
+          ps.g_v = gv32.data();
+          ps.gt_v = gtv32.data();
+        } else {
+          gv64.resize(ng);
+          gtv64.resize(ngt);
+          net->fill_factor<double>(gv64.data(), gtv64.data());
+          ps.g_v = gv64.data();
+          ps.gt_v = gtv64.data();
+        }
+        stall_window = p->stall_window > 0 ? p->stall_window : 400;
+      }
       if (!mixed) {
         RBMat<double> rb{A.d_bo.data(), A.d_bc.data(), A.d_bv.data()};
-        FusedCG<double> F(ctx, n, rb, invd64, ps);
+        const bool graph = p->mode == ALIEN_HIP_MODE_GRAPH;
+        DeviceArray<double> x0;
+        if (learned) {
+          x0.resize(n);
+          launch_copy<double>(ctx->stream, n, x.data(), x0.data());
+        }
         res->setup_s = tm.stop();
         tm.start();
-        CGState s = F.solve(b.data(), x.data(), false, p->tol, p->max_iter, p->mode == ALIEN_HIP_MODE_GRAPH,
-                            p->poll_every, nullptr);
-        finish_breakdown(F, s, &res->error);
+        bool stalled = false;
+        CGState s;
+        {
+          FusedCG<double> F(ctx, n, rb, invd64, ps);
+          s = F.solve(b.data(), x.data(), false, p->tol, p->max_iter, graph, p->poll_every, nullptr, stall_window,
+                      &stalled);
+          if (!stalled) finish_breakdown(F, s, &res->error);
+        }
+        if (stalled) {
+          // Jacobi from the last finite iterate, for the remaining iterations
+          res->fallback = 1;
+          if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
+          const int used = s.it;
+          FusedCG<double> J(ctx, n, rb, invd64, PrecondSpec{});
+          s = J.solve(b.data(), x.data(), false, p->tol, std::max(0, p->max_iter - used), graph, p->poll_every,
+                      nullptr);
+          finish_breakdown(J, s, &res->error);
+          s.it += used;
+        }
         res->solve_s = tm.stop();
         res->iterations = s.it;
         res->status = s.status;
@@ -526,9 +633,18 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         RBMat<double> rb64{A.d_bo.data(), A.d_bc.data(), A.d_bv.data()};
         res->setup_s = tm.stop();
         tm.start();
+        const double delta = p->inner_tol > 0 ? p->inner_tol : 0.1;
         MixedResult m = mixed_reliable_cg(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol,
-                                          p->max_iter, p->inner_tol > 0 ? p->inner_tol : 0.1, p->poll_every,
-                                          true);
+                                          p->max_iter, delta, p->poll_every, true, stall_window);
+        if (m.stalled) {
+          // Jacobi from the last reliable update, for the remaining iterations
+          res->fallback = 1;
+          const int used = m.it, ups = m.updates;
+          m = mixed_reliable_cg(ctx, n, rb64, rb32, invd32.data(), PrecondSpec{}, b.data(), x.data(), p->tol,
+                                std::max(0, p->max_iter - used), delta, p->poll_every, true);
+          m.it += used;
+          m.updates += ups;
+        }
         res->solve_s = tm.stop();
         res->iterations = m.it;
         res->outer = m.updates;

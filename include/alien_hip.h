@@ -188,7 +188,8 @@ int alien_hip_ilu0_solve_f64(alien_hip_ctx* ctx, int nrows, const int* row_offse
 /* ------------------------------------------------------------------------ */
 enum { ALIEN_HIP_CG = 0, ALIEN_HIP_BICGS = 1 };
 enum { ALIEN_HIP_PREC_DIAG = 0, ALIEN_HIP_PREC_CHEB = 1, ALIEN_HIP_PREC_ILU0 = 2,
-       ALIEN_HIP_PREC_NONE = 3 };
+       ALIEN_HIP_PREC_NONE = 3,
+       ALIEN_HIP_PREC_LEARNED = 4 /* M = G G^T from the graph network (FUSED/GRAPH/MIXED modes) */ };
 enum { ALIEN_HIP_LAYOUT_CSR = 0, ALIEN_HIP_LAYOUT_BELLPACK = 1, ALIEN_HIP_LAYOUT_RBCSR = 2 };
 enum {
   ALIEN_HIP_MODE_REFERENCE = 0, /* the reference call sequence, one launch per algebra call */
@@ -212,6 +213,10 @@ typedef struct {
   int    inner_max_iter;/* MIXED: inner iteration cap */
   int    max_outer;     /* MIXED: refinement steps cap */
   int    poll_every;    /* GRAPH: iterations per graph launch / stop-flag poll */
+  const float* gnn_theta; /* LEARNED: network parameters (alien_hip_gnn_nparams() floats) */
+  int    stall_window;  /* LEARNED: iterations without a tenfold drop of the residual norm
+                           (MIXED: without a reliable update) before the Jacobi fallback;
+                           0 = 400. A non-finite residual falls back at once. */
 } alien_hip_solve_params;
 
 typedef struct {
@@ -225,6 +230,8 @@ typedef struct {
   double cheb_beta;
   double setup_s;       /* conversion + preconditioner init, hipEvent-timed */
   double solve_s;       /* the Krylov solve, hipEvent-timed */
+  int    fallback;      /* LEARNED: 1 if the learned M stalled and Jacobi took over */
+  int    pad_;
 } alien_hip_solve_result;
 
 /* Host CSR in, host x out (x holds the initial guess on entry). Uploads,
@@ -232,6 +239,37 @@ typedef struct {
 int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_params* p, int nrows,
                         const int* row_offset, const int* cols, const double* vals,
                         const double* b, double* x, alien_hip_solve_result* res);
+
+/* ------------------------------------------------------------------------ */
+/* Learned preconditioner M = G G^T (no counterpart in the reference)        */
+/* ------------------------------------------------------------------------ */
+/* A graph network (3 message-passing layers, hidden width 32, bf16 weights on
+ * the matrix cores, FP32 accumulation) maps an SPD matrix with a symmetric
+ * pattern to a lower-triangular G on A's lower pattern. Parameters are one
+ * FP32 vector of alien_hip_gnn_nparams() entries. All pointers are HOST
+ * pointers; the calls upload, compute on the GPU and synchronise. */
+int alien_hip_gnn_nparams(void);
+/* Ghat per CSR entry of A (strictly-upper entries 0), G = D^-1/2 Ghat */
+int alien_hip_gnn_forward(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                          const double* vals, const float* theta, float* ghat);
+/* loss = Kaporin term + beta * probe term on 4 probe vectors V (nrows x 4, row-major);
+ * parts[0] = Kaporin term, parts[1] = probe term; grad has nparams entries */
+int alien_hip_gnn_loss_grad(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                            const double* vals, const float* theta, const float* probes, double beta,
+                            double* loss, double* parts, float* grad);
+typedef struct {
+  int      steps;            /* Adam steps */
+  int      steps_per_system; /* steps on one generated system before drawing the next */
+  int      n_min, n_max;     /* grid edge of the generated systems (log-normal diffusion, n^3) */
+  double   sigma_min, sigma_max; /* log-normal sigma, uniform */
+  double   aniso_max;        /* per-axis anisotropy, log-uniform in [1, aniso_max] */
+  double   lr;               /* Adam learning rate */
+  double   beta;             /* weight of the probe term */
+  uint64_t seed;
+} alien_hip_gnn_train_params;
+/* trains theta in place on the GPU; loss_hist (may be NULL) gets one value per step */
+int alien_hip_gnn_train(alien_hip_ctx* ctx, const alien_hip_gnn_train_params* p, float* theta,
+                        double* loss_hist);
 
 #ifdef __cplusplus
 }

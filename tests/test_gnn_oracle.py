@@ -1,0 +1,71 @@
+"""Pins the FP64 CPU oracle of the learned preconditioner (oracle/gnn_oracle.py):
+its analytic gradient against central finite differences, the SPD property of
+M = G G^T, the initial factor G = D^-1/2 (I + E_lower), and the bf16 rounding
+against numpy's own float32 round-to-nearest-even on exact cases."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import alien_amd as aa
+from oracle import gnn_oracle as go
+
+
+def small_graph(seed=3, dims=(5, 4, 3)):
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, *dims, params=[1.5, 1.0, 4.0, 0.5], seed=seed)
+    return go.Graph(ro, cols, vals)
+
+
+def perturbed_theta(seed=0):
+    th = go.init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 100)
+    return th + 0.05 * rng.standard_normal(th.shape)
+
+
+def test_gradient_matches_finite_differences():
+    g = small_graph()
+    th = perturbed_theta()
+    V = np.random.default_rng(7).choice([-1.0, 1.0], size=(g.n, 3))
+    L, grad, _ = go.loss_and_grad(g, th, V)
+    rng = np.random.default_rng(11)
+    idx = rng.choice(go.NPARAM, size=60, replace=False)
+    h = 1e-6
+    for k in idx:
+        tp, tm = th.copy(), th.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (go.loss(g, go.forward(g, tp)[0], V)[0] - go.loss(g, go.forward(g, tm)[0], V)[0]) / (2 * h)
+        assert abs(fd - grad[k]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (k, fd, grad[k])
+
+
+def test_initial_factor_and_spd():
+    g = small_graph()
+    G = go.preconditioner(g, go.init_params(0).astype(np.float64))
+    Dm = sp.diags(1.0 / np.sqrt(g.d))
+    expect = (Dm @ (sp.identity(g.n) + g.EL)).tocsr()
+    np.testing.assert_allclose(G.toarray(), expect.toarray(), rtol=1e-12, atol=1e-14)
+    th = perturbed_theta(1)
+    G = go.preconditioner(g, th)
+    M = (G @ G.T).toarray()
+    np.testing.assert_allclose(M, M.T, rtol=1e-13, atol=1e-15)
+    assert np.linalg.eigvalsh(M).min() > 0
+    assert np.all(np.diag(G.toarray()) > 0)
+    assert sp.triu(G, 1).nnz == 0
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7, 1.0 + 3 * 2 ** -8, -2.5, 0.0, 3.0e38])
+    r = go.bf16(x)
+    # 1 + 2^-8 is a tie -> even (1.0); 1 + 3*2^-8 is a tie -> even (1 + 2^-6)
+    np.testing.assert_array_equal(r[:6], [1.0, 1.0, 1.0 + 2 ** -7, 1.0 + 2 ** -6, -2.5, 0.0])
+    assert np.isfinite(r[6])
+
+
+def test_bf16_gradient_consistent_with_fp64():
+    g = small_graph(5)
+    th = perturbed_theta(2)
+    V = np.random.default_rng(3).choice([-1.0, 1.0], size=(g.n, 2))
+    L64, g64, _ = go.loss_and_grad(g, th, V, bf=False)
+    L16, g16, _ = go.loss_and_grad(g, th, V, bf=True)
+    assert abs(L16 - L64) <= 2e-2 * L64
+    cos = g64 @ g16 / (np.linalg.norm(g64) * np.linalg.norm(g16))
+    assert cos > 0.99

@@ -1,0 +1,200 @@
+"""GPU learned preconditioner M = G G^T against the FP64 oracle
+(oracle/gnn_oracle.py, itself pinned by finite differences).
+
+Tolerances: the GPU rounds FP32 activations to bf16 where the oracle rounds
+FP64 ones, so a value sitting on a bf16 rounding boundary can round the other
+way (one bf16 ulp, 2^-8 relative, on one term). Perturbing the oracle's inputs
+by 1e-7 relative moves Ghat by < 1e-4 of its max and the gradient by < 1e-5 in
+norm; the bounds below leave a factor >= 10 over that.
+"""
+import numpy as np
+import pytest
+
+import alien_amd as aa
+from oracle import gnn_oracle as go
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def system(n, sigma=1.5, seed=3, aniso=(1.0, 1.0, 1.0)):
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, n, n, [sigma, *aniso], seed)
+    return ro, cols, vals
+
+
+def active_theta(seed=5):
+    """Initial parameters with every head weight switched on."""
+    rng = np.random.default_rng(seed)
+    p = go.unpack(go.init_params(0))
+    p["w_d"] = rng.standard_normal(32) * 0.05
+    p["w_a"] = rng.standard_normal(32) * 0.05
+    p["w_b"] = rng.standard_normal(32) * 0.05
+    p["c_d"] = np.array([0.1])
+    return go.pack(p).astype(np.float32)
+
+
+def lower_mask(ro, cols):
+    rows = np.repeat(np.arange(len(ro) - 1), np.diff(ro))
+    return cols <= rows
+
+
+def test_nparams():
+    assert aa.gnn_nparams() == go.NPARAM == 6498
+
+
+@pytest.mark.parametrize("n", [6, 12])
+def test_forward_matches_oracle(ctx, n):
+    ro, cols, vals = system(n)
+    th = active_theta()
+    say("gnn forward", n)
+    gh = aa.gnn_forward(ctx, ro, cols, vals, th).astype(np.float64)
+    g = go.Graph(ro, cols, vals)
+    Gh, _ = go.forward(g, th, bf=True)
+    ref = np.asarray(sp_entries(Gh, ro, cols))
+    low = lower_mask(ro, cols)
+    assert np.all(gh[~low] == 0.0)
+    err = np.abs(gh - ref).max() / np.abs(ref).max()
+    say("  ghat max rel err", err)
+    assert err < 2e-3
+    assert np.median(np.abs(gh - ref)[low]) < 1e-5 * np.abs(ref).max()
+
+
+def sp_entries(M, ro, cols):
+    """Values of the sparse matrix M at the CSR positions (ro, cols)."""
+    M = M.tocsr()
+    rows = np.repeat(np.arange(len(ro) - 1), np.diff(ro))
+    return np.asarray(M[rows, cols]).ravel()
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.1])
+def test_loss_and_gradient_match_oracle(ctx, beta):
+    ro, cols, vals = system(10, seed=7)
+    th = active_theta(6)
+    rng = np.random.default_rng(11)
+    V = rng.choice([-1.0, 1.0], size=(len(ro) - 1, 4)).astype(np.float32)
+    say("gnn loss/grad beta", beta)
+    L, (lk, lp), grad = aa.gnn_loss_grad(ctx, ro, cols, vals, th, V, beta=beta)
+    g = go.Graph(ro, cols, vals)
+    Lo, go_grad, Gh = go.total_loss_and_grad(g, th, V.astype(np.float64), beta=beta, bf=True)
+    lko, _ = go.kcond_loss(g, Gh)
+    lpo, _ = go.loss(g, Gh, V.astype(np.float64))
+    say("  loss", L, Lo, "parts", lk, lko, lp, lpo)
+    assert L == pytest.approx(Lo, rel=1e-4, abs=1e-6)
+    assert lk == pytest.approx(lko, rel=1e-4, abs=1e-6)
+    assert lp == pytest.approx(lpo, rel=1e-4)
+    rel = np.linalg.norm(grad - go_grad) / np.linalg.norm(go_grad)
+    say("  grad rel err", rel)
+    assert rel < 1e-3
+    # per block: every parameter group is right, not just the largest
+    o = 0
+    for name, shape in go.LAYOUT:
+        k = int(np.prod(shape))
+        gb, ob = grad[o:o + k], go_grad[o:o + k]
+        assert np.linalg.norm(gb - ob) <= 1e-2 * np.linalg.norm(ob) + 1e-7, name
+        o += k
+
+
+def test_training_lowers_the_objective(ctx):
+    ro, cols, vals = system(10, sigma=1.0, seed=21)
+    rng = np.random.default_rng(2)
+    V = rng.choice([-1.0, 1.0], size=(len(ro) - 1, 4)).astype(np.float32)
+    th0 = go.init_params(0)
+    say("gnn train 60 steps")
+    th1, hist = aa.gnn_train(ctx, th0, steps=60, steps_per_system=4, n_min=8, n_max=12, sigma_min=0.5,
+                             sigma_max=1.5, aniso_max=2.0, lr=1e-3, beta=0.1, seed=4)
+    assert np.all(np.isfinite(hist)) and np.all(np.isfinite(th1))
+    L0, _, _ = aa.gnn_loss_grad(ctx, ro, cols, vals, th0, V, beta=0.1)
+    L1, _, _ = aa.gnn_loss_grad(ctx, ro, cols, vals, th1, V, beta=0.1)
+    say("  held-out objective", L0, "->", L1)
+    assert L1 < L0
+
+
+def pcg_iterations(A, b, apply_m, tol=1e-8, maxit=5000):
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = apply_m(r)
+    p = z.copy()
+    rho = r @ z
+    bb = b @ b
+    for it in range(1, maxit + 1):
+        q = A @ p
+        al = rho / (p @ q)
+        x += al * p
+        r -= al * q
+        if r @ r < tol * tol * bb:
+            return it
+        z = apply_m(r)
+        rn = r @ z
+        p = z + (rn / rho) * p
+        rho = rn
+    return maxit
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED])
+def test_learned_pcg_beats_jacobi(ctx, mode):
+    import scipy.sparse as sp
+    n = 20
+    ro, cols, vals = system(n, sigma=1.5, seed=3)
+    N = n ** 3
+    b = aa.uniform01(42, N)
+    th = go.init_params(0)
+    say("learned PCG", mode)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_LEARNED, mode=mode, tol=1e-8,
+                        max_iter=2000, gnn_theta=th, poll_every=4)
+    _, rj = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_DIAG, mode=mode, tol=1e-8,
+                         max_iter=2000, poll_every=4)
+    say("  learned", r["iterations"], "jacobi", rj["iterations"], "fallback", r["fallback"])
+    assert r["status"] == 1 and r["fallback"] == 0
+    assert r["true_rel_res"] < 1.05e-8
+    assert r["iterations"] < 0.7 * rj["iterations"]
+    if mode != aa.MODE_MIXED:
+        # same preconditioner from the oracle, FP64 PCG on the CPU
+        A = sp.csr_matrix((vals, cols, ro), shape=(N, N))
+        G = go.preconditioner(go.Graph(ro, cols, vals), th, bf=True)
+        GT = G.T.tocsr()
+        it_o = pcg_iterations(A, b, lambda v: G @ (GT @ v))
+        say("  oracle iterations", it_o)
+        assert abs(r["iterations"] - it_o) <= 2
+
+
+def broken_theta(which):
+    p = go.unpack(go.init_params(0))
+    if which == "overflow":
+        p["c_d"] = np.array([200.0])  # Ghat_ii = exp(200) overflows FP32
+    else:
+        p["c_e"] = np.array([100.0])  # G far from any inverse factor: CG stagnates
+    return go.pack(p).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
+@pytest.mark.parametrize("which", ["overflow", "stagnation"])
+def test_fallback_to_jacobi(ctx, mode, which):
+    n = 12
+    ro, cols, vals = system(n)
+    b = aa.uniform01(42, n ** 3)
+    say("learned fallback", which, mode)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_LEARNED, mode=mode, tol=1e-8,
+                        max_iter=3000, gnn_theta=broken_theta(which), stall_window=50, poll_every=4)
+    say("  ", r)
+    assert r["fallback"] == 1
+    assert r["status"] == 1
+    assert np.all(np.isfinite(x))
+    assert r["true_rel_res"] < 1.05e-8
+
+
+def test_learned_rejects_bad_input(ctx):
+    ro, cols, vals = system(4)
+    b = np.ones(64)
+    with pytest.raises(aa.AlienHipError):
+        aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_LEARNED, mode=aa.MODE_FUSED)  # no theta
+    with pytest.raises(aa.AlienHipError):
+        aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_LEARNED, gnn_theta=go.init_params(0))  # reference mode
+    v2 = vals.copy()
+    rows = np.repeat(np.arange(64), np.diff(ro))
+    v2[(cols == rows) & (rows == 5)] = -1.0
+    with pytest.raises(aa.AlienHipError):
+        aa.solve_csr(ctx, ro, cols, v2, b, precond=aa.PREC_LEARNED, mode=aa.MODE_FUSED,
+                     gnn_theta=go.init_params(0))
