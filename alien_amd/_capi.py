@@ -112,6 +112,7 @@ _SIGS = [
     ("alien_hip_solve_csr", C.c_int, [c_void_p, C.POINTER(SolveParams), C.c_int, c_int_p, c_int_p, c_double_p,
                                       c_double_p, c_double_p, C.POINTER(SolveResult)]),
     ("alien_hip_gnn_nparams", C.c_int, []),
+    ("alien_hip_gnn_init", C.c_int, [C.c_uint64, c_float_p]),
     ("alien_hip_gnn_forward", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, c_float_p, c_float_p]),
     ("alien_hip_gnn_loss_grad", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, c_float_p, c_float_p,
                                           C.c_double, c_double_p, c_double_p, c_float_p]),
@@ -293,7 +294,7 @@ class Context:
 
 def solve_csr(ctx: Context, ro, cols, vals, b, x0=None, *, solver=CG, precond=PREC_CHEB, layout=LAYOUT_CSR,
               mode=MODE_REFERENCE, tol=1e-8, max_iter=1000, poly_factor=0.5, poly_order=3, poly_max_iter=10,
-              ilu_schedule=0, inner_tol=1e-4, inner_max_iter=0, max_outer=50, poll_every=8, gnn_theta=None,
+              ilu_schedule=0, inner_tol=0.0, inner_max_iter=0, max_outer=50, poll_every=8, gnn_theta=None,
               stall_window=0):
     """Run one Krylov solve through the hip backend; returns (x, result dict)."""
     n = len(ro) - 1
@@ -319,6 +320,13 @@ def fptr(a: np.ndarray):
 
 def gnn_nparams() -> int:
     return lib().alien_hip_gnn_nparams()
+
+
+def gnn_init(seed: int = 0) -> np.ndarray:
+    """Initial parameters: random layers, heads at the first-order inverse factor."""
+    th = np.zeros(gnn_nparams(), dtype=np.float32)
+    check(lib().alien_hip_gnn_init(seed, fptr(th)))
+    return th
 
 
 def _csr_args(ro, cols, vals):

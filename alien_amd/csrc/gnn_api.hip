@@ -2,6 +2,7 @@
 // training loop over systems drawn from the heterogeneous-diffusion generator.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -33,6 +34,27 @@ void check_csr(int n, const int* ro, const int* cols, const double* vals) {
 extern "C" {
 
 int alien_hip_gnn_nparams(void) { return gnn::kNParam; }
+
+int alien_hip_gnn_init(uint64_t seed, float* theta) {
+  if (!theta) return fail(ALIEN_HIP_EINVAL, "gnn_init: NULL theta");
+  using namespace gnn;
+  // layers ~ N(0, s^2) with s = 1/sqrt(fan_in) (input layer) and 0.1/sqrt(fan_in)
+  // (message-passing layers: each starts close to the identity through its
+  // residual connection); biases 0; heads at the first-order inverse factor
+  // Ghat = I + E_lower (Ghat_ii = exp(0) = 1, Ghat_ij = e_ij * c_e with c_e = 1)
+  const int nw = kFeat * kHidden + 3 * 2 * kHidden * kHidden;
+  std::vector<double> z(nw);
+  fill_normal(seed, nw, z.data());
+  std::fill(theta, theta + kNParam, 0.f);
+  int q = 0;
+  for (int t = 0; t < kFeat * kHidden; ++t) theta[kOffWin + t] = (float)(z[q++] / std::sqrt((double)kFeat));
+  const int offs[3] = {kOffW1, kOffW2, kOffW3};
+  for (int l = 0; l < 3; ++l)
+    for (int t = 0; t < 2 * kHidden * kHidden; ++t)
+      theta[offs[l] + t] = (float)(z[q++] * 0.1 / std::sqrt(2.0 * kHidden));
+  theta[kOffCe] = 1.f;
+  return 0;
+}
 
 int alien_hip_gnn_forward(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
                           const float* theta, float* ghat) {

@@ -5,12 +5,14 @@ Workload (default): heterogeneous log-normal diffusion on a 256^3 grid
 harmonic-mean transmissibilities, Dirichlet x = 0 on the face i = 0;
 x_true ~ U[0,1) (splitmix64, seed 42), b = A x_true computed on the GPU.
 
-A "step" is one solve to an FP64 relative residual of 1e-8 (true residual
+A "step" is one CG solve to an FP64 relative residual of 1e-8 (true residual
 recomputed in FP64 on the GPU after the solve), inputs already resident in
-HBM. `value` is the mean time to 1e-8 of the mixed-precision solver (FP32
-matrix / preconditioner / vectors, FP64 dots, FP64 iterative refinement);
-the plain-FP64 solve of the same system is timed the same way and reported
-beside it.
+HBM. `value` is the mean solve time of the learned-preconditioner,
+mixed-precision solver: M = G G^T from the graph network (trained on the GPU
+first, on small systems of the same generator), FP32 matrix / factor / Krylov
+vectors with FP64 dots, FP64 reliable updates. The plain-FP64 solvers of the
+same system (Jacobi-CG, Chebyshev-CG, multicolour ILU0-CG) and mixed Jacobi-CG
+are timed beside it.
 
 The roofline object covers the dominant kernel, the SpMV, timed with HIP
 events on the stream it runs on; `cpu_baseline` times the CPU oracle (a
@@ -102,13 +104,49 @@ def cpu_baseline(aa, ro, cols, vals, b, fp64_iters, budget_s):
                       f"x {fp64_iters} iterations of the FP64 GPU solve"}
 
 
+def train_network(aa, ctx, a):
+    """Adam on generated log-normal systems (GPU forward/backward kernels), then model selection
+    between the initial and the trained parameters by FP64 learned-PCG iterations on a held-out
+    48^3 system (sigma 2, seed not used in training)."""
+    th0 = aa.gnn_init(0)
+    t0 = time.time()
+    th1, hist = aa.gnn_train(ctx, th0, steps=a.train_steps, steps_per_system=4, n_min=16, n_max=32,
+                             sigma_min=0.5, sigma_max=3.0, aniso_max=4.0, lr=1e-3, beta=0.1, seed=7)
+    t_train = time.time() - t0
+    hro, hcols, hvals = aa.gen_csr(aa.GEN_LOGNORMAL7, 48, params=[2.0, 1.0, 1.0, 1.0], seed=99991)
+    hb = aa.uniform01(5, len(hro) - 1)
+    its = {}
+    for name, th in (("init", th0), ("trained", th1)):
+        _, r = aa.solve_csr(ctx, hro, hcols, hvals, hb, solver=aa.CG, precond=aa.PREC_LEARNED,
+                            mode=aa.MODE_FUSED, tol=1e-8, max_iter=5000, gnn_theta=th)
+        its[name] = r["iterations"] if r["status"] == 1 and not r["fallback"] else 10 ** 9
+    _, rj = aa.solve_csr(ctx, hro, hcols, hvals, hb, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED,
+                         tol=1e-8, max_iter=5000)
+    chosen = "trained" if its["trained"] < its["init"] else "init"
+    log(f"training: {a.train_steps} steps in {t_train:.1f}s, loss {hist[:8].mean():.4f} -> {hist[-8:].mean():.4f}; "
+        f"held-out 48^3 iterations init={its['init']} trained={its['trained']} jacobi={rj['iterations']} "
+        f"-> {chosen}")
+    info = {"steps": a.train_steps, "time_s": t_train, "loss_first8": float(hist[:8].mean()),
+            "loss_last8": float(hist[-8:].mean()), "heldout_48cubed_iterations": dict(its, jacobi=rj["iterations"]),
+            "chosen": chosen,
+            "systems": "log-normal 16^3..32^3, sigma U[0.5,3], anisotropy log-U[1,4], 4 Adam steps per system"}
+    return (th1 if chosen == "trained" else th0), info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256, help="grid edge (256 -> the 256^3 workload)")
-    ap.add_argument("--precond", default="diag", choices=["diag", "cheb"])
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--no-slow-baselines", action="store_true",
+                    help="skip the FP64 Chebyshev-CG and multicolour ILU0-CG baselines")
+    ap.add_argument("--solvers", default="",
+                    help="comma list of solver runs to keep (learned_mixed, mixed_jacobi, fp64_jacobi, "
+                         "fp64_learned, fp64_chebyshev, fp64_ilu0_multicolour); default all")
+    ap.add_argument("--cheb-max-iter", type=int, default=3000)
+    ap.add_argument("--ilu-max-iter", type=int, default=5000)
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -164,30 +202,55 @@ def main():
         ctx.close()
         return
 
-    prec = aa.PREC_DIAG if a.precond == "diag" else aa.PREC_CHEB
-    common = dict(solver=aa.CG, precond=prec, tol=1e-8, max_iter=20000, poll_every=16)
+    # ---- learned preconditioner: train on the GPU, keep the better of init / trained on a held-out system
+    if a.train_steps > 0:
+        theta, training = train_network(aa, ctx, a)
+    else:
+        theta, training = aa.gnn_init(0), {"steps": 0, "chosen": "init"}
 
-    def run(mode, **kw):
-        x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, **common, **kw)
+    common = dict(solver=aa.CG, tol=1e-8, max_iter=20000, poll_every=16)
+
+    def run(mode, precond, **kw):
+        args = dict(common)
+        args.update(kw)
+        x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, precond=precond, **args)
         err = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
         return r, err
 
+    runs = [("learned_mixed", aa.MODE_MIXED, aa.PREC_LEARNED, dict(inner_tol=0.1, gnn_theta=theta), a.steps),
+            ("mixed_jacobi", aa.MODE_MIXED, aa.PREC_DIAG, dict(inner_tol=0.1), a.steps),
+            ("fp64_jacobi", aa.MODE_GRAPH, aa.PREC_DIAG, {}, a.steps),
+            ("fp64_learned", aa.MODE_GRAPH, aa.PREC_LEARNED, dict(gnn_theta=theta), 1)]
+    if not a.no_slow_baselines:
+        runs += [("fp64_chebyshev", aa.MODE_GRAPH, aa.PREC_CHEB, dict(max_iter=a.cheb_max_iter), 1),
+                 ("fp64_ilu0_multicolour", aa.MODE_REFERENCE, aa.PREC_ILU0,
+                  dict(ilu_schedule=1, layout=aa.LAYOUT_RBCSR, max_iter=a.ilu_max_iter), 1)]
+    if a.solvers:
+        keep = set(a.solvers.split(","))
+        runs = [r for r in runs if r[0] in keep]
     results = {}
-    for name, mode, kw in (("fp64", aa.MODE_GRAPH, {}),
-                           ("mixed", aa.MODE_MIXED, dict(inner_tol=0.1))):
-        for w in range(a.warmup):
+    for name, mode, prec, kw, steps in runs:
+        for w in range(a.warmup if steps > 1 else 0):
             log(f"warmup {name} {w}")
-            run(mode, **kw)
+            run(mode, prec, **kw)
         times, rs = [], []
-        for s in range(a.steps):
+        for s in range(steps):
             torch.cuda.synchronize()
-            r, err = run(mode, **kw)
+            r, err = run(mode, prec, **kw)
             torch.cuda.synchronize()
             times.append(r["solve_s"])
             rs.append(r)
-            log(f"{name} step {s}: {r['solve_s']:.3f}s its={r['iterations']} outer={r['outer']} "
-                f"status={r['status']} true_rel_res={r['true_rel_res']:.3e} err_vs_xtrue={err:.3e}")
-        results[name] = (float(np.mean(times)), rs[-1], err)
+            log(f"{name} step {s}: solve {r['solve_s']:.3f}s setup {r['setup_s']:.3f}s its={r['iterations']} "
+                f"outer={r['outer']} status={r['status']} fallback={r['fallback']} "
+                f"true_rel_res={r['true_rel_res']:.3e} err_vs_xtrue={err:.3e}")
+        r = rs[-1]
+        results[name] = {"time_s": float(np.mean(times)), "setup_s": r["setup_s"], "iterations": r["iterations"],
+                         "converged": bool(r["status"] == 1 and r["true_rel_res"] < 1.05e-8),
+                         "true_rel_res": r["true_rel_res"], "rel_err_vs_x_true": err, "steps": steps}
+        if mode == aa.MODE_MIXED:
+            results[name]["reliable_updates"] = r["outer"]
+        if prec == aa.PREC_LEARNED:
+            results[name]["fallback"] = r["fallback"]
 
     log("timing SpMV kernels")
     sp = time_spmv(aa, torch, ctx, ro, cols, vals, a.spmv_reps)
@@ -195,33 +258,36 @@ def main():
     b32 = spmv_bytes_fp32(N, nnz)
     achieved = b64 / sp["fp64"] / 1e9
 
-    fp64_t, fp64_r, _ = results["fp64"]
-    mix_t, mix_r, mix_err = results["mixed"]
+    if "learned_mixed" not in results:  # partial run (profiling): no JSON line
+        ctx.close()
+        return
+    main_t = results["learned_mixed"]["time_s"]
+    speedups = {f"vs_{k}": (v["time_s"] / main_t if v["converged"] else None)
+                for k, v in results.items() if k != "learned_mixed"}
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and "fp64_jacobi" in results:
         log("cpu baseline (oracle)")
-        cpu = cpu_baseline(aa, ro, cols, vals, b, fp64_r["iterations"], a.cpu_budget_s)
+        cpu = cpu_baseline(aa, ro, cols, vals, b, results["fp64_jacobi"]["iterations"], a.cpu_budget_s)
 
     out = {
         "metric": "time-to-1e-8 residual (s), SpMV HBM GB/s; learned+mixed vs plain FP64",
-        "value": mix_t,
+        "value": main_t,
         "unit": "s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": mix_t * 1e3,
+        "ms_per_step": main_t * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"log-normal diffusion {n}^3 (sigma=2), CG+{a.precond}, tol 1e-8", "N": N,
-                   "nnz": nnz, "precond": a.precond, "seed": 42},
-        "mixed": {"time_s": mix_t, "iterations": mix_r["iterations"], "reliable_updates": mix_r["outer"],
-                  "true_rel_res": mix_r["true_rel_res"], "rel_err_vs_x_true": mix_err},
-        "plain_fp64": {"time_s": fp64_t, "iterations": fp64_r["iterations"],
-                       "true_rel_res": fp64_r["true_rel_res"]},
-        "speedup_mixed_vs_fp64": fp64_t / mix_t,
+        "config": {"workload": f"log-normal diffusion {n}^3 (sigma=2), CG to FP64 relative residual 1e-8, "
+                               f"learned M=GG^T inside FP32/FP64 reliable-update CG", "N": N, "nnz": nnz,
+                   "seed": 42},
+        "solvers": results,
+        "speedup_learned_mixed": speedups,
+        "training": training,
         "spmv": {"fp64_gbs": achieved, "fp32_gbs": b32 / sp["fp32"] / 1e9,
                  "fp64_csr_gbs": b64 / sp["fp64_csr"] / 1e9, "fp64_us": sp["fp64"] * 1e6,
                  "fp32_us": sp["fp32"] * 1e6, "fp64_gflops": 2 * nnz / sp["fp64"] / 1e9},

@@ -209,9 +209,10 @@ typedef struct {
   int    poly_order;    /* --poly-order */
   int    poly_max_iter; /* --poly-factor-max-iter */
   int    ilu_schedule;  /* 0 = level sets (natural order, bit-exact), 1 = multicolour */
-  double inner_tol;     /* MIXED: inner relative tolerance */
-  int    inner_max_iter;/* MIXED: inner iteration cap */
-  int    max_outer;     /* MIXED: refinement steps cap */
+  double inner_tol;     /* MIXED: reliable-update threshold delta: the FP64 true residual
+                           replaces the FP32 one once it has dropped by delta (0 = 0.1) */
+  int    inner_max_iter;/* unused (kept for layout) */
+  int    max_outer;     /* unused (kept for layout) */
   int    poll_every;    /* GRAPH: iterations per graph launch / stop-flag poll */
   const float* gnn_theta; /* LEARNED: network parameters (alien_hip_gnn_nparams() floats) */
   int    stall_window;  /* LEARNED: iterations without a tenfold drop of the residual norm
@@ -249,6 +250,9 @@ int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_params* p, int
  * FP32 vector of alien_hip_gnn_nparams() entries. All pointers are HOST
  * pointers; the calls upload, compute on the GPU and synchronise. */
 int alien_hip_gnn_nparams(void);
+/* deterministic initial parameters (host only): random layers, output heads at
+ * the first-order inverse factor Ghat = I + E_lower */
+int alien_hip_gnn_init(uint64_t seed, float* theta);
 /* Ghat per CSR entry of A (strictly-upper entries 0), G = D^-1/2 Ghat */
 int alien_hip_gnn_forward(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
                           const double* vals, const float* theta, float* ghat);

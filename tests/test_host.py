@@ -162,3 +162,27 @@ def test_rbcsr_bundle_offsets():
     n = len(ro) - 1
     widths = [int(np.max(np.diff(ro[b * 64:min(n, b * 64 + 64) + 1]))) for b in range((n + 63) // 64)]
     np.testing.assert_array_equal(bo, np.concatenate([[0], np.cumsum(widths)]))
+
+
+def test_gnn_init_parameters():
+    """Host-side initial parameters: layout and scale per block, heads at the
+    first-order inverse factor, deterministic in the seed."""
+    from oracle import gnn_oracle as go
+    th = aa.gnn_init(0)
+    assert th.shape == (go.NPARAM,) and th.dtype == np.float32
+    np.testing.assert_array_equal(th, aa.gnn_init(0))
+    assert not np.array_equal(th, aa.gnn_init(1))
+    p = go.unpack(th)
+    assert p["c_e"][0] == 1.0 and p["c_d"][0] == 0.0
+    for k in ("w_d", "w_a", "w_b", "b_in", "b1", "b2", "b3"):
+        assert np.all(p[k] == 0.0), k
+    assert abs(np.std(p["W_in"]) - 0.5) < 0.1
+    for l in (1, 2, 3):
+        assert abs(np.std(p[f"W{l}"]) - 0.1 / 8) < 0.002
+    # with these heads the oracle's factor is D^-1/2 (I + E_lower) whatever the layers hold
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 4, 4, 4, [1.0, 1.0, 1.0, 1.0], 3)
+    g = go.Graph(ro, cols, vals)
+    Gh, _ = go.forward(g, th, bf=True)
+    import scipy.sparse as sp
+    ref = sp.eye(g.n) + g.EL
+    assert abs(Gh - ref).max() < 1e-6
