@@ -25,7 +25,7 @@ c_void_p = C.c_void_p
 GEN_POISSON7, GEN_CONVDIFF7, GEN_LOGNORMAL7 = 0, 1, 2
 CG, BICGS = 0, 1
 PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_LEARNED = 0, 1, 2, 3, 4
-LAYOUT_CSR, LAYOUT_BELLPACK, LAYOUT_RBCSR = 0, 1, 2
+LAYOUT_CSR, LAYOUT_BELLPACK, LAYOUT_RBCSR, LAYOUT_RBDIA = 0, 1, 2, 3
 MODE_REFERENCE, MODE_FUSED, MODE_GRAPH, MODE_MIXED = 0, 1, 2, 3
 EINVAL = -22
 
@@ -91,6 +91,16 @@ _SIGS = [
     ("alien_hip_rbcsr_fill_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p]),
     ("alien_hip_rbcsr_fill_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    ("alien_hip_rbdia_count", C.c_int64, [C.c_int, c_int_p, c_int_p, C.c_int, c_int_p]),
+    ("alien_hip_rbdia_struct", C.c_int, [C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, C.POINTER(C.c_uint64)]),
+    ("alien_hip_rbdia_fill_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    ("alien_hip_rbdia_fill_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    ("alien_hip_spmv_rbdia_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    ("alien_hip_spmv_rbdia_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p]),
     ("alien_hip_spmv_rbcsr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_spmv_rbcsr_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -253,6 +263,23 @@ def rbcsr_bundle_offsets(ro):
     bo = np.empty((n + 63) // 64 + 1, np.int32)
     check(lib().alien_hip_rbcsr_bundle_offsets(n, iptr(ro), iptr(bo)))
     return bo
+
+
+def rbdia_build(ro, cols, max_width=64):
+    """Host RB-DIA structure: (bundle_off, doff, dmask) or None when a bundle holds
+    more than max_width distinct column offsets."""
+    ro = np.ascontiguousarray(ro, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    n = len(ro) - 1
+    bo = np.zeros((n + 63) // 64 + 1, dtype=np.int32)
+    slots = lib().alien_hip_rbdia_count(n, iptr(ro), iptr(cols), max_width, iptr(bo))
+    if slots < 0:
+        return None
+    doff = np.zeros(max(slots, 1), dtype=np.int32)
+    dmask = np.zeros(max(slots, 1), dtype=np.uint64)
+    check(lib().alien_hip_rbdia_struct(n, iptr(ro), iptr(cols), max_width, iptr(doff),
+                                       dmask.ctypes.data_as(C.POINTER(C.c_uint64))))
+    return bo, doff[:slots], dmask[:slots]
 
 
 def group_by_level(level: np.ndarray, nlevels: int):

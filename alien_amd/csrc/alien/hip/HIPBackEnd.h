@@ -123,7 +123,7 @@ struct CSRHostView {
 };
 
 enum HIPLayout { kLayoutCSR = ALIEN_HIP_LAYOUT_CSR, kLayoutBEllPack = ALIEN_HIP_LAYOUT_BELLPACK,
-                 kLayoutRBCSR = ALIEN_HIP_LAYOUT_RBCSR };
+                 kLayoutRBCSR = ALIEN_HIP_LAYOUT_RBCSR, kLayoutRBDIA = ALIEN_HIP_LAYOUT_RBDIA };
 
 class HIPMatrix {
  public:
@@ -172,13 +172,75 @@ class HIPMatrix {
   // Builds the layout on first use and selects it for mult().
   void setSpmvLayout(int layout) {
     if (layout == kLayoutRBCSR) ensureRBCSR();
+    else if (layout == kLayoutRBDIA) ensureRB(true);
     else if (layout == kLayoutBEllPack) ensureBEllPack(false);
     else if (layout != kLayoutCSR) throw alien_hip::Error(ALIEN_HIP_EINVAL, "unknown SpMV layout");
     m_layout = layout;
   }
 
+  // Row-bundled layout of the fused solvers: RB-DIA (no column stream) when
+  // allowed and it stores at most kRbdiaMaxGrowth times the RB-CSR slots,
+  // RB-CSR otherwise. FP32 values follow with ensureRB32().
+  void ensureRB(bool allow_dia) {
+    if (!d_bo.empty() && (allow_dia || !m_rbdia)) return;
+    if (!d_bo.empty()) freeRB();
+    if (allow_dia) {
+      const int nb = (m_n + alien_hip::kBundle - 1) / alien_hip::kBundle;
+      std::vector<int> csr_bo(nb + 1);
+      const bool csr_ok = alien_hip::rbcsr_bundle_offsets(m_n, m_kcol.data(), csr_bo.data()) == 0;
+      std::vector<int> doff;
+      std::vector<uint64_t> dmask;
+      const long slots = alien_hip::rbdia_structure(m_n, m_kcol.data(), m_cols.data(), nullptr, nullptr,
+                                                    alien_hip::kRbdiaMaxWidth, m_bo, doff, dmask);
+      if (slots >= 0 && (!csr_ok || slots <= alien_hip::kRbdiaMaxGrowth * csr_bo[nb])) {
+        hipStream_t st = m_ctx->stream;
+        d_bo.upload(m_bo.data(), m_bo.size(), st);
+        d_doff.upload(doff.data(), doff.size(), st);
+        d_dmask.upload(reinterpret_cast<const unsigned long long*>(dmask.data()), dmask.size(), st);
+        d_bv.resize((size_t)slots * alien_hip::kBundle);
+        alien_hip::launch_rbdia_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
+                                             d_doff.data(), d_bv.data());
+        alien_hip::check_launch("rbdia_fill");
+        m_rbdia = true;
+        return;
+      }
+    }
+    ensureRBCSR();
+  }
+  void ensureRB32() {
+    if (d_bo.empty()) ensureRB(true);
+    if (!d_bv32.empty()) return;
+    d_bv32.resize(d_bv.size());
+    if (m_rbdia)
+      alien_hip::launch_rbdia_fill<float>(m_ctx->stream, m_n, d_ro.data(), d_cols.data(), d_vals.data(),
+                                          d_bo.data(), d_doff.data(), d_bv32.data());
+    else
+      alien_hip::launch_rbcsr_fill<float>(m_ctx->stream, m_n, d_ro.data(), d_cols.data(), d_vals.data(),
+                                          d_bo.data(), d_bc.data(), d_bv32.data());
+    alien_hip::check_launch("row-bundled fill<float>");
+  }
+  bool isRBDIA() const { return m_rbdia; }
+  alien_hip::RBMat<double> rb64() const {
+    return alien_hip::RBMat<double>{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv.data(),
+                                    m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr};
+  }
+  alien_hip::RBMat<float> rb32() const {
+    return alien_hip::RBMat<float>{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv32.data(),
+                                   m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr};
+  }
+  void freeRB() {
+    d_bo.reset();
+    d_bc.reset();
+    d_bv.reset();
+    d_bv32.reset();
+    d_doff.reset();
+    d_dmask.reset();
+    m_rbdia = false;
+  }
+
   void ensureRBCSR() {
-    if (!d_bo.empty()) return;
+    if (!d_bo.empty() && !m_rbdia) return;
+    if (!d_bo.empty()) freeRB();
     const int nb = (m_n + alien_hip::kBundle - 1) / alien_hip::kBundle;
     m_bo.resize(nb + 1);
     if (alien_hip::rbcsr_bundle_offsets(m_n, m_kcol.data(), m_bo.data()) != 0)
@@ -191,15 +253,6 @@ class HIPMatrix {
     alien_hip::launch_rbcsr_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
                                          d_bc.data(), d_bv.data());
     alien_hip::check_launch("rbcsr_fill");
-  }
-  void ensureRBCSR32() {
-    ensureRBCSR();
-    if (!d_bv32.empty()) return;
-    d_bv32.resize(d_bv.size());
-    // the same fill with float values (columns are rewritten identically)
-    alien_hip::launch_rbcsr_fill<float>(m_ctx->stream, m_n, d_ro.data(), d_cols.data(), d_vals.data(),
-                                        d_bo.data(), d_bc.data(), d_bv32.data());
-    alien_hip::check_launch("rbcsr_fill<float>");
   }
   void ensureBEllPack(bool masks) {
     if (d_bro.empty()) {
@@ -233,15 +286,18 @@ class HIPMatrix {
   // device arrays (public for the kernels' callers inside the backend)
   alien_hip::DeviceArray<int> d_ro, d_cols, d_dcol;
   alien_hip::DeviceArray<double> d_vals;
-  alien_hip::DeviceArray<int> d_bo, d_bc;        // RB-CSR
+  alien_hip::DeviceArray<int> d_bo, d_bc;        // RB-CSR (d_bo, d_bv, d_bv32 shared with RB-DIA)
   alien_hip::DeviceArray<double> d_bv;
   alien_hip::DeviceArray<float> d_bv32;
+  alien_hip::DeviceArray<int> d_doff;            // RB-DIA
+  alien_hip::DeviceArray<unsigned long long> d_dmask;
   alien_hip::DeviceArray<int> d_bro, d_bellc;    // BEllPack
   alien_hip::DeviceArray<double> d_bellv;
   alien_hip::DeviceArray<uint8_t> d_lmask, d_umask;
 
  private:
   alien_hip_ctx* m_ctx = nullptr;
+  bool m_rbdia = false;
   int m_n = 0, m_nnz = 0;
   int m_layout = kLayoutCSR;
   std::vector<int> m_kcol, m_cols, m_dcol, m_bo, m_bro;
@@ -323,8 +379,8 @@ class HIPInternalLinearAlgebra {
   void mult(const Matrix& A, const Vector& x, Vector& y) const {
     switch (A.spmvLayout()) {
     case kLayoutRBCSR:
-      alien_hip::launch_spmv_rbcsr<double>(stream(), A.getNRows(), A.d_bo.data(), A.d_bc.data(), A.d_bv.data(),
-                                           x.data(), y.data());
+    case kLayoutRBDIA:
+      alien_hip::launch_spmv_rb<double>(stream(), A.getNRows(), A.rb64(), x.data(), y.data());
       break;
     case kLayoutBEllPack:
       alien_hip::launch_spmv_bellpack<double>(stream(), A.getNRows(), HIPMatrix::kBEllPackSlice, A.d_bro.data(),

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace alien_hip {
 
@@ -37,5 +38,14 @@ void group_by_level(int nrows, const int* level, int nlevels, int* ptr, int* row
 
 constexpr int kBundle = 64; // rows per RB-CSR bundle = one wavefront
 int rbcsr_bundle_offsets(int nrows, const int* ro, int* bundle_off);
+// RB-DIA structure of the entries [kb(r), ke(r)) of each row (NULL kb/ke: whole
+// rows): bundle offsets (nb + 1), per-slot column offsets and row masks.
+// Returns the slot count per lane summed over bundles, or -1 when a bundle has
+// more than max_w distinct offsets or int32 indexing would overflow.
+long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, const int* ke, int max_w,
+                     std::vector<int>& bo, std::vector<int>& doff, std::vector<uint64_t>& dmask);
+// RB-DIA is chosen over RB-CSR when it stores at most this many more slots
+constexpr double kRbdiaMaxGrowth = 1.25;
+constexpr int kRbdiaMaxWidth = 64;
 
 } // namespace alien_hip

@@ -4,9 +4,11 @@
 // codes plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "alien/hip/HIPBackEnd.h"
 #include "alien_hip_internal.h"
@@ -186,6 +188,35 @@ int alien_hip_rbcsr_bundle_offsets(int nrows, const int* ro, int* bo) {
   return 0;
 }
 
+int64_t alien_hip_rbdia_count(int nrows, const int* ro, const int* cols, int max_w, int* bo) {
+  if (!(nrows >= 0 && ro && (cols || nrows == 0) && bo && max_w > 0))
+    return fail(ALIEN_HIP_EINVAL, "rbdia_count: bad arguments"), -1;
+  try {
+    std::vector<int> b, doff;
+    std::vector<uint64_t> m;
+    const long slots = rbdia_structure(nrows, ro, cols, nullptr, nullptr, max_w, b, doff, m);
+    if (slots < 0) return fail(ALIEN_HIP_EINVAL, "rbdia_count: too many offsets in a bundle"), -1;
+    std::copy(b.begin(), b.end(), bo);
+    return slots;
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what()), -1;
+  }
+}
+
+int alien_hip_rbdia_struct(int nrows, const int* ro, const int* cols, int max_w, int* doff_out,
+                           uint64_t* dmask_out) {
+  REQUIRE(nrows >= 0 && ro && (nrows == 0 || (cols && doff_out && dmask_out)) && max_w > 0,
+          "rbdia_struct: bad arguments");
+  API_BEGIN
+  std::vector<int> b, doff;
+  std::vector<uint64_t> m;
+  const long slots = rbdia_structure(nrows, ro, cols, nullptr, nullptr, max_w, b, doff, m);
+  if (slots < 0) return fail(ALIEN_HIP_EINVAL, "rbdia_struct: too many offsets in a bundle");
+  std::copy(doff.begin(), doff.end(), doff_out);
+  std::copy(m.begin(), m.end(), dmask_out);
+  API_END
+}
+
 // ---- device kernels -------------------------------------------------------------------
 #define CTX_OK(ctx) REQUIRE(ctx, "ctx is NULL")
 
@@ -302,6 +333,48 @@ int alien_hip_rbcsr_fill_f32(alien_hip_ctx* ctx, int n, const int* ro, const int
   API_BEGIN
   launch_rbcsr_fill<float>(ctx->stream, n, ro, cols, vals, bo, bc, bv);
   LAUNCHED("rbcsr_fill_f32");
+  API_END
+}
+
+int alien_hip_rbdia_fill_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                             const int* bo, const int* doff, double* bv) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && vals && bo && doff && bv)), "rbdia_fill_f64: bad arguments");
+  API_BEGIN
+  launch_rbdia_fill<double>(ctx->stream, n, ro, cols, vals, bo, doff, bv);
+  LAUNCHED("rbdia_fill_f64");
+  API_END
+}
+
+int alien_hip_rbdia_fill_f32(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                             const int* bo, const int* doff, float* bv) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && vals && bo && doff && bv)), "rbdia_fill_f32: bad arguments");
+  API_BEGIN
+  launch_rbdia_fill<float>(ctx->stream, n, ro, cols, vals, bo, doff, bv);
+  LAUNCHED("rbdia_fill_f32");
+  API_END
+}
+
+int alien_hip_spmv_rbdia_f64(alien_hip_ctx* ctx, int n, const int* bo, const int* doff, const uint64_t* dmask,
+                             const double* bv, const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (bo && doff && dmask && bv && x && y)), "spmv_rbdia_f64: bad arguments");
+  API_BEGIN
+  RBMat<double> A{bo, nullptr, bv, doff, reinterpret_cast<const unsigned long long*>(dmask)};
+  launch_spmv_rb<double>(ctx->stream, n, A, x, y);
+  LAUNCHED("spmv_rbdia_f64");
+  API_END
+}
+
+int alien_hip_spmv_rbdia_f32(alien_hip_ctx* ctx, int n, const int* bo, const int* doff, const uint64_t* dmask,
+                             const float* bv, const float* x, float* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (bo && doff && dmask && bv && x && y)), "spmv_rbdia_f32: bad arguments");
+  API_BEGIN
+  RBMat<float> A{bo, nullptr, bv, doff, reinterpret_cast<const unsigned long long*>(dmask)};
+  launch_spmv_rb<float>(ctx->stream, n, A, x, y);
+  LAUNCHED("spmv_rbdia_f32");
   API_END
 }
 

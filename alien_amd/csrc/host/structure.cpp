@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <iterator>
 #include <vector>
 
 namespace alien_hip {
@@ -155,6 +156,56 @@ int rbcsr_bundle_offsets(int nrows, const int* ro, int* bundle_off) {
   }
   bundle_off[nb] = (int)acc;
   return 0;
+}
+
+// RB-DIA: for each bundle of 64 rows the ascending distinct column offsets
+// c - r of the entries k in [kb(r), ke(r)) of its rows (kb = ro[r], ke =
+// ro[r+1] unless given), and for each (bundle, offset) the 64-bit mask of the
+// rows that hold it. Columns ascend inside a row, so a row's offsets ascend
+// too and each bundle's set is a merge of sorted lists; consecutive rows of a
+// stencil matrix repeat the same list, which is detected and skipped.
+long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, const int* ke, int max_w,
+                     std::vector<int>& bo, std::vector<int>& doff, std::vector<uint64_t>& dmask) {
+  const int nb = (nrows + kBundle - 1) / kBundle;
+  bo.assign(nb + 1, 0);
+  doff.clear();
+  dmask.clear();
+  std::vector<int> set, tmp, row_off, prev;
+  long acc = 0;
+  for (int b = 0; b < nb; ++b) {
+    bo[b] = (int)acc;
+    const int r0 = b * kBundle, r1 = std::min(nrows, r0 + kBundle);
+    set.clear();
+    prev.clear();
+    for (int r = r0; r < r1; ++r) {
+      const int k0 = kb ? kb[r] : ro[r], k1 = ke ? ke[r] : ro[r + 1];
+      row_off.clear();
+      for (int k = k0; k < k1; ++k) row_off.push_back(cols[k] - r);
+      if (row_off == prev) continue;
+      tmp.clear();
+      std::set_union(set.begin(), set.end(), row_off.begin(), row_off.end(), std::back_inserter(tmp));
+      set.swap(tmp);
+      prev.swap(row_off);
+      if ((int)set.size() > max_w) return -1;
+    }
+    const size_t s0 = doff.size();
+    doff.insert(doff.end(), set.begin(), set.end());
+    dmask.resize(doff.size(), 0);
+    for (int r = r0; r < r1; ++r) {
+      const int k0 = kb ? kb[r] : ro[r], k1 = ke ? ke[r] : ro[r + 1];
+      size_t j = s0;
+      for (int k = k0; k < k1; ++k) {
+        const int d = cols[k] - r;
+        while (j < doff.size() && doff[j] != d) ++j;
+        if (j == doff.size()) return -1; // columns not ascending
+        dmask[j] |= 1ull << (r - r0);
+      }
+    }
+    acc += (long)set.size();
+    if (acc * kBundle > 0x7fffffffL) return -1;
+  }
+  bo[nb] = (int)acc;
+  return acc;
 }
 
 } // namespace alien_hip

@@ -32,6 +32,7 @@ constexpr int kMaxProbes = 8;
 } // namespace alien_hip
 
 #include "../alien/hip/HIPContext.h"
+#include "krylov_state.h"
 
 namespace alien_hip {
 namespace gnn {
@@ -48,13 +49,14 @@ class Engine {
   // loss = Kaporin term + beta * probe term (4 probes V, device, N x 4); grad (device, kNParam)
   double loss_grad(const float* P, const float* V, int K, float beta, float* grad, double* parts);
   const float* ghat() const { return d_ghat.data(); }
-  // RB-CSR layouts of G and G^T and their values (after forward)
+  // Row-bundled layouts (RB-DIA or RB-CSR) of G and G^T and their values (after forward)
   void build_factor_layout();
   template <typename T> void fill_factor(T* g_vals, T* gt_vals);
-  const DeviceArray<int>& g_bo() const { return d_g_bo; }
-  const DeviceArray<int>& g_bc() const { return d_g_bc; }
-  const DeviceArray<int>& gt_bo() const { return d_gt_bo; }
-  const DeviceArray<int>& gt_bc() const { return d_gt_bc; }
+  size_t g_slots() const { return m_g.slots; }
+  size_t gt_slots() const { return m_gt.slots; }
+  bool factor_is_dia() const { return m_g.dia && m_gt.dia; }
+  template <typename T> RBMat<T> g_mat(const T* v) const { return m_g.mat(v); }
+  template <typename T> RBMat<T> gt_mat(const T* v) const { return m_gt.mat(v); }
 
  private:
   alien_hip_ctx* m_ctx;
@@ -64,7 +66,17 @@ class Engine {
   DeviceArray<double> d_vals;
   DeviceArray<float> d_dis, d_e, d_f, d_ghat;
   DeviceArray<float> d_h[4], d_z[4], d_m[4];
-  DeviceArray<int> d_g_bo, d_g_bc, d_g_sm, d_g_sr, d_gt_bo, d_gt_bc, d_gt_sm, d_gt_sr;
+  struct Factor {
+    bool dia = false;
+    size_t slots = 0;
+    DeviceArray<int> bo, bc, doff, sm, sr;
+    DeviceArray<unsigned long long> dmask;
+    template <typename T> RBMat<T> mat(const T* v) const {
+      return RBMat<T>{bo.data(), dia ? nullptr : bc.data(), v, dia ? doff.data() : nullptr,
+                      dia ? dmask.data() : nullptr};
+    }
+  };
+  Factor m_g, m_gt;
 };
 
 void launch_adam(hipStream_t s, int np, float* P, const float* g, float* m, float* v, float lr, int step);

@@ -604,61 +604,77 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, floa
   return lk + beta * lp;
 }
 
-// RB-CSR of G (rows = lower + diagonal of A) and of G^T (rows = diagonal + upper of A)
+// Row-bundled layouts of G (rows = lower + diagonal of A) and of G^T (rows =
+// diagonal + upper of A): RB-DIA when it stores at most kRbdiaMaxGrowth times
+// the RB-CSR slots, RB-CSR otherwise; plus the slot -> CSR position map of the
+// values (-1 = padding) and the row whose D^-1/2 scales the slot.
 void Engine::build_factor_layout() {
   if (!m_gbo.empty()) return;
   const int n = m_n;
   const int nb = (n + kBundle - 1) / kBundle;
+  std::vector<int> kb(n), ke(n);
   for (int t = 0; t < 2; ++t) {
-    std::vector<int>& bo = t == 0 ? m_gbo : m_gtbo;
-    bo.assign(nb + 1, 0);
+    for (int r = 0; r < n; ++r) {
+      kb[r] = t == 0 ? h_ro[r] : h_dcol[r];
+      ke[r] = t == 0 ? h_dcol[r] + 1 : h_ro[r + 1];
+    }
+    // RB-CSR widths
+    std::vector<int> cbo(nb + 1, 0);
     long acc = 0;
     for (int b = 0; b < nb; ++b) {
-      bo[b] = (int)acc;
+      cbo[b] = (int)acc;
       int w = 0;
-      for (int r = b * kBundle; r < std::min(n, (b + 1) * kBundle); ++r) {
-        const int len = t == 0 ? h_dcol[r] - h_ro[r] + 1 : h_ro[r + 1] - h_dcol[r];
-        w = std::max(w, len);
-      }
+      for (int r = b * kBundle; r < std::min(n, (b + 1) * kBundle); ++r) w = std::max(w, ke[r] - kb[r]);
       acc += w;
     }
-    bo[nb] = (int)acc;
-    const size_t slots = (size_t)acc * kBundle;
-    std::vector<int> bc(slots, -1), sm(slots, -1), sr(slots, 0);
+    cbo[nb] = (int)acc;
+    if (acc * kBundle > 0x7fffffffL) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: factor too large");
+    std::vector<int> dbo, doff;
+    std::vector<uint64_t> dmask;
+    const long dslots = rbdia_structure(n, h_ro.data(), h_cols.data(), kb.data(), ke.data(), kRbdiaMaxWidth, dbo,
+                                        doff, dmask);
+    const bool dia = dslots >= 0 && dslots <= kRbdiaMaxGrowth * cbo[nb];
+    std::vector<int>& bo = t == 0 ? m_gbo : m_gtbo;
+    bo = dia ? dbo : cbo;
+    const size_t slots = (size_t)bo[nb] * kBundle;
+    std::vector<int> bc(dia ? 0 : slots, -1), sm(slots, -1), sr(slots, 0);
     for (int r = 0; r < n; ++r) {
       const int b = r / kBundle, lane = r % kBundle;
-      const int k0 = t == 0 ? h_ro[r] : h_dcol[r];
-      const int k1 = t == 0 ? h_dcol[r] + 1 : h_ro[r + 1];
-      for (int k = k0; k < k1; ++k) {
-        const size_t sl = ((size_t)bo[b] + (k - k0)) * kBundle + lane;
-        bc[sl] = h_cols[k];
+      int j = 0;
+      for (int k = kb[r]; k < ke[r]; ++k, ++j) {
+        if (dia)
+          while (doff[bo[b] + j] != h_cols[k] - r) ++j; // present: the bundle's set holds the row's offsets
+        const size_t sl = ((size_t)bo[b] + j) * kBundle + lane;
+        if (!dia) bc[sl] = h_cols[k];
         sm[sl] = t == 0 ? k : h_tpos[k];
         sr[sl] = t == 0 ? r : h_cols[k]; // G_ij = dis_i Ghat_ij: scale by the row of G
       }
     }
     hipStream_t s = m_ctx->stream;
-    if (t == 0) {
-      d_g_bo.upload(bo.data(), bo.size(), s);
-      d_g_bc.upload(bc.data(), slots, s);
-      d_g_sm.upload(sm.data(), slots, s);
-      d_g_sr.upload(sr.data(), slots, s);
+    Factor& F = t == 0 ? m_g : m_gt;
+    F.dia = dia;
+    F.bo.upload(bo.data(), bo.size(), s);
+    if (dia) {
+      F.doff.upload(doff.data(), doff.size(), s);
+      F.dmask.upload(reinterpret_cast<const unsigned long long*>(dmask.data()), dmask.size(), s);
     } else {
-      d_gt_bo.upload(bo.data(), bo.size(), s);
-      d_gt_bc.upload(bc.data(), slots, s);
-      d_gt_sm.upload(sm.data(), slots, s);
-      d_gt_sr.upload(sr.data(), slots, s);
+      F.bc.upload(bc.data(), slots, s);
     }
+    F.sm.upload(sm.data(), slots, s);
+    F.sr.upload(sr.data(), slots, s);
+    F.slots = slots;
   }
+  ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
 }
 
 template <typename T>
 void Engine::fill_factor(T* g_vals, T* gt_vals) {
   build_factor_layout();
   hipStream_t s = m_ctx->stream;
-  const long ng = (long)d_g_bc.size(), ngt = (long)d_gt_bc.size();
-  hipLaunchKernelGGL(k_fill_g<T>, g1(ng), dim3(256), 0, s, ng, d_g_sm.data(), d_g_sr.data(), d_ghat.data(),
+  const long ng = (long)m_g.slots, ngt = (long)m_gt.slots;
+  hipLaunchKernelGGL(k_fill_g<T>, g1(ng), dim3(256), 0, s, ng, m_g.sm.data(), m_g.sr.data(), d_ghat.data(),
                      d_dis.data(), g_vals);
-  hipLaunchKernelGGL(k_fill_g<T>, g1(ngt), dim3(256), 0, s, ngt, d_gt_sm.data(), d_gt_sr.data(), d_ghat.data(),
+  hipLaunchKernelGGL(k_fill_g<T>, g1(ngt), dim3(256), 0, s, ngt, m_gt.sm.data(), m_gt.sr.data(), d_ghat.data(),
                      d_dis.data(), gt_vals);
   check_launch("gnn fill factor");
 }

@@ -24,13 +24,14 @@
 // hipGraph holding several iterations can be replayed without overshooting
 // and the iteration count stays exact.
 //
-// Row kernels process RB-CSR bundles, one wavefront per 64-row bundle;
+// Row kernels process RB-CSR / RB-DIA bundles, one wavefront per 64-row bundle;
 // streaming kernels handle kEPT elements per lane. Launch shapes depend on n
 // only, so the reductions are deterministic.
 #include "../alien_hip_internal.h"
 #include "device_common.h"
 #include "krylov_state.h"
 #include "launch.h"
+#include "rb_rows.h"
 
 namespace alien_hip {
 
@@ -40,39 +41,6 @@ namespace alien_hip {
 __device__ __forceinline__ int my_bundle(int n) {
   const int b = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kRowBlock + threadIdx.x) / kWave));
   return b < (n + kBundle - 1) / kBundle ? b : -1;
-}
-
-// Row sum of the RB-CSR row of this lane over the vector produced by `xs(j)`.
-template <typename T, typename XS>
-__device__ __forceinline__ T rb_sum(RBMat<T> A, int b, XS xs) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int s0 = A.bo[b];
-  const int w = A.bo[b + 1] - s0;
-  const long base = (long)s0 * kBundle + lane;
-  T acc = 0;
-  int j = 0;
-  constexpr int U = 7;
-  for (; j + U <= w; j += U) {
-    int c[U];
-    T v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      c[u] = __builtin_nontemporal_load(A.bc + base + (long)(j + u) * kBundle);
-      v[u] = __builtin_nontemporal_load(A.bv + base + (long)(j + u) * kBundle);
-    }
-    T xv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) xv[u] = (c[u] >= 0) ? xs(c[u]) : T(0);
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (c[u] >= 0) acc = acc + v[u] * xv[u];
-  }
-  for (; j < w; ++j) {
-    const int c = __builtin_nontemporal_load(A.bc + base + (long)j * kBundle);
-    const T v = __builtin_nontemporal_load(A.bv + base + (long)j * kBundle);
-    if (c >= 0) acc = acc + v * xs(c);
-  }
-  return acc;
 }
 
 // r = b - A x  (copy(b,r); mult(A,x,p); axpy(-1,p,r)); out = r.r

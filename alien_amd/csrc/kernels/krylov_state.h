@@ -8,12 +8,14 @@ namespace alien_hip {
 
 struct RedWork;
 
-// Row-bundled CSR operand (see spmv.hip)
+// Row-bundled operand (rb_rows.h): RB-CSR when doff is NULL, RB-DIA otherwise
 template <typename T>
 struct RBMat {
   const int* bo; // bundle offsets in 64-entry slots, nbundles + 1
-  const int* bc; // columns, -1 = padding
+  const int* bc; // RB-CSR: columns, -1 = padding
   const T* bv;   // values
+  const int* doff = nullptr;                  // RB-DIA: column offset of each bundle slot
+  const unsigned long long* dmask = nullptr;  // RB-DIA: rows of the bundle holding that slot
 };
 
 // Scalars of one CG solve; lives in device memory, read by the host only to

@@ -6,10 +6,11 @@
 
 #include <cstdint>
 
+#include "krylov_state.h"
+
 namespace alien_hip {
 
 struct RedWork;
-struct CGState;
 
 // what the second stage of a reduction does with the totals (reduce.hip)
 enum FinOp { kFinStore = 0, kFinRho1 = 1, kFinPQ = 2, kFinXR = 3, kFinXRDiag = 4 };
@@ -35,6 +36,12 @@ void launch_spmv_rbcsr(hipStream_t, int n, const int* bo, const int* bc, const T
 template <typename T>
 void launch_rbcsr_fill(hipStream_t, int n, const int* ro, const int* cols, const double* vals, const int* bo,
                        int* bc, T* bv);
+// RB-CSR (doff NULL) or RB-DIA SpMV, y = A x
+template <typename T>
+void launch_spmv_rb(hipStream_t, int n, RBMat<T> A, const T* x, T* y);
+template <typename T>
+void launch_rbdia_fill(hipStream_t, int n, const int* ro, const int* cols, const double* vals, const int* bo,
+                       const int* doff, T* bv);
 template <typename T>
 void launch_spmv_bellpack(hipStream_t, int n, int slice, const int* bro, const int* bc, const T* bv,
                           const T* x, T* y);

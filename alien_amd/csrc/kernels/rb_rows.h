@@ -1,0 +1,94 @@
+// Row sums over the row-bundled layouts (one wavefront per bundle of 64 rows,
+// lane l = row 64 b + l; entry j of the bundle's rows sits at slot
+// (bo[b] + j) * 64 + l).
+//
+//  * RB-CSR (column mode): each slot holds its column (-1 = padding).
+//  * RB-DIA (offset mode): slot j of bundle b is one "diagonal" of the bundle,
+//    the j-th smallest distinct column offset c - r among its rows' entries
+//    (doff[bo[b] + j]); bit l of dmask[bo[b] + j] says whether row 64 b + l has
+//    that entry. No column index is streamed: the gather address is r + d.
+//
+// In both modes each row's entries are visited in ascending column order and
+// absent entries are skipped, so the sum is the sequential CSR sum, bit for bit.
+// A batch of slots is loaded before any is used, so a wave keeps that many
+// value loads and gathers in flight even for short rows.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "krylov_state.h"
+
+namespace alien_hip {
+
+// U slots per batch. Slot indices past the bundle width are clamped to its
+// last slot, so every load is in bounds and unconditional (no scalar waits
+// between them); the clamped duplicates are masked out of the sum.
+template <int U, typename T, typename XS>
+__device__ __forceinline__ T rb_sum_u(const RBMat<T>& A, int b, int s0, int w, XS xs) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const long base = (long)s0 * kWave + lane;
+  T acc = 0;
+  if (A.doff) {
+    const int row = b * kWave + lane;
+    const unsigned long long bit = 1ull << lane;
+    for (int j = 0; j < w; j += U) {
+      int d[U];
+      unsigned long long m[U];
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int sl = min(j + u, w - 1);
+        d[u] = A.doff[s0 + sl];
+        m[u] = A.dmask[s0 + sl];
+        v[u] = __builtin_nontemporal_load(A.bv + base + (long)sl * kWave);
+      }
+      bool on[U];
+      T xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        on[u] = (j + u < w) && (m[u] & bit) != 0;
+        xv[u] = on[u] ? xs(row + d[u]) : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (on[u]) acc = acc + v[u] * xv[u];
+    }
+  } else {
+    for (int j = 0; j < w; j += U) {
+      int c[U];
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int sl = min(j + u, w - 1);
+        c[u] = __builtin_nontemporal_load(A.bc + base + (long)sl * kWave);
+        v[u] = __builtin_nontemporal_load(A.bv + base + (long)sl * kWave);
+      }
+      bool on[U];
+      T xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        on[u] = (j + u < w) && c[u] >= 0;
+        xv[u] = on[u] ? xs(c[u]) : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (on[u]) acc = acc + v[u] * xv[u];
+    }
+  }
+  return acc;
+}
+
+// Row sum of this lane's row of bundle b over the vector given by xs(column).
+// Short bundles (factors of M = G G^T: 4 slots at a 7-point stencil) use a
+// batch of 4, longer ones a batch of 7.
+template <typename T, typename XS>
+__device__ __forceinline__ T rb_sum(const RBMat<T>& A, int b, XS xs) {
+  const int s0 = A.bo[b];
+  const int w = A.bo[b + 1] - s0;
+  if (w <= 0) return T(0);
+  if (w <= 4) return rb_sum_u<4>(A, b, s0, w, xs);
+  return rb_sum_u<7>(A, b, s0, w, xs);
+}
+
+} // namespace alien_hip

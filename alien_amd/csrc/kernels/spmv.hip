@@ -21,6 +21,7 @@
 #include "../alien_hip_internal.h"
 #include "device_common.h"
 #include "launch.h"
+#include "rb_rows.h"
 
 namespace alien_hip {
 
@@ -64,52 +65,56 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot_csr(int n, const int* __res
   store_partials<1>(v, w);
 }
 
-// ---- RB-CSR -------------------------------------------------------------------------
-// One wavefront per bundle; U entries of a row are loaded before any is used,
-// so a wave keeps 2U independent streaming loads and U gathers in flight.
-template <typename T, int U>
-__device__ __forceinline__ T rb_row_sum(long base, int w, const int* __restrict__ bc,
-                                        const T* __restrict__ bv, const T* __restrict__ x) {
-  T acc = 0;
-  int j = 0;
-  for (; j + U <= w; j += U) {
-    int c[U];
-    T v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      c[u] = stream_load(bc + base + (long)(j + u) * kBundle);
-      v[u] = stream_load(bv + base + (long)(j + u) * kBundle);
-    }
-    T xv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) xv[u] = (c[u] >= 0) ? x[c[u]] : T(0);
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (c[u] >= 0) acc = acc + v[u] * xv[u];
-  }
-  for (; j < w; ++j) {
-    const int c = stream_load(bc + base + (long)j * kBundle);
-    const T v = stream_load(bv + base + (long)j * kBundle);
-    if (c >= 0) acc = acc + v * x[c];
-  }
-  return acc;
-}
-
+// ---- RB-CSR / RB-DIA --------------------------------------------------------------
+// One wavefront per bundle (rb_rows.h); two entry points so that profiles name
+// the layout.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_spmv_rbcsr(int n, const int* __restrict__ bo,
-                                                        const int* __restrict__ bc,
-                                                        const T* __restrict__ bv,
-                                                        const T* __restrict__ x, T* __restrict__ y) {
+__device__ __forceinline__ void spmv_rb_body(int n, const RBMat<T>& A, const T* __restrict__ x,
+                                             T* __restrict__ y) {
   const int lane = threadIdx.x & (kWave - 1);
   const int b = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) / kWave));
   const int nb = (n + kBundle - 1) / kBundle;
   if (b >= nb) return;
-  const int s0 = bo[b];
-  const int w = bo[b + 1] - s0;
-  const long base = (long)s0 * kBundle + lane;
-  const T acc = rb_row_sum<T, 7>(base, w, bc, bv, x);
+  const T acc = rb_sum<T>(A, b, [&](int c) { return x[c]; });
   const int row = b * kBundle + lane;
   if (row < n) y[row] = acc;
+}
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_rbcsr(int n, RBMat<T> A, const T* __restrict__ x,
+                                                        T* __restrict__ y) {
+  spmv_rb_body<T>(n, A, x, y);
+}
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_rbdia(int n, RBMat<T> A, const T* __restrict__ x,
+                                                        T* __restrict__ y) {
+  spmv_rb_body<T>(n, A, x, y);
+}
+
+// Device conversion CSR -> RB-DIA: lane = row; walks the row's entries and the
+// bundle's ascending offsets together, zero where the row has no entry.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_rbdia_fill(int n, const int* __restrict__ ro,
+                                                        const int* __restrict__ cols,
+                                                        const double* __restrict__ vals,
+                                                        const int* __restrict__ bo, const int* __restrict__ doff,
+                                                        T* __restrict__ bv) {
+  const int nb = (n + kBundle - 1) / kBundle;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int b = t / kBundle, lane = t % kBundle;
+  if (b >= nb) return;
+  const int row = t;
+  const int s0 = bo[b], w = bo[b + 1] - s0;
+  const long base = (long)s0 * kBundle + lane;
+  int k = 0, e = 0;
+  if (row < n) {
+    k = ro[row];
+    e = ro[row + 1];
+  }
+  for (int j = 0; j < w; ++j) {
+    T v = T(0);
+    if (k < e && cols[k] - row == doff[s0 + j]) v = (T)vals[k++];
+    bv[base + (long)j * kBundle] = v;
+  }
 }
 
 // Device conversion CSR -> RB-CSR (one lane per row writes its slots).
@@ -276,8 +281,19 @@ void launch_spmv_dot_csr(hipStream_t st, RedWork w, int n, const int* ro, const 
   launch_finalize(st, w, rows_grid(n).x, 1, kFinStore, out, nullptr);
 }
 template <typename T>
+void launch_spmv_rb(hipStream_t st, int n, RBMat<T> A, const T* x, T* y) {
+  if (n <= 0) return;
+  if (A.doff) hipLaunchKernelGGL(k_spmv_rbdia<T>, rows_grid(n), dim3(kBlock), 0, st, n, A, x, y);
+  else hipLaunchKernelGGL(k_spmv_rbcsr<T>, rows_grid(n), dim3(kBlock), 0, st, n, A, x, y);
+}
+template <typename T>
 void launch_spmv_rbcsr(hipStream_t st, int n, const int* bo, const int* bc, const T* bv, const T* x, T* y) {
-  if (n > 0) hipLaunchKernelGGL(k_spmv_rbcsr<T>, rows_grid(n), dim3(kBlock), 0, st, n, bo, bc, bv, x, y);
+  launch_spmv_rb<T>(st, n, RBMat<T>{bo, bc, bv}, x, y);
+}
+template <typename T>
+void launch_rbdia_fill(hipStream_t st, int n, const int* ro, const int* cols, const double* vals, const int* bo,
+                       const int* doff, T* bv) {
+  if (n > 0) hipLaunchKernelGGL(k_rbdia_fill<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, bo, doff, bv);
 }
 template <typename T>
 void launch_rbcsr_fill(hipStream_t st, int n, const int* ro, const int* cols, const double* vals,
@@ -320,6 +336,9 @@ void launch_invdiag_bellpack(hipStream_t st, int n, int slice, const int* bro, c
   template void launch_spmv_dot_csr<T>(hipStream_t, RedWork, int, const int*, const int*, const T*,     \
                                        const T*, T*, double*);                                          \
   template void launch_spmv_rbcsr<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*); \
+  template void launch_spmv_rb<T>(hipStream_t, int, RBMat<T>, const T*, T*);                           \
+  template void launch_rbdia_fill<T>(hipStream_t, int, const int*, const int*, const double*, const int*,\
+                                     const int*, T*);                                                    \
   template void launch_rbcsr_fill<T>(hipStream_t, int, const int*, const int*, const double*,           \
                                      const int*, int*, T*);                                             \
   template void launch_spmv_bellpack<T>(hipStream_t, int, int, const int*, const int*, const T*,        \

@@ -142,11 +142,17 @@ struct PrecondSpec {
   PrecKind kind = kPrecDiag;
   int order = 0;
   std::vector<double> a, b; // Chebyshev step coefficients
-  // factored approximate inverse M = G G^T: RB-CSR structure of G and G^T and
-  // their values, in the precision of the solver using them
-  const int *g_bo = nullptr, *g_bc = nullptr, *gt_bo = nullptr, *gt_bc = nullptr;
-  const void *g_v = nullptr, *gt_v = nullptr;
+  // factored approximate inverse M = G G^T: G and G^T row-bundled, in the
+  // precision of the solver using them (one pair set)
+  RBMat<double> g64{}, gt64{};
+  RBMat<float> g32{}, gt32{};
+  template <typename T> RBMat<T> g() const;
+  template <typename T> RBMat<T> gt() const;
 };
+template <> RBMat<double> PrecondSpec::g<double>() const { return g64; }
+template <> RBMat<double> PrecondSpec::gt<double>() const { return gt64; }
+template <> RBMat<float> PrecondSpec::g<float>() const { return g32; }
+template <> RBMat<float> PrecondSpec::gt<float>() const { return gt32; }
 
 template <typename T>
 struct FusedCG {
@@ -184,8 +190,7 @@ struct FusedCG {
       return;
     }
     if (P.kind == kPrecFactor) {
-      RBMat<T> G{P.g_bo, P.g_bc, (const T*)P.g_v}, GT{P.gt_bo, P.gt_bc, (const T*)P.gt_v};
-      launch_factor_apply<T>(s, ctx->red(), n, G, GT, rin, tmp.data(), zout, st.data());
+      launch_factor_apply<T>(s, ctx->red(), n, P.g<T>(), P.gt<T>(), rin, tmp.data(), zout, st.data());
       return;
     }
     const int m = P.order;
@@ -501,8 +506,8 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
 
     tm.start();
     Alien::CSRHostView view{n, Aro, Acols, Avals};
-    int layout = (fused || mixed) ? ALIEN_HIP_LAYOUT_RBCSR : p->layout;
-    HIPMatrix A(ctx, view, layout);
+    HIPMatrix A(ctx, view, (fused || mixed) ? ALIEN_HIP_LAYOUT_CSR : p->layout);
+    if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
     HIPVector b, x;
     alg.allocate(A.resource(), b, x);
     b.setValues(b_in);
@@ -570,28 +575,24 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         net->forward(theta.data());
         net->build_factor_layout();
         ps.kind = kPrecFactor;
-        ps.g_bo = net->g_bo().data();
-        ps.g_bc = net->g_bc().data();
-        ps.gt_bo = net->gt_bo().data();
-        ps.gt_bc = net->gt_bc().data();
-        const size_t ng = net->g_bc().size(), ngt = net->gt_bc().size();
+        const size_t ng = net->g_slots(), ngt = net->gt_slots();
         if (mixed) {
           gv32.resize(ng);
           gtv32.resize(ngt);
           net->fill_factor<float>(gv32.data(), gtv32.data());
-          ps.g_v = gv32.data();
-          ps.gt_v = gtv32.data();
+          ps.g32 = net->g_mat<float>(gv32.data());
+          ps.gt32 = net->gt_mat<float>(gtv32.data());
         } else {
           gv64.resize(ng);
           gtv64.resize(ngt);
           net->fill_factor<double>(gv64.data(), gtv64.data());
-          ps.g_v = gv64.data();
-          ps.gt_v = gtv64.data();
+          ps.g64 = net->g_mat<double>(gv64.data());
+          ps.gt64 = net->gt_mat<double>(gtv64.data());
         }
         stall_window = p->stall_window > 0 ? p->stall_window : 400;
       }
       if (!mixed) {
-        RBMat<double> rb{A.d_bo.data(), A.d_bc.data(), A.d_bv.data()};
+        RBMat<double> rb = A.rb64();
         const bool graph = p->mode == ALIEN_HIP_MODE_GRAPH;
         DeviceArray<double> x0;
         if (learned) {
@@ -626,11 +627,11 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
       } else {
         // FP32 matrix, preconditioner and Krylov vectors, FP64 dots; FP64 defect correction
-        A.ensureRBCSR32();
+        A.ensureRB32();
         DeviceArray<float> invd32(n);
         launch_demote(ctx->stream, n, invd64, invd32.data());
-        RBMat<float> rb32{A.d_bo.data(), A.d_bc.data(), A.d_bv32.data()};
-        RBMat<double> rb64{A.d_bo.data(), A.d_bc.data(), A.d_bv.data()};
+        RBMat<float> rb32 = A.rb32();
+        RBMat<double> rb64 = A.rb64();
         res->setup_s = tm.stop();
         tm.start();
         const double delta = p->inner_tol > 0 ? p->inner_tol : 0.1;

@@ -32,6 +32,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the roofline line covers the FP64 RB-DIA SpMV (the row sum of the fused FP64 solvers)
+ROOF, ROOF_KERNEL = "rbdia_fp64", "k_spmv_rbdia<double>"
+# HBM bytes per launch at 256^3 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by
+# the same counter on a copy kernel of known size (profiles/spmv_lognormal256_hbm_traffic.md)
+TRAFFIC = {}
 
 
 def log(*a):
@@ -39,6 +44,7 @@ def log(*a):
 
 
 def spmv_bytes_fp64(n, nnz):
+    """CSR / RB-CSR, FP64: values + columns + row offsets, x once, y."""
     return 12 * nnz + 4 * (n + 1) + 16 * n
 
 
@@ -46,11 +52,21 @@ def spmv_bytes_fp32(n, nnz):
     return 8 * nnz + 4 * (n + 1) + 8 * n
 
 
+def rbdia_bytes(n, nnz, slots, word):
+    """RB-DIA: values only (unpadded), x once, y, and the per-(bundle, slot)
+    descriptors (4-byte offset + 8-byte mask) and bundle offsets."""
+    nb = (n + 63) // 64
+    return word * nnz + 2 * word * n + 12 * slots + 4 * (nb + 1)
+
+
 def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
-    """Average duration of the FP64 and FP32 RB-CSR SpMV kernels (HIP events on the ctx stream)."""
+    """Average duration (s) of the SpMV kernels, HIP events on the ctx stream:
+    CSR FP64, RB-CSR FP64/FP32 and RB-DIA FP64/FP32."""
     import ctypes as C
     n = len(ro) - 1
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    P = lambda t: C.c_void_p(t.data_ptr())
+    L = aa.lib()
     dro, dc, dv = dev(ro), dev(cols), dev(vals)
     bo = aa.rbcsr_bundle_offsets(ro)
     dbo = dev(bo)
@@ -58,21 +74,34 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     bc = torch.empty(slots, dtype=torch.int32, device="cuda")
     bv = torch.empty(slots, dtype=torch.float64, device="cuda")
     bv32 = torch.empty(slots, dtype=torch.float32, device="cuda")
-    P = lambda t: C.c_void_p(t.data_ptr())
-    L = aa.lib()
+    dbo_d, doff, dmask = aa.rbdia_build(ro, cols)
+    dslots = int(dbo_d[-1])
+    ddbo, ddoff, ddmask = dev(dbo_d), dev(doff), dev(dmask.view(np.int64))
+    dv64 = torch.empty(dslots * 64, dtype=torch.float64, device="cuda")
+    dv32 = torch.empty(dslots * 64, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     aa.check(L.alien_hip_rbcsr_fill_f64(ctx.h, n, P(dro), P(dc), P(dv), P(dbo), P(bc), P(bv)))
     aa.check(L.alien_hip_rbcsr_fill_f32(ctx.h, n, P(dro), P(dc), P(dv), P(dbo), P(bc), P(bv32)))
+    aa.check(L.alien_hip_rbdia_fill_f64(ctx.h, n, P(dro), P(dc), P(dv), P(ddbo), P(ddoff), P(dv64)))
+    aa.check(L.alien_hip_rbdia_fill_f32(ctx.h, n, P(dro), P(dc), P(dv), P(ddbo), P(ddoff), P(dv32)))
     x = torch.rand(n, dtype=torch.float64, device="cuda")
     x32 = x.float()
     y = torch.empty(n, dtype=torch.float64, device="cuda")
+    y2 = torch.empty(n, dtype=torch.float64, device="cuda")
     y32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    y32b = torch.empty(n, dtype=torch.float32, device="cuda")
     ctx.synchronize()
     torch.cuda.synchronize()
-    res = {}
-    for name, fn in (("fp64", lambda: L.alien_hip_spmv_rbcsr_f64(ctx.h, n, P(dbo), P(bc), P(bv), P(x), P(y))),
-                     ("fp32", lambda: L.alien_hip_spmv_rbcsr_f32(ctx.h, n, P(dbo), P(bc), P(bv32), P(x32), P(y32))),
-                     ("fp64_csr", lambda: L.alien_hip_spmv_csr_f64(ctx.h, n, P(dro), P(dc), P(dv), P(x), P(y)))):
+    res = {"rbdia_slots": dslots}
+    kernels = (
+        ("rbcsr_fp64", lambda: L.alien_hip_spmv_rbcsr_f64(ctx.h, n, P(dbo), P(bc), P(bv), P(x), P(y))),
+        ("rbcsr_fp32", lambda: L.alien_hip_spmv_rbcsr_f32(ctx.h, n, P(dbo), P(bc), P(bv32), P(x32), P(y32))),
+        ("rbdia_fp64", lambda: L.alien_hip_spmv_rbdia_f64(ctx.h, n, P(ddbo), P(ddoff), P(ddmask), P(dv64), P(x),
+                                                          P(y2))),
+        ("rbdia_fp32", lambda: L.alien_hip_spmv_rbdia_f32(ctx.h, n, P(ddbo), P(ddoff), P(ddmask), P(dv32), P(x32),
+                                                          P(y32b))),
+        ("csr_fp64", lambda: L.alien_hip_spmv_csr_f64(ctx.h, n, P(dro), P(dc), P(dv), P(x), P(y))))
+    for name, fn in kernels:
         for _ in range(3):
             aa.check(fn())
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -82,7 +111,11 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
         e.record()
         e.synchronize()
         res[name] = s.elapsed_time(e) / reps * 1e-3
-    del bc, bv, bv32, dro, dc, dv, dbo
+    ctx.synchronize()
+    # the two layouts are the same sum: check once at full size
+    res["rbdia_equals_rbcsr_fp64"] = bool(torch.equal(y, y2))
+    res["rbdia_equals_rbcsr_fp32"] = bool(torch.equal(y32, y32b))
+    del bc, bv, bv32, dro, dc, dv, dbo, dv64, dv32, ddbo, ddoff, ddmask
     torch.cuda.empty_cache()
     return res
 
@@ -197,8 +230,12 @@ def main():
         for _ in range(a.spmv_reps):
             aa.check(aa.lib().alien_hip_copy_f64(ctx.h, N, P(src), P(dst)))
         ctx.synchronize()
-        print(json.dumps({"spmv_us": {k: v * 1e6 for k, v in sp.items()}, "copy_bytes_each_way": 8 * N,
-                          "spmv_fp64_algorithmic_bytes": spmv_bytes_fp64(N, nnz)}), flush=True)
+        print(json.dumps({"spmv_us": {k: v * 1e6 for k, v in sp.items() if isinstance(v, float)},
+                          "copy_bytes_each_way": 8 * N,
+                          "algorithmic_bytes": {
+                              "rbcsr_fp64": spmv_bytes_fp64(N, nnz), "rbcsr_fp32": spmv_bytes_fp32(N, nnz),
+                              "rbdia_fp64": rbdia_bytes(N, nnz, sp["rbdia_slots"], 8),
+                              "rbdia_fp32": rbdia_bytes(N, nnz, sp["rbdia_slots"], 4)}}), flush=True)
         ctx.close()
         return
 
@@ -254,9 +291,15 @@ def main():
 
     log("timing SpMV kernels")
     sp = time_spmv(aa, torch, ctx, ro, cols, vals, a.spmv_reps)
-    b64 = spmv_bytes_fp64(N, nnz)
-    b32 = spmv_bytes_fp32(N, nnz)
-    achieved = b64 / sp["fp64"] / 1e9
+    csr64 = spmv_bytes_fp64(N, nnz)
+    spmv = {"nnz": nnz, "rbdia_slots_per_row": sp["rbdia_slots"] * 64 / N,
+            "rbdia_equals_rbcsr": sp["rbdia_equals_rbcsr_fp64"] and sp["rbdia_equals_rbcsr_fp32"]}
+    for name, nbytes in (("csr_fp64", csr64), ("rbcsr_fp64", csr64), ("rbcsr_fp32", spmv_bytes_fp32(N, nnz)),
+                         ("rbdia_fp64", rbdia_bytes(N, nnz, sp["rbdia_slots"], 8)),
+                         ("rbdia_fp32", rbdia_bytes(N, nnz, sp["rbdia_slots"], 4))):
+        spmv[name] = {"us": sp[name] * 1e6, "bytes": nbytes, "gbs": nbytes / sp[name] / 1e9,
+                      "gflops": 2 * nnz / sp[name] / 1e9}
+    spmv["rbdia_fp64"]["csr_equivalent_gbs"] = csr64 / sp["rbdia_fp64"] / 1e9
 
     if "learned_mixed" not in results:  # partial run (profiling): no JSON line
         ctx.close()
@@ -288,12 +331,10 @@ def main():
         "solvers": results,
         "speedup_learned_mixed": speedups,
         "training": training,
-        "spmv": {"fp64_gbs": achieved, "fp32_gbs": b32 / sp["fp32"] / 1e9,
-                 "fp64_csr_gbs": b64 / sp["fp64_csr"] / 1e9, "fp64_us": sp["fp64"] * 1e6,
-                 "fp32_us": sp["fp32"] * 1e6, "fp64_gflops": 2 * nnz / sp["fp64"] / 1e9},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv_rbcsr<double>", "bytes_per_launch": b64},
+        "spmv": spmv,
+        "roofline": {"bound": "hbm", "achieved": spmv[ROOF]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": spmv[ROOF]["gbs"] / HBM_PEAK_GBS, "traffic": TRAFFIC.get(ROOF),
+                     "kernel": ROOF_KERNEL, "bytes_per_launch": spmv[ROOF]["bytes"]},
         "cpu_baseline": cpu,
     }
     ctx.close()

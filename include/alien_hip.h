@@ -142,6 +142,28 @@ int alien_hip_spmv_rbcsr_f64(alien_hip_ctx* ctx, int nrows, const int* bundle_of
 int alien_hip_spmv_rbcsr_f32(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* bcols,
                              const float* bvals, const float* x, float* y);
 
+/* Row-bundled diagonal offsets ("RB-DIA"): the bundles of RB-CSR, but slot j of
+ * bundle b stands for the j-th smallest distinct column offset c - r among the
+ * bundle's entries (doff[bundle_off[b] + j]); bit l of dmask[...] marks the
+ * rows 64 b + l holding that entry (value 0 elsewhere). No column index is
+ * streamed, only values; the per-row sum is still the sequential CSR sum,
+ * bit-exact with _seqMult. Host structure in two calls: rbdia_count fills
+ * bundle_off and returns the slot count (bundle_off[nbundles]), or -1 when a
+ * bundle holds more than max_width distinct offsets; rbdia_struct fills doff
+ * and dmask (that many entries each). Values are filled on the device. */
+int64_t alien_hip_rbdia_count(int nrows, const int* row_offset_host, const int* cols_host, int max_width,
+                              int* bundle_off_host);
+int alien_hip_rbdia_struct(int nrows, const int* row_offset_host, const int* cols_host, int max_width,
+                           int* doff_host, uint64_t* dmask_host);
+int alien_hip_rbdia_fill_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                             const double* vals, const int* bundle_off, const int* doff, double* bvals);
+int alien_hip_rbdia_fill_f32(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                             const double* vals, const int* bundle_off, const int* doff, float* bvals);
+int alien_hip_spmv_rbdia_f64(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* doff,
+                             const uint64_t* dmask, const double* bvals, const double* x, double* y);
+int alien_hip_spmv_rbdia_f32(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* doff,
+                             const uint64_t* dmask, const float* bvals, const float* x, float* y);
+
 /* ------------------------------------------------------------------------ */
 /* Device kernels: BLAS-1 (CBLASMPIKernel.h:37-125 / SYCLKernelInternal.h:110-246) */
 /* ------------------------------------------------------------------------ */
@@ -190,7 +212,11 @@ enum { ALIEN_HIP_CG = 0, ALIEN_HIP_BICGS = 1 };
 enum { ALIEN_HIP_PREC_DIAG = 0, ALIEN_HIP_PREC_CHEB = 1, ALIEN_HIP_PREC_ILU0 = 2,
        ALIEN_HIP_PREC_NONE = 3,
        ALIEN_HIP_PREC_LEARNED = 4 /* M = G G^T from the graph network (FUSED/GRAPH/MIXED modes) */ };
-enum { ALIEN_HIP_LAYOUT_CSR = 0, ALIEN_HIP_LAYOUT_BELLPACK = 1, ALIEN_HIP_LAYOUT_RBCSR = 2 };
+/* RB-CSR: row-bundled CSR; RB-DIA: row-bundled diagonal offsets (no column
+ * stream; RB-CSR when it would store over 1.25x the slots). In the FUSED, GRAPH
+ * and MIXED modes layout RBCSR forces RB-CSR, any other value selects RB-DIA. */
+enum { ALIEN_HIP_LAYOUT_CSR = 0, ALIEN_HIP_LAYOUT_BELLPACK = 1, ALIEN_HIP_LAYOUT_RBCSR = 2,
+       ALIEN_HIP_LAYOUT_RBDIA = 3 };
 enum {
   ALIEN_HIP_MODE_REFERENCE = 0, /* the reference call sequence, one launch per algebra call */
   ALIEN_HIP_MODE_FUSED = 1,     /* fused kernels, device-resident scalars, host polls stop test */

@@ -76,6 +76,42 @@ def test_spmv_csr_and_rbcsr_bit_exact(ctx, name, A):
     np.testing.assert_array_equal(host(y4, ctx)[:n], host(y3, ctx)[:n])
 
 
+@pytest.mark.parametrize("name,A", SYSTEMS, ids=[s[0] for s in SYSTEMS])
+def test_spmv_rbdia_bit_exact(ctx, name, A):
+    """RB-DIA (no column stream): FP64 bit-exact with the sequential CSR product,
+    FP32 bit-exact with the FP32 RB-CSR kernel (same per-row order)."""
+    ro, cols, vals = A
+    n = len(ro) - 1
+    built = aa.rbdia_build(ro, cols, max_width=1 << 20)
+    assert built is not None
+    bo, doff, dmask = built
+    slots = int(bo[-1]) * 64
+    say("case spmv rbdia", name, n, "slots/row", slots / max(n, 1))
+    x = np.random.default_rng(7).standard_normal(n)
+    ref = orc.spmv(ro, cols, vals, x)
+    dro, dc, dv, dx = dev(ro), dev(cols), dev(vals), dev(x)
+    dbo, ddo, ddm = dev(bo), dev(doff), dev(dmask.view(np.int64))
+    bv = empty(slots)
+    call("alien_hip_rbdia_fill_f64", ctx.h, n, p(dro), p(dc), p(dv), p(dbo), p(ddo), p(bv))
+    y = empty(n)
+    call("alien_hip_spmv_rbdia_f64", ctx.h, n, p(dbo), p(ddo), p(ddm), p(bv), p(dx), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:n], ref)
+    bv32 = empty(slots, torch.float32)
+    call("alien_hip_rbdia_fill_f32", ctx.h, n, p(dro), p(dc), p(dv), p(dbo), p(ddo), p(bv32))
+    dx32 = dev(x.astype(np.float32))
+    y32 = empty(n, torch.float32)
+    call("alien_hip_spmv_rbdia_f32", ctx.h, n, p(dbo), p(ddo), p(ddm), p(bv32), p(dx32), p(y32))
+    cbo = aa.rbcsr_bundle_offsets(ro)
+    cslots = int(cbo[-1]) * 64
+    dcbo = dev(cbo)
+    cbc = empty(cslots, torch.int32)
+    cbv = empty(cslots, torch.float32)
+    call("alien_hip_rbcsr_fill_f32", ctx.h, n, p(dro), p(dc), p(dv), p(dcbo), p(cbc), p(cbv))
+    yc = empty(n, torch.float32)
+    call("alien_hip_spmv_rbcsr_f32", ctx.h, n, p(dcbo), p(cbc), p(cbv), p(dx32), p(yc))
+    np.testing.assert_array_equal(host(y32, ctx)[:n], host(yc, ctx)[:n])
+
+
 @pytest.mark.parametrize("name,A", SYSTEMS[:3], ids=[s[0] for s in SYSTEMS[:3]])
 def test_spmv_bellpack_bit_exact(ctx, name, A):
     ro, cols, vals = A

@@ -127,3 +127,37 @@ def test_lognormal_small_cg_chebyshev_and_diag(ctx):
         _, o = orc.solve(ro, cols, vals, b, precond=ORC_PREC[precond], tol=1e-8, max_iter=3000)
         assert r["status"] == o["status"]
         assert abs(r["iterations"] - o["iterations"]) <= max(2, 0.02 * o["iterations"])
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED])
+@pytest.mark.parametrize("precond", [aa.PREC_DIAG, aa.PREC_CHEB])
+def test_rbdia_and_rbcsr_solves_identical(ctx, mode, precond):
+    """The fused solvers give bit-identical iterates on RB-DIA (default) and
+    RB-CSR: both layouts sum each row in the sequential CSR order."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 24, params=[2.0, 1, 1, 1], seed=11)
+    b = aa.uniform01(3, len(ro) - 1)
+    say("case rbdia vs rbcsr", mode, precond)
+    kw = dict(solver=aa.CG, precond=precond, mode=mode, tol=1e-8, max_iter=3000, poll_every=4)
+    x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBDIA, **kw)
+    x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBCSR, **kw)
+    assert r1["iterations"] == r2["iterations"] and r1["status"] == r2["status"]
+    np.testing.assert_array_equal(x1, x2)
+
+
+def test_unstructured_matrix_falls_back_to_rbcsr(ctx):
+    """A random SPD matrix has too many distinct offsets per bundle for RB-DIA:
+    the fused solver runs on RB-CSR and still matches the oracle's iterations."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(9)
+    n = 4000
+    R = sp.random(n, n, density=0.002, random_state=rng, format="csr")
+    A = (R + R.T + sp.diags(np.full(n, 10.0))).tocsr()
+    A.sort_indices()
+    ro, cols, vals = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+    assert aa.rbdia_build(ro, cols) is None
+    b = aa.uniform01(4, n)
+    say("case unstructured fused CG")
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED, tol=1e-10)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_DIAG, tol=1e-10)
+    assert r["status"] == 1 and abs(r["iterations"] - o["iterations"]) <= 2
+    assert r["true_rel_res"] < 1e-10

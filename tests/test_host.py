@@ -186,3 +186,66 @@ def test_gnn_init_parameters():
     import scipy.sparse as sp
     ref = sp.eye(g.n) + g.EL
     assert abs(Gh - ref).max() < 1e-6
+
+
+def rbdia_brute(ro, cols):
+    """Per bundle of 64 rows: sorted distinct offsets c - r, and the lane masks."""
+    n = len(ro) - 1
+    bo, doff, dmask = [0], [], []
+    for b in range((n + 63) // 64):
+        rows = range(b * 64, min(n, b * 64 + 64))
+        offs = sorted({int(cols[k]) - r for r in rows for k in range(ro[r], ro[r + 1])})
+        for d in offs:
+            m = 0
+            for r in rows:
+                if d + r in set(int(c) for c in cols[ro[r]:ro[r + 1]]):
+                    m |= 1 << (r - b * 64)
+            doff.append(d)
+            dmask.append(m)
+        bo.append(bo[-1] + len(offs))
+    return np.array(bo), np.array(doff, dtype=np.int64), np.array(dmask, dtype=np.uint64)
+
+
+def ragged_matrix(n=700, seed=5):
+    rng = np.random.default_rng(seed)
+    import scipy.sparse as sp
+    A = (sp.random(n, n, density=0.01, random_state=rng, format="csr") + sp.identity(n)).tolil()
+    for i in rng.choice(n, 20, replace=False):  # empty rows
+        A.rows[i] = []
+        A.data[i] = []
+    A = A.tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32)
+
+
+@pytest.mark.parametrize("case", ["poisson6", "convdiff5x4x3", "lognormal7", "ragged"])
+def test_rbdia_structure_vs_brute_force(case):
+    if case == "poisson6":
+        ro, cols, _ = aa.gen_csr(aa.GEN_POISSON7, 6)
+    elif case == "convdiff5x4x3":
+        ro, cols, _ = aa.gen_csr(aa.GEN_CONVDIFF7, 5, 4, 3)
+    elif case == "lognormal7":
+        ro, cols, _ = aa.gen_csr(aa.GEN_LOGNORMAL7, 7, params=[1.0, 1, 1, 1], seed=2)
+    else:
+        ro, cols = ragged_matrix()
+    got = aa.rbdia_build(ro, cols, max_width=100000)
+    exp = rbdia_brute(ro, cols)
+    for g, e in zip(got, exp):
+        np.testing.assert_array_equal(g.astype(np.int64) if g.dtype != np.uint64 else g, e)
+    # every CSR entry is in exactly one (bundle, slot) and nothing else is
+    bo, doff, dmask = got
+    total = sum(bin(int(m)).count("1") for m in dmask)
+    assert total == ro[-1]
+
+
+def test_rbdia_width_limit_and_stencil_width():
+    ro, cols, _ = aa.gen_csr(aa.GEN_POISSON7, 16)
+    bo, doff, _ = aa.rbdia_build(ro, cols)
+    # 7-point stencil on a 16^3 grid: a bundle is 4 x-lines; all 7 offsets except on
+    # the z = 0 and z = 15 planes (4 bundles each), which lack -256 / +256
+    w = np.diff(bo)
+    assert np.count_nonzero(w == 6) == 8 and np.count_nonzero(w == 7) == len(w) - 8
+    np.testing.assert_array_equal(np.unique(doff), [-256, -16, -1, 0, 1, 16, 256])
+    assert aa.rbdia_build(ro, cols, max_width=6) is None
+    ro2, cols2 = ragged_matrix()
+    assert aa.rbdia_build(ro2, cols2, max_width=64) is None
