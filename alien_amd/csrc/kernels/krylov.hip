@@ -206,6 +206,41 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_guarded(int n, RBMat<T> A
   if (row < n) y[row] = t;
 }
 
+// Factored preconditioner M = G G^T, first half: t = G^T r and rho1 = t.t
+// (= r.M r, the r.z of CG.h:124 with z = G t, without forming z)
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock) k_factor_t(int n, RBMat<T> GT, const T* __restrict__ r,
+                                                         T* __restrict__ t, CGState* st, RedWork w) {
+  if (st->done) return;
+  __shared__ double lds4[kRowBlock / kWave];
+  double s = 0.0;
+  const int b = my_bundle(n);
+  if (b >= 0) {
+    const T v = rb_sum<T>(GT, b, [&](int c) { return r[c]; });
+    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    if (row < n) {
+      t[row] = v;
+      s = (double)v * (double)v;
+    }
+  }
+  double v[1] = {block_sum<kRowBlock>(s, lds4)};
+  store_partials<1>(v, w);
+}
+
+// Second half fused with the direction update: z_i = (G t)_i is formed in
+// registers and p = z + (rho1/rho) p (p = z on the first step), as k_cg_p.
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock) k_factor_p(int n, RBMat<T> G, const T* __restrict__ t,
+                                                         T* __restrict__ p, const CGState* st, int first) {
+  if (st->done) return;
+  const int b = my_bundle(n);
+  if (b < 0) return;
+  const T beta = first ? T(0) : (T)(st->rho1 / st->rho);
+  const T z = rb_sum<T>(G, b, [&](int c) { return t[c]; });
+  const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+  if (row < n) p[row] = first ? z : z + beta * p[row];
+}
+
 // alpha = rho1 / pq ; x += alpha p ; r += (-alpha) q ; rr = r.r ; stop test.
 // DIAG: also z = invD r and rho1' = r.z for the next iteration.
 template <typename T, bool DIAG>
@@ -333,6 +368,14 @@ void launch_factor_apply(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> G
   launch_finalize(s, w, bundle_blocks(n), 1, kFinRho1, nullptr, st);
 }
 
+template <typename T>
+void launch_factor_p(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* t, T* p,
+                     CGState* st, int first) {
+  hipLaunchKernelGGL(k_factor_t<T>, bgrid(n), dim3(kRowBlock), 0, s, n, GT, r, t, st, w);
+  launch_finalize(s, w, bundle_blocks(n), 1, kFinRho1, nullptr, st);
+  hipLaunchKernelGGL(k_factor_p<T>, bgrid(n), dim3(kRowBlock), 0, s, n, G, t, p, (const CGState*)st, first);
+}
+
 void launch_fold_increment(hipStream_t s, int n, float* x32, double* x) {
   hipLaunchKernelGGL(k_fold_increment, egrid(n), dim3(kBlock), 0, s, n, x32, x);
 }
@@ -348,6 +391,8 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
   template void launch_cg_p<T>(hipStream_t, int, const T*, T*, CGState*, int);                       \
   template void launch_factor_apply<T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,  \
                                        CGState*);                                                        \
+  template void launch_factor_p<T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,     \
+                                   CGState*, int);                                                       \
   template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*);          \
   template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, T*, T*, const T*, T*,     \
                                 CGState*);

@@ -51,6 +51,10 @@ template <typename T>
 void launch_cg_xr(hipStream_t, RedWork w, int n, const T* p, const T* q, T* x, T* r, const T* invd_or_null,
                   T* z_or_null, CGState* st);
 // z = G (G^T r) and rho1 = r.z (factored approximate inverse M = G G^T)
+// M = G G^T fused with the direction update: t = G^T r, rho1 = t.t, p = G t + (rho1/rho) p
+template <typename T>
+void launch_factor_p(hipStream_t, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* t, T* p,
+                     CGState* st, int first);
 template <typename T>
 void launch_factor_apply(hipStream_t, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* tmp, T* z,
                          CGState* st);

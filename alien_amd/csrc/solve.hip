@@ -170,7 +170,8 @@ struct FusedCG {
   FusedCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
   : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
     ctx->reserve_for(n);
-    r.resize(n); z.resize(n); p.resize(n); q.resize(n);
+    r.resize(n); p.resize(n); q.resize(n);
+    if (P.kind != kPrecFactor) z.resize(n);
     if (P.kind == kPrecCheb) cp.resize(n);
     if (P.kind == kPrecCheb && P.order > 1) { y0.resize(n); y1.resize(n); }
     if (P.kind == kPrecFactor) tmp.resize(n);
@@ -209,8 +210,14 @@ struct FusedCG {
   void iteration(T* x, bool first) {
     hipStream_t s = ctx->stream;
     const bool diag = P.kind == kPrecDiag;
-    if (!diag || first) precond(r.data(), z.data());
-    launch_cg_p<T>(s, n, z.data(), p.data(), st.data(), first ? 1 : 0);
+    if (P.kind == kPrecFactor) {
+      // z = G (G^T r) never stored: rho1 = |G^T r|^2 and p = G t + beta p in one pass
+      launch_factor_p<T>(s, ctx->red(), n, P.g<T>(), P.gt<T>(), r.data(), tmp.data(), p.data(), st.data(),
+                         first ? 1 : 0);
+    } else {
+      if (!diag || first) precond(r.data(), z.data());
+      launch_cg_p<T>(s, n, z.data(), p.data(), st.data(), first ? 1 : 0);
+    }
     launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
     launch_cg_xr<T>(s, ctx->red(), n, p.data(), q.data(), x, r.data(), diag ? invd : nullptr,
                     diag ? z.data() : nullptr, st.data());
