@@ -101,7 +101,7 @@ struct alien_hip_ctx {
   }
   // Room for every kernel of this library on vectors of n rows.
   void reserve_for(long n) {
-    long b = alien_hip::bundle_blocks(n);
+    long b = alien_hip::bundle_blocks_max(n);
     long e = alien_hip::ept_blocks(n);
     long r = (n + alien_hip::kBlock - 1) / alien_hip::kBlock;
     reserve_red(std::max(b, std::max(e, r)));

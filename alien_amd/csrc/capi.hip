@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -97,6 +98,14 @@ const char* alien_hip_version(void) { return "alien_hip 0.1.0 (gfx950)"; }
 int alien_hip_ctx_create(void* stream, alien_hip_ctx** out) {
   REQUIRE(out, "ctx_create: out is NULL");
   API_BEGIN
+  // row-kernel grid caps (device_common.h)
+  const char* names[2] = {"ALIEN_HIP_ROW_BLOCKS_F64", "ALIEN_HIP_ROW_BLOCKS_F32"};
+  long* caps[2] = {&g_row_blocks_f64, &g_row_blocks_f32};
+  for (int i = 0; i < 2; ++i)
+    if (const char* e = std::getenv(names[i])) {
+      const long v = std::strtol(e, nullptr, 10);
+      if (v >= kXcds) *caps[i] = v / kXcds * kXcds;
+    }
   *out = make_ctx(stream);
   API_END
 }

@@ -37,11 +37,7 @@ namespace alien_hip {
 
 #define EPT_INDEX(k) (blockIdx.x * (long)(kBlock * kEPT) + (k) * kBlock + threadIdx.x)
 
-// Wave-uniform bundle of this wavefront, or -1 past the last bundle.
-__device__ __forceinline__ int my_bundle(int n) {
-  const int b = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kRowBlock + threadIdx.x) / kWave));
-  return b < (n + kBundle - 1) / kBundle ? b : -1;
-}
+// Row kernels walk their bundles with bundle_walk (device_common.h).
 
 // r = b - A x  (copy(b,r); mult(A,x,p); axpy(-1,p,r)); out = r.r
 template <typename T>
@@ -50,14 +46,14 @@ __global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const
                                                       RedWork w) {
   __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
-  const int b = my_bundle(n);
-  if (b >= 0) {
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
     const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       const T v = rhs[row] + T(-1) * t;
       r[row] = v;
-      s = (double)v * (double)v;
+      s += (double)v * (double)v;
     }
   }
   double v[1] = {block_sum<kRowBlock>(s, lds4)};
@@ -76,8 +72,8 @@ __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, cons
   __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
   const T* src = FIRST ? xin : yin;
-  const int b = my_bundle(n);
-  if (b >= 0) {
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
     const T t = rb_sum<T>(A, b, [&](int c) { return src[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
@@ -95,7 +91,7 @@ __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, cons
       }
       p[row] = pn;
       yout[row] = yo;
-      if (LAST) s = (double)xi * (double)yo;
+      if (LAST) s += (double)xi * (double)yo;
     }
   }
   if (LAST) {
@@ -159,13 +155,13 @@ __global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* 
   if (st->done) return;
   __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
-  const int b = my_bundle(n);
-  if (b >= 0) {
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
     const T t = rb_sum<T>(A, b, [&](int c) { return p[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       q[row] = t;
-      s = (double)p[row] * (double)t;
+      s += (double)p[row] * (double)t;
     }
   }
   double v[1] = {block_sum<kRowBlock>(s, lds4)};
@@ -181,13 +177,13 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_dot(int n, RBMat<T> A, co
   if (st->done) return;
   __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
-  const int b = my_bundle(n);
-  if (b >= 0) {
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
     const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       y[row] = t;
-      s = (double)wv[row] * (double)t;
+      s += (double)wv[row] * (double)t;
     }
   }
   double v[1] = {block_sum<kRowBlock>(s, lds4)};
@@ -199,11 +195,12 @@ template <typename T>
 __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_guarded(int n, RBMat<T> A, const T* __restrict__ x,
                                                                 T* __restrict__ y, const CGState* st) {
   if (st->done) return;
-  const int b = my_bundle(n);
-  if (b < 0) return;
-  const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
-  const int row = b * kBundle + (threadIdx.x & (kWave - 1));
-  if (row < n) y[row] = t;
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
+    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    if (row < n) y[row] = t;
+  }
 }
 
 // Factored preconditioner M = G G^T, first half: t = G^T r and rho1 = t.t
@@ -214,13 +211,13 @@ __global__ void __launch_bounds__(kRowBlock) k_factor_t(int n, RBMat<T> GT, cons
   if (st->done) return;
   __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
-  const int b = my_bundle(n);
-  if (b >= 0) {
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
     const T v = rb_sum<T>(GT, b, [&](int c) { return r[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       t[row] = v;
-      s = (double)v * (double)v;
+      s += (double)v * (double)v;
     }
   }
   double v[1] = {block_sum<kRowBlock>(s, lds4)};
@@ -233,12 +230,13 @@ template <typename T>
 __global__ void __launch_bounds__(kRowBlock) k_factor_p(int n, RBMat<T> G, const T* __restrict__ t,
                                                          T* __restrict__ p, const CGState* st, int first) {
   if (st->done) return;
-  const int b = my_bundle(n);
-  if (b < 0) return;
   const T beta = first ? T(0) : (T)(st->rho1 / st->rho);
-  const T z = rb_sum<T>(G, b, [&](int c) { return t[c]; });
-  const int row = b * kBundle + (threadIdx.x & (kWave - 1));
-  if (row < n) p[row] = first ? z : z + beta * p[row];
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
+    const T z = rb_sum<T>(G, b, [&](int c) { return t[c]; });
+    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    if (row < n) p[row] = first ? z : z + beta * p[row];
+  }
 }
 
 // alpha = rho1 / pq ; x += alpha p ; r += (-alpha) q ; rr = r.r ; stop test.
@@ -307,19 +305,20 @@ __global__ void __launch_bounds__(kBlock) k_demote(int n, const double* __restri
 }
 
 // ---- launchers -------------------------------------------------------------------------------
-static inline dim3 bgrid(long n) { return dim3((unsigned)bundle_blocks(n)); }
+template <typename T> static inline long rblocks(long n) { return bundle_blocks(n, (int)sizeof(T)); }
+template <typename T> static inline dim3 bgrid(long n) { return dim3((unsigned)rblocks<T>(n)); }
 static inline dim3 egrid(long n) { return dim3((unsigned)ept_blocks(n)); }
 
 template <typename T>
 void launch_residual(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out) {
-  hipLaunchKernelGGL(k_residual<T>, bgrid(n), dim3(kRowBlock), 0, s, n, A, x, b, r, w);
-  launch_finalize(s, w, bundle_blocks(n), 1, kFinStore, out, nullptr);
+  hipLaunchKernelGGL(k_residual<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, A, x, b, r, w);
+  launch_finalize(s, w, rblocks<T>(n), 1, kFinStore, out, nullptr);
 }
 
 template <typename T>
 void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin, const T* yin, T* yout, T* p,
                       const T* invd, double a, double b, bool first, bool last, CGState* st) {
-  dim3 g = bgrid(n), blk(kRowBlock);
+  dim3 g = bgrid<T>(n), blk(kRowBlock);
   if (first && last)
     hipLaunchKernelGGL((k_cheb_step<T, true, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
   else if (first)
@@ -328,7 +327,7 @@ void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin,
     hipLaunchKernelGGL((k_cheb_step<T, false, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
   else
     hipLaunchKernelGGL((k_cheb_step<T, false, false>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
-  if (last) launch_finalize(s, w, bundle_blocks(n), 1, kFinRho1, nullptr, st);
+  if (last) launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
 }
 
 template <typename T>
@@ -344,8 +343,8 @@ void launch_cg_p(hipStream_t s, int n, const T* z, T* p, CGState* st, int first)
 
 template <typename T>
 void launch_cg_q(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st) {
-  hipLaunchKernelGGL(k_cg_q<T>, bgrid(n), dim3(kRowBlock), 0, s, n, A, p, q, st, w);
-  launch_finalize(s, w, bundle_blocks(n), 1, kFinPQ, nullptr, st);
+  hipLaunchKernelGGL(k_cg_q<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, A, p, q, st, w);
+  launch_finalize(s, w, rblocks<T>(n), 1, kFinPQ, nullptr, st);
 }
 
 template <typename T>
@@ -363,17 +362,17 @@ void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* p, const T* q, T* x,
 template <typename T>
 void launch_factor_apply(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* tmp, T* z,
                          CGState* st) {
-  hipLaunchKernelGGL(k_rb_spmv_guarded<T>, bgrid(n), dim3(kRowBlock), 0, s, n, GT, r, tmp, (const CGState*)st);
-  hipLaunchKernelGGL(k_rb_spmv_dot<T>, bgrid(n), dim3(kRowBlock), 0, s, n, G, tmp, r, z, st, w);
-  launch_finalize(s, w, bundle_blocks(n), 1, kFinRho1, nullptr, st);
+  hipLaunchKernelGGL(k_rb_spmv_guarded<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, GT, r, tmp, (const CGState*)st);
+  hipLaunchKernelGGL(k_rb_spmv_dot<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, G, tmp, r, z, st, w);
+  launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
 }
 
 template <typename T>
 void launch_factor_p(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* t, T* p,
                      CGState* st, int first) {
-  hipLaunchKernelGGL(k_factor_t<T>, bgrid(n), dim3(kRowBlock), 0, s, n, GT, r, t, st, w);
-  launch_finalize(s, w, bundle_blocks(n), 1, kFinRho1, nullptr, st);
-  hipLaunchKernelGGL(k_factor_p<T>, bgrid(n), dim3(kRowBlock), 0, s, n, G, t, p, (const CGState*)st, first);
+  hipLaunchKernelGGL(k_factor_t<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, GT, r, t, st, w);
+  launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
+  hipLaunchKernelGGL(k_factor_p<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, G, t, p, (const CGState*)st, first);
 }
 
 void launch_fold_increment(hipStream_t s, int n, float* x32, double* x) {

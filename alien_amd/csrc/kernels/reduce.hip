@@ -9,6 +9,9 @@
 
 namespace alien_hip {
 
+long g_row_blocks_f64 = kUncapped;
+long g_row_blocks_f32 = 4096;
+
 constexpr int kFin = 1024;
 
 __global__ void __launch_bounds__(kFin) k_finalize(RedWork w, int nblk, int nslot, int op, double* out,

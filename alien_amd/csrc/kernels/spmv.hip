@@ -72,21 +72,21 @@ template <typename T>
 __device__ __forceinline__ void spmv_rb_body(int n, const RBMat<T>& A, const T* __restrict__ x,
                                              T* __restrict__ y) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int b = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) / kWave));
-  const int nb = (n + kBundle - 1) / kBundle;
-  if (b >= nb) return;
-  const T acc = rb_sum<T>(A, b, [&](int c) { return x[c]; });
-  const int row = b * kBundle + lane;
-  if (row < n) y[row] = acc;
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
+    const T acc = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const int row = b * kBundle + lane;
+    if (row < n) y[row] = acc;
+  }
 }
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_spmv_rbcsr(int n, RBMat<T> A, const T* __restrict__ x,
-                                                        T* __restrict__ y) {
+__global__ void __launch_bounds__(kRowBlock) k_spmv_rbcsr(int n, RBMat<T> A, const T* __restrict__ x,
+                                                           T* __restrict__ y) {
   spmv_rb_body<T>(n, A, x, y);
 }
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_spmv_rbdia(int n, RBMat<T> A, const T* __restrict__ x,
-                                                        T* __restrict__ y) {
+__global__ void __launch_bounds__(kRowBlock) k_spmv_rbdia(int n, RBMat<T> A, const T* __restrict__ x,
+                                                           T* __restrict__ y) {
   spmv_rb_body<T>(n, A, x, y);
 }
 
@@ -283,8 +283,9 @@ void launch_spmv_dot_csr(hipStream_t st, RedWork w, int n, const int* ro, const 
 template <typename T>
 void launch_spmv_rb(hipStream_t st, int n, RBMat<T> A, const T* x, T* y) {
   if (n <= 0) return;
-  if (A.doff) hipLaunchKernelGGL(k_spmv_rbdia<T>, rows_grid(n), dim3(kBlock), 0, st, n, A, x, y);
-  else hipLaunchKernelGGL(k_spmv_rbcsr<T>, rows_grid(n), dim3(kBlock), 0, st, n, A, x, y);
+  const dim3 g((unsigned)bundle_blocks(n, (int)sizeof(T)));
+  if (A.doff) hipLaunchKernelGGL(k_spmv_rbdia<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
+  else hipLaunchKernelGGL(k_spmv_rbcsr<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
 }
 template <typename T>
 void launch_spmv_rbcsr(hipStream_t st, int n, const int* bo, const int* bc, const T* bv, const T* x, T* y) {
