@@ -61,7 +61,7 @@ class Engine {
  private:
   alien_hip_ctx* m_ctx;
   int m_n, m_nnz = 0;
-  std::vector<int> h_ro, h_cols, h_dcol, h_tpos, m_gbo, m_gtbo;
+  std::vector<int> h_ro, h_cols, h_dcol, m_gbo, m_gtbo;
   DeviceArray<int> d_ro, d_cols, d_dcol, d_tpos;
   DeviceArray<double> d_vals;
   DeviceArray<float> d_dis, d_e, d_f, d_ghat;
@@ -69,7 +69,7 @@ class Engine {
   struct Factor {
     bool dia = false;
     size_t slots = 0;
-    DeviceArray<int> bo, bc, doff, sm, sr;
+    DeviceArray<int> bo, bc, doff;
     DeviceArray<unsigned long long> dmask;
     template <typename T> RBMat<T> mat(const T* v) const {
       return RBMat<T>{bo.data(), dia ? nullptr : bc.data(), v, dia ? doff.data() : nullptr,

@@ -444,15 +444,55 @@ __global__ void k_adam(int np, float* __restrict__ P, const float* __restrict__ 
   P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// RB-CSR values of G (rows: lower + diagonal of A) or G^T (rows: diagonal + upper),
-// slot -> CSR position map (-1 = padding); G_ij = dis_i Ghat_ij
-template <typename T>
-__global__ void k_fill_g(long nslots, const int* __restrict__ smap, const int* __restrict__ srow,
-                         const float* __restrict__ ghat, const float* __restrict__ dis, T* __restrict__ bv) {
-  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
-  const int p = smap[s];
-  bv[s] = p >= 0 ? (T)((double)dis[srow[s]] * (double)ghat[p]) : T(0);
+// Transpose positions: tpos[k] = position of (j, i) in row j for the entry
+// k = (i, j); a missing mirror entry (pattern not symmetric) raises *bad.
+__global__ void k_tpos(int n, const int* __restrict__ ro, const int* __restrict__ cols, int* __restrict__ tpos,
+                       int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = ro[i]; k < ro[i + 1]; ++k) {
+    const int j = cols[k];
+    int q = -1;
+    for (int t = ro[j]; t < ro[j + 1]; ++t)
+      if (cols[t] == i) {
+        q = t;
+        break;
+      }
+    tpos[k] = q;
+    if (q < 0) *bad = 1;
+  }
+}
+
+// Row-bundled values of G (TRANS = false: rows = lower + diagonal of A,
+// G_ij = dis_i Ghat_ij) or of G^T (TRANS: rows = diagonal + upper, (G^T)_ij =
+// G_ji = dis_j Ghat_ji, read at the transpose position). Lane = row; the row's
+// entries are walked together with the bundle's slots: slot j of an RB-CSR
+// bundle is the row's j-th entry, an RB-DIA slot the entry at offset doff.
+template <typename T, bool TRANS>
+__global__ void k_fill_factor(int n, const int* __restrict__ ro, const int* __restrict__ cols,
+                              const int* __restrict__ dcol, const int* __restrict__ tpos, const int* __restrict__ bo,
+                              const int* __restrict__ doff, const float* __restrict__ ghat,
+                              const float* __restrict__ dis, T* __restrict__ bv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = (n + kBundle - 1) / kBundle;
+  const int b = t / kBundle, lane = t % kBundle;
+  if (b >= nb) return;
+  const int row = t;
+  const int s0 = bo[b], w = bo[b + 1] - s0;
+  int k = 0, e = 0;
+  if (row < n) {
+    k = TRANS ? dcol[row] : ro[row];
+    e = TRANS ? ro[row + 1] : dcol[row] + 1;
+  }
+  for (int j = 0; j < w; ++j) {
+    T v = T(0);
+    if (k < e && (!doff || cols[k] - row == doff[s0 + j])) {
+      const double g = TRANS ? (double)dis[cols[k]] * (double)ghat[tpos[k]] : (double)dis[row] * (double)ghat[k];
+      v = (T)g;
+      ++k;
+    }
+    bv[((long)s0 + j) * kBundle + lane] = v;
+  }
 }
 
 } // namespace gnn
@@ -478,24 +518,26 @@ Engine::Engine(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const 
   h_cols.assign(cols, cols + nnz);
   h_dcol.resize(n);
   host_dcol(n, ro, cols, h_dcol.data());
-  h_tpos.assign(nnz, -1);
   for (int i = 0; i < n; ++i) {
-    if (h_dcol[i] >= ro[i + 1] || cols[h_dcol[i]] != i) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: missing diagonal");
+    if (h_dcol[i] >= ro[i + 1] || cols[h_dcol[i]] != i)
+      throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: missing diagonal");
     if (!(vals[h_dcol[i]] > 0)) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: non-positive diagonal");
-    for (int k = ro[i]; k < ro[i + 1]; ++k) {
-      const int j = cols[k];
-      int q = -1;
-      for (int t = ro[j]; t < ro[j + 1]; ++t)
-        if (cols[t] == i) { q = t; break; }
-      if (q < 0) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: the pattern must be symmetric");
-      h_tpos[k] = q;
-    }
   }
   hipStream_t s = m_ctx->stream;
   d_ro.upload(h_ro.data(), h_ro.size(), s);
   d_cols.upload(h_cols.data(), h_cols.size(), s);
   d_dcol.upload(h_dcol.data(), h_dcol.size(), s);
-  d_tpos.upload(h_tpos.data(), h_tpos.size(), s);
+  d_tpos.resize(nnz);
+  {
+    DeviceArray<int> bad(1);
+    ALIEN_HIP_CHECK(hipMemsetAsync(bad.data(), 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_tpos, g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_tpos.data(), bad.data());
+    check_launch("gnn tpos");
+    int hbad = 0;
+    ALIEN_HIP_CHECK(hipMemcpyAsync(&hbad, bad.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+    if (hbad) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: the pattern must be symmetric");
+  }
   d_vals.upload(vals, nnz, s);
   d_dis.resize(n);
   d_e.resize(nnz);
@@ -637,19 +679,6 @@ void Engine::build_factor_layout() {
     std::vector<int>& bo = t == 0 ? m_gbo : m_gtbo;
     bo = dia ? dbo : cbo;
     const size_t slots = (size_t)bo[nb] * kBundle;
-    std::vector<int> bc(dia ? 0 : slots, -1), sm(slots, -1), sr(slots, 0);
-    for (int r = 0; r < n; ++r) {
-      const int b = r / kBundle, lane = r % kBundle;
-      int j = 0;
-      for (int k = kb[r]; k < ke[r]; ++k, ++j) {
-        if (dia)
-          while (doff[bo[b] + j] != h_cols[k] - r) ++j; // present: the bundle's set holds the row's offsets
-        const size_t sl = ((size_t)bo[b] + j) * kBundle + lane;
-        if (!dia) bc[sl] = h_cols[k];
-        sm[sl] = t == 0 ? k : h_tpos[k];
-        sr[sl] = t == 0 ? r : h_cols[k]; // G_ij = dis_i Ghat_ij: scale by the row of G
-      }
-    }
     hipStream_t s = m_ctx->stream;
     Factor& F = t == 0 ? m_g : m_gt;
     F.dia = dia;
@@ -658,10 +687,13 @@ void Engine::build_factor_layout() {
       F.doff.upload(doff.data(), doff.size(), s);
       F.dmask.upload(reinterpret_cast<const unsigned long long*>(dmask.data()), dmask.size(), s);
     } else {
+      std::vector<int> bc(slots, -1);
+      for (int r = 0; r < n; ++r) {
+        const int b = r / kBundle, lane = r % kBundle;
+        for (int k = kb[r]; k < ke[r]; ++k) bc[((size_t)bo[b] + (k - kb[r])) * kBundle + lane] = h_cols[k];
+      }
       F.bc.upload(bc.data(), slots, s);
     }
-    F.sm.upload(sm.data(), slots, s);
-    F.sr.upload(sr.data(), slots, s);
     F.slots = slots;
   }
   ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
@@ -671,11 +703,13 @@ template <typename T>
 void Engine::fill_factor(T* g_vals, T* gt_vals) {
   build_factor_layout();
   hipStream_t s = m_ctx->stream;
-  const long ng = (long)m_g.slots, ngt = (long)m_gt.slots;
-  hipLaunchKernelGGL(k_fill_g<T>, g1(ng), dim3(256), 0, s, ng, m_g.sm.data(), m_g.sr.data(), d_ghat.data(),
-                     d_dis.data(), g_vals);
-  hipLaunchKernelGGL(k_fill_g<T>, g1(ngt), dim3(256), 0, s, ngt, m_gt.sm.data(), m_gt.sr.data(), d_ghat.data(),
-                     d_dis.data(), gt_vals);
+  const int nrows64 = (m_n + kBundle - 1) / kBundle * kBundle;
+  hipLaunchKernelGGL((k_fill_factor<T, false>), g1(nrows64), dim3(256), 0, s, m_n, d_ro.data(), d_cols.data(),
+                     d_dcol.data(), d_tpos.data(), m_g.bo.data(), m_g.dia ? m_g.doff.data() : nullptr,
+                     d_ghat.data(), d_dis.data(), g_vals);
+  hipLaunchKernelGGL((k_fill_factor<T, true>), g1(nrows64), dim3(256), 0, s, m_n, d_ro.data(), d_cols.data(),
+                     d_dcol.data(), d_tpos.data(), m_gt.bo.data(), m_gt.dia ? m_gt.doff.data() : nullptr,
+                     d_ghat.data(), d_dis.data(), gt_vals);
   check_launch("gnn fill factor");
 }
 template void Engine::fill_factor<float>(float*, float*);
