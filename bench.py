@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 ROOF, ROOF_KERNEL = "rbdia_fp64", "k_spmv_rbdia<double>"
 # HBM bytes per launch at 256^3 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by
 # the same counter on a copy kernel of known size (profiles/spmv_lognormal256_hbm_traffic.md)
-TRAFFIC = {}
+TRAFFIC = {"rbdia_fp64": 1537999856, "rbdia_fp32": 719756593, "rbcsr_fp64": 1952203693, "rbcsr_fp32": 1144810795}
 
 
 def log(*a):
@@ -333,7 +333,7 @@ def main():
         "training": training,
         "spmv": spmv,
         "roofline": {"bound": "hbm", "achieved": spmv[ROOF]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv[ROOF]["gbs"] / HBM_PEAK_GBS, "traffic": TRAFFIC.get(ROOF),
+                     "frac": spmv[ROOF]["gbs"] / HBM_PEAK_GBS, "traffic": TRAFFIC.get(ROOF) if n == 256 else None,
                      "kernel": ROOF_KERNEL, "bytes_per_launch": spmv[ROOF]["bytes"]},
         "cpu_baseline": cpu,
     }
