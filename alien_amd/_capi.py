@@ -51,7 +51,8 @@ class SolveResult(C.Structure):
 class GnnTrainParams(C.Structure):
     _fields_ = [("steps", C.c_int), ("steps_per_system", C.c_int), ("n_min", C.c_int), ("n_max", C.c_int),
                 ("sigma_min", C.c_double), ("sigma_max", C.c_double), ("aniso_max", C.c_double),
-                ("lr", C.c_double), ("beta", C.c_double), ("seed", C.c_uint64)]
+                ("lr", C.c_double), ("beta", C.c_double), ("seed", C.c_uint64), ("cg_steps", C.c_int),
+                ("pad_", C.c_int)]
 
 
 # (name, restype, argtypes)
@@ -125,7 +126,7 @@ _SIGS = [
     ("alien_hip_gnn_init", C.c_int, [C.c_uint64, c_float_p]),
     ("alien_hip_gnn_forward", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, c_float_p, c_float_p]),
     ("alien_hip_gnn_loss_grad", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, c_float_p, c_float_p,
-                                          C.c_double, c_double_p, c_double_p, c_float_p]),
+                                          C.c_double, C.c_int, c_double_p, c_double_p, c_float_p]),
     ("alien_hip_gnn_train", C.c_int, [c_void_p, C.POINTER(GnnTrainParams), c_float_p, c_double_p]),
 ]
 
@@ -373,8 +374,9 @@ def gnn_forward(ctx: Context, ro, cols, vals, theta) -> np.ndarray:
     return out
 
 
-def gnn_loss_grad(ctx: Context, ro, cols, vals, theta, probes, beta=0.1):
-    """(loss, (kaporin, probe), grad) of the training objective on 4 probes (N x 4)."""
+def gnn_loss_grad(ctx: Context, ro, cols, vals, theta, probes, beta=0.1, cg_steps=0):
+    """(loss, (L1, probe), grad) of the training objective on 4 probes (N x 4); L1 = Kaporin
+    (cg_steps = 0) or the residual reduction after cg_steps PCG steps."""
     ro, cols, vals = _csr_args(ro, cols, vals)
     theta = np.ascontiguousarray(theta, dtype=np.float32)
     probes = np.ascontiguousarray(probes, dtype=np.float32)
@@ -383,15 +385,16 @@ def gnn_loss_grad(ctx: Context, ro, cols, vals, theta, probes, beta=0.1):
     parts = np.zeros(2)
     grad = np.zeros(gnn_nparams(), dtype=np.float32)
     check(lib().alien_hip_gnn_loss_grad(ctx.h, len(ro) - 1, iptr(ro), iptr(cols), dptr(vals), fptr(theta),
-                                        fptr(probes), beta, C.byref(loss), dptr(parts), fptr(grad)))
+                                        fptr(probes), beta, cg_steps, C.byref(loss), dptr(parts), fptr(grad)))
     return loss.value, (parts[0], parts[1]), grad
 
 
 def gnn_train(ctx: Context, theta, *, steps=200, steps_per_system=4, n_min=8, n_max=16, sigma_min=0.5,
-              sigma_max=2.0, aniso_max=4.0, lr=2e-3, beta=0.1, seed=1):
+              sigma_max=2.0, aniso_max=4.0, lr=2e-3, beta=0.1, seed=1, cg_steps=0):
     """Adam on generated log-normal diffusion systems; returns (theta, loss history)."""
     th = np.array(theta, dtype=np.float32, copy=True)
     hist = np.zeros(steps)
-    p = GnnTrainParams(steps, steps_per_system, n_min, n_max, sigma_min, sigma_max, aniso_max, lr, beta, seed)
+    p = GnnTrainParams(steps, steps_per_system, n_min, n_max, sigma_min, sigma_max, aniso_max, lr, beta, seed,
+                       cg_steps, 0)
     check(lib().alien_hip_gnn_train(ctx.h, C.byref(p), fptr(th), dptr(hist)))
     return th, hist

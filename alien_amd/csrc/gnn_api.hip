@@ -76,9 +76,10 @@ int alien_hip_gnn_forward(alien_hip_ctx* ctx, int n, const int* ro, const int* c
 }
 
 int alien_hip_gnn_loss_grad(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
-                            const float* theta, const float* probes, double beta, double* loss, double* parts,
-                            float* grad) {
+                            const float* theta, const float* probes, double beta, int cg_steps, double* loss,
+                            double* parts, float* grad) {
   if (!ctx || !theta || !probes || !loss || !grad) return fail(ALIEN_HIP_EINVAL, "gnn_loss_grad: NULL argument");
+  if (cg_steps < 0 || cg_steps > gnn::kMaxCgSteps) return fail(ALIEN_HIP_EINVAL, "gnn_loss_grad: bad cg_steps");
   try {
     check_csr(n, ro, cols, vals);
     gnn::Engine E(ctx, n, ro, cols, vals);
@@ -86,7 +87,7 @@ int alien_hip_gnn_loss_grad(alien_hip_ctx* ctx, int n, const int* ro, const int*
     P.upload(theta, gnn::kNParam, ctx->stream);
     V.upload(probes, (size_t)n * 4, ctx->stream);
     double pp[2];
-    *loss = E.loss_grad(P.data(), V.data(), 4, (float)beta, G.data(), pp);
+    *loss = E.loss_grad(P.data(), V.data(), 4, (float)beta, cg_steps, G.data(), pp);
     if (parts) {
       parts[0] = pp[0];
       parts[1] = pp[1];
@@ -104,7 +105,8 @@ int alien_hip_gnn_loss_grad(alien_hip_ctx* ctx, int n, const int* ro, const int*
 int alien_hip_gnn_train(alien_hip_ctx* ctx, const alien_hip_gnn_train_params* p, float* theta, double* hist) {
   if (!ctx || !p || !theta) return fail(ALIEN_HIP_EINVAL, "gnn_train: NULL argument");
   if (p->steps < 0 || p->steps_per_system < 1 || p->n_min < 2 || p->n_max < p->n_min || p->n_max > 128 ||
-      !(p->sigma_min >= 0) || !(p->sigma_max >= p->sigma_min) || !(p->aniso_max >= 1) || !(p->lr > 0))
+      !(p->sigma_min >= 0) || !(p->sigma_max >= p->sigma_min) || !(p->aniso_max >= 1) || !(p->lr > 0) ||
+      p->cg_steps < 0 || p->cg_steps > gnn::kMaxCgSteps)
     return fail(ALIEN_HIP_EINVAL, "gnn_train: bad parameters");
   try {
     hipStream_t s = ctx->stream;
@@ -136,7 +138,7 @@ int alien_hip_gnn_train(alien_hip_ctx* ctx, const alien_hip_gnn_train_params* p,
       for (int t = 0; t < p->steps_per_system && step < p->steps; ++t, ++step) {
         for (auto& v : probes) v = (splitmix64_next(rs) >> 63) ? 1.f : -1.f; // Rademacher
         V.upload(probes.data(), probes.size(), s);
-        const double L = E.loss_grad(P.data(), V.data(), 4, (float)p->beta, G.data(), nullptr);
+        const double L = E.loss_grad(P.data(), V.data(), 4, (float)p->beta, p->cg_steps, G.data(), nullptr);
         if (hist) hist[step] = L;
         gnn::launch_adam(s, gnn::kNParam, P.data(), G.data(), M.data(), Vv.data(), (float)p->lr, step + 1);
         check_launch("gnn adam");

@@ -27,6 +27,7 @@ constexpr int kOffCe = kOffWb + kHidden;
 constexpr int kNParam = kOffCe + 1; // 6498
 constexpr int kHeadGrads = kNParam - kOffWd; // w_d, c_d, w_a, w_b, c_e = 98
 constexpr int kMaxProbes = 8;
+constexpr int kMaxCgSteps = 200;
 
 } // namespace gnn
 } // namespace alien_hip
@@ -46,8 +47,10 @@ class Engine {
   int nnz() const { return m_nnz; }
   // activations and Ghat (per CSR entry; upper entries 0) for the parameters P (device)
   void forward(const float* P);
-  // loss = Kaporin term + beta * probe term (4 probes V, device, N x 4); grad (device, kNParam)
-  double loss_grad(const float* P, const float* V, int K, float beta, float* grad, double* parts);
+  // loss = L1 + beta * probe term on 4 probes V (device, N x 4); L1 = Kaporin's condition number
+  // (cg_steps = 0) or the residual reduction after cg_steps PCG steps with V as right-hand sides;
+  // grad (device, kNParam); parts = {L1, probe term}
+  double loss_grad(const float* P, const float* V, int K, float beta, int cg_steps, float* grad, double* parts);
   const float* ghat() const { return d_ghat.data(); }
   // Row-bundled layouts (RB-DIA or RB-CSR) of G and G^T and their values (after forward)
   void build_factor_layout();
@@ -59,6 +62,7 @@ class Engine {
   template <typename T> RBMat<T> gt_mat(const T* v) const { return m_gt.mat(v); }
 
  private:
+  double cg_term(const float* B, int k, float* gg);
   alien_hip_ctx* m_ctx;
   int m_n, m_nnz = 0;
   std::vector<int> h_ro, h_cols, h_dcol, m_gbo, m_gtbo;

@@ -14,6 +14,7 @@
 // order: the results are deterministic.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <vector>
 
@@ -257,19 +258,23 @@ __global__ void __launch_bounds__(256) k_probe_resid(int n, const float* __restr
 }
 
 // ---- dL/dGhat -> dL/d(head inputs) --------------------------------------------------------------
-// g_ij (per lower CSR entry) = 2 AG_ij / tr - (2/N) [i==j] / Ghat_ii  + beta (sum_c dY_ic U_jc + W_ic Q_jc)
+// g_ij (per lower CSR entry) = kw (2 AG_ij / tr - (2/N) [i==j] / Ghat_ii) + beta (sum_c dY_ic U_jc + W_ic Q_jc)
 template <int K>
 __global__ void k_grad_ghat(int n, const int* __restrict__ ro, const int* __restrict__ cols, const int* __restrict__ dcol,
                             const float* __restrict__ ghat, const float* __restrict__ ag, const double* __restrict__ sums,
-                            float beta, const float* __restrict__ dY, const float* __restrict__ U,
+                            float kw, float beta, const float* __restrict__ dY, const float* __restrict__ U,
                             const float* __restrict__ Wp, const float* __restrict__ Q, float* __restrict__ gg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float tr = (float)sums[0];
+  const float tr = kw != 0.f ? (float)sums[0] : 1.f;
   for (int k = ro[i]; k <= dcol[i]; ++k) {
     const int j = cols[k];
-    float g = 2.f * ag[k] / tr;
-    if (k == dcol[i]) g -= 2.f / ((float)n * ghat[k]);
+    float g = 0.f;
+    if (kw != 0.f) {
+      g = 2.f * ag[k] / tr;
+      if (k == dcol[i]) g -= 2.f / ((float)n * ghat[k]);
+      g *= kw;
+    }
     if (beta != 0.f) {
       float p = 0.f;
 #pragma unroll
@@ -277,6 +282,55 @@ __global__ void k_grad_ghat(int n, const int* __restrict__ ro, const int* __rest
       g += beta * p;
     }
     gg[k] = g;
+  }
+}
+
+// ---- k-step CG loss (N x 4 column blocks, one right-hand side per column) ----
+struct Coef4 {
+  float c[4];
+};
+// per-column dot products X_c . Y_c into 4 partial slots
+__global__ void __launch_bounds__(256) k_dot4(int n, const float* __restrict__ X, const float* __restrict__ Y,
+                                              RedWork w) {
+  __shared__ double lds[4];
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = (double)X[(long)i * 4 + c] * (double)Y[(long)i * 4 + c];
+  }
+  double v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    v[c] = block_sum(a[c], lds);
+    __syncthreads();
+  }
+  store_partials<4>(v, w);
+}
+// Y_c = a_c X_c + b_c Z_c (Z may be NULL; Y may alias X or Z)
+__global__ void k_lin4(int n, Coef4 a, const float* X, Coef4 b, const float* Z, float* Y) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n * 4) return;
+  const int c = (int)(t & 3);
+  float v = a.c[c] * X[t];
+  if (Z) v += b.c[c] * Z[t];
+  Y[t] = v;
+}
+// dL/dGhat_il += sum_c U_ic V_lc (+ W_ic X_lc) over the lower pattern (l <= i)
+__global__ void k_edge_acc4(int n, const int* __restrict__ ro, const int* __restrict__ cols,
+                            const int* __restrict__ dcol, const float* __restrict__ U, const float* __restrict__ V,
+                            const float* __restrict__ W, const float* __restrict__ X, float* __restrict__ gg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = ro[i]; k <= dcol[i]; ++k) {
+    const int l = cols[k];
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      s += U[(long)i * 4 + c] * V[(long)l * 4 + c];
+      if (W) s += W[(long)i * 4 + c] * X[(long)l * 4 + c];
+    }
+    gg[k] += s;
   }
 }
 
@@ -572,20 +626,25 @@ void Engine::forward(const float* P) {
   check_launch("gnn forward");
 }
 
-double Engine::loss_grad(const float* P, const float* V, int K, float beta, float* grad, double* parts) {
+double Engine::loss_grad(const float* P, const float* V, int K, float beta, int cg_steps, float* grad,
+                         double* parts) {
   if (K != 4) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: 4 probe vectors are supported");
+  if (cg_steps < 0 || cg_steps > kMaxCgSteps) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: bad cg_steps");
   hipStream_t s = m_ctx->stream;
   const int n = m_n;
   forward(P);
-  // ---- Kaporin term
+  // ---- Kaporin term (cg_steps == 0)
   m_ctx->reserve_for(n);
   DeviceArray<float> ag(m_nnz), gg(m_nnz);
   DeviceArray<double> sums(4);
   const long nb = (n + 255) / 256;
   RedWork w = m_ctx->red();
-  hipLaunchKernelGGL(k_kcond, dim3((unsigned)nb), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
-                     d_e.data(), d_ghat.data(), ag.data(), w);
-  launch_finalize(s, w, nb, 2, kFinStore, sums.data(), nullptr);
+  const bool kap = cg_steps == 0;
+  if (kap) {
+    hipLaunchKernelGGL(k_kcond, dim3((unsigned)nb), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+                       d_e.data(), d_ghat.data(), ag.data(), w);
+    launch_finalize(s, w, nb, 2, kFinStore, sums.data(), nullptr);
+  }
   // ---- probe term
   constexpr int KP = 4;
   DeviceArray<float> W_((size_t)n * KP), U((size_t)n * KP), Y((size_t)n * KP), dY((size_t)n * KP), Q((size_t)n * KP);
@@ -600,7 +659,11 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, floa
   hipLaunchKernelGGL((k_ghat_mult<KP, true>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
                      d_tpos.data(), d_ghat.data(), dY.data(), Q.data());
   hipLaunchKernelGGL((k_grad_ghat<KP>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
-                     d_ghat.data(), ag.data(), sums.data(), beta, dY.data(), U.data(), W_.data(), Q.data(), gg.data());
+                     d_ghat.data(), ag.data(), sums.data(), kap ? 1.f : 0.f, beta, dY.data(), U.data(), W_.data(),
+                     Q.data(), gg.data());
+  // ---- CG term (cg_steps > 0): k PCG steps on Ahat with Ghat Ghat^T, reverse pass into gg
+  double lcg = 0.0;
+  if (!kap) lcg = cg_term(V, cg_steps, gg.data());
   // ---- head backward
   DeviceArray<float> dh[2], dz((size_t)n * kHidden), dmsg((size_t)n * kHidden), prow((size_t)n * kHeadGrads);
   dh[0].resize((size_t)n * kHidden);
@@ -637,13 +700,149 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, floa
   double hs[4];
   ALIEN_HIP_CHECK(hipMemcpyAsync(hs, sums.data(), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
   ALIEN_HIP_CHECK(hipStreamSynchronize(s));
-  const double lk = std::log(hs[0] / n) - 2.0 * hs[1] / n;
+  const double lk = kap ? std::log(hs[0] / n) - 2.0 * hs[1] / n : lcg;
   const double lp = hs[2] / ((double)n * KP);
   if (parts) {
     parts[0] = lk;
     parts[1] = lp;
   }
   return lk + beta * lp;
+}
+
+// Residual reduction after k PCG steps on Ahat with M = Ghat Ghat^T, x0 = 0, the
+// 4 columns of B as right-hand sides: L = mean_c log(|r_k|^2 / |r_0|^2), rho =
+// |Ghat^T r|^2. Forward keeps r_j, t_j = Ghat^T r_j, p_j, q_j = Ahat p_j; the
+// reverse pass (the adjoint of each step, last to first) accumulates dL/dGhat
+// on the lower pattern into gg. Step scalars live on the host (one dot4
+// readback each), vectors in FP32, dots in FP64.
+double Engine::cg_term(const float* B, int k, float* gg) {
+  hipStream_t s = m_ctx->stream;
+  const int n = m_n;
+  const size_t nv = (size_t)n * 4;
+  const dim3 gv = g1((long)nv), gr = g1(n);
+  RedWork w = m_ctx->red();
+  const long nb = (n + 255) / 256;
+  DeviceArray<double> dsum(4);
+  auto dot4 = [&](const float* X, const float* Y) {
+    std::array<double, 4> h{};
+    hipLaunchKernelGGL(k_dot4, dim3((unsigned)nb), dim3(256), 0, s, n, X, Y, w);
+    launch_finalize(s, w, nb, 4, kFinStore, dsum.data(), nullptr);
+    ALIEN_HIP_CHECK(hipMemcpyAsync(h.data(), dsum.data(), sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+    return h;
+  };
+  auto lin = [&](std::array<double, 4> a, const float* X, std::array<double, 4> b, const float* Z, float* Y) {
+    Coef4 ca, cb;
+    for (int c = 0; c < 4; ++c) {
+      ca.c[c] = (float)a[c];
+      cb.c[c] = (float)b[c];
+    }
+    hipLaunchKernelGGL(k_lin4, gv, dim3(256), 0, s, n, ca, X, cb, Z, Y);
+  };
+  auto amul = [&](const float* X, float* Y) {
+    hipLaunchKernelGGL((k_ahat_mult<4>), gr, dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_e.data(), X, Y);
+  };
+  auto gmul = [&](const float* X, float* Y) { // Y = Ghat X
+    hipLaunchKernelGGL((k_ghat_mult<4, false>), gr, dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+                       d_tpos.data(), d_ghat.data(), X, Y);
+  };
+  auto gtmul = [&](const float* X, float* Y) { // Y = Ghat^T X
+    hipLaunchKernelGGL((k_ghat_mult<4, true>), gr, dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+                       d_tpos.data(), d_ghat.data(), X, Y);
+  };
+  auto acc = [&](const float* U, const float* V, const float* W, const float* X) {
+    hipLaunchKernelGGL(k_edge_acc4, gr, dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(), U, V, W, X,
+                       gg);
+  };
+  typedef std::array<double, 4> S4;
+  const S4 one{1, 1, 1, 1}, zero{0, 0, 0, 0};
+  std::vector<DeviceArray<float>> R(k + 1), T(k), Pv(k), Q(k);
+  for (auto& a : R) a.resize(nv);
+  for (auto& a : T) a.resize(nv);
+  for (auto& a : Pv) a.resize(nv);
+  for (auto& a : Q) a.resize(nv);
+  DeviceArray<float> z(nv), rbar(nv), pbar(nv), pbar2(nv), qbar(nv), tbar(nv), tmp(nv);
+  std::vector<S4> rho(k), sig(k), alpha(k), beta(k);
+  // forward
+  ALIEN_HIP_CHECK(hipMemcpyAsync(R[0].data(), B, sizeof(float) * nv, hipMemcpyDeviceToDevice, s));
+  gtmul(R[0].data(), T[0].data());
+  gmul(T[0].data(), Pv[0].data());
+  rho[0] = dot4(T[0].data(), T[0].data());
+  const S4 rr0 = dot4(R[0].data(), R[0].data());
+  for (int j = 0; j < k; ++j) {
+    amul(Pv[j].data(), Q[j].data());
+    sig[j] = dot4(Pv[j].data(), Q[j].data());
+    S4 na;
+    for (int c = 0; c < 4; ++c) {
+      alpha[j][c] = rho[j][c] / sig[j][c];
+      na[c] = -alpha[j][c];
+    }
+    lin(one, R[j].data(), na, Q[j].data(), R[j + 1].data());
+    if (j < k - 1) {
+      gtmul(R[j + 1].data(), T[j + 1].data());
+      gmul(T[j + 1].data(), z.data());
+      rho[j + 1] = dot4(T[j + 1].data(), T[j + 1].data());
+      for (int c = 0; c < 4; ++c) beta[j][c] = rho[j + 1][c] / rho[j][c];
+      lin(one, z.data(), beta[j], Pv[j].data(), Pv[j + 1].data());
+    }
+  }
+  const S4 rrk = dot4(R[k].data(), R[k].data());
+  double L = 0.0;
+  S4 c0;
+  for (int c = 0; c < 4; ++c) {
+    L += std::log(rrk[c] / rr0[c]) / 4.0;
+    c0[c] = 2.0 / (4.0 * rrk[c]);
+  }
+  if (!std::isfinite(L)) return L;
+  // reverse
+  lin(c0, R[k].data(), zero, nullptr, rbar.data());
+  S4 rhobar_next = zero;
+  for (int j = k - 1; j >= 0; --j) {
+    S4 rhobar_j = zero;
+    if (j < k - 1) {
+      // p_{j+1} = z_{j+1} + beta_j p_j ; zbar = pbar_{j+1} (held in pbar)
+      const S4 betabar = dot4(pbar.data(), Pv[j].data());
+      S4 rhobar_jp1, two_rho;
+      for (int c = 0; c < 4; ++c) {
+        rhobar_jp1[c] = rhobar_next[c] + betabar[c] / rho[j][c];
+        rhobar_j[c] -= betabar[c] * rho[j + 1][c] / (rho[j][c] * rho[j][c]);
+        two_rho[c] = 2.0 * rhobar_jp1[c];
+      }
+      // z = Ghat t: dGhat += zbar t^T ; tbar = 2 rhobar t + Ghat^T zbar
+      gtmul(pbar.data(), tmp.data());
+      lin(two_rho, T[j + 1].data(), one, tmp.data(), tbar.data());
+      // t = Ghat^T r: dGhat += r tbar^T ; rbar += Ghat tbar
+      acc(pbar.data(), T[j + 1].data(), R[j + 1].data(), tbar.data());
+      gmul(tbar.data(), tmp.data());
+      lin(one, rbar.data(), one, tmp.data(), rbar.data());
+      lin(beta[j], pbar.data(), zero, nullptr, pbar2.data()); // pbar_j = beta_j pbar_{j+1}
+    } else {
+      ALIEN_HIP_CHECK(hipMemsetAsync(pbar2.data(), 0, sizeof(float) * nv, s)); // p_k is not formed
+    }
+    // r_{j+1} = r_j - alpha q_j
+    const S4 qr = dot4(Q[j].data(), rbar.data());
+    S4 na, sigbar;
+    for (int c = 0; c < 4; ++c) {
+      const double alphabar = -qr[c];
+      rhobar_j[c] += alphabar / sig[j][c];
+      sigbar[c] = -alphabar * rho[j][c] / (sig[j][c] * sig[j][c]);
+      na[c] = -alpha[j][c];
+    }
+    lin(na, rbar.data(), sigbar, Pv[j].data(), qbar.data()); // qbar = -alpha rbar + sigbar p
+    lin(one, pbar2.data(), sigbar, Q[j].data(), pbar2.data()); // pbar += sigbar q
+    amul(qbar.data(), tmp.data());
+    lin(one, pbar2.data(), one, tmp.data(), pbar.data()); // pbar_j = ... + Ahat qbar
+    rhobar_next = rhobar_j;
+  }
+  // p_0 = Ghat t_0, rho_0 = t_0.t_0, t_0 = Ghat^T r_0
+  S4 two_rho;
+  for (int c = 0; c < 4; ++c) two_rho[c] = 2.0 * rhobar_next[c];
+  gtmul(pbar.data(), tmp.data());
+  lin(two_rho, T[0].data(), one, tmp.data(), tbar.data());
+  acc(pbar.data(), T[0].data(), R[0].data(), tbar.data());
+  check_launch("gnn cg loss");
+  ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+  return L;
 }
 
 // Row-bundled layouts of G (rows = lower + diagonal of A) and of G^T (rows =

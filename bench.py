@@ -138,13 +138,15 @@ def cpu_baseline(aa, ro, cols, vals, b, fp64_iters, budget_s):
 
 
 def train_network(aa, ctx, a):
-    """Adam on generated log-normal systems (GPU forward/backward kernels), then model selection
+    """Adam on generated log-normal systems (GPU forward/backward kernels, loss = residual reduction
+    after k PCG steps + probe term), then model selection
     between the initial and the trained parameters by FP64 learned-PCG iterations on a held-out
     48^3 system (sigma 2, seed not used in training)."""
     th0 = aa.gnn_init(0)
     t0 = time.time()
     th1, hist = aa.gnn_train(ctx, th0, steps=a.train_steps, steps_per_system=4, n_min=16, n_max=32,
-                             sigma_min=0.5, sigma_max=3.0, aniso_max=4.0, lr=1e-3, beta=0.1, seed=7)
+                             sigma_min=0.5, sigma_max=3.0, aniso_max=4.0, lr=1e-3, beta=0.1, seed=7,
+                             cg_steps=a.train_cg_steps)
     t_train = time.time() - t0
     hro, hcols, hvals = aa.gen_csr(aa.GEN_LOGNORMAL7, 48, params=[2.0, 1.0, 1.0, 1.0], seed=99991)
     hb = aa.uniform01(5, len(hro) - 1)
@@ -161,7 +163,8 @@ def train_network(aa, ctx, a):
         f"-> {chosen}")
     info = {"steps": a.train_steps, "time_s": t_train, "loss_first8": float(hist[:8].mean()),
             "loss_last8": float(hist[-8:].mean()), "heldout_48cubed_iterations": dict(its, jacobi=rj["iterations"]),
-            "chosen": chosen,
+            "chosen": chosen, "loss": (f"log residual reduction after {a.train_cg_steps} PCG steps" if a.train_cg_steps
+                                        else "Kaporin condition number") + " + 0.1 x |I - M A| probe term",
             "systems": "log-normal 16^3..32^3, sigma U[0.5,3], anisotropy log-U[1,4], 4 Adam steps per system"}
     return (th1 if chosen == "trained" else th0), info
 
@@ -173,6 +176,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256, help="grid edge (256 -> the 256^3 workload)")
     ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--train-cg-steps", type=int, default=16,
+                    help="training loss: residual reduction after this many PCG steps (0: Kaporin's bound)")
     ap.add_argument("--no-slow-baselines", action="store_true",
                     help="skip the FP64 Chebyshev-CG and multicolour ILU0-CG baselines")
     ap.add_argument("--solvers", default="",

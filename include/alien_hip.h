@@ -282,11 +282,14 @@ int alien_hip_gnn_init(uint64_t seed, float* theta);
 /* Ghat per CSR entry of A (strictly-upper entries 0), G = D^-1/2 Ghat */
 int alien_hip_gnn_forward(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
                           const double* vals, const float* theta, float* ghat);
-/* loss = Kaporin term + beta * probe term on 4 probe vectors V (nrows x 4, row-major);
- * parts[0] = Kaporin term, parts[1] = probe term; grad has nparams entries */
+/* loss = L1 + beta * probe term |V - Ghat Ghat^T Ahat V|^2 / (4 nrows) on 4 probe vectors V
+ * (nrows x 4, row-major). L1 = the residual reduction after cg_steps preconditioned CG steps,
+ * mean over the 4 columns of V as right-hand sides of log(|r_k|^2 / |r_0|^2) (cg_steps >= 1), or
+ * Kaporin's condition number log(tr(Ghat^T Ahat Ghat)/N) - (2/N) sum log Ghat_ii (cg_steps = 0).
+ * parts[0] = L1, parts[1] = probe term; grad has nparams entries */
 int alien_hip_gnn_loss_grad(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
                             const double* vals, const float* theta, const float* probes, double beta,
-                            double* loss, double* parts, float* grad);
+                            int cg_steps, double* loss, double* parts, float* grad);
 typedef struct {
   int      steps;            /* Adam steps */
   int      steps_per_system; /* steps on one generated system before drawing the next */
@@ -296,6 +299,8 @@ typedef struct {
   double   lr;               /* Adam learning rate */
   double   beta;             /* weight of the probe term */
   uint64_t seed;
+  int      cg_steps;         /* L1 of alien_hip_gnn_loss_grad: CG steps, 0 = Kaporin */
+  int      pad_;
 } alien_hip_gnn_train_params;
 /* trains theta in place on the GPU; loss_hist (may be NULL) gets one value per step */
 int alien_hip_gnn_train(alien_hip_ctx* ctx, const alien_hip_gnn_train_params* p, float* theta,

@@ -223,3 +223,98 @@ def total_loss_and_grad(g: Graph, theta, V, beta=0.1, bf=False):
     dgd = dK[0] + beta * dP[0]
     dge = dK[1] + beta * dP[1]
     return LK + beta * LP, backward(g, theta, cache, dgd, dge, bf), Ghat
+
+
+def cg_loss(g: Graph, Ghat, B, k):
+    """Residual reduction after k preconditioned CG steps: CG on Ahat with
+    M = Ghat Ghat^T, x0 = 0, one right-hand side per column of B;
+    L = mean_c log(|r_k,c|^2 / |r_0,c|^2). rho = (Ghat^T r).(Ghat^T r) = r.M r."""
+    A, G = g.Ahat, Ghat.tocsr()
+    GT = G.T.tocsr()
+    K = B.shape[1]
+    r = [B.copy()]
+    t = [GT @ B]
+    z = G @ t[0]
+    p = [z]
+    rho = [np.sum(t[0] * t[0], axis=0)]
+    q, sig, alpha, beta = [], [], [], []
+    for j in range(k):
+        qj = A @ p[j]
+        q.append(qj)
+        sig.append(np.sum(p[j] * qj, axis=0))
+        alpha.append(rho[j] / sig[j])
+        r.append(r[j] - alpha[j] * qj)
+        if j < k - 1:
+            tj = GT @ r[j + 1]
+            t.append(tj)
+            zj = G @ tj
+            rho.append(np.sum(tj * tj, axis=0))
+            beta.append(rho[j + 1] / rho[j])
+            p.append(zj + beta[j] * p[j])
+    rr0 = np.sum(B * B, axis=0)
+    rrk = np.sum(r[k] * r[k], axis=0)
+    L = float(np.mean(np.log(rrk / rr0)))
+    return L, dict(r=r, t=t, p=p, q=q, rho=rho, sig=sig, alpha=alpha, beta=beta, rrk=rrk, K=K, k=k)
+
+
+def grad_cg_loss(g: Graph, Ghat, aux):
+    """dL/dGhat of cg_loss on the lower pattern (diagonal, strictly-lower edges),
+    by reverse-mode differentiation of the k CG steps."""
+    A, G = g.Ahat, Ghat.tocsr()
+    GT = G.T.tocsr()
+    r, t, p, q = aux["r"], aux["t"], aux["p"], aux["q"]
+    rho, sig, alpha, beta = aux["rho"], aux["sig"], aux["alpha"], aux["beta"]
+    K, k = aux["K"], aux["k"]
+    n = g.n
+    gd = np.zeros(n)
+    ge = np.zeros(len(g.li))
+
+    def acc(U, V):  # dGhat_il += sum_c U_ic V_lc on the lower pattern
+        gd[:] += np.sum(U * V, axis=1)
+        ge[:] += np.sum(U[g.li] * V[g.lj], axis=1)
+
+    rbar = 2.0 * r[k] / (K * aux["rrk"])
+    pbar_next = None       # dL/dp_{j+1}
+    rhobar_next = None     # dL/drho_{j+1} accumulated so far
+    for j in range(k - 1, -1, -1):
+        rhobar_j = np.zeros(K)
+        if j < k - 1:
+            zbar = pbar_next
+            betabar = np.sum(pbar_next * p[j], axis=0)
+            pbar = beta[j] * pbar_next
+            rhobar_jp1 = rhobar_next + betabar / rho[j]
+            rhobar_j += -betabar * rho[j + 1] / rho[j] ** 2
+            tbar = 2.0 * rhobar_jp1 * t[j + 1]
+            acc(zbar, t[j + 1])
+            tbar = tbar + GT @ zbar
+            acc(r[j + 1], tbar)
+            rbar = rbar + G @ tbar
+        else:
+            pbar = np.zeros_like(p[j])
+        alphabar = -np.sum(q[j] * rbar, axis=0)
+        qbar = -alpha[j] * rbar
+        rhobar_j += alphabar / sig[j]
+        sigbar = -alphabar * rho[j] / sig[j] ** 2
+        pbar = pbar + sigbar * q[j]
+        qbar = qbar + sigbar * p[j]
+        pbar = pbar + A @ qbar
+        pbar_next, rhobar_next = pbar, rhobar_j
+    # p_0 = z_0 = Ghat t_0, rho_0 = t_0.t_0, t_0 = Ghat^T r_0 (r_0 = B fixed)
+    zbar = pbar_next
+    tbar = 2.0 * rhobar_next * t[0]
+    acc(zbar, t[0])
+    tbar = tbar + GT @ zbar
+    acc(r[0], tbar)
+    return gd, ge
+
+
+def cg_total_loss_and_grad(g: Graph, theta, B, k=8, beta=0.1, bf=False):
+    """Training objective with the CG loss: cg_loss(k) + beta * probe term, and its gradient."""
+    Ghat, cache = forward(g, theta, bf)
+    LC, auxC = cg_loss(g, Ghat, B, k)
+    LP, auxP = loss(g, Ghat, B)
+    dC = grad_cg_loss(g, Ghat, auxC)
+    dP = grad_ghat(g, Ghat, B, auxP)
+    dgd = dC[0] + beta * dP[0]
+    dge = dC[1] + beta * dP[1]
+    return LC + beta * LP, backward(g, theta, cache, dgd, dge, bf), Ghat

@@ -1,7 +1,8 @@
 """Pins the FP64 CPU oracle of the learned preconditioner (oracle/gnn_oracle.py):
 its analytic gradient against central finite differences, the SPD property of
 M = G G^T, the initial factor G = D^-1/2 (I + E_lower), and the bf16 rounding
-against numpy's own float32 round-to-nearest-even on exact cases."""
+against numpy's own float32 round-to-nearest-even on exact cases; the k-step
+CG residual loss against finite differences, an exact factor and plain CG."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -69,3 +70,46 @@ def test_bf16_gradient_consistent_with_fp64():
     assert abs(L16 - L64) <= 2e-2 * L64
     cos = g64 @ g16 / (np.linalg.norm(g64) * np.linalg.norm(g16))
     assert cos > 0.99
+
+
+def test_cg_loss_gradient_matches_finite_differences():
+    """The reverse-mode gradient of the k-step CG residual loss agrees with
+    central differences on the parameters (FP64, no bf16 rounding)."""
+    g = small_graph()
+    rng = np.random.default_rng(3)
+    th = perturbed_theta(1)
+    B = rng.choice([-1.0, 1.0], size=(g.n, 4))
+    L, grad, _ = go.cg_total_loss_and_grad(g, th, B, k=5, beta=0.1)
+    idx = rng.choice(go.NPARAM, 40, replace=False)
+    idx = np.union1d(idx, np.arange(go.NPARAM - 98, go.NPARAM, 7))  # heads
+    h = 1e-6
+    for i in idx:
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (go.cg_total_loss_and_grad(g, tp, B, k=5, beta=0.1)[0] -
+              go.cg_total_loss_and_grad(g, tm, B, k=5, beta=0.1)[0]) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-5 * max(1.0, abs(fd)), (i, fd, grad[i])
+
+
+def test_cg_loss_measures_convergence():
+    """With M = A^-1 (exact factor) one CG step solves every system; with the
+    identity the k-step loss equals plain CG's residual reduction."""
+    g = small_graph()
+    rng = np.random.default_rng(4)
+    B = rng.standard_normal((g.n, 2))
+    # exact: Ghat = chol(Ahat^-1) lower-triangular (dense, small system)
+    Ai = np.linalg.inv(g.Ahat.toarray())
+    C = np.linalg.cholesky(Ai)
+    L1, _ = go.cg_loss(g, sp.csr_matrix(C), B, 1)
+    assert L1 < -20
+    # identity: compare with a plain CG loop
+    Li, _ = go.cg_loss(g, sp.identity(g.n, format="csr"), B, 3)
+    A = g.Ahat.toarray()
+    ref = []
+    for c in range(2):
+        x = np.zeros(g.n); r = B[:, c].copy(); p = r.copy(); rr = r @ r
+        for _ in range(3):
+            q = A @ p; a = rr / (p @ q); x += a * p; r -= a * q; rn = r @ r; p = r + (rn / rr) * p; rr = rn
+        ref.append(np.log(rr / (B[:, c] @ B[:, c])))
+    assert abs(Li - np.mean(ref)) < 1e-10

@@ -97,6 +97,49 @@ def test_loss_and_gradient_match_oracle(ctx, beta):
         o += k
 
 
+@pytest.mark.parametrize("k", [1, 5])
+def test_cg_loss_and_gradient_match_oracle(ctx, k):
+    """k-step PCG residual-reduction loss: FP32 vectors / FP64 dots on the GPU
+    against the FP64 oracle (same bf16 rounding of the network); bounds 1e-4 on
+    the loss and 2e-3 on the gradient norm (FP32 recurrences over k steps)."""
+    ro, cols, vals = system(10, seed=7)
+    th = active_theta(6)
+    rng = np.random.default_rng(12)
+    V = rng.choice([-1.0, 1.0], size=(len(ro) - 1, 4)).astype(np.float32)
+    say("gnn cg loss/grad k", k)
+    L, (lc, lp), grad = aa.gnn_loss_grad(ctx, ro, cols, vals, th, V, beta=0.1, cg_steps=k)
+    g = go.Graph(ro, cols, vals)
+    Lo, go_grad, Gh = go.cg_total_loss_and_grad(g, th, V.astype(np.float64), k=k, beta=0.1, bf=True)
+    lco, _ = go.cg_loss(g, Gh, V.astype(np.float64), k)
+    say("  loss", L, Lo, "cg part", lc, lco)
+    assert L == pytest.approx(Lo, rel=1e-4, abs=1e-5)
+    assert lc == pytest.approx(lco, rel=1e-4, abs=1e-5)
+    rel = np.linalg.norm(grad - go_grad) / np.linalg.norm(go_grad)
+    say("  grad rel err", rel)
+    assert rel < 2e-3
+    o = 0
+    for name, shape in go.LAYOUT:
+        n_ = int(np.prod(shape))
+        gb, ob = grad[o:o + n_], go_grad[o:o + n_]
+        assert np.linalg.norm(gb - ob) <= 2e-2 * np.linalg.norm(ob) + 1e-6, name
+        o += n_
+
+
+def test_cg_loss_training_lowers_the_objective(ctx):
+    ro, cols, vals = system(10, sigma=1.0, seed=21)
+    rng = np.random.default_rng(2)
+    V = rng.choice([-1.0, 1.0], size=(len(ro) - 1, 4)).astype(np.float32)
+    th0 = go.init_params(0)
+    say("gnn train with the CG loss, 40 steps")
+    th1, hist = aa.gnn_train(ctx, th0, steps=40, steps_per_system=4, n_min=8, n_max=12, sigma_min=0.5,
+                             sigma_max=1.5, aniso_max=2.0, lr=1e-3, beta=0.1, seed=4, cg_steps=8)
+    assert np.all(np.isfinite(hist)) and np.all(np.isfinite(th1))
+    L0, _, _ = aa.gnn_loss_grad(ctx, ro, cols, vals, th0, V, beta=0.1, cg_steps=8)
+    L1, _, _ = aa.gnn_loss_grad(ctx, ro, cols, vals, th1, V, beta=0.1, cg_steps=8)
+    say("  held-out objective", L0, "->", L1)
+    assert L1 < L0
+
+
 def test_training_lowers_the_objective(ctx):
     ro, cols, vals = system(10, sigma=1.0, seed=21)
     rng = np.random.default_rng(2)
