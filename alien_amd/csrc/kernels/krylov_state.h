@@ -35,6 +35,46 @@ struct CGState {
   int pad[3];
 };
 
+// Scalars of one BiCGStab solve (BiCGStab.h:151-300), device-resident like CGState.
+struct BiCGState {
+  double rho;    // rho of the previous step
+  double rho1;   // r.r0 of the current step
+  double alpha;  // rho1 / (v.r0)
+  double omega;  // (t.s) / (t.t)
+  double rr;     // last (r,r) or (s,s) of a stop test (Iteration's m_value)
+  double crit;   // tol^2 (b,b)
+  double rr0;    // (r,r) of the initial residual (first-block breakdown rule)
+  int it, max_it;
+  int done;      // stop test passed, max_it reached or breakdown
+  int status;    // Iteration::getStatus()
+  int breakdown; // 0, 1 = v.r0 == 0 (alpha), 2 = t.t == 0 (beta), 3 = rho == 0
+  int half;      // stopped on s: x += alpha phat is still to be applied
+  int first;     // the first block (before the while loop), counted differently (BiCGStab.h:194-199)
+  int pad;
+};
+
+// BiCGStab step kernels (bicgstab.hip). DIAG: Jacobi fused (phat = invD p, shat = invD s).
+template <typename T>
+void launch_bicg_p(hipStream_t, int n, const T* r, const T* v, T* p, const T* invd_or_null, T* phat,
+                   BiCGState* st);
+template <typename T>
+void launch_bicg_v(hipStream_t, RedWork w, int n, RBMat<T> A, const T* phat, const T* r0, T* v, BiCGState* st);
+template <typename T>
+void launch_bicg_s(hipStream_t, RedWork w, int n, const T* r, const T* v, T* s, const T* invd_or_null, T* shat,
+                   BiCGState* st);
+template <typename T>
+void launch_bicg_t(hipStream_t, RedWork w, int n, RBMat<T> A, const T* shat, const T* s, T* t, BiCGState* st);
+template <typename T>
+void launch_bicg_xr(hipStream_t, RedWork w, int n, const T* shat, const T* phat, const T* s, const T* t,
+                    const T* r0, T* x, T* r, BiCGState* st);
+// x += alpha phat when the solve stopped on s (half step)
+template <typename T>
+void launch_bicg_half(hipStream_t, int n, const T* phat, T* x, const BiCGState* st);
+// y = M x with M = G G^T (tmp = G^T x), skipped once done is set
+template <typename T>
+void launch_factor_apply_flag(hipStream_t, int n, RBMat<T> G, RBMat<T> GT, const T* x, T* tmp, T* y,
+                              const int* done);
+
 template <typename T>
 void launch_residual(hipStream_t, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out);
 template <typename T>

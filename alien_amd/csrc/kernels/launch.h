@@ -15,6 +15,9 @@ struct RedWork;
 // what the second stage of a reduction does with the totals (reduce.hip)
 enum FinOp { kFinStore = 0, kFinRho1 = 1, kFinPQ = 2, kFinXR = 3, kFinXRDiag = 4 };
 void launch_finalize(hipStream_t, RedWork w, long nblk, int nslot, int op, double* out, CGState* st);
+// the same second stage for the BiCGStab state (reduce.hip)
+enum BFinOp { kBFinAlpha = 0, kBFinS = 1, kBFinOmega = 2, kBFinXR = 3 };
+void launch_finalize_bicg(hipStream_t, RedWork w, long nblk, int nslot, int op, BiCGState* st);
 
 // ---- BLAS-1 (blas1.hip) ----
 template <typename T> void launch_copy(hipStream_t, int n, const T* x, T* y);

@@ -14,14 +14,14 @@ long g_row_blocks_f32 = 4096;
 
 constexpr int kFin = 1024;
 
-__global__ void __launch_bounds__(kFin) k_finalize(RedWork w, int nblk, int nslot, int op, double* out,
-                                                    CGState* st) {
-  if (op != kFinStore && st->done) return; // the producing kernel returned early too
+// Sums of nslot partial arrays (nblk entries each) by one workgroup of kFin
+// lanes: lane t takes partials t, t + kFin, ... in order (8 loads issued before
+// they are added), then a fixed tree. The totals are valid on lane 0 only,
+// which is the lane that returns true.
+__device__ bool finalize_totals(RedWork w, int nblk, int nslot, double (&tot)[kMaxSlots]) {
   __shared__ double lds[kMaxSlots][kFin / kWave];
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wid = tid / kWave;
-  // lane t sums partials t, t + kFin, t + 2 kFin, ... (fixed order); 8 loads
-  // are issued before they are added
   constexpr int U = 8;
   for (int s = 0; s < nslot; ++s) {
     const double* p = w.partial + (long)s * w.cap;
@@ -39,13 +39,20 @@ __global__ void __launch_bounds__(kFin) k_finalize(RedWork w, int nblk, int nslo
     if (lane == 0) lds[s][wid] = acc;
   }
   __syncthreads();
-  if (tid != 0) return;
-  double tot[kMaxSlots];
+  if (tid != 0) return false;
   for (int s = 0; s < nslot; ++s) {
     double t = 0.0;
     for (int q = 0; q < kFin / kWave; ++q) t += lds[s][q];
     tot[s] = t;
   }
+  return true;
+}
+
+__global__ void __launch_bounds__(kFin) k_finalize(RedWork w, int nblk, int nslot, int op, double* out,
+                                                    CGState* st) {
+  if (op != kFinStore && st->done) return; // the producing kernel returned early too
+  double tot[kMaxSlots];
+  if (!finalize_totals(w, nblk, nslot, tot)) return;
   switch (op) {
   case kFinStore:
     for (int s = 0; s < nslot; ++s) out[s] = tot[s];
@@ -77,6 +84,76 @@ __global__ void __launch_bounds__(kFin) k_finalize(RedWork w, int nblk, int nslo
   default:
     break;
   }
+}
+
+// BiCGStab steps (BiCGStab.h:151-300): the totals of each reduction and the
+// decisions the reference takes on the host after it.
+__global__ void __launch_bounds__(kFin) k_finalize_bicg(RedWork w, int nblk, int nslot, int op, BiCGState* st) {
+  if (st->done) return;
+  double tot[kMaxSlots];
+  if (!finalize_totals(w, nblk, nslot, tot)) return;
+  switch (op) {
+  case kBFinAlpha: // alpha = rho1 / (v.r0); NullValueException("alpha") on 0 (:207, :250)
+    if (tot[0] == 0) {
+      st->breakdown = 1;
+      st->done = 1;
+    } else {
+      st->alpha = st->rho1 / tot[0];
+    }
+    break;
+  case kBFinS: // Iteration::stop(s): half-step exit, counted only in the first block (:194-199, :259-263)
+    if (st->it >= st->max_it) {
+      st->done = 1;
+      st->half = 1;
+    } else {
+      st->rr = tot[0];
+      if (tot[0] < st->crit) {
+        st->status = 1;
+        st->done = 1;
+        st->half = 1;
+      }
+    }
+    if (st->half && st->first) st->it += 1;
+    break;
+  case kBFinOmega: // omega = (t.s) / (t.t); t.t == 0: stop(r) in the first block, stop(s) in the loop (:210-218)
+    if (tot[1] == 0) {
+      const bool stop = st->first && (st->it >= st->max_it || st->rr0 < st->crit);
+      if (stop) {
+        if (st->it < st->max_it) st->status = 1;
+        st->it += 1;
+        st->half = 1;
+      } else {
+        st->breakdown = 2;
+      }
+      st->done = 1;
+    } else {
+      st->omega = tot[0] / tot[1];
+    }
+    break;
+  case kBFinXR: // rho = rho1; ++iter; while (!stop(r)); next rho1 = r.r0 (:224-230, :232-234)
+    st->rho = st->rho1;
+    st->rho1 = tot[1];
+    st->it += 1;
+    st->first = 0;
+    if (st->it >= st->max_it) {
+      st->done = 1;
+    } else {
+      st->rr = tot[0];
+      st->status = (tot[0] < st->crit) ? 1 : 0;
+      st->done = st->status;
+      if (!st->done && st->rho == 0) { // NullValueException("rho") at the next step (:240)
+        st->breakdown = 3;
+        st->done = 1;
+      }
+    }
+    break;
+  default:
+    break;
+  }
+}
+
+void launch_finalize_bicg(hipStream_t s, RedWork w, long nblk, int nslot, int op, BiCGState* st) {
+  hipLaunchKernelGGL(k_finalize_bicg, dim3(1), dim3(kFin), 0, s, w, (int)nblk, nslot, op, st);
 }
 
 void launch_finalize(hipStream_t s, RedWork w, long nblk, int nslot, int op, double* out, CGState* st) {

@@ -450,6 +450,242 @@ MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMa
   return out;
 }
 
+// Fused BiCGStab (kernels/bicgstab.hip) with the Diag or the learned
+// preconditioner; the reference's call sequence, with the step scalars and
+// the stop/breakdown decisions on the device.
+template <typename T>
+struct FusedBiCG {
+  alien_hip_ctx* ctx;
+  int n;
+  RBMat<T> A;
+  const T* invd;
+  PrecondSpec P;
+  DeviceArray<T> r, r0, p, phat, v, sv, shat, t, tmp;
+  DeviceArray<BiCGState> st;
+  BiCGState* h_st = nullptr; // pinned
+
+  FusedBiCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
+  : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
+    if (P.kind != kPrecDiag && P.kind != kPrecFactor)
+      throw Error(ALIEN_HIP_EINVAL, "fused BiCGStab implements the Diag and learned preconditioners");
+    ctx->reserve_for(n);
+    for (auto* a : {&r, &r0, &p, &phat, &v, &sv, &shat, &t}) a->resize(n);
+    if (P.kind == kPrecFactor) tmp.resize(n);
+    st.resize(1);
+    ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(BiCGState), hipHostMallocDefault));
+  }
+  ~FusedBiCG() {
+    release_graph();
+    if (h_st) (void)hipHostFree(h_st);
+  }
+  const int* done_flag() { return &st.data()->done; }
+  void precond(const T* in, T* out) {
+    launch_factor_apply_flag<T>(ctx->stream, n, P.g<T>(), P.gt<T>(), in, tmp.data(), out, done_flag());
+  }
+  // one step of the while loop (or the first block, per the state's `first`)
+  void step(T* x) {
+    hipStream_t s = ctx->stream;
+    const bool diag = P.kind == kPrecDiag;
+    const T* dinv = diag ? invd : nullptr;
+    launch_bicg_p<T>(s, n, r.data(), v.data(), p.data(), dinv, phat.data(), st.data());
+    if (!diag) precond(p.data(), phat.data());
+    launch_bicg_v<T>(s, ctx->red(), n, A, phat.data(), r0.data(), v.data(), st.data());
+    launch_bicg_s<T>(s, ctx->red(), n, r.data(), v.data(), sv.data(), dinv, shat.data(), st.data());
+    if (!diag) precond(sv.data(), shat.data());
+    launch_bicg_t<T>(s, ctx->red(), n, A, shat.data(), sv.data(), t.data(), st.data());
+    launch_bicg_xr<T>(s, ctx->red(), n, shat.data(), phat.data(), sv.data(), t.data(), r0.data(), x, r.data(),
+                      st.data());
+  }
+  BiCGState read_state() {
+    ALIEN_HIP_CHECK(hipMemcpyAsync(h_st, st.data(), sizeof(BiCGState), hipMemcpyDeviceToHost, ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return *h_st;
+  }
+  void write_state(const BiCGState& h) {
+    *h_st = h;
+    ALIEN_HIP_CHECK(hipMemcpyAsync(st.data(), h_st, sizeof(BiCGState), hipMemcpyHostToDevice, ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  T* g_x = nullptr;
+  int g_k = 0;
+  void release_graph() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    ge = nullptr;
+    g = nullptr;
+  }
+  BiCGState advance(T* x, bool graph, int k) {
+    if (graph) {
+      if (!ge || g_x != x || g_k != k) {
+        release_graph();
+        ALIEN_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < k; ++i) step(x);
+        ALIEN_HIP_CHECK(hipStreamEndCapture(ctx->stream, &g));
+        ALIEN_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        g_x = x;
+        g_k = k;
+      }
+      ALIEN_HIP_CHECK(hipGraphLaunch(ge, ctx->stream));
+    } else {
+      for (int i = 0; i < k; ++i) step(x);
+      check_launch("fused BiCGStab step");
+    }
+    return read_state();
+  }
+  // (re)start from r (already in place): r0 = r, rho1 = r.r0, first block next
+  void start(double rr, double crit, int it, int max_it) {
+    launch_copy<T>(ctx->stream, n, r.data(), r0.data());
+    BiCGState h{};
+    h.rho1 = rr;
+    h.rr = rr;
+    h.rr0 = rr;
+    h.crit = crit;
+    h.it = it;
+    h.max_it = max_it;
+    h.first = 1;
+    write_state(h);
+  }
+  // Solve from the given x; stall_window > 0 stops (*stalled) on a non-finite
+  // residual or no tenfold drop within the window, as FusedCG::solve.
+  BiCGState solve(const T* b, T* x, double tol, int max_it, bool graph, int poll_every, double* bb_out,
+                  int stall_window = 0, bool* stalled = nullptr) {
+    hipStream_t s = ctx->stream;
+    const int slot = ctx->take_slot();
+    launch_dot<T>(s, ctx->red(), n, b, b, ctx->dslot(slot));
+    const double bb = ctx->read_slot(slot);
+    if (bb_out) *bb_out = bb;
+    BiCGState h{};
+    if (bb == 0) { // Iteration::nullRhs
+      h.status = 1;
+      return h;
+    }
+    launch_residual<T>(s, ctx->red(), n, A, x, b, r.data(), ctx->dslot(slot));
+    const double rr = ctx->read_slot(slot);
+    start(rr, tol * tol * bb, 0, max_it);
+    const int k = graph ? std::max(1, poll_every) : 1;
+    BiCGState cur = advance(x, graph, 1);
+    double rr_ref = rr;
+    int it_ref = 0;
+    while (!cur.done) {
+      cur = advance(x, graph, k);
+      if (stall_window <= 0) continue;
+      if (!std::isfinite(cur.rr)) {
+        *stalled = true;
+        break;
+      }
+      if (cur.rr < 0.01 * rr_ref) {
+        rr_ref = cur.rr;
+        it_ref = cur.it;
+      } else if (cur.it - it_ref >= stall_window && !cur.done) {
+        *stalled = true;
+        break;
+      }
+    }
+    launch_bicg_half<T>(s, n, phat.data(), x, st.data());
+    return cur;
+  }
+};
+
+// Mixed-precision BiCGStab with reliable updates (Sleijpen & van der Vorst 1996,
+// the reference for the scheme): the FP32 recurrence updates an increment
+// x32; when the recursive residual has dropped by delta since the last update,
+// or the FP32 solve stops, x32 is folded into x64 and r = b - A x64 (FP64)
+// replaces the FP32 residual (rho1 = r.r0 recomputed); the shadow residual r0
+// and the search direction are kept. After a stop on s, a breakdown or an FP32
+// stop that the true residual does not confirm, the recurrence restarts from
+// the true residual. Convergence is judged on the FP64 true residual only.
+MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMat<float> A32,
+                                const float* invd32, const PrecondSpec& ps, const double* b, double* x,
+                                double tol, int max_it, double delta, int poll_every, bool graph,
+                                int stall_window = 0) {
+  MixedResult out;
+  hipStream_t s = ctx->stream;
+  const int slot = ctx->take_slot();
+  launch_dot<double>(s, ctx->red(), n, b, b, ctx->dslot(slot));
+  const double bb = ctx->read_slot(slot);
+  if (bb == 0) {
+    out.status = 1;
+    return out;
+  }
+  const double crit = tol * tol * bb;
+  DeviceArray<double> r64(n);
+  DeviceArray<float> x32(n);
+  ALIEN_HIP_CHECK(hipMemsetAsync(x32.data(), 0, sizeof(float) * n, s));
+  FusedBiCG<float> F(ctx, n, A32, invd32, ps);
+  const int rslot = ctx->take_slot();
+  auto true_res = [&]() {
+    launch_residual<double>(s, ctx->red(), n, A64, x, b, r64.data(), ctx->dslot(rslot));
+    return ctx->read_slot(rslot);
+  };
+  double rr64 = true_res();
+  out.rel = std::sqrt(rr64 / bb);
+  if (rr64 < crit) {
+    out.status = 1;
+    return out;
+  }
+  auto restart = [&](int it) {
+    launch_demote(s, n, r64.data(), F.r.data());
+    const int sl = ctx->take_slot();
+    launch_dot<float>(s, ctx->red(), n, F.r.data(), F.r.data(), ctx->dslot(sl));
+    F.start(ctx->read_slot(sl), crit, it, max_it);
+  };
+  restart(0);
+  BiCGState cur = F.advance(x32.data(), graph, 1);
+  double rr_last = rr64;
+  int it_last = 0, restarts = 0;
+  const double d2 = delta * delta;
+  const int k = graph ? std::max(1, poll_every) : 1;
+  for (;;) {
+    if (stall_window > 0 && !std::isfinite(cur.rr)) {
+      out.stalled = true;
+      break;
+    }
+    const bool stall = stall_window > 0 && cur.it - it_last >= stall_window;
+    const bool update = cur.done || cur.rr < d2 * rr_last || stall;
+    if (!update) {
+      cur = F.advance(x32.data(), graph, k);
+      continue;
+    }
+    launch_bicg_half<float>(s, n, F.phat.data(), x32.data(), F.st.data());
+    launch_fold_increment(s, n, x32.data(), x);
+    rr64 = true_res();
+    ++out.updates;
+    out.rel = std::sqrt(rr64 / bb);
+    if (rr64 < crit) {
+      out.status = 1;
+      break;
+    }
+    if (cur.it >= max_it || !(rr64 == rr64)) break;
+    if (stall && !(rr64 < d2 * rr_last)) {
+      out.stalled = true;
+      break;
+    }
+    rr_last = rr64;
+    it_last = cur.it;
+    if (cur.done) { // stop on s, FP32 stop not confirmed, or breakdown: restart from the true residual
+      if (cur.breakdown && ++restarts > 10) {
+        out.error = 1;
+        break;
+      }
+      restart(cur.it);
+      cur = F.advance(x32.data(), graph, 1);
+      continue;
+    }
+    // reliable update: r <- b - A x64, rho1 <- r.r0
+    launch_demote(s, n, r64.data(), F.r.data());
+    const int sl = ctx->take_slot();
+    launch_dot<float>(s, ctx->red(), n, F.r.data(), F.r0.data(), ctx->dslot(sl));
+    cur.rho1 = ctx->read_slot(sl);
+    cur.rr = rr64;
+    F.write_state(cur);
+    cur = F.advance(x32.data(), graph, k);
+  }
+  out.it = cur.it;
+  return out;
+}
+
 double true_residual(Alg& alg, const HIPMatrix& A, const HIPVector& b, const HIPVector& x) {
   // CSR SpMV (bit-exact with _seqMult), FP64
   HIPVector r;
@@ -480,12 +716,15 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
     const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
     const bool learned = p->precond == ALIEN_HIP_PREC_LEARNED;
-    if ((fused || mixed) && (p->solver != ALIEN_HIP_CG || (p->precond != ALIEN_HIP_PREC_CHEB &&
-                                                           p->precond != ALIEN_HIP_PREC_DIAG && !learned)))
+    const bool bicg = p->solver == ALIEN_HIP_BICGS;
+    if ((fused || mixed) && !bicg && (p->solver != ALIEN_HIP_CG || (p->precond != ALIEN_HIP_PREC_CHEB &&
+                                                                    p->precond != ALIEN_HIP_PREC_DIAG && !learned)))
       throw Error(ALIEN_HIP_EINVAL,
                   "fused/graph/mixed modes implement CG with the Diag, Chebyshev or learned preconditioner");
+    if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !learned)
+      throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement BiCGStab with the Diag or learned preconditioner");
     if (learned && !(fused || mixed))
-      throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed CG modes");
+      throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
     if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
     if (p->stall_window < 0) throw Error(ALIEN_HIP_EINVAL, "solve_csr: stall_window < 0");
     if (p->precond == ALIEN_HIP_PREC_CHEB && p->poly_order < 1)
@@ -609,6 +848,33 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         res->setup_s = tm.stop();
         tm.start();
         bool stalled = false;
+        if (bicg) {
+          BiCGState s;
+          {
+            FusedBiCG<double> F(ctx, n, rb, invd64, ps);
+            s = F.solve(b.data(), x.data(), p->tol, p->max_iter, graph, p->poll_every, nullptr, stall_window,
+                        &stalled);
+          }
+          if (stalled) {
+            res->fallback = 1;
+            if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
+            const int used = s.it;
+            FusedBiCG<double> J(ctx, n, rb, invd64, PrecondSpec{});
+            s = J.solve(b.data(), x.data(), p->tol, std::max(0, p->max_iter - used), graph, p->poll_every,
+                        nullptr);
+            s.it += used;
+          }
+          res->solve_s = tm.stop();
+          res->iterations = s.it;
+          res->status = s.status;
+          res->error = s.breakdown ? 1 : 0; // NullValueException
+          double bb = alg.dot(b, b);
+          res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
+          res->true_rel_res = true_residual(alg, A, b, x);
+          x.copyValuesTo(x_io);
+          ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+          return 0;
+        }
         CGState s;
         {
           FusedCG<double> F(ctx, n, rb, invd64, ps);
@@ -642,14 +908,15 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         res->setup_s = tm.stop();
         tm.start();
         const double delta = p->inner_tol > 0 ? p->inner_tol : 0.1;
-        MixedResult m = mixed_reliable_cg(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol,
-                                          p->max_iter, delta, p->poll_every, true, stall_window);
+        auto run = bicg ? mixed_reliable_bicg : mixed_reliable_cg;
+        MixedResult m = run(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol, p->max_iter, delta,
+                            p->poll_every, true, stall_window);
         if (m.stalled) {
           // Jacobi from the last reliable update, for the remaining iterations
           res->fallback = 1;
           const int used = m.it, ups = m.updates;
-          m = mixed_reliable_cg(ctx, n, rb64, rb32, invd32.data(), PrecondSpec{}, b.data(), x.data(), p->tol,
-                                std::max(0, p->max_iter - used), delta, p->poll_every, true);
+          m = run(ctx, n, rb64, rb32, invd32.data(), PrecondSpec{}, b.data(), x.data(), p->tol,
+                  std::max(0, p->max_iter - used), delta, p->poll_every, true, 0);
           m.it += used;
           m.updates += ups;
         }

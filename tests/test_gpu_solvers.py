@@ -161,3 +161,69 @@ def test_unstructured_matrix_falls_back_to_rbcsr(ctx):
     _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_DIAG, tol=1e-10)
     assert r["status"] == 1 and abs(r["iterations"] - o["iterations"]) <= 2
     assert r["true_rel_res"] < 1e-10
+
+
+def bsr2_scalar_csr(n, sigma=2.0, coupling=0.3, seed=42):
+    """The 2-unknown system of the block generator as scalar CSR (rows 2c, 2c+1)."""
+    import scipy.sparse as sp
+    ro, cols, vals = aa.gen_bsr2(n, params=[sigma, coupling], seed=seed)
+    nb = len(ro) - 1
+    blocks = vals.reshape(-1, 2, 2).transpose(0, 2, 1)  # column-major m[ieq + 2 iu] -> [row][col]
+    A = sp.bsr_matrix((blocks, cols, ro), shape=(2 * nb, 2 * nb)).tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH])
+@pytest.mark.parametrize("case", ["convdiff", "bsr2"])
+def test_fused_bicgstab_matches_oracle(ctx, mode, case):
+    """Fused BiCGStab (device-resident scalars, stop/breakdown rules on the device)
+    against the oracle's restatement of BiCGStab.h with the Diag preconditioner:
+    same status, iterations within +-2, true residual at the tolerance."""
+    if case == "convdiff":
+        ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 20)
+    else:
+        ro, cols, vals = bsr2_scalar_csr(10)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case fused BiCGStab", case, mode)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_DIAG, mode=mode, tol=1e-8,
+                        max_iter=5000, poll_every=3)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_DIAG, tol=1e-8, max_iter=5000)
+    say("  gpu", r["iterations"], r["status"], r["true_rel_res"], "oracle", o["iterations"], o["status"])
+    assert r["status"] == o["status"] == 1 and r["error"] == 0
+    assert abs(r["iterations"] - o["iterations"]) <= 2
+    assert r["true_rel_res"] < 1.5e-8
+
+
+def test_fused_bicgstab_max_iter_and_null_rhs(ctx):
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 12)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    _, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED,
+                        tol=1e-12, max_iter=5)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_DIAG, tol=1e-12, max_iter=5)
+    assert r["status"] == o["status"] == 0 and r["iterations"] == o["iterations"]
+    x, r = aa.solve_csr(ctx, ro, cols, vals, np.zeros(n), solver=aa.BICGS, precond=aa.PREC_DIAG,
+                        mode=aa.MODE_FUSED)
+    assert r["iterations"] == 0 and np.all(x == 0)
+
+
+@pytest.mark.parametrize("case", ["convdiff", "bsr2"])
+def test_mixed_bicgstab_reaches_fp64_tolerance(ctx, case):
+    """FP32 BiCGStab with FP64 reliable updates reaches an FP64 relative residual
+    of 1e-10 (the 2-unknown configuration's tolerance), judged on the true residual."""
+    if case == "convdiff":
+        ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 20)
+    else:
+        ro, cols, vals = bsr2_scalar_csr(12)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case mixed BiCGStab", case)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_DIAG, mode=aa.MODE_MIXED,
+                        tol=1e-10, max_iter=10000, poll_every=4)
+    _, r64 = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED,
+                          tol=1e-10, max_iter=10000)
+    say("  mixed", r["iterations"], r["outer"], r["true_rel_res"], "fp64", r64["iterations"])
+    assert r["status"] == 1 and r["true_rel_res"] < 1.05e-10
+    assert r["iterations"] <= 2 * r64["iterations"] + 20
