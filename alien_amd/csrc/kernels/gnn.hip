@@ -169,25 +169,30 @@ __device__ __forceinline__ int find_col(const int* __restrict__ cols, int b, int
     if (cols[k] == j) return k;
   return -1;
 }
+// The gradient uses the symmetric part, ((A + A^T) Ghat)_ij / 2, read through the
+// transpose positions (A^T)_ic = A_ci = -e at tpos (equal to AG for symmetric A).
 __global__ void __launch_bounds__(256) k_kcond(int n, const int* __restrict__ ro, const int* __restrict__ cols,
-                                               const int* __restrict__ dcol, const float* __restrict__ e,
-                                               const float* __restrict__ ghat, float* __restrict__ ag, RedWork w) {
+                                               const int* __restrict__ dcol, const int* __restrict__ tpos,
+                                               const float* __restrict__ e, const float* __restrict__ ghat,
+                                               float* __restrict__ ag, RedWork w) {
   __shared__ double lds[4];
   double tr = 0.0, ld = 0.0;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) {
     for (int p = ro[i]; p <= dcol[i]; ++p) { // lower entries and the diagonal of row i
       const int j = cols[p];
-      float s = 0.f;
+      float s = 0.f, st = 0.f;
       for (int k2 = ro[i]; k2 < ro[i + 1]; ++k2) {
         const int c = cols[k2];
         if (c < j) continue;                       // Ghat_cj = 0 above the diagonal
         const int q = find_col(cols, ro[c], dcol[c] + 1, j); // (c, j) in c's lower part
         if (q < 0) continue;
         const float a = (c == i) ? 1.f : -e[k2];
+        const float at = (c == i) ? 1.f : -e[tpos[k2]];
         s += a * ghat[q];
+        st += at * ghat[q];
       }
-      ag[p] = s;
+      ag[p] = 0.5f * (s + st);
       tr += (double)ghat[p] * (double)s;
     }
     ld = log((double)ghat[dcol[i]]);
@@ -399,17 +404,17 @@ __global__ void k_dense_bwd(int n, const float* __restrict__ dh, const float* __
   }
 }
 
-// dh[i] += sum_{j != i} e_ij dmsg[j]   (E is symmetric)
+// dh[i] += sum_{j != i} e_ji dmsg[j]   (E^T dmsg; e_ji read at the transpose position)
 __global__ void __launch_bounds__(256) k_aggregate_add(int n, const int* __restrict__ ro, const int* __restrict__ cols,
-                                                       const float* __restrict__ e, const float* __restrict__ dmsg,
-                                                       float* __restrict__ dh) {
+                                                       const int* __restrict__ tpos, const float* __restrict__ e,
+                                                       const float* __restrict__ dmsg, float* __restrict__ dh) {
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
   const int i = (int)(t >> 5);
   const int c = (int)(t & 31);
   if (i >= n) return;
   float acc = 0.f;
   for (int k = ro[i]; k < ro[i + 1]; ++k) {
-    const float w = e[k];
+    const float w = e[tpos[k]];
     if (w != 0.f) acc += w * dmsg[(long)cols[k] * kHidden + c];
   }
   dh[(long)i * kHidden + c] += acc;
@@ -642,7 +647,7 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, int 
   const bool kap = cg_steps == 0;
   if (kap) {
     hipLaunchKernelGGL(k_kcond, dim3((unsigned)nb), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
-                       d_e.data(), d_ghat.data(), ag.data(), w);
+                       d_tpos.data(), d_e.data(), d_ghat.data(), ag.data(), w);
     launch_finalize(s, w, nb, 2, kFinStore, sums.data(), nullptr);
   }
   // ---- probe term
@@ -681,8 +686,8 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, int 
     const float* Wl = P + offs[l - 1];
     hipLaunchKernelGGL(k_dense_bwd<64>, g1(n), dim3(256), 0, s, n, dh[cur].data(), d_z[l].data(), Wl, 2 * kHidden,
                        dz.data(), dh[cur ^ 1].data(), dmsg.data());
-    hipLaunchKernelGGL(k_aggregate_add, g1((long)n * 32), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_e.data(),
-                       dmsg.data(), dh[cur ^ 1].data());
+    hipLaunchKernelGGL(k_aggregate_add, g1((long)n * 32), dim3(256), 0, s, n, d_ro.data(), d_cols.data(),
+                       d_tpos.data(), d_e.data(), dmsg.data(), dh[cur ^ 1].data());
     hipLaunchKernelGGL(k_wgrad_partials<64>, dim3((unsigned)nb), dim3(256), 0, s, n, d_h[l - 1].data(), kHidden, kHidden,
                        d_m[l].data(), kHidden, 2 * kHidden, dz.data(), part.data(), cap);
     hipLaunchKernelGGL(k_colsum, g1(kLayerSize), dim3(256), 0, s, kLayerSize, (int)nb, part.data(), cap,

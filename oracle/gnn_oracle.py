@@ -196,20 +196,23 @@ def kcond_loss(g: Graph, Ghat):
     """Kaporin's condition number of B = Ghat^T Ahat Ghat, per row and up to the
     constant log det(Ahat)/N:  L = log(tr(B)/N) - (2/N) sum_j log Ghat_jj.
     It bounds the CG iteration count (Kaporin 1994) and its minimiser over the
-    pattern is the factorised sparse approximate inverse (FSAI)."""
+    pattern is the factorised sparse approximate inverse (FSAI). For a
+    nonsymmetric Ahat the trace sees its symmetric part only."""
     AG = (g.Ahat @ Ghat).tocsr()
     tr = float(Ghat.multiply(AG).sum())
     gd = Ghat.diagonal()
     L = np.log(tr / g.n) - 2.0 * np.sum(np.log(gd)) / g.n
-    return L, (AG, tr)
+    AGs = (0.5 * ((g.Ahat + g.Ahat.T) @ Ghat)).tocsr()
+    return L, (AGs, tr)
 
 
 def grad_kcond(g: Graph, Ghat, aux):
-    """dL/dGhat of kcond_loss on the lower pattern (diagonal, strictly-lower edges)."""
-    AG, tr = aux
-    AGd = np.asarray(AG[np.arange(g.n), np.arange(g.n)]).ravel()
+    """dL/dGhat of kcond_loss on the lower pattern (diagonal, strictly-lower
+    edges): d tr(G^T A G)/dG = (A + A^T) G."""
+    AGs, tr = aux
+    AGd = np.asarray(AGs[np.arange(g.n), np.arange(g.n)]).ravel()
     gd = 2.0 * AGd / tr - 2.0 / (g.n * Ghat.diagonal())
-    ge = 2.0 * np.asarray(AG[g.li, g.lj]).ravel() / tr
+    ge = 2.0 * np.asarray(AGs[g.li, g.lj]).ravel() / tr
     return gd, ge
 
 

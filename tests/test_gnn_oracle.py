@@ -113,3 +113,32 @@ def test_cg_loss_measures_convergence():
             q = A @ p; a = rr / (p @ q); x += a * p; r -= a * q; rn = r @ r; p = r + (rn / rr) * p; rr = rn
         ref.append(np.log(rr / (B[:, c] @ B[:, c])))
     assert abs(Li - np.mean(ref)) < 1e-10
+
+
+def bsr2_graph(n=3, seed=5):
+    """Scalarised 2-unknown block system (nonsymmetric values, symmetric pattern)."""
+    ro, cols, vals = aa.gen_bsr2(n, params=[1.5, 0.3], seed=seed)
+    nb = len(ro) - 1
+    blocks = vals.reshape(-1, 2, 2).transpose(0, 2, 1)
+    A = sp.bsr_matrix((blocks, cols, ro), shape=(2 * nb, 2 * nb)).tocsr()
+    A.sort_indices()
+    return go.Graph(A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data)
+
+
+def test_total_loss_gradient_nonsymmetric_system():
+    """Kaporin + probe objective and its gradient on a nonsymmetric matrix (the
+    2-unknown block system): central differences through the whole network."""
+    g = bsr2_graph()
+    assert abs(g.Ahat - g.Ahat.T).max() > 1e-3
+    th = perturbed_theta(2)
+    V = np.random.default_rng(8).choice([-1.0, 1.0], size=(g.n, 4))
+    L, grad, _ = go.total_loss_and_grad(g, th, V, beta=0.1)
+    rng = np.random.default_rng(12)
+    idx = np.union1d(rng.choice(go.NPARAM, 40, replace=False), np.arange(go.NPARAM - 98, go.NPARAM, 5))
+    h = 1e-6
+    for k in idx:
+        tp, tm = th.copy(), th.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (go.total_loss_and_grad(g, tp, V, beta=0.1)[0] - go.total_loss_and_grad(g, tm, V, beta=0.1)[0]) / (2 * h)
+        assert abs(fd - grad[k]) <= 1e-5 * max(1.0, abs(fd)), (k, fd, grad[k])

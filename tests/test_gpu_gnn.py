@@ -241,3 +241,25 @@ def test_learned_rejects_bad_input(ctx):
     with pytest.raises(aa.AlienHipError):
         aa.solve_csr(ctx, ro, cols, v2, b, precond=aa.PREC_LEARNED, mode=aa.MODE_FUSED,
                      gnn_theta=go.init_params(0))
+
+
+def test_gradient_matches_oracle_on_nonsymmetric_system(ctx):
+    """2-unknown block system (nonsymmetric values, symmetric pattern): the
+    backward pass uses E^T and the symmetric part of Ahat in Kaporin's gradient."""
+    import scipy.sparse as sp
+    ro, cols, vals = aa.gen_bsr2(6, params=[1.5, 0.3], seed=5)
+    nb = len(ro) - 1
+    blocks = vals.reshape(-1, 2, 2).transpose(0, 2, 1)
+    A = sp.bsr_matrix((blocks, cols, ro), shape=(2 * nb, 2 * nb)).tocsr()
+    A.sort_indices()
+    ro, cols, vals = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+    th = active_theta(9)
+    V = np.random.default_rng(3).choice([-1.0, 1.0], size=(len(ro) - 1, 4)).astype(np.float32)
+    say("gnn grad nonsymmetric")
+    L, _, grad = aa.gnn_loss_grad(ctx, ro, cols, vals, th, V, beta=0.1)
+    g = go.Graph(ro, cols, vals)
+    Lo, go_grad, _ = go.total_loss_and_grad(g, th, V.astype(np.float64), beta=0.1, bf=True)
+    rel = np.linalg.norm(grad - go_grad) / np.linalg.norm(go_grad)
+    say("  loss", L, Lo, "grad rel err", rel)
+    assert L == pytest.approx(Lo, rel=1e-4, abs=1e-6)
+    assert rel < 1e-3
