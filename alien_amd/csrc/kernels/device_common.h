@@ -28,7 +28,7 @@ inline long ept_blocks(long n) {
 // and the waves of an XCD stride through its share together: the waves running
 // at any moment read neighbouring bundles and their x-gathers hit that XCD's
 // L2, and per-wave start-up and the per-workgroup reduction are paid once per
-// walk. Measured at 256^3 (7-point): the FP32 row kernels run fastest capped at
+// walk. One-bundle-per-wave grids are mapped in chunks per XCD (bundle_walk). Measured at 256^3 (7-point): the FP32 row kernels run fastest capped at
 // 4096 workgroups, the FP64 ones uncapped. The environment variables
 // ALIEN_HIP_ROW_BLOCKS_F64 / _F32, read when a context is created, override the caps.
 constexpr int kXcds = 8;
@@ -40,17 +40,24 @@ inline long round_xcds(long g) {
   return g < kXcds ? kXcds : g;
 }
 // workgroups of a row kernel over n rows with elements of `elem` bytes
+// Uncapped grids are rounded to whole chunks of kXcdChunk workgroups per XCD
+// (bundle_walk's chunked XCD mapping needs a bijection).
+constexpr long kXcdChunk = 64;
+inline long round_chunks(long g) {
+  const long q = kXcds * kXcdChunk;
+  return (g + q - 1) / q * q;
+}
 inline long bundle_blocks(long n, int elem) {
   const long nb = (n + kWave - 1) / kWave;
-  long g = (nb + kRowWaves - 1) / kRowWaves;
+  const long g = (nb + kRowWaves - 1) / kRowWaves;
   const long cap = elem == 8 ? g_row_blocks_f64 : g_row_blocks_f32;
-  if (g > cap) g = cap;
-  return round_xcds(g);
+  if (g > cap) return round_xcds(cap);
+  return round_chunks(g);
 }
 // the largest grid any row kernel uses (reduction workspace)
 inline long bundle_blocks_max(long n) {
   const long nb = (n + kWave - 1) / kWave;
-  return round_xcds((nb + kRowWaves - 1) / kRowWaves);
+  return round_chunks((nb + kRowWaves - 1) / kRowWaves);
 }
 #if defined(__HIPCC__)
 // Bundles of this wave: b = first, first + stride, ... < end.
@@ -61,10 +68,18 @@ __device__ __forceinline__ BundleWalk bundle_walk(int n) {
   const int nb = (n + kWave - 1) / kWave;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
   BundleWalk w;
-  if ((long)gridDim.x * kRowWaves >= nb) { // one bundle per wave
-    w.first = (int)blockIdx.x * kRowWaves + wave;
-    w.end = nb;
-    w.stride = (int)gridDim.x * kRowWaves;
+  if ((long)gridDim.x * kRowWaves >= nb) {
+    // one bundle per wave; workgroup p (dealt to XCD p % 8) takes logical
+    // workgroup (i / C) 8C + (p % 8) C + i % C, i = p / 8: each XCD gets chunks of
+    // C consecutive workgroups (its rows' y-neighbours, and z-neighbours 256
+    // workgroups away at 256^2 rows per plane, stay in its L2), while the
+    // workgroups in flight still cover one contiguous window of 8C.
+    const int p = blockIdx.x, i = p / kXcds;
+    const int logical = (i / (int)kXcdChunk) * (int)(kXcds * kXcdChunk) + (p % kXcds) * (int)kXcdChunk +
+                        i % (int)kXcdChunk;
+    w.first = logical * kRowWaves + wave;
+    w.end = min(w.first + 1, nb);
+    w.stride = 1;
     return w;
   }
   const int per_xcd = gridDim.x / kXcds; // a capped grid is a multiple of kXcds
