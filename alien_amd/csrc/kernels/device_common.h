@@ -71,9 +71,9 @@ __device__ __forceinline__ BundleWalk bundle_walk(int n) {
   if ((long)gridDim.x * kRowWaves >= nb) {
     // one bundle per wave; workgroup p (dealt to XCD p % 8) takes logical
     // workgroup (i / C) 8C + (p % 8) C + i % C, i = p / 8: each XCD gets chunks of
-    // C consecutive workgroups (its rows' y-neighbours, and z-neighbours 256
-    // workgroups away at 256^2 rows per plane, stay in its L2), while the
-    // workgroups in flight still cover one contiguous window of 8C.
+    // C consecutive workgroups (the y-neighbour rows of a bundle mostly fall in
+    // its own chunk, so in its XCD's L2), while the workgroups in flight still
+    // cover one contiguous window of 8C. C = 64 measured best at 256^3 (8..128 swept).
     const int p = blockIdx.x, i = p / kXcds;
     const int logical = (i / (int)kXcdChunk) * (int)(kXcds * kXcdChunk) + (p % kXcds) * (int)kXcdChunk +
                         i % (int)kXcdChunk;
