@@ -62,11 +62,12 @@ __global__ void __launch_bounds__(kRowBlock) k_bicg_v(int n, RBMat<T> A, const T
   double s = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T>(A, b, [&](int c) { return phat[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    const T r0v = r0[min(row, n - 1)];
+    const T t = rb_sum<T>(A, b, [&](int c) { return phat[c]; });
     if (row < n) {
       v[row] = t;
-      s += (double)t * (double)r0[row];
+      s += (double)t * (double)r0v;
     }
   }
   double vv[1] = {block_sum<kRowBlock>(s, lds4)};
@@ -105,11 +106,12 @@ __global__ void __launch_bounds__(kRowBlock) k_bicg_t(int n, RBMat<T> A, const T
   double ts = 0.0, tt = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T tv = rb_sum<T>(A, b, [&](int c) { return shat[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    const T srow = sv[min(row, n - 1)];
+    const T tv = rb_sum<T>(A, b, [&](int c) { return shat[c]; });
     if (row < n) {
       t[row] = tv;
-      ts += (double)tv * (double)sv[row];
+      ts += (double)tv * (double)srow;
       tt += (double)tv * (double)tv;
     }
   }

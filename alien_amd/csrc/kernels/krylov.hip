@@ -48,10 +48,11 @@ __global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const
   double s = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    const T br = rhs[min(row, n - 1)];
+    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
     if (row < n) {
-      const T v = rhs[row] + T(-1) * t;
+      const T v = br + T(-1) * t;
       r[row] = v;
       s += (double)v * (double)v;
     }
@@ -157,11 +158,12 @@ __global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* 
   double s = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T>(A, b, [&](int c) { return p[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    const T prow = p[min(row, n - 1)]; // unconditional (clamped): issued with the row's loads
+    const T t = rb_sum<T>(A, b, [&](int c) { return p[c]; });
     if (row < n) {
       q[row] = t;
-      s += (double)p[row] * (double)t;
+      s += (double)prow * (double)t;
     }
   }
   double v[1] = {block_sum<kRowBlock>(s, lds4)};
@@ -233,9 +235,10 @@ __global__ void __launch_bounds__(kRowBlock) k_factor_p(int n, RBMat<T> G, const
   const T beta = first ? T(0) : (T)(st->rho1 / st->rho);
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T z = rb_sum<T>(G, b, [&](int c) { return t[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
-    if (row < n) p[row] = first ? z : z + beta * p[row];
+    const T pold = p[min(row, n - 1)];
+    const T z = rb_sum<T>(G, b, [&](int c) { return t[c]; });
+    if (row < n) p[row] = first ? z : z + beta * pold;
   }
 }
 
