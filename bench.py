@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 ROOF, ROOF_KERNEL = "rbdia_fp64", "k_spmv_rbdia<double>"
 # HBM bytes per launch at 256^3 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by
 # the same counter on a copy kernel of known size (profiles/spmv_lognormal256_hbm_traffic.md)
-TRAFFIC = {"rbdia_fp64": 1537999856, "rbdia_fp32": 719756593, "rbcsr_fp64": 1952203693, "rbcsr_fp32": 1144810795}
+TRAFFIC = {"rbdia_fp64": 1389471276, "rbdia_fp32": 719750450, "rbcsr_fp64": 1813950096, "rbcsr_fp32": 1144821033}
 
 
 def log(*a):
