@@ -387,6 +387,17 @@ int alien_hip_spmv_rbdia_f32(alien_hip_ctx* ctx, int n, const int* bo, const int
   API_END
 }
 
+int alien_hip_filu0_sweep_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const int* dcol,
+                              const double* vals, const double* prev, double* lu) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && dcol && vals && prev && lu)) && prev != lu,
+          "filu0_sweep_f64: bad arguments");
+  API_BEGIN
+  launch_filu0_sweep(ctx->stream, n, ro, cols, dcol, vals, prev, lu);
+  LAUNCHED("filu0_sweep_f64");
+  API_END
+}
+
 int alien_hip_spmv_rbcsr_f64(alien_hip_ctx* ctx, int n, const int* bo, const int* bc, const double* bv,
                              const double* x, double* y) {
   CTX_OK(ctx);

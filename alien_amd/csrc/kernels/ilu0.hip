@@ -46,6 +46,43 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_factor_level(int count, const i
   }
 }
 
+// One fixed-point sweep of FILU0's factorisation (FLUFactorisationAlgo::
+// factorizeMultiIter / _factorizeRow, FILU0Preconditioner.h:146-170, 328-357):
+// row i restarts from A's values and runs the IKJ update of ILU(0) with the
+// other rows' entries (pivot a_kk and a_kj) read from the previous sweep, so
+// all rows are independent: one lane per row. Row 0 keeps A's values.
+__global__ void __launch_bounds__(kBlock) k_filu0_sweep(int n, const int* __restrict__ ro,
+                                                         const int* __restrict__ cols,
+                                                         const int* __restrict__ dcol,
+                                                         const double* __restrict__ a,
+                                                         const double* __restrict__ prev, double* __restrict__ lu) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int rb = ro[i], re = ro[i + 1], di = dcol[i];
+  for (int k = rb; k < re; ++k) lu[k] = a[k];
+  if (i == 0) return;
+  for (int k = rb; k < di; ++k) {
+    const int kr = cols[k];
+    const double aik = lu[k] / prev[dcol[kr]];
+    lu[k] = aik;
+    int p = ro[kr];
+    const int pe = ro[kr + 1];
+    for (int j = k + 1; j < re; ++j) {
+      const int cj = cols[j];
+      while (p < pe && cols[p] < cj) ++p;
+      if (p == pe) break;
+      if (cols[p] == cj) lu[j] = lu[j] - aik * prev[p];
+    }
+  }
+}
+
+void launch_filu0_sweep(hipStream_t st, int n, const int* ro, const int* cols, const int* dcol, const double* a,
+                        const double* prev, double* lu) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_filu0_sweep, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n, ro, cols,
+                       dcol, a, prev, lu);
+}
+
 // L sweep of one level: t_i = y_i - sum_{k<dcol} l_ik t_k (solveL, :181-198)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_ilu0_lsolve_level(int count, const int* __restrict__ rows,

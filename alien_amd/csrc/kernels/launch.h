@@ -52,6 +52,9 @@ void launch_addmult_bellpack(hipStream_t, int n, int slice, double alpha, const 
                              const double* bv, const uint8_t* mask, const double* x, double* y);
 void launch_addmult_csr(hipStream_t, int n, int upper, double alpha, const int* ro, const int* cols,
                         const int* dcol, const double* vals, const double* x, double* y);
+// one fixed-point sweep of FILU0's factorisation (ilu0.hip)
+void launch_filu0_sweep(hipStream_t, int n, const int* ro, const int* cols, const int* dcol, const double* a,
+                        const double* prev, double* lu);
 template <typename T>
 void launch_spmv_bsr2(hipStream_t, int nb, const int* ro, const int* cols, const T* vals, const T* x, T* y);
 void launch_invdiag_csr(hipStream_t, int n, const int* dcol, const double* vals, double* y);

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "alien/hip/HIPBackEnd.h"
+#include "alien/hip/krylov/HIPFILU0Preconditioner.h"
 #include "alien/hip/krylov/HIPILU0Preconditioner.h"
 #include "alien/hip/krylov/Krylov.h"
 #include "alien_hip_internal.h"
@@ -727,6 +728,10 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
     if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
     if (p->stall_window < 0) throw Error(ALIEN_HIP_EINVAL, "solve_csr: stall_window < 0");
+    if (p->precond == ALIEN_HIP_PREC_FILU0 && (fused || mixed))
+      throw Error(ALIEN_HIP_EINVAL, "FILU0 runs in the REFERENCE mode");
+    if (p->precond == ALIEN_HIP_PREC_FILU0 && (p->filu_factor_niter < 0 || p->filu_solver_niter < 0))
+      throw Error(ALIEN_HIP_EINVAL, "FILU0: negative iteration counts");
     if (p->precond == ALIEN_HIP_PREC_CHEB && p->poly_order < 1)
       throw Error(ALIEN_HIP_EINVAL, "Chebyshev order must be >= 1");
 
@@ -779,6 +784,15 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       }
       case ALIEN_HIP_PREC_ILU0: {
         Alien::ILU0Preconditioner<Alg> P(alg, A);
+        P.init();
+        res->setup_s = tm.stop();
+        run_reference(alg, p, P, A, b, x, res, tm);
+        break;
+      }
+      case ALIEN_HIP_PREC_FILU0: {
+        Alien::FILU0Preconditioner<Alg> P(alg, A);
+        P.setParameter("nb-factor-iter", p->filu_factor_niter);
+        P.setParameter("nb-solver-iter", p->filu_solver_niter);
         P.init();
         res->setup_s = tm.stop();
         run_reference(alg, p, P, A, b, x, res, tm);

@@ -197,6 +197,11 @@ int alien_hip_spmv_dot_csr_f64(alien_hip_ctx* ctx, int nrows, const int* row_off
 int alien_hip_ilu0_factor_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
                               const int* dcol, double* lu, int nlevels, const int* level_ptr_host,
                               const int* level_rows);
+/* one fixed-point sweep of FILU0's factorisation (FLUFactorisationAlgo::factorizeMultiIter /
+ * _factorizeRow, FILU0Preconditioner.h:146-170, 328-357): lu = A's values, then the IKJ update
+ * of every row i >= 1 with the other rows read from prev (device pointers, CSR of A) */
+int alien_hip_filu0_sweep_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                              const int* dcol, const double* vals, const double* prev, double* lu);
 /* L x1 = y then U x = x1 (solveL/solveU, ILU0Preconditioner.h:181-231), level by
  * level; t is an n-vector of workspace. lower/upper schedules as above. */
 int alien_hip_ilu0_solve_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
@@ -211,7 +216,9 @@ int alien_hip_ilu0_solve_f64(alien_hip_ctx* ctx, int nrows, const int* row_offse
 enum { ALIEN_HIP_CG = 0, ALIEN_HIP_BICGS = 1 };
 enum { ALIEN_HIP_PREC_DIAG = 0, ALIEN_HIP_PREC_CHEB = 1, ALIEN_HIP_PREC_ILU0 = 2,
        ALIEN_HIP_PREC_NONE = 3,
-       ALIEN_HIP_PREC_LEARNED = 4 /* M = G G^T from the graph network (FUSED/GRAPH/MIXED modes) */ };
+       ALIEN_HIP_PREC_LEARNED = 4, /* M = G G^T from the graph network (FUSED/GRAPH/MIXED modes) */
+       ALIEN_HIP_PREC_FILU0 = 5 /* FILU0Preconditioner (REFERENCE mode): fixed-point ILU(0) factor,
+                                   Jacobi triangular sweeps */ };
 /* RB-CSR: row-bundled CSR; RB-DIA: row-bundled diagonal offsets (no column
  * stream; RB-CSR when it would store over 1.25x the slots). In the FUSED, GRAPH
  * and MIXED modes layout RBCSR forces RB-CSR, any other value selects RB-DIA. */
@@ -244,6 +251,8 @@ typedef struct {
   int    stall_window;  /* LEARNED: iterations without a tenfold drop of the residual norm
                            (MIXED: without a reliable update) before the Jacobi fallback;
                            0 = 400. A non-finite residual falls back at once. */
+  int    filu_factor_niter; /* FILU0: nb-factor-iter (0 = exact ILU(0) factor) */
+  int    filu_solver_niter; /* FILU0: nb-solver-iter (test_krylov default 3) */
 } alien_hip_solve_params;
 
 typedef struct {

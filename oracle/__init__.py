@@ -16,7 +16,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 ORC_CG, ORC_BICGS = 0, 1
-PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE = 0, 1, 2, 3
+PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_FILU0 = 0, 1, 2, 3, 4
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -26,7 +26,7 @@ _u8 = C.POINTER(C.c_uint8)
 class SolveParams(C.Structure):
     _fields_ = [("solver", C.c_int), ("precond", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int),
                 ("poly_factor", C.c_double), ("poly_order", C.c_int), ("poly_max_iter", C.c_int),
-                ("nthreads", C.c_int)]
+                ("nthreads", C.c_int), ("filu_factor_niter", C.c_int), ("filu_solver_niter", C.c_int)]
 
 
 class SolveResult(C.Structure):
@@ -53,6 +53,7 @@ _SIGS = [
     ("orc_bellpack_invdiag", None, [C.c_int, C.c_int, _ip, _ip, _dp, _dp]),
     ("orc_dot", C.c_double, [C.c_int, _dp, _dp]),
     ("orc_ilu0_factor", None, [C.c_int, _ip, _ip, _ip, _dp]),
+    ("orc_filu0_factor", None, [C.c_int, _ip, _ip, _ip, _dp, C.c_int, _dp]),
     ("orc_ilu0_solve", None, [C.c_int, _ip, _ip, _ip, _dp, _dp, _dp]),
     ("orc_levels_lower", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
     ("orc_levels_upper", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
@@ -195,14 +196,23 @@ def color(ro, cols):
 
 
 def solve(ro, cols, vals, b, x0=None, *, solver=ORC_CG, precond=PREC_CHEB, tol=1e-8, max_iter=1000,
-          poly_factor=0.5, poly_order=3, poly_max_iter=10, nthreads=1):
+          poly_factor=0.5, poly_order=3, poly_max_iter=10, nthreads=1, filu_factor_niter=0, filu_solver_niter=3):
     n = len(ro) - 1
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
-    p = SolveParams(solver, precond, tol, max_iter, poly_factor, poly_order, poly_max_iter, nthreads)
+    p = SolveParams(solver, precond, tol, max_iter, poly_factor, poly_order, poly_max_iter, nthreads,
+                    filu_factor_niter, filu_solver_niter)
     r = SolveResult()
     lib().orc_solve(C.byref(p), n, _i(ro), _i(cols), _d(vals), _d(np.ascontiguousarray(b, np.float64)), _d(x),
                     C.byref(r))
     return x, r.as_dict()
+
+
+def filu0_factor(ro, cols, vals, niter):
+    """FILU0's fixed-point factorisation after niter sweeps (FILU0Preconditioner.h:146-170)."""
+    d = dcol(ro, cols)
+    lu = np.empty(len(vals))
+    lib().orc_filu0_factor(len(ro) - 1, _i(ro), _i(cols), _i(d), _d(vals), niter, _d(lu))
+    return lu
 
 
 def time_cg_iters(ro, cols, vals, b, precond=PREC_DIAG, poly_order=3, n_iter=5, nthreads=1):

@@ -577,6 +577,30 @@ void orc_ilu0_factor(int n, const int* ro, const int* cols, const int* dcol, dou
   }
 }
 
+void orc_filu0_factor(int n, const int* ro, const int* cols, const int* dcol, const double* a, int niter,
+                      double* lu) {
+  std::vector<double> prev(ro[n]);
+  std::vector<int> work(n, -1);
+  std::copy(a, a + ro[n], lu);
+  for (int it = 0; it < niter; ++it) {
+    std::copy(lu, lu + ro[n], prev.begin()); /* values0 = current factor */
+    std::copy(a, a + ro[n], lu);             /* values = A */
+    for (int i = 1; i < n; ++i) {            /* _factorizeRow (:328-357) */
+      for (int k = ro[i]; k < dcol[i]; ++k) {
+        const int kr = cols[k];
+        const double aik = lu[k] / prev[dcol[kr]];
+        lu[k] = aik;
+        for (int l = ro[kr]; l < ro[kr + 1]; ++l) work[cols[l]] = l;
+        for (int j = k + 1; j < ro[i + 1]; ++j) {
+          const int kj = work[cols[j]];
+          if (kj != -1) lu[j] -= aik * prev[kj];
+        }
+        for (int l = ro[kr]; l < ro[kr + 1]; ++l) work[cols[l]] = -1;
+      }
+    }
+  }
+}
+
 void orc_ilu0_solve(int n, const int* ro, const int* cols, const int* dcol, const double* lu,
                     const double* y, double* x) {
   std::vector<double> t(n);
@@ -638,6 +662,47 @@ int orc_color_greedy(int n, const int* ro, const int* cols, int* color) {
   return nc;
 }
 
+/* FILU0Preconditioner / FLUFactorisationAlgo (FILU0Preconditioner.h:35-370), the device branch of its
+ * triangular solves (on_host_only = false, :226-230 and :297-303): Jacobi sweeps through the masked
+ * products addLMult / addUMult and the inverse diagonal of the factor. */
+struct FILU0Prec : Precond {
+  int n, nsolve;
+  const int* ro;
+  const int* cols;
+  std::vector<int> dcol;
+  Vec lu, inv_diag, mx, xk;
+  FILU0Prec(const Algebra& alg, int nfactor, int nsolve_) : n(alg.n), nsolve(nsolve_), ro(alg.ro), cols(alg.cols),
+                                                            dcol(alg.dcol) {
+    lu.assign(ro[n], 0.0);
+    if (nfactor == 0) { /* BaseType::factorize (:88-90) */
+      std::copy(alg.vals, alg.vals + ro[n], lu.begin());
+      orc_ilu0_factor(n, ro, cols, dcol.data(), lu.data());
+    } else {
+      orc_filu0_factor(n, ro, cols, dcol.data(), alg.vals, nfactor, lu.data());
+    }
+    inv_diag.assign(n, 1.0); /* assign(m_inv_diag, 1); computeInvDiag (:94-96) */
+    orc_invdiag_csr(n, dcol.data(), lu.data(), inv_diag.data());
+    mx.assign(n, 0.0);
+    xk.assign(n, 0.0);
+  }
+  /* solve (:236-256): L sweeps then U sweeps */
+  void solve(const Algebra& alg, const Vec& x, Vec& y) override {
+    alg.copy(x, mx);
+    for (int it = 0; it < nsolve; ++it) { /* solveL, device branch (:226-230) */
+      alg.copy(mx, xk);
+      alg.copy(x, mx);
+      orc_addlmult(n, ro, cols, dcol.data(), lu.data(), -1., xk.data(), mx.data());
+    }
+    alg.copy(mx, y);
+    for (int it = 0; it < nsolve; ++it) { /* solveU, device branch (:297-303) */
+      alg.copy(y, xk);
+      alg.copy(mx, y);
+      orc_addumult(n, ro, cols, dcol.data(), lu.data(), -1., xk.data(), y.data());
+      alg.pointwiseMult(inv_diag, y, y);
+    }
+  }
+};
+
 int orc_solve(const orc_solve_params* p, int n, const int* ro, const int* cols, const double* vals,
               const double* bptr, double* xptr, orc_solve_result* res) {
   std::memset(res, 0, sizeof(*res));
@@ -655,6 +720,7 @@ int orc_solve(const orc_solve_params* p, int n, const int* ro, const int* cols, 
     res->cheb_beta = cheb->beta;
     break;
   case ORC_PREC_ILU0: P = new ILU0Prec(alg); break;
+  case ORC_PREC_FILU0: P = new FILU0Prec(alg, p->filu_factor_niter, p->filu_solver_niter); break;
   default: P = new NonePrec(); break;
   }
   double t1 = now_s();

@@ -77,7 +77,7 @@ int  orc_color_greedy(int nrows, const int* row_offset, const int* cols, int* co
  *               DiagPreconditioner.h, ChebyshevPreconditioner.h:73-202,392-444,
  *               ILU0Preconditioner.h) ------------------------------------- */
 enum { ORC_CG = 0, ORC_BICGS = 1 };
-enum { ORC_PREC_DIAG = 0, ORC_PREC_CHEB = 1, ORC_PREC_ILU0 = 2, ORC_PREC_NONE = 3 };
+enum { ORC_PREC_DIAG = 0, ORC_PREC_CHEB = 1, ORC_PREC_ILU0 = 2, ORC_PREC_NONE = 3, ORC_PREC_FILU0 = 4 };
 
 typedef struct {
   int    solver;        /* ORC_CG / ORC_BICGS */
@@ -88,6 +88,8 @@ typedef struct {
   int    poly_order;
   int    poly_max_iter; /* power-method iterations */
   int    nthreads;      /* >1: row-block partition, dots summed per block then in block order */
+  int    filu_factor_niter; /* FILU0: fixed-point factorisation sweeps (0 = exact ILU0) */
+  int    filu_solver_niter; /* FILU0: Jacobi sweeps per triangular solve */
 } orc_solve_params;
 
 typedef struct {
@@ -101,6 +103,11 @@ typedef struct {
   double setup_s;       /* steady_clock phase timers (StdTimer semantics) */
   double solve_s;
 } orc_solve_result;
+
+/* FLUFactorisationAlgo::factorizeMultiIter (FILU0Preconditioner.h:146-170): niter sweeps, each
+ * restarting from A's values and reading the previous sweep's factor for the other rows */
+void orc_filu0_factor(int n, const int* ro, const int* cols, const int* dcol, const double* a, int niter,
+                      double* lu);
 
 /* x is the initial guess on input and the solution on output */
 int orc_solve(const orc_solve_params* p, int nrows, const int* row_offset, const int* cols,

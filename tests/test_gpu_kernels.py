@@ -303,3 +303,23 @@ def test_full_size_spmv_properties_256(ctx):
     call("alien_hip_spmv_rbcsr_f64", ctx.h, n, p(dbo), p(bc), p(bv), p(x), p(y2))
     ctx.synchronize()
     assert torch.equal(y1, y2)
+
+
+def test_filu0_factor_sweeps_bit_exact(ctx):
+    """The GPU's fixed-point sweeps give the oracle's factor bit for bit (same
+    per-row operations), and reach the exact ILU(0) factor after as many sweeps as
+    the lower DAG has levels."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 9, 7, 5)
+    n = len(ro) - 1
+    d = orc.dcol(ro, cols)
+    _, nl = orc.levels(ro, cols, upper=False)
+    for niter in (1, 3, nl):
+        say("case filu0 sweeps", niter)
+        dro, dc, dd, da = dev(ro), dev(cols), dev(d), dev(vals)
+        cur = dev(vals)
+        for _ in range(niter):
+            prev = cur.clone()
+            torch.cuda.synchronize()
+            call("alien_hip_filu0_sweep_f64", ctx.h, n, p(dro), p(dc), p(dd), p(da), p(prev), p(cur))
+        np.testing.assert_array_equal(host(cur, ctx), orc.filu0_factor(ro, cols, vals, niter))
+    np.testing.assert_array_equal(orc.filu0_factor(ro, cols, vals, nl), orc.ilu0_factor(ro, cols, vals))

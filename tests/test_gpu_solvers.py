@@ -244,3 +244,22 @@ def test_learned_bicgstab_on_two_unknown_system(ctx, mode):
                         max_iter=6000, gnn_theta=go.init_params(0), stall_window=60, poll_every=4)
     say("  ", r)
     assert r["status"] == 1 and r["true_rel_res"] < 1.05e-10
+
+
+@pytest.mark.parametrize("factor_niter,solver_niter", [(0, 3), (2, 3), (5, 2)])
+def test_bicgstab_filu0_matches_oracle(ctx, factor_niter, solver_niter):
+    """FILU0 (FILU0Preconditioner.h, device branch): fixed-point factor sweeps and
+    Jacobi triangular sweeps through addLMult/addUMult, against the oracle's
+    restatement on the same inputs: iterations within +-2."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 20)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case BiCGStab FILU0", factor_niter, solver_niter)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_FILU0, tol=1e-8, max_iter=3000,
+                        filu_factor_niter=factor_niter, filu_solver_niter=solver_niter)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_FILU0, tol=1e-8, max_iter=3000,
+                     filu_factor_niter=factor_niter, filu_solver_niter=solver_niter)
+    say("  gpu", r["iterations"], r["status"], "oracle", o["iterations"], o["status"])
+    assert r["status"] == o["status"] == 1 and r["error"] == 0
+    assert abs(r["iterations"] - o["iterations"]) <= 2
+    assert r["true_rel_res"] < 1e-8

@@ -316,3 +316,39 @@ def test_threaded_oracle_matches_sequential_iterations():
     _, r1 = orc.solve(ro, cols, vals, b, precond=orc.PREC_CHEB)
     _, r4 = orc.solve(ro, cols, vals, b, precond=orc.PREC_CHEB, nthreads=4)
     assert abs(r1["iterations"] - r4["iterations"]) <= 2 and r4["status"] == 1
+
+
+def test_filu0_factor_fixed_point_reaches_ilu0():
+    """FILU0's fixed-point sweeps (FILU0Preconditioner.h:146-170): after as many
+    sweeps as the lower DAG has levels every row has read final rows, so the factor
+    equals the sequential ILU(0) factor bit for bit (Chow & Patel 2015); before
+    that it differs."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 6, 5, 4)
+    exact = orc.ilu0_factor(ro, cols, vals)
+    d = orc.dcol(ro, cols)
+    lv, nl = orc.levels(ro, cols, upper=False)
+    np.testing.assert_array_equal(orc.filu0_factor(ro, cols, vals, nl), exact)
+    np.testing.assert_array_equal(orc.filu0_factor(ro, cols, vals, nl + 3), exact)
+    assert not np.array_equal(orc.filu0_factor(ro, cols, vals, 1), exact)
+    # one sweep: rows read A's own values for the other rows (closed form on row 1)
+    lu1 = orc.filu0_factor(ro, cols, vals, 1)
+    np.testing.assert_array_equal(lu1[ro[0]:ro[1]], vals[ro[0]:ro[1]])
+
+
+def test_filu0_solver_sweeps_converge_to_ilu0():
+    """With as many Jacobi sweeps as levels, FILU0's triangular solves equal the
+    exact ILU(0) solves (L part bit for bit; U within rounding of 1/u_ii), so a
+    BiCGStab run with FILU0 matches the ILU0 run's iteration count."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 8)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    d = orc.dcol(ro, cols)
+    _, nl = orc.levels(ro, cols, upper=False)
+    _, ri = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-8)
+    _, rf = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_FILU0, tol=1e-8,
+                      filu_factor_niter=0, filu_solver_niter=nl)
+    assert rf["status"] == ri["status"] == 1
+    assert abs(rf["iterations"] - ri["iterations"]) <= 1
+    _, r3 = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_FILU0, tol=1e-8,
+                      filu_factor_niter=2, filu_solver_niter=3)
+    assert r3["status"] == 1 and r3["iterations"] >= ri["iterations"]
