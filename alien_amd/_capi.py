@@ -125,6 +125,10 @@ _SIGS = [
                                            c_void_p]),
     ("alien_hip_solve_csr", C.c_int, [c_void_p, C.POINTER(SolveParams), C.c_int, c_int_p, c_int_p, c_double_p,
                                       c_double_p, c_double_p, C.POINTER(SolveResult)]),
+    ("alien_hip_mtx_read_csr", C.c_int, [C.c_char_p, c_int_p, C.POINTER(C.c_int64), c_int_p, c_int_p, c_double_p]),
+    ("alien_hip_mtx_read_vector", C.c_int, [C.c_char_p, c_int_p, c_double_p]),
+    ("alien_hip_mtx_write_csr", C.c_int, [C.c_char_p, C.c_int, c_int_p, c_int_p, c_double_p, C.c_char_p]),
+    ("alien_hip_mtx_write_vector", C.c_int, [C.c_char_p, C.c_int, c_double_p, C.c_char_p]),
     ("alien_hip_gnn_nparams", C.c_int, []),
     ("alien_hip_gnn_init", C.c_int, [C.c_uint64, c_float_p]),
     ("alien_hip_gnn_forward", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, c_float_p, c_float_p]),
@@ -344,6 +348,43 @@ def solve_csr(ctx: Context, ro, cols, vals, b, x0=None, *, solver=CG, precond=PR
 # --------------------------------------------------------------------------
 # learned preconditioner M = G G^T
 # --------------------------------------------------------------------------
+# --------------------------------------------------------------------------
+# MatrixMarket system I/O
+# --------------------------------------------------------------------------
+def mtx_read_csr(path):
+    """(row_offset, cols, vals) of a MatrixMarket 'coordinate real' matrix."""
+    n, nnz = C.c_int(), C.c_int64()
+    p = str(path).encode()
+    check(lib().alien_hip_mtx_read_csr(p, C.byref(n), C.byref(nnz), None, None, None))
+    ro = np.empty(n.value + 1, dtype=np.int32)
+    cols = np.empty(max(nnz.value, 1), dtype=np.int32)
+    vals = np.empty(max(nnz.value, 1))
+    check(lib().alien_hip_mtx_read_csr(p, C.byref(n), C.byref(nnz), iptr(ro), iptr(cols), dptr(vals)))
+    return ro, cols[:nnz.value], vals[:nnz.value]
+
+
+def mtx_read_vector(path):
+    n = C.c_int()
+    p = str(path).encode()
+    check(lib().alien_hip_mtx_read_vector(p, C.byref(n), None))
+    v = np.empty(max(n.value, 1))
+    check(lib().alien_hip_mtx_read_vector(p, C.byref(n), dptr(v)))
+    return v[:n.value]
+
+
+def mtx_write_csr(path, ro, cols, vals, description=""):
+    ro = np.ascontiguousarray(ro, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    check(lib().alien_hip_mtx_write_csr(str(path).encode(), len(ro) - 1, iptr(ro), iptr(cols), dptr(vals),
+                                        description.encode()))
+
+
+def mtx_write_vector(path, v, description=""):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    check(lib().alien_hip_mtx_write_vector(str(path).encode(), len(v), dptr(v), description.encode()))
+
+
 def fptr(a: np.ndarray):
     assert a.dtype == np.float32 and a.flags.c_contiguous
     return a.ctypes.data_as(c_float_p)

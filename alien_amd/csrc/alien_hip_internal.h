@@ -44,6 +44,14 @@ int rbcsr_bundle_offsets(int nrows, const int* ro, int* bundle_off);
 // more than max_w distinct offsets or int32 indexing would overflow.
 long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, const int* ke, int max_w,
                      std::vector<int>& bo, std::vector<int>& doff, std::vector<uint64_t>& dmask);
+// ---- MatrixMarket system I/O (host/mtx.cpp) ----
+int mtx_read_csr(const char* path, int& n, std::vector<int>& ro, std::vector<int>& cols, std::vector<double>& vals,
+                 std::string& err);
+int mtx_read_vector(const char* path, std::vector<double>& v, std::string& err);
+int mtx_write_csr(const char* path, int n, const int* ro, const int* cols, const double* vals, const char* desc,
+                  std::string& err);
+int mtx_write_vector(const char* path, int n, const double* v, const char* desc, std::string& err);
+
 // RB-DIA is chosen over RB-CSR when it stores at most this many more slots
 constexpr double kRbdiaMaxGrowth = 1.25;
 constexpr int kRbdiaMaxWidth = 64;

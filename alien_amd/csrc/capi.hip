@@ -226,6 +226,55 @@ int alien_hip_rbdia_struct(int nrows, const int* ro, const int* cols, int max_w,
   API_END
 }
 
+int alien_hip_mtx_read_csr(const char* path, int* nrows, int64_t* nnz, int* ro, int* cols, double* vals) {
+  REQUIRE(path && nrows && nnz, "mtx_read_csr: bad arguments");
+  API_BEGIN
+  int n = 0;
+  std::vector<int> r, c;
+  std::vector<double> v;
+  std::string err;
+  if (mtx_read_csr(path, n, r, c, v, err) != 0) return fail(ALIEN_HIP_EINVAL, "mtx_read_csr: " + err);
+  *nrows = n;
+  *nnz = (int64_t)c.size();
+  if (ro) {
+    if (!cols || !vals) return fail(ALIEN_HIP_EINVAL, "mtx_read_csr: NULL cols/vals");
+    std::copy(r.begin(), r.end(), ro);
+    std::copy(c.begin(), c.end(), cols);
+    std::copy(v.begin(), v.end(), vals);
+  }
+  API_END
+}
+
+int alien_hip_mtx_read_vector(const char* path, int* n, double* values) {
+  REQUIRE(path && n, "mtx_read_vector: bad arguments");
+  API_BEGIN
+  std::vector<double> v;
+  std::string err;
+  if (mtx_read_vector(path, v, err) != 0) return fail(ALIEN_HIP_EINVAL, "mtx_read_vector: " + err);
+  *n = (int)v.size();
+  if (values) std::copy(v.begin(), v.end(), values);
+  API_END
+}
+
+int alien_hip_mtx_write_csr(const char* path, int nrows, const int* ro, const int* cols, const double* vals,
+                            const char* description) {
+  REQUIRE(path && nrows >= 0 && ro && (nrows == 0 || (cols && vals)), "mtx_write_csr: bad arguments");
+  API_BEGIN
+  std::string err;
+  if (mtx_write_csr(path, nrows, ro, cols, vals, description, err) != 0)
+    return fail(ALIEN_HIP_EINVAL, "mtx_write_csr: " + err);
+  API_END
+}
+
+int alien_hip_mtx_write_vector(const char* path, int n, const double* values, const char* description) {
+  REQUIRE(path && n >= 0 && (n == 0 || values), "mtx_write_vector: bad arguments");
+  API_BEGIN
+  std::string err;
+  if (mtx_write_vector(path, n, values, description, err) != 0)
+    return fail(ALIEN_HIP_EINVAL, "mtx_write_vector: " + err);
+  API_END
+}
+
 // ---- device kernels -------------------------------------------------------------------
 #define CTX_OK(ctx) REQUIRE(ctx, "ctx is NULL")
 

@@ -277,6 +277,21 @@ int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_params* p, int
                         const double* b, double* x, alien_hip_solve_result* res);
 
 /* ------------------------------------------------------------------------ */
+/* MatrixMarket system I/O (host). Reading follows MatrixMarketReader        */
+/* (src/movesemantic/alien/move/impl/MatrixMarketReader.cc): "coordinate     */
+/* real" matrices, non-"general" symmetry mirrored, repeated entries summed, */
+/* CSR with ascending columns; "array real" one-column vectors. Writing      */
+/* follows MatrixMarketSystemWriter::dump (src/refsemantic/alien/ref/        */
+/* import_export/MatrixMarketSystemWriter.cc:34-140), values with 17 digits. */
+/* ------------------------------------------------------------------------ */
+/* Two calls: with row_offset (values) NULL only the sizes are returned. */
+int alien_hip_mtx_read_csr(const char* path, int* nrows, int64_t* nnz, int* row_offset, int* cols, double* vals);
+int alien_hip_mtx_read_vector(const char* path, int* n, double* values);
+int alien_hip_mtx_write_csr(const char* path, int nrows, const int* row_offset, const int* cols, const double* vals,
+                            const char* description);
+int alien_hip_mtx_write_vector(const char* path, int n, const double* values, const char* description);
+
+/* ------------------------------------------------------------------------ */
 /* Learned preconditioner M = G G^T (no counterpart in the reference)        */
 /* ------------------------------------------------------------------------ */
 /* A graph network (3 message-passing layers, hidden width 32, bf16 weights on

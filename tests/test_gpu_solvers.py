@@ -263,3 +263,18 @@ def test_bicgstab_filu0_matches_oracle(ctx, factor_niter, solver_niter):
     assert r["status"] == o["status"] == 1 and r["error"] == 0
     assert abs(r["iterations"] - o["iterations"]) <= 2
     assert r["true_rel_res"] < 1e-8
+
+
+def test_matrixmarket_system_solve(ctx):
+    """A system read from the reference's MatrixMarket example (mesh1em6, SPD,
+    symmetric storage) solved by fused CG and checked against the oracle."""
+    import pathlib
+    gold = pathlib.Path(__file__).resolve().parent / "golden"
+    ro, cols, vals = aa.mtx_read_csr(gold / "mesh1em6.mtx")
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, np.ones(n))
+    say("case MatrixMarket mesh1em6 CG")
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED, tol=1e-10)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_DIAG, tol=1e-10)
+    assert r["status"] == o["status"] == 1 and abs(r["iterations"] - o["iterations"]) <= 2
+    np.testing.assert_allclose(x, np.ones(n), rtol=1e-7)
