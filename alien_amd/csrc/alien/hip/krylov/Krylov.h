@@ -274,6 +274,71 @@ class DiagPreconditioner {
   VectorType m_inv_diag;
 };
 
+// Neumann polynomial preconditioner (NeumannPolyPreconditioner.h:85-175):
+// y = factor * sum_{k<=order} (I - factor A)^k x, evaluated as the reference
+// does (y = x; order times: m_y = y; m_y += -factor (A y); m_y += x; y = m_y;
+// then y *= factor). With factor == 0 the factor is 1 / |A|_2 from a power
+// method whose start vector comes from std::random_device, as in the
+// reference (non-deterministic; test_krylov passes a non-zero factor).
+template <typename AlgebraT>
+class NeumannPolyPreconditioner {
+ public:
+  typedef AlgebraT AlgebraType;
+  typedef typename AlgebraType::Matrix MatrixType;
+  typedef typename AlgebraType::Vector VectorType;
+
+  NeumannPolyPreconditioner(AlgebraType& alg, const MatrixType& A, double factor, int order, int factor_max_iter)
+  : m_alg(alg), m_A(A), m_factor(factor), m_order(order), m_factor_max_iter(factor_max_iter) {}
+  void init() {
+    m_alg.allocate(AlgebraType::resource(m_A), m_y, m_yy);
+    m_alg.assign(m_y, 0.);
+    m_alg.assign(m_yy, 0.);
+    if (m_factor == 0) computeFactor();
+  }
+  void update() {}
+  template <typename Alg>
+  void solve(Alg& alg, const VectorType& x, VectorType& y) const {
+    VectorType& my = const_cast<VectorType&>(m_y);
+    VectorType& myy = const_cast<VectorType&>(m_yy);
+    alg.copy(x, y);
+    for (int i = 0; i < m_order; ++i) {
+      alg.mult(m_A, y, myy); // evalG: yy = A y ; m_y = y - factor yy
+      alg.copy(y, my);
+      alg.axpy(-m_factor, myy, my);
+      alg.axpy(1., x, my);
+      alg.copy(my, y);
+    }
+    alg.scal(m_factor, y);
+  }
+  double factor() const { return m_factor; }
+
+ private:
+  void computeFactor() {
+    VectorType x;
+    m_alg.allocate(AlgebraType::resource(m_A), x);
+    std::random_device rd;
+    std::mt19937 gen(rd());
+    std::uniform_real_distribution<> dis(-1., 1.);
+    m_alg.assign(x, [&](int) { return dis(gen); });
+    m_alg.mult(m_A, x, m_y);
+    double nk = m_alg.norm2(m_y);
+    for (int i = 0; i < m_factor_max_iter; ++i) {
+      m_alg.copy(m_y, x);
+      m_alg.mult(m_A, x, m_y);
+      const double nn = m_alg.norm2(m_y);
+      m_factor = nn / nk;
+      nk = nn;
+    }
+    m_factor = 1. / m_factor;
+    m_alg.free(x);
+  }
+  AlgebraType& m_alg;
+  const MatrixType& m_A;
+  double m_factor;
+  int m_order, m_factor_max_iter;
+  VectorType m_y, m_yy;
+};
+
 // Chebyshev polynomial preconditioner, non-Saad variant. Quirks kept on
 // purpose (they decide the iteration counts): the interval is [alpha, beta]
 // with alpha overwritten by beta/factor when factor != 0; the apply starts

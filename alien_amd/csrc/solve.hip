@@ -728,8 +728,8 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
     if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
     if (p->stall_window < 0) throw Error(ALIEN_HIP_EINVAL, "solve_csr: stall_window < 0");
-    if (p->precond == ALIEN_HIP_PREC_FILU0 && (fused || mixed))
-      throw Error(ALIEN_HIP_EINVAL, "FILU0 runs in the REFERENCE mode");
+    if ((p->precond == ALIEN_HIP_PREC_FILU0 || p->precond == ALIEN_HIP_PREC_NEUMANN) && (fused || mixed))
+      throw Error(ALIEN_HIP_EINVAL, "FILU0 and Neumann run in the REFERENCE mode");
     if (p->precond == ALIEN_HIP_PREC_FILU0 && (p->filu_factor_niter < 0 || p->filu_solver_niter < 0))
       throw Error(ALIEN_HIP_EINVAL, "FILU0: negative iteration counts");
     if (p->precond == ALIEN_HIP_PREC_CHEB && p->poly_order < 1)
@@ -784,6 +784,13 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       }
       case ALIEN_HIP_PREC_ILU0: {
         Alien::ILU0Preconditioner<Alg> P(alg, A);
+        P.init();
+        res->setup_s = tm.stop();
+        run_reference(alg, p, P, A, b, x, res, tm);
+        break;
+      }
+      case ALIEN_HIP_PREC_NEUMANN: {
+        Alien::NeumannPolyPreconditioner<Alg> P(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter);
         P.init();
         res->setup_s = tm.stop();
         run_reference(alg, p, P, A, b, x, res, tm);

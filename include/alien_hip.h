@@ -217,8 +217,10 @@ enum { ALIEN_HIP_CG = 0, ALIEN_HIP_BICGS = 1 };
 enum { ALIEN_HIP_PREC_DIAG = 0, ALIEN_HIP_PREC_CHEB = 1, ALIEN_HIP_PREC_ILU0 = 2,
        ALIEN_HIP_PREC_NONE = 3,
        ALIEN_HIP_PREC_LEARNED = 4, /* M = G G^T from the graph network (FUSED/GRAPH/MIXED modes) */
-       ALIEN_HIP_PREC_FILU0 = 5 /* FILU0Preconditioner (REFERENCE mode): fixed-point ILU(0) factor,
-                                   Jacobi triangular sweeps */ };
+       ALIEN_HIP_PREC_FILU0 = 5, /* FILU0Preconditioner (REFERENCE mode): fixed-point ILU(0) factor,
+                                    Jacobi triangular sweeps */
+       ALIEN_HIP_PREC_NEUMANN = 6 /* NeumannPolyPreconditioner (REFERENCE mode): poly_factor,
+                                     poly_order, poly_max_iter */ };
 /* RB-CSR: row-bundled CSR; RB-DIA: row-bundled diagonal offsets (no column
  * stream; RB-CSR when it would store over 1.25x the slots). In the FUSED, GRAPH
  * and MIXED modes layout RBCSR forces RB-CSR, any other value selects RB-DIA. */

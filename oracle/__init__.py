@@ -16,7 +16,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 ORC_CG, ORC_BICGS = 0, 1
-PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_FILU0 = 0, 1, 2, 3, 4
+PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_FILU0, PREC_NEUMANN = 0, 1, 2, 3, 4, 5
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)

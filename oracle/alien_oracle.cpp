@@ -703,6 +703,29 @@ struct FILU0Prec : Precond {
   }
 };
 
+/* NeumannPolyPreconditioner (NeumannPolyPreconditioner.h:85-175), factor != 0 (the
+ * factor == 0 power method starts from std::random_device: not restated) */
+struct NeumannPrec : Precond {
+  double factor;
+  int order;
+  Vec my, myy;
+  NeumannPrec(const Algebra& alg, double f, int o) : factor(f), order(o) {
+    my.assign(alg.n, 0.0);
+    myy.assign(alg.n, 0.0);
+  }
+  void solve(const Algebra& alg, const Vec& x, Vec& y) override {
+    alg.copy(x, y);
+    for (int i = 0; i < order; ++i) {
+      alg.mult(y, myy); /* evalG */
+      alg.copy(y, my);
+      alg.axpy(-factor, myy, my);
+      alg.axpy(1., x, my);
+      alg.copy(my, y);
+    }
+    alg.scal(factor, y);
+  }
+};
+
 int orc_solve(const orc_solve_params* p, int n, const int* ro, const int* cols, const double* vals,
               const double* bptr, double* xptr, orc_solve_result* res) {
   std::memset(res, 0, sizeof(*res));
@@ -721,6 +744,7 @@ int orc_solve(const orc_solve_params* p, int n, const int* ro, const int* cols, 
     break;
   case ORC_PREC_ILU0: P = new ILU0Prec(alg); break;
   case ORC_PREC_FILU0: P = new FILU0Prec(alg, p->filu_factor_niter, p->filu_solver_niter); break;
+  case ORC_PREC_NEUMANN: P = new NeumannPrec(alg, p->poly_factor, p->poly_order); break;
   default: P = new NonePrec(); break;
   }
   double t1 = now_s();

@@ -77,7 +77,8 @@ int  orc_color_greedy(int nrows, const int* row_offset, const int* cols, int* co
  *               DiagPreconditioner.h, ChebyshevPreconditioner.h:73-202,392-444,
  *               ILU0Preconditioner.h) ------------------------------------- */
 enum { ORC_CG = 0, ORC_BICGS = 1 };
-enum { ORC_PREC_DIAG = 0, ORC_PREC_CHEB = 1, ORC_PREC_ILU0 = 2, ORC_PREC_NONE = 3, ORC_PREC_FILU0 = 4 };
+enum { ORC_PREC_DIAG = 0, ORC_PREC_CHEB = 1, ORC_PREC_ILU0 = 2, ORC_PREC_NONE = 3, ORC_PREC_FILU0 = 4,
+       ORC_PREC_NEUMANN = 5 };
 
 typedef struct {
   int    solver;        /* ORC_CG / ORC_BICGS */

@@ -278,3 +278,24 @@ def test_matrixmarket_system_solve(ctx):
     _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_DIAG, tol=1e-10)
     assert r["status"] == o["status"] == 1 and abs(r["iterations"] - o["iterations"]) <= 2
     np.testing.assert_allclose(x, np.ones(n), rtol=1e-7)
+
+
+@pytest.mark.parametrize("layout", [aa.LAYOUT_CSR, aa.LAYOUT_RBDIA])
+def test_cg_neumann_matches_oracle(ctx, layout):
+    """NeumannPolyPreconditioner (REFERENCE mode, the reference's call sequence)
+    against the oracle on the 7-point Laplacian: iterations within +-2."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 16)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case CG Neumann", layout)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_NEUMANN, layout=layout, tol=1e-8,
+                        max_iter=1000, poly_factor=0.125, poly_order=3)
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_NEUMANN, tol=1e-8, max_iter=1000,
+                     poly_factor=0.125, poly_order=3)
+    say("  gpu", r["iterations"], "oracle", o["iterations"])
+    assert r["status"] == o["status"] == 1 and abs(r["iterations"] - o["iterations"]) <= 2
+    assert r["true_rel_res"] < 1e-8
+    # factor 0: the power method's start vector comes from std::random_device (as the reference)
+    _, r0 = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_NEUMANN, layout=layout, tol=1e-8,
+                         max_iter=1000, poly_factor=0.0, poly_order=3, poly_max_iter=20)
+    assert r0["status"] == 1

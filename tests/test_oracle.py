@@ -352,3 +352,25 @@ def test_filu0_solver_sweeps_converge_to_ilu0():
     _, r3 = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_FILU0, tol=1e-8,
                       filu_factor_niter=2, filu_solver_niter=3)
     assert r3["status"] == 1 and r3["iterations"] >= ri["iterations"]
+
+
+def test_neumann_polynomial_closed_form():
+    """NeumannPolyPreconditioner (NeumannPolyPreconditioner.h:85-175): the apply
+    equals factor * sum_{k<=order} (I - factor A)^k x (dense check), and CG with
+    it converges on the 7-point Laplacian for a factor below 2 / lambda_max."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 4)
+    n = len(ro) - 1
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, cols[ro[i]:ro[i + 1]]] = vals[ro[i]:ro[i + 1]]
+    f, order = 0.125, 3
+    S = sum(np.linalg.matrix_power(np.eye(n) - f * A, k) for k in range(order + 1))
+    b = aa.uniform01(3, n)
+    # one CG step with M = f S from x0 = 0: x1 = (r.z / z.A z) z, z = f S b
+    x1, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_NEUMANN, tol=1e-30, max_iter=1,
+                      poly_factor=f, poly_order=order)
+    z = f * S @ b
+    np.testing.assert_allclose(x1, (b @ z) / (z @ A @ z) * z, rtol=1e-12)
+    _, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_NEUMANN, tol=1e-10, max_iter=500,
+                     poly_factor=f, poly_order=order)
+    assert r["status"] == 1
