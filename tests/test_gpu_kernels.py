@@ -303,6 +303,35 @@ def test_full_size_spmv_properties_256(ctx):
     call("alien_hip_spmv_rbcsr_f64", ctx.h, n, p(dbo), p(bc), p(bv), p(x), p(y2))
     ctx.synchronize()
     assert torch.equal(y1, y2)
+    del bc, bv
+    # RB-DIA at full size: same sums bit for bit, FP64 and (against RB-CSR) FP32
+    dbo_d, doff, dmask = aa.rbdia_build(ro, cols)
+    assert int(dbo_d[-1]) * 64 <= 1.01 * len(vals) + 64 * len(dbo_d)
+    ddbo, ddoff, ddm = dev(dbo_d), dev(doff), dev(dmask.view(np.int64))
+    dvv = empty(int(dbo_d[-1]) * 64)
+    call("alien_hip_rbdia_fill_f64", ctx.h, n, p(dro), p(dc), p(dv), p(ddbo), p(ddoff), p(dvv))
+    y3 = empty(n)
+    call("alien_hip_spmv_rbdia_f64", ctx.h, n, p(ddbo), p(ddoff), p(ddm), p(dvv), p(x), p(y3))
+    ctx.synchronize()
+    assert torch.equal(y1, y3)
+
+
+def test_full_size_learned_factor_closed_form_256(ctx):
+    """256^3 log-normal system: with the initial parameters the network's output
+    is exactly the first-order factor, Ghat_ii = exp(0) = 1 and Ghat_ij = e_ij =
+    -a_ij / sqrt(d_i d_j) (rounded to FP32 as the kernel does), for all 60M lower
+    entries: a size-independent check of the full forward at its real shape."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 256, params=[2.0, 1, 1, 1], seed=42)
+    n = len(ro) - 1
+    say("case full-size learned factor closed form", n)
+    gh = aa.gnn_forward(ctx, ro, cols, vals, aa.gnn_init(0))
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(ro))
+    d = vals[orc.dcol(ro, cols)]
+    dis = (1.0 / np.sqrt(d)).astype(np.float32).astype(np.float64)
+    e = ((-vals * dis[rows]) * dis[cols]).astype(np.float32)
+    ref = np.where(cols < rows, e, np.float32(0.0))
+    ref[cols == rows] = 1.0
+    np.testing.assert_array_equal(gh, ref)
 
 
 def test_filu0_factor_sweeps_bit_exact(ctx):
