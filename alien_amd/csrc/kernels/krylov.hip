@@ -206,24 +206,58 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_guarded(int n, RBMat<T> A
 }
 
 // Factored preconditioner M = G G^T, first half: t = G^T r and rho1 = t.t
-// (= r.M r, the r.z of CG.h:124 with z = G t, without forming z)
+// (= r.M r, the r.z of CG.h:124 with z = G t, without forming z), plus r.r:
+// the stop test of the previous update is taken here (its finalize runs
+// kFinXRDiag), so the x/r update itself needs no reduction.
 template <typename T>
 __global__ void __launch_bounds__(kRowBlock) k_factor_t(int n, RBMat<T> GT, const T* __restrict__ r,
                                                          T* __restrict__ t, CGState* st, RedWork w) {
   if (st->done) return;
   __shared__ double lds4[kRowBlock / kWave];
-  double s = 0.0;
+  double srr = 0.0, stt = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T v = rb_sum<T>(GT, b, [&](int c) { return r[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    const T rv = r[min(row, n - 1)];
+    const T v = rb_sum<T>(GT, b, [&](int c) { return r[c]; });
     if (row < n) {
       t[row] = v;
-      s += (double)v * (double)v;
+      srr += (double)rv * (double)rv;
+      stt += (double)v * (double)v;
     }
   }
-  double v[1] = {block_sum<kRowBlock>(s, lds4)};
-  store_partials<1>(v, w);
+  double v[2];
+  v[0] = block_sum<kRowBlock>(srr, lds4);
+  __syncthreads();
+  v[1] = block_sum<kRowBlock>(stt, lds4);
+  store_partials<2>(v, w);
+}
+
+// x += alpha p ; r += (-alpha) q without a reduction (the learned-M iteration
+// takes its stop test in the next k_factor_t)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_update(int n, const T* __restrict__ p, const T* __restrict__ q,
+                                                       T* __restrict__ x, T* __restrict__ r, const CGState* st) {
+  if (st->done) return;
+  const T a = (T)(st->rho1 / st->pq);
+  T pv[kEPT], qv[kEPT], xv[kEPT], rv[kEPT];
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    const bool ok = i < n;
+    pv[k] = ok ? p[i] : T(0);
+    qv[k] = ok ? q[i] : T(0);
+    xv[k] = ok ? x[i] : T(0);
+    rv[k] = ok ? r[i] : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) {
+      x[i] = xv[k] + a * pv[k];
+      r[i] = rv[k] + (-a) * qv[k];
+    }
+  }
 }
 
 // Second half fused with the direction update: z_i = (G t)_i is formed in
@@ -372,10 +406,16 @@ void launch_factor_apply(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> G
 
 template <typename T>
 void launch_factor_p(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* t, T* p,
-                     CGState* st, int first) {
+                     CGState* st, int first, int rho1_only) {
   hipLaunchKernelGGL(k_factor_t<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, GT, r, t, st, w);
-  launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
+  // first step (or a resumed one): rho1 only; later: rho = rho1, ++iter and the stop test of the
+  // previous update, rho1 = t.t
+  launch_finalize(s, w, rblocks<T>(n), 2, (first || rho1_only) ? kFinRho1B : kFinXRDiag, nullptr, st);
   hipLaunchKernelGGL(k_factor_p<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, G, t, p, (const CGState*)st, first);
+}
+template <typename T>
+void launch_cg_update(hipStream_t s, int n, const T* p, const T* q, T* x, T* r, CGState* st) {
+  hipLaunchKernelGGL(k_cg_update<T>, egrid(n), dim3(kBlock), 0, s, n, p, q, x, r, (const CGState*)st);
 }
 
 void launch_fold_increment(hipStream_t s, int n, float* x32, double* x) {
@@ -394,8 +434,9 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
   template void launch_factor_apply<T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,  \
                                        CGState*);                                                        \
   template void launch_factor_p<T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,     \
-                                   CGState*, int);                                                       \
+                                   CGState*, int, int);                                                  \
   template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*);          \
+  template void launch_cg_update<T>(hipStream_t, int, const T*, const T*, T*, T*, CGState*);            \
   template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, T*, T*, const T*, T*,     \
                                 CGState*);
 ALIEN_HIP_INST_KRYLOV(double)

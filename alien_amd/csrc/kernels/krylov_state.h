@@ -91,10 +91,14 @@ template <typename T>
 void launch_cg_xr(hipStream_t, RedWork w, int n, const T* p, const T* q, T* x, T* r, const T* invd_or_null,
                   T* z_or_null, CGState* st);
 // z = G (G^T r) and rho1 = r.z (factored approximate inverse M = G G^T)
-// M = G G^T fused with the direction update: t = G^T r, rho1 = t.t, p = G t + (rho1/rho) p
+// x += alpha p, r -= alpha q with no reduction (learned-M iteration)
+template <typename T>
+void launch_cg_update(hipStream_t, int n, const T* p, const T* q, T* x, T* r, CGState* st);
+// M = G G^T fused with the direction update: t = G^T r, rho1 = t.t, p = G t + (rho1/rho) p;
+// after the first step its finalize also completes the previous update (++iter, stop test on r.r)
 template <typename T>
 void launch_factor_p(hipStream_t, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* t, T* p,
-                     CGState* st, int first);
+                     CGState* st, int first, int rho1_only = 0);
 template <typename T>
 void launch_factor_apply(hipStream_t, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* tmp, T* z,
                          CGState* st);

@@ -13,7 +13,7 @@ namespace alien_hip {
 struct RedWork;
 
 // what the second stage of a reduction does with the totals (reduce.hip)
-enum FinOp { kFinStore = 0, kFinRho1 = 1, kFinPQ = 2, kFinXR = 3, kFinXRDiag = 4 };
+enum FinOp { kFinStore = 0, kFinRho1 = 1, kFinPQ = 2, kFinXR = 3, kFinXRDiag = 4, kFinRho1B = 5 };
 void launch_finalize(hipStream_t, RedWork w, long nblk, int nslot, int op, double* out, CGState* st);
 // the same second stage for the BiCGStab state (reduce.hip)
 enum BFinOp { kBFinAlpha = 0, kBFinS = 1, kBFinOmega = 2, kBFinXR = 3 };

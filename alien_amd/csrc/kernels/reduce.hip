@@ -60,6 +60,9 @@ __global__ void __launch_bounds__(kFin) k_finalize(RedWork w, int nblk, int nslo
   case kFinRho1:
     st->rho1 = tot[0];
     break;
+  case kFinRho1B: // rho1 from the second slot (the first holds r.r, unused here)
+    st->rho1 = tot[1];
+    break;
   case kFinPQ:
     st->pq = tot[0];
     if (tot[0] == 0) { // CG breakdown (p.q == 0), handled by the host

@@ -212,9 +212,13 @@ struct FusedCG {
     hipStream_t s = ctx->stream;
     const bool diag = P.kind == kPrecDiag;
     if (P.kind == kPrecFactor) {
-      // z = G (G^T r) never stored: rho1 = |G^T r|^2 and p = G t + beta p in one pass
+      // z = G (G^T r) never stored: rho1 = |G^T r|^2 and p = G t + beta p; the stop test of this
+      // update runs in the next iteration's first finalize, so the update has no reduction
       launch_factor_p<T>(s, ctx->red(), n, P.g<T>(), P.gt<T>(), r.data(), tmp.data(), p.data(), st.data(),
                          first ? 1 : 0);
+      launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
+      launch_cg_update<T>(s, n, p.data(), q.data(), x, r.data(), st.data());
+      return;
     } else {
       if (!diag || first) precond(r.data(), z.data());
       launch_cg_p<T>(s, n, z.data(), p.data(), st.data(), first ? 1 : 0);
@@ -223,9 +227,21 @@ struct FusedCG {
     launch_cg_xr<T>(s, ctx->red(), n, p.data(), q.data(), x, r.data(), diag ? invd : nullptr,
                     diag ? z.data() : nullptr, st.data());
   }
-  // After r has been replaced from outside (reliable update): refresh z and rho1.
+  // After r has been replaced from outside (reliable update) at the end of an
+  // iteration: refresh z and rho1 (Jacobi; the learned M recomputes them in its
+  // next step).
   void residual_replaced() {
     if (P.kind == kPrecDiag) precond(r.data(), z.data());
+  }
+  // Learned M only: the solve stopped inside an iteration (the stop test runs in
+  // the next step's first kernel, after rho and the count advanced) and r was
+  // replaced; finish that iteration with rho1 recomputed from the new r.
+  void resume_inside_step(T* x) {
+    hipStream_t s = ctx->stream;
+    launch_factor_p<T>(s, ctx->red(), n, P.g<T>(), P.gt<T>(), r.data(), tmp.data(), p.data(), st.data(), 0, 1);
+    launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
+    launch_cg_update<T>(s, n, p.data(), q.data(), x, r.data(), st.data());
+    check_launch("learned CG resumed step");
   }
   CGState read_state() {
     ALIEN_HIP_CHECK(hipMemcpyAsync(h_st, st.data(), sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
@@ -299,7 +315,7 @@ struct FusedCG {
     CGState cur = read_state();
     // without a graph, poll after every iteration (the reference's host-side stop test)
     const int k = graph ? std::max(1, poll_every) : 1;
-    double rr_ref = cur.rr;
+    double rr_ref = INFINITY; // the first measured residual sets the reference
     int it_ref = cur.it;
     while (!cur.done) {
       cur = advance(x, graph, k);
@@ -440,11 +456,13 @@ MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMa
     launch_demote(s, n, r64.data(), F.r.data());
     rr_last = rr64;
     it_last = cur.it;
+    const bool inside = cur.done && ps.kind == kPrecFactor; // FP32 stop test fired at a step's start
     cur.done = 0;
     cur.status = 0;
     cur.rr = rr64;
     F.write_state(cur);
-    F.residual_replaced();
+    if (inside) F.resume_inside_step(x32.data());
+    else F.residual_replaced();
     cur = F.advance(x32.data(), graph, k);
   }
   out.it = cur.it;
