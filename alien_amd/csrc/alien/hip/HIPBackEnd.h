@@ -133,25 +133,38 @@ class HIPMatrix {
   static constexpr int kBEllPackSlice = 1024; // BEllPackStructInfo<1024,int>
 
   HIPMatrix() = default;
-  HIPMatrix(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR) { build(ctx, a, spmv_layout); }
+  HIPMatrix(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR, bool borrow_host = false) {
+    build(ctx, a, spmv_layout, borrow_host);
+  }
 
   // Upload a host CSR (ascending columns) and build the requested SpMV layout:
-  // the body of the SimpleCSR -> hip matrix converter.
-  void build(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR) {
+  // the body of the SimpleCSR -> hip matrix converter. The host profile is
+  // copied, or read in place when borrow_host says the caller's arrays outlive
+  // the matrix.
+  void build(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR, bool borrow_host = false) {
     m_ctx = ctx;
     m_n = a.nrows;
     m_nnz = a.row_offset[m_n];
     m_ctx->reserve_for(m_n);
-    m_kcol.assign(a.row_offset, a.row_offset + m_n + 1);
-    m_cols.assign(a.cols, a.cols + m_nnz);
-    m_vals.assign(a.vals, a.vals + m_nnz);
+    if (borrow_host) {
+      h_ro = a.row_offset;
+      h_cols = a.cols;
+      h_vals = a.vals;
+    } else {
+      m_kcol.assign(a.row_offset, a.row_offset + m_n + 1);
+      m_cols.assign(a.cols, a.cols + m_nnz);
+      m_vals.assign(a.vals, a.vals + m_nnz);
+      h_ro = m_kcol.data();
+      h_cols = m_cols.data();
+      h_vals = m_vals.data();
+    }
     m_dcol.resize(m_n);
-    alien_hip::host_dcol(m_n, m_kcol.data(), m_cols.data(), m_dcol.data());
+    alien_hip::host_dcol(m_n, h_ro, h_cols, m_dcol.data());
     hipStream_t st = m_ctx->stream;
-    d_ro.upload(m_kcol.data(), m_kcol.size(), st);
-    d_cols.upload(m_cols.data(), m_cols.size(), st);
+    d_ro.upload(h_ro, m_n + 1, st);
+    d_cols.upload(h_cols, m_nnz, st);
     d_dcol.upload(m_dcol.data(), m_dcol.size(), st);
-    d_vals.upload(m_vals.data(), m_vals.size(), st);
+    d_vals.upload(h_vals, m_nnz, st);
     setSpmvLayout(spmv_layout);
     ALIEN_HIP_CHECK(hipStreamSynchronize(st));
   }
@@ -164,10 +177,12 @@ class HIPMatrix {
   int spmvLayout() const { return m_layout; }
 
   // host profile (CSRStructInfo: kcol/cols/dcol)
-  const int* kcol() const { return m_kcol.data(); }
-  const int* cols() const { return m_cols.data(); }
+  const int* kcol() const { return h_ro; }
+  const int* cols() const { return h_cols; }
   const int* dcol() const { return m_dcol.data(); }
-  const double* hostValues() const { return m_vals.data(); }
+  const double* hostValues() const { return h_vals; }
+  // host bundle offsets of the row-bundled layout (after ensureRB / ensureRBCSR)
+  const std::vector<int>& bundleOffsets() const { return m_bo; }
 
   // Builds the layout on first use and selects it for mult().
   void setSpmvLayout(int layout) {
@@ -178,25 +193,27 @@ class HIPMatrix {
     m_layout = layout;
   }
 
-  // Row-bundled layout of the fused solvers: RB-DIA (no column stream) when
-  // allowed and it stores at most kRbdiaMaxGrowth times the RB-CSR slots,
-  // RB-CSR otherwise. FP32 values follow with ensureRB32().
+  // Row-bundled layout of the fused solvers: RB-DIA (no column stream, its
+  // structure built on the device) when allowed and it stores at most
+  // kRbdiaMaxGrowth times the RB-CSR slots, RB-CSR otherwise. FP32 values
+  // follow with ensureRB32().
   void ensureRB(bool allow_dia) {
     if (!d_bo.empty() && (allow_dia || !m_rbdia)) return;
     if (!d_bo.empty()) freeRB();
     if (allow_dia) {
       const int nb = (m_n + alien_hip::kBundle - 1) / alien_hip::kBundle;
       std::vector<int> csr_bo(nb + 1);
-      const bool csr_ok = alien_hip::rbcsr_bundle_offsets(m_n, m_kcol.data(), csr_bo.data()) == 0;
-      std::vector<int> doff;
-      std::vector<uint64_t> dmask;
-      const long slots = alien_hip::rbdia_structure(m_n, m_kcol.data(), m_cols.data(), nullptr, nullptr,
-                                                    alien_hip::kRbdiaMaxWidth, m_bo, doff, dmask);
+      const bool csr_ok = alien_hip::rbcsr_bundle_offsets(m_n, h_ro, csr_bo.data()) == 0;
+      hipStream_t st = m_ctx->stream;
+      const long slots = alien_hip::rbdia_count_device(st, m_n, d_ro.data(), d_cols.data(),
+                                                       alien_hip::kRbdiaMaxWidth, m_bo);
       if (slots >= 0 && (!csr_ok || slots <= alien_hip::kRbdiaMaxGrowth * csr_bo[nb])) {
-        hipStream_t st = m_ctx->stream;
         d_bo.upload(m_bo.data(), m_bo.size(), st);
-        d_doff.upload(doff.data(), doff.size(), st);
-        d_dmask.upload(reinterpret_cast<const unsigned long long*>(dmask.data()), dmask.size(), st);
+        d_doff.resize(slots);
+        d_dmask.resize(slots);
+        alien_hip::launch_rbdia_struct(st, m_n, d_ro.data(), d_cols.data(), d_bo.data(), d_doff.data(),
+                                       d_dmask.data());
+        alien_hip::check_launch("rbdia_struct");
         d_bv.resize((size_t)slots * alien_hip::kBundle);
         alien_hip::launch_rbdia_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
                                              d_doff.data(), d_bv.data());
@@ -243,7 +260,7 @@ class HIPMatrix {
     if (!d_bo.empty()) freeRB();
     const int nb = (m_n + alien_hip::kBundle - 1) / alien_hip::kBundle;
     m_bo.resize(nb + 1);
-    if (alien_hip::rbcsr_bundle_offsets(m_n, m_kcol.data(), m_bo.data()) != 0)
+    if (alien_hip::rbcsr_bundle_offsets(m_n, h_ro, m_bo.data()) != 0)
       throw alien_hip::Error(ALIEN_HIP_EINVAL, "RB-CSR storage exceeds int32 indexing");
     hipStream_t st = m_ctx->stream;
     d_bo.upload(m_bo.data(), m_bo.size(), st);
@@ -258,11 +275,11 @@ class HIPMatrix {
     if (d_bro.empty()) {
       const int ns = alien_hip::bellpack_nslices(m_n, kBEllPackSlice);
       m_bro.resize(ns + 1);
-      alien_hip::bellpack_bro(m_n, kBEllPackSlice, m_kcol.data(), m_bro.data());
+      alien_hip::bellpack_bro(m_n, kBEllPackSlice, h_ro, m_bro.data());
       const size_t total = (size_t)m_bro[ns] * kBEllPackSlice;
       std::vector<int> bc(total);
       std::vector<double> bv(total);
-      alien_hip::bellpack_fill(m_n, kBEllPackSlice, m_kcol.data(), m_cols.data(), m_vals.data(), m_bro.data(),
+      alien_hip::bellpack_fill(m_n, kBEllPackSlice, h_ro, h_cols, h_vals, m_bro.data(),
                                bc.data(), bv.data(), nullptr, nullptr);
       hipStream_t st = m_ctx->stream;
       d_bro.upload(m_bro.data(), m_bro.size(), st);
@@ -275,7 +292,7 @@ class HIPMatrix {
       const size_t total = (size_t)m_bro[ns] * kBEllPackSlice;
       std::vector<int> bc(total);
       std::vector<uint8_t> lm(total), um(total);
-      alien_hip::bellpack_fill(m_n, kBEllPackSlice, m_kcol.data(), m_cols.data(), nullptr, m_bro.data(),
+      alien_hip::bellpack_fill(m_n, kBEllPackSlice, h_ro, h_cols, nullptr, m_bro.data(),
                                bc.data(), nullptr, lm.data(), um.data());
       d_lmask.upload(lm.data(), lm.size(), m_ctx->stream);
       d_umask.upload(um.data(), um.size(), m_ctx->stream);
@@ -302,6 +319,9 @@ class HIPMatrix {
   int m_layout = kLayoutCSR;
   std::vector<int> m_kcol, m_cols, m_dcol, m_bo, m_bro;
   std::vector<double> m_vals;
+  const int* h_ro = nullptr;   // host profile: the copies above or the caller's arrays
+  const int* h_cols = nullptr;
+  const double* h_vals = nullptr;
 };
 
 // Future of a device-resident scalar: dot(x, y, f) leaves the result on the

@@ -41,7 +41,8 @@ int rbcsr_bundle_offsets(int nrows, const int* ro, int* bundle_off);
 // RB-DIA structure of the entries [kb(r), ke(r)) of each row (NULL kb/ke: whole
 // rows): bundle offsets (nb + 1), per-slot column offsets and row masks.
 // Returns the slot count per lane summed over bundles, or -1 when a bundle has
-// more than max_w distinct offsets or int32 indexing would overflow.
+// more than max_w distinct offsets, a row's columns are not strictly ascending
+// or int32 indexing would overflow.
 long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, const int* ke, int max_w,
                      std::vector<int>& bo, std::vector<int>& doff, std::vector<uint64_t>& dmask);
 // ---- MatrixMarket system I/O (host/mtx.cpp) ----

@@ -226,6 +226,34 @@ int alien_hip_rbdia_struct(int nrows, const int* ro, const int* cols, int max_w,
   API_END
 }
 
+int64_t alien_hip_rbdia_count_device(alien_hip_ctx* ctx, int nrows, const int* ro, const int* cols, int max_w,
+                                     int* bo) {
+  if (!(ctx && nrows >= 0 && (nrows == 0 || (ro && cols)) && bo && max_w > 0))
+    return fail(ALIEN_HIP_EINVAL, "rbdia_count_device: bad arguments"), -1;
+  try {
+    std::vector<int> b;
+    const long slots = rbdia_count_device(ctx->stream, nrows, ro, cols, max_w, b);
+    if (slots < 0)
+      return fail(ALIEN_HIP_EINVAL, "rbdia_count_device: too many offsets in a bundle or columns not ascending"), -1;
+    std::copy(b.begin(), b.end(), bo);
+    return slots;
+  } catch (const Error& e) {
+    return fail(e.code, e.what()), -1;
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what()), -1;
+  }
+}
+
+int alien_hip_rbdia_struct_device(alien_hip_ctx* ctx, int nrows, const int* ro, const int* cols, const int* bo,
+                                  int* doff, uint64_t* dmask) {
+  REQUIRE(ctx, "ctx is NULL");
+  REQUIRE(nrows >= 0 && (nrows == 0 || (ro && cols && bo && doff && dmask)), "rbdia_struct_device: bad arguments");
+  API_BEGIN
+  launch_rbdia_struct(ctx->stream, nrows, ro, cols, bo, doff, reinterpret_cast<unsigned long long*>(dmask));
+  LAUNCHED("rbdia_struct_device");
+  API_END
+}
+
 int alien_hip_mtx_read_csr(const char* path, int* nrows, int64_t* nnz, int* ro, int* cols, double* vals) {
   REQUIRE(path && nrows && nnz, "mtx_read_csr: bad arguments");
   API_BEGIN

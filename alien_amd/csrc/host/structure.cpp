@@ -161,9 +161,10 @@ int rbcsr_bundle_offsets(int nrows, const int* ro, int* bundle_off) {
 // RB-DIA: for each bundle of 64 rows the ascending distinct column offsets
 // c - r of the entries k in [kb(r), ke(r)) of its rows (kb = ro[r], ke =
 // ro[r+1] unless given), and for each (bundle, offset) the 64-bit mask of the
-// rows that hold it. Columns ascend inside a row, so a row's offsets ascend
-// too and each bundle's set is a merge of sorted lists; consecutive rows of a
-// stencil matrix repeat the same list, which is detected and skipped.
+// rows that hold it. Columns must ascend strictly inside a row (-1 otherwise),
+// so a row's offsets ascend too and each bundle's set is a merge of sorted
+// lists; consecutive rows of a stencil matrix repeat the same list, which is
+// detected and skipped.
 long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, const int* ke, int max_w,
                      std::vector<int>& bo, std::vector<int>& doff, std::vector<uint64_t>& dmask) {
   const int nb = (nrows + kBundle - 1) / kBundle;
@@ -180,7 +181,10 @@ long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, c
     for (int r = r0; r < r1; ++r) {
       const int k0 = kb ? kb[r] : ro[r], k1 = ke ? ke[r] : ro[r + 1];
       row_off.clear();
-      for (int k = k0; k < k1; ++k) row_off.push_back(cols[k] - r);
+      for (int k = k0; k < k1; ++k) {
+        if (k > k0 && cols[k] <= cols[k - 1]) return -1; // columns not strictly ascending
+        row_off.push_back(cols[k] - r);
+      }
       if (row_off == prev) continue;
       tmp.clear();
       std::set_union(set.begin(), set.end(), row_off.begin(), row_off.end(), std::back_inserter(tmp));
@@ -197,8 +201,8 @@ long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, c
       for (int k = k0; k < k1; ++k) {
         const int d = cols[k] - r;
         while (j < doff.size() && doff[j] != d) ++j;
-        if (j == doff.size()) return -1; // columns not ascending
-        dmask[j] |= 1ull << (r - r0);
+        if (j == doff.size()) return -1;
+        dmask[j++] |= 1ull << (r - r0);
       }
     }
     acc += (long)set.size();

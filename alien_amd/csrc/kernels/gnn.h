@@ -38,15 +38,40 @@ constexpr int kMaxCgSteps = 200;
 namespace alien_hip {
 namespace gnn {
 
+// A system read in place: host CSR with the diagonal positions, the same
+// arrays on the device and, when the system is stored as RB-DIA, its host
+// bundle offsets and device offsets / lane masks (the factors' structures are
+// then the system's split by the sign of the offset).
+struct SystemView {
+  int n = 0;
+  const int* ro = nullptr;
+  const int* cols = nullptr;
+  const int* dcol = nullptr;
+  const double* vals = nullptr;
+  const int* d_ro = nullptr;
+  const int* d_cols = nullptr;
+  const int* d_dcol = nullptr;
+  const double* d_vals = nullptr;
+  const int* rb_bo = nullptr;
+  const int* d_rb_doff = nullptr;
+  const unsigned long long* d_rb_dmask = nullptr;
+};
+
 // Device state of the network applied to one matrix (SPD, symmetric pattern,
 // stored positive diagonal): scaled couplings, features, activations, Ghat.
 class Engine {
  public:
+  // copies the host CSR
   Engine(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals);
+  // reads the system in place: its arrays must outlive the engine
+  Engine(alien_hip_ctx* ctx, const SystemView& v);
   int n() const { return m_n; }
   int nnz() const { return m_nnz; }
-  // activations and Ghat (per CSR entry; upper entries 0) for the parameters P (device)
+  // activations of every layer and Ghat (per CSR entry; upper entries 0) for
+  // the parameters P (device), as loss_grad needs them
   void forward(const float* P);
+  // Ghat only, through two activation buffers (the solver's set-up)
+  void infer(const float* P);
   // loss = L1 + beta * probe term on 4 probes V (device, N x 4); L1 = Kaporin's condition number
   // (cg_steps = 0) or the residual reduction after cg_steps PCG steps with V as right-hand sides;
   // grad (device, kNParam); parts = {L1, probe term}
@@ -55,6 +80,8 @@ class Engine {
   // Row-bundled layouts (RB-DIA or RB-CSR) of G and G^T and their values (after forward)
   void build_factor_layout();
   template <typename T> void fill_factor(T* g_vals, T* gt_vals);
+  // frees everything but the factors' structures (after fill_factor)
+  void release_network_buffers();
   size_t g_slots() const { return m_g.slots; }
   size_t gt_slots() const { return m_gt.slots; }
   bool factor_is_dia() const { return m_g.dia && m_gt.dia; }
@@ -63,8 +90,12 @@ class Engine {
 
  private:
   double cg_term(const float* B, int k, float* gg);
+  void init(const SystemView& v);
+  void prep_features();
+  void ensure_activations(bool all_layers);
   alien_hip_ctx* m_ctx;
   int m_n, m_nnz = 0;
+  SystemView m_v;
   std::vector<int> h_ro, h_cols, h_dcol, m_gbo, m_gtbo;
   DeviceArray<int> d_ro, d_cols, d_dcol, d_tpos;
   DeviceArray<double> d_vals;

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_dense(int n, const float* __restrict__ 
     if (row < n) {
       const float z = acc[r] + bias;
       const long o = (long)row * kHidden + col;
-      Z[o] = z;
+      if (Z) Z[o] = z;
       Hout[o] = (Hres ? Hres[o] : 0.f) + fmaxf(z, 0.f);
     }
   }
@@ -571,64 +571,125 @@ inline dim3 g1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); 
 
 Engine::Engine(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals) : m_ctx(ctx), m_n(n) {
   const int nnz = ro[n];
-  m_nnz = nnz;
-  // host structure: diagonal positions, transpose positions (symmetric pattern required)
   h_ro.assign(ro, ro + n + 1);
   h_cols.assign(cols, cols + nnz);
   h_dcol.resize(n);
   host_dcol(n, ro, cols, h_dcol.data());
-  for (int i = 0; i < n; ++i) {
-    if (h_dcol[i] >= ro[i + 1] || cols[h_dcol[i]] != i)
-      throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: missing diagonal");
-    if (!(vals[h_dcol[i]] > 0)) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: non-positive diagonal");
-  }
   hipStream_t s = m_ctx->stream;
   d_ro.upload(h_ro.data(), h_ro.size(), s);
   d_cols.upload(h_cols.data(), h_cols.size(), s);
   d_dcol.upload(h_dcol.data(), h_dcol.size(), s);
-  d_tpos.resize(nnz);
+  d_vals.upload(vals, nnz, s);
+  SystemView v;
+  v.n = n;
+  v.ro = h_ro.data();
+  v.cols = h_cols.data();
+  v.dcol = h_dcol.data();
+  v.vals = vals;
+  v.d_ro = d_ro.data();
+  v.d_cols = d_cols.data();
+  v.d_dcol = d_dcol.data();
+  v.d_vals = d_vals.data();
+  init(v);
+}
+
+Engine::Engine(alien_hip_ctx* ctx, const SystemView& v) : m_ctx(ctx), m_n(v.n) { init(v); }
+
+// Checks of the system (stored positive diagonal, symmetric pattern), the
+// transpose positions and the per-system buffers; host values are read here only.
+void Engine::init(const SystemView& v) {
+  const int n = v.n;
+  m_nnz = v.ro[n];
+  for (int i = 0; i < n; ++i) {
+    if (v.dcol[i] >= v.ro[i + 1] || v.cols[v.dcol[i]] != i)
+      throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: missing diagonal");
+    if (!(v.vals[v.dcol[i]] > 0)) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: non-positive diagonal");
+  }
+  m_v = v;
+  m_v.vals = nullptr;
+  hipStream_t s = m_ctx->stream;
+  d_tpos.resize(m_nnz);
   {
     DeviceArray<int> bad(1);
     ALIEN_HIP_CHECK(hipMemsetAsync(bad.data(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_tpos, g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_tpos.data(), bad.data());
+    hipLaunchKernelGGL(k_tpos, g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_tpos.data(), bad.data());
     check_launch("gnn tpos");
     int hbad = 0;
     ALIEN_HIP_CHECK(hipMemcpyAsync(&hbad, bad.data(), sizeof(int), hipMemcpyDeviceToHost, s));
     ALIEN_HIP_CHECK(hipStreamSynchronize(s));
     if (hbad) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: the pattern must be symmetric");
   }
-  d_vals.upload(vals, nnz, s);
   d_dis.resize(n);
-  d_e.resize(nnz);
+  d_e.resize(m_nnz);
   d_f.resize((size_t)n * kFeat);
-  for (auto& a : d_h) a.resize((size_t)n * kHidden);
-  for (auto& a : d_z) a.resize((size_t)n * kHidden);
-  for (auto& a : d_m) a.resize((size_t)n * kHidden);
-  d_ghat.resize(nnz);
-  ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+  d_ghat.resize(m_nnz);
+}
+
+// Activation buffers (n x 32 each): every layer's input, pre-activation and
+// message for the backward pass, or two inputs and one message for inference.
+void Engine::ensure_activations(bool all_layers) {
+  const size_t sz = (size_t)m_n * kHidden;
+  for (int l = 0; l < 4; ++l) {
+    const bool need_h = all_layers || l < 2, need_z = all_layers, need_m = all_layers || l == 1;
+    if (need_h && d_h[l].empty()) d_h[l].resize(sz);
+    if (need_z && d_z[l].empty()) d_z[l].resize(sz);
+    if (need_m && d_m[l].empty()) d_m[l].resize(sz);
+  }
+}
+
+// scaled couplings and node features from A's values
+void Engine::prep_features() {
+  hipStream_t s = m_ctx->stream;
+  const int n = m_n;
+  hipLaunchKernelGGL(k_prep_dis, g1(n), dim3(256), 0, s, n, m_v.d_dcol, m_v.d_vals, d_dis.data());
+  hipLaunchKernelGGL(k_prep_feat, g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_vals, d_dis.data(),
+                     d_e.data(), d_f.data());
 }
 
 void Engine::forward(const float* P) {
+  ensure_activations(true);
   hipStream_t s = m_ctx->stream;
   const int n = m_n;
-  // scaled couplings and node features from A's values
-  hipLaunchKernelGGL(k_prep_dis, g1(n), dim3(256), 0, s, n, d_dcol.data(), d_vals.data(), d_dis.data());
-  hipLaunchKernelGGL(k_prep_feat, g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_vals.data(), d_dis.data(),
-                     d_e.data(), d_f.data());
+  prep_features();
   const dim3 gd((unsigned)(((n + 31) / 32 + 3) / 4)); // 32 rows per wavefront, 4 wavefronts per workgroup
   hipLaunchKernelGGL(k_dense<8>, gd, dim3(256), 0, s, n, d_f.data(), kFeat, kFeat, (const float*)nullptr, 0, kFeat,
                      P + kOffWin, P + kOffBin, (const float*)nullptr, d_z[0].data(), d_h[0].data());
   const int offs[3] = {kOffW1, kOffW2, kOffW3};
   for (int l = 1; l <= 3; ++l) {
-    hipLaunchKernelGGL(k_aggregate, g1((long)n * 32), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_e.data(),
+    hipLaunchKernelGGL(k_aggregate, g1((long)n * 32), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_e.data(),
                        d_h[l - 1].data(), d_m[l].data());
     hipLaunchKernelGGL(k_dense<64>, gd, dim3(256), 0, s, n, d_h[l - 1].data(), kHidden, kHidden, d_m[l].data(), kHidden,
                        2 * kHidden, P + offs[l - 1], P + offs[l - 1] + 2 * kHidden * kHidden, d_h[l - 1].data(),
                        d_z[l].data(), d_h[l].data());
   }
-  hipLaunchKernelGGL(k_head, g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(), d_e.data(),
+  hipLaunchKernelGGL(k_head, g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol, d_e.data(),
                      d_h[3].data(), P, d_ghat.data());
   check_launch("gnn forward");
+}
+
+// The same arithmetic as forward(), layer l reading buffer (l - 1) % 2 and
+// writing l % 2, pre-activations not stored.
+void Engine::infer(const float* P) {
+  ensure_activations(false);
+  hipStream_t s = m_ctx->stream;
+  const int n = m_n;
+  prep_features();
+  const dim3 gd((unsigned)(((n + 31) / 32 + 3) / 4));
+  hipLaunchKernelGGL(k_dense<8>, gd, dim3(256), 0, s, n, d_f.data(), kFeat, kFeat, (const float*)nullptr, 0, kFeat,
+                     P + kOffWin, P + kOffBin, (const float*)nullptr, (float*)nullptr, d_h[0].data());
+  const int offs[3] = {kOffW1, kOffW2, kOffW3};
+  for (int l = 1; l <= 3; ++l) {
+    const float* hin = d_h[(l - 1) & 1].data();
+    float* hout = d_h[l & 1].data();
+    hipLaunchKernelGGL(k_aggregate, g1((long)n * 32), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_e.data(), hin,
+                       d_m[1].data());
+    hipLaunchKernelGGL(k_dense<64>, gd, dim3(256), 0, s, n, hin, kHidden, kHidden, d_m[1].data(), kHidden,
+                       2 * kHidden, P + offs[l - 1], P + offs[l - 1] + 2 * kHidden * kHidden, hin, (float*)nullptr,
+                       hout);
+  }
+  hipLaunchKernelGGL(k_head, g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol, d_e.data(),
+                     d_h[1].data(), P, d_ghat.data());
+  check_launch("gnn inference");
 }
 
 double Engine::loss_grad(const float* P, const float* V, int K, float beta, int cg_steps, float* grad,
@@ -646,24 +707,24 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, int 
   RedWork w = m_ctx->red();
   const bool kap = cg_steps == 0;
   if (kap) {
-    hipLaunchKernelGGL(k_kcond, dim3((unsigned)nb), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+    hipLaunchKernelGGL(k_kcond, dim3((unsigned)nb), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol,
                        d_tpos.data(), d_e.data(), d_ghat.data(), ag.data(), w);
     launch_finalize(s, w, nb, 2, kFinStore, sums.data(), nullptr);
   }
   // ---- probe term
   constexpr int KP = 4;
   DeviceArray<float> W_((size_t)n * KP), U((size_t)n * KP), Y((size_t)n * KP), dY((size_t)n * KP), Q((size_t)n * KP);
-  hipLaunchKernelGGL((k_ahat_mult<KP>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_e.data(), V, W_.data());
-  hipLaunchKernelGGL((k_ghat_mult<KP, true>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+  hipLaunchKernelGGL((k_ahat_mult<KP>), g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_e.data(), V, W_.data());
+  hipLaunchKernelGGL((k_ghat_mult<KP, true>), g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol,
                      d_tpos.data(), d_ghat.data(), W_.data(), U.data());
-  hipLaunchKernelGGL((k_ghat_mult<KP, false>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+  hipLaunchKernelGGL((k_ghat_mult<KP, false>), g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol,
                      d_tpos.data(), d_ghat.data(), U.data(), Y.data());
   hipLaunchKernelGGL((k_probe_resid<KP>), dim3((unsigned)nb), dim3(256), 0, s, n, V, Y.data(), dY.data(),
                      (float)(1.0 / ((double)n * KP)), w);
   launch_finalize(s, w, nb, 1, kFinStore, sums.data() + 2, nullptr);
-  hipLaunchKernelGGL((k_ghat_mult<KP, true>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+  hipLaunchKernelGGL((k_ghat_mult<KP, true>), g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol,
                      d_tpos.data(), d_ghat.data(), dY.data(), Q.data());
-  hipLaunchKernelGGL((k_grad_ghat<KP>), g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+  hipLaunchKernelGGL((k_grad_ghat<KP>), g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol,
                      d_ghat.data(), ag.data(), sums.data(), kap ? 1.f : 0.f, beta, dY.data(), U.data(), W_.data(),
                      Q.data(), gg.data());
   // ---- CG term (cg_steps > 0): k PCG steps on Ahat with Ghat Ghat^T, reverse pass into gg
@@ -673,7 +734,7 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, int 
   DeviceArray<float> dh[2], dz((size_t)n * kHidden), dmsg((size_t)n * kHidden), prow((size_t)n * kHeadGrads);
   dh[0].resize((size_t)n * kHidden);
   dh[1].resize((size_t)n * kHidden);
-  hipLaunchKernelGGL(k_head_bwd, g1(n), dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(), d_tpos.data(),
+  hipLaunchKernelGGL(k_head_bwd, g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol, d_tpos.data(),
                      d_e.data(), d_ghat.data(), gg.data(), d_h[3].data(), P, dh[0].data(), prow.data());
   const int cap = (int)nb;
   DeviceArray<double> part((size_t)(2 * kHidden * kHidden + kHidden) * cap);
@@ -686,7 +747,7 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, int 
     const float* Wl = P + offs[l - 1];
     hipLaunchKernelGGL(k_dense_bwd<64>, g1(n), dim3(256), 0, s, n, dh[cur].data(), d_z[l].data(), Wl, 2 * kHidden,
                        dz.data(), dh[cur ^ 1].data(), dmsg.data());
-    hipLaunchKernelGGL(k_aggregate_add, g1((long)n * 32), dim3(256), 0, s, n, d_ro.data(), d_cols.data(),
+    hipLaunchKernelGGL(k_aggregate_add, g1((long)n * 32), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols,
                        d_tpos.data(), d_e.data(), dmsg.data(), dh[cur ^ 1].data());
     hipLaunchKernelGGL(k_wgrad_partials<64>, dim3((unsigned)nb), dim3(256), 0, s, n, d_h[l - 1].data(), kHidden, kHidden,
                        d_m[l].data(), kHidden, 2 * kHidden, dz.data(), part.data(), cap);
@@ -745,18 +806,18 @@ double Engine::cg_term(const float* B, int k, float* gg) {
     hipLaunchKernelGGL(k_lin4, gv, dim3(256), 0, s, n, ca, X, cb, Z, Y);
   };
   auto amul = [&](const float* X, float* Y) {
-    hipLaunchKernelGGL((k_ahat_mult<4>), gr, dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_e.data(), X, Y);
+    hipLaunchKernelGGL((k_ahat_mult<4>), gr, dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_e.data(), X, Y);
   };
   auto gmul = [&](const float* X, float* Y) { // Y = Ghat X
-    hipLaunchKernelGGL((k_ghat_mult<4, false>), gr, dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+    hipLaunchKernelGGL((k_ghat_mult<4, false>), gr, dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol,
                        d_tpos.data(), d_ghat.data(), X, Y);
   };
   auto gtmul = [&](const float* X, float* Y) { // Y = Ghat^T X
-    hipLaunchKernelGGL((k_ghat_mult<4, true>), gr, dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(),
+    hipLaunchKernelGGL((k_ghat_mult<4, true>), gr, dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol,
                        d_tpos.data(), d_ghat.data(), X, Y);
   };
   auto acc = [&](const float* U, const float* V, const float* W, const float* X) {
-    hipLaunchKernelGGL(k_edge_acc4, gr, dim3(256), 0, s, n, d_ro.data(), d_cols.data(), d_dcol.data(), U, V, W, X,
+    hipLaunchKernelGGL(k_edge_acc4, gr, dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol, U, V, W, X,
                        gg);
   };
   typedef std::array<double, 4> S4;
@@ -851,18 +912,32 @@ double Engine::cg_term(const float* B, int k, float* gg) {
 }
 
 // Row-bundled layouts of G (rows = lower + diagonal of A) and of G^T (rows =
-// diagonal + upper of A): RB-DIA when it stores at most kRbdiaMaxGrowth times
-// the RB-CSR slots, RB-CSR otherwise; plus the slot -> CSR position map of the
-// values (-1 = padding) and the row whose D^-1/2 scales the slot.
+// diagonal + upper of A; the pattern is symmetric): RB-DIA when it stores at
+// most kRbdiaMaxGrowth times the RB-CSR slots, RB-CSR otherwise. When A itself
+// is RB-DIA, the factors' offsets and lane masks are A's with offset <= 0 (G)
+// and >= 0 (G^T): a split of A's structure, no pass over the entries.
 void Engine::build_factor_layout() {
   if (!m_gbo.empty()) return;
   const int n = m_n;
   const int nb = (n + kBundle - 1) / kBundle;
+  hipStream_t s = m_ctx->stream;
+  const bool split = m_v.rb_bo && m_v.d_rb_doff && m_v.d_rb_dmask;
+  std::vector<int> adoff;
+  std::vector<unsigned long long> admask;
+  if (split) {
+    const size_t aslots = (size_t)m_v.rb_bo[nb];
+    adoff.resize(aslots);
+    admask.resize(aslots);
+    ALIEN_HIP_CHECK(hipMemcpyAsync(adoff.data(), m_v.d_rb_doff, aslots * sizeof(int), hipMemcpyDeviceToHost, s));
+    ALIEN_HIP_CHECK(hipMemcpyAsync(admask.data(), m_v.d_rb_dmask, aslots * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, s));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(s));
+  }
   std::vector<int> kb(n), ke(n);
   for (int t = 0; t < 2; ++t) {
     for (int r = 0; r < n; ++r) {
-      kb[r] = t == 0 ? h_ro[r] : h_dcol[r];
-      ke[r] = t == 0 ? h_dcol[r] + 1 : h_ro[r + 1];
+      kb[r] = t == 0 ? m_v.ro[r] : m_v.dcol[r];
+      ke[r] = t == 0 ? m_v.dcol[r] + 1 : m_v.ro[r + 1];
     }
     // RB-CSR widths
     std::vector<int> cbo(nb + 1, 0);
@@ -877,13 +952,28 @@ void Engine::build_factor_layout() {
     if (acc * kBundle > 0x7fffffffL) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: factor too large");
     std::vector<int> dbo, doff;
     std::vector<uint64_t> dmask;
-    const long dslots = rbdia_structure(n, h_ro.data(), h_cols.data(), kb.data(), ke.data(), kRbdiaMaxWidth, dbo,
-                                        doff, dmask);
+    long dslots;
+    if (split) {
+      dbo.assign(nb + 1, 0);
+      doff.reserve(adoff.size());
+      dmask.reserve(adoff.size());
+      for (int b = 0; b < nb; ++b) {
+        dbo[b] = (int)doff.size();
+        for (int j = m_v.rb_bo[b]; j < m_v.rb_bo[b + 1]; ++j) {
+          if (t == 0 ? adoff[j] > 0 : adoff[j] < 0) continue;
+          doff.push_back(adoff[j]);
+          dmask.push_back(admask[j]);
+        }
+      }
+      dbo[nb] = (int)doff.size();
+      dslots = dbo[nb];
+    } else {
+      dslots = rbdia_structure(n, m_v.ro, m_v.cols, kb.data(), ke.data(), kRbdiaMaxWidth, dbo, doff, dmask);
+    }
     const bool dia = dslots >= 0 && dslots <= kRbdiaMaxGrowth * cbo[nb];
     std::vector<int>& bo = t == 0 ? m_gbo : m_gtbo;
     bo = dia ? dbo : cbo;
     const size_t slots = (size_t)bo[nb] * kBundle;
-    hipStream_t s = m_ctx->stream;
     Factor& F = t == 0 ? m_g : m_gt;
     F.dia = dia;
     F.bo.upload(bo.data(), bo.size(), s);
@@ -894,13 +984,13 @@ void Engine::build_factor_layout() {
       std::vector<int> bc(slots, -1);
       for (int r = 0; r < n; ++r) {
         const int b = r / kBundle, lane = r % kBundle;
-        for (int k = kb[r]; k < ke[r]; ++k) bc[((size_t)bo[b] + (k - kb[r])) * kBundle + lane] = h_cols[k];
+        for (int k = kb[r]; k < ke[r]; ++k) bc[((size_t)bo[b] + (k - kb[r])) * kBundle + lane] = m_v.cols[k];
       }
       F.bc.upload(bc.data(), slots, s);
     }
     F.slots = slots;
   }
-  ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
+  ALIEN_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 template <typename T>
@@ -908,14 +998,28 @@ void Engine::fill_factor(T* g_vals, T* gt_vals) {
   build_factor_layout();
   hipStream_t s = m_ctx->stream;
   const int nrows64 = (m_n + kBundle - 1) / kBundle * kBundle;
-  hipLaunchKernelGGL((k_fill_factor<T, false>), g1(nrows64), dim3(256), 0, s, m_n, d_ro.data(), d_cols.data(),
-                     d_dcol.data(), d_tpos.data(), m_g.bo.data(), m_g.dia ? m_g.doff.data() : nullptr,
+  hipLaunchKernelGGL((k_fill_factor<T, false>), g1(nrows64), dim3(256), 0, s, m_n, m_v.d_ro, m_v.d_cols,
+                     m_v.d_dcol, d_tpos.data(), m_g.bo.data(), m_g.dia ? m_g.doff.data() : nullptr,
                      d_ghat.data(), d_dis.data(), g_vals);
-  hipLaunchKernelGGL((k_fill_factor<T, true>), g1(nrows64), dim3(256), 0, s, m_n, d_ro.data(), d_cols.data(),
-                     d_dcol.data(), d_tpos.data(), m_gt.bo.data(), m_gt.dia ? m_gt.doff.data() : nullptr,
+  hipLaunchKernelGGL((k_fill_factor<T, true>), g1(nrows64), dim3(256), 0, s, m_n, m_v.d_ro, m_v.d_cols,
+                     m_v.d_dcol, d_tpos.data(), m_gt.bo.data(), m_gt.dia ? m_gt.doff.data() : nullptr,
                      d_ghat.data(), d_dis.data(), gt_vals);
   check_launch("gnn fill factor");
 }
+void Engine::release_network_buffers() {
+  ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
+  for (int l = 0; l < 4; ++l) {
+    d_h[l].reset();
+    d_z[l].reset();
+    d_m[l].reset();
+  }
+  d_e.reset();
+  d_f.reset();
+  d_ghat.reset();
+  d_dis.reset();
+  d_tpos.reset();
+}
+
 template void Engine::fill_factor<float>(float*, float*);
 template void Engine::fill_factor<double>(double*, double*);
 

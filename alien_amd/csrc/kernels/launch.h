@@ -5,12 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "krylov_state.h"
 
 namespace alien_hip {
 
 struct RedWork;
+template <typename T> class DeviceArray;
 
 // what the second stage of a reduction does with the totals (reduce.hip)
 enum FinOp { kFinStore = 0, kFinRho1 = 1, kFinPQ = 2, kFinXR = 3, kFinXRDiag = 4, kFinRho1B = 5 };
@@ -45,6 +47,17 @@ void launch_spmv_rb(hipStream_t, int n, RBMat<T> A, const T* x, T* y);
 template <typename T>
 void launch_rbdia_fill(hipStream_t, int n, const int* ro, const int* cols, const double* vals, const int* bo,
                        const int* doff, T* bv);
+// RB-DIA structure of a device CSR built on the device: rbdia_count_device
+// returns the slot count and the bundle offsets on the host (-1 when a bundle
+// holds more than max_w distinct offsets, a row's columns are not strictly
+// ascending or the slots exceed int32 indexing); launch_rbdia_struct writes
+// the offsets and lane masks at device bundle offsets bo; rbdia_structure_device
+// does both and keeps the structure in the given device arrays.
+long rbdia_count_device(hipStream_t, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo);
+void launch_rbdia_struct(hipStream_t, int n, const int* ro, const int* cols, const int* bo, int* doff,
+                         unsigned long long* dmask);
+long rbdia_structure_device(hipStream_t, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo,
+                            DeviceArray<int>& d_bo, DeviceArray<int>& doff, DeviceArray<unsigned long long>& dmask);
 template <typename T>
 void launch_spmv_bellpack(hipStream_t, int n, int slice, const int* bro, const int* bc, const T* bv,
                           const T* x, T* y);

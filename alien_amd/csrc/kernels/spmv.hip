@@ -19,6 +19,7 @@
 // The matrix streams are read once per SpMV and loaded non-temporally so that
 // L2 / Infinity Cache capacity stays with the gathered x vector.
 #include "../alien_hip_internal.h"
+#include "../alien/hip/HIPContext.h"
 #include "device_common.h"
 #include "launch.h"
 #include "rb_rows.h"
@@ -114,6 +115,64 @@ __global__ void __launch_bounds__(kBlock) k_rbdia_fill(int n, const int* __restr
     T v = T(0);
     if (k < e && cols[k] - row == doff[s0 + j]) v = (T)vals[k++];
     bv[base + (long)j * kBundle] = v;
+  }
+}
+
+// Device RB-DIA structure, one wavefront per bundle (lane = row): the 64 rows'
+// ascending offsets col - row are merged by a wave-wide minimum per step, so
+// the bundle's distinct offsets come out ascending, each with the ballot of the
+// rows that hold it as its lane mask -- the structure rbdia_structure builds on
+// the host. COUNT pass: width[b] = number of distinct offsets, or -1 when it
+// exceeds max_w or a row's columns are not strictly ascending. WRITE pass:
+// offsets and masks at the bundle offsets bo of a successful COUNT pass.
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <bool WRITE>
+__global__ void __launch_bounds__(kBlock) k_rbdia_struct(int n, const int* __restrict__ ro,
+                                                          const int* __restrict__ cols, int max_w,
+                                                          const int* __restrict__ bo, int* __restrict__ width,
+                                                          int* __restrict__ doff,
+                                                          unsigned long long* __restrict__ dmask) {
+  constexpr int kEnd = 0x7fffffff;
+  const int nb = (n + kBundle - 1) / kBundle;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int b = t / kBundle, lane = t % kBundle;
+  if (b >= nb) return;
+  const int row = t;
+  int k = 0, e = 0;
+  if (row < n) {
+    k = ro[row];
+    e = ro[row + 1];
+  }
+  int cur = k < e ? cols[k] - row : kEnd;
+  bool bad = false;
+  const int s0 = WRITE ? bo[b] : 0;
+  const int cap = WRITE ? bo[b + 1] - s0 : max_w + 1;
+  int w = 0;
+  while (w < cap) {
+    const int m = wave_min_i32(cur);
+    if (m == kEnd) break;
+    const bool hit = cur == m;
+    const unsigned long long mask = __ballot(hit);
+    if (WRITE && lane == 0) {
+      doff[s0 + w] = m;
+      dmask[s0 + w] = mask;
+    }
+    ++w;
+    if (hit) {
+      ++k;
+      const int nx = k < e ? cols[k] - row : kEnd;
+      bad |= nx <= m;
+      cur = nx;
+    }
+  }
+  if (!WRITE) {
+    const bool any_bad = __ballot(bad) != 0ull;
+    if (lane == 0) width[b] = (any_bad || w > max_w) ? -1 : w;
   }
 }
 
@@ -296,6 +355,47 @@ void launch_rbdia_fill(hipStream_t st, int n, const int* ro, const int* cols, co
                        const int* doff, T* bv) {
   if (n > 0) hipLaunchKernelGGL(k_rbdia_fill<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, bo, doff, bv);
 }
+long rbdia_count_device(hipStream_t st, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo) {
+  const int nb = (n + kBundle - 1) / kBundle;
+  bo.assign(nb + 1, 0);
+  if (nb == 0) return 0;
+  DeviceArray<int> width(nb);
+  hipLaunchKernelGGL(k_rbdia_struct<false>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, max_w,
+                     (const int*)nullptr, width.data(), (int*)nullptr, (unsigned long long*)nullptr);
+  check_launch("rbdia_struct count");
+  ALIEN_HIP_CHECK(hipMemcpyAsync(bo.data(), width.data(), nb * sizeof(int), hipMemcpyDeviceToHost, st));
+  ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+  long acc = 0;
+  for (int b = 0; b < nb; ++b) {
+    const int w = bo[b];
+    if (w < 0) return -1;
+    bo[b] = (int)acc;
+    acc += w;
+    if (acc * kBundle > 0x7fffffffL) return -1;
+  }
+  bo[nb] = (int)acc;
+  return acc;
+}
+
+void launch_rbdia_struct(hipStream_t st, int n, const int* ro, const int* cols, const int* bo, int* doff,
+                         unsigned long long* dmask) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_rbdia_struct<true>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, 0, bo,
+                       (int*)nullptr, doff, dmask);
+}
+
+long rbdia_structure_device(hipStream_t st, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo,
+                            DeviceArray<int>& d_bo, DeviceArray<int>& doff, DeviceArray<unsigned long long>& dmask) {
+  const long slots = rbdia_count_device(st, n, ro, cols, max_w, bo);
+  if (slots < 0) return -1;
+  d_bo.upload(bo.data(), bo.size(), st);
+  doff.resize(slots);
+  dmask.resize(slots);
+  launch_rbdia_struct(st, n, ro, cols, d_bo.data(), doff.data(), dmask.data());
+  check_launch("rbdia_struct write");
+  return slots;
+}
+
 template <typename T>
 void launch_rbcsr_fill(hipStream_t st, int n, const int* ro, const int* cols, const double* vals,
                        const int* bo, int* bc, T* bv) {

@@ -775,7 +775,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
 
     tm.start();
     Alien::CSRHostView view{n, Aro, Acols, Avals};
-    HIPMatrix A(ctx, view, (fused || mixed) ? ALIEN_HIP_LAYOUT_CSR : p->layout);
+    HIPMatrix A(ctx, view, (fused || mixed) ? ALIEN_HIP_LAYOUT_CSR : p->layout, /*borrow_host=*/true);
     if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
     HIPVector b, x;
     alg.allocate(A.resource(), b, x);
@@ -848,16 +848,31 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         diag->init();
         invd64 = diag->invDiag().data();
       }
-      // learned preconditioner: network forward on A, then G and G^T as RB-CSR
-      // in the solver's precision
+      // learned preconditioner: network inference on A (read in place), then G
+      // and G^T row-bundled in the solver's precision
       std::unique_ptr<gnn::Engine> net;
       DeviceArray<float> theta, gv32, gtv32;
       DeviceArray<double> gv64, gtv64;
       int stall_window = 0;
       if (learned) {
-        net.reset(new gnn::Engine(ctx, n, Aro, Acols, Avals));
+        gnn::SystemView v;
+        v.n = n;
+        v.ro = A.kcol();
+        v.cols = A.cols();
+        v.dcol = A.dcol();
+        v.vals = A.hostValues();
+        v.d_ro = A.d_ro.data();
+        v.d_cols = A.d_cols.data();
+        v.d_dcol = A.d_dcol.data();
+        v.d_vals = A.d_vals.data();
+        if (A.isRBDIA()) {
+          v.rb_bo = A.bundleOffsets().data();
+          v.d_rb_doff = A.d_doff.data();
+          v.d_rb_dmask = A.d_dmask.data();
+        }
+        net.reset(new gnn::Engine(ctx, v));
         theta.upload(p->gnn_theta, gnn::kNParam, ctx->stream);
-        net->forward(theta.data());
+        net->infer(theta.data());
         net->build_factor_layout();
         ps.kind = kPrecFactor;
         const size_t ng = net->g_slots(), ngt = net->gt_slots();
@@ -874,6 +889,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
           ps.g64 = net->g_mat<double>(gv64.data());
           ps.gt64 = net->gt_mat<double>(gtv64.data());
         }
+        net->release_network_buffers();
         stall_window = p->stall_window > 0 ? p->stall_window : 400;
       }
       if (!mixed) {

@@ -149,12 +149,22 @@ int alien_hip_spmv_rbcsr_f32(alien_hip_ctx* ctx, int nrows, const int* bundle_of
  * streamed, only values; the per-row sum is still the sequential CSR sum,
  * bit-exact with _seqMult. Host structure in two calls: rbdia_count fills
  * bundle_off and returns the slot count (bundle_off[nbundles]), or -1 when a
- * bundle holds more than max_width distinct offsets; rbdia_struct fills doff
+ * bundle holds more than max_width distinct offsets or a row's columns are
+ * not strictly ascending; rbdia_struct fills doff
  * and dmask (that many entries each). Values are filled on the device. */
 int64_t alien_hip_rbdia_count(int nrows, const int* row_offset_host, const int* cols_host, int max_width,
                               int* bundle_off_host);
 int alien_hip_rbdia_struct(int nrows, const int* row_offset_host, const int* cols_host, int max_width,
                            int* doff_host, uint64_t* dmask_host);
+/* The same structure built on the device from a device CSR (one wavefront per
+ * bundle): rbdia_count_device fills the HOST array bundle_off and returns the
+ * slot count, or -1 as rbdia_count; rbdia_struct_device writes doff and dmask
+ * (device arrays of that many entries) given the device copy of bundle_off.
+ * HIPMatrix's RB-DIA conversion runs these. */
+int64_t alien_hip_rbdia_count_device(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                                     int max_width, int* bundle_off_host);
+int alien_hip_rbdia_struct_device(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                                  const int* bundle_off, int* doff, uint64_t* dmask);
 int alien_hip_rbdia_fill_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
                              const double* vals, const int* bundle_off, const int* doff, double* bvals);
 int alien_hip_rbdia_fill_f32(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,

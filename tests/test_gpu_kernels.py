@@ -112,6 +112,57 @@ def test_spmv_rbdia_bit_exact(ctx, name, A):
     np.testing.assert_array_equal(host(y32, ctx)[:n], host(yc, ctx)[:n])
 
 
+def rbdia_device(ctx, dro, dc, n, max_width):
+    """RB-DIA structure built on the device: (bo, doff, dmask) on the host, or None."""
+    nb = (n + 63) // 64
+    bo = np.zeros(nb + 1, dtype=np.int32)
+    slots = aa.lib().alien_hip_rbdia_count_device(ctx.h, n, C.c_void_p(dro.data_ptr()), C.c_void_p(dc.data_ptr()),
+                                                  max_width, bo.ctypes.data_as(C.POINTER(C.c_int)))
+    if slots < 0:
+        return None
+    ddo = empty(max(slots, 1), torch.int32)
+    ddm = empty(max(slots, 1), torch.int64)
+    call("alien_hip_rbdia_struct_device", ctx.h, n, p(dro), p(dc), p(dev(bo)), p(ddo), p(ddm))
+    return bo, host(ddo, ctx)[:slots], host(ddm, ctx)[:slots].view(np.uint64)
+
+
+@pytest.mark.parametrize("name,A", SYSTEMS, ids=[s[0] for s in SYSTEMS])
+def test_rbdia_structure_device_equals_host(ctx, name, A):
+    """The device builder (wave-wide merge of the rows' offsets) gives the host
+    builder's bundle offsets, offsets and lane masks exactly; both refuse a
+    bundle wider than max_width."""
+    ro, cols, _ = A
+    n = len(ro) - 1
+    say("case rbdia structure device/host", name, n)
+    dro, dc = dev(ro), dev(cols)
+    for mw in (64, 1 << 20):
+        ref = aa.rbdia_build(ro, cols, max_width=mw)
+        got = rbdia_device(ctx, dro, dc, n, mw)
+        if ref is None:
+            assert got is None
+            continue
+        for r, g in zip(ref, got):
+            np.testing.assert_array_equal(g, r)
+
+
+def test_rbdia_structure_device_rejects_unsorted_rows(ctx):
+    """A row whose columns are not strictly ascending (descending pair, or a
+    repeated column) has no RB-DIA structure, on the device as on the host."""
+    ro, cols, _ = aa.gen_csr(aa.GEN_POISSON7, 6)
+    n = len(ro) - 1
+    for case in ("swap", "repeat"):
+        c = cols.copy()
+        k = ro[70]
+        if case == "swap":
+            c[k], c[k + 1] = c[k + 1], c[k]
+        else:
+            c[k + 1] = c[k]
+        say("case rbdia structure unsorted", case)
+        assert aa.rbdia_build(ro, c, max_width=1 << 20) is None
+        assert rbdia_device(ctx, dev(ro), dev(c), n, 1 << 20) is None
+    assert rbdia_device(ctx, dev(ro), dev(cols), n, 64) is not None
+
+
 @pytest.mark.parametrize("name,A", SYSTEMS[:3], ids=[s[0] for s in SYSTEMS[:3]])
 def test_spmv_bellpack_bit_exact(ctx, name, A):
     ro, cols, vals = A
@@ -304,8 +355,11 @@ def test_full_size_spmv_properties_256(ctx):
     ctx.synchronize()
     assert torch.equal(y1, y2)
     del bc, bv
-    # RB-DIA at full size: same sums bit for bit, FP64 and (against RB-CSR) FP32
+    # RB-DIA at full size: same sums bit for bit, FP64 and (against RB-CSR) FP32;
+    # the device-built structure equals the host-built one
     dbo_d, doff, dmask = aa.rbdia_build(ro, cols)
+    for r, g in zip((dbo_d, doff, dmask), rbdia_device(ctx, dro, dc, n, 64)):
+        np.testing.assert_array_equal(g, r)
     assert int(dbo_d[-1]) * 64 <= 1.01 * len(vals) + 64 * len(dbo_d)
     ddbo, ddoff, ddm = dev(dbo_d), dev(doff), dev(dmask.view(np.int64))
     dvv = empty(int(dbo_d[-1]) * 64)

@@ -144,6 +144,23 @@ def test_rbdia_and_rbcsr_solves_identical(ctx, mode, precond):
     np.testing.assert_array_equal(x1, x2)
 
 
+@pytest.mark.parametrize("mode", [aa.MODE_GRAPH, aa.MODE_MIXED])
+def test_learned_factor_layout_split_and_host_built_identical(ctx, mode):
+    """With A stored as RB-DIA the factors' structures are split from A's (by
+    the sign of the offset); with A as RB-CSR they are built from the CSR on the
+    host. Both give the same iterates bit for bit."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 24, params=[2.0, 1, 1, 1], seed=11)
+    b = aa.uniform01(3, len(ro) - 1)
+    say("case learned factor split vs host-built", mode)
+    kw = dict(solver=aa.CG, precond=aa.PREC_LEARNED, gnn_theta=aa.gnn_init(0), mode=mode, tol=1e-8,
+              max_iter=3000, poll_every=4)
+    x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBDIA, **kw)
+    x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBCSR, **kw)
+    assert r1["status"] == 1 and r1["fallback"] == 0
+    assert r1["iterations"] == r2["iterations"] and r1["status"] == r2["status"]
+    np.testing.assert_array_equal(x1, x2)
+
+
 def test_unstructured_matrix_falls_back_to_rbcsr(ctx):
     """A random SPD matrix has too many distinct offsets per bundle for RB-DIA:
     the fused solver runs on RB-CSR and still matches the oracle's iterations."""

@@ -249,3 +249,7 @@ def test_rbdia_width_limit_and_stencil_width():
     assert aa.rbdia_build(ro, cols, max_width=6) is None
     ro2, cols2 = ragged_matrix()
     assert aa.rbdia_build(ro2, cols2, max_width=64) is None
+    # a repeated column would drop an entry from the product: no RB-DIA structure
+    c = cols.copy()
+    c[ro[70] + 1] = c[ro[70]]
+    assert aa.rbdia_build(ro, c, max_width=1 << 20) is None
