@@ -380,9 +380,18 @@ void finish_breakdown(FusedCG<T>& F, CGState& s, int* error) {
 // reliable update has come within stall_window iterations.
 struct MixedResult {
   int it = 0, updates = 0, status = 0, error = 0;
-  bool stalled = false;
+  bool stalled = false;      // learned M: no progress within the stall window (Jacobi takes over)
+  bool fp32_stalled = false; // the FP32 inner iteration stopped making progress (FP64 takes over)
   double rel = 0;
 };
+
+// A BiCGStab group (iterations between reliable updates) longer than
+// max(kMinGroup, kGroupFactor x the longest group so far) is taken as a
+// stagnated FP32 inner iteration: FP32 storage of A carries a relative error
+// of ~6e-8 per entry, and when that times the condition number nears 1 the
+// inner iteration stops reducing the true residual.
+constexpr int kMinGroup = 250;
+constexpr int kGroupFactor = 3;
 
 MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMat<float> A32, const float* invd32,
                               const PrecondSpec& ps, const double* b, double* x, double tol, int max_it,
@@ -423,9 +432,10 @@ MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMa
   const double d2 = delta * delta;
   const int k = graph ? std::max(1, poll_every) : 1;
   for (;;) {
-    if (stall_window > 0 && !std::isfinite(cur.rr)) {
+    if (!std::isfinite(cur.rr)) {
       // the increment is lost; x holds the last reliable update
-      out.stalled = true;
+      if (stall_window > 0) out.stalled = true;
+      else out.fp32_stalled = true;
       break;
     }
     const bool stall = stall_window > 0 && cur.it - it_last >= stall_window;
@@ -653,12 +663,30 @@ MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RB
   restart(0);
   BiCGState cur = F.advance(x32.data(), graph, 1);
   double rr_last = rr64;
-  int it_last = 0, restarts = 0;
+  int it_last = 0, restarts = 0, group_max = 0;
   const double d2 = delta * delta;
   const int k = graph ? std::max(1, poll_every) : 1;
   for (;;) {
-    if (stall_window > 0 && !std::isfinite(cur.rr)) {
-      out.stalled = true;
+    if (!std::isfinite(cur.rr) || !std::isfinite(cur.alpha) || !std::isfinite(cur.omega)) {
+      // the FP32 recursion overflowed: its increment is lost, x holds the last reliable update
+      if (stall_window > 0) out.stalled = true;
+      else out.fp32_stalled = true;
+      break;
+    }
+    if (stall_window == 0 && !cur.done && cur.it - it_last >= std::max(kMinGroup, kGroupFactor * group_max) &&
+        cur.it < max_it) {
+      // stagnated: keep the increment only if it lowers the true residual
+      DeviceArray<double> xb(n);
+      launch_copy<double>(s, n, x, xb.data());
+      launch_fold_increment(s, n, x32.data(), x);
+      const double rrc = true_res();
+      if (rrc < rr_last) {
+        ++out.updates;
+        out.rel = std::sqrt(rrc / bb);
+      } else {
+        launch_copy<double>(s, n, xb.data(), x);
+      }
+      out.fp32_stalled = true;
       break;
     }
     const bool stall = stall_window > 0 && cur.it - it_last >= stall_window;
@@ -682,6 +710,7 @@ MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RB
       break;
     }
     rr_last = rr64;
+    group_max = std::max(group_max, cur.it - it_last);
     it_last = cur.it;
     if (cur.done) { // stop on s, FP32 stop not confirmed, or breakdown: restart from the true residual
       if (cur.breakdown && ++restarts > 10) {
@@ -974,6 +1003,28 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
                   std::max(0, p->max_iter - used), delta, p->poll_every, true, 0);
           m.it += used;
           m.updates += ups;
+        }
+        if (m.fp32_stalled) {
+          // FP64 Jacobi from the last reliable update, for the remaining iterations (a run
+          // with the learned M reports a stall instead, so Jacobi is the inner preconditioner)
+          res->fallback |= 2;
+          const int used = m.it;
+          const int left = std::max(0, p->max_iter - used);
+          if (bicg) {
+            FusedBiCG<double> F(ctx, n, rb64, invd64, PrecondSpec{});
+            const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, true, p->poll_every, nullptr);
+            m.it = used + s.it;
+            m.status = s.status;
+            m.error = s.breakdown ? 1 : 0;
+            m.rel = std::sqrt(s.rr / alg.dot(b, b));
+          } else {
+            FusedCG<double> F(ctx, n, rb64, invd64, PrecondSpec{});
+            CGState s = F.solve(b.data(), x.data(), false, p->tol, left, true, p->poll_every, nullptr);
+            finish_breakdown(F, s, &m.error);
+            m.it = used + s.it;
+            m.status = s.status;
+            m.rel = std::sqrt(s.rr / alg.dot(b, b));
+          }
         }
         res->solve_s = tm.stop();
         res->iterations = m.it;

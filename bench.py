@@ -238,12 +238,193 @@ def run_convdiff(a):
     ctx.close()
 
 
+def scalarise_bsr2(ro, cols, vals):
+    """Interleaved scalar CSR of a 2x2-block CSR: unknown e of cell i is row 2i+e, block entry
+    (e, u) of block k (column-major, vals[4k + e + 2u], the _seqMultBlock layout) goes to column
+    2 cols[k] + u. Columns stay ascending inside each row."""
+    nb = len(ro) - 1
+    ln = np.diff(ro).astype(np.int64)
+    sro = np.zeros(2 * nb + 1, dtype=np.int64)
+    np.cumsum(np.repeat(2 * ln, 2), out=sro[1:])
+    nnzb = len(cols)
+    k = np.arange(nnzb, dtype=np.int64)
+    rowblk = np.repeat(np.arange(nb, dtype=np.int64), ln)
+    off = 2 * (k - ro[:-1].astype(np.int64)[rowblk])
+    scols = np.empty(4 * nnzb, dtype=np.int32)
+    svals = np.empty(4 * nnzb, dtype=np.float64)
+    for e in (0, 1):
+        start = sro[2 * rowblk + e] + off
+        for u in (0, 1):
+            scols[start + u] = 2 * cols + u
+            svals[start + u] = vals[4 * k + e + 2 * u]
+    return sro.astype(np.int32), scols, svals
+
+
+def gpu_rhs(aa, torch, ctx, ro, cols, vals, xt):
+    """b = A x_true with the GPU's FP64 CSR SpMV (bit-exact with the sequential reference product)."""
+    import ctypes as C
+    P = lambda t: C.c_void_p(t.data_ptr())
+    dev = lambda arr: torch.from_numpy(np.ascontiguousarray(arr)).to("cuda")
+    dro, dc, dv, dx = dev(ro), dev(cols), dev(vals), dev(xt)
+    db = torch.empty(len(xt), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    aa.check(aa.lib().alien_hip_spmv_csr_f64(ctx.h, len(xt), P(dro), P(dc), P(dv), P(dx), P(db)))
+    ctx.synchronize()
+    b = db.cpu().numpy()
+    del dro, dc, dv, dx, db
+    torch.cuda.empty_cache()
+    return b
+
+
+def run_blocks2(a):
+    """Two unknowns per cell (not the default line): n^3 cells (192^3: N = 14,155,776 unknowns,
+    49,324,032 2x2 blocks), log-normal diffusion with off-diagonal coupling 0.3 x diagonal inside
+    each block (nonsymmetric), BiCGStab to FP64 relative residual 1e-10 on the interleaved scalar
+    CSR: multicolour ILU0 (REFERENCE mode), fused FP64 Jacobi (GRAPH), FP32/FP64 mixed Jacobi and
+    the learned M inside the mixed solver (falls back to Jacobi on this operator); 2x2-block SpMV
+    throughput with FP64 and FP32 block storage."""
+    import ctypes as C
+
+    import torch
+
+    import alien_amd as aa
+    n = a.n
+    log(f"generating 2-unknown block system {n}^3")
+    bro, bcols, bvals = aa.gen_bsr2(n, params=[2.0, 0.3], seed=42)
+    ro, cols, vals = scalarise_bsr2(bro, bcols, bvals)
+    ncell, nnzb = len(bro) - 1, len(bcols)
+    N, nnz = len(ro) - 1, len(vals)
+    xt = aa.uniform01(42, N)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx = aa.Context(stream.cuda_stream)
+    b = gpu_rhs(aa, torch, ctx, ro, cols, vals, xt)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    dev = lambda arr: torch.from_numpy(np.ascontiguousarray(arr)).to("cuda")
+    # 2x2-block SpMV (the _seqMultBlock product), FP64 and FP32 block storage
+    dro, dc, dv, dv32 = dev(bro), dev(bcols), dev(bvals), dev(bvals.astype(np.float32))
+    x = torch.rand(N, dtype=torch.float64, device="cuda")
+    x32 = x.float()
+    y = torch.empty(N, dtype=torch.float64, device="cuda")
+    y32 = torch.empty(N, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    L = aa.lib()
+    bsr = {}
+    for name, fn, nbytes in (
+            ("bsr2_fp64", lambda: L.alien_hip_spmv_bsr2_f64(ctx.h, ncell, P(dro), P(dc), P(dv), P(x), P(y)),
+             36 * nnzb + 4 * (ncell + 1) + 16 * N),
+            ("bsr2_fp32", lambda: L.alien_hip_spmv_bsr2_f32(ctx.h, ncell, P(dro), P(dc), P(dv32), P(x32), P(y32)),
+             20 * nnzb + 4 * (ncell + 1) + 8 * N)):
+        for _ in range(3):
+            aa.check(fn())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(a.spmv_reps):
+            aa.check(fn())
+        e1.record()
+        e1.synchronize()
+        t = e0.elapsed_time(e1) / a.spmv_reps * 1e-3
+        bsr[name] = {"us": t * 1e6, "bytes": nbytes, "gbs": nbytes / t / 1e9}
+    del dro, dc, dv, dv32, x, x32, y, y32
+    torch.cuda.empty_cache()
+    theta = aa.gnn_init(0)
+    runs = [("fp64_bicgstab_ilu0_multicolour", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_ILU0, ilu_schedule=1,
+                                                    layout=aa.LAYOUT_RBCSR)),
+            ("fp64_bicgstab_jacobi_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
+            ("mixed_bicgstab_jacobi", dict(mode=aa.MODE_MIXED, precond=aa.PREC_DIAG)),
+            ("mixed_bicgstab_learned", dict(mode=aa.MODE_MIXED, precond=aa.PREC_LEARNED, gnn_theta=theta))]
+    if a.solvers:
+        keep = set(a.solvers.split(","))
+        runs = [r for r in runs if r[0] in keep]
+    res = {}
+    for name, kw in runs:
+        x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, tol=1e-10, max_iter=a.ilu_max_iter,
+                            poll_every=8, **kw)
+        err = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
+        log(f"{name}: solve {r['solve_s']:.3f}s setup {r['setup_s']:.3f}s its={r['iterations']} "
+            f"status={r['status']} fallback={r['fallback']} true_rel_res={r['true_rel_res']:.3e}")
+        res[name] = {"time_s": r["solve_s"], "setup_s": r["setup_s"], "iterations": r["iterations"],
+                     "converged": bool(r["status"] == 1 and r["true_rel_res"] < 1.05e-10),
+                     "true_rel_res": r["true_rel_res"], "rel_err_vs_x_true": err}
+        if kw["precond"] == aa.PREC_LEARNED:
+            res[name]["fallback"] = r["fallback"]
+    sp = time_spmv(aa, torch, ctx, ro, cols, vals, a.spmv_reps)
+    spmv = {k: {"us": sp[k] * 1e6, "gbs": nbytes / sp[k] / 1e9} for k, nbytes in
+            (("csr_fp64", spmv_bytes_fp64(N, nnz)), ("rbcsr_fp64", spmv_bytes_fp64(N, nnz)),
+             ("rbdia_fp64", rbdia_bytes(N, nnz, sp["rbdia_slots"], 8)))}
+    spmv.update(bsr)
+    best = min((v["time_s"] for v in res.values() if v["converged"]), default=None)
+    print(json.dumps({"metric": "time-to-1e-10 residual (s), BiCGStab, 2 unknowns per cell", "value": best,
+                      "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64 (mixed: f32+f64)",
+                      "data": "synthetic",
+                      "config": {"workload": f"2x2-block log-normal diffusion {n}^3 cells (sigma=2, coupling 0.3), "
+                                             f"BiCGStab on the interleaved scalar CSR", "cells": ncell, "N": N,
+                                 "blocks": nnzb, "nnz": nnz},
+                      "solvers": res, "spmv": spmv}), flush=True)
+    ctx.close()
+
+
+def run_eval(a):
+    """Learned preconditioner on unseen systems (not the default line): `--seeds` log-normal systems
+    (sigma 2, seeds 1..S, none used in training) at n^3, CG to FP64 relative residual 1e-8 with the
+    learned M (mixed and FP64), Jacobi (mixed and FP64) and multicolour ILU0 (REFERENCE mode);
+    per-seed iterations and solve times and their summary."""
+    import torch
+
+    import alien_amd as aa
+    n = a.n
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx = aa.Context(stream.cuda_stream)
+    if a.train_steps > 0:
+        theta, training = train_network(aa, ctx, a)
+    else:
+        theta, training = aa.gnn_init(0), {"steps": 0, "chosen": "init"}
+    runs = [("learned_mixed", dict(mode=aa.MODE_MIXED, precond=aa.PREC_LEARNED, gnn_theta=theta)),
+            ("mixed_jacobi", dict(mode=aa.MODE_MIXED, precond=aa.PREC_DIAG)),
+            ("fp64_learned", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_LEARNED, gnn_theta=theta)),
+            ("fp64_jacobi", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
+            ("fp64_ilu0_multicolour", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_ILU0, ilu_schedule=1,
+                                           layout=aa.LAYOUT_RBCSR))]
+    per = {name: {"iterations": [], "time_s": [], "setup_s": [], "converged": []} for name, _ in runs}
+    for seed in range(1, a.seeds + 1):
+        ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=[2.0, 1.0, 1.0, 1.0], seed=seed)
+        xt = aa.uniform01(seed, len(ro) - 1)
+        b = gpu_rhs(aa, torch, ctx, ro, cols, vals, xt)
+        line = []
+        for name, kw in runs:
+            _, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, tol=1e-8, max_iter=20000, poll_every=16,
+                                **kw)
+            ok = bool(r["status"] == 1 and r["true_rel_res"] < 1.05e-8 and not r["fallback"])
+            for k, v in (("iterations", r["iterations"]), ("time_s", r["solve_s"]), ("setup_s", r["setup_s"]),
+                         ("converged", ok)):
+                per[name][k].append(v)
+            line.append(f"{name} {r['iterations']} its {r['solve_s'] * 1e3:.1f} ms")
+        log(f"seed {seed}: " + ", ".join(line))
+    summary = {}
+    lt = np.array(per["learned_mixed"]["time_s"])
+    for name, d in per.items():
+        summary[name] = {"mean_iterations": float(np.mean(d["iterations"])), "mean_time_s": float(np.mean(d["time_s"])),
+                         "mean_setup_s": float(np.mean(d["setup_s"])), "all_converged": bool(all(d["converged"]))}
+        if name != "learned_mixed":
+            summary[name]["geomean_speedup_of_learned_mixed"] = float(np.exp(np.mean(np.log(np.array(d["time_s"]) / lt))))
+    print(json.dumps({"metric": "time-to-1e-8 residual (s) on unseen systems, learned+mixed vs baselines",
+                      "value": summary["learned_mixed"]["mean_time_s"], "unit": "s", "n_gpus": 1,
+                      "higher_is_better": False, "dtype": "f32+f64", "data": "synthetic",
+                      "config": {"workload": f"log-normal diffusion {n}^3 (sigma=2), seeds 1..{a.seeds}, CG to 1e-8",
+                                 "N": n ** 3},
+                      "training": training, "summary": summary, "per_seed": per}), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=256, help="grid edge (256 -> the 256^3 workload)")
+    ap.add_argument("--n", type=int, default=None,
+                    help="grid edge (default 256 for lognormal, 128 for convdiff and eval, 192 for blocks2)")
+    ap.add_argument("--seeds", type=int, default=20, help="eval workload: number of unseen systems")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--train-cg-steps", type=int, default=16,
                     help="training loss: residual reduction after this many PCG steps (0: Kaporin's bound)")
@@ -259,17 +440,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernels and a reference copy kernel (for rocprofv3 --pmc runs)")
-    ap.add_argument("--workload", default="lognormal", choices=["lognormal", "convdiff"],
-                    help="lognormal: the default line; convdiff: the nonsymmetric BiCGStab configuration "
-                         "(use with --n 128)")
+    ap.add_argument("--workload", default="lognormal", choices=["lognormal", "convdiff", "blocks2", "eval"],
+                    help="lognormal: the default line; convdiff: the nonsymmetric BiCGStab configuration; "
+                         "blocks2: two unknowns per cell, BiCGStab; eval: the learned M on unseen seeds")
     a = ap.parse_args()
+    if a.n is None:
+        a.n = {"lognormal": 256, "convdiff": 128, "blocks2": 192, "eval": 128}[a.workload]
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported in this build "
               f"(the row-block sharding over several GPUs is a design, not an implementation)", file=sys.stderr)
         sys.exit(2)
 
-    if a.workload == "convdiff":
-        run_convdiff(a)
+    if a.workload != "lognormal":
+        {"convdiff": run_convdiff, "blocks2": run_blocks2, "eval": run_eval}[a.workload](a)
         return
 
     import torch
@@ -289,18 +472,9 @@ def main():
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     ctx = aa.Context(stream.cuda_stream)
-    # b = A x_true with the GPU's CSR SpMV (bit-exact with the sequential reference product)
     import ctypes as C
     P = lambda t: C.c_void_p(t.data_ptr())
-    dev = lambda arr: torch.from_numpy(np.ascontiguousarray(arr)).to("cuda")
-    dro, dc, dv, dx = dev(ro), dev(cols), dev(vals), dev(xt)
-    db = torch.empty(N, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    aa.check(aa.lib().alien_hip_spmv_csr_f64(ctx.h, N, P(dro), P(dc), P(dv), P(dx), P(db)))
-    ctx.synchronize()
-    b = db.cpu().numpy()
-    del dro, dc, dv, dx, db
-    torch.cuda.empty_cache()
+    b = gpu_rhs(aa, torch, ctx, ro, cols, vals, xt)
 
     if a.spmv_only:
         sp = time_spmv(aa, torch, ctx, ro, cols, vals, a.spmv_reps)

@@ -278,7 +278,9 @@ typedef struct {
   double cheb_beta;
   double setup_s;       /* conversion + preconditioner init, hipEvent-timed */
   double solve_s;       /* the Krylov solve, hipEvent-timed */
-  int    fallback;      /* LEARNED: 1 if the learned M stalled and Jacobi took over */
+  int    fallback;      /* bit 0 (LEARNED): the learned M stalled and Jacobi took over;
+                           bit 1 (MIXED): the FP32 inner iteration overflowed, or (BiCGStab)
+                           stagnated, and the solve finished in FP64 from the last reliable x */
   int    pad_;
 } alien_hip_solve_result;
 

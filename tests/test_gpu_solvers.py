@@ -246,6 +246,27 @@ def test_mixed_bicgstab_reaches_fp64_tolerance(ctx, case):
     assert r["iterations"] <= 2 * r64["iterations"] + 20
 
 
+def test_mixed_bicgstab_hands_over_to_fp64_when_fp32_stagnates(ctx):
+    """2-unknown block system with sigma 3 on 20^3 cells, to 1e-12: stored in
+    FP32 the operator is too ill-conditioned for the FP32 inner iteration to keep
+    reducing the true residual (a CPU emulation of the driver overflows after 4
+    reliable updates). The driver detects the stagnation or the overflow, keeps
+    the last reliable x and finishes in FP64 (fallback bit 2); the true residual
+    meets the tolerance."""
+    ro, cols, vals = bsr2_scalar_csr(20, sigma=3.0)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case mixed BiCGStab FP32 stagnation")
+    kw = dict(solver=aa.BICGS, precond=aa.PREC_DIAG, tol=1e-12, max_iter=8000, poll_every=4)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_MIXED, **kw)
+    _, r64 = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_GRAPH, **kw)
+    say("  mixed", r["iterations"], r["outer"], r["fallback"], r["true_rel_res"], "fp64", r64["iterations"])
+    assert r64["status"] == 1
+    assert r["status"] == 1 and r["fallback"] == 2
+    assert r["true_rel_res"] < 2e-12
+    assert np.all(np.isfinite(x))
+
+
 @pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
 def test_learned_bicgstab_on_two_unknown_system(ctx, mode):
     """BiCGStab with the learned M = G G^T on the 2-unknown block system, to an
