@@ -11,6 +11,6 @@ from ._capi import (  # noqa: F401
     gen_csr, gen_nnz,
     gnn_forward, gnn_init, gnn_loss_grad, gnn_nparams, gnn_train, group_by_level, host_color, host_dcol,
     host_levels, lib, mtx_read_csr, mtx_read_vector, mtx_write_csr, mtx_write_vector,
-    rbcsr_bundle_offsets, rbdia_build, solve_csr, uniform01)
+    rbcsr_bundle_offsets, rbdia_build, solve_csr, uniform01, vblock_to_csr)
 
 __version__ = "0.1.0"

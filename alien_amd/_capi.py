@@ -87,6 +87,16 @@ _SIGS = [
                                             c_void_p, c_void_p, c_void_p]),
     ("alien_hip_spmv_bsr2_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_spmv_bsr2_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_vblock_f64", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 8),
+    ("alien_hip_vblock_scalar_nnz", C.c_int64, [C.c_int, c_int_p, c_int_p, c_int_p]),
+    ("alien_hip_vblock_to_csr", C.c_int, [C.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_double_p,
+                                          c_int_p, c_int_p, c_double_p]),
+    ("alien_hip_rbbsr2_fill_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_void_p]),
+    ("alien_hip_rbbsr2_fill_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_void_p]),
+    ("alien_hip_spmv_rbbsr2_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_rbbsr2_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_invdiag_csr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_invdiag_bellpack_f64", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p,
                                                  c_void_p]),
@@ -233,6 +243,25 @@ def host_dcol(ro, cols):
     d = np.empty(n, np.int32)
     check(lib().alien_hip_host_dcol(n, iptr(ro), iptr(cols), iptr(d)))
     return d
+
+
+def vblock_to_csr(ro, cols, block_cols, block_size, block_offset, vals=None):
+    """Scalar CSR (row_offset, cols, vals or None) of a variable-block CSR matrix, entries in the
+    order of the variable-block product."""
+    ro, cols, bcols, bsize, boff = (np.ascontiguousarray(a, dtype=np.int32)
+                                    for a in (ro, cols, block_cols, block_size, block_offset))
+    nb = len(ro) - 1
+    nnz = lib().alien_hip_vblock_scalar_nnz(nb, iptr(ro), iptr(cols), iptr(bsize))
+    check(0 if nnz >= 0 else -1)
+    n = int(boff[-1] + bsize[-1]) if nb else 0
+    sro = np.zeros(n + 1, np.int32)
+    scols = np.empty(max(nnz, 1), np.int32)
+    svals = None if vals is None else np.empty(max(nnz, 1))
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.float64)
+    check(lib().alien_hip_vblock_to_csr(nb, iptr(ro), iptr(cols), iptr(bcols), iptr(bsize), iptr(boff),
+                                        None if v is None else dptr(v), iptr(sro), iptr(scols),
+                                        None if svals is None else dptr(svals)))
+    return sro, scols[:nnz], (None if svals is None else svals[:nnz])
 
 
 def bellpack_build(ro, cols, vals=None, slice_=1024, masks=False):

@@ -46,6 +46,14 @@ int rbcsr_bundle_offsets(int nrows, const int* ro, int* bundle_off);
 long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, const int* ke, int max_w,
                      std::vector<int>& bo, std::vector<int>& doff, std::vector<uint64_t>& dmask);
 // ---- MatrixMarket system I/O (host/mtx.cpp) ----
+// Variable-block CSR (VBlock: block row i is block_size[i] scalar rows starting at
+// block_offset[i]; block j of a row is row-major from vals[block_cols[j]]) -> scalar CSR.
+// Scalar row block_offset[i] + k holds, block by block and column by column, the
+// entries the variable-block product sums for it, in its order. Returns -1 when
+// block_offset is not the running sum of block_size or a block column is out of range.
+long vblock_scalar_nnz(int nbrows, const int* ro, const int* cols, const int* block_size);
+int vblock_to_csr(int nbrows, const int* ro, const int* cols, const int* block_cols, const int* block_size,
+                  const int* block_offset, const double* vals, int* sro, int* scols, double* svals);
 int mtx_read_csr(const char* path, int& n, std::vector<int>& ro, std::vector<int>& cols, std::vector<double>& vals,
                  std::string& err);
 int mtx_read_vector(const char* path, std::vector<double>& v, std::string& err);

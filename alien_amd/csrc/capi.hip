@@ -372,6 +372,76 @@ int alien_hip_spmv_bsr2_f64(alien_hip_ctx* ctx, int nb, const int* ro, const int
   API_END
 }
 
+int alien_hip_rbbsr2_fill_f64(alien_hip_ctx* ctx, int nb, const int* ro, const int* cols, const double* vals,
+                              const int* bo, int* bc, double* bv) {
+  CTX_OK(ctx);
+  REQUIRE(nb >= 0 && (nb == 0 || (ro && cols && vals && bo && bc && bv)), "rbbsr2_fill_f64: bad arguments");
+  API_BEGIN
+  launch_rbbsr2_fill<double>(ctx->stream, nb, ro, cols, vals, bo, bc, bv);
+  LAUNCHED("rbbsr2_fill_f64");
+  API_END
+}
+
+int alien_hip_rbbsr2_fill_f32(alien_hip_ctx* ctx, int nb, const int* ro, const int* cols, const double* vals,
+                              const int* bo, int* bc, float* bv) {
+  CTX_OK(ctx);
+  REQUIRE(nb >= 0 && (nb == 0 || (ro && cols && vals && bo && bc && bv)), "rbbsr2_fill_f32: bad arguments");
+  API_BEGIN
+  launch_rbbsr2_fill<float>(ctx->stream, nb, ro, cols, vals, bo, bc, bv);
+  LAUNCHED("rbbsr2_fill_f32");
+  API_END
+}
+
+int alien_hip_spmv_rbbsr2_f64(alien_hip_ctx* ctx, int nb, const int* bo, const int* bc, const double* bv,
+                              const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(nb >= 0 && (nb == 0 || (bo && bc && bv && x && y)), "spmv_rbbsr2_f64: bad arguments");
+  REQUIRE(aligned(x, 16) && aligned(y, 16), "spmv_rbbsr2_f64: misaligned vectors");
+  API_BEGIN
+  launch_spmv_rbbsr2<double>(ctx->stream, nb, bo, bc, bv, x, y);
+  LAUNCHED("spmv_rbbsr2_f64");
+  API_END
+}
+
+int alien_hip_spmv_rbbsr2_f32(alien_hip_ctx* ctx, int nb, const int* bo, const int* bc, const float* bv,
+                              const float* x, float* y) {
+  CTX_OK(ctx);
+  REQUIRE(nb >= 0 && (nb == 0 || (bo && bc && bv && x && y)), "spmv_rbbsr2_f32: bad arguments");
+  REQUIRE(aligned(x, 8) && aligned(y, 8), "spmv_rbbsr2_f32: misaligned vectors");
+  API_BEGIN
+  launch_spmv_rbbsr2<float>(ctx->stream, nb, bo, bc, bv, x, y);
+  LAUNCHED("spmv_rbbsr2_f32");
+  API_END
+}
+
+int alien_hip_spmv_vblock_f64(alien_hip_ctx* ctx, int nb, const int* ro, const int* cols, const int* bcols,
+                              const int* bsize, const int* boff, const double* vals, const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(nb >= 0 && (nb == 0 || (ro && cols && bcols && bsize && boff && vals && x && y)),
+          "spmv_vblock_f64: bad arguments");
+  API_BEGIN
+  launch_spmv_vblock(ctx->stream, nb, ro, cols, bcols, bsize, boff, vals, x, y);
+  LAUNCHED("spmv_vblock_f64");
+  API_END
+}
+
+int64_t alien_hip_vblock_scalar_nnz(int nb, const int* ro, const int* cols, const int* bsize) {
+  if (!(nb >= 0 && ro && (nb == 0 || (cols && bsize))))
+    return fail(ALIEN_HIP_EINVAL, "vblock_scalar_nnz: bad arguments"), -1;
+  return vblock_scalar_nnz(nb, ro, cols, bsize);
+}
+
+int alien_hip_vblock_to_csr(int nb, const int* ro, const int* cols, const int* bcols, const int* bsize,
+                            const int* boff, const double* vals, int* sro, int* scols, double* svals) {
+  REQUIRE(nb >= 0 && ro && sro && (nb == 0 || (cols && bcols && bsize && boff && scols)),
+          "vblock_to_csr: bad arguments");
+  API_BEGIN
+  if (vblock_to_csr(nb, ro, cols, bcols, bsize, boff, vals, sro, scols, svals) != 0)
+    return fail(ALIEN_HIP_EINVAL, "vblock_to_csr: block_offset is not the running sum of block_size, "
+                                  "or a block column is out of range");
+  API_END
+}
+
 int alien_hip_spmv_bsr2_f32(alien_hip_ctx* ctx, int nb, const int* ro, const int* cols, const float* vals,
                             const float* x, float* y) {
   CTX_OK(ctx);

@@ -212,4 +212,42 @@ long rbdia_structure(int nrows, const int* ro, const int* cols, const int* kb, c
   return acc;
 }
 
+// Variable-block CSR -> scalar CSR, the order of _seqMultVariableBlock
+// (SimpleCSRMatrixMultT.h:439-476): for output row (i, kr) the blocks of row i in
+// order, inside each its columns kc of row kr.
+long vblock_scalar_nnz(int nbrows, const int* ro, const int* cols, const int* block_size) {
+  long nnz = 0;
+  for (int i = 0; i < nbrows; ++i)
+    for (int j = ro[i]; j < ro[i + 1]; ++j) nnz += (long)block_size[i] * block_size[cols[j]];
+  return nnz;
+}
+
+int vblock_to_csr(int nbrows, const int* ro, const int* cols, const int* block_cols, const int* block_size,
+                  const int* block_offset, const double* vals, int* sro, int* scols, double* svals) {
+  long off = 0;
+  for (int i = 0; i < nbrows; ++i) {
+    if (block_size[i] < 1 || block_offset[i] != off) return -1;
+    off += block_size[i];
+  }
+  for (int j = 0; j < (nbrows ? ro[nbrows] : 0); ++j)
+    if (cols[j] < 0 || cols[j] >= nbrows) return -1;
+  long pos = 0;
+  sro[0] = 0;
+  for (int i = 0; i < nbrows; ++i) {
+    const int bsr = block_size[i];
+    for (int kr = 0; kr < bsr; ++kr) {
+      for (int j = ro[i]; j < ro[i + 1]; ++j) {
+        const int col = cols[j], bsc = block_size[col];
+        for (int kc = 0; kc < bsc; ++kc) {
+          scols[pos] = block_offset[col] + kc;
+          if (vals) svals[pos] = vals[(long)block_cols[j] + (long)kr * bsc + kc];
+          ++pos;
+        }
+      }
+      sro[block_offset[i] + kr + 1] = (int)pos;
+    }
+  }
+  return 0;
+}
+
 } // namespace alien_hip

@@ -70,6 +70,15 @@ void launch_filu0_sweep(hipStream_t, int n, const int* ro, const int* cols, cons
                         const double* prev, double* lu);
 template <typename T>
 void launch_spmv_bsr2(hipStream_t, int nb, const int* ro, const int* cols, const T* vals, const T* x, T* y);
+// RB-BSR2: 2x2 block rows bundled by 64 (bundle offsets as RB-CSR's over the block rows)
+template <typename T>
+void launch_spmv_rbbsr2(hipStream_t, int nbr, const int* bo, const int* bc, const T* bv, const T* x, T* y);
+template <typename T>
+void launch_rbbsr2_fill(hipStream_t, int nbr, const int* ro, const int* cols, const double* vals, const int* bo,
+                        int* bc, T* bv);
+// variable-block CSR product, one lane per block row
+void launch_spmv_vblock(hipStream_t, int nbr, const int* ro, const int* cols, const int* bcols, const int* bsize,
+                        const int* boff, const double* vals, const double* x, double* y);
 void launch_invdiag_csr(hipStream_t, int n, const int* dcol, const double* vals, double* y);
 void launch_invdiag_bellpack(hipStream_t, int n, int slice, const int* bro, const int* bc, const double* bv,
                              double* y);

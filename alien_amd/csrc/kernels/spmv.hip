@@ -299,6 +299,115 @@ __global__ void __launch_bounds__(kBlock) k_spmv_bsr2(int nb, const int* __restr
   reinterpret_cast<V2*>(y)[i] = out;
 }
 
+// ---- row-bundled 2x2 block CSR ("RB-BSR2") -------------------------------------------------
+// Bundles of 64 block rows (one wavefront, lane = block row); slot j of bundle
+// b holds block j of each lane's row: column at bc[s 64 + lane] (-1 = padding)
+// and the four values in planes, value q (column-major m[ieq + 2 iu]) at
+// bv[(4 s + q) 64 + lane], s = bo[b] + j. Every load of a slot is one
+// contiguous 64-lane access; per lane the blocks and the four products are
+// taken in the order of k_spmv_bsr2, so the result is the same bit for bit.
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock) k_spmv_rbbsr2(int nbr, const int* __restrict__ bo,
+                                                            const int* __restrict__ bc,
+                                                            const T* __restrict__ bv,
+                                                            const T* __restrict__ x, T* __restrict__ y) {
+  typedef typename Vec4<T>::half V2;
+  constexpr int U = 4;
+  const int lane = threadIdx.x & (kWave - 1);
+  const BundleWalk bw = bundle_walk(nbr);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
+    const int s0 = bo[b], w = bo[b + 1] - s0;
+    T y0 = 0, y1 = 0;
+    for (int j = 0; j < w; j += U) {
+      int c[U];
+      T m[U][4];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long sl = (long)s0 + min(j + u, w - 1);
+        c[u] = __builtin_nontemporal_load(bc + sl * kWave + lane);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) m[u][q] = __builtin_nontemporal_load(bv + (4 * sl + q) * kWave + lane);
+      }
+      bool on[U];
+      V2 xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        on[u] = (j + u < w) && c[u] >= 0;
+        xv[u] = reinterpret_cast<const V2*>(x)[on[u] ? c[u] : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (on[u]) {
+          y0 = y0 + m[u][0] * xv[u].x;
+          y0 = y0 + m[u][2] * xv[u].y;
+          y1 = y1 + m[u][1] * xv[u].x;
+          y1 = y1 + m[u][3] * xv[u].y;
+        }
+    }
+    const int row = b * kWave + lane;
+    if (row < nbr) {
+      V2 out;
+      out.x = y0;
+      out.y = y1;
+      reinterpret_cast<V2*>(y)[row] = out;
+    }
+  }
+}
+
+// Device conversion 2x2 block CSR -> RB-BSR2 (one lane per block row writes its slots).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_rbbsr2_fill(int nbr, const int* __restrict__ ro,
+                                                         const int* __restrict__ cols,
+                                                         const double* __restrict__ vals,
+                                                         const int* __restrict__ bo, int* __restrict__ bc,
+                                                         T* __restrict__ bv) {
+  const int nb = (nbr + kBundle - 1) / kBundle;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int b = t / kBundle, lane = t % kBundle;
+  if (b >= nb) return;
+  const int row = t;
+  const int s0 = bo[b], w = bo[b + 1] - s0;
+  int k0 = 0, len = 0;
+  if (row < nbr) {
+    k0 = ro[row];
+    len = ro[row + 1] - k0;
+  }
+  for (int j = 0; j < w; ++j) {
+    const long sl = (long)s0 + j;
+    const bool on = j < len;
+    bc[sl * kWave + lane] = on ? cols[k0 + j] : -1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bv[(4 * sl + q) * kWave + lane] = on ? (T)vals[4L * (k0 + j) + q] : T(0);
+  }
+}
+
+// ---- variable-block CSR -------------------------------------------------------------------
+// _seqMultVariableBlock (SimpleCSRMatrixMultT.h:439-476), one lane per block row:
+// output row kr of block row i sums, block by block, the row-major block row kr
+// against x's block, in the reference's order.
+__global__ void __launch_bounds__(kBlock) k_spmv_vblock(int nbr, const int* __restrict__ ro,
+                                                         const int* __restrict__ cols,
+                                                         const int* __restrict__ bcols,
+                                                         const int* __restrict__ bsize,
+                                                         const int* __restrict__ boff,
+                                                         const double* __restrict__ vals,
+                                                         const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nbr) return;
+  const int bsr = bsize[i], k0 = ro[i], k1 = ro[i + 1];
+  for (int kr = 0; kr < bsr; ++kr) {
+    double acc = 0.;
+    for (int j = k0; j < k1; ++j) {
+      const int col = cols[j];
+      const int bsc = bsize[col];
+      const double* m = vals + (long)bcols[j] + (long)kr * bsc;
+      const double* xb = x + boff[col];
+      for (int kc = 0; kc < bsc; ++kc) acc = acc + m[kc] * xb[kc];
+    }
+    y[boff[i] + kr] = acc;
+  }
+}
+
 // ---- inverse diagonal -------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_invdiag_csr(int n, const int* __restrict__ dcol,
                                                          const double* __restrict__ vals,
@@ -423,6 +532,24 @@ template <typename T>
 void launch_spmv_bsr2(hipStream_t st, int nb, const int* ro, const int* cols, const T* vals, const T* x, T* y) {
   if (nb > 0) hipLaunchKernelGGL(k_spmv_bsr2<T>, rows_grid(nb), dim3(kBlock), 0, st, nb, ro, cols, vals, x, y);
 }
+template <typename T>
+void launch_spmv_rbbsr2(hipStream_t st, int nbr, const int* bo, const int* bc, const T* bv, const T* x, T* y) {
+  if (nbr > 0)
+    hipLaunchKernelGGL(k_spmv_rbbsr2<T>, dim3((unsigned)bundle_blocks(nbr, (int)sizeof(T))), dim3(kRowBlock), 0, st,
+                       nbr, bo, bc, bv, x, y);
+}
+template <typename T>
+void launch_rbbsr2_fill(hipStream_t st, int nbr, const int* ro, const int* cols, const double* vals, const int* bo,
+                        int* bc, T* bv) {
+  if (nbr > 0)
+    hipLaunchKernelGGL(k_rbbsr2_fill<T>, rows_grid(nbr), dim3(kBlock), 0, st, nbr, ro, cols, vals, bo, bc, bv);
+}
+void launch_spmv_vblock(hipStream_t st, int nbr, const int* ro, const int* cols, const int* bcols, const int* bsize,
+                        const int* boff, const double* vals, const double* x, double* y) {
+  if (nbr > 0)
+    hipLaunchKernelGGL(k_spmv_vblock, rows_grid(nbr), dim3(kBlock), 0, st, nbr, ro, cols, bcols, bsize, boff, vals,
+                       x, y);
+}
 void launch_invdiag_csr(hipStream_t st, int n, const int* dcol, const double* vals, double* y) {
   if (n > 0) hipLaunchKernelGGL(k_invdiag_csr, rows_grid(n), dim3(kBlock), 0, st, n, dcol, vals, y);
 }
@@ -444,7 +571,10 @@ void launch_invdiag_bellpack(hipStream_t st, int n, int slice, const int* bro, c
                                      const int*, int*, T*);                                             \
   template void launch_spmv_bellpack<T>(hipStream_t, int, int, const int*, const int*, const T*,        \
                                         const T*, T*);                                                  \
-  template void launch_spmv_bsr2<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*);
+  template void launch_spmv_bsr2<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*);     \
+  template void launch_spmv_rbbsr2<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*);   \
+  template void launch_rbbsr2_fill<T>(hipStream_t, int, const int*, const int*, const double*, const int*, \
+                                      int*, T*);
 ALIEN_HIP_INST_SPMV(double)
 ALIEN_HIP_INST_SPMV(float)
 

@@ -301,20 +301,45 @@ def run_blocks2(a):
     b = gpu_rhs(aa, torch, ctx, ro, cols, vals, xt)
     P = lambda t: C.c_void_p(t.data_ptr())
     dev = lambda arr: torch.from_numpy(np.ascontiguousarray(arr)).to("cuda")
-    # 2x2-block SpMV (the _seqMultBlock product), FP64 and FP32 block storage
+    # 2x2-block SpMV (the _seqMultBlock product), FP64 and FP32 block storage: block CSR with one
+    # lane per block row, and the row-bundled layout (RB-BSR2)
     dro, dc, dv, dv32 = dev(bro), dev(bcols), dev(bvals), dev(bvals.astype(np.float32))
     x = torch.rand(N, dtype=torch.float64, device="cuda")
     x32 = x.float()
     y = torch.empty(N, dtype=torch.float64, device="cuda")
     y32 = torch.empty(N, dtype=torch.float32, device="cuda")
+    bbo = aa.rbcsr_bundle_offsets(bro)
+    dbbo = dev(bbo)
+    rslots = int(bbo[-1]) * 64
+    rbc = torch.empty(rslots, dtype=torch.int32, device="cuda")
+    rbv = torch.empty(4 * rslots, dtype=torch.float64, device="cuda")
+    rbv32 = torch.empty(4 * rslots, dtype=torch.float32, device="cuda")
+    yr = torch.empty(N, dtype=torch.float64, device="cuda")
+    yr32 = torch.empty(N, dtype=torch.float32, device="cuda")
+    dvv = dev(bvals.reshape(-1, 2, 2).transpose(0, 2, 1).reshape(-1))  # row-major blocks
+    dvbc = torch.arange(0, 4 * nnzb, 4, dtype=torch.int32, device="cuda")
+    dvbs = torch.full((ncell,), 2, dtype=torch.int32, device="cuda")
+    dvbo = torch.arange(0, 2 * ncell, 2, dtype=torch.int32, device="cuda")
+    yv = torch.empty(N, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     L = aa.lib()
+    aa.check(L.alien_hip_rbbsr2_fill_f64(ctx.h, ncell, P(dro), P(dc), P(dv), P(dbbo), P(rbc), P(rbv)))
+    aa.check(L.alien_hip_rbbsr2_fill_f32(ctx.h, ncell, P(dro), P(dc), P(dv), P(dbbo), P(rbc), P(rbv32)))
     bsr = {}
+    b64, b32 = 36 * nnzb + 4 * (ncell + 1) + 16 * N, 20 * nnzb + 4 * (ncell + 1) + 8 * N
     for name, fn, nbytes in (
-            ("bsr2_fp64", lambda: L.alien_hip_spmv_bsr2_f64(ctx.h, ncell, P(dro), P(dc), P(dv), P(x), P(y)),
-             36 * nnzb + 4 * (ncell + 1) + 16 * N),
+            ("bsr2_fp64", lambda: L.alien_hip_spmv_bsr2_f64(ctx.h, ncell, P(dro), P(dc), P(dv), P(x), P(y)), b64),
             ("bsr2_fp32", lambda: L.alien_hip_spmv_bsr2_f32(ctx.h, ncell, P(dro), P(dc), P(dv32), P(x32), P(y32)),
-             20 * nnzb + 4 * (ncell + 1) + 8 * N)):
+             b32),
+            ("rbbsr2_fp64", lambda: L.alien_hip_spmv_rbbsr2_f64(ctx.h, ncell, P(dbbo), P(rbc), P(rbv), P(x), P(yr)),
+             b64),
+            ("rbbsr2_fp32", lambda: L.alien_hip_spmv_rbbsr2_f32(ctx.h, ncell, P(dbbo), P(rbc), P(rbv32), P(x32),
+                                                                 P(yr32)), b32),
+            # the same matrix as a variable-block matrix (all blocks 2x2, row-major values, block value
+            # offsets and block sizes / offsets streamed as well), one lane per block row
+            ("vblock_fp64", lambda: L.alien_hip_spmv_vblock_f64(ctx.h, ncell, P(dro), P(dc), P(dvbc), P(dvbs),
+                                                                P(dvbo), P(dvv), P(x), P(yv)),
+             44 * nnzb + 12 * ncell + 4 + 16 * N)):
         for _ in range(3):
             aa.check(fn())
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -325,7 +350,10 @@ def run_blocks2(a):
         e1.synchronize()
         t = e0.elapsed_time(e1) / a.spmv_reps * 1e-3
         bsr[name] = {"us": t * 1e6, "bytes": nbytes, "gbs": nbytes / t / 1e9}
-    del dro, dc, dv, dv32, x, x32, y, y32
+    ctx.synchronize()
+    bsr["rbbsr2_equals_bsr2"] = bool(torch.equal(y, yr) and torch.equal(y32, yr32))
+    bsr["vblock_equals_bsr2"] = bool(torch.equal(y, yv))
+    del dro, dc, dv, dv32, x, x32, y, y32, rbc, rbv, rbv32, yr, yr32, dbbo, dvv, dvbc, dvbs, dvbo, yv
     torch.cuda.empty_cache()
     theta = aa.gnn_init(0)
     runs = [("fp64_bicgstab_ilu0_multicolour", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_ILU0, ilu_schedule=1,

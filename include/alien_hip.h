@@ -118,6 +118,44 @@ int alien_hip_spmv_bsr2_f64(alien_hip_ctx* ctx, int nbrows, const int* row_offse
                             const double* vals, const double* x, double* y);
 int alien_hip_spmv_bsr2_f32(alien_hip_ctx* ctx, int nbrows, const int* row_offset, const int* cols,
                             const float* vals, const float* x, float* y);
+/* The same product over row-bundled 2x2 blocks ("RB-BSR2"): 64 block rows per
+ * bundle, bundle offsets as alien_hip_rbcsr_bundle_offsets over the block row
+ * offsets; slot s = bundle_off[b] + j holds block j of each row, column at
+ * bcols[64 s + l] (-1 = padding) and value q (column-major, m[ieq + 2 iu]) at
+ * bvals[64 (4 s + q) + l]. rbbsr2_fill converts a device 2x2 block CSR (FP64
+ * values, 4 per block) into bcols / bvals (64 x bundle_off[nbundles] columns,
+ * 4x that many values). Bit-exact with the spmv_bsr2 kernels. */
+int alien_hip_rbbsr2_fill_f64(alien_hip_ctx* ctx, int nbrows, const int* row_offset, const int* cols,
+                              const double* vals, const int* bundle_off, int* bcols, double* bvals);
+int alien_hip_rbbsr2_fill_f32(alien_hip_ctx* ctx, int nbrows, const int* row_offset, const int* cols,
+                              const double* vals, const int* bundle_off, int* bcols, float* bvals);
+int alien_hip_spmv_rbbsr2_f64(alien_hip_ctx* ctx, int nbrows, const int* bundle_off, const int* bcols,
+                              const double* bvals, const double* x, double* y);
+int alien_hip_spmv_rbbsr2_f32(alien_hip_ctx* ctx, int nbrows, const int* bundle_off, const int* bcols,
+                              const float* bvals, const float* x, float* y);
+/* Variable-block CSR product: SimpleCSRMatrixMultT::_seqMultVariableBlock
+ * (SimpleCSRMatrixMultT.h:439-476). Block row i holds block_size[i] scalar rows,
+ * starting at block_offset[i] in x and y (block_offset = running sum of
+ * block_size, VBlockImpl::offset); block j of a row (column cols[j]) is
+ * block_size[i] x block_size[cols[j]], row-major, from vals[block_cols[j]]
+ * (CSRStructInfo::getBlockCols). One lane per block row, the reference's order. */
+int alien_hip_spmv_vblock_f64(alien_hip_ctx* ctx, int nbrows, const int* row_offset, const int* cols,
+                              const int* block_cols, const int* block_size, const int* block_offset,
+                              const double* vals, const double* x, double* y);
+/* Host: the scalar CSR of a variable-block matrix. Row block_offset[i] + k lists,
+ * block by block and column by column, the entries the product above sums for
+ * it, in its order, so the CSR / RB-CSR / RB-DIA kernels compute the same sums
+ * bit for bit. vblock_scalar_nnz gives the entry count; vblock_to_csr fills
+ * srow_offset[n + 1], scols[nnz] and (when svals is not NULL) svals[nnz], with
+ * n = block_offset[nbrows - 1] + block_size[nbrows - 1]; it fails when
+ * block_offset is not the running sum of block_size or a block column is out of
+ * range. */
+int64_t alien_hip_vblock_scalar_nnz(int nbrows, const int* row_offset_host, const int* cols_host,
+                                    const int* block_size_host);
+int alien_hip_vblock_to_csr(int nbrows, const int* row_offset_host, const int* cols_host,
+                            const int* block_cols_host, const int* block_size_host,
+                            const int* block_offset_host, const double* vals_host, int* srow_offset,
+                            int* scols, double* svals);
 /* inv_diag_i = 1 / a[dcol_i]: SimpleCSRMatrixMultT::computeInvDiag (:494-509).
  * Every row must store its diagonal (dcol_i then points at a_ii). */
 int alien_hip_invdiag_csr_f64(alien_hip_ctx* ctx, int nrows, const int* dcol, const double* vals,

@@ -42,6 +42,7 @@ _SIGS = [
     ("orc_dcol", None, [C.c_int, _ip, _ip, _ip]),
     ("orc_spmv_csr", None, [C.c_int, _ip, _ip, _dp, _dp, _dp]),
     ("orc_spmv_bsr", None, [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp]),
+    ("orc_spmv_vblock", None, [C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp]),
     ("orc_addlmult", None, [C.c_int, _ip, _ip, _ip, _dp, C.c_double, _dp, _dp]),
     ("orc_addumult", None, [C.c_int, _ip, _ip, _ip, _dp, C.c_double, _dp, _dp]),
     ("orc_invdiag_csr", None, [C.c_int, _ip, _dp, _dp]),
@@ -110,6 +111,16 @@ def spmv(ro, cols, vals, x):
 def spmv_bsr(ro, cols, vals, x, bs=2):
     y = np.empty((len(ro) - 1) * bs)
     lib().orc_spmv_bsr(len(ro) - 1, bs, _i(ro), _i(cols), _d(vals), _d(np.ascontiguousarray(x, np.float64)), _d(y))
+    return y
+
+
+def spmv_vblock(ro, cols, block_cols, block_size, block_offset, vals, x):
+    """Variable-block product; y has block_offset[-1] + block_size[-1] entries."""
+    nb = len(ro) - 1
+    n = int(block_offset[nb - 1] + block_size[nb - 1]) if nb else 0
+    y = np.empty(n)
+    lib().orc_spmv_vblock(nb, _i(ro), _i(cols), _i(block_cols), _i(block_size), _i(block_offset), _d(vals),
+                          _d(np.ascontiguousarray(x, np.float64)), _d(y))
     return y
 
 

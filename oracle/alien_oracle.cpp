@@ -447,6 +447,24 @@ void orc_spmv_bsr(int nbrows, int bs, const int* ro, const int* cols, const doub
   }
 }
 
+void orc_spmv_vblock(int nbrows, const int* ro, const int* cols, const int* block_cols, const int* block_size,
+                     const int* block_offset, const double* vals, const double* x, double* y) {
+  std::vector<double> tmpy;
+  for (int irow = 0; irow < nbrows; ++irow) {
+    const int bsr = block_size[irow];
+    tmpy.assign(bsr, 0.);
+    for (int j = ro[irow]; j < ro[irow + 1]; ++j) {
+      const int col = cols[j];
+      const int bsc = block_size[col];
+      const double* xb = x + block_offset[col];
+      const double* m = vals + block_cols[j];
+      for (int kr = 0; kr < bsr; ++kr)
+        for (int kc = 0; kc < bsc; ++kc) tmpy[kr] += m[kr * bsc + kc] * xb[kc];
+    }
+    for (int kr = 0; kr < bsr; ++kr) y[block_offset[irow] + kr] = tmpy[kr];
+  }
+}
+
 void orc_addlmult(int nrows, const int* ro, const int* cols, const int* dcol,
                   const double* vals, double alpha, const double* x, double* y) {
   for (int i = 0; i < nrows; ++i) {

@@ -31,6 +31,12 @@ void orc_spmv_csr(int nrows, const int* row_offset, const int* cols, const doubl
 /* SimpleCSRMatrixMultT::_seqMultBlock (SimpleCSRMatrixMultT.h:317-342), column-major blocks */
 void orc_spmv_bsr(int nbrows, int bs, const int* row_offset, const int* cols, const double* vals,
                   const double* x, double* y);
+/* SimpleCSRMatrixMultT::_seqMultVariableBlock (SimpleCSRMatrixMultT.h:439-476): block j of
+ * block row i is size[i] x size[cols[j]], row-major from vals[block_cols[j]]; block i of x
+ * and y starts at block_offset[i] */
+void orc_spmv_vblock(int nbrows, const int* row_offset, const int* cols, const int* block_cols,
+                     const int* block_size, const int* block_offset, const double* vals, const double* x,
+                     double* y);
 /* _seqAddLMult / _seqAddUMult (SimpleCSRMatrixMultT.h:193-233) */
 void orc_addlmult(int nrows, const int* row_offset, const int* cols, const int* dcol,
                   const double* vals, double alpha, const double* x, double* y);

@@ -228,6 +228,85 @@ def test_spmv_bsr2_bit_exact(ctx):
     np.testing.assert_allclose(host(y32, ctx)[:2 * nb], ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
 
 
+def ragged_bsr2(nb=700, seed=4):
+    """Random 2x2 block pattern: empty rows, rows of up to 40 blocks, ascending columns."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 12, nb)
+    lens[rng.choice(nb, 30, replace=False)] = 0
+    lens[rng.choice(nb, 5, replace=False)] = 40
+    ro = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(nb, l, replace=False)) for l in lens]).astype(np.int32)
+    vals = rng.standard_normal(4 * len(cols))
+    return ro, cols, vals
+
+
+@pytest.mark.parametrize("case", ["grid 11x9x7", "ragged 700", "one row"])
+def test_spmv_rbbsr2_bit_exact(ctx, case):
+    """Row-bundled 2x2 blocks: FP64 bit-exact with the oracle's _seqMultBlock
+    restatement, FP32 bit-exact with the block-CSR FP32 kernel (same order)."""
+    if case == "grid 11x9x7":
+        ro, cols, vals = aa.gen_bsr2(11, 9, 7, params=[2.0, 0.3], seed=9)
+    elif case == "ragged 700":
+        ro, cols, vals = ragged_bsr2()
+    else:
+        ro, cols, vals = aa.gen_bsr2(1, 1, 1, params=[2.0, 0.3], seed=1)
+    nb = len(ro) - 1
+    say("case rbbsr2", case, nb)
+    x = np.random.default_rng(6).standard_normal(2 * nb)
+    ref = orc.spmv_bsr(ro, cols, vals, x)
+    bo = aa.rbcsr_bundle_offsets(ro)
+    slots = int(bo[-1]) * 64
+    dro, dc, dv, dbo = dev(ro), dev(cols), dev(vals), dev(bo)
+    bc = empty(slots, torch.int32)
+    bv = empty(4 * slots)
+    call("alien_hip_rbbsr2_fill_f64", ctx.h, nb, p(dro), p(dc), p(dv), p(dbo), p(bc), p(bv))
+    y = empty(2 * nb)
+    call("alien_hip_spmv_rbbsr2_f64", ctx.h, nb, p(dbo), p(bc), p(bv), p(dev(x)), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:2 * nb], ref)
+    bv32 = empty(4 * slots, torch.float32)
+    call("alien_hip_rbbsr2_fill_f32", ctx.h, nb, p(dro), p(dc), p(dv), p(dbo), p(bc), p(bv32))
+    x32 = dev(x.astype(np.float32))
+    y32 = empty(2 * nb, torch.float32)
+    call("alien_hip_spmv_rbbsr2_f32", ctx.h, nb, p(dbo), p(bc), p(bv32), p(x32), p(y32))
+    yb = empty(2 * nb, torch.float32)
+    call("alien_hip_spmv_bsr2_f32", ctx.h, nb, p(dro), p(dc), p(dev(vals.astype(np.float32))), p(x32), p(yb))
+    np.testing.assert_array_equal(host(y32, ctx)[:2 * nb], host(yb, ctx)[:2 * nb])
+
+
+@pytest.mark.parametrize("case", ["sizes 1..4", "sizes 1..7", "all 3"])
+def test_spmv_vblock_bit_exact(ctx, case):
+    """Variable-block product (_seqMultVariableBlock order) on the device, bit-exact
+    with the oracle; the same matrix converted to scalar CSR on the host and run
+    through the RB-CSR kernel gives the same vector bit for bit."""
+    from test_oracle import random_vblock
+    if case == "sizes 1..4":
+        ro, cols, bcols, bsize, boff, vals = random_vblock(nb=3000, max_bs=4, seed=11)
+    elif case == "sizes 1..7":
+        ro, cols, bcols, bsize, boff, vals = random_vblock(nb=500, max_bs=7, seed=12)
+    else:
+        ro, cols, bcols, bsize, boff, vals = random_vblock(nb=1000, sizes=3, seed=13)
+    nb = len(ro) - 1
+    n = int(boff[-1] + bsize[-1])
+    say("case vblock", case, nb, n)
+    x = np.random.default_rng(3).standard_normal(n)
+    ref = orc.spmv_vblock(ro, cols, bcols, bsize, boff, vals, x)
+    dx = dev(x)
+    y = empty(n)
+    call("alien_hip_spmv_vblock_f64", ctx.h, nb, p(dev(ro)), p(dev(cols)), p(dev(bcols)), p(dev(bsize)),
+         p(dev(boff)), p(dev(vals)), p(dx), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:n], ref)
+    sro, scols, svals = aa.vblock_to_csr(ro, cols, bcols, bsize, boff, vals)
+    bo = aa.rbcsr_bundle_offsets(sro)
+    slots = int(bo[-1]) * 64
+    dbo = dev(bo)
+    bc = empty(slots, torch.int32)
+    bv = empty(slots)
+    call("alien_hip_rbcsr_fill_f64", ctx.h, n, p(dev(sro)), p(dev(scols)), p(dev(svals)), p(dbo), p(bc), p(bv))
+    y2 = empty(n)
+    call("alien_hip_spmv_rbcsr_f64", ctx.h, n, p(dbo), p(bc), p(bv), p(dx), p(y2))
+    np.testing.assert_array_equal(host(y2, ctx)[:n], ref)
+
+
 @pytest.mark.parametrize("n", [0, 1, 63, 1000, 1 << 20])
 def test_blas1_bit_exact_and_dot_tolerance(ctx, n):
     say("case blas1", n)

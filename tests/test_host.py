@@ -253,3 +253,25 @@ def test_rbdia_width_limit_and_stencil_width():
     c = cols.copy()
     c[ro[70] + 1] = c[ro[70]]
     assert aa.rbdia_build(ro, c, max_width=1 << 20) is None
+
+
+def test_vblock_to_csr_layout_and_order():
+    """Host variable-block -> scalar CSR: the dense matrix assembled block by block,
+    ascending scalar columns, and the scalar CSR product equal to the oracle's
+    variable-block product bit for bit (same summation order); a block offset that
+    is not the running sum of the sizes is refused."""
+    import oracle as orc
+    from test_oracle import dense_vblock, random_vblock
+    ro, cols, bcols, bsize, boff, vals = random_vblock(nb=60, seed=8)
+    sro, scols, svals = aa.vblock_to_csr(ro, cols, bcols, bsize, boff, vals)
+    A = dense_vblock(ro, cols, bcols, bsize, boff, vals)
+    S = sp.csr_matrix((svals, scols, sro), shape=A.shape)
+    np.testing.assert_array_equal(S.toarray(), A)
+    assert all(np.all(np.diff(scols[sro[r]:sro[r + 1]]) > 0) for r in range(len(sro) - 1))
+    x = np.random.default_rng(4).standard_normal(A.shape[0])
+    np.testing.assert_array_equal(orc.spmv(sro, scols, svals, x),
+                                  orc.spmv_vblock(ro, cols, bcols, bsize, boff, vals, x))
+    bad = boff.copy()
+    bad[3] += 1
+    with pytest.raises(aa.AlienHipError):
+        aa.vblock_to_csr(ro, cols, bcols, bsize, bad, vals)

@@ -374,3 +374,53 @@ def test_neumann_polynomial_closed_form():
     _, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_NEUMANN, tol=1e-10, max_iter=500,
                      poly_factor=f, poly_order=order)
     assert r["status"] == 1
+
+
+def random_vblock(nb=40, max_bs=4, seed=3, sizes=None):
+    """Random variable-block matrix (ascending block columns, diagonal block in every
+    row) with start offsets for the blocks' values (CSRStructInfo::getBlockCols)."""
+    rng = np.random.default_rng(seed)
+    bsize = (rng.integers(1, max_bs + 1, nb) if sizes is None else np.full(nb, sizes)).astype(np.int32)
+    boff = np.concatenate([[0], np.cumsum(bsize)[:-1]]).astype(np.int32)
+    rows = [np.union1d([i], rng.choice(nb, rng.integers(0, 6), replace=False)) for i in range(nb)]
+    ro = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    cols = np.concatenate(rows).astype(np.int32)
+    sizes_k = np.array([bsize[i] * bsize[c] for i, r in enumerate(rows) for c in r])
+    bcols = np.concatenate([[0], np.cumsum(sizes_k)[:-1]]).astype(np.int32)
+    vals = rng.standard_normal(int(sizes_k.sum()))
+    return ro, cols, bcols, bsize, boff, vals
+
+
+def dense_vblock(ro, cols, bcols, bsize, boff, vals):
+    """The scalar matrix, assembled block by block with numpy (row-major blocks)."""
+    n = int(boff[-1] + bsize[-1])
+    A = np.zeros((n, n))
+    for i in range(len(ro) - 1):
+        for j in range(ro[i], ro[i + 1]):
+            c = cols[j]
+            blk = vals[bcols[j]:bcols[j] + bsize[i] * bsize[c]].reshape(bsize[i], bsize[c])
+            A[boff[i]:boff[i] + bsize[i], boff[c]:boff[c] + bsize[c]] = blk
+    return A
+
+
+def test_vblock_spmv_pins():
+    """_seqMultVariableBlock restatement: against the block-assembled dense product
+    (random sizes 1..4, non-symmetric blocks, so a transposed block fails), against
+    the scalar CSR product bit for bit when every block is 1x1, and against the
+    fixed-block product (_seqMultBlock, column-major) bit for bit when every block
+    is 2x2 and stored transposed."""
+    ro, cols, bcols, bsize, boff, vals = random_vblock()
+    A = dense_vblock(ro, cols, bcols, bsize, boff, vals)
+    x = np.random.default_rng(9).standard_normal(A.shape[0])
+    y = orc.spmv_vblock(ro, cols, bcols, bsize, boff, vals, x)
+    np.testing.assert_allclose(y, A @ x, rtol=1e-13, atol=1e-13 * np.abs(A).sum(1).max())
+    # every block 1x1: the CSR product
+    ro1, cols1, bcols1, bs1, bo1, v1 = random_vblock(sizes=1, seed=4)
+    x1 = np.random.default_rng(1).standard_normal(len(bs1))
+    np.testing.assert_array_equal(orc.spmv_vblock(ro1, cols1, bcols1, bs1, bo1, v1, x1), orc.spmv(ro1, cols1, v1, x1))
+    # every block 2x2: the fixed-block product with column-major blocks
+    ro2, cols2, bcols2, bs2, bo2, v2 = random_vblock(sizes=2, seed=5)
+    x2 = np.random.default_rng(2).standard_normal(2 * len(bs2))
+    colmajor = v2.reshape(-1, 2, 2).transpose(0, 2, 1).reshape(-1)
+    np.testing.assert_array_equal(orc.spmv_vblock(ro2, cols2, bcols2, bs2, bo2, v2, x2),
+                                  orc.spmv_bsr(ro2, cols2, colmajor, x2))
