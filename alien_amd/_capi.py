@@ -36,7 +36,7 @@ class SolveParams(C.Structure):
                 ("poly_order", C.c_int), ("poly_max_iter", C.c_int), ("ilu_schedule", C.c_int),
                 ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("max_outer", C.c_int),
                 ("poll_every", C.c_int), ("gnn_theta", c_void_p), ("stall_window", C.c_int),
-                ("filu_factor_niter", C.c_int), ("filu_solver_niter", C.c_int)]
+                ("filu_factor_niter", C.c_int), ("filu_solver_niter", C.c_int), ("factor_storage", C.c_int)]
 
 
 class SolveResult(C.Structure):
@@ -362,7 +362,7 @@ class Context:
 def solve_csr(ctx: Context, ro, cols, vals, b, x0=None, *, solver=CG, precond=PREC_CHEB, layout=LAYOUT_CSR,
               mode=MODE_REFERENCE, tol=1e-8, max_iter=1000, poly_factor=0.5, poly_order=3, poly_max_iter=10,
               ilu_schedule=0, inner_tol=0.0, inner_max_iter=0, max_outer=50, poll_every=8, gnn_theta=None,
-              stall_window=0, filu_factor_niter=0, filu_solver_niter=3):
+              stall_window=0, filu_factor_niter=0, filu_solver_niter=3, factor_storage=0):
     """Run one Krylov solve through the hip backend; returns (x, result dict)."""
     n = len(ro) - 1
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
@@ -370,7 +370,8 @@ def solve_csr(ctx: Context, ro, cols, vals, b, x0=None, *, solver=CG, precond=PR
     theta = None if gnn_theta is None else np.ascontiguousarray(gnn_theta, dtype=np.float32)
     p = SolveParams(solver, precond, layout, mode, tol, max_iter, poly_factor, poly_order, poly_max_iter,
                     ilu_schedule, inner_tol, inner_max_iter, max_outer, poll_every,
-                    None if theta is None else theta.ctypes.data, stall_window, filu_factor_niter, filu_solver_niter)
+                    None if theta is None else theta.ctypes.data, stall_window, filu_factor_niter, filu_solver_niter,
+                    factor_storage)
     r = SolveResult()
     check(lib().alien_hip_solve_csr(ctx.h, C.byref(p), n, iptr(ro), iptr(cols), dptr(vals), dptr(b), dptr(x),
                                     C.byref(r)))

@@ -42,7 +42,10 @@ inline long round_xcds(long g) {
 // workgroups of a row kernel over n rows with elements of `elem` bytes
 // Uncapped grids are rounded to whole chunks of kXcdChunk workgroups per XCD
 // (bundle_walk's chunked XCD mapping needs a bijection).
-constexpr long kXcdChunk = 64;
+#ifndef ALIEN_HIP_XCD_CHUNK
+#define ALIEN_HIP_XCD_CHUNK 64
+#endif
+constexpr long kXcdChunk = ALIEN_HIP_XCD_CHUNK;
 inline long round_chunks(long g) {
   const long q = kXcds * kXcdChunk;
   return (g + q - 1) / q * q;

@@ -85,8 +85,8 @@ class Engine {
   size_t g_slots() const { return m_g.slots; }
   size_t gt_slots() const { return m_gt.slots; }
   bool factor_is_dia() const { return m_g.dia && m_gt.dia; }
-  template <typename T> RBMat<T> g_mat(const T* v) const { return m_g.mat(v); }
-  template <typename T> RBMat<T> gt_mat(const T* v) const { return m_gt.mat(v); }
+  template <typename T, typename V = T> RBMat<T, V> g_mat(const V* v) const { return m_g.mat<T, V>(v); }
+  template <typename T, typename V = T> RBMat<T, V> gt_mat(const V* v) const { return m_gt.mat<T, V>(v); }
 
  private:
   double cg_term(const float* B, int k, float* gg);
@@ -106,9 +106,9 @@ class Engine {
     size_t slots = 0;
     DeviceArray<int> bo, bc, doff;
     DeviceArray<unsigned long long> dmask;
-    template <typename T> RBMat<T> mat(const T* v) const {
-      return RBMat<T>{bo.data(), dia ? nullptr : bc.data(), v, dia ? doff.data() : nullptr,
-                      dia ? dmask.data() : nullptr};
+    template <typename T, typename V = T> RBMat<T, V> mat(const V* v) const {
+      return RBMat<T, V>{bo.data(), dia ? nullptr : bc.data(), v, dia ? doff.data() : nullptr,
+                         dia ? dmask.data() : nullptr};
     }
   };
   Factor m_g, m_gt;

@@ -527,6 +527,17 @@ __global__ void k_tpos(int n, const int* __restrict__ ro, const int* __restrict_
 // G_ji = dis_j Ghat_ji, read at the transpose position). Lane = row; the row's
 // entries are walked together with the bundle's slots: slot j of an RB-CSR
 // bundle is the row's j-th entry, an RB-DIA slot the entry at offset doff.
+// a factor value in its storage type: FP64, FP32, or bfloat16 (FP32 rounded to
+// nearest even, then its upper half rounded to nearest even)
+template <typename T>
+__device__ __forceinline__ T factor_store(double g) {
+  return (T)g;
+}
+template <>
+__device__ __forceinline__ bf16_bits factor_store<bf16_bits>(double g) {
+  return (bf16_bits)to_bf16((float)g);
+}
+
 template <typename T, bool TRANS>
 __global__ void k_fill_factor(int n, const int* __restrict__ ro, const int* __restrict__ cols,
                               const int* __restrict__ dcol, const int* __restrict__ tpos, const int* __restrict__ bo,
@@ -547,7 +558,7 @@ __global__ void k_fill_factor(int n, const int* __restrict__ ro, const int* __re
     T v = T(0);
     if (k < e && (!doff || cols[k] - row == doff[s0 + j])) {
       const double g = TRANS ? (double)dis[cols[k]] * (double)ghat[tpos[k]] : (double)dis[row] * (double)ghat[k];
-      v = (T)g;
+      v = factor_store<T>(g);
       ++k;
     }
     bv[((long)s0 + j) * kBundle + lane] = v;
@@ -1021,6 +1032,7 @@ void Engine::release_network_buffers() {
 }
 
 template void Engine::fill_factor<float>(float*, float*);
+template void Engine::fill_factor<bf16_bits>(bf16_bits*, bf16_bits*);
 template void Engine::fill_factor<double>(double*, double*);
 
 void launch_adam(hipStream_t s, int np, float* P, const float* g, float* m, float* v, float lr, int step) {

@@ -209,8 +209,8 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_guarded(int n, RBMat<T> A
 // (= r.M r, the r.z of CG.h:124 with z = G t, without forming z), plus r.r:
 // the stop test of the previous update is taken here (its finalize runs
 // kFinXRDiag), so the x/r update itself needs no reduction.
-template <typename T>
-__global__ void __launch_bounds__(kRowBlock) k_factor_t(int n, RBMat<T> GT, const T* __restrict__ r,
+template <typename T, typename V>
+__global__ void __launch_bounds__(kRowBlock) k_factor_t(int n, RBMat<T, V> GT, const T* __restrict__ r,
                                                          T* __restrict__ t, CGState* st, RedWork w) {
   if (st->done) return;
   __shared__ double lds4[kRowBlock / kWave];
@@ -262,8 +262,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int n, const T* __restrict
 
 // Second half fused with the direction update: z_i = (G t)_i is formed in
 // registers and p = z + (rho1/rho) p (p = z on the first step), as k_cg_p.
-template <typename T>
-__global__ void __launch_bounds__(kRowBlock) k_factor_p(int n, RBMat<T> G, const T* __restrict__ t,
+template <typename T, typename V>
+__global__ void __launch_bounds__(kRowBlock) k_factor_p(int n, RBMat<T, V> G, const T* __restrict__ t,
                                                          T* __restrict__ p, const CGState* st, int first) {
   if (st->done) return;
   const T beta = first ? T(0) : (T)(st->rho1 / st->rho);
@@ -404,14 +404,15 @@ void launch_factor_apply(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> G
   launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
 }
 
-template <typename T>
-void launch_factor_p(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* t, T* p,
+template <typename T, typename V>
+void launch_factor_p(hipStream_t s, RedWork w, int n, RBMat<T, V> G, RBMat<T, V> GT, const T* r, T* t, T* p,
                      CGState* st, int first, int rho1_only) {
-  hipLaunchKernelGGL(k_factor_t<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, GT, r, t, st, w);
+  hipLaunchKernelGGL((k_factor_t<T, V>), bgrid<T>(n), dim3(kRowBlock), 0, s, n, GT, r, t, st, w);
   // first step (or a resumed one): rho1 only; later: rho = rho1, ++iter and the stop test of the
   // previous update, rho1 = t.t
   launch_finalize(s, w, rblocks<T>(n), 2, (first || rho1_only) ? kFinRho1B : kFinXRDiag, nullptr, st);
-  hipLaunchKernelGGL(k_factor_p<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, G, t, p, (const CGState*)st, first);
+  hipLaunchKernelGGL((k_factor_p<T, V>), bgrid<T>(n), dim3(kRowBlock), 0, s, n, G, t, p, (const CGState*)st,
+                     first);
 }
 template <typename T>
 void launch_cg_update(hipStream_t s, int n, const T* p, const T* q, T* x, T* r, CGState* st) {
@@ -433,13 +434,17 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
   template void launch_cg_p<T>(hipStream_t, int, const T*, T*, CGState*, int);                       \
   template void launch_factor_apply<T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,  \
                                        CGState*);                                                        \
-  template void launch_factor_p<T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,     \
-                                   CGState*, int, int);                                                  \
+  template void launch_factor_p<T, T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,  \
+                                      CGState*, int, int);                                               \
   template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*);          \
   template void launch_cg_update<T>(hipStream_t, int, const T*, const T*, T*, T*, CGState*);            \
   template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, T*, T*, const T*, T*,     \
                                 CGState*);
 ALIEN_HIP_INST_KRYLOV(double)
 ALIEN_HIP_INST_KRYLOV(float)
+// FP32 arithmetic on bfloat16 factor values (learned M in the mixed solver)
+template void launch_factor_p<float, bf16_bits>(hipStream_t, RedWork, int, RBMat<float, bf16_bits>,
+                                                RBMat<float, bf16_bits>, const float*, float*, float*, CGState*,
+                                                int, int);
 
 } // namespace alien_hip

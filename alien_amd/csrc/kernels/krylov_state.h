@@ -8,12 +8,17 @@ namespace alien_hip {
 
 struct RedWork;
 
-// Row-bundled operand (rb_rows.h): RB-CSR when doff is NULL, RB-DIA otherwise
-template <typename T>
+// bfloat16 bits (the upper half of an FP32 value): value storage of the learned
+// factor in the FP32 solver
+typedef unsigned short bf16_bits;
+
+// Row-bundled operand (rb_rows.h): RB-CSR when doff is NULL, RB-DIA otherwise.
+// T is the arithmetic type, V the stored value type (T, or bf16_bits for T = float).
+template <typename T, typename V = T>
 struct RBMat {
   const int* bo; // bundle offsets in 64-entry slots, nbundles + 1
   const int* bc; // RB-CSR: columns, -1 = padding
-  const T* bv;   // values
+  const V* bv;   // values
   const int* doff = nullptr;                  // RB-DIA: column offset of each bundle slot
   const unsigned long long* dmask = nullptr;  // RB-DIA: rows of the bundle holding that slot
 };
@@ -96,8 +101,8 @@ template <typename T>
 void launch_cg_update(hipStream_t, int n, const T* p, const T* q, T* x, T* r, CGState* st);
 // M = G G^T fused with the direction update: t = G^T r, rho1 = t.t, p = G t + (rho1/rho) p;
 // after the first step its finalize also completes the previous update (++iter, stop test on r.r)
-template <typename T>
-void launch_factor_p(hipStream_t, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* t, T* p,
+template <typename T, typename V>
+void launch_factor_p(hipStream_t, RedWork w, int n, RBMat<T, V> G, RBMat<T, V> GT, const T* r, T* t, T* p,
                      CGState* st, int first, int rho1_only = 0);
 template <typename T>
 void launch_factor_apply(hipStream_t, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* tmp, T* z,

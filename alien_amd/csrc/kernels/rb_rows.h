@@ -21,11 +21,21 @@
 
 namespace alien_hip {
 
+// stored value -> arithmetic type (bf16 bits are the upper half of an FP32 value)
+template <typename T>
+__device__ __forceinline__ T widen(T v) {
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T widen(bf16_bits v) {
+  return __uint_as_float((unsigned)v << 16);
+}
+
 // U slots per batch. Slot indices past the bundle width are clamped to its
 // last slot, so every load is in bounds and unconditional (no scalar waits
 // between them); the clamped duplicates are masked out of the sum.
-template <int U, typename T, typename XS>
-__device__ __forceinline__ T rb_sum_u(const RBMat<T>& A, int b, int s0, int w, XS xs) {
+template <int U, typename T, typename V, typename XS>
+__device__ __forceinline__ T rb_sum_u(const RBMat<T, V>& A, int b, int s0, int w, XS xs) {
   const int lane = threadIdx.x & (kWave - 1);
   const long base = (long)s0 * kWave + lane;
   T acc = 0;
@@ -41,7 +51,7 @@ __device__ __forceinline__ T rb_sum_u(const RBMat<T>& A, int b, int s0, int w, X
         const int sl = min(j + u, w - 1);
         d[u] = A.doff[s0 + sl];
         m[u] = A.dmask[s0 + sl];
-        v[u] = __builtin_nontemporal_load(A.bv + base + (long)sl * kWave);
+        v[u] = widen<T>(__builtin_nontemporal_load(A.bv + base + (long)sl * kWave));
       }
       bool on[U];
       T xv[U];
@@ -62,7 +72,7 @@ __device__ __forceinline__ T rb_sum_u(const RBMat<T>& A, int b, int s0, int w, X
       for (int u = 0; u < U; ++u) {
         const int sl = min(j + u, w - 1);
         c[u] = __builtin_nontemporal_load(A.bc + base + (long)sl * kWave);
-        v[u] = __builtin_nontemporal_load(A.bv + base + (long)sl * kWave);
+        v[u] = widen<T>(__builtin_nontemporal_load(A.bv + base + (long)sl * kWave));
       }
       bool on[U];
       T xv[U];
@@ -82,8 +92,8 @@ __device__ __forceinline__ T rb_sum_u(const RBMat<T>& A, int b, int s0, int w, X
 // Row sum of this lane's row of bundle b over the vector given by xs(column).
 // Short bundles (factors of M = G G^T: 4 slots at a 7-point stencil) use a
 // batch of 4, longer ones a batch of 7.
-template <typename T, typename XS>
-__device__ __forceinline__ T rb_sum(const RBMat<T>& A, int b, XS xs) {
+template <typename T, typename V, typename XS>
+__device__ __forceinline__ T rb_sum(const RBMat<T, V>& A, int b, XS xs) {
   const int s0 = A.bo[b];
   const int w = A.bo[b + 1] - s0;
   if (w <= 0) return T(0);

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "alien/hip/HIPBackEnd.h"
@@ -147,9 +148,25 @@ struct PrecondSpec {
   // precision of the solver using them (one pair set)
   RBMat<double> g64{}, gt64{};
   RBMat<float> g32{}, gt32{};
+  // FP32 solver with bfloat16 factor values (used instead of g32 / gt32 when `half`)
+  RBMat<float, bf16_bits> g16{}, gt16{};
+  bool half = false;
   template <typename T> RBMat<T> g() const;
   template <typename T> RBMat<T> gt() const;
 };
+
+// t = G^T r, p = G t + beta p in the factor's storage precision
+template <typename T>
+void factor_p(hipStream_t s, RedWork w, int n, const PrecondSpec& P, const T* r, T* t, T* p, CGState* st,
+              int first, int rho1_only = 0) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (P.half) {
+      launch_factor_p<float, bf16_bits>(s, w, n, P.g16, P.gt16, r, t, p, st, first, rho1_only);
+      return;
+    }
+  }
+  launch_factor_p<T, T>(s, w, n, P.g<T>(), P.gt<T>(), r, t, p, st, first, rho1_only);
+}
 template <> RBMat<double> PrecondSpec::g<double>() const { return g64; }
 template <> RBMat<double> PrecondSpec::gt<double>() const { return gt64; }
 template <> RBMat<float> PrecondSpec::g<float>() const { return g32; }
@@ -214,8 +231,7 @@ struct FusedCG {
     if (P.kind == kPrecFactor) {
       // z = G (G^T r) never stored: rho1 = |G^T r|^2 and p = G t + beta p; the stop test of this
       // update runs in the next iteration's first finalize, so the update has no reduction
-      launch_factor_p<T>(s, ctx->red(), n, P.g<T>(), P.gt<T>(), r.data(), tmp.data(), p.data(), st.data(),
-                         first ? 1 : 0);
+      factor_p<T>(s, ctx->red(), n, P, r.data(), tmp.data(), p.data(), st.data(), first ? 1 : 0);
       launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
       launch_cg_update<T>(s, n, p.data(), q.data(), x, r.data(), st.data());
       return;
@@ -238,7 +254,7 @@ struct FusedCG {
   // replaced; finish that iteration with rho1 recomputed from the new r.
   void resume_inside_step(T* x) {
     hipStream_t s = ctx->stream;
-    launch_factor_p<T>(s, ctx->red(), n, P.g<T>(), P.gt<T>(), r.data(), tmp.data(), p.data(), st.data(), 0, 1);
+    factor_p<T>(s, ctx->red(), n, P, r.data(), tmp.data(), p.data(), st.data(), 0, 1);
     launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
     launch_cg_update<T>(s, n, p.data(), q.data(), x, r.data(), st.data());
     check_launch("learned CG resumed step");
@@ -779,6 +795,8 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       throw Error(ALIEN_HIP_EINVAL, "FILU0 and Neumann run in the REFERENCE mode");
     if (p->precond == ALIEN_HIP_PREC_FILU0 && (p->filu_factor_niter < 0 || p->filu_solver_niter < 0))
       throw Error(ALIEN_HIP_EINVAL, "FILU0: negative iteration counts");
+    if (p->factor_storage != 0 && p->factor_storage != 1)
+      throw Error(ALIEN_HIP_EINVAL, "solve_csr: factor_storage must be 0 (FP32) or 1 (bfloat16)");
     if (p->precond == ALIEN_HIP_PREC_CHEB && p->poly_order < 1)
       throw Error(ALIEN_HIP_EINVAL, "Chebyshev order must be >= 1");
 
@@ -882,6 +900,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       std::unique_ptr<gnn::Engine> net;
       DeviceArray<float> theta, gv32, gtv32;
       DeviceArray<double> gv64, gtv64;
+      DeviceArray<bf16_bits> gv16, gtv16;
       int stall_window = 0;
       if (learned) {
         gnn::SystemView v;
@@ -905,7 +924,14 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
         net->build_factor_layout();
         ps.kind = kPrecFactor;
         const size_t ng = net->g_slots(), ngt = net->gt_slots();
-        if (mixed) {
+        if (mixed && !bicg && p->factor_storage == 1) {
+          gv16.resize(ng);
+          gtv16.resize(ngt);
+          net->fill_factor<bf16_bits>(gv16.data(), gtv16.data());
+          ps.g16 = net->g_mat<float, bf16_bits>(gv16.data());
+          ps.gt16 = net->gt_mat<float, bf16_bits>(gtv16.data());
+          ps.half = true;
+        } else if (mixed) {
           gv32.resize(ng);
           gtv32.resize(ngt);
           net->fill_factor<float>(gv32.data(), gtv32.data());

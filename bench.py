@@ -120,6 +120,30 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     return res
 
 
+def copy_ceiling(aa, torch, ctx, reps):
+    """HBM bandwidth (GB/s) the library's own copy kernel reaches on 1 GiB vectors (2^27 doubles,
+    read + write, far beyond the 256 MB Infinity Cache): the practical ceiling the SpMV's
+    roofline fraction is measured against, beside the nominal 8 TB/s."""
+    import ctypes as C
+    n = 1 << 27
+    P = lambda t: C.c_void_p(t.data_ptr())
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        aa.check(aa.lib().alien_hip_copy_f64(ctx.h, n, P(x), P(y)))
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        aa.check(aa.lib().alien_hip_copy_f64(ctx.h, n, P(x), P(y)))
+    e.record()
+    e.synchronize()
+    t = s.elapsed_time(e) / reps * 1e-3
+    del x, y
+    torch.cuda.empty_cache()
+    return 16 * n / t / 1e9
+
+
 def cpu_baseline(aa, ro, cols, vals, b, fp64_iters, budget_s):
     """Oracle (C++ restatement of SimpleCSR) Diag-CG iterations on host threads; time-to-1e-8 is the
     per-iteration time times the iteration count the FP64 GPU solve needed (same algorithm)."""
@@ -584,6 +608,7 @@ def main():
                       "gflops": 2 * nnz / sp[name] / 1e9}
     spmv["rbdia_fp64"]["csr_equivalent_gbs"] = csr64 / sp["rbdia_fp64"] / 1e9
 
+    ceiling = copy_ceiling(aa, torch, ctx, a.spmv_reps)
     if "learned_mixed" not in results:  # partial run (profiling): no JSON line
         ctx.close()
         return
@@ -617,7 +642,8 @@ def main():
         "spmv": spmv,
         "roofline": {"bound": "hbm", "achieved": spmv[ROOF]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv[ROOF]["gbs"] / HBM_PEAK_GBS, "traffic": TRAFFIC.get(ROOF) if n == 256 else None,
-                     "kernel": ROOF_KERNEL, "bytes_per_launch": spmv[ROOF]["bytes"]},
+                     "kernel": ROOF_KERNEL, "bytes_per_launch": spmv[ROOF]["bytes"],
+                     "copy_ceiling_gbs": ceiling},
         "cpu_baseline": cpu,
     }
     ctx.close()

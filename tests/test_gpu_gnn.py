@@ -203,6 +203,25 @@ def test_learned_pcg_beats_jacobi(ctx, mode):
         assert abs(r["iterations"] - it_o) <= 2
 
 
+def test_learned_mixed_bf16_factor_storage(ctx):
+    """MIXED CG with the factor values stored as bfloat16 (FP32 arithmetic): M is
+    still G G^T of the rounded G, SPD; the solve converges on the FP64 true
+    residual within 2% of the iterations of FP32 storage."""
+    n = 20
+    ro, cols, vals = system(n, sigma=1.5, seed=3)
+    b = aa.uniform01(42, n ** 3)
+    kw = dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, tol=1e-8, max_iter=2000,
+              gnn_theta=go.init_params(0), poll_every=4)
+    say("learned mixed bf16 factor")
+    _, r16 = aa.solve_csr(ctx, ro, cols, vals, b, factor_storage=1, **kw)
+    _, r32 = aa.solve_csr(ctx, ro, cols, vals, b, factor_storage=0, **kw)
+    say("  bf16", r16["iterations"], "fp32", r32["iterations"])
+    assert r16["status"] == 1 and r16["fallback"] == 0 and r16["true_rel_res"] < 1.05e-8
+    assert abs(r16["iterations"] - r32["iterations"]) <= max(2, 0.02 * r32["iterations"])
+    with pytest.raises(aa.AlienHipError):
+        aa.solve_csr(ctx, ro, cols, vals, b, factor_storage=2, **kw)
+
+
 def broken_theta(which):
     p = go.unpack(go.init_params(0))
     if which == "overflow":
