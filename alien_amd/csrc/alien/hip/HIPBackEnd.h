@@ -9,8 +9,9 @@
 //   algebra            SYCLInternalLinearAlgebra (sycl/algebra/SYCLInternalLinearAlgebra.h:46-141)
 //
 // Outside a full Alien build there is no MultiMatrixImpl to own the matrix, so
-// HIPMatrix is built directly from a host CSR view (what the SimpleCSR -> hip
-// converter does inside Alien; the converter code is in INTEGRATION.md).
+// HIPMatrix is built directly from a host CSR view (what a SimpleCSR -> hip
+// matrix converter registered in Alien would do with the source's profile and
+// values).
 #pragma once
 
 #include <hip/hip_runtime.h>

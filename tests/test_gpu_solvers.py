@@ -249,8 +249,8 @@ def test_mixed_bicgstab_reaches_fp64_tolerance(ctx, case):
 def test_mixed_bicgstab_hands_over_to_fp64_when_fp32_stagnates(ctx):
     """2-unknown block system with sigma 3 on 20^3 cells, to 1e-12: stored in
     FP32 the operator is too ill-conditioned for the FP32 inner iteration to keep
-    reducing the true residual (a CPU emulation of the driver overflows after 4
-    reliable updates). The driver detects the stagnation or the overflow, keeps
+    reducing the true residual (a CPU emulation of the mixed solver overflows
+    after 4 reliable updates). The solver detects the stagnation or the overflow, keeps
     the last reliable x and finishes in FP64 (fallback bit 2); the true residual
     meets the tolerance."""
     ro, cols, vals = bsr2_scalar_csr(20, sigma=3.0)

@@ -1,7 +1,7 @@
 """Pins the CPU oracle (oracle/alien_oracle.cpp) against what it must reproduce:
 
 * golden runs of the reference's own Krylov templates on the 32^3 Poisson
-  system (recorded in the build's survey: Chebyshev-CG 38 iterations,
+  system (recorded values from compiling the reference's templates: Chebyshev-CG 38 iterations,
   relative residual 7.03e-9, ALPHA 3.6392 / BETA 1.8196; Diag-CG 108
   iterations; Chebyshev factor 30 not converged after 1000 iterations,
   relative residual 3.9e-2);
