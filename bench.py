@@ -469,6 +469,64 @@ def run_eval(a):
     ctx.close()
 
 
+def run_train(a):
+    """Training configuration (not the default line): `--systems` generated log-normal systems of
+    edge n in [32, 64], sigma ~ U[0.5, 3], per-axis anisotropy log-uniform in [1, 100], one Adam step
+    each (loss: residual reduction after --train-cg-steps PCG steps + probe term), then the initial
+    and the trained parameters on `--seeds` unseen n^3 systems (sigma 2), CG to FP64 1e-8 in the
+    mixed solver, beside Jacobi (mixed, FP64), multicolour ILU0 and Chebyshev (FP64)."""
+    import torch
+
+    import alien_amd as aa
+    n = a.n
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx = aa.Context(stream.cuda_stream)
+    th0 = aa.gnn_init(0)
+    log(f"training on {a.systems} systems (32^3..64^3)")
+    t0 = time.time()
+    th1, hist = aa.gnn_train(ctx, th0, steps=a.systems, steps_per_system=1, n_min=32, n_max=64, sigma_min=0.5,
+                             sigma_max=3.0, aniso_max=100.0, lr=1e-3, beta=0.1, seed=7,
+                             cg_steps=a.train_cg_steps)
+    t_train = time.time() - t0
+    log(f"trained in {t_train:.1f}s, loss {hist[:50].mean():.4f} -> {hist[-50:].mean():.4f}")
+    runs = [("learned_mixed_init", dict(mode=aa.MODE_MIXED, precond=aa.PREC_LEARNED, gnn_theta=th0)),
+            ("learned_mixed_trained", dict(mode=aa.MODE_MIXED, precond=aa.PREC_LEARNED, gnn_theta=th1)),
+            ("mixed_jacobi", dict(mode=aa.MODE_MIXED, precond=aa.PREC_DIAG)),
+            ("fp64_jacobi", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
+            ("fp64_ilu0_multicolour", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_ILU0, ilu_schedule=1,
+                                           layout=aa.LAYOUT_RBCSR)),
+            ("fp64_chebyshev", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_CHEB, max_iter=a.cheb_max_iter))]
+    per = {name: {"iterations": [], "time_s": [], "converged": []} for name, _ in runs}
+    for seed in range(1, a.seeds + 1):
+        ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=[2.0, 1.0, 1.0, 1.0], seed=seed)
+        xt = aa.uniform01(seed, len(ro) - 1)
+        b = gpu_rhs(aa, torch, ctx, ro, cols, vals, xt)
+        line = []
+        for name, kw in runs:
+            args = dict(solver=aa.CG, tol=1e-8, max_iter=20000, poll_every=16)
+            args.update(kw)
+            _, r = aa.solve_csr(ctx, ro, cols, vals, b, **args)
+            ok = bool(r["status"] == 1 and r["true_rel_res"] < 1.05e-8 and not r["fallback"])
+            per[name]["iterations"].append(r["iterations"])
+            per[name]["time_s"].append(r["solve_s"])
+            per[name]["converged"].append(ok)
+            line.append(f"{name} {r['iterations']}{'' if ok else '*'}")
+        log(f"seed {seed}: " + ", ".join(line))
+    summary = {name: {"mean_iterations": float(np.mean(d["iterations"])), "mean_time_s": float(np.mean(d["time_s"])),
+                      "all_converged": bool(all(d["converged"]))} for name, d in per.items()}
+    print(json.dumps({"metric": "CG iterations / time to 1e-8 on unseen systems, initial vs trained network",
+                      "value": summary["learned_mixed_trained"]["mean_time_s"], "unit": "s", "n_gpus": 1,
+                      "higher_is_better": False, "dtype": "f32+f64", "data": "synthetic",
+                      "config": {"workload": f"training on {a.systems} log-normal systems 32^3..64^3 (sigma U[0.5,3], "
+                                             f"anisotropy log-U[1,100]); evaluation on {a.seeds} unseen {n}^3 systems",
+                                 "N": n ** 3},
+                      "training": {"systems": a.systems, "time_s": t_train, "loss_first50": float(hist[:50].mean()),
+                                   "loss_last50": float(hist[-50:].mean()), "cg_steps": a.train_cg_steps},
+                      "summary": summary, "per_seed": per}), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -476,7 +534,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None,
                     help="grid edge (default 256 for lognormal, 128 for convdiff and eval, 192 for blocks2)")
-    ap.add_argument("--seeds", type=int, default=20, help="eval workload: number of unseen systems")
+    ap.add_argument("--seeds", type=int, default=20, help="eval / train workloads: number of unseen systems")
+    ap.add_argument("--systems", type=int, default=2000, help="train workload: number of training systems")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--train-cg-steps", type=int, default=16,
                     help="training loss: residual reduction after this many PCG steps (0: Kaporin's bound)")
@@ -492,19 +551,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernels and a reference copy kernel (for rocprofv3 --pmc runs)")
-    ap.add_argument("--workload", default="lognormal", choices=["lognormal", "convdiff", "blocks2", "eval"],
+    ap.add_argument("--workload", default="lognormal", choices=["lognormal", "convdiff", "blocks2", "eval", "train"],
                     help="lognormal: the default line; convdiff: the nonsymmetric BiCGStab configuration; "
-                         "blocks2: two unknowns per cell, BiCGStab; eval: the learned M on unseen seeds")
+                         "blocks2: two unknowns per cell, BiCGStab; eval: the learned M on unseen seeds; "
+                         "train: the full training configuration, then initial vs trained on unseen seeds")
     a = ap.parse_args()
     if a.n is None:
-        a.n = {"lognormal": 256, "convdiff": 128, "blocks2": 192, "eval": 128}[a.workload]
+        a.n = {"lognormal": 256, "convdiff": 128, "blocks2": 192, "eval": 128, "train": 128}[a.workload]
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported in this build "
               f"(the row-block sharding over several GPUs is a design, not an implementation)", file=sys.stderr)
         sys.exit(2)
 
     if a.workload != "lognormal":
-        {"convdiff": run_convdiff, "blocks2": run_blocks2, "eval": run_eval}[a.workload](a)
+        {"convdiff": run_convdiff, "blocks2": run_blocks2, "eval": run_eval, "train": run_train}[a.workload](a)
         return
 
     import torch
