@@ -47,6 +47,7 @@ void bellpack_fill(int nrows, int slice, const int* ro, const int* cols, const d
                    const int* bro, int* bcols, double* bvals, uint8_t* lmask, uint8_t* umask) {
   const int ns = bellpack_nslices(nrows, slice);
   const size_t total = (size_t)bro[ns] * (size_t)slice;
+  if (total == 0) return;
   std::memset(bcols, 0, total * sizeof(int));
   if (bvals) std::fill(bvals, bvals + total, 0.0);
   if (lmask) std::memset(lmask, 0, total);
