@@ -16,7 +16,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 ORC_CG, ORC_BICGS = 0, 1
-PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_FILU0, PREC_NEUMANN = 0, 1, 2, 3, 4, 5
+PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_FILU0, PREC_NEUMANN, PREC_FACTOR = 0, 1, 2, 3, 4, 5, 6
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -26,7 +26,8 @@ _u8 = C.POINTER(C.c_uint8)
 class SolveParams(C.Structure):
     _fields_ = [("solver", C.c_int), ("precond", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int),
                 ("poly_factor", C.c_double), ("poly_order", C.c_int), ("poly_max_iter", C.c_int),
-                ("nthreads", C.c_int), ("filu_factor_niter", C.c_int), ("filu_solver_niter", C.c_int)]
+                ("nthreads", C.c_int), ("filu_factor_niter", C.c_int), ("filu_solver_niter", C.c_int),
+                ("g_ro", _ip), ("g_cols", _ip), ("g_vals", _dp), ("gt_ro", _ip), ("gt_cols", _ip), ("gt_vals", _dp)]
 
 
 class SolveResult(C.Structure):
@@ -207,11 +208,22 @@ def color(ro, cols):
 
 
 def solve(ro, cols, vals, b, x0=None, *, solver=ORC_CG, precond=PREC_CHEB, tol=1e-8, max_iter=1000,
-          poly_factor=0.5, poly_order=3, poly_max_iter=10, nthreads=1, filu_factor_niter=0, filu_solver_niter=3):
+          poly_factor=0.5, poly_order=3, poly_max_iter=10, nthreads=1, filu_factor_niter=0, filu_solver_niter=3,
+          factor=None):
+    """factor: (G, G^T) as scipy CSR matrices with sorted indices, for precond=PREC_FACTOR."""
     n = len(ro) - 1
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     p = SolveParams(solver, precond, tol, max_iter, poly_factor, poly_order, poly_max_iter, nthreads,
                     filu_factor_niter, filu_solver_niter)
+    keep = []
+    if factor is not None:
+        arrs = []
+        for M in factor:
+            a = (np.ascontiguousarray(M.indptr, np.int32), np.ascontiguousarray(M.indices, np.int32),
+                 np.ascontiguousarray(M.data, np.float64))
+            keep.append(a)
+            arrs += [_i(a[0]), _i(a[1]), _d(a[2])]
+        p.g_ro, p.g_cols, p.g_vals, p.gt_ro, p.gt_cols, p.gt_vals = arrs
     r = SolveResult()
     lib().orc_solve(C.byref(p), n, _i(ro), _i(cols), _d(vals), _d(np.ascontiguousarray(b, np.float64)), _d(x),
                     C.byref(r))

@@ -744,6 +744,22 @@ struct NeumannPrec : Precond {
   }
 };
 
+/* Factored approximate inverse M = G G^T applied as the learned preconditioner is:
+ * t = G^T x, y = G t, each a sequential CSR row sum (_seqMult order). */
+struct FactorPrec : Precond {
+  int n;
+  const int *gro, *gcols, *tro, *tcols;
+  const double *gvals, *tvals;
+  Vec t;
+  FactorPrec(int n_, const orc_solve_params* p)
+  : n(n_), gro(p->g_ro), gcols(p->g_cols), tro(p->gt_ro), tcols(p->gt_cols), gvals(p->g_vals), tvals(p->gt_vals),
+    t(n_, 0.0) {}
+  void solve(const Algebra&, const Vec& x, Vec& y) override {
+    orc_spmv_csr(n, tro, tcols, tvals, x.data(), t.data());
+    orc_spmv_csr(n, gro, gcols, gvals, t.data(), y.data());
+  }
+};
+
 int orc_solve(const orc_solve_params* p, int n, const int* ro, const int* cols, const double* vals,
               const double* bptr, double* xptr, orc_solve_result* res) {
   std::memset(res, 0, sizeof(*res));
@@ -763,6 +779,10 @@ int orc_solve(const orc_solve_params* p, int n, const int* ro, const int* cols, 
   case ORC_PREC_ILU0: P = new ILU0Prec(alg); break;
   case ORC_PREC_FILU0: P = new FILU0Prec(alg, p->filu_factor_niter, p->filu_solver_niter); break;
   case ORC_PREC_NEUMANN: P = new NeumannPrec(alg, p->poly_factor, p->poly_order); break;
+  case ORC_PREC_FACTOR:
+    if (!p->g_ro || !p->gt_ro) return -1;
+    P = new FactorPrec(n, p);
+    break;
   default: P = new NonePrec(); break;
   }
   double t1 = now_s();

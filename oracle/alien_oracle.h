@@ -84,7 +84,7 @@ int  orc_color_greedy(int nrows, const int* row_offset, const int* cols, int* co
  *               ILU0Preconditioner.h) ------------------------------------- */
 enum { ORC_CG = 0, ORC_BICGS = 1 };
 enum { ORC_PREC_DIAG = 0, ORC_PREC_CHEB = 1, ORC_PREC_ILU0 = 2, ORC_PREC_NONE = 3, ORC_PREC_FILU0 = 4,
-       ORC_PREC_NEUMANN = 5 };
+       ORC_PREC_NEUMANN = 5, ORC_PREC_FACTOR = 6 };
 
 typedef struct {
   int    solver;        /* ORC_CG / ORC_BICGS */
@@ -97,6 +97,13 @@ typedef struct {
   int    nthreads;      /* >1: row-block partition, dots summed per block then in block order */
   int    filu_factor_niter; /* FILU0: fixed-point factorisation sweeps (0 = exact ILU0) */
   int    filu_solver_niter; /* FILU0: Jacobi sweeps per triangular solve */
+  /* FACTOR: y = G (G^T x), G and G^T given as CSR (ascending columns), sequential row sums */
+  const int* g_ro;
+  const int* g_cols;
+  const double* g_vals;
+  const int* gt_ro;
+  const int* gt_cols;
+  const double* gt_vals;
 } orc_solve_params;
 
 typedef struct {

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import alien_amd as aa
+import oracle as orc
 from oracle import gnn_oracle as go
 
 pytestmark = pytest.mark.gpu
@@ -155,6 +156,39 @@ def test_training_lowers_the_objective(ctx):
     assert L1 < L0
 
 
+def oracle_factor(ro, cols, vals, th):
+    """(G, G^T) of the oracle's network as CSR with sorted indices."""
+    G = go.preconditioner(go.Graph(ro, cols, vals), th, bf=True).tocsr()
+    G.sort_indices()
+    GT = G.T.tocsr()
+    GT.sort_indices()
+    return G, GT
+
+
+@pytest.mark.parametrize("seed", [4, 9])
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH])
+def test_learned_bicgstab_matches_oracle(ctx, seed, mode):
+    """FP64 BiCGStab with the learned M (two factor SpMVs per preconditioner apply)
+    against the oracle running BiCGStab.h's call sequence with the same G:
+    iterations within 2 (48 at seed 4 where Jacobi needs 91)."""
+    import oracle as orc_
+    case = f"lognormal 16^3 seed {seed}"
+    ro, cols, vals = system(16, sigma=1.5, seed=seed)
+    n = len(ro) - 1
+    b = orc_.spmv(ro, cols, vals, aa.uniform01(42, n))
+    th = go.init_params(0)
+    say("learned BiCGStab vs oracle", case, mode)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_LEARNED, mode=mode, tol=1e-8,
+                        max_iter=2000, gnn_theta=th, poll_every=3)
+    G, GT = oracle_factor(ro, cols, vals, th)
+    _, o = orc_.solve(ro, cols, vals, b, solver=orc_.ORC_BICGS, precond=orc_.PREC_FACTOR, factor=(G, GT),
+                      max_iter=2000)
+    say("  gpu", r["iterations"], r["status"], r["fallback"], "oracle", o["iterations"], o["status"])
+    assert r["status"] == o["status"] == 1 and r["fallback"] == 0
+    assert abs(r["iterations"] - o["iterations"]) <= 2
+    assert r["true_rel_res"] < 1.5e-8
+
+
 def pcg_iterations(A, b, apply_m, tol=1e-8, maxit=5000):
     x = np.zeros_like(b)
     r = b.copy()
@@ -194,13 +228,16 @@ def test_learned_pcg_beats_jacobi(ctx, mode):
     assert r["true_rel_res"] < 1.05e-8
     assert r["iterations"] < 0.7 * rj["iterations"]
     if mode != aa.MODE_MIXED:
-        # same preconditioner from the oracle, FP64 PCG on the CPU
+        # same preconditioner from the oracle: FP64 PCG on the CPU (plain loop) and the
+        # reference's CG call sequence with y = G (G^T x)
         A = sp.csr_matrix((vals, cols, ro), shape=(N, N))
-        G = go.preconditioner(go.Graph(ro, cols, vals), th, bf=True)
-        GT = G.T.tocsr()
+        G, GT = oracle_factor(ro, cols, vals, th)
         it_o = pcg_iterations(A, b, lambda v: G @ (GT @ v))
-        say("  oracle iterations", it_o)
+        _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_FACTOR, factor=(G, GT),
+                         max_iter=2000)
+        say("  oracle iterations", it_o, o["iterations"])
         assert abs(r["iterations"] - it_o) <= 2
+        assert abs(r["iterations"] - o["iterations"]) <= 2
 
 
 def test_learned_mixed_bf16_factor_storage(ctx):

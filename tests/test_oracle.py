@@ -424,3 +424,32 @@ def test_vblock_spmv_pins():
     colmajor = v2.reshape(-1, 2, 2).transpose(0, 2, 1).reshape(-1)
     np.testing.assert_array_equal(orc.spmv_vblock(ro2, cols2, bcols2, bs2, bo2, v2, x2),
                                   orc.spmv_bsr(ro2, cols2, colmajor, x2))
+
+
+def test_factor_preconditioner_pins():
+    """The factored preconditioner y = G (G^T x) in the reference's CG / BiCGStab call
+    sequence: the identity factor is the unpreconditioned solver (same iterations,
+    same x bit for bit), G = D^-1/2 is Jacobi up to rounding (iterations within 1),
+    and the exact inverse factor (G G^T = A^-1, dense Cholesky) solves in one step."""
+    import scipy.sparse as sp
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 8, params=[1.5, 1, 1, 1], seed=3)
+    n = len(ro) - 1
+    b = aa.uniform01(5, n)
+    A = sp.csr_matrix((vals, cols, ro), shape=(n, n))
+    I = sp.identity(n, format="csr")
+    for solver in (orc.ORC_CG, orc.ORC_BICGS):
+        x1, r1 = orc.solve(ro, cols, vals, b, solver=solver, precond=orc.PREC_FACTOR, factor=(I, I), max_iter=2000)
+        x0, r0 = orc.solve(ro, cols, vals, b, solver=solver, precond=orc.PREC_NONE, max_iter=2000)
+        assert r1["iterations"] == r0["iterations"] and r1["status"] == 1
+        np.testing.assert_array_equal(x1, x0)
+        D = sp.diags(1.0 / np.sqrt(A.diagonal())).tocsr()
+        _, rd = orc.solve(ro, cols, vals, b, solver=solver, precond=orc.PREC_FACTOR, factor=(D, D), max_iter=2000)
+        _, rj = orc.solve(ro, cols, vals, b, solver=solver, precond=orc.PREC_DIAG, max_iter=2000)
+        assert abs(rd["iterations"] - rj["iterations"]) <= 1
+    C = np.linalg.cholesky(np.linalg.inv(A.toarray()))
+    G = sp.csr_matrix(C)
+    G.sort_indices()
+    GT = G.T.tocsr()
+    GT.sort_indices()
+    _, re = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_FACTOR, factor=(G, GT), max_iter=10)
+    assert re["iterations"] <= 2 and re["status"] == 1
