@@ -159,12 +159,16 @@ class HIPMatrix {
       h_cols = m_cols.data();
       h_vals = m_vals.data();
     }
-    m_dcol.resize(m_n);
-    alien_hip::host_dcol(m_n, h_ro, h_cols, m_dcol.data());
     hipStream_t st = m_ctx->stream;
     d_ro.upload(h_ro, m_n + 1, st);
     d_cols.upload(h_cols, m_nnz, st);
-    d_dcol.upload(m_dcol.data(), m_dcol.size(), st);
+    // the diagonal positions computed on the device, the host profile's copy read back
+    d_dcol.resize(m_n);
+    alien_hip::launch_dcol(st, m_n, d_ro.data(), d_cols.data(), d_dcol.data());
+    alien_hip::check_launch("dcol");
+    m_dcol.resize(m_n);
+    if (m_n)
+      ALIEN_HIP_CHECK(hipMemcpyAsync(m_dcol.data(), d_dcol.data(), sizeof(int) * m_n, hipMemcpyDeviceToHost, st));
     d_vals.upload(h_vals, m_nnz, st);
     setSpmvLayout(spmv_layout);
     ALIEN_HIP_CHECK(hipStreamSynchronize(st));

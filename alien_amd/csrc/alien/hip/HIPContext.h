@@ -82,6 +82,7 @@ class DeviceArray {
 struct alien_hip_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  bool trace = false; // ALIEN_HIP_TRACE=1 at creation: set-up phase times on stderr
   // reduction workspace (see RedWork); grows on demand, never while a graph is captured
   alien_hip::DeviceArray<double> partials;
   int red_cap = 0;

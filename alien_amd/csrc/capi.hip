@@ -107,6 +107,7 @@ int alien_hip_ctx_create(void* stream, alien_hip_ctx** out) {
       if (v >= kXcds) *caps[i] = v / kXcds * kXcds;
     }
   *out = make_ctx(stream);
+  if (const char* e = std::getenv("ALIEN_HIP_TRACE")) (*out)->trace = std::atoi(e) != 0;
   API_END
 }
 

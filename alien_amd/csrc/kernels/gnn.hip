@@ -505,10 +505,13 @@ __global__ void k_adam(int np, float* __restrict__ P, const float* __restrict__ 
 
 // Transpose positions: tpos[k] = position of (j, i) in row j for the entry
 // k = (i, j); a missing mirror entry (pattern not symmetric) raises *bad.
-__global__ void k_tpos(int n, const int* __restrict__ ro, const int* __restrict__ cols, int* __restrict__ tpos,
-                       int* __restrict__ bad) {
+__global__ void k_tpos(int n, const int* __restrict__ ro, const int* __restrict__ cols, const int* __restrict__ dcol,
+                       const double* __restrict__ vals, int* __restrict__ tpos, int* __restrict__ bad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int d = dcol[i];
+  if (d >= ro[i + 1] || cols[d] != i) atomicOr(bad, 2);
+  else if (!(vals[d] > 0)) atomicOr(bad, 4);
   for (int k = ro[i]; k < ro[i + 1]; ++k) {
     const int j = cols[k];
     int q = -1;
@@ -518,15 +521,10 @@ __global__ void k_tpos(int n, const int* __restrict__ ro, const int* __restrict_
         break;
       }
     tpos[k] = q;
-    if (q < 0) *bad = 1;
+    if (q < 0) atomicOr(bad, 1);
   }
 }
 
-// Row-bundled values of G (TRANS = false: rows = lower + diagonal of A,
-// G_ij = dis_i Ghat_ij) or of G^T (TRANS: rows = diagonal + upper, (G^T)_ij =
-// G_ji = dis_j Ghat_ji, read at the transpose position). Lane = row; the row's
-// entries are walked together with the bundle's slots: slot j of an RB-CSR
-// bundle is the row's j-th entry, an RB-DIA slot the entry at offset doff.
 // a factor value in its storage type: FP64, FP32, or bfloat16 (FP32 rounded to
 // nearest even, then its upper half rounded to nearest even)
 template <typename T>
@@ -538,6 +536,11 @@ __device__ __forceinline__ bf16_bits factor_store<bf16_bits>(double g) {
   return (bf16_bits)to_bf16((float)g);
 }
 
+// Row-bundled values of G (TRANS = false: rows = lower + diagonal of A,
+// G_ij = dis_i Ghat_ij) or of G^T (TRANS: rows = diagonal + upper, (G^T)_ij =
+// G_ji = dis_j Ghat_ji, read at the transpose position). Lane = row; the row's
+// entries are walked together with the bundle's slots: slot j of an RB-CSR
+// bundle is the row's j-th entry, an RB-DIA slot the entry at offset doff.
 template <typename T, bool TRANS>
 __global__ void k_fill_factor(int n, const int* __restrict__ ro, const int* __restrict__ cols,
                               const int* __restrict__ dcol, const int* __restrict__ tpos, const int* __restrict__ bo,
@@ -611,24 +614,23 @@ Engine::Engine(alien_hip_ctx* ctx, const SystemView& v) : m_ctx(ctx), m_n(v.n) {
 void Engine::init(const SystemView& v) {
   const int n = v.n;
   m_nnz = v.ro[n];
-  for (int i = 0; i < n; ++i) {
-    if (v.dcol[i] >= v.ro[i + 1] || v.cols[v.dcol[i]] != i)
-      throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: missing diagonal");
-    if (!(v.vals[v.dcol[i]] > 0)) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: non-positive diagonal");
-  }
   m_v = v;
   m_v.vals = nullptr;
   hipStream_t s = m_ctx->stream;
   d_tpos.resize(m_nnz);
   {
+    // transpose positions, and the checks of the stored diagonal and the pattern's symmetry
     DeviceArray<int> bad(1);
     ALIEN_HIP_CHECK(hipMemsetAsync(bad.data(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_tpos, g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_tpos.data(), bad.data());
+    hipLaunchKernelGGL(k_tpos, g1(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, m_v.d_dcol, m_v.d_vals, d_tpos.data(),
+                       bad.data());
     check_launch("gnn tpos");
     int hbad = 0;
     ALIEN_HIP_CHECK(hipMemcpyAsync(&hbad, bad.data(), sizeof(int), hipMemcpyDeviceToHost, s));
     ALIEN_HIP_CHECK(hipStreamSynchronize(s));
-    if (hbad) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: the pattern must be symmetric");
+    if (hbad & 2) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: missing diagonal");
+    if (hbad & 4) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: non-positive diagonal");
+    if (hbad & 1) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: the pattern must be symmetric");
   }
   d_dis.resize(n);
   d_e.resize(m_nnz);
@@ -944,22 +946,37 @@ void Engine::build_factor_layout() {
                                    hipMemcpyDeviceToHost, s));
     ALIEN_HIP_CHECK(hipStreamSynchronize(s));
   }
-  std::vector<int> kb(n), ke(n);
-  for (int t = 0; t < 2; ++t) {
-    for (int r = 0; r < n; ++r) {
-      kb[r] = t == 0 ? m_v.ro[r] : m_v.dcol[r];
-      ke[r] = t == 0 ? m_v.dcol[r] + 1 : m_v.ro[r + 1];
-    }
-    // RB-CSR widths
-    std::vector<int> cbo(nb + 1, 0);
-    long acc = 0;
+  // RB-CSR widths of both factors in one pass: row r of G has dcol - ro + 1 entries, of G^T ro' - dcol
+  std::vector<int> cbos[2] = {std::vector<int>(nb + 1, 0), std::vector<int>(nb + 1, 0)};
+  long totals[2] = {0, 0};
+  {
+    long* acc = totals;
     for (int b = 0; b < nb; ++b) {
-      cbo[b] = (int)acc;
-      int w = 0;
-      for (int r = b * kBundle; r < std::min(n, (b + 1) * kBundle); ++r) w = std::max(w, ke[r] - kb[r]);
-      acc += w;
+      cbos[0][b] = (int)acc[0];
+      cbos[1][b] = (int)acc[1];
+      int w0 = 0, w1 = 0;
+      for (int r = b * kBundle; r < std::min(n, (b + 1) * kBundle); ++r) {
+        w0 = std::max(w0, m_v.dcol[r] - m_v.ro[r] + 1);
+        w1 = std::max(w1, m_v.ro[r + 1] - m_v.dcol[r]);
+      }
+      acc[0] += w0;
+      acc[1] += w1;
     }
-    cbo[nb] = (int)acc;
+    cbos[0][nb] = (int)acc[0];
+    cbos[1][nb] = (int)acc[1];
+  }
+  std::vector<int> kb, ke; // entry ranges per row, built only for a host structure pass
+  for (int t = 0; t < 2; ++t) {
+    const std::vector<int>& cbo = cbos[t];
+    const long acc = totals[t];
+    auto ranges = [&]() {
+      kb.resize(n);
+      ke.resize(n);
+      for (int r = 0; r < n; ++r) {
+        kb[r] = t == 0 ? m_v.ro[r] : m_v.dcol[r];
+        ke[r] = t == 0 ? m_v.dcol[r] + 1 : m_v.ro[r + 1];
+      }
+    };
     if (acc * kBundle > 0x7fffffffL) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: factor too large");
     std::vector<int> dbo, doff;
     std::vector<uint64_t> dmask;
@@ -979,6 +996,7 @@ void Engine::build_factor_layout() {
       dbo[nb] = (int)doff.size();
       dslots = dbo[nb];
     } else {
+      ranges();
       dslots = rbdia_structure(n, m_v.ro, m_v.cols, kb.data(), ke.data(), kRbdiaMaxWidth, dbo, doff, dmask);
     }
     const bool dia = dslots >= 0 && dslots <= kRbdiaMaxGrowth * cbo[nb];
@@ -992,6 +1010,7 @@ void Engine::build_factor_layout() {
       F.doff.upload(doff.data(), doff.size(), s);
       F.dmask.upload(reinterpret_cast<const unsigned long long*>(dmask.data()), dmask.size(), s);
     } else {
+      ranges();
       std::vector<int> bc(slots, -1);
       for (int r = 0; r < n; ++r) {
         const int b = r / kBundle, lane = r % kBundle;

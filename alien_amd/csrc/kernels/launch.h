@@ -79,6 +79,8 @@ void launch_rbbsr2_fill(hipStream_t, int nbr, const int* ro, const int* cols, co
 // variable-block CSR product, one lane per block row
 void launch_spmv_vblock(hipStream_t, int nbr, const int* ro, const int* cols, const int* bcols, const int* bsize,
                         const int* boff, const double* vals, const double* x, double* y);
+// diagonal positions of a device CSR (first entry with column >= row)
+void launch_dcol(hipStream_t, int n, const int* ro, const int* cols, int* dcol);
 void launch_invdiag_csr(hipStream_t, int n, const int* dcol, const double* vals, double* y);
 void launch_invdiag_bellpack(hipStream_t, int n, int slice, const int* bro, const int* bc, const double* bv,
                              double* y);

@@ -408,6 +408,18 @@ __global__ void __launch_bounds__(kBlock) k_spmv_vblock(int nbr, const int* __re
   }
 }
 
+// dcol[i] = first k of row i with cols[k] >= i (CSRStructInfo::computeUpperDiagOffset,
+// simple_csr/CSRStructInfo.h:291-309), one lane per row
+__global__ void __launch_bounds__(kBlock) k_dcol(int n, const int* __restrict__ ro, const int* __restrict__ cols,
+                                                  int* __restrict__ dcol) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  int k = ro[i];
+  const int e = ro[i + 1];
+  while (k < e && cols[k] < i) ++k;
+  dcol[i] = k;
+}
+
 // ---- inverse diagonal -------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_invdiag_csr(int n, const int* __restrict__ dcol,
                                                          const double* __restrict__ vals,
@@ -549,6 +561,9 @@ void launch_spmv_vblock(hipStream_t st, int nbr, const int* ro, const int* cols,
   if (nbr > 0)
     hipLaunchKernelGGL(k_spmv_vblock, rows_grid(nbr), dim3(kBlock), 0, st, nbr, ro, cols, bcols, bsize, boff, vals,
                        x, y);
+}
+void launch_dcol(hipStream_t st, int n, const int* ro, const int* cols, int* dcol) {
+  if (n > 0) hipLaunchKernelGGL(k_dcol, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, dcol);
 }
 void launch_invdiag_csr(hipStream_t st, int n, const int* dcol, const double* vals, double* y) {
   if (n > 0) hipLaunchKernelGGL(k_invdiag_csr, rows_grid(n), dim3(kBlock), 0, st, n, dcol, vals, y);

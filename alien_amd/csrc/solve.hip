@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -821,9 +823,21 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     }
 
     tm.start();
+    // ALIEN_HIP_TRACE: wall time of each set-up phase (the stream drained at each mark)
+    auto t_last = std::chrono::steady_clock::now();
+    auto phase = [&](const char* what) {
+      if (!ctx->trace) return;
+      ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      const auto now = std::chrono::steady_clock::now();
+      std::fprintf(stderr, "alien_hip set-up: %-28s %8.2f ms\n", what,
+                   std::chrono::duration<double, std::milli>(now - t_last).count());
+      t_last = now;
+    };
     Alien::CSRHostView view{n, Aro, Acols, Avals};
     HIPMatrix A(ctx, view, (fused || mixed) ? ALIEN_HIP_LAYOUT_CSR : p->layout, /*borrow_host=*/true);
+    phase("matrix upload");
     if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
+    phase("row-bundled layout");
     HIPVector b, x;
     alg.allocate(A.resource(), b, x);
     b.setValues(b_in);
@@ -918,10 +932,14 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
           v.d_rb_doff = A.d_doff.data();
           v.d_rb_dmask = A.d_dmask.data();
         }
+        phase("Jacobi set-up");
         net.reset(new gnn::Engine(ctx, v));
         theta.upload(p->gnn_theta, gnn::kNParam, ctx->stream);
+        phase("network engine");
         net->infer(theta.data());
+        phase("network inference");
         net->build_factor_layout();
+        phase("factor layout");
         ps.kind = kPrecFactor;
         const size_t ng = net->g_slots(), ngt = net->gt_slots();
         if (mixed && !bicg && p->factor_storage == 1) {
@@ -945,6 +963,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
           ps.gt64 = net->gt_mat<double>(gtv64.data());
         }
         net->release_network_buffers();
+        phase("factor fill + release");
         stall_window = p->stall_window > 0 ? p->stall_window : 400;
       }
       if (!mixed) {
@@ -1011,6 +1030,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       } else {
         // FP32 matrix, preconditioner and Krylov vectors, FP64 dots; FP64 defect correction
         A.ensureRB32();
+        phase("FP32 matrix");
         DeviceArray<float> invd32(n);
         launch_demote(ctx->stream, n, invd64, invd32.data());
         RBMat<float> rb32 = A.rb32();
