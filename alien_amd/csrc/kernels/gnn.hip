@@ -121,19 +121,38 @@ __global__ void __launch_bounds__(256) k_dense(int n, const float* __restrict__ 
 }
 
 // m_i = sum_{j != i} e_ij h_j   (32 lanes per row, one feature per lane)
+// 8 lanes per row, 4 features per lane (one 16-byte load of the neighbour's row
+// chunk), 32 rows per workgroup; workgroups are remapped so that each XCD takes
+// chunks of consecutive rows (its L2 then holds the x- and y-neighbours' rows).
+// Per feature the sum runs over the row's entries in order, as one lane per feature.
+constexpr int kAggRows = 32;
+constexpr int kAggChunk = 64;
 __global__ void __launch_bounds__(256) k_aggregate(int n, const int* __restrict__ ro, const int* __restrict__ cols,
                                                    const float* __restrict__ e, const float* __restrict__ h,
                                                    float* __restrict__ m) {
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  const int i = (int)(t >> 5);
-  const int c = (int)(t & 31);
+  const int p = blockIdx.x, q = p / kXcds;
+  const int logical = (q / kAggChunk) * (kXcds * kAggChunk) + (p % kXcds) * kAggChunk + q % kAggChunk;
+  const int i = logical * kAggRows + (threadIdx.x >> 3);
+  const int c4 = threadIdx.x & 7;
   if (i >= n) return;
-  float acc = 0.f;
+  float4 acc = {0.f, 0.f, 0.f, 0.f};
+  const float4* __restrict__ h4 = reinterpret_cast<const float4*>(h);
   for (int k = ro[i]; k < ro[i + 1]; ++k) {
     const float w = e[k];
-    if (w != 0.f) acc += w * h[(long)cols[k] * kHidden + c];
+    if (w != 0.f) {
+      const float4 v = h4[(long)cols[k] * (kHidden / 4) + c4];
+      acc.x += w * v.x;
+      acc.y += w * v.y;
+      acc.z += w * v.z;
+      acc.w += w * v.w;
+    }
   }
-  m[(long)i * kHidden + c] = acc;
+  reinterpret_cast<float4*>(m)[(long)i * (kHidden / 4) + c4] = acc;
+}
+inline dim3 aggregate_grid(int n) {
+  const long g = (n + kAggRows - 1) / kAggRows;
+  const long q = (long)kXcds * kAggChunk;
+  return dim3((unsigned)((g + q - 1) / q * q));
 }
 
 // ---- output head ------------------------------------------------------------------------------
@@ -669,7 +688,7 @@ void Engine::forward(const float* P) {
                      P + kOffWin, P + kOffBin, (const float*)nullptr, d_z[0].data(), d_h[0].data());
   const int offs[3] = {kOffW1, kOffW2, kOffW3};
   for (int l = 1; l <= 3; ++l) {
-    hipLaunchKernelGGL(k_aggregate, g1((long)n * 32), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_e.data(),
+    hipLaunchKernelGGL(k_aggregate, aggregate_grid(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_e.data(),
                        d_h[l - 1].data(), d_m[l].data());
     hipLaunchKernelGGL(k_dense<64>, gd, dim3(256), 0, s, n, d_h[l - 1].data(), kHidden, kHidden, d_m[l].data(), kHidden,
                        2 * kHidden, P + offs[l - 1], P + offs[l - 1] + 2 * kHidden * kHidden, d_h[l - 1].data(),
@@ -694,7 +713,7 @@ void Engine::infer(const float* P) {
   for (int l = 1; l <= 3; ++l) {
     const float* hin = d_h[(l - 1) & 1].data();
     float* hout = d_h[l & 1].data();
-    hipLaunchKernelGGL(k_aggregate, g1((long)n * 32), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_e.data(), hin,
+    hipLaunchKernelGGL(k_aggregate, aggregate_grid(n), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols, d_e.data(), hin,
                        d_m[1].data());
     hipLaunchKernelGGL(k_dense<64>, gd, dim3(256), 0, s, n, hin, kHidden, kHidden, d_m[1].data(), kHidden,
                        2 * kHidden, P + offs[l - 1], P + offs[l - 1] + 2 * kHidden * kHidden, hin, (float*)nullptr,
