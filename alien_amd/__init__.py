@@ -9,7 +9,7 @@ from ._capi import (  # noqa: F401
     LAYOUT_RBDIA, MODE_FUSED, MODE_GRAPH, MODE_MIXED, MODE_REFERENCE, PREC_CHEB, PREC_DIAG, PREC_ILU0,
     PREC_FILU0, PREC_LEARNED, PREC_NEUMANN, PREC_NONE, AlienHipError, Context, bellpack_build, check, gen_bsr2,
     gen_csr, gen_nnz,
-    gnn_forward, gnn_init, gnn_loss_grad, gnn_nparams, gnn_train, group_by_level, host_color, host_dcol,
+    Matrix, gnn_forward, gnn_init, gnn_loss_grad, gnn_nparams, gnn_train, group_by_level, host_color, host_dcol,
     host_levels, lib, mtx_read_csr, mtx_read_vector, mtx_write_csr, mtx_write_vector,
     rbcsr_bundle_offsets, rbdia_build, solve_csr, uniform01, vblock_to_csr)
 

@@ -148,6 +148,11 @@ _SIGS = [
     ("alien_hip_gnn_loss_grad", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, c_float_p, c_float_p,
                                           C.c_double, C.c_int, c_double_p, c_double_p, c_float_p]),
     ("alien_hip_gnn_train", C.c_int, [c_void_p, C.POINTER(GnnTrainParams), c_float_p, c_double_p]),
+    ("alien_hip_matrix_create", C.c_int, [c_void_p, C.c_int, c_int_p, c_int_p, c_double_p, C.POINTER(c_void_p)]),
+    ("alien_hip_matrix_update_values", C.c_int, [c_void_p, c_double_p]),
+    ("alien_hip_matrix_destroy", C.c_int, [c_void_p]),
+    ("alien_hip_solve", C.c_int, [c_void_p, C.POINTER(SolveParams), c_void_p, c_double_p, c_double_p,
+                                  C.POINTER(SolveResult)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]
@@ -359,23 +364,68 @@ class Context:
         self.close()
 
 
-def solve_csr(ctx: Context, ro, cols, vals, b, x0=None, *, solver=CG, precond=PREC_CHEB, layout=LAYOUT_CSR,
-              mode=MODE_REFERENCE, tol=1e-8, max_iter=1000, poly_factor=0.5, poly_order=3, poly_max_iter=10,
-              ilu_schedule=0, inner_tol=0.0, inner_max_iter=0, max_outer=50, poll_every=8, gnn_theta=None,
-              stall_window=0, filu_factor_niter=0, filu_solver_niter=3, factor_storage=0):
-    """Run one Krylov solve through the hip backend; returns (x, result dict)."""
-    n = len(ro) - 1
-    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
-    b = np.ascontiguousarray(b, dtype=np.float64)
+def _solve_params(solver=CG, precond=PREC_CHEB, layout=LAYOUT_CSR, mode=MODE_REFERENCE, tol=1e-8, max_iter=1000,
+                  poly_factor=0.5, poly_order=3, poly_max_iter=10, ilu_schedule=0, inner_tol=0.0, inner_max_iter=0,
+                  max_outer=50, poll_every=8, gnn_theta=None, stall_window=0, filu_factor_niter=0,
+                  filu_solver_niter=3, factor_storage=0):
+    """(SolveParams, the float32 theta array it points to)."""
     theta = None if gnn_theta is None else np.ascontiguousarray(gnn_theta, dtype=np.float32)
     p = SolveParams(solver, precond, layout, mode, tol, max_iter, poly_factor, poly_order, poly_max_iter,
                     ilu_schedule, inner_tol, inner_max_iter, max_outer, poll_every,
                     None if theta is None else theta.ctypes.data, stall_window, filu_factor_niter, filu_solver_niter,
                     factor_storage)
+    return p, theta
+
+
+def solve_csr(ctx: Context, ro, cols, vals, b, x0=None, **kw):
+    """Run one Krylov solve through the hip backend; returns (x, result dict). Keywords: the fields of
+    alien_hip_solve_params (solver, precond, layout, mode, tol, max_iter, ...)."""
+    n = len(ro) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    p, _theta = _solve_params(**kw)
     r = SolveResult()
     check(lib().alien_hip_solve_csr(ctx.h, C.byref(p), n, iptr(ro), iptr(cols), dptr(vals), dptr(b), dptr(x),
                                     C.byref(r)))
     return x, r.as_dict()
+
+
+class Matrix:
+    """A converted matrix kept between solves (alien_hip_matrix_create / _update_values / alien_hip_solve)."""
+
+    def __init__(self, ctx: Context, ro, cols, vals):
+        self.ctx = ctx
+        self.n = len(ro) - 1
+        self.nnz = len(cols)
+        h = c_void_p()
+        check(lib().alien_hip_matrix_create(ctx.h, self.n, iptr(np.ascontiguousarray(ro, np.int32)),
+                                            iptr(np.ascontiguousarray(cols, np.int32)),
+                                            dptr(np.ascontiguousarray(vals, np.float64)), C.byref(h)))
+        self.h = h
+
+    def update_values(self, vals):
+        v = np.ascontiguousarray(vals, np.float64)
+        assert len(v) == self.nnz
+        check(lib().alien_hip_matrix_update_values(self.h, dptr(v)))
+
+    def solve(self, b, x0=None, **kw):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        p, _theta = _solve_params(**kw)
+        r = SolveResult()
+        check(lib().alien_hip_solve(self.ctx.h, C.byref(p), self.h, dptr(b), dptr(x), C.byref(r)))
+        return x, r.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().alien_hip_matrix_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 # --------------------------------------------------------------------------

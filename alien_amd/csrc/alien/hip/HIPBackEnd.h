@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -185,7 +186,47 @@ class HIPMatrix {
   const int* kcol() const { return h_ro; }
   const int* cols() const { return h_cols; }
   const int* dcol() const { return m_dcol.data(); }
-  const double* hostValues() const { return h_vals; }
+  const double* hostValues() const {
+    syncHostValues();
+    return h_vals;
+  }
+  // New values on the same structure (the matrix owns its host copy): the
+  // device CSR values and every value layout already built are refilled.
+  // The host copy is refreshed from the device only when a host-side builder
+  // needs it (BEllPack).
+  void updateValues(const double* vals) {
+    if (h_vals != m_vals.data())
+      throw alien_hip::Error(ALIEN_HIP_EINVAL, "updateValues: the matrix reads the caller's arrays");
+    hipStream_t st = m_ctx->stream;
+    d_vals.upload(vals, m_nnz, st);
+    m_host_vals_stale = true;
+    if (!d_bv.empty()) {
+      if (m_rbdia)
+        alien_hip::launch_rbdia_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
+                                             d_doff.data(), d_bv.data());
+      else
+        alien_hip::launch_rbcsr_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
+                                             d_bc.data(), d_bv.data());
+    }
+    if (!d_bv32.empty()) {
+      if (m_rbdia)
+        alien_hip::launch_rbdia_fill<float>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
+                                            d_doff.data(), d_bv32.data());
+      else
+        alien_hip::launch_rbcsr_fill<float>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
+                                            d_bc.data(), d_bv32.data());
+    }
+    alien_hip::check_launch("updateValues refill");
+    if (!d_bellv.empty()) { // BEllPack values are filled on the host: rebuild them
+      syncHostValues();
+      d_bro.reset();
+      d_bellc.reset();
+      d_bellv.reset();
+      ensureBEllPack(false);
+    }
+    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+  }
+
   // host bundle offsets of the row-bundled layout (after ensureRB / ensureRBCSR)
   const std::vector<int>& bundleOffsets() const { return m_bo; }
 
@@ -278,6 +319,7 @@ class HIPMatrix {
   }
   void ensureBEllPack(bool masks) {
     if (d_bro.empty()) {
+      syncHostValues();
       const int ns = alien_hip::bellpack_nslices(m_n, kBEllPackSlice);
       m_bro.resize(ns + 1);
       alien_hip::bellpack_bro(m_n, kBEllPackSlice, h_ro, m_bro.data());
@@ -324,6 +366,15 @@ class HIPMatrix {
   int m_layout = kLayoutCSR;
   std::vector<int> m_kcol, m_cols, m_dcol, m_bo, m_bro;
   std::vector<double> m_vals;
+  // host values behind the device's after updateValues (read back on demand)
+  void syncHostValues() const {
+    if (!m_host_vals_stale) return;
+    ALIEN_HIP_CHECK(hipMemcpyAsync(const_cast<double*>(m_vals.data()), d_vals.data(), sizeof(double) * m_nnz,
+                                   hipMemcpyDeviceToHost, m_ctx->stream));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
+    m_host_vals_stale = false;
+  }
+  mutable bool m_host_vals_stale = false;
   const int* h_ro = nullptr;   // host profile: the copies above or the caller's arrays
   const int* h_cols = nullptr;
   const double* h_vals = nullptr;

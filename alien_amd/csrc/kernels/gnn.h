@@ -47,7 +47,7 @@ struct SystemView {
   const int* ro = nullptr;
   const int* cols = nullptr;
   const int* dcol = nullptr;
-  const double* vals = nullptr;
+  const double* vals = nullptr; // not read by the engine (its checks run on the device)
   const int* d_ro = nullptr;
   const int* d_cols = nullptr;
   const int* d_dcol = nullptr;

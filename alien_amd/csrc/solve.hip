@@ -768,6 +768,310 @@ double true_residual(Alg& alg, const HIPMatrix& A, const HIPVector& b, const HIP
 
 } // namespace alien_hip
 
+namespace alien_hip {
+namespace {
+
+// Parameter checks shared by the solve entry points.
+void validate_params(const alien_hip_solve_params* p) {
+  const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
+  const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
+  const bool learned = p->precond == ALIEN_HIP_PREC_LEARNED;
+  const bool bicg = p->solver == ALIEN_HIP_BICGS;
+  if ((fused || mixed) && !bicg && (p->solver != ALIEN_HIP_CG || (p->precond != ALIEN_HIP_PREC_CHEB &&
+                                                                  p->precond != ALIEN_HIP_PREC_DIAG && !learned)))
+    throw Error(ALIEN_HIP_EINVAL,
+                "fused/graph/mixed modes implement CG with the Diag, Chebyshev or learned preconditioner");
+  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !learned)
+    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement BiCGStab with the Diag or learned preconditioner");
+  if (learned && !(fused || mixed))
+    throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
+  if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
+  if (p->stall_window < 0) throw Error(ALIEN_HIP_EINVAL, "solve_csr: stall_window < 0");
+  if ((p->precond == ALIEN_HIP_PREC_FILU0 || p->precond == ALIEN_HIP_PREC_NEUMANN) && (fused || mixed))
+    throw Error(ALIEN_HIP_EINVAL, "FILU0 and Neumann run in the REFERENCE mode");
+  if (p->precond == ALIEN_HIP_PREC_FILU0 && (p->filu_factor_niter < 0 || p->filu_solver_niter < 0))
+    throw Error(ALIEN_HIP_EINVAL, "FILU0: negative iteration counts");
+  if (p->factor_storage != 0 && p->factor_storage != 1)
+    throw Error(ALIEN_HIP_EINVAL, "solve_csr: factor_storage must be 0 (FP32) or 1 (bfloat16)");
+  if (p->precond == ALIEN_HIP_PREC_CHEB && p->poly_order < 1)
+    throw Error(ALIEN_HIP_EINVAL, "Chebyshev order must be >= 1");
+  if (p->max_iter < 0 || !(p->tol >= 0)) throw Error(ALIEN_HIP_EINVAL, "solve: bad tol / max_iter");
+}
+
+// ALIEN_HIP_TRACE: wall time of each set-up phase on stderr (the stream drained at each mark)
+struct SetupTrace {
+  alien_hip_ctx* ctx;
+  std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+  explicit SetupTrace(alien_hip_ctx* c) : ctx(c) {}
+  void operator()(const char* what) {
+    if (!ctx->trace) return;
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "alien_hip set-up: %-28s %8.2f ms\n", what,
+                 std::chrono::duration<double, std::milli>(now - last).count());
+    last = now;
+  }
+};
+
+// The solve proper on a converted matrix: preconditioner set-up, Krylov loop,
+// true residual, solution copied to x_io. tm was started at the beginning of
+// the set-up; phase marks the set-up trace.
+void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A, const double* b_in, double* x_io,
+              alien_hip_solve_result* res, EventTimer& tm, SetupTrace& phase) {
+  const int n = A.getNRows();
+  const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
+  const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
+  const bool learned = p->precond == ALIEN_HIP_PREC_LEARNED;
+  const bool bicg = p->solver == ALIEN_HIP_BICGS;
+  Alg alg(ctx);
+  HIPVector b, x;
+  alg.allocate(A.resource(), b, x);
+  b.setValues(b_in);
+  x.setValues(x_io);
+
+  if (!fused && !mixed) {
+    switch (p->precond) {
+    case ALIEN_HIP_PREC_DIAG: {
+      Alien::DiagPreconditioner<Alg> P(alg, A);
+      P.init();
+      res->setup_s = tm.stop();
+      run_reference(alg, p, P, A, b, x, res, tm);
+      break;
+    }
+    case ALIEN_HIP_PREC_CHEB: {
+      Alien::ChebyshevPreconditioner<Alg> P(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter);
+      P.init();
+      res->cheb_alpha = P.alpha();
+      res->cheb_beta = P.beta();
+      res->setup_s = tm.stop();
+      run_reference(alg, p, P, A, b, x, res, tm);
+      break;
+    }
+    case ALIEN_HIP_PREC_ILU0: {
+      Alien::ILU0Preconditioner<Alg> P(alg, A);
+      P.init();
+      res->setup_s = tm.stop();
+      run_reference(alg, p, P, A, b, x, res, tm);
+      break;
+    }
+    case ALIEN_HIP_PREC_NEUMANN: {
+      Alien::NeumannPolyPreconditioner<Alg> P(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter);
+      P.init();
+      res->setup_s = tm.stop();
+      run_reference(alg, p, P, A, b, x, res, tm);
+      break;
+    }
+    case ALIEN_HIP_PREC_FILU0: {
+      Alien::FILU0Preconditioner<Alg> P(alg, A);
+      P.setParameter("nb-factor-iter", p->filu_factor_niter);
+      P.setParameter("nb-solver-iter", p->filu_solver_niter);
+      P.init();
+      res->setup_s = tm.stop();
+      run_reference(alg, p, P, A, b, x, res, tm);
+      break;
+    }
+    default: {
+      IdentityPreconditioner<Alg> P;
+      res->setup_s = tm.stop();
+      run_reference(alg, p, P, A, b, x, res, tm);
+    }
+    }
+  } else {
+    // preconditioner set-up shared by the fused paths (FP64, as the reference)
+    std::unique_ptr<Alien::ChebyshevPreconditioner<Alg>> cheb;
+    std::unique_ptr<Alien::DiagPreconditioner<Alg>> diag;
+    PrecondSpec ps;
+    const double* invd64;
+    if (p->precond == ALIEN_HIP_PREC_CHEB) {
+      cheb.reset(new Alien::ChebyshevPreconditioner<Alg>(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter));
+      cheb->init();
+      res->cheb_alpha = cheb->alpha();
+      res->cheb_beta = cheb->beta();
+      ps = cheb_spec(*cheb);
+      invd64 = cheb->invDiag().data();
+    } else {
+      // Diag, and the learned preconditioner's fallback
+      diag.reset(new Alien::DiagPreconditioner<Alg>(alg, A));
+      diag->init();
+      invd64 = diag->invDiag().data();
+    }
+    // learned preconditioner: network inference on A (read in place), then G
+    // and G^T row-bundled in the solver's precision
+    std::unique_ptr<gnn::Engine> net;
+    DeviceArray<float> theta, gv32, gtv32;
+    DeviceArray<double> gv64, gtv64;
+    DeviceArray<bf16_bits> gv16, gtv16;
+    int stall_window = 0;
+    if (learned) {
+      gnn::SystemView v;
+      v.n = n;
+      v.ro = A.kcol();
+      v.cols = A.cols();
+      v.dcol = A.dcol();
+      v.d_ro = A.d_ro.data();
+      v.d_cols = A.d_cols.data();
+      v.d_dcol = A.d_dcol.data();
+      v.d_vals = A.d_vals.data();
+      if (A.isRBDIA()) {
+        v.rb_bo = A.bundleOffsets().data();
+        v.d_rb_doff = A.d_doff.data();
+        v.d_rb_dmask = A.d_dmask.data();
+      }
+      phase("Jacobi set-up");
+      net.reset(new gnn::Engine(ctx, v));
+      theta.upload(p->gnn_theta, gnn::kNParam, ctx->stream);
+      phase("network engine");
+      net->infer(theta.data());
+      phase("network inference");
+      net->build_factor_layout();
+      phase("factor layout");
+      ps.kind = kPrecFactor;
+      const size_t ng = net->g_slots(), ngt = net->gt_slots();
+      if (mixed && !bicg && p->factor_storage == 1) {
+        gv16.resize(ng);
+        gtv16.resize(ngt);
+        net->fill_factor<bf16_bits>(gv16.data(), gtv16.data());
+        ps.g16 = net->g_mat<float, bf16_bits>(gv16.data());
+        ps.gt16 = net->gt_mat<float, bf16_bits>(gtv16.data());
+        ps.half = true;
+      } else if (mixed) {
+        gv32.resize(ng);
+        gtv32.resize(ngt);
+        net->fill_factor<float>(gv32.data(), gtv32.data());
+        ps.g32 = net->g_mat<float>(gv32.data());
+        ps.gt32 = net->gt_mat<float>(gtv32.data());
+      } else {
+        gv64.resize(ng);
+        gtv64.resize(ngt);
+        net->fill_factor<double>(gv64.data(), gtv64.data());
+        ps.g64 = net->g_mat<double>(gv64.data());
+        ps.gt64 = net->gt_mat<double>(gtv64.data());
+      }
+      net->release_network_buffers();
+      phase("factor fill + release");
+      stall_window = p->stall_window > 0 ? p->stall_window : 400;
+    }
+    if (!mixed) {
+      RBMat<double> rb = A.rb64();
+      const bool graph = p->mode == ALIEN_HIP_MODE_GRAPH;
+      DeviceArray<double> x0;
+      if (learned) {
+        x0.resize(n);
+        launch_copy<double>(ctx->stream, n, x.data(), x0.data());
+      }
+      res->setup_s = tm.stop();
+      tm.start();
+      bool stalled = false;
+      if (bicg) {
+        BiCGState s;
+        {
+          FusedBiCG<double> F(ctx, n, rb, invd64, ps);
+          s = F.solve(b.data(), x.data(), p->tol, p->max_iter, graph, p->poll_every, nullptr, stall_window,
+                      &stalled);
+        }
+        if (stalled) {
+          res->fallback = 1;
+          if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
+          const int used = s.it;
+          FusedBiCG<double> J(ctx, n, rb, invd64, PrecondSpec{});
+          s = J.solve(b.data(), x.data(), p->tol, std::max(0, p->max_iter - used), graph, p->poll_every,
+                      nullptr);
+          s.it += used;
+        }
+        res->solve_s = tm.stop();
+        res->iterations = s.it;
+        res->status = s.status;
+        res->error = s.breakdown ? 1 : 0; // NullValueException
+        double bb = alg.dot(b, b);
+        res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
+        res->true_rel_res = true_residual(alg, A, b, x);
+        x.copyValuesTo(x_io);
+        ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return;
+      }
+      CGState s;
+      {
+        FusedCG<double> F(ctx, n, rb, invd64, ps);
+        s = F.solve(b.data(), x.data(), false, p->tol, p->max_iter, graph, p->poll_every, nullptr, stall_window,
+                    &stalled);
+        if (!stalled) finish_breakdown(F, s, &res->error);
+      }
+      if (stalled) {
+        // Jacobi from the last finite iterate, for the remaining iterations
+        res->fallback = 1;
+        if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
+        const int used = s.it;
+        FusedCG<double> J(ctx, n, rb, invd64, PrecondSpec{});
+        s = J.solve(b.data(), x.data(), false, p->tol, std::max(0, p->max_iter - used), graph, p->poll_every,
+                    nullptr);
+        finish_breakdown(J, s, &res->error);
+        s.it += used;
+      }
+      res->solve_s = tm.stop();
+      res->iterations = s.it;
+      res->status = s.status;
+      double bb = alg.dot(b, b);
+      res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
+    } else {
+      // FP32 matrix, preconditioner and Krylov vectors, FP64 dots; FP64 defect correction
+      A.ensureRB32();
+      phase("FP32 matrix");
+      DeviceArray<float> invd32(n);
+      launch_demote(ctx->stream, n, invd64, invd32.data());
+      RBMat<float> rb32 = A.rb32();
+      RBMat<double> rb64 = A.rb64();
+      res->setup_s = tm.stop();
+      tm.start();
+      const double delta = p->inner_tol > 0 ? p->inner_tol : 0.1;
+      auto run = bicg ? mixed_reliable_bicg : mixed_reliable_cg;
+      MixedResult m = run(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol, p->max_iter, delta,
+                          p->poll_every, true, stall_window);
+      if (m.stalled) {
+        // Jacobi from the last reliable update, for the remaining iterations
+        res->fallback = 1;
+        const int used = m.it, ups = m.updates;
+        m = run(ctx, n, rb64, rb32, invd32.data(), PrecondSpec{}, b.data(), x.data(), p->tol,
+                std::max(0, p->max_iter - used), delta, p->poll_every, true, 0);
+        m.it += used;
+        m.updates += ups;
+      }
+      if (m.fp32_stalled) {
+        // FP64 Jacobi from the last reliable update, for the remaining iterations (a run
+        // with the learned M reports a stall instead, so Jacobi is the inner preconditioner)
+        res->fallback |= 2;
+        const int used = m.it;
+        const int left = std::max(0, p->max_iter - used);
+        if (bicg) {
+          FusedBiCG<double> F(ctx, n, rb64, invd64, PrecondSpec{});
+          const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, true, p->poll_every, nullptr);
+          m.it = used + s.it;
+          m.status = s.status;
+          m.error = s.breakdown ? 1 : 0;
+          m.rel = std::sqrt(s.rr / alg.dot(b, b));
+        } else {
+          FusedCG<double> F(ctx, n, rb64, invd64, PrecondSpec{});
+          CGState s = F.solve(b.data(), x.data(), false, p->tol, left, true, p->poll_every, nullptr);
+          finish_breakdown(F, s, &m.error);
+          m.it = used + s.it;
+          m.status = s.status;
+          m.rel = std::sqrt(s.rr / alg.dot(b, b));
+        }
+      }
+      res->solve_s = tm.stop();
+      res->iterations = m.it;
+      res->outer = m.updates;
+      res->status = m.status;
+      res->error = m.error;
+      res->value = m.rel;
+    }
+  }
+  res->true_rel_res = true_residual(alg, A, b, x);
+  x.copyValuesTo(x_io);
+}
+
+} // namespace
+} // namespace alien_hip
+
 using namespace alien_hip;
 
 extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_params* p, int n, const int* ro,
@@ -778,32 +1082,10 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
   try {
     const bool need_diag = p->precond != ALIEN_HIP_PREC_NONE;
     validate_csr(n, ro, cols, need_diag);
-    if (p->max_iter < 0 || !(p->tol >= 0)) throw Error(ALIEN_HIP_EINVAL, "solve_csr: bad tol / max_iter");
+    validate_params(p);
     const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
     const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
-    const bool learned = p->precond == ALIEN_HIP_PREC_LEARNED;
-    const bool bicg = p->solver == ALIEN_HIP_BICGS;
-    if ((fused || mixed) && !bicg && (p->solver != ALIEN_HIP_CG || (p->precond != ALIEN_HIP_PREC_CHEB &&
-                                                                    p->precond != ALIEN_HIP_PREC_DIAG && !learned)))
-      throw Error(ALIEN_HIP_EINVAL,
-                  "fused/graph/mixed modes implement CG with the Diag, Chebyshev or learned preconditioner");
-    if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !learned)
-      throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement BiCGStab with the Diag or learned preconditioner");
-    if (learned && !(fused || mixed))
-      throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
-    if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
-    if (p->stall_window < 0) throw Error(ALIEN_HIP_EINVAL, "solve_csr: stall_window < 0");
-    if ((p->precond == ALIEN_HIP_PREC_FILU0 || p->precond == ALIEN_HIP_PREC_NEUMANN) && (fused || mixed))
-      throw Error(ALIEN_HIP_EINVAL, "FILU0 and Neumann run in the REFERENCE mode");
-    if (p->precond == ALIEN_HIP_PREC_FILU0 && (p->filu_factor_niter < 0 || p->filu_solver_niter < 0))
-      throw Error(ALIEN_HIP_EINVAL, "FILU0: negative iteration counts");
-    if (p->factor_storage != 0 && p->factor_storage != 1)
-      throw Error(ALIEN_HIP_EINVAL, "solve_csr: factor_storage must be 0 (FP32) or 1 (bfloat16)");
-    if (p->precond == ALIEN_HIP_PREC_CHEB && p->poly_order < 1)
-      throw Error(ALIEN_HIP_EINVAL, "Chebyshev order must be >= 1");
-
     EventTimer tm(ctx->stream);
-    Alg alg(ctx);
 
     // multicolour ILU0: solve the permuted system P A P^T (P x) = P b
     std::vector<int> perm;
@@ -823,267 +1105,96 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     }
 
     tm.start();
-    // ALIEN_HIP_TRACE: wall time of each set-up phase (the stream drained at each mark)
-    auto t_last = std::chrono::steady_clock::now();
-    auto phase = [&](const char* what) {
-      if (!ctx->trace) return;
-      ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      const auto now = std::chrono::steady_clock::now();
-      std::fprintf(stderr, "alien_hip set-up: %-28s %8.2f ms\n", what,
-                   std::chrono::duration<double, std::milli>(now - t_last).count());
-      t_last = now;
-    };
+    SetupTrace phase(ctx);
     Alien::CSRHostView view{n, Aro, Acols, Avals};
     HIPMatrix A(ctx, view, (fused || mixed) ? ALIEN_HIP_LAYOUT_CSR : p->layout, /*borrow_host=*/true);
     phase("matrix upload");
     if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
     phase("row-bundled layout");
-    HIPVector b, x;
-    alg.allocate(A.resource(), b, x);
-    b.setValues(b_in);
-    x.setValues(x_io);
-
-    if (!fused && !mixed) {
-      switch (p->precond) {
-      case ALIEN_HIP_PREC_DIAG: {
-        Alien::DiagPreconditioner<Alg> P(alg, A);
-        P.init();
-        res->setup_s = tm.stop();
-        run_reference(alg, p, P, A, b, x, res, tm);
-        break;
-      }
-      case ALIEN_HIP_PREC_CHEB: {
-        Alien::ChebyshevPreconditioner<Alg> P(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter);
-        P.init();
-        res->cheb_alpha = P.alpha();
-        res->cheb_beta = P.beta();
-        res->setup_s = tm.stop();
-        run_reference(alg, p, P, A, b, x, res, tm);
-        break;
-      }
-      case ALIEN_HIP_PREC_ILU0: {
-        Alien::ILU0Preconditioner<Alg> P(alg, A);
-        P.init();
-        res->setup_s = tm.stop();
-        run_reference(alg, p, P, A, b, x, res, tm);
-        break;
-      }
-      case ALIEN_HIP_PREC_NEUMANN: {
-        Alien::NeumannPolyPreconditioner<Alg> P(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter);
-        P.init();
-        res->setup_s = tm.stop();
-        run_reference(alg, p, P, A, b, x, res, tm);
-        break;
-      }
-      case ALIEN_HIP_PREC_FILU0: {
-        Alien::FILU0Preconditioner<Alg> P(alg, A);
-        P.setParameter("nb-factor-iter", p->filu_factor_niter);
-        P.setParameter("nb-solver-iter", p->filu_solver_niter);
-        P.init();
-        res->setup_s = tm.stop();
-        run_reference(alg, p, P, A, b, x, res, tm);
-        break;
-      }
-      default: {
-        IdentityPreconditioner<Alg> P;
-        res->setup_s = tm.stop();
-        run_reference(alg, p, P, A, b, x, res, tm);
-      }
-      }
-    } else {
-      // preconditioner set-up shared by the fused paths (FP64, as the reference)
-      std::unique_ptr<Alien::ChebyshevPreconditioner<Alg>> cheb;
-      std::unique_ptr<Alien::DiagPreconditioner<Alg>> diag;
-      PrecondSpec ps;
-      const double* invd64;
-      if (p->precond == ALIEN_HIP_PREC_CHEB) {
-        cheb.reset(new Alien::ChebyshevPreconditioner<Alg>(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter));
-        cheb->init();
-        res->cheb_alpha = cheb->alpha();
-        res->cheb_beta = cheb->beta();
-        ps = cheb_spec(*cheb);
-        invd64 = cheb->invDiag().data();
-      } else {
-        // Diag, and the learned preconditioner's fallback
-        diag.reset(new Alien::DiagPreconditioner<Alg>(alg, A));
-        diag->init();
-        invd64 = diag->invDiag().data();
-      }
-      // learned preconditioner: network inference on A (read in place), then G
-      // and G^T row-bundled in the solver's precision
-      std::unique_ptr<gnn::Engine> net;
-      DeviceArray<float> theta, gv32, gtv32;
-      DeviceArray<double> gv64, gtv64;
-      DeviceArray<bf16_bits> gv16, gtv16;
-      int stall_window = 0;
-      if (learned) {
-        gnn::SystemView v;
-        v.n = n;
-        v.ro = A.kcol();
-        v.cols = A.cols();
-        v.dcol = A.dcol();
-        v.vals = A.hostValues();
-        v.d_ro = A.d_ro.data();
-        v.d_cols = A.d_cols.data();
-        v.d_dcol = A.d_dcol.data();
-        v.d_vals = A.d_vals.data();
-        if (A.isRBDIA()) {
-          v.rb_bo = A.bundleOffsets().data();
-          v.d_rb_doff = A.d_doff.data();
-          v.d_rb_dmask = A.d_dmask.data();
-        }
-        phase("Jacobi set-up");
-        net.reset(new gnn::Engine(ctx, v));
-        theta.upload(p->gnn_theta, gnn::kNParam, ctx->stream);
-        phase("network engine");
-        net->infer(theta.data());
-        phase("network inference");
-        net->build_factor_layout();
-        phase("factor layout");
-        ps.kind = kPrecFactor;
-        const size_t ng = net->g_slots(), ngt = net->gt_slots();
-        if (mixed && !bicg && p->factor_storage == 1) {
-          gv16.resize(ng);
-          gtv16.resize(ngt);
-          net->fill_factor<bf16_bits>(gv16.data(), gtv16.data());
-          ps.g16 = net->g_mat<float, bf16_bits>(gv16.data());
-          ps.gt16 = net->gt_mat<float, bf16_bits>(gtv16.data());
-          ps.half = true;
-        } else if (mixed) {
-          gv32.resize(ng);
-          gtv32.resize(ngt);
-          net->fill_factor<float>(gv32.data(), gtv32.data());
-          ps.g32 = net->g_mat<float>(gv32.data());
-          ps.gt32 = net->gt_mat<float>(gtv32.data());
-        } else {
-          gv64.resize(ng);
-          gtv64.resize(ngt);
-          net->fill_factor<double>(gv64.data(), gtv64.data());
-          ps.g64 = net->g_mat<double>(gv64.data());
-          ps.gt64 = net->gt_mat<double>(gtv64.data());
-        }
-        net->release_network_buffers();
-        phase("factor fill + release");
-        stall_window = p->stall_window > 0 ? p->stall_window : 400;
-      }
-      if (!mixed) {
-        RBMat<double> rb = A.rb64();
-        const bool graph = p->mode == ALIEN_HIP_MODE_GRAPH;
-        DeviceArray<double> x0;
-        if (learned) {
-          x0.resize(n);
-          launch_copy<double>(ctx->stream, n, x.data(), x0.data());
-        }
-        res->setup_s = tm.stop();
-        tm.start();
-        bool stalled = false;
-        if (bicg) {
-          BiCGState s;
-          {
-            FusedBiCG<double> F(ctx, n, rb, invd64, ps);
-            s = F.solve(b.data(), x.data(), p->tol, p->max_iter, graph, p->poll_every, nullptr, stall_window,
-                        &stalled);
-          }
-          if (stalled) {
-            res->fallback = 1;
-            if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
-            const int used = s.it;
-            FusedBiCG<double> J(ctx, n, rb, invd64, PrecondSpec{});
-            s = J.solve(b.data(), x.data(), p->tol, std::max(0, p->max_iter - used), graph, p->poll_every,
-                        nullptr);
-            s.it += used;
-          }
-          res->solve_s = tm.stop();
-          res->iterations = s.it;
-          res->status = s.status;
-          res->error = s.breakdown ? 1 : 0; // NullValueException
-          double bb = alg.dot(b, b);
-          res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
-          res->true_rel_res = true_residual(alg, A, b, x);
-          x.copyValuesTo(x_io);
-          ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-          return 0;
-        }
-        CGState s;
-        {
-          FusedCG<double> F(ctx, n, rb, invd64, ps);
-          s = F.solve(b.data(), x.data(), false, p->tol, p->max_iter, graph, p->poll_every, nullptr, stall_window,
-                      &stalled);
-          if (!stalled) finish_breakdown(F, s, &res->error);
-        }
-        if (stalled) {
-          // Jacobi from the last finite iterate, for the remaining iterations
-          res->fallback = 1;
-          if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
-          const int used = s.it;
-          FusedCG<double> J(ctx, n, rb, invd64, PrecondSpec{});
-          s = J.solve(b.data(), x.data(), false, p->tol, std::max(0, p->max_iter - used), graph, p->poll_every,
-                      nullptr);
-          finish_breakdown(J, s, &res->error);
-          s.it += used;
-        }
-        res->solve_s = tm.stop();
-        res->iterations = s.it;
-        res->status = s.status;
-        double bb = alg.dot(b, b);
-        res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
-      } else {
-        // FP32 matrix, preconditioner and Krylov vectors, FP64 dots; FP64 defect correction
-        A.ensureRB32();
-        phase("FP32 matrix");
-        DeviceArray<float> invd32(n);
-        launch_demote(ctx->stream, n, invd64, invd32.data());
-        RBMat<float> rb32 = A.rb32();
-        RBMat<double> rb64 = A.rb64();
-        res->setup_s = tm.stop();
-        tm.start();
-        const double delta = p->inner_tol > 0 ? p->inner_tol : 0.1;
-        auto run = bicg ? mixed_reliable_bicg : mixed_reliable_cg;
-        MixedResult m = run(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol, p->max_iter, delta,
-                            p->poll_every, true, stall_window);
-        if (m.stalled) {
-          // Jacobi from the last reliable update, for the remaining iterations
-          res->fallback = 1;
-          const int used = m.it, ups = m.updates;
-          m = run(ctx, n, rb64, rb32, invd32.data(), PrecondSpec{}, b.data(), x.data(), p->tol,
-                  std::max(0, p->max_iter - used), delta, p->poll_every, true, 0);
-          m.it += used;
-          m.updates += ups;
-        }
-        if (m.fp32_stalled) {
-          // FP64 Jacobi from the last reliable update, for the remaining iterations (a run
-          // with the learned M reports a stall instead, so Jacobi is the inner preconditioner)
-          res->fallback |= 2;
-          const int used = m.it;
-          const int left = std::max(0, p->max_iter - used);
-          if (bicg) {
-            FusedBiCG<double> F(ctx, n, rb64, invd64, PrecondSpec{});
-            const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, true, p->poll_every, nullptr);
-            m.it = used + s.it;
-            m.status = s.status;
-            m.error = s.breakdown ? 1 : 0;
-            m.rel = std::sqrt(s.rr / alg.dot(b, b));
-          } else {
-            FusedCG<double> F(ctx, n, rb64, invd64, PrecondSpec{});
-            CGState s = F.solve(b.data(), x.data(), false, p->tol, left, true, p->poll_every, nullptr);
-            finish_breakdown(F, s, &m.error);
-            m.it = used + s.it;
-            m.status = s.status;
-            m.rel = std::sqrt(s.rr / alg.dot(b, b));
-          }
-        }
-        res->solve_s = tm.stop();
-        res->iterations = m.it;
-        res->outer = m.updates;
-        res->status = m.status;
-        res->error = m.error;
-        res->value = m.rel;
-      }
-    }
-    res->true_rel_res = true_residual(alg, A, b, x);
-    x.copyValuesTo(x_io);
+    solve_on(ctx, p, A, b_in, x_io, res, tm, phase);
     if (!perm.empty())
       for (int r = 0; r < n; ++r) xh[perm[r]] = px[r];
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what());
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Converted matrix kept between solves (Alien's MultiMatrixImpl keeps the hip
+// matrix and re-converts on a new timestamp: MultiMatrixImpl.cc:226-248). The
+// structure (device CSR, row-bundled layouts) is built once; new values are
+// uploaded and refilled into the kept layouts.
+// ---------------------------------------------------------------------------
+struct alien_hip_matrix {
+  alien_hip_ctx* ctx = nullptr;
+  alien_hip::HIPMatrix A;
+  bool diag_ok = false; // every row stores its diagonal entry
+};
+
+extern "C" int alien_hip_matrix_create(alien_hip_ctx* ctx, int n, const int* ro, const int* cols,
+                                       const double* vals, alien_hip_matrix** out) {
+  if (!ctx || !out || !vals) return fail(ALIEN_HIP_EINVAL, "matrix_create: NULL argument");
+  *out = nullptr;
+  try {
+    validate_csr(n, ro, cols, false);
+    std::unique_ptr<alien_hip_matrix> m(new alien_hip_matrix);
+    m->ctx = ctx;
+    Alien::CSRHostView view{n, ro, cols, vals};
+    m->A.build(ctx, view, ALIEN_HIP_LAYOUT_CSR, /*borrow_host=*/false);
+    const int* d = m->A.dcol();
+    m->diag_ok = true;
+    for (int i = 0; i < n && m->diag_ok; ++i) m->diag_ok = d[i] < ro[i + 1] && cols[d[i]] == i;
+    *out = m.release();
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what());
+  }
+  return 0;
+}
+
+extern "C" int alien_hip_matrix_update_values(alien_hip_matrix* m, const double* vals) {
+  if (!m || (!vals && m->A.getNnz() > 0)) return fail(ALIEN_HIP_EINVAL, "matrix_update_values: NULL argument");
+  try {
+    m->A.updateValues(vals);
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(ALIEN_HIP_EINVAL, e.what());
+  }
+  return 0;
+}
+
+extern "C" int alien_hip_matrix_destroy(alien_hip_matrix* m) {
+  delete m;
+  return 0;
+}
+
+extern "C" int alien_hip_solve(alien_hip_ctx* ctx, const alien_hip_solve_params* p, alien_hip_matrix* m,
+                               const double* bh, double* xh, alien_hip_solve_result* res) {
+  if (!ctx || !p || !m || !res || !bh || !xh) return fail(ALIEN_HIP_EINVAL, "solve: NULL argument");
+  std::memset(res, 0, sizeof(*res));
+  try {
+    validate_params(p);
+    if (p->precond != ALIEN_HIP_PREC_NONE && !m->diag_ok)
+      throw Error(ALIEN_HIP_EINVAL, "CSR: missing diagonal entry");
+    if (p->precond == ALIEN_HIP_PREC_ILU0 && p->ilu_schedule == 1)
+      throw Error(ALIEN_HIP_EINVAL, "solve: multicolour ILU0 permutes the system; use alien_hip_solve_csr");
+    const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
+    const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
+    HIPMatrix& A = m->A;
+    ctx->reserve_for(A.getNRows());
+    EventTimer tm(ctx->stream);
+    tm.start();
+    SetupTrace phase(ctx);
+    if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
+    else A.setSpmvLayout(p->layout);
+    phase("row-bundled layout");
+    solve_on(ctx, p, A, bh, xh, res, tm, phase);
     ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   } catch (const Error& e) {
     return fail(e.code, e.what());

@@ -656,6 +656,27 @@ def main():
         if prec == aa.PREC_LEARNED:
             results[name]["fallback"] = r["fallback"]
 
+    # the matrix converted once and kept (a time-stepping caller's case): new values on the same
+    # structure, then the learned and the Jacobi mixed solves; their set-up no longer includes the
+    # conversion
+    kept = None
+    if "learned_mixed" in results:
+        with aa.Matrix(ctx, ro, cols, vals) as M:
+            M.solve(b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_MIXED, tol=1e-8, max_iter=20000,
+                    poll_every=16)  # builds the row-bundled layouts once
+            t0 = time.perf_counter()
+            M.update_values(vals)
+            t_upd = time.perf_counter() - t0
+            _, rl = M.solve(b, solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, tol=1e-8, max_iter=20000,
+                            poll_every=16, inner_tol=0.1, gnn_theta=theta)
+            _, rj = M.solve(b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_MIXED, tol=1e-8, max_iter=20000,
+                            poll_every=16, inner_tol=0.1)
+        kept = {"update_values_s": t_upd, "learned_mixed": {"setup_s": rl["setup_s"], "time_s": rl["solve_s"],
+                                                             "iterations": rl["iterations"]},
+                "mixed_jacobi": {"setup_s": rj["setup_s"], "time_s": rj["solve_s"], "iterations": rj["iterations"]}}
+        log(f"kept matrix: update_values {t_upd * 1e3:.1f} ms, learned set-up {rl['setup_s'] * 1e3:.1f} ms, "
+            f"jacobi set-up {rj['setup_s'] * 1e3:.1f} ms")
+
     log("timing SpMV kernels")
     sp = time_spmv(aa, torch, ctx, ro, cols, vals, a.spmv_reps)
     csr64 = spmv_bytes_fp64(N, nnz)
@@ -699,6 +720,7 @@ def main():
         "solvers": results,
         "speedup_learned_mixed": speedups,
         "training": training,
+        "kept_matrix": kept,
         "spmv": spmv,
         "roofline": {"bound": "hbm", "achieved": spmv[ROOF]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv[ROOF]["gbs"] / HBM_PEAK_GBS, "traffic": TRAFFIC.get(ROOF) if n == 256 else None,

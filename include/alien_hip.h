@@ -326,6 +326,24 @@ typedef struct {
 
 /* Host CSR in, host x out (x holds the initial guess on entry). Uploads,
  * converts to the requested layout, runs, downloads x. Synchronous. */
+/* A converted matrix kept between solves, as Alien's MultiMatrixImpl keeps the
+ * hip matrix and converts again only on a new timestamp (MultiMatrixImpl.cc:226-248):
+ * matrix_create uploads a host CSR (ascending columns; the arrays are copied and
+ * may be freed on return); matrix_update_values replaces the values on the same
+ * structure (host array of nnz values) and refills every layout already built;
+ * alien_hip_solve runs alien_hip_solve_csr's solve on it (row-bundled layouts are
+ * built at the first solve that needs them and kept). Multicolour ILU0
+ * (ilu_schedule = 1) permutes the system and is refused here (EINVAL): use
+ * alien_hip_solve_csr. A matrix is used with contexts of the device it was
+ * created on, one solve at a time. */
+typedef struct alien_hip_matrix alien_hip_matrix;
+int alien_hip_matrix_create(alien_hip_ctx* ctx, int nrows, const int* row_offset_host, const int* cols_host,
+                            const double* vals_host, alien_hip_matrix** out);
+int alien_hip_matrix_update_values(alien_hip_matrix* m, const double* vals_host);
+int alien_hip_matrix_destroy(alien_hip_matrix* m);
+int alien_hip_solve(alien_hip_ctx* ctx, const alien_hip_solve_params* p, alien_hip_matrix* m,
+                    const double* b_host, double* x_host, alien_hip_solve_result* result);
+
 int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_params* p, int nrows,
                         const int* row_offset, const int* cols, const double* vals,
                         const double* b, double* x, alien_hip_solve_result* res);

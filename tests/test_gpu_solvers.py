@@ -337,3 +337,64 @@ def test_cg_neumann_matches_oracle(ctx, layout):
     _, r0 = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_NEUMANN, layout=layout, tol=1e-8,
                          max_iter=1000, poly_factor=0.0, poly_order=3, poly_max_iter=20)
     assert r0["status"] == 1
+
+
+KEPT_CASES = [
+    ("cg jacobi graph", dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
+    ("cg learned mixed", dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, learned=True)),
+    ("cg chebyshev reference bellpack", dict(solver=aa.CG, precond=aa.PREC_CHEB, mode=aa.MODE_REFERENCE,
+                                             layout=aa.LAYOUT_BELLPACK, max_iter=3000)),
+    ("bicgstab jacobi fused", dict(solver=aa.BICGS, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED)),
+    ("bicgstab ilu0 reference", dict(solver=aa.BICGS, precond=aa.PREC_ILU0, mode=aa.MODE_REFERENCE)),
+]
+
+
+@pytest.mark.parametrize("name,kw", KEPT_CASES, ids=[c[0] for c in KEPT_CASES])
+def test_kept_matrix_solves_equal_solve_csr(ctx, name, kw):
+    """A matrix converted once (alien_hip_matrix_create) and solved twice, the
+    second time after new values on the same structure (D A D, D random positive
+    diagonal: still SPD), gives the iterates of alien_hip_solve_csr on the same
+    values bit for bit (Chebyshev, the reference's preconditioner, does not
+    converge on this system: both stop at max_iter with the same x)."""
+    from oracle import gnn_oracle as go
+    kw = dict(kw)
+    if kw.pop("learned", False):
+        kw["gnn_theta"] = go.init_params(0)
+    kw.setdefault("max_iter", 2000)
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 14, params=[1.5, 1, 1, 1], seed=6)
+    n = len(ro) - 1
+    b = aa.uniform01(8, n)
+    rows = np.repeat(np.arange(n), np.diff(ro))
+    d = 0.5 + np.random.default_rng(2).random(n)
+    vals2 = vals * d[rows] * d[cols]
+    say("case kept matrix", name)
+    with aa.Matrix(ctx, ro, cols, vals) as M:
+        for v in (vals, vals2):
+            if v is vals2:
+                M.update_values(v)
+            x1, r1 = M.solve(b, tol=1e-8, **kw)
+            x2, r2 = aa.solve_csr(ctx, ro, cols, v, b, tol=1e-8, **kw)
+            say("  ", r1["iterations"], r2["iterations"], r1["status"])
+            assert r1["status"] == r2["status"]
+            assert r1["iterations"] == r2["iterations"]
+            np.testing.assert_array_equal(x1, x2)
+
+
+def test_kept_matrix_refuses_what_it_cannot_run(ctx):
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 6)
+    b = np.ones(len(ro) - 1)
+    with aa.Matrix(ctx, ro, cols, vals) as M:
+        with pytest.raises(aa.AlienHipError):
+            M.solve(b, solver=aa.CG, precond=aa.PREC_ILU0, ilu_schedule=1)
+        with pytest.raises(aa.AlienHipError):
+            M.solve(b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED, tol=-1.0)
+    # a row without its diagonal: Diag-preconditioned solves are refused, unpreconditioned ones run
+    ro2 = np.array([0, 1, 3], np.int32)
+    cols2 = np.array([1, 0, 1], np.int32)
+    vals2 = np.array([1.0, 1.0, 2.0])
+    with aa.Matrix(ctx, ro2, cols2, vals2) as M:
+        with pytest.raises(aa.AlienHipError):
+            M.solve(np.ones(2), solver=aa.CG, precond=aa.PREC_DIAG)
+        x, r = M.solve(np.ones(2), solver=aa.CG, precond=aa.PREC_NONE, tol=1e-10, max_iter=50)
+        assert r["status"] == 1 and r["iterations"] == 2
+        np.testing.assert_allclose(x, [-1.0, 1.0], rtol=1e-14)
