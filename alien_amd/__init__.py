@@ -6,7 +6,8 @@ binds its C ABI through ctypes (`alien_amd._capi`) for tests and benchmarks.
 """
 from ._capi import (  # noqa: F401
     BICGS, CG, EXPORTED, GEN_CONVDIFF7, GEN_LOGNORMAL7, GEN_POISSON7, LAYOUT_BELLPACK, LAYOUT_CSR, LAYOUT_RBCSR,
-    LAYOUT_RBDIA, MODE_FUSED, MODE_GRAPH, MODE_MIXED, MODE_REFERENCE, PREC_CHEB, PREC_DIAG, PREC_ILU0,
+    LAYOUT_RBDIA, MODE_FUSED, MODE_GRAPH, MODE_MIXED, MODE_REFERENCE, MODE_REFERENCE_FUTURE, PREC_CHEB, PREC_DIAG,
+    PREC_ILU0,
     PREC_FILU0, PREC_LEARNED, PREC_NEUMANN, PREC_NONE, AlienHipError, Context, bellpack_build, check, gen_bsr2,
     gen_csr, gen_nnz,
     Matrix, gnn_forward, gnn_init, gnn_loss_grad, gnn_nparams, gnn_train, group_by_level, host_color, host_dcol,

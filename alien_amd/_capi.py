@@ -26,7 +26,7 @@ GEN_POISSON7, GEN_CONVDIFF7, GEN_LOGNORMAL7 = 0, 1, 2
 CG, BICGS = 0, 1
 PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_LEARNED, PREC_FILU0, PREC_NEUMANN = 0, 1, 2, 3, 4, 5, 6
 LAYOUT_CSR, LAYOUT_BELLPACK, LAYOUT_RBCSR, LAYOUT_RBDIA = 0, 1, 2, 3
-MODE_REFERENCE, MODE_FUSED, MODE_GRAPH, MODE_MIXED = 0, 1, 2, 3
+MODE_REFERENCE, MODE_FUSED, MODE_GRAPH, MODE_MIXED, MODE_REFERENCE_FUTURE = 0, 1, 2, 3, 4
 EINVAL = -22
 
 

@@ -126,6 +126,50 @@ class CG {
     return 0;
   }
 
+  // The same iteration with the dot products taken as FutureType: both dots of
+  // the first step are launched before the first read-back (CG::solve2,
+  // CG.h:152-244). Same arithmetic as solve, same iterates.
+  template <typename PrecondT, typename IterT>
+  int solve2(PrecondT& M, IterT& it, const MatrixType& A, const VectorType& b, VectorType& x) {
+    typedef typename AlgebraType::FutureType FutureType;
+    if (it.nullRhs()) return 0;
+    ValueType rho = 0, rho1 = 0, alpha = 0;
+    FutureType frho1(rho1), falpha(alpha);
+    VectorType p, z, q, r;
+    m_alg.allocate(AlgebraType::resource(A), p, z, q, r);
+    m_alg.copy(b, r);
+    m_alg.mult(A, x, p);
+    m_alg.axpy(-1., p, r);
+    m_alg.exec(M, r, z);
+    m_alg.copy(z, p);
+    m_alg.mult(A, p, q);
+    m_alg.dot(r, p, frho1);
+    m_alg.dot(p, q, falpha);
+    if (falpha.get() == 0) return breakdown(it, r, p, z, q);
+    alpha = frho1.get() / alpha;
+    m_alg.axpy(alpha, p, x);
+    m_alg.axpy(-alpha, q, r);
+    rho = rho1;
+    ++it;
+    while (!it.stop(r)) {
+      m_alg.exec(M, r, z);
+      m_alg.dot(r, z, frho1);
+      alpha = frho1.get() / rho;
+      m_alg.axpy(alpha, p, z);
+      m_alg.copy(z, p);
+      m_alg.mult(A, p, q);
+      m_alg.dot(p, q, falpha);
+      if (falpha.get() == 0) return breakdown(it, r, p, z, q);
+      alpha = rho1 / alpha;
+      m_alg.axpy(alpha, p, x);
+      m_alg.axpy(-alpha, q, r);
+      rho = rho1;
+      ++it;
+    }
+    m_alg.free(p, z, q, r);
+    return 0;
+  }
+
  private:
   template <typename IterT>
   int breakdown(IterT& it, VectorType& r, VectorType& p, VectorType& z, VectorType& q) {
@@ -234,6 +278,102 @@ class BiCGStab {
         throw NullValue("beta");
       }
       omega = omega / beta;
+      m_alg.axpy(omega, shat, x);
+      m_alg.axpy(alpha, phat, x);
+      m_alg.copy(s, r);
+      m_alg.axpy(-omega, t, r);
+      rho = rho1;
+      ++it;
+    }
+    return 0;
+  }
+
+  // The same iteration with FutureType dots (BiCGStab::solve2, BiCGStab.h:302-457):
+  // t.s and t.t are launched together, one read-back each when needed. Same
+  // arithmetic and exits as solve.
+  template <typename PrecondT, typename IterT>
+  int solve2(PrecondT& M, IterT& it, const MatrixType& A, const VectorType& b, VectorType& x) {
+    typedef typename AlgebraType::FutureType FutureType;
+    typedef typename AlgebraType::NullValueException NullValue;
+    if (it.nullRhs()) return 0;
+    ValueType rho = 0, rho1 = 0, alpha = 0, beta = 0, omega = 0;
+    FutureType frho1(rho1), falpha(alpha), fbeta(beta), fomega(omega);
+    VectorType p, phat, s, shat, t, v, r, r0;
+    m_alg.allocate(AlgebraType::resource(A), p, phat, s, shat, t, v, r, r0);
+    struct Release {
+      AlgebraType& a;
+      VectorType &p, &phat, &s, &shat, &t, &v, &r, &r0;
+      ~Release() { a.free(p, phat, s, shat, t, v, r, r0); }
+    } release{m_alg, p, phat, s, shat, t, v, r, r0};
+
+    m_alg.copy(b, r);
+    m_alg.mult(A, x, r0);
+    m_alg.axpy(-1., r0, r);
+    m_alg.copy(r, r0);
+    m_alg.copy(r, p);
+    m_alg.dot(r, r0, frho1);
+    m_alg.exec(M, p, phat);
+    m_alg.mult(A, phat, v);
+    m_alg.dot(v, r0, falpha);
+    if (falpha.get() == 0) throw NullValue("alpha");
+    alpha = frho1.get() / alpha;
+    m_alg.copy(r, s);
+    m_alg.axpy(-alpha, v, s);
+    if (it.stop(s)) {
+      ++it;
+      m_alg.axpy(alpha, phat, x);
+      return 0;
+    }
+    m_alg.exec(M, s, shat);
+    m_alg.mult(A, shat, t);
+    m_alg.dot(t, s, fomega);
+    m_alg.dot(t, t, fbeta);
+    if (fbeta.get() == 0) {
+      if (it.stop(r)) {
+        ++it;
+        m_alg.axpy(alpha, phat, x);
+        return 0;
+      }
+      throw NullValue("beta");
+    }
+    omega = fomega.get() / beta;
+    m_alg.axpy(omega, shat, x);
+    m_alg.axpy(alpha, phat, x);
+    m_alg.copy(s, r);
+    m_alg.axpy(-omega, t, r);
+    rho = rho1;
+    ++it;
+
+    while (!it.stop(r)) {
+      m_alg.dot(r, r0, frho1);
+      beta = (frho1.get() / rho) * (alpha / omega);
+      m_alg.axpy(-omega, v, p);
+      m_alg.scal(beta, p);
+      m_alg.axpy(1., r, p);
+      if (rho == 0) throw NullValue("rho");
+      m_alg.exec(M, p, phat);
+      m_alg.mult(A, phat, v);
+      m_alg.dot(v, r0, falpha);
+      if (falpha.get() == 0) throw NullValue("alpha");
+      alpha = rho1 / alpha;
+      m_alg.copy(r, s);
+      m_alg.axpy(-alpha, v, s);
+      if (it.stop(s)) {
+        m_alg.axpy(alpha, phat, x);
+        return 0;
+      }
+      m_alg.exec(M, s, shat);
+      m_alg.mult(A, shat, t);
+      m_alg.dot(t, s, fomega);
+      m_alg.dot(t, t, fbeta);
+      if (fbeta.get() == 0) {
+        if (it.stop(s)) {
+          m_alg.axpy(alpha, phat, x);
+          return 0;
+        }
+        throw NullValue("beta");
+      }
+      omega = fomega.get() / beta;
       m_alg.axpy(omega, shat, x);
       m_alg.axpy(alpha, phat, x);
       m_alg.copy(s, r);

@@ -124,12 +124,15 @@ void run_reference(Alg& alg, const alien_hip_solve_params* p, Precond& P, const 
   Alien::Iteration<Alg> it(alg, b, p->tol, p->max_iter);
   tm.start();
   try {
+    const bool future = p->mode == ALIEN_HIP_MODE_REFERENCE_FUTURE;
     if (p->solver == ALIEN_HIP_CG) {
       Alien::CG<Alg> s(alg);
-      s.solve(P, it, A, b, x);
+      if (future) s.solve2(P, it, A, b, x);
+      else s.solve(P, it, A, b, x);
     } else {
       Alien::BiCGStab<Alg> s(alg);
-      s.solve(P, it, A, b, x);
+      if (future) s.solve2(P, it, A, b, x);
+      else s.solve(P, it, A, b, x);
     }
   } catch (const Alg::NullValueException&) {
     res->error = 1;
@@ -773,6 +776,9 @@ namespace {
 
 // Parameter checks shared by the solve entry points.
 void validate_params(const alien_hip_solve_params* p) {
+  if (p->mode < ALIEN_HIP_MODE_REFERENCE || p->mode > ALIEN_HIP_MODE_REFERENCE_FUTURE)
+    throw Error(ALIEN_HIP_EINVAL, "solve: unknown mode");
+  if (p->solver != ALIEN_HIP_CG && p->solver != ALIEN_HIP_BICGS) throw Error(ALIEN_HIP_EINVAL, "solve: unknown solver");
   const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
   const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
   const bool learned = p->precond == ALIEN_HIP_PREC_LEARNED;

@@ -278,7 +278,9 @@ enum {
   ALIEN_HIP_MODE_REFERENCE = 0, /* the reference call sequence, one launch per algebra call */
   ALIEN_HIP_MODE_FUSED = 1,     /* fused kernels, device-resident scalars, host polls stop test */
   ALIEN_HIP_MODE_GRAPH = 2,     /* fused iteration captured in a hipGraph, batched stop polls */
-  ALIEN_HIP_MODE_MIXED = 3      /* FP32 inner solver inside FP64 iterative refinement */
+  ALIEN_HIP_MODE_MIXED = 3,     /* FP32 inner solver inside FP64 iterative refinement */
+  ALIEN_HIP_MODE_REFERENCE_FUTURE = 4 /* the templates' solve2 (CG.h:152-244, BiCGStab.h:302-457):
+                                         the REFERENCE sequence with FutureType dots */
 };
 
 typedef struct {

@@ -398,3 +398,29 @@ def test_kept_matrix_refuses_what_it_cannot_run(ctx):
         x, r = M.solve(np.ones(2), solver=aa.CG, precond=aa.PREC_NONE, tol=1e-10, max_iter=50)
         assert r["status"] == 1 and r["iterations"] == 2
         np.testing.assert_allclose(x, [-1.0, 1.0], rtol=1e-14)
+
+
+@pytest.mark.parametrize("solver,precond,case", [
+    (aa.CG, aa.PREC_CHEB, "poisson"), (aa.CG, aa.PREC_DIAG, "lognormal"),
+    (aa.BICGS, aa.PREC_DIAG, "convdiff"), (aa.BICGS, aa.PREC_ILU0, "convdiff")],
+    ids=["cg-cheb", "cg-diag", "bicgstab-diag", "bicgstab-ilu0"])
+def test_solve2_future_variant_equals_solve(ctx, solver, precond, case):
+    """CG::solve2 / BiCGStab::solve2 (FutureType dots, the dots of a step launched
+    before their read-back) run the same arithmetic as solve: same iterations,
+    same x bit for bit."""
+    if case == "poisson":
+        ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 16)
+    elif case == "lognormal":
+        ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 14, params=[1.5, 1, 1, 1], seed=6)
+    else:
+        ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 14)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case solve2", solver, precond, case)
+    kw = dict(solver=solver, precond=precond, tol=1e-8, max_iter=3000)
+    x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, **kw)
+    x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE_FUTURE, **kw)
+    say("  ", r1["iterations"], r2["iterations"], r1["status"])
+    assert r1["status"] == 1
+    assert (r1["iterations"], r1["status"], r1["error"]) == (r2["iterations"], r2["status"], r2["error"])
+    np.testing.assert_array_equal(x1, x2)
