@@ -106,9 +106,13 @@ _SIGS = [
                                            c_void_p]),
     ("alien_hip_rbdia_count", C.c_int64, [C.c_int, c_int_p, c_int_p, C.c_int, c_int_p]),
     ("alien_hip_rbdia_struct", C.c_int, [C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, C.POINTER(C.c_uint64)]),
-    ("alien_hip_rbdia_count_device", C.c_int64, [c_void_p, C.c_int, c_void_p, c_void_p, C.c_int, c_int_p]),
-    ("alien_hip_rbdia_struct_device", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+    ("alien_hip_rbdia_count_device", C.c_int64, [c_void_p, C.c_int, c_void_p, c_void_p, C.c_int, C.c_int, c_int_p]),
+    ("alien_hip_rbdia_struct_device", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, C.c_int, c_void_p, c_void_p,
                                                 c_void_p]),
+    ("alien_hip_rbdia2_fill_f64", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 6),
+    ("alien_hip_rbdia2_fill_f32", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 6),
+    ("alien_hip_spmv_rbdia2_f64", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 6),
+    ("alien_hip_spmv_rbdia2_f32", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 6),
     ("alien_hip_rbdia_fill_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p]),
     ("alien_hip_rbdia_fill_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -202,16 +202,14 @@ class HIPMatrix {
     m_host_vals_stale = true;
     if (!d_bv.empty()) {
       if (m_rbdia)
-        alien_hip::launch_rbdia_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
-                                             d_doff.data(), d_bv.data());
+        fillDIA<double>(d_bv.data());
       else
         alien_hip::launch_rbcsr_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
                                              d_bc.data(), d_bv.data());
     }
     if (!d_bv32.empty()) {
       if (m_rbdia)
-        alien_hip::launch_rbdia_fill<float>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
-                                            d_doff.data(), d_bv32.data());
+        fillDIA<float>(d_bv32.data());
       else
         alien_hip::launch_rbcsr_fill<float>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
                                             d_bc.data(), d_bv32.data());
@@ -254,42 +252,66 @@ class HIPMatrix {
       const long slots = alien_hip::rbdia_count_device(st, m_n, d_ro.data(), d_cols.data(),
                                                        alien_hip::kRbdiaMaxWidth, m_bo);
       if (slots >= 0 && (!csr_ok || slots <= alien_hip::kRbdiaMaxGrowth * csr_bo[nb])) {
-        d_bo.upload(m_bo.data(), m_bo.size(), st);
-        d_doff.resize(slots);
-        d_dmask.resize(slots);
-        alien_hip::launch_rbdia_struct(st, m_n, d_ro.data(), d_cols.data(), d_bo.data(), d_doff.data(),
-                                       d_dmask.data());
-        alien_hip::check_launch("rbdia_struct");
-        d_bv.resize((size_t)slots * alien_hip::kBundle);
-        alien_hip::launch_rbdia_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
-                                             d_doff.data(), d_bv.data());
-        alien_hip::check_launch("rbdia_fill");
-        m_rbdia = true;
+        buildDIA(slots, false);
+        return;
+      }
+      // rows of column pairs (two unknowns per cell, interleaved): one slot per pair
+      const long pslots = alien_hip::rbdia_count_device(st, m_n, d_ro.data(), d_cols.data(),
+                                                        alien_hip::kRbdiaMaxWidth, m_bo, true);
+      if (pslots >= 0 && (!csr_ok || 2 * pslots <= alien_hip::kRbdiaMaxGrowth * csr_bo[nb])) {
+        buildDIA(pslots, true);
         return;
       }
     }
     ensureRBCSR();
+  }
+  // RB-DIA from the bundle offsets in m_bo (slots per lane; pair mode: pairs per lane)
+  void buildDIA(long slots, bool pair) {
+    hipStream_t st = m_ctx->stream;
+    d_bo.upload(m_bo.data(), m_bo.size(), st);
+    d_doff.resize(slots);
+    d_dmask.resize(slots);
+    alien_hip::launch_rbdia_struct(st, m_n, d_ro.data(), d_cols.data(), d_bo.data(), d_doff.data(), d_dmask.data(),
+                                   pair);
+    alien_hip::check_launch("rbdia_struct");
+    m_rbdia = true;
+    m_pair = pair;
+    d_bv.resize((size_t)slots * alien_hip::kBundle * (pair ? 2 : 1));
+    fillDIA<double>(d_bv.data());
+  }
+  template <typename T>
+  void fillDIA(T* bv) {
+    hipStream_t st = m_ctx->stream;
+    if (m_pair)
+      alien_hip::launch_rbdia2_fill<T>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
+                                       d_doff.data(), bv);
+    else
+      alien_hip::launch_rbdia_fill<T>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
+                                      d_doff.data(), bv);
+    alien_hip::check_launch("rbdia_fill");
   }
   void ensureRB32() {
     if (d_bo.empty()) ensureRB(true);
     if (!d_bv32.empty()) return;
     d_bv32.resize(d_bv.size());
     if (m_rbdia)
-      alien_hip::launch_rbdia_fill<float>(m_ctx->stream, m_n, d_ro.data(), d_cols.data(), d_vals.data(),
-                                          d_bo.data(), d_doff.data(), d_bv32.data());
+      fillDIA<float>(d_bv32.data());
     else
       alien_hip::launch_rbcsr_fill<float>(m_ctx->stream, m_n, d_ro.data(), d_cols.data(), d_vals.data(),
                                           d_bo.data(), d_bc.data(), d_bv32.data());
     alien_hip::check_launch("row-bundled fill<float>");
   }
   bool isRBDIA() const { return m_rbdia; }
+  bool isPairDIA() const { return m_rbdia && m_pair; }
   alien_hip::RBMat<double> rb64() const {
     return alien_hip::RBMat<double>{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv.data(),
-                                    m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr};
+                                    m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr,
+                                    m_pair ? 1 : 0};
   }
   alien_hip::RBMat<float> rb32() const {
     return alien_hip::RBMat<float>{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv32.data(),
-                                   m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr};
+                                   m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr,
+                                   m_pair ? 1 : 0};
   }
   void freeRB() {
     d_bo.reset();
@@ -298,6 +320,7 @@ class HIPMatrix {
     d_bv32.reset();
     d_doff.reset();
     d_dmask.reset();
+    m_pair = false;
     m_rbdia = false;
   }
 
@@ -362,6 +385,7 @@ class HIPMatrix {
  private:
   alien_hip_ctx* m_ctx = nullptr;
   bool m_rbdia = false;
+  bool m_pair = false; // RB-DIA in pair mode
   int m_n = 0, m_nnz = 0;
   int m_layout = kLayoutCSR;
   std::vector<int> m_kcol, m_cols, m_dcol, m_bo, m_bro;

@@ -228,12 +228,12 @@ int alien_hip_rbdia_struct(int nrows, const int* ro, const int* cols, int max_w,
 }
 
 int64_t alien_hip_rbdia_count_device(alien_hip_ctx* ctx, int nrows, const int* ro, const int* cols, int max_w,
-                                     int* bo) {
+                                     int pair, int* bo) {
   if (!(ctx && nrows >= 0 && (nrows == 0 || (ro && cols)) && bo && max_w > 0))
     return fail(ALIEN_HIP_EINVAL, "rbdia_count_device: bad arguments"), -1;
   try {
     std::vector<int> b;
-    const long slots = rbdia_count_device(ctx->stream, nrows, ro, cols, max_w, b);
+    const long slots = rbdia_count_device(ctx->stream, nrows, ro, cols, max_w, b, pair != 0);
     if (slots < 0)
       return fail(ALIEN_HIP_EINVAL, "rbdia_count_device: too many offsets in a bundle or columns not ascending"), -1;
     std::copy(b.begin(), b.end(), bo);
@@ -245,12 +245,13 @@ int64_t alien_hip_rbdia_count_device(alien_hip_ctx* ctx, int nrows, const int* r
   }
 }
 
-int alien_hip_rbdia_struct_device(alien_hip_ctx* ctx, int nrows, const int* ro, const int* cols, const int* bo,
-                                  int* doff, uint64_t* dmask) {
+int alien_hip_rbdia_struct_device(alien_hip_ctx* ctx, int nrows, const int* ro, const int* cols, int pair,
+                                  const int* bo, int* doff, uint64_t* dmask) {
   REQUIRE(ctx, "ctx is NULL");
   REQUIRE(nrows >= 0 && (nrows == 0 || (ro && cols && bo && doff && dmask)), "rbdia_struct_device: bad arguments");
   API_BEGIN
-  launch_rbdia_struct(ctx->stream, nrows, ro, cols, bo, doff, reinterpret_cast<unsigned long long*>(dmask));
+  launch_rbdia_struct(ctx->stream, nrows, ro, cols, bo, doff, reinterpret_cast<unsigned long long*>(dmask),
+                      pair != 0);
   LAUNCHED("rbdia_struct_device");
   API_END
 }
@@ -510,6 +511,48 @@ int alien_hip_rbdia_fill_f32(alien_hip_ctx* ctx, int n, const int* ro, const int
   API_BEGIN
   launch_rbdia_fill<float>(ctx->stream, n, ro, cols, vals, bo, doff, bv);
   LAUNCHED("rbdia_fill_f32");
+  API_END
+}
+
+int alien_hip_rbdia2_fill_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                              const int* bo, const int* doff, double* bv) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && vals && bo && doff && bv)), "rbdia2_fill_f64: bad arguments");
+  API_BEGIN
+  launch_rbdia2_fill<double>(ctx->stream, n, ro, cols, vals, bo, doff, bv);
+  LAUNCHED("rbdia2_fill_f64");
+  API_END
+}
+
+int alien_hip_rbdia2_fill_f32(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const double* vals,
+                              const int* bo, const int* doff, float* bv) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (ro && cols && vals && bo && doff && bv)), "rbdia2_fill_f32: bad arguments");
+  API_BEGIN
+  launch_rbdia2_fill<float>(ctx->stream, n, ro, cols, vals, bo, doff, bv);
+  LAUNCHED("rbdia2_fill_f32");
+  API_END
+}
+
+int alien_hip_spmv_rbdia2_f64(alien_hip_ctx* ctx, int n, const int* bo, const int* doff, const uint64_t* dmask,
+                              const double* bv, const double* x, double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (bo && doff && dmask && bv && x && y)), "spmv_rbdia2_f64: bad arguments");
+  API_BEGIN
+  RBMat<double> A{bo, nullptr, bv, doff, reinterpret_cast<const unsigned long long*>(dmask), 1};
+  launch_spmv_rb<double>(ctx->stream, n, A, x, y);
+  LAUNCHED("spmv_rbdia2_f64");
+  API_END
+}
+
+int alien_hip_spmv_rbdia2_f32(alien_hip_ctx* ctx, int n, const int* bo, const int* doff, const uint64_t* dmask,
+                              const float* bv, const float* x, float* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && (n == 0 || (bo && doff && dmask && bv && x && y)), "spmv_rbdia2_f32: bad arguments");
+  API_BEGIN
+  RBMat<float> A{bo, nullptr, bv, doff, reinterpret_cast<const unsigned long long*>(dmask), 1};
+  launch_spmv_rb<float>(ctx->stream, n, A, x, y);
+  LAUNCHED("spmv_rbdia2_f32");
   API_END
 }
 

@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_flag(int n, RBMat<T> A, c
   if (*done) return;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const T t = rb_sum_single<T>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) y[row] = t;
   }

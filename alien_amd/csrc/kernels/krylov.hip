@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_dot(int n, RBMat<T> A, co
   double s = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const T t = rb_sum_single<T>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       y[row] = t;
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_guarded(int n, RBMat<T> A
   if (st->done) return;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const T t = rb_sum_single<T>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) y[row] = t;
   }
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(kRowBlock) k_factor_t(int n, RBMat<T, V> GT, c
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T rv = r[min(row, n - 1)];
-    const T v = rb_sum<T>(GT, b, [&](int c) { return r[c]; });
+    const T v = rb_sum_single<T>(GT, b, [&](int c) { return r[c]; });
     if (row < n) {
       t[row] = v;
       srr += (double)rv * (double)rv;
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(kRowBlock) k_factor_p(int n, RBMat<T, V> G, co
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T pold = p[min(row, n - 1)];
-    const T z = rb_sum<T>(G, b, [&](int c) { return t[c]; });
+    const T z = rb_sum_single<T>(G, b, [&](int c) { return t[c]; });
     if (row < n) p[row] = first ? z : z + beta * pold;
   }
 }

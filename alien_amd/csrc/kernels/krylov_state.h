@@ -21,6 +21,10 @@ struct RBMat {
   const V* bv;   // values
   const int* doff = nullptr;                  // RB-DIA: column offset of each bundle slot
   const unsigned long long* dmask = nullptr;  // RB-DIA: rows of the bundle holding that slot
+  // RB-DIA of a matrix whose rows hold column pairs (c, c + 1), c even (two unknowns
+  // per cell, interleaved): a slot is one pair at c = (row & ~1) + doff, its two
+  // values in planes 2 s and 2 s + 1
+  int pair = 0;
 };
 
 // Scalars of one CG solve; lives in device memory, read by the host only to

@@ -53,9 +53,15 @@ void launch_rbdia_fill(hipStream_t, int n, const int* ro, const int* cols, const
 // ascending or the slots exceed int32 indexing); launch_rbdia_struct writes
 // the offsets and lane masks at device bundle offsets bo; rbdia_structure_device
 // does both and keeps the structure in the given device arrays.
-long rbdia_count_device(hipStream_t, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo);
+// pair: the structure of RBMat's pair mode (rows of column pairs (c, c + 1), c even)
+long rbdia_count_device(hipStream_t, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo,
+                        bool pair = false);
 void launch_rbdia_struct(hipStream_t, int n, const int* ro, const int* cols, const int* bo, int* doff,
-                         unsigned long long* dmask);
+                         unsigned long long* dmask, bool pair = false);
+// values of the pair mode (two planes per slot)
+template <typename T>
+void launch_rbdia2_fill(hipStream_t, int n, const int* ro, const int* cols, const double* vals, const int* bo,
+                        const int* doff, T* bv);
 long rbdia_structure_device(hipStream_t, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo,
                             DeviceArray<int>& d_bo, DeviceArray<int>& doff, DeviceArray<unsigned long long>& dmask);
 template <typename T>

@@ -89,16 +89,68 @@ __device__ __forceinline__ T rb_sum_u(const RBMat<T, V>& A, int b, int s0, int w
   return acc;
 }
 
+// Pair mode of RB-DIA (RBMat::pair): slot j of the bundle is the column pair
+// (base + d, base + d + 1), base = row & ~1, d = doff[s0 + j]; the row's two
+// values of the pair are taken in column order, as the CSR sum takes them.
+template <int U, typename T, typename V, typename XS>
+__device__ __forceinline__ T rb_sum_pair_u(const RBMat<T, V>& A, int b, int s0, int w, XS xs) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int base = (b * kWave + lane) & ~1;
+  const unsigned long long bit = 1ull << lane;
+  T acc = 0;
+  for (int j = 0; j < w; j += U) {
+    int d[U];
+    unsigned long long m[U];
+    T v0[U], v1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int sl = min(j + u, w - 1);
+      d[u] = A.doff[s0 + sl];
+      m[u] = A.dmask[s0 + sl];
+      const long e = (2L * (s0 + sl)) * kWave + lane;
+      v0[u] = widen<T>(__builtin_nontemporal_load(A.bv + e));
+      v1[u] = widen<T>(__builtin_nontemporal_load(A.bv + e + kWave));
+    }
+    bool on[U];
+    T x0[U], x1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      on[u] = (j + u < w) && (m[u] & bit) != 0;
+      x0[u] = on[u] ? xs(base + d[u]) : T(0);
+      x1[u] = on[u] ? xs(base + d[u] + 1) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (on[u]) {
+        acc = acc + v0[u] * x0[u];
+        acc = acc + v1[u] * x1[u];
+      }
+  }
+  return acc;
+}
+
 // Row sum of this lane's row of bundle b over the vector given by xs(column).
 // Short bundles (factors of M = G G^T: 4 slots at a 7-point stencil) use a
 // batch of 4, longer ones a batch of 7.
+// rb_sum_single: operands never in pair mode (the learned factors), so the
+// kernel carries no pair-mode code.
 template <typename T, typename V, typename XS>
-__device__ __forceinline__ T rb_sum(const RBMat<T, V>& A, int b, XS xs) {
+__device__ __forceinline__ T rb_sum_single(const RBMat<T, V>& A, int b, XS xs) {
   const int s0 = A.bo[b];
   const int w = A.bo[b + 1] - s0;
   if (w <= 0) return T(0);
   if (w <= 4) return rb_sum_u<4>(A, b, s0, w, xs);
   return rb_sum_u<7>(A, b, s0, w, xs);
+}
+template <typename T, typename V, typename XS>
+__device__ __forceinline__ T rb_sum(const RBMat<T, V>& A, int b, XS xs) {
+  if (A.pair) {
+    const int s0 = A.bo[b];
+    const int w = A.bo[b + 1] - s0;
+    if (w <= 0) return T(0);
+    return w <= 4 ? rb_sum_pair_u<4>(A, b, s0, w, xs) : rb_sum_pair_u<7>(A, b, s0, w, xs);
+  }
+  return rb_sum_single<T>(A, b, xs);
 }
 
 } // namespace alien_hip

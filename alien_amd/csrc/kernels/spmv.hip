@@ -125,13 +125,15 @@ __global__ void __launch_bounds__(kBlock) k_rbdia_fill(int n, const int* __restr
 // the host. COUNT pass: width[b] = number of distinct offsets, or -1 when it
 // exceeds max_w or a row's columns are not strictly ascending. WRITE pass:
 // offsets and masks at the bundle offsets bo of a successful COUNT pass.
+// PAIR: the structure of RBMat's pair mode; every row must hold column pairs
+// (c, c + 1) with c even, and a slot is a pair at offset c - (row & ~1).
 __device__ __forceinline__ int wave_min_i32(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
   return v;
 }
 
-template <bool WRITE>
+template <bool WRITE, bool PAIR>
 __global__ void __launch_bounds__(kBlock) k_rbdia_struct(int n, const int* __restrict__ ro,
                                                           const int* __restrict__ cols, int max_w,
                                                           const int* __restrict__ bo, int* __restrict__ width,
@@ -148,8 +150,12 @@ __global__ void __launch_bounds__(kBlock) k_rbdia_struct(int n, const int* __res
     k = ro[row];
     e = ro[row + 1];
   }
-  int cur = k < e ? cols[k] - row : kEnd;
-  bool bad = false;
+  constexpr int step = PAIR ? 2 : 1;
+  const int base = PAIR ? (row & ~1) : row;
+  // a pair starts at an even column and is followed by the next column
+  auto bad_pair = [&](int kk) { return PAIR && kk < e && (kk + 1 >= e || (cols[kk] & 1) || cols[kk + 1] != cols[kk] + 1); };
+  int cur = k < e ? cols[k] - base : kEnd;
+  bool bad = bad_pair(k);
   const int s0 = WRITE ? bo[b] : 0;
   const int cap = WRITE ? bo[b + 1] - s0 : max_w + 1;
   int w = 0;
@@ -164,15 +170,48 @@ __global__ void __launch_bounds__(kBlock) k_rbdia_struct(int n, const int* __res
     }
     ++w;
     if (hit) {
-      ++k;
-      const int nx = k < e ? cols[k] - row : kEnd;
-      bad |= nx <= m;
+      k += step;
+      const int nx = k < e ? cols[k] - base : kEnd;
+      bad |= nx <= m || bad_pair(k);
       cur = nx;
     }
   }
   if (!WRITE) {
     const bool any_bad = __ballot(bad) != 0ull;
     if (lane == 0) width[b] = (any_bad || w > max_w) ? -1 : w;
+  }
+}
+
+// Device conversion CSR -> RB-DIA pair mode: lane = row; a slot whose pair the
+// row holds takes its two values (planes 2 s, 2 s + 1), zeros elsewhere.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_rbdia2_fill(int n, const int* __restrict__ ro,
+                                                         const int* __restrict__ cols,
+                                                         const double* __restrict__ vals,
+                                                         const int* __restrict__ bo, const int* __restrict__ doff,
+                                                         T* __restrict__ bv) {
+  const int nb = (n + kBundle - 1) / kBundle;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int b = t / kBundle, lane = t % kBundle;
+  if (b >= nb) return;
+  const int row = t;
+  const int base = row & ~1;
+  const int s0 = bo[b], w = bo[b + 1] - s0;
+  int k = 0, e = 0;
+  if (row < n) {
+    k = ro[row];
+    e = ro[row + 1];
+  }
+  for (int j = 0; j < w; ++j) {
+    T v0 = T(0), v1 = T(0);
+    if (k + 1 < e && cols[k] - base == doff[s0 + j]) {
+      v0 = (T)vals[k];
+      v1 = (T)vals[k + 1];
+      k += 2;
+    }
+    const long s = (long)s0 + j;
+    bv[(2 * s) * kWave + lane] = v0;
+    bv[(2 * s + 1) * kWave + lane] = v1;
   }
 }
 
@@ -476,13 +515,18 @@ void launch_rbdia_fill(hipStream_t st, int n, const int* ro, const int* cols, co
                        const int* doff, T* bv) {
   if (n > 0) hipLaunchKernelGGL(k_rbdia_fill<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, bo, doff, bv);
 }
-long rbdia_count_device(hipStream_t st, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo) {
+long rbdia_count_device(hipStream_t st, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo,
+                        bool pair) {
   const int nb = (n + kBundle - 1) / kBundle;
   bo.assign(nb + 1, 0);
   if (nb == 0) return 0;
   DeviceArray<int> width(nb);
-  hipLaunchKernelGGL(k_rbdia_struct<false>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, max_w,
-                     (const int*)nullptr, width.data(), (int*)nullptr, (unsigned long long*)nullptr);
+  if (pair)
+    hipLaunchKernelGGL((k_rbdia_struct<false, true>), rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, max_w,
+                       (const int*)nullptr, width.data(), (int*)nullptr, (unsigned long long*)nullptr);
+  else
+    hipLaunchKernelGGL((k_rbdia_struct<false, false>), rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, max_w,
+                       (const int*)nullptr, width.data(), (int*)nullptr, (unsigned long long*)nullptr);
   check_launch("rbdia_struct count");
   ALIEN_HIP_CHECK(hipMemcpyAsync(bo.data(), width.data(), nb * sizeof(int), hipMemcpyDeviceToHost, st));
   ALIEN_HIP_CHECK(hipStreamSynchronize(st));
@@ -499,20 +543,29 @@ long rbdia_count_device(hipStream_t st, int n, const int* ro, const int* cols, i
 }
 
 void launch_rbdia_struct(hipStream_t st, int n, const int* ro, const int* cols, const int* bo, int* doff,
-                         unsigned long long* dmask) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_rbdia_struct<true>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, 0, bo,
+                         unsigned long long* dmask, bool pair) {
+  if (n <= 0) return;
+  if (pair)
+    hipLaunchKernelGGL((k_rbdia_struct<true, true>), rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, 0, bo,
                        (int*)nullptr, doff, dmask);
+  else
+    hipLaunchKernelGGL((k_rbdia_struct<true, false>), rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, 0, bo,
+                       (int*)nullptr, doff, dmask);
+}
+template <typename T>
+void launch_rbdia2_fill(hipStream_t st, int n, const int* ro, const int* cols, const double* vals, const int* bo,
+                        const int* doff, T* bv) {
+  if (n > 0) hipLaunchKernelGGL(k_rbdia2_fill<T>, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, vals, bo, doff, bv);
 }
 
 long rbdia_structure_device(hipStream_t st, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo,
                             DeviceArray<int>& d_bo, DeviceArray<int>& doff, DeviceArray<unsigned long long>& dmask) {
-  const long slots = rbdia_count_device(st, n, ro, cols, max_w, bo);
+  const long slots = rbdia_count_device(st, n, ro, cols, max_w, bo, false);
   if (slots < 0) return -1;
   d_bo.upload(bo.data(), bo.size(), st);
   doff.resize(slots);
   dmask.resize(slots);
-  launch_rbdia_struct(st, n, ro, cols, d_bo.data(), doff.data(), dmask.data());
+  launch_rbdia_struct(st, n, ro, cols, d_bo.data(), doff.data(), dmask.data(), false);
   check_launch("rbdia_struct write");
   return slots;
 }
@@ -582,6 +635,8 @@ void launch_invdiag_bellpack(hipStream_t st, int n, int slice, const int* bro, c
   template void launch_spmv_rb<T>(hipStream_t, int, RBMat<T>, const T*, T*);                           \
   template void launch_rbdia_fill<T>(hipStream_t, int, const int*, const int*, const double*, const int*,\
                                      const int*, T*);                                                    \
+  template void launch_rbdia2_fill<T>(hipStream_t, int, const int*, const int*, const double*,           \
+                                      const int*, const int*, T*);                                       \
   template void launch_rbcsr_fill<T>(hipStream_t, int, const int*, const int*, const double*,           \
                                      const int*, int*, T*);                                             \
   template void launch_spmv_bellpack<T>(hipStream_t, int, int, const int*, const int*, const T*,        \

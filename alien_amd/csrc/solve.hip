@@ -918,7 +918,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       v.d_cols = A.d_cols.data();
       v.d_dcol = A.d_dcol.data();
       v.d_vals = A.d_vals.data();
-      if (A.isRBDIA()) {
+      if (A.isRBDIA() && !A.isPairDIA()) {
         v.rb_bo = A.bundleOffsets().data();
         v.d_rb_doff = A.d_doff.data();
         v.d_rb_dmask = A.d_dmask.data();

@@ -198,11 +198,23 @@ int alien_hip_rbdia_struct(int nrows, const int* row_offset_host, const int* col
  * bundle): rbdia_count_device fills the HOST array bundle_off and returns the
  * slot count, or -1 as rbdia_count; rbdia_struct_device writes doff and dmask
  * (device arrays of that many entries) given the device copy of bundle_off.
- * HIPMatrix's RB-DIA conversion runs these. */
+ * HIPMatrix's RB-DIA conversion runs these. pair != 0: the pair mode for rows
+ * made of column pairs (c, c + 1), c even (two unknowns per cell, interleaved;
+ * -1 when a row is not): a slot is one pair at column (row & ~1) + doff, its two
+ * values at bvals[64 (2 s) + l] and bvals[64 (2 s + 1) + l] (rbdia2_fill,
+ * spmv_rbdia2); same sums as CSR bit for bit. */
 int64_t alien_hip_rbdia_count_device(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
-                                     int max_width, int* bundle_off_host);
+                                     int max_width, int pair, int* bundle_off_host);
 int alien_hip_rbdia_struct_device(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
-                                  const int* bundle_off, int* doff, uint64_t* dmask);
+                                  int pair, const int* bundle_off, int* doff, uint64_t* dmask);
+int alien_hip_rbdia2_fill_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                              const double* vals, const int* bundle_off, const int* doff, double* bvals);
+int alien_hip_rbdia2_fill_f32(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
+                              const double* vals, const int* bundle_off, const int* doff, float* bvals);
+int alien_hip_spmv_rbdia2_f64(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* doff,
+                              const uint64_t* dmask, const double* bvals, const double* x, double* y);
+int alien_hip_spmv_rbdia2_f32(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* doff,
+                              const uint64_t* dmask, const float* bvals, const float* x, float* y);
 int alien_hip_rbdia_fill_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
                              const double* vals, const int* bundle_off, const int* doff, double* bvals);
 int alien_hip_rbdia_fill_f32(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,

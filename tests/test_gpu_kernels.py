@@ -112,17 +112,17 @@ def test_spmv_rbdia_bit_exact(ctx, name, A):
     np.testing.assert_array_equal(host(y32, ctx)[:n], host(yc, ctx)[:n])
 
 
-def rbdia_device(ctx, dro, dc, n, max_width):
+def rbdia_device(ctx, dro, dc, n, max_width, pair=0):
     """RB-DIA structure built on the device: (bo, doff, dmask) on the host, or None."""
     nb = (n + 63) // 64
     bo = np.zeros(nb + 1, dtype=np.int32)
     slots = aa.lib().alien_hip_rbdia_count_device(ctx.h, n, C.c_void_p(dro.data_ptr()), C.c_void_p(dc.data_ptr()),
-                                                  max_width, bo.ctypes.data_as(C.POINTER(C.c_int)))
+                                                  max_width, pair, bo.ctypes.data_as(C.POINTER(C.c_int)))
     if slots < 0:
         return None
     ddo = empty(max(slots, 1), torch.int32)
     ddm = empty(max(slots, 1), torch.int64)
-    call("alien_hip_rbdia_struct_device", ctx.h, n, p(dro), p(dc), p(dev(bo)), p(ddo), p(ddm))
+    call("alien_hip_rbdia_struct_device", ctx.h, n, p(dro), p(dc), pair, p(dev(bo)), p(ddo), p(ddm))
     return bo, host(ddo, ctx)[:slots], host(ddm, ctx)[:slots].view(np.uint64)
 
 
@@ -485,3 +485,67 @@ def test_filu0_factor_sweeps_bit_exact(ctx):
             call("alien_hip_filu0_sweep_f64", ctx.h, n, p(dro), p(dc), p(dd), p(da), p(prev), p(cur))
         np.testing.assert_array_equal(host(cur, ctx), orc.filu0_factor(ro, cols, vals, niter))
     np.testing.assert_array_equal(orc.filu0_factor(ro, cols, vals, nl), orc.ilu0_factor(ro, cols, vals))
+
+
+def pair_structure_brute(ro, cols):
+    """Pair-mode RB-DIA structure by brute force: per bundle of 64 rows the sorted
+    distinct offsets c - (r & ~1) of the rows' even columns, one mask bit per row."""
+    n = len(ro) - 1
+    bo, doff, dmask = [0], [], []
+    for b in range((n + 63) // 64):
+        offs = {}
+        for r in range(b * 64, min(n, b * 64 + 64)):
+            for k in range(ro[r], ro[r + 1], 2):
+                offs.setdefault(int(cols[k]) - (r & ~1), 0)
+                offs[int(cols[k]) - (r & ~1)] |= 1 << (r - b * 64)
+        for d in sorted(offs):
+            doff.append(d)
+            dmask.append(offs[d])
+        bo.append(len(doff))
+    return np.array(bo), np.array(doff), np.array(dmask, dtype=np.uint64)
+
+
+@pytest.mark.parametrize("n", [5, 12])
+def test_rbdia_pair_mode_bit_exact(ctx, n):
+    """Pair-mode RB-DIA (two unknowns per cell, interleaved): structure equal to
+    brute force, FP64 SpMV bit-exact with the sequential CSR product, FP32 with
+    the FP32 RB-CSR kernel; a matrix whose rows are not column pairs has none."""
+    import scipy.sparse as sp
+    bro, bcols, bvals = aa.gen_bsr2(n, params=[2.0, 0.3], seed=5)
+    nb = len(bro) - 1
+    blocks = bvals.reshape(-1, 2, 2).transpose(0, 2, 1)
+    A = sp.bsr_matrix((blocks, bcols, bro), shape=(2 * nb, 2 * nb)).tocsr()
+    A.sort_indices()
+    ro, cols, vals = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    N = len(ro) - 1
+    say("case rbdia pair", n, N)
+    dro, dc, dv = dev(ro), dev(cols), dev(vals)
+    got = rbdia_device(ctx, dro, dc, N, 64, pair=1)
+    assert got is not None
+    for g, e in zip(got, pair_structure_brute(ro, cols)):
+        np.testing.assert_array_equal(g.astype(np.int64) if g.dtype != np.uint64 else g, e)
+    bo, doff, dmask = got
+    slots = int(bo[-1]) * 64
+    dbo, ddo, ddm = dev(bo), dev(doff), dev(dmask.view(np.int64))
+    bv = empty(2 * slots)
+    call("alien_hip_rbdia2_fill_f64", ctx.h, N, p(dro), p(dc), p(dv), p(dbo), p(ddo), p(bv))
+    x = np.random.default_rng(3).standard_normal(N)
+    y = empty(N)
+    call("alien_hip_spmv_rbdia2_f64", ctx.h, N, p(dbo), p(ddo), p(ddm), p(bv), p(dev(x)), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:N], orc.spmv(ro, cols, vals, x))
+    bv32 = empty(2 * slots, torch.float32)
+    call("alien_hip_rbdia2_fill_f32", ctx.h, N, p(dro), p(dc), p(dv), p(dbo), p(ddo), p(bv32))
+    x32 = dev(x.astype(np.float32))
+    y32 = empty(N, torch.float32)
+    call("alien_hip_spmv_rbdia2_f32", ctx.h, N, p(dbo), p(ddo), p(ddm), p(bv32), p(x32), p(y32))
+    cbo = aa.rbcsr_bundle_offsets(ro)
+    cslots = int(cbo[-1]) * 64
+    dcbo = dev(cbo)
+    cbc = empty(cslots, torch.int32)
+    cbv = empty(cslots, torch.float32)
+    call("alien_hip_rbcsr_fill_f32", ctx.h, N, p(dro), p(dc), p(dv), p(dcbo), p(cbc), p(cbv))
+    yc = empty(N, torch.float32)
+    call("alien_hip_spmv_rbcsr_f32", ctx.h, N, p(dcbo), p(cbc), p(cbv), p(x32), p(yc))
+    np.testing.assert_array_equal(host(y32, ctx)[:N], host(yc, ctx)[:N])
+    pro, pcols, _ = aa.gen_csr(aa.GEN_POISSON7, 6)
+    assert rbdia_device(ctx, dev(pro), dev(pcols), len(pro) - 1, 64, pair=1) is None

@@ -161,6 +161,22 @@ def test_learned_factor_layout_split_and_host_built_identical(ctx, mode):
     np.testing.assert_array_equal(x1, x2)
 
 
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
+def test_two_unknown_system_pair_layout_solves_identical(ctx, mode):
+    """The 2-unknown system is converted to pair-mode RB-DIA by default (one slot
+    per column pair, no column stream); BiCGStab gives the iterates of RB-CSR bit
+    for bit."""
+    ro, cols, vals = bsr2_scalar_csr(10)
+    b = aa.uniform01(3, len(ro) - 1)
+    say("case pair layout vs rbcsr", mode)
+    kw = dict(solver=aa.BICGS, precond=aa.PREC_DIAG, mode=mode, tol=1e-10, max_iter=5000, poll_every=4)
+    x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBDIA, **kw)
+    x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBCSR, **kw)
+    assert r1["status"] == 1
+    assert r1["iterations"] == r2["iterations"]
+    np.testing.assert_array_equal(x1, x2)
+
+
 def test_unstructured_matrix_falls_back_to_rbcsr(ctx):
     """A random SPD matrix has too many distinct offsets per bundle for RB-DIA:
     the fused solver runs on RB-CSR and still matches the oracle's iterations."""
