@@ -144,6 +144,57 @@ def copy_ceiling(aa, torch, ctx, reps):
     return 16 * n / t / 1e9
 
 
+def time_spmv_pair(aa, torch, ctx, ro, cols, vals, reps):
+    """RB-DIA pair mode (rows of column pairs), structure built on the device: FP64 and FP32 SpMV
+    average durations and rates (bytes: values of the unpadded entries, x once, y, descriptors)."""
+    import ctypes as C
+    n, nnz = len(ro) - 1, len(vals)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    dev = lambda arr: torch.from_numpy(np.ascontiguousarray(arr)).to("cuda")
+    L = aa.lib()
+    dro, dc, dv = dev(ro), dev(cols), dev(vals)
+    bo = np.zeros((n + 63) // 64 + 1, dtype=np.int32)
+    torch.cuda.synchronize()
+    slots = L.alien_hip_rbdia_count_device(ctx.h, n, P(dro), P(dc), 64, 1, bo.ctypes.data_as(C.POINTER(C.c_int)))
+    if slots < 0:
+        return {}
+    dbo = dev(bo)
+    doff = torch.empty(slots, dtype=torch.int32, device="cuda")
+    dmask = torch.empty(slots, dtype=torch.int64, device="cuda")
+    bv = torch.empty(2 * 64 * slots, dtype=torch.float64, device="cuda")
+    bv32 = torch.empty(2 * 64 * slots, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    aa.check(L.alien_hip_rbdia_struct_device(ctx.h, n, P(dro), P(dc), 1, P(dbo), P(doff), P(dmask)))
+    aa.check(L.alien_hip_rbdia2_fill_f64(ctx.h, n, P(dro), P(dc), P(dv), P(dbo), P(doff), P(bv)))
+    aa.check(L.alien_hip_rbdia2_fill_f32(ctx.h, n, P(dro), P(dc), P(dv), P(dbo), P(doff), P(bv32)))
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    x32 = x.float()
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    y32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    out = {}
+    for name, fn, word in (
+            ("rbdia_pair_fp64", lambda: L.alien_hip_spmv_rbdia2_f64(ctx.h, n, P(dbo), P(doff), P(dmask), P(bv), P(x),
+                                                                    P(y)), 8),
+            ("rbdia_pair_fp32", lambda: L.alien_hip_spmv_rbdia2_f32(ctx.h, n, P(dbo), P(doff), P(dmask), P(bv32),
+                                                                    P(x32), P(y32)), 4)):
+        for _ in range(3):
+            aa.check(fn())
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            aa.check(fn())
+        e.record()
+        e.synchronize()
+        t = s.elapsed_time(e) / reps * 1e-3
+        nbytes = rbdia_bytes(n, nnz, slots, word)
+        out[name] = {"us": t * 1e6, "bytes": nbytes, "gbs": nbytes / t / 1e9}
+    del dro, dc, dv, dbo, doff, dmask, bv, bv32, x, x32, y, y32
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(aa, ro, cols, vals, b, fp64_iters, budget_s):
     """Oracle (C++ restatement of SimpleCSR) Diag-CG iterations on host threads; time-to-1e-8 is the
     per-iteration time times the iteration count the FP64 GPU solve needed (same algorithm)."""
@@ -405,6 +456,7 @@ def run_blocks2(a):
             (("csr_fp64", spmv_bytes_fp64(N, nnz)), ("rbcsr_fp64", spmv_bytes_fp64(N, nnz)),
              ("rbdia_fp64", rbdia_bytes(N, nnz, sp["rbdia_slots"], 8)))}
     spmv.update(bsr)
+    spmv.update(time_spmv_pair(aa, torch, ctx, ro, cols, vals, a.spmv_reps))
     best = min((v["time_s"] for v in res.values() if v["converged"]), default=None)
     print(json.dumps({"metric": "time-to-1e-10 residual (s), BiCGStab, 2 unknowns per cell", "value": best,
                       "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64 (mixed: f32+f64)",
