@@ -118,6 +118,114 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_usolve_level(int count, const i
   x[i] = v / lu[dk];
 }
 
+// ---- multicolour sweeps over the split factor --------------------------------
+// The factor LU of the colour-permuted matrix is stored as two row-bundled
+// parts (64 rows per bundle, slot (bo[b] + j) * 64 + lane, column -1 = padding)
+// holding each row's strictly-lower and strictly-upper entries in ascending
+// column order, and the diagonal u_ii. The rows of one colour class are
+// independent and every row they read belongs to a colour swept before (the
+// lower sweep takes the colours in order, the upper sweep in reverse), so one
+// launch per colour and sweep computes solveL / solveU
+// (ILU0Preconditioner.h:181-231) with the reference's operations in its order:
+// t_i = y_i - l_ic t_c ... (ascending c), x_i = (t_i - u_ic x_c ...) / u_ii.
+
+// One thread per row: its entries [ro, dcol) into the lower part, (dcol, ro+1)
+// into the upper part, padding up to the bundle width; u_ii into diag. cols
+// are written when lbc is not null (the structure is shared by FP64 and FP32).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ilu0_split_fill(int n, const int* __restrict__ ro,
+                                                             const int* __restrict__ cols,
+                                                             const int* __restrict__ dcol,
+                                                             const double* __restrict__ lu,
+                                                             const int* __restrict__ lbo, const int* __restrict__ ubo,
+                                                             int* __restrict__ lbc, int* __restrict__ ubc,
+                                                             T* __restrict__ lbv, T* __restrict__ ubv,
+                                                             T* __restrict__ diag) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int b = i / kWave, lane = i % kWave;
+  const int k0 = ro[i], kd = dcol[i], k1 = ro[i + 1];
+  diag[i] = (T)lu[kd];
+  const int lw = lbo[b + 1] - lbo[b], uw = ubo[b + 1] - ubo[b];
+  for (int j = 0; j < lw; ++j) {
+    const long sl = (long)(lbo[b] + j) * kWave + lane;
+    const bool on = k0 + j < kd;
+    if (lbc) lbc[sl] = on ? cols[k0 + j] : -1;
+    lbv[sl] = on ? (T)lu[k0 + j] : T(0);
+  }
+  for (int j = 0; j < uw; ++j) {
+    const long sl = (long)(ubo[b] + j) * kWave + lane;
+    const bool on = kd + 1 + j < k1;
+    if (ubc) ubc[sl] = on ? cols[kd + 1 + j] : -1;
+    ubv[sl] = on ? (T)lu[kd + 1 + j] : T(0);
+  }
+}
+
+// One colour class [r0, r1), one wavefront per bundle from bundle b0. The
+// slots are read in groups of kSweepGroup (columns, then values and the
+// gathered out[c]) before the group's subtractions, which keep their order.
+// Returns at entry once *done.
+constexpr int kSweepGroup = 4;
+template <typename T, bool UPPER>
+__global__ void __launch_bounds__(kRowBlock) k_ilu0_split_sweep(int b0, int r0, int r1, const int* __restrict__ bo,
+                                                                 const int* __restrict__ bc,
+                                                                 const T* __restrict__ bv,
+                                                                 const T* __restrict__ diag,
+                                                                 const T* __restrict__ in, T* __restrict__ out,
+                                                                 const int* __restrict__ done) {
+  if (done && *done) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int b = b0 + (int)((blockIdx.x * (long)kRowBlock + threadIdx.x) / kWave);
+  if ((long)b * kWave >= r1) return;
+  const int row = b * kWave + lane;
+  const bool act = row >= r0 && row < r1;
+  const int s0 = bo[b], w = bo[b + 1] - s0;
+  T acc = act ? in[row] : T(0);
+  for (int j0 = 0; j0 < w; j0 += kSweepGroup) {
+    int c[kSweepGroup];
+    T v[kSweepGroup], xv[kSweepGroup];
+#pragma unroll
+    for (int u = 0; u < kSweepGroup; ++u)
+      c[u] = (act && j0 + u < w) ? bc[(long)(s0 + j0 + u) * kWave + lane] : -1;
+#pragma unroll
+    for (int u = 0; u < kSweepGroup; ++u) {
+      v[u] = c[u] >= 0 ? bv[(long)(s0 + j0 + u) * kWave + lane] : T(0);
+      xv[u] = c[u] >= 0 ? out[c[u]] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kSweepGroup; ++u)
+      if (c[u] >= 0) acc = acc - v[u] * xv[u];
+  }
+  if (act) out[row] = UPPER ? acc / diag[row] : acc;
+}
+
+template <typename T>
+void launch_ilu0_split_fill(hipStream_t st, int n, const int* ro, const int* cols, const int* dcol, const double* lu,
+                            const int* lbo, const int* ubo, int* lbc, int* ubc, T* lbv, T* ubv, T* diag) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_ilu0_split_fill<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, ro, cols, dcol,
+                     lu, lbo, ubo, lbc, ubc, lbv, ubv, diag);
+}
+
+template <typename T>
+void launch_ilu0_split_apply(hipStream_t st, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t,
+                             T* x, const int* done) {
+  auto sweep = [&](int c, bool upper) {
+    const int r0 = cptr_host[c], r1 = cptr_host[c + 1];
+    if (r1 <= r0) return;
+    const int b0 = r0 / kWave, b1 = (r1 - 1) / kWave;
+    const dim3 g((unsigned)((b1 - b0 + 1 + kRowWaves - 1) / kRowWaves));
+    if (upper)
+      hipLaunchKernelGGL((k_ilu0_split_sweep<T, true>), g, dim3(kRowBlock), 0, st, b0, r0, r1, F.ubo, F.ubc, F.ubv,
+                         F.diag, (const T*)t, x, done);
+    else
+      hipLaunchKernelGGL((k_ilu0_split_sweep<T, false>), g, dim3(kRowBlock), 0, st, b0, r0, r1, F.lbo, F.lbc, F.lbv,
+                         F.diag, y, t, done);
+  };
+  for (int c = 0; c < ncolours; ++c) sweep(c, false);
+  for (int c = ncolours - 1; c >= 0; --c) sweep(c, true);
+}
+
 template <typename T>
 void launch_ilu0_factor(hipStream_t st, int nlevels, const int* ptr_host, const int* rows, const int* ro,
                         const int* cols, const int* dcol, T* lu) {
@@ -152,7 +260,11 @@ void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* 
                                       const int*, T*);                                                    \
   template void launch_ilu0_solve<T>(hipStream_t, int, const int*, const int*, int, const int*,           \
                                      const int*, const int*, const int*, const int*, const T*, const T*,  \
-                                     T*, T*);
+                                     T*, T*);                                                             \
+  template void launch_ilu0_split_fill<T>(hipStream_t, int, const int*, const int*, const int*,           \
+                                          const double*, const int*, const int*, int*, int*, T*, T*, T*);  \
+  template void launch_ilu0_split_apply<T>(hipStream_t, int, const int*, ILUSplit<T>, const T*, T*, T*,   \
+                                           const int*);
 ALIEN_HIP_INST_ILU0(double)
 ALIEN_HIP_INST_ILU0(float)
 

@@ -95,6 +95,28 @@ void launch_invdiag_bellpack(hipStream_t, int n, int slice, const int* bro, cons
 template <typename T>
 void launch_ilu0_factor(hipStream_t, int nlevels, const int* ptr_host, const int* rows, const int* ro,
                         const int* cols, const int* dcol, T* lu);
+// ILU(0) factor of a colour-permuted matrix split into row-bundled strictly-lower
+// and strictly-upper parts (bundle offsets lbo / ubo in 64-entry slots, columns
+// -1 = padding) and the diagonal
+template <typename T>
+struct ILUSplit {
+  const int* lbo = nullptr;
+  const int* lbc = nullptr;
+  const T* lbv = nullptr;
+  const int* ubo = nullptr;
+  const int* ubc = nullptr;
+  const T* ubv = nullptr;
+  const T* diag = nullptr;
+};
+// fills the split parts from LU in CSR order (columns too when lbc / ubc are not null)
+template <typename T>
+void launch_ilu0_split_fill(hipStream_t, int n, const int* ro, const int* cols, const int* dcol, const double* lu,
+                            const int* lbo, const int* ubo, int* lbc, int* ubc, T* lbv, T* ubv, T* diag);
+// multicolour ILU(0) apply x = (LU)^-1 y, one launch per colour and sweep
+// (cptr_host: colour c = rows [cptr[c], cptr[c+1])); returns early once *done
+template <typename T>
+void launch_ilu0_split_apply(hipStream_t, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t, T* x,
+                             const int* done);
 template <typename T>
 void launch_ilu0_solve(hipStream_t, int nl, const int* lptr_host, const int* lrows, int nu,
                        const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,

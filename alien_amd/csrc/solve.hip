@@ -100,14 +100,16 @@ void permute_csr(int n, const int* ro, const int* cols, const double* vals, cons
   }
 }
 
-// rows sorted by colour, natural order inside a colour
-std::vector<int> colour_permutation(int n, const int* ro, const int* cols) {
+// rows sorted by colour, natural order inside a colour; cptr[c] = first permuted
+// row of colour c (nc + 1 entries)
+std::vector<int> colour_permutation(int n, const int* ro, const int* cols, std::vector<int>& cptr) {
   std::vector<int> color(n);
   int nc = host_color(n, ro, cols, color.data());
   std::vector<int> perm(n);
   std::vector<int> cnt(nc + 1, 0);
   for (int i = 0; i < n; ++i) ++cnt[color[i] + 1];
   for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+  cptr = cnt;
   for (int i = 0; i < n; ++i) perm[cnt[color[i]]++] = i;
   return perm;
 }
@@ -144,7 +146,7 @@ void run_reference(Alg& alg, const alien_hip_solve_params* p, Precond& P, const 
 }
 
 // ---- fused CG ------------------------------------------------------------------------------
-enum PrecKind { kPrecDiag = 0, kPrecCheb = 1, kPrecFactor = 2 };
+enum PrecKind { kPrecDiag = 0, kPrecCheb = 1, kPrecFactor = 2, kPrecILU0 = 3 };
 struct PrecondSpec {
   PrecKind kind = kPrecDiag;
   int order = 0;
@@ -156,8 +158,63 @@ struct PrecondSpec {
   // FP32 solver with bfloat16 factor values (used instead of g32 / gt32 when `half`)
   RBMat<float, bf16_bits> g16{}, gt16{};
   bool half = false;
+  // multicolour ILU(0): the colour classes of the permuted rows and the split
+  // factor in the solver's precision
+  std::vector<int> cptr;
+  ILUSplit<double> lu64{};
+  ILUSplit<float> lu32{};
   template <typename T> RBMat<T> g() const;
   template <typename T> RBMat<T> gt() const;
+  template <typename T> ILUSplit<T> lu() const;
+};
+
+// Storage of the split ILU(0) factor: bundle widths from the host CSR (longest
+// strictly-lower / strictly-upper part of the bundle's rows), columns filled
+// with the first values.
+struct ILUSplitStore {
+  DeviceArray<int> lbo, ubo, lbc, ubc;
+  DeviceArray<double> lbv64, ubv64, d64;
+  DeviceArray<float> lbv32, ubv32, d32;
+  size_t lslots = 0, uslots = 0;
+  void build(hipStream_t st, int n, const int* ro, const int* dcol) {
+    const int nb = (n + kBundle - 1) / kBundle;
+    std::vector<int> hl(nb + 1), hu(nb + 1);
+    long al = 0, au = 0;
+    for (int b = 0; b < nb; ++b) {
+      hl[b] = (int)al;
+      hu[b] = (int)au;
+      int wl = 0, wu = 0;
+      for (int i = b * kBundle; i < std::min(n, (b + 1) * kBundle); ++i) {
+        wl = std::max(wl, dcol[i] - ro[i]);
+        wu = std::max(wu, ro[i + 1] - dcol[i] - 1);
+      }
+      al += wl;
+      au += wu;
+      if (std::max(al, au) * kBundle > 0x7fffffffL)
+        throw Error(ALIEN_HIP_EINVAL, "ILU0 split factor exceeds int32 indexing");
+    }
+    hl[nb] = (int)al;
+    hu[nb] = (int)au;
+    lslots = (size_t)al * kBundle;
+    uslots = (size_t)au * kBundle;
+    lbo.upload(hl.data(), hl.size(), st);
+    ubo.upload(hu.data(), hu.size(), st);
+    lbc.resize(lslots);
+    ubc.resize(uslots);
+  }
+  template <typename T>
+  ILUSplit<T> fill(hipStream_t st, const HIPMatrix& A, const double* lu, DeviceArray<T>& lv, DeviceArray<T>& uv,
+                   DeviceArray<T>& d) {
+    const bool cols = lv.empty() && d32.empty() && d64.empty(); // first fill writes the columns
+    lv.resize(lslots);
+    uv.resize(uslots);
+    d.resize(A.getNRows());
+    launch_ilu0_split_fill<T>(st, A.getNRows(), A.d_ro.data(), A.d_cols.data(), A.d_dcol.data(), lu, lbo.data(),
+                              ubo.data(), cols ? lbc.data() : nullptr, cols ? ubc.data() : nullptr, lv.data(),
+                              uv.data(), d.data());
+    check_launch("ILU0 split fill");
+    return ILUSplit<T>{lbo.data(), lbc.data(), lv.data(), ubo.data(), ubc.data(), uv.data(), d.data()};
+  }
 };
 
 // t = G^T r, p = G t + beta p in the factor's storage precision
@@ -176,6 +233,8 @@ template <> RBMat<double> PrecondSpec::g<double>() const { return g64; }
 template <> RBMat<double> PrecondSpec::gt<double>() const { return gt64; }
 template <> RBMat<float> PrecondSpec::g<float>() const { return g32; }
 template <> RBMat<float> PrecondSpec::gt<float>() const { return gt32; }
+template <> ILUSplit<double> PrecondSpec::lu<double>() const { return lu64; }
+template <> ILUSplit<float> PrecondSpec::lu<float>() const { return lu32; }
 
 template <typename T>
 struct FusedCG {
@@ -197,7 +256,7 @@ struct FusedCG {
     if (P.kind != kPrecFactor) z.resize(n);
     if (P.kind == kPrecCheb) cp.resize(n);
     if (P.kind == kPrecCheb && P.order > 1) { y0.resize(n); y1.resize(n); }
-    if (P.kind == kPrecFactor) tmp.resize(n);
+    if (P.kind == kPrecFactor || P.kind == kPrecILU0) tmp.resize(n);
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
   }
@@ -215,6 +274,12 @@ struct FusedCG {
     }
     if (P.kind == kPrecFactor) {
       launch_factor_apply<T>(s, ctx->red(), n, P.g<T>(), P.gt<T>(), rin, tmp.data(), zout, st.data());
+      return;
+    }
+    if (P.kind == kPrecILU0) {
+      launch_ilu0_split_apply<T>(s, (int)P.cptr.size() - 1, P.cptr.data(), P.lu<T>(), rin, tmp.data(), zout,
+                                 &st.data()->done);
+      launch_rz<T>(s, ctx->red(), n, rin, zout, st.data());
       return;
     }
     const int m = P.order;
@@ -516,11 +581,11 @@ struct FusedBiCG {
 
   FusedBiCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
   : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
-    if (P.kind != kPrecDiag && P.kind != kPrecFactor)
-      throw Error(ALIEN_HIP_EINVAL, "fused BiCGStab implements the Diag and learned preconditioners");
+    if (P.kind == kPrecCheb)
+      throw Error(ALIEN_HIP_EINVAL, "fused BiCGStab implements the Diag, ILU0 and learned preconditioners");
     ctx->reserve_for(n);
     for (auto* a : {&r, &r0, &p, &phat, &v, &sv, &shat, &t}) a->resize(n);
-    if (P.kind == kPrecFactor) tmp.resize(n);
+    if (P.kind == kPrecFactor || P.kind == kPrecILU0) tmp.resize(n);
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(BiCGState), hipHostMallocDefault));
   }
@@ -530,7 +595,11 @@ struct FusedBiCG {
   }
   const int* done_flag() { return &st.data()->done; }
   void precond(const T* in, T* out) {
-    launch_factor_apply_flag<T>(ctx->stream, n, P.g<T>(), P.gt<T>(), in, tmp.data(), out, done_flag());
+    if (P.kind == kPrecILU0)
+      launch_ilu0_split_apply<T>(ctx->stream, (int)P.cptr.size() - 1, P.cptr.data(), P.lu<T>(), in, tmp.data(),
+                                 out, done_flag());
+    else
+      launch_factor_apply_flag<T>(ctx->stream, n, P.g<T>(), P.gt<T>(), in, tmp.data(), out, done_flag());
   }
   // one step of the while loop (or the first block, per the state's `first`)
   void step(T* x) {
@@ -783,12 +852,16 @@ void validate_params(const alien_hip_solve_params* p) {
   const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
   const bool learned = p->precond == ALIEN_HIP_PREC_LEARNED;
   const bool bicg = p->solver == ALIEN_HIP_BICGS;
-  if ((fused || mixed) && !bicg && (p->solver != ALIEN_HIP_CG || (p->precond != ALIEN_HIP_PREC_CHEB &&
-                                                                  p->precond != ALIEN_HIP_PREC_DIAG && !learned)))
+  const bool ilu = p->precond == ALIEN_HIP_PREC_ILU0;
+  if ((fused || mixed) && !bicg && p->precond != ALIEN_HIP_PREC_CHEB && p->precond != ALIEN_HIP_PREC_DIAG &&
+      !ilu && !learned)
     throw Error(ALIEN_HIP_EINVAL,
-                "fused/graph/mixed modes implement CG with the Diag, Chebyshev or learned preconditioner");
-  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !learned)
-    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement BiCGStab with the Diag or learned preconditioner");
+                "fused/graph/mixed modes implement CG with the Diag, Chebyshev, ILU0 or learned preconditioner");
+  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !ilu && !learned)
+    throw Error(ALIEN_HIP_EINVAL,
+                "fused/graph/mixed modes implement BiCGStab with the Diag, ILU0 or learned preconditioner");
+  if ((fused || mixed) && ilu && p->ilu_schedule != 1)
+    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes run ILU0 on the multicolour schedule (ilu_schedule = 1)");
   if (learned && !(fused || mixed))
     throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
   if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
@@ -823,7 +896,8 @@ struct SetupTrace {
 // true residual, solution copied to x_io. tm was started at the beginning of
 // the set-up; phase marks the set-up trace.
 void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A, const double* b_in, double* x_io,
-              alien_hip_solve_result* res, EventTimer& tm, SetupTrace& phase) {
+              alien_hip_solve_result* res, EventTimer& tm, SetupTrace& phase,
+              const std::vector<int>* colours = nullptr) {
   const int n = A.getNRows();
   const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
   const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
@@ -907,6 +981,26 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
     DeviceArray<float> theta, gv32, gtv32;
     DeviceArray<double> gv64, gtv64;
     DeviceArray<bf16_bits> gv16, gtv16;
+    // multicolour ILU(0): factorised in FP64 on the level-set schedule of the
+    // colour-permuted matrix (the reference's factor), then split into its
+    // strictly-lower and strictly-upper row-bundled parts
+    std::unique_ptr<Alien::ILU0Preconditioner<Alg>> ilu;
+    ILUSplitStore lus;
+    auto ilu_fp64 = [&](PrecondSpec& q) {
+      q.lu64 = lus.fill<double>(ctx->stream, A, ilu->luValues(), lus.lbv64, lus.ubv64, lus.d64);
+    };
+    if (p->precond == ALIEN_HIP_PREC_ILU0) {
+      if (!colours || colours->size() < 2) throw Error(ALIEN_HIP_EINVAL, "fused ILU0: no colour classes");
+      ilu.reset(new Alien::ILU0Preconditioner<Alg>(alg, A));
+      ilu->init();
+      phase("ILU0 factorisation");
+      ps.kind = kPrecILU0;
+      ps.cptr = *colours;
+      lus.build(ctx->stream, n, A.kcol(), A.dcol());
+      if (mixed) ps.lu32 = lus.fill<float>(ctx->stream, A, ilu->luValues(), lus.lbv32, lus.ubv32, lus.d32);
+      else ilu_fp64(ps);
+      phase("ILU0 split factor");
+    }
     int stall_window = 0;
     if (learned) {
       gnn::SystemView v;
@@ -1047,15 +1141,21 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
         res->fallback |= 2;
         const int used = m.it;
         const int left = std::max(0, p->max_iter - used);
+        PrecondSpec p64; // Jacobi, or ILU0 with the FP64 factor
+        if (ps.kind == kPrecILU0) {
+          p64.kind = kPrecILU0;
+          p64.cptr = ps.cptr;
+          ilu_fp64(p64);
+        }
         if (bicg) {
-          FusedBiCG<double> F(ctx, n, rb64, invd64, PrecondSpec{});
+          FusedBiCG<double> F(ctx, n, rb64, invd64, p64);
           const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, true, p->poll_every, nullptr);
           m.it = used + s.it;
           m.status = s.status;
           m.error = s.breakdown ? 1 : 0;
           m.rel = std::sqrt(s.rr / alg.dot(b, b));
         } else {
-          FusedCG<double> F(ctx, n, rb64, invd64, PrecondSpec{});
+          FusedCG<double> F(ctx, n, rb64, invd64, p64);
           CGState s = F.solve(b.data(), x.data(), false, p->tol, left, true, p->poll_every, nullptr);
           finish_breakdown(F, s, &m.error);
           m.it = used + s.it;
@@ -1102,8 +1202,9 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     const double* Avals = vals;
     const double* b_in = bh;
     double* x_io = xh;
+    std::vector<int> cptr;
     if (p->precond == ALIEN_HIP_PREC_ILU0 && p->ilu_schedule == 1) {
-      perm = colour_permutation(n, ro, cols);
+      perm = colour_permutation(n, ro, cols, cptr);
       permute_csr(n, ro, cols, vals, perm, pro, pcols, pvals);
       for (int r = 0; r < n; ++r) { pb[r] = bh[perm[r]]; px[r] = xh[perm[r]]; }
       Aro = pro.data(); Acols = pcols.data(); Avals = pvals.data();
@@ -1117,7 +1218,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     phase("matrix upload");
     if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
     phase("row-bundled layout");
-    solve_on(ctx, p, A, b_in, x_io, res, tm, phase);
+    solve_on(ctx, p, A, b_in, x_io, res, tm, phase, &cptr);
     if (!perm.empty())
       for (int r = 0; r < n; ++r) xh[perm[r]] = px[r];
     ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));

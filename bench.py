@@ -288,6 +288,8 @@ def run_convdiff(a):
                                                     layout=aa.LAYOUT_RBDIA)),
             ("fp64_bicgstab_ilu0_levelset", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_ILU0, ilu_schedule=0,
                                                  layout=aa.LAYOUT_RBDIA)),
+            ("fp64_bicgstab_ilu0_multicolour_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_ILU0, ilu_schedule=1)),
+            ("mixed_bicgstab_ilu0_multicolour", dict(mode=aa.MODE_MIXED, precond=aa.PREC_ILU0, ilu_schedule=1)),
             ("fp64_bicgstab_jacobi_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
             ("mixed_bicgstab_jacobi", dict(mode=aa.MODE_MIXED, precond=aa.PREC_DIAG))]
     res = {}
@@ -433,6 +435,8 @@ def run_blocks2(a):
     theta = aa.gnn_init(0)
     runs = [("fp64_bicgstab_ilu0_multicolour", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_ILU0, ilu_schedule=1,
                                                     layout=aa.LAYOUT_RBCSR)),
+            ("fp64_bicgstab_ilu0_multicolour_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_ILU0, ilu_schedule=1)),
+            ("mixed_bicgstab_ilu0_multicolour", dict(mode=aa.MODE_MIXED, precond=aa.PREC_ILU0, ilu_schedule=1)),
             ("fp64_bicgstab_jacobi_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
             ("mixed_bicgstab_jacobi", dict(mode=aa.MODE_MIXED, precond=aa.PREC_DIAG)),
             ("mixed_bicgstab_learned", dict(mode=aa.MODE_MIXED, precond=aa.PREC_LEARNED, gnn_theta=theta))]
@@ -489,7 +493,8 @@ def run_eval(a):
             ("fp64_learned", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_LEARNED, gnn_theta=theta)),
             ("fp64_jacobi", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
             ("fp64_ilu0_multicolour", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_ILU0, ilu_schedule=1,
-                                           layout=aa.LAYOUT_RBCSR))]
+                                           layout=aa.LAYOUT_RBCSR)),
+            ("mixed_ilu0_multicolour", dict(mode=aa.MODE_MIXED, precond=aa.PREC_ILU0, ilu_schedule=1))]
     per = {name: {"iterations": [], "time_s": [], "setup_s": [], "converged": []} for name, _ in runs}
     for seed in range(1, a.seeds + 1):
         ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=[2.0, 1.0, 1.0, 1.0], seed=seed)
@@ -595,7 +600,8 @@ def main():
                     help="skip the FP64 Chebyshev-CG and multicolour ILU0-CG baselines")
     ap.add_argument("--solvers", default="",
                     help="comma list of solver runs to keep (learned_mixed, mixed_jacobi, fp64_jacobi, "
-                         "fp64_learned, fp64_chebyshev, fp64_ilu0_multicolour); default all")
+                         "fp64_learned, fp64_chebyshev, fp64_ilu0_multicolour, fp64_ilu0_multicolour_graph, "
+                         "mixed_ilu0_multicolour); default all")
     ap.add_argument("--cheb-max-iter", type=int, default=3000)
     ap.add_argument("--ilu-max-iter", type=int, default=5000)
     ap.add_argument("--spmv-reps", type=int, default=20)
@@ -680,7 +686,11 @@ def main():
     if not a.no_slow_baselines:
         runs += [("fp64_chebyshev", aa.MODE_GRAPH, aa.PREC_CHEB, dict(max_iter=a.cheb_max_iter), 1),
                  ("fp64_ilu0_multicolour", aa.MODE_REFERENCE, aa.PREC_ILU0,
-                  dict(ilu_schedule=1, layout=aa.LAYOUT_RBCSR, max_iter=a.ilu_max_iter), 1)]
+                  dict(ilu_schedule=1, layout=aa.LAYOUT_RBCSR, max_iter=a.ilu_max_iter), 1),
+                 ("fp64_ilu0_multicolour_graph", aa.MODE_GRAPH, aa.PREC_ILU0,
+                  dict(ilu_schedule=1, max_iter=a.ilu_max_iter), 1),
+                 ("mixed_ilu0_multicolour", aa.MODE_MIXED, aa.PREC_ILU0,
+                  dict(ilu_schedule=1, inner_tol=0.1, max_iter=a.ilu_max_iter), 1)]
     if a.solvers:
         keep = set(a.solvers.split(","))
         runs = [r for r in runs if r[0] in keep]

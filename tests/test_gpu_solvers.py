@@ -283,6 +283,89 @@ def test_mixed_bicgstab_hands_over_to_fp64_when_fp32_stagnates(ctx):
     assert np.all(np.isfinite(x))
 
 
+def poisson2d(k):
+    """5-point Laplacian on a k x k grid, CSR with ascending columns. Its
+    colour-permuted (red-black) rows repeat a few column offsets per 64-row
+    bundle, so the fused solvers store it in the RB-DIA layout."""
+    ro, cols, vals = [0], [], []
+    for i in range(k * k):
+        y, x = divmod(i, k)
+        for c, v in ((i - k, -1.0), (i - 1, -1.0), (i, 4.0), (i + 1, -1.0), (i + k, -1.0)):
+            if (c == i - 1 and x == 0) or (c == i + 1 and x == k - 1) or c < 0 or c >= k * k:
+                continue
+            cols.append(c)
+            vals.append(v)
+        ro.append(len(cols))
+    return np.array(ro, np.int32), np.array(cols, np.int32), np.array(vals)
+
+
+def ilu_case(case):
+    """(matrix, solver, tolerance) of the multicolour ILU(0) cases. BiCGStab on the
+    2-unknown system amplifies rounding: with the Diag preconditioner the oracle and
+    the GPU REFERENCE mode, which differ only in the summation order of the dots,
+    take 150 and 135 iterations at 10^3 cells to 1e-10 and 155 and 170 at 14^3 to
+    1e-8; at 10^3 cells to 1e-8 they agree (122), and that case is used."""
+    if case == "cg_poisson":
+        return aa.gen_csr(aa.GEN_POISSON7, 24), aa.CG, 1e-10
+    if case == "cg_poisson2d":
+        return poisson2d(96), aa.CG, 1e-10
+    if case == "bicgstab_convdiff":
+        return aa.gen_csr(aa.GEN_CONVDIFF7, 24), aa.BICGS, 1e-10
+    return bsr2_scalar_csr(10), aa.BICGS, 1e-8
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED])
+@pytest.mark.parametrize("case", ["cg_poisson", "cg_poisson2d", "bicgstab_convdiff", "bicgstab_bsr2"])
+def test_fused_multicolour_ilu0_matches_reference_mode(ctx, mode, case):
+    """Multicolour ILU(0) inside the fused CG / BiCGStab: the factor is the
+    REFERENCE-mode multicolour factor and its sweeps (one colour class per launch
+    over the row-bundled factor) subtract in the sequential order, so the FP64
+    solves take the REFERENCE-mode iteration count within +-2; the mixed solve
+    reaches the tolerance on the FP64 true residual."""
+    (ro, cols, vals), solver, tol = ilu_case(case)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    kw = dict(solver=solver, precond=aa.PREC_ILU0, ilu_schedule=1, tol=tol, max_iter=4000)
+    say("case fused multicolour ILU0", case, mode)
+    _, ref = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, poll_every=4, **kw)
+    say("  ", mode, r["iterations"], r["outer"], r["fallback"], r["true_rel_res"], "reference", ref["iterations"])
+    assert ref["status"] == 1
+    assert r["status"] == 1 and r["error"] == 0
+    if mode == aa.MODE_MIXED:
+        assert r["true_rel_res"] < 1.05 * tol
+        assert r["iterations"] <= 2 * ref["iterations"] + 20
+    else:
+        assert abs(r["iterations"] - ref["iterations"]) <= 2
+        assert r["true_rel_res"] < 1.5 * tol
+
+
+@pytest.mark.parametrize("case", ["cg_poisson", "cg_poisson2d", "bicgstab_convdiff", "bicgstab_bsr2"])
+def test_fused_ilu0_iterates_equal_reference_mode(ctx, case):
+    """Three iterations of the fused solver with multicolour ILU(0) against the
+    REFERENCE mode: the preconditioner applies agree bit for bit, the dots differ
+    only in their summation order, so the iterates agree to rounding."""
+    (ro, cols, vals), solver, _ = ilu_case(case)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(7, n))
+    kw = dict(solver=solver, precond=aa.PREC_ILU0, ilu_schedule=1, tol=1e-15, max_iter=3)
+    say("case fused ILU0 iterates", case)
+    xr, rr = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    xf, rf = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_FUSED, **kw)
+    say("  ", rr["iterations"], rf["iterations"], np.abs(xf - xr).max())
+    assert rr["iterations"] == rf["iterations"] == 3
+    np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
+
+
+def test_fused_ilu0_needs_the_multicolour_schedule(ctx):
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 8)
+    b = np.ones(len(ro) - 1)
+    with pytest.raises(aa.AlienHipError):
+        aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, ilu_schedule=0)
+    _, r = aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, ilu_schedule=1)
+    assert r["status"] == 1
+
+
 @pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
 def test_learned_bicgstab_on_two_unknown_system(ctx, mode):
     """BiCGStab with the learned M = G G^T on the 2-unknown block system, to an
