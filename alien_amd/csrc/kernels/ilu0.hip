@@ -164,7 +164,9 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_split_fill(int n, const int* __
 // One colour class [r0, r1), one wavefront per bundle from bundle b0. The
 // slots are read in groups of kSweepGroup (columns, then values and the
 // gathered out[c]) before the group's subtractions, which keep their order.
-// Returns at entry once *done.
+// With rdot (upper sweep of CG) each workgroup also stores its partial of
+// rdot . x at partial[poff + blockIdx.x], so that the colour launches together
+// leave the partials of r . z for one finalize. Returns at entry once *done.
 constexpr int kSweepGroup = 4;
 template <typename T, bool UPPER>
 __global__ void __launch_bounds__(kRowBlock) k_ilu0_split_sweep(int b0, int r0, int r1, const int* __restrict__ bo,
@@ -172,31 +174,44 @@ __global__ void __launch_bounds__(kRowBlock) k_ilu0_split_sweep(int b0, int r0, 
                                                                  const T* __restrict__ bv,
                                                                  const T* __restrict__ diag,
                                                                  const T* __restrict__ in, T* __restrict__ out,
-                                                                 const int* __restrict__ done) {
+                                                                 const int* __restrict__ done,
+                                                                 const T* __restrict__ rdot, RedWork red, int poff) {
   if (done && *done) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int b = b0 + (int)((blockIdx.x * (long)kRowBlock + threadIdx.x) / kWave);
-  if ((long)b * kWave >= r1) return;
+  const bool wave_on = (long)b * kWave < r1;
   const int row = b * kWave + lane;
-  const bool act = row >= r0 && row < r1;
-  const int s0 = bo[b], w = bo[b + 1] - s0;
-  T acc = act ? in[row] : T(0);
-  for (int j0 = 0; j0 < w; j0 += kSweepGroup) {
-    int c[kSweepGroup];
-    T v[kSweepGroup], xv[kSweepGroup];
+  const bool act = wave_on && row >= r0 && row < r1;
+  T res = T(0);
+  if (wave_on) {
+    const int s0 = bo[b], w = bo[b + 1] - s0;
+    T acc = act ? in[row] : T(0);
+    for (int j0 = 0; j0 < w; j0 += kSweepGroup) {
+      int c[kSweepGroup];
+      T v[kSweepGroup], xv[kSweepGroup];
 #pragma unroll
-    for (int u = 0; u < kSweepGroup; ++u)
-      c[u] = (act && j0 + u < w) ? bc[(long)(s0 + j0 + u) * kWave + lane] : -1;
+      for (int u = 0; u < kSweepGroup; ++u)
+        c[u] = (act && j0 + u < w) ? bc[(long)(s0 + j0 + u) * kWave + lane] : -1;
 #pragma unroll
-    for (int u = 0; u < kSweepGroup; ++u) {
-      v[u] = c[u] >= 0 ? bv[(long)(s0 + j0 + u) * kWave + lane] : T(0);
-      xv[u] = c[u] >= 0 ? out[c[u]] : T(0);
+      for (int u = 0; u < kSweepGroup; ++u) {
+        v[u] = c[u] >= 0 ? bv[(long)(s0 + j0 + u) * kWave + lane] : T(0);
+        xv[u] = c[u] >= 0 ? out[c[u]] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kSweepGroup; ++u)
+        if (c[u] >= 0) acc = acc - v[u] * xv[u];
     }
-#pragma unroll
-    for (int u = 0; u < kSweepGroup; ++u)
-      if (c[u] >= 0) acc = acc - v[u] * xv[u];
+    if (act) {
+      res = UPPER ? acc / diag[row] : acc;
+      out[row] = res;
+    }
   }
-  if (act) out[row] = UPPER ? acc / diag[row] : acc;
+  if (UPPER && rdot) {
+    __shared__ double lds4[kRowWaves];
+    const double sv = act ? (double)rdot[row] * (double)res : 0.0;
+    const double tot = block_sum<kRowBlock>(sv, lds4);
+    if (threadIdx.x == 0) red.partial[poff + blockIdx.x] = tot;
+  }
 }
 
 template <typename T>
@@ -207,23 +222,36 @@ void launch_ilu0_split_fill(hipStream_t st, int n, const int* ro, const int* col
                      lu, lbo, ubo, lbc, ubc, lbv, ubv, diag);
 }
 
+int ilu0_split_partials(int ncolours, const int* cptr_host) {
+  int tot = 0;
+  for (int c = 0; c < ncolours; ++c) {
+    const int r0 = cptr_host[c], r1 = cptr_host[c + 1];
+    if (r1 > r0) tot += ((r1 - 1) / kWave - r0 / kWave + 1 + kRowWaves - 1) / kRowWaves;
+  }
+  return tot;
+}
+
 template <typename T>
-void launch_ilu0_split_apply(hipStream_t st, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t,
-                             T* x, const int* done) {
+int launch_ilu0_split_apply(hipStream_t st, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t,
+                            T* x, const int* done, const T* rdot, RedWork red) {
+  int poff = 0;
   auto sweep = [&](int c, bool upper) {
     const int r0 = cptr_host[c], r1 = cptr_host[c + 1];
     if (r1 <= r0) return;
     const int b0 = r0 / kWave, b1 = (r1 - 1) / kWave;
-    const dim3 g((unsigned)((b1 - b0 + 1 + kRowWaves - 1) / kRowWaves));
-    if (upper)
-      hipLaunchKernelGGL((k_ilu0_split_sweep<T, true>), g, dim3(kRowBlock), 0, st, b0, r0, r1, F.ubo, F.ubc, F.ubv,
-                         F.diag, (const T*)t, x, done);
-    else
-      hipLaunchKernelGGL((k_ilu0_split_sweep<T, false>), g, dim3(kRowBlock), 0, st, b0, r0, r1, F.lbo, F.lbc, F.lbv,
-                         F.diag, y, t, done);
+    const int nblk = (b1 - b0 + 1 + kRowWaves - 1) / kRowWaves;
+    if (upper) {
+      hipLaunchKernelGGL((k_ilu0_split_sweep<T, true>), dim3(nblk), dim3(kRowBlock), 0, st, b0, r0, r1, F.ubo,
+                         F.ubc, F.ubv, F.diag, (const T*)t, x, done, rdot, red, poff);
+      poff += nblk;
+    } else {
+      hipLaunchKernelGGL((k_ilu0_split_sweep<T, false>), dim3(nblk), dim3(kRowBlock), 0, st, b0, r0, r1, F.lbo,
+                         F.lbc, F.lbv, F.diag, y, t, done, (const T*)nullptr, red, 0);
+    }
   };
   for (int c = 0; c < ncolours; ++c) sweep(c, false);
   for (int c = ncolours - 1; c >= 0; --c) sweep(c, true);
+  return poff;
 }
 
 template <typename T>
@@ -263,8 +291,8 @@ void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* 
                                      T*, T*);                                                             \
   template void launch_ilu0_split_fill<T>(hipStream_t, int, const int*, const int*, const int*,           \
                                           const double*, const int*, const int*, int*, int*, T*, T*, T*);  \
-  template void launch_ilu0_split_apply<T>(hipStream_t, int, const int*, ILUSplit<T>, const T*, T*, T*,   \
-                                           const int*);
+  template int launch_ilu0_split_apply<T>(hipStream_t, int, const int*, ILUSplit<T>, const T*, T*, T*,    \
+                                          const int*, const T*, RedWork);
 ALIEN_HIP_INST_ILU0(double)
 ALIEN_HIP_INST_ILU0(float)
 

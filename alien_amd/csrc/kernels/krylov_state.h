@@ -91,9 +91,6 @@ void launch_cheb_step(hipStream_t, RedWork w, int n, RBMat<T> A, const T* xin, c
                       const T* invd, double a, double b, bool first, bool last, CGState* st);
 template <typename T>
 void launch_diag_apply(hipStream_t, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st);
-// rho1 = r . z (after a preconditioner applied by separate launches)
-template <typename T>
-void launch_rz(hipStream_t, RedWork w, int n, const T* r, const T* z, CGState* st);
 template <typename T>
 void launch_cg_p(hipStream_t, int n, const T* z, T* p, CGState* st, int first);
 template <typename T>

@@ -113,10 +113,13 @@ template <typename T>
 void launch_ilu0_split_fill(hipStream_t, int n, const int* ro, const int* cols, const int* dcol, const double* lu,
                             const int* lbo, const int* ubo, int* lbc, int* ubc, T* lbv, T* ubv, T* diag);
 // multicolour ILU(0) apply x = (LU)^-1 y, one launch per colour and sweep
-// (cptr_host: colour c = rows [cptr[c], cptr[c+1])); returns early once *done
+// (cptr_host: colour c = rows [cptr[c], cptr[c+1])); returns early once *done.
+// With rdot the upper sweeps also leave the partials of rdot . x in red (their
+// count is returned, ilu0_split_partials() beforehand) for a finalize.
+int ilu0_split_partials(int ncolours, const int* cptr_host);
 template <typename T>
-void launch_ilu0_split_apply(hipStream_t, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t, T* x,
-                             const int* done);
+int launch_ilu0_split_apply(hipStream_t, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t, T* x,
+                            const int* done, const T* rdot, RedWork red);
 template <typename T>
 void launch_ilu0_solve(hipStream_t, int nl, const int* lptr_host, const int* lrows, int nu,
                        const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,

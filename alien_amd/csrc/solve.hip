@@ -257,6 +257,7 @@ struct FusedCG {
     if (P.kind == kPrecCheb) cp.resize(n);
     if (P.kind == kPrecCheb && P.order > 1) { y0.resize(n); y1.resize(n); }
     if (P.kind == kPrecFactor || P.kind == kPrecILU0) tmp.resize(n);
+    if (P.kind == kPrecILU0) ctx->reserve_red(ilu0_split_partials((int)P.cptr.size() - 1, P.cptr.data()));
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
   }
@@ -277,9 +278,10 @@ struct FusedCG {
       return;
     }
     if (P.kind == kPrecILU0) {
-      launch_ilu0_split_apply<T>(s, (int)P.cptr.size() - 1, P.cptr.data(), P.lu<T>(), rin, tmp.data(), zout,
-                                 &st.data()->done);
-      launch_rz<T>(s, ctx->red(), n, rin, zout, st.data());
+      // rho1 = r . z from the upper sweeps' partials
+      const int np = launch_ilu0_split_apply<T>(s, (int)P.cptr.size() - 1, P.cptr.data(), P.lu<T>(), rin,
+                                                tmp.data(), zout, &st.data()->done, rin, ctx->red());
+      launch_finalize(s, ctx->red(), np, 1, kFinRho1, nullptr, st.data());
       return;
     }
     const int m = P.order;
@@ -597,7 +599,7 @@ struct FusedBiCG {
   void precond(const T* in, T* out) {
     if (P.kind == kPrecILU0)
       launch_ilu0_split_apply<T>(ctx->stream, (int)P.cptr.size() - 1, P.cptr.data(), P.lu<T>(), in, tmp.data(),
-                                 out, done_flag());
+                                 out, done_flag(), nullptr, ctx->red());
     else
       launch_factor_apply_flag<T>(ctx->stream, n, P.g<T>(), P.gt<T>(), in, tmp.data(), out, done_flag());
   }
