@@ -91,6 +91,7 @@ class FILU0Preconditioner<HIPInternalLinearAlgebra> {
   void solve(const VectorType& x, VectorType& y) const { solve(m_alg, x, y); }
 
   const double* luValues() const { return d_lu.data(); }
+  const VectorType& invDiag() const { return m_inv_diag; }
 
  private:
   // algebra.addLMult / addUMult(-1, LU, xk, y): A's structure with the factor's values

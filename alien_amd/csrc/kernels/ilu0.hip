@@ -128,6 +128,10 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_usolve_level(int count, const i
 // launch per colour and sweep computes solveL / solveU
 // (ILU0Preconditioner.h:181-231) with the reference's operations in its order:
 // t_i = y_i - l_ic t_c ... (ascending c), x_i = (t_i - u_ic x_c ...) / u_ii.
+// The same layout of FILU0's factor (natural order) serves its Jacobi sweeps:
+// addLMult / addUMult with alpha = -1 (FILU0Preconditioner.h:226-230, 297-303)
+// add (-a) x = -(a x) term by term, the subtraction above, reading the
+// previous sweep's vector instead of the one being written.
 
 // One thread per row: its entries [ro, dcol) into the lower part, (dcol, ro+1)
 // into the upper part, padding up to the bundle width; u_ii into diag. cols
@@ -161,19 +165,24 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_split_fill(int n, const int* __
   }
 }
 
-// One colour class [r0, r1), one wavefront per bundle from bundle b0. The
-// slots are read in groups of kSweepGroup (columns, then values and the
-// gathered out[c]) before the group's subtractions, which keep their order.
-// With rdot (upper sweep of CG) each workgroup also stores its partial of
-// rdot . x at partial[poff + blockIdx.x], so that the colour launches together
-// leave the partials of r . z for one finalize. Returns at entry once *done.
+// Rows [r0, r1), one wavefront per bundle from bundle b0: out_i = in_i -
+// sum_c a_ic src_c over the part's entries, then (KIND) as is (lower), divided
+// by diag_i (ILU(0) upper) or multiplied by diag_i holding the inverse diagonal
+// (FILU0 upper, pointwiseMult after addUMult). The slots are read in groups of
+// kSweepGroup (columns, then values and the gathered src[c]) before the
+// group's subtractions, which keep their order. Multicolour: src = out (rows
+// of earlier colours); FILU0: src = the previous sweep. With rdot (the last
+// upper sweep of CG) each workgroup also stores its partial of rdot . out at
+// partial[poff + blockIdx.x], so that the launches together leave the
+// partials of r . z for one finalize. Returns at entry once *done.
 constexpr int kSweepGroup = 4;
-template <typename T, bool UPPER>
+enum SweepKind { kSweepLower = 0, kSweepUpperDiv = 1, kSweepUpperInvMul = 2 };
+template <typename T, int KIND>
 __global__ void __launch_bounds__(kRowBlock) k_ilu0_split_sweep(int b0, int r0, int r1, const int* __restrict__ bo,
                                                                  const int* __restrict__ bc,
                                                                  const T* __restrict__ bv,
                                                                  const T* __restrict__ diag,
-                                                                 const T* __restrict__ in, T* __restrict__ out,
+                                                                 const T* __restrict__ in, const T* src, T* out,
                                                                  const int* __restrict__ done,
                                                                  const T* __restrict__ rdot, RedWork red, int poff) {
   if (done && *done) return;
@@ -195,18 +204,18 @@ __global__ void __launch_bounds__(kRowBlock) k_ilu0_split_sweep(int b0, int r0, 
 #pragma unroll
       for (int u = 0; u < kSweepGroup; ++u) {
         v[u] = c[u] >= 0 ? bv[(long)(s0 + j0 + u) * kWave + lane] : T(0);
-        xv[u] = c[u] >= 0 ? out[c[u]] : T(0);
+        xv[u] = c[u] >= 0 ? src[c[u]] : T(0);
       }
 #pragma unroll
       for (int u = 0; u < kSweepGroup; ++u)
         if (c[u] >= 0) acc = acc - v[u] * xv[u];
     }
     if (act) {
-      res = UPPER ? acc / diag[row] : acc;
+      res = KIND == kSweepUpperDiv ? acc / diag[row] : KIND == kSweepUpperInvMul ? diag[row] * acc : acc;
       out[row] = res;
     }
   }
-  if (UPPER && rdot) {
+  if (KIND != kSweepLower && rdot) {
     __shared__ double lds4[kRowWaves];
     const double sv = act ? (double)rdot[row] * (double)res : 0.0;
     const double tot = block_sum<kRowBlock>(sv, lds4);
@@ -241,17 +250,46 @@ int launch_ilu0_split_apply(hipStream_t st, int ncolours, const int* cptr_host, 
     const int b0 = r0 / kWave, b1 = (r1 - 1) / kWave;
     const int nblk = (b1 - b0 + 1 + kRowWaves - 1) / kRowWaves;
     if (upper) {
-      hipLaunchKernelGGL((k_ilu0_split_sweep<T, true>), dim3(nblk), dim3(kRowBlock), 0, st, b0, r0, r1, F.ubo,
-                         F.ubc, F.ubv, F.diag, (const T*)t, x, done, rdot, red, poff);
+      hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepUpperDiv>), dim3(nblk), dim3(kRowBlock), 0, st, b0, r0, r1,
+                         F.ubo, F.ubc, F.ubv, F.diag, (const T*)t, (const T*)x, x, done, rdot, red, poff);
       poff += nblk;
     } else {
-      hipLaunchKernelGGL((k_ilu0_split_sweep<T, false>), dim3(nblk), dim3(kRowBlock), 0, st, b0, r0, r1, F.lbo,
-                         F.lbc, F.lbv, F.diag, y, t, done, (const T*)nullptr, red, 0);
+      hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepLower>), dim3(nblk), dim3(kRowBlock), 0, st, b0, r0, r1, F.lbo,
+                         F.lbc, F.lbv, F.diag, y, (const T*)t, t, done, (const T*)nullptr, red, 0);
     }
   };
   for (int c = 0; c < ncolours; ++c) sweep(c, false);
   for (int c = ncolours - 1; c >= 0; --c) sweep(c, true);
   return poff;
+}
+
+// FILU0 solve (FILU0Preconditioner.h:236-256, device branch): niter >= 1 Jacobi
+// sweeps with the lower part from mx = y, then niter with the upper part from
+// z = mx, each upper sweep scaled by the inverse diagonal (F.diag). t1 ends as
+// mx; t2 and z alternate so that the last upper sweep writes z.
+template <typename T>
+int launch_filu0_split_apply(hipStream_t st, int n, int niter, ILUSplit<T> F, const T* y, T* t1, T* t2, T* z,
+                             const int* done, const T* rdot, RedWork red) {
+  const int nb = (n + kWave - 1) / kWave;
+  const dim3 g((unsigned)((nb + kRowWaves - 1) / kRowWaves));
+  // lower: src_0 = y, outputs alternate t2 / t1 ending in t1
+  const T* src = y;
+  for (int it = 0; it < niter; ++it) {
+    T* out = ((niter - 1 - it) % 2 == 0) ? t1 : t2;
+    hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepLower>), g, dim3(kRowBlock), 0, st, 0, 0, n, F.lbo, F.lbc, F.lbv,
+                       F.diag, y, src, out, done, (const T*)nullptr, red, 0);
+    src = out;
+  }
+  // upper: src_0 = mx = t1, outputs alternate t2 / z ending in z
+  src = t1;
+  for (int it = 0; it < niter; ++it) {
+    T* out = ((niter - 1 - it) % 2 == 0) ? z : t2;
+    const bool last = it == niter - 1;
+    hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepUpperInvMul>), g, dim3(kRowBlock), 0, st, 0, 0, n, F.ubo, F.ubc,
+                       F.ubv, F.diag, (const T*)t1, src, out, done, last ? rdot : (const T*)nullptr, red, 0);
+    src = out;
+  }
+  return (rdot && niter > 0) ? (int)g.x : 0;
 }
 
 template <typename T>
@@ -292,7 +330,9 @@ void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* 
   template void launch_ilu0_split_fill<T>(hipStream_t, int, const int*, const int*, const int*,           \
                                           const double*, const int*, const int*, int*, int*, T*, T*, T*);  \
   template int launch_ilu0_split_apply<T>(hipStream_t, int, const int*, ILUSplit<T>, const T*, T*, T*,    \
-                                          const int*, const T*, RedWork);
+                                          const int*, const T*, RedWork);                                  \
+  template int launch_filu0_split_apply<T>(hipStream_t, int, int, ILUSplit<T>, const T*, T*, T*, T*,       \
+                                           const int*, const T*, RedWork);
 ALIEN_HIP_INST_ILU0(double)
 ALIEN_HIP_INST_ILU0(float)
 

@@ -117,6 +117,12 @@ void launch_ilu0_split_fill(hipStream_t, int n, const int* ro, const int* cols, 
 // With rdot the upper sweeps also leave the partials of rdot . x in red (their
 // count is returned, ilu0_split_partials() beforehand) for a finalize.
 int ilu0_split_partials(int ncolours, const int* cptr_host);
+// FILU0 solve z = (LU)^-1 y by niter >= 1 Jacobi sweeps per factor part over the
+// split factor of the natural-order matrix (F.diag: the inverse diagonal); t1,
+// t2: workspace. With rdot the last sweep leaves the partials of rdot . z.
+template <typename T>
+int launch_filu0_split_apply(hipStream_t, int n, int niter, ILUSplit<T> F, const T* y, T* t1, T* t2, T* z,
+                             const int* done, const T* rdot, RedWork red);
 template <typename T>
 int launch_ilu0_split_apply(hipStream_t, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t, T* x,
                             const int* done, const T* rdot, RedWork red);

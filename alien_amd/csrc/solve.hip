@@ -146,7 +146,7 @@ void run_reference(Alg& alg, const alien_hip_solve_params* p, Precond& P, const 
 }
 
 // ---- fused CG ------------------------------------------------------------------------------
-enum PrecKind { kPrecDiag = 0, kPrecCheb = 1, kPrecFactor = 2, kPrecILU0 = 3 };
+enum PrecKind { kPrecDiag = 0, kPrecCheb = 1, kPrecFactor = 2, kPrecILU0 = 3, kPrecFILU0 = 4 };
 struct PrecondSpec {
   PrecKind kind = kPrecDiag;
   int order = 0;
@@ -159,8 +159,10 @@ struct PrecondSpec {
   RBMat<float, bf16_bits> g16{}, gt16{};
   bool half = false;
   // multicolour ILU(0): the colour classes of the permuted rows and the split
-  // factor in the solver's precision
+  // factor in the solver's precision; FILU0: the split factor (diag = its
+  // inverse diagonal) and the Jacobi sweeps per part
   std::vector<int> cptr;
+  int filu_iters = 0;
   ILUSplit<double> lu64{};
   ILUSplit<float> lu32{};
   template <typename T> RBMat<T> g() const;
@@ -245,7 +247,7 @@ struct FusedCG {
   PrecondSpec P;
   // r, z: residual and preconditioned residual; p, q: direction and A p;
   // cp: the Chebyshev sweep's own direction vector
-  DeviceArray<T> r, z, p, q, y0, y1, cp, tmp;
+  DeviceArray<T> r, z, p, q, y0, y1, cp, tmp, tmp2;
   DeviceArray<CGState> st;
   CGState* h_st = nullptr; // pinned
 
@@ -256,7 +258,8 @@ struct FusedCG {
     if (P.kind != kPrecFactor) z.resize(n);
     if (P.kind == kPrecCheb) cp.resize(n);
     if (P.kind == kPrecCheb && P.order > 1) { y0.resize(n); y1.resize(n); }
-    if (P.kind == kPrecFactor || P.kind == kPrecILU0) tmp.resize(n);
+    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0) tmp.resize(n);
+    if (P.kind == kPrecFILU0) tmp2.resize(n);
     if (P.kind == kPrecILU0) ctx->reserve_red(ilu0_split_partials((int)P.cptr.size() - 1, P.cptr.data()));
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
@@ -281,6 +284,12 @@ struct FusedCG {
       // rho1 = r . z from the upper sweeps' partials
       const int np = launch_ilu0_split_apply<T>(s, (int)P.cptr.size() - 1, P.cptr.data(), P.lu<T>(), rin,
                                                 tmp.data(), zout, &st.data()->done, rin, ctx->red());
+      launch_finalize(s, ctx->red(), np, 1, kFinRho1, nullptr, st.data());
+      return;
+    }
+    if (P.kind == kPrecFILU0) {
+      const int np = launch_filu0_split_apply<T>(s, n, P.filu_iters, P.lu<T>(), rin, tmp.data(), tmp2.data(), zout,
+                                                 &st.data()->done, rin, ctx->red());
       launch_finalize(s, ctx->red(), np, 1, kFinRho1, nullptr, st.data());
       return;
     }
@@ -577,17 +586,18 @@ struct FusedBiCG {
   RBMat<T> A;
   const T* invd;
   PrecondSpec P;
-  DeviceArray<T> r, r0, p, phat, v, sv, shat, t, tmp;
+  DeviceArray<T> r, r0, p, phat, v, sv, shat, t, tmp, tmp2;
   DeviceArray<BiCGState> st;
   BiCGState* h_st = nullptr; // pinned
 
   FusedBiCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
   : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
     if (P.kind == kPrecCheb)
-      throw Error(ALIEN_HIP_EINVAL, "fused BiCGStab implements the Diag, ILU0 and learned preconditioners");
+      throw Error(ALIEN_HIP_EINVAL, "fused BiCGStab implements the Diag, ILU0, FILU0 and learned preconditioners");
     ctx->reserve_for(n);
     for (auto* a : {&r, &r0, &p, &phat, &v, &sv, &shat, &t}) a->resize(n);
-    if (P.kind == kPrecFactor || P.kind == kPrecILU0) tmp.resize(n);
+    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0) tmp.resize(n);
+    if (P.kind == kPrecFILU0) tmp2.resize(n);
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(BiCGState), hipHostMallocDefault));
   }
@@ -600,6 +610,9 @@ struct FusedBiCG {
     if (P.kind == kPrecILU0)
       launch_ilu0_split_apply<T>(ctx->stream, (int)P.cptr.size() - 1, P.cptr.data(), P.lu<T>(), in, tmp.data(),
                                  out, done_flag(), nullptr, ctx->red());
+    else if (P.kind == kPrecFILU0)
+      launch_filu0_split_apply<T>(ctx->stream, n, P.filu_iters, P.lu<T>(), in, tmp.data(), tmp2.data(), out,
+                                  done_flag(), nullptr, ctx->red());
     else
       launch_factor_apply_flag<T>(ctx->stream, n, P.g<T>(), P.gt<T>(), in, tmp.data(), out, done_flag());
   }
@@ -855,21 +868,24 @@ void validate_params(const alien_hip_solve_params* p) {
   const bool learned = p->precond == ALIEN_HIP_PREC_LEARNED;
   const bool bicg = p->solver == ALIEN_HIP_BICGS;
   const bool ilu = p->precond == ALIEN_HIP_PREC_ILU0;
+  const bool filu = p->precond == ALIEN_HIP_PREC_FILU0;
   if ((fused || mixed) && !bicg && p->precond != ALIEN_HIP_PREC_CHEB && p->precond != ALIEN_HIP_PREC_DIAG &&
-      !ilu && !learned)
+      !ilu && !filu && !learned)
     throw Error(ALIEN_HIP_EINVAL,
-                "fused/graph/mixed modes implement CG with the Diag, Chebyshev, ILU0 or learned preconditioner");
-  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !ilu && !learned)
+                "fused/graph/mixed modes implement CG with the Diag, Chebyshev, ILU0, FILU0 or learned preconditioner");
+  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !ilu && !filu && !learned)
     throw Error(ALIEN_HIP_EINVAL,
-                "fused/graph/mixed modes implement BiCGStab with the Diag, ILU0 or learned preconditioner");
+                "fused/graph/mixed modes implement BiCGStab with the Diag, ILU0, FILU0 or learned preconditioner");
+  if ((fused || mixed) && filu && p->filu_solver_niter < 1)
+    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed FILU0 needs filu_solver_niter >= 1");
   if ((fused || mixed) && ilu && p->ilu_schedule != 1)
     throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes run ILU0 on the multicolour schedule (ilu_schedule = 1)");
   if (learned && !(fused || mixed))
     throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
   if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
   if (p->stall_window < 0) throw Error(ALIEN_HIP_EINVAL, "solve_csr: stall_window < 0");
-  if ((p->precond == ALIEN_HIP_PREC_FILU0 || p->precond == ALIEN_HIP_PREC_NEUMANN) && (fused || mixed))
-    throw Error(ALIEN_HIP_EINVAL, "FILU0 and Neumann run in the REFERENCE mode");
+  if (p->precond == ALIEN_HIP_PREC_NEUMANN && (fused || mixed))
+    throw Error(ALIEN_HIP_EINVAL, "Neumann runs in the REFERENCE mode");
   if (p->precond == ALIEN_HIP_PREC_FILU0 && (p->filu_factor_niter < 0 || p->filu_solver_niter < 0))
     throw Error(ALIEN_HIP_EINVAL, "FILU0: negative iteration counts");
   if (p->factor_storage != 0 && p->factor_storage != 1)
@@ -1002,6 +1018,33 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       if (mixed) ps.lu32 = lus.fill<float>(ctx->stream, A, ilu->luValues(), lus.lbv32, lus.ubv32, lus.d32);
       else ilu_fp64(ps);
       phase("ILU0 split factor");
+    }
+    // FILU0: the REFERENCE-mode factor (exact or fixed-point sweeps) split the
+    // same way, its inverse diagonal in place of the diagonal
+    std::unique_ptr<Alien::FILU0Preconditioner<Alg>> filu;
+    DeviceArray<float> finvd32;
+    auto filu_fp64 = [&](PrecondSpec& q) {
+      q.lu64 = lus.fill<double>(ctx->stream, A, filu->luValues(), lus.lbv64, lus.ubv64, lus.d64);
+      q.lu64.diag = filu->invDiag().data();
+    };
+    if (p->precond == ALIEN_HIP_PREC_FILU0) {
+      filu.reset(new Alien::FILU0Preconditioner<Alg>(alg, A));
+      filu->setParameter("nb-factor-iter", p->filu_factor_niter);
+      filu->setParameter("nb-solver-iter", p->filu_solver_niter);
+      filu->init();
+      phase("FILU0 factorisation");
+      ps.kind = kPrecFILU0;
+      ps.filu_iters = p->filu_solver_niter;
+      lus.build(ctx->stream, n, A.kcol(), A.dcol());
+      if (mixed) {
+        ps.lu32 = lus.fill<float>(ctx->stream, A, filu->luValues(), lus.lbv32, lus.ubv32, lus.d32);
+        finvd32.resize(n);
+        launch_demote(ctx->stream, n, filu->invDiag().data(), finvd32.data());
+        ps.lu32.diag = finvd32.data();
+      } else {
+        filu_fp64(ps);
+      }
+      phase("FILU0 split factor");
     }
     int stall_window = 0;
     if (learned) {
@@ -1143,11 +1186,15 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
         res->fallback |= 2;
         const int used = m.it;
         const int left = std::max(0, p->max_iter - used);
-        PrecondSpec p64; // Jacobi, or ILU0 with the FP64 factor
+        PrecondSpec p64; // Jacobi, or (F)ILU0 with the FP64 factor
         if (ps.kind == kPrecILU0) {
           p64.kind = kPrecILU0;
           p64.cptr = ps.cptr;
           ilu_fp64(p64);
+        } else if (ps.kind == kPrecFILU0) {
+          p64.kind = kPrecFILU0;
+          p64.filu_iters = ps.filu_iters;
+          filu_fp64(p64);
         }
         if (bicg) {
           FusedBiCG<double> F(ctx, n, rb64, invd64, p64);

@@ -290,6 +290,13 @@ def run_convdiff(a):
                                                  layout=aa.LAYOUT_RBDIA)),
             ("fp64_bicgstab_ilu0_multicolour_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_ILU0, ilu_schedule=1)),
             ("mixed_bicgstab_ilu0_multicolour", dict(mode=aa.MODE_MIXED, precond=aa.PREC_ILU0, ilu_schedule=1)),
+            # FILU0 with test_krylov's defaults (0 factor sweeps: the exact ILU(0) factor, 3 solver sweeps)
+            ("fp64_bicgstab_filu0", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_FILU0, layout=aa.LAYOUT_RBDIA,
+                                         filu_factor_niter=0, filu_solver_niter=3)),
+            ("fp64_bicgstab_filu0_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_FILU0, filu_factor_niter=0,
+                                               filu_solver_niter=3)),
+            ("mixed_bicgstab_filu0", dict(mode=aa.MODE_MIXED, precond=aa.PREC_FILU0, filu_factor_niter=0,
+                                          filu_solver_niter=3)),
             ("fp64_bicgstab_jacobi_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
             ("mixed_bicgstab_jacobi", dict(mode=aa.MODE_MIXED, precond=aa.PREC_DIAG))]
     res = {}

@@ -357,6 +357,73 @@ def test_fused_ilu0_iterates_equal_reference_mode(ctx, case):
     np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
 
 
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED])
+@pytest.mark.parametrize("case", ["bicgstab_convdiff"])
+@pytest.mark.parametrize("niter", [(0, 3), (2, 2)])
+def test_fused_filu0_matches_reference_mode(ctx, mode, case, niter):
+    """FILU0 inside the fused BiCGStab: the REFERENCE-mode factor (exact, or
+    fixed-point sweeps) split into its lower and upper parts, the solver's Jacobi
+    sweeps one launch each with the reference's arithmetic; FP64 iteration counts
+    within +-2 of the REFERENCE mode, the mixed solve at the tolerance. (On the
+    2-unknown system at 10^3 cells BiCGStab-FILU0 does not converge: the oracle
+    ends 4000 iterations at relative residual 4.9 with (0, 3) sweeps and 90 with
+    (2, 2); the GPU REFERENCE mode happens to converge, in 256 and 55 iterations,
+    and the fused solver diverges as the oracle does, from iterates equal to
+    1e-15 after two iterations.)"""
+    (ro, cols, vals), solver, tol = ilu_case(case)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    kw = dict(solver=solver, precond=aa.PREC_FILU0, filu_factor_niter=niter[0], filu_solver_niter=niter[1],
+              tol=tol, max_iter=4000)
+    say("case fused FILU0", case, mode, niter)
+    _, ref = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, poll_every=4, **kw)
+    say("  ", mode, r["iterations"], r["outer"], r["fallback"], r["true_rel_res"], "reference", ref["iterations"])
+    assert ref["status"] == 1
+    assert r["status"] == 1 and r["error"] == 0
+    if mode == aa.MODE_MIXED:
+        assert r["true_rel_res"] < 1.05 * tol
+        assert r["iterations"] <= 2 * ref["iterations"] + 20
+    else:
+        assert abs(r["iterations"] - ref["iterations"]) <= 2
+        assert r["true_rel_res"] < 1.5 * tol
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH])
+def test_fused_filu0_cg_stalls_as_the_reference(ctx, mode):
+    """CG with FILU0 on the 3-D Poisson matrix: the Jacobi sweeps make M
+    nonsymmetric and the REFERENCE-mode CG stalls (relative residual 0.2 after
+    4000 iterations); the fused solver stalls the same way, to the same residual."""
+    (ro, cols, vals), solver, _ = ilu_case("cg_poisson")
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    kw = dict(solver=solver, precond=aa.PREC_FILU0, filu_factor_niter=0, filu_solver_niter=3, tol=1e-10,
+              max_iter=300)
+    _, ref = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    _, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, poll_every=4, **kw)
+    say("case fused FILU0 CG", mode, r["value"], ref["value"])
+    assert ref["status"] == r["status"] == 0 and r["iterations"] == ref["iterations"] == 300
+    assert r["value"] == pytest.approx(ref["value"], rel=1e-6)
+
+
+@pytest.mark.parametrize("case", ["cg_poisson", "bicgstab_convdiff"])
+def test_fused_filu0_iterates_equal_reference_mode(ctx, case):
+    """Three fused iterations with FILU0 (2 factor sweeps, 3 solver sweeps) against
+    the REFERENCE mode: the applies agree bit for bit, the iterates to rounding of
+    the dots."""
+    (ro, cols, vals), solver, _ = ilu_case(case)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(7, n))
+    kw = dict(solver=solver, precond=aa.PREC_FILU0, filu_factor_niter=2, filu_solver_niter=3, tol=1e-15,
+              max_iter=3)
+    say("case fused FILU0 iterates", case)
+    xr, rr = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    xf, rf = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_FUSED, **kw)
+    say("  ", rr["iterations"], rf["iterations"], np.abs(xf - xr).max())
+    assert rr["iterations"] == rf["iterations"] == 3
+    np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
+
+
 def test_fused_ilu0_needs_the_multicolour_schedule(ctx):
     ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 8)
     b = np.ones(len(ro) - 1)
@@ -364,6 +431,8 @@ def test_fused_ilu0_needs_the_multicolour_schedule(ctx):
         aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, ilu_schedule=0)
     _, r = aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, ilu_schedule=1)
     assert r["status"] == 1
+    with pytest.raises(aa.AlienHipError):
+        aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_FILU0, mode=aa.MODE_FUSED, filu_solver_niter=0)
 
 
 @pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
