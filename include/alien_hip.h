@@ -276,12 +276,12 @@ int alien_hip_ilu0_solve_f64(alien_hip_ctx* ctx, int nrows, const int* row_offse
 enum { ALIEN_HIP_CG = 0, ALIEN_HIP_BICGS = 1 };
 /* Preconditioners per mode: REFERENCE and REFERENCE_FUTURE run all of them;
  * FUSED / GRAPH / MIXED run Diag, ILU0 (ilu_schedule = 1 only: the colour
- * classes of the permuted system, one launch per colour and sweep) and LEARNED
- * with both solvers, and Chebyshev with CG. */
+ * classes of the permuted system, one launch per colour and sweep), FILU0
+ * (filu_solver_niter >= 1) and LEARNED with both solvers, and Chebyshev with CG. */
 enum { ALIEN_HIP_PREC_DIAG = 0, ALIEN_HIP_PREC_CHEB = 1, ALIEN_HIP_PREC_ILU0 = 2,
        ALIEN_HIP_PREC_NONE = 3,
        ALIEN_HIP_PREC_LEARNED = 4, /* M = G G^T from the graph network (FUSED/GRAPH/MIXED modes) */
-       ALIEN_HIP_PREC_FILU0 = 5, /* FILU0Preconditioner (REFERENCE mode): fixed-point ILU(0) factor,
+       ALIEN_HIP_PREC_FILU0 = 5, /* FILU0Preconditioner: fixed-point ILU(0) factor,
                                     Jacobi triangular sweeps */
        ALIEN_HIP_PREC_NEUMANN = 6 /* NeumannPolyPreconditioner (REFERENCE mode): poly_factor,
                                      poly_order, poly_max_iter */ };
