@@ -1008,7 +1008,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       q.lu64 = lus.fill<double>(ctx->stream, A, ilu->luValues(), lus.lbv64, lus.ubv64, lus.d64);
     };
     if (p->precond == ALIEN_HIP_PREC_ILU0) {
-      if (!colours || colours->size() < 2) throw Error(ALIEN_HIP_EINVAL, "fused ILU0: no colour classes");
+      if (!colours || colours->empty()) throw Error(ALIEN_HIP_EINVAL, "fused ILU0: no colour classes");
       ilu.reset(new Alien::ILU0Preconditioner<Alg>(alg, A));
       ilu->init();
       phase("ILU0 factorisation");

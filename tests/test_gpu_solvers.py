@@ -424,6 +424,58 @@ def test_fused_filu0_iterates_equal_reference_mode(ctx, case):
     np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
 
 
+def small_spd(n, seed):
+    """Symmetric M-matrix of n rows: a chain plus random symmetric couplings,
+    diagonal = 1 + the row's absolute off-diagonal sum."""
+    rng = np.random.default_rng(seed)
+    A = np.zeros((n, n))
+    for i in range(n - 1):
+        A[i, i + 1] = A[i + 1, i] = -rng.uniform(0.1, 1.0)
+    for _ in range(n):
+        i, j = rng.integers(0, max(n, 1), 2)
+        if i != j:
+            A[i, j] = A[j, i] = -rng.uniform(0.1, 1.0)
+    np.fill_diagonal(A, 1.0 + np.abs(A).sum(axis=1))
+    ro, cols, vals = [0], [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        cols += nz.tolist()
+        vals += A[i, nz].tolist()
+        ro.append(len(cols))
+    return np.array(ro, np.int32), np.array(cols, np.int32), np.array(vals, np.float64)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 65, 130])
+@pytest.mark.parametrize("prec", ["ilu0", "filu0"])
+@pytest.mark.parametrize("solver", [aa.CG, aa.BICGS])
+def test_fused_ilu_small_and_ragged_systems(ctx, n, prec, solver):
+    """Fused multicolour ILU0 / FILU0 on systems of 0, 1, 2 rows and of row
+    counts that leave a partial last 64-row bundle: same status and iteration
+    count (+-2) and the same solution as the REFERENCE mode; the empty system
+    is a null right-hand side (status 1, no iteration). CG-FILU0 stalls from 63
+    rows on (the sweeps make M nonsymmetric) in both modes alike: same
+    iteration cap, same recursive residual."""
+    ro, cols, vals = small_spd(n, seed=n + 1)
+    b = orc.spmv(ro, cols, vals, aa.uniform01(3, n)) if n else np.zeros(0)
+    kw = dict(solver=solver, tol=1e-10, max_iter=500)
+    kw.update(dict(precond=aa.PREC_ILU0, ilu_schedule=1) if prec == "ilu0" else
+              dict(precond=aa.PREC_FILU0, filu_factor_niter=1, filu_solver_niter=2))
+    say("case small system", n, prec, solver)
+    xr, rr = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    for mode in (aa.MODE_FUSED, aa.MODE_GRAPH):
+        x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, **kw)
+        say("  ", mode, r["iterations"], r["status"], "reference", rr["iterations"], rr["status"])
+        assert r["status"] == rr["status"]
+        assert abs(r["iterations"] - rr["iterations"]) <= 2
+        if not rr["status"]:
+            assert solver == aa.CG and prec == "filu0"
+            assert r["value"] == pytest.approx(rr["value"], rel=1e-6)
+        elif n:
+            assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+        else:
+            assert r["iterations"] == 0
+
+
 def test_fused_ilu0_needs_the_multicolour_schedule(ctx):
     ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 8)
     b = np.ones(len(ro) - 1)
