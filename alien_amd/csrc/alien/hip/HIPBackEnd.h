@@ -197,8 +197,22 @@ class HIPMatrix {
   void updateValues(const double* vals) {
     if (h_vals != m_vals.data())
       throw alien_hip::Error(ALIEN_HIP_EINVAL, "updateValues: the matrix reads the caller's arrays");
+    d_vals.upload(vals, m_nnz, m_ctx->stream);
+    valuesChanged();
+  }
+  // New values gathered on the device, vals[k] = src[map[k]] (a permuted copy of
+  // another matrix whose device values src holds)
+  void updateValuesGather(const int* d_map, const double* d_src) {
+    if (h_vals != m_vals.data())
+      throw alien_hip::Error(ALIEN_HIP_EINVAL, "updateValues: the matrix reads the caller's arrays");
+    alien_hip::launch_gather_f64(m_ctx->stream, m_nnz, d_map, d_src, d_vals.data());
+    alien_hip::check_launch("gather values");
+    valuesChanged();
+  }
+
+ private:
+  void valuesChanged() {
     hipStream_t st = m_ctx->stream;
-    d_vals.upload(vals, m_nnz, st);
     m_host_vals_stale = true;
     if (!d_bv.empty()) {
       if (m_rbdia)
@@ -225,6 +239,7 @@ class HIPMatrix {
     ALIEN_HIP_CHECK(hipStreamSynchronize(st));
   }
 
+ public:
   // host bundle offsets of the row-bundled layout (after ensureRB / ensureRBCSR)
   const std::vector<int>& bundleOffsets() const { return m_bo; }
 

@@ -151,4 +151,15 @@ void launch_axpy_dot(hipStream_t st, RedWork w, int n, T a, const T* x, T* y, do
 ALIEN_HIP_INST_BLAS1(double)
 ALIEN_HIP_INST_BLAS1(float)
 
+// y[k] = x[idx[k]]: the values of a permuted copy of a matrix
+__global__ void __launch_bounds__(kBlock) k_gather_f64(long n, const int* __restrict__ idx,
+                                                        const double* __restrict__ x, double* __restrict__ y) {
+  const long i = blockIdx.x * (long)kBlock + threadIdx.x;
+  if (i < n) y[i] = x[idx[i]];
+}
+void launch_gather_f64(hipStream_t st, long n, const int* idx, const double* x, double* y) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_gather_f64, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n, idx, x, y);
+}
+
 } // namespace alien_hip

@@ -23,6 +23,8 @@ void launch_finalize_bicg(hipStream_t, RedWork w, long nblk, int nslot, int op, 
 
 // ---- BLAS-1 (blas1.hip) ----
 template <typename T> void launch_copy(hipStream_t, int n, const T* x, T* y);
+// y[k] = x[idx[k]], k < n
+void launch_gather_f64(hipStream_t, long n, const int* idx, const double* x, double* y);
 template <typename T> void launch_axpy(hipStream_t, int n, T a, const T* x, T* y);
 template <typename T> void launch_scal(hipStream_t, int n, T a, T* x);
 template <typename T> void launch_assign(hipStream_t, int n, T a, T* x);

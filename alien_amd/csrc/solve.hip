@@ -1289,6 +1289,37 @@ struct alien_hip_matrix {
   alien_hip_ctx* ctx = nullptr;
   alien_hip::HIPMatrix A;
   bool diag_ok = false; // every row stores its diagonal entry
+  // multicolour ILU0: the colour-permuted matrix P A P^T, built by the first
+  // such solve and kept with A (its values gathered from A's on each update)
+  std::unique_ptr<alien_hip::HIPMatrix> PA;
+  std::vector<int> perm, cptr;
+  alien_hip::DeviceArray<int> d_map; // PA's entry k = A's entry d_map[k]
+
+  void build_coloured() {
+    const int n = A.getNRows();
+    const int* ro = A.kcol();
+    const int* cols = A.cols();
+    perm = alien_hip::colour_permutation(n, ro, cols, cptr);
+    // permute the entry indices, then read the values through them
+    std::vector<double> idx(A.getNnz());
+    for (size_t k = 0; k < idx.size(); ++k) idx[k] = (double)k;
+    std::vector<int> pro, pcols;
+    std::vector<double> pidx;
+    alien_hip::permute_csr(n, ro, cols, idx.data(), perm, pro, pcols, pidx);
+    std::vector<int> map(pidx.size());
+    std::vector<double> pvals(pidx.size());
+    const double* v = A.hostValues();
+    for (size_t k = 0; k < map.size(); ++k) {
+      map[k] = (int)pidx[k];
+      pvals[k] = v[map[k]];
+    }
+    std::unique_ptr<alien_hip::HIPMatrix> P(new alien_hip::HIPMatrix);
+    Alien::CSRHostView view{n, pro.data(), pcols.data(), pvals.data()};
+    P->build(ctx, view, ALIEN_HIP_LAYOUT_CSR, /*borrow_host=*/false);
+    d_map.upload(map.data(), map.size(), ctx->stream);
+    ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    PA = std::move(P);
+  }
 };
 
 extern "C" int alien_hip_matrix_create(alien_hip_ctx* ctx, int n, const int* ro, const int* cols,
@@ -1317,6 +1348,7 @@ extern "C" int alien_hip_matrix_update_values(alien_hip_matrix* m, const double*
   if (!m || (!vals && m->A.getNnz() > 0)) return fail(ALIEN_HIP_EINVAL, "matrix_update_values: NULL argument");
   try {
     m->A.updateValues(vals);
+    if (m->PA) m->PA->updateValuesGather(m->d_map.data(), m->A.d_vals.data());
   } catch (const Error& e) {
     return fail(e.code, e.what());
   } catch (const std::exception& e) {
@@ -1338,19 +1370,39 @@ extern "C" int alien_hip_solve(alien_hip_ctx* ctx, const alien_hip_solve_params*
     validate_params(p);
     if (p->precond != ALIEN_HIP_PREC_NONE && !m->diag_ok)
       throw Error(ALIEN_HIP_EINVAL, "CSR: missing diagonal entry");
-    if (p->precond == ALIEN_HIP_PREC_ILU0 && p->ilu_schedule == 1)
-      throw Error(ALIEN_HIP_EINVAL, "solve: multicolour ILU0 permutes the system; use alien_hip_solve_csr");
     const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
     const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
-    HIPMatrix& A = m->A;
-    ctx->reserve_for(A.getNRows());
+    const bool coloured = p->precond == ALIEN_HIP_PREC_ILU0 && p->ilu_schedule == 1;
+    const int n = m->A.getNRows();
+    ctx->reserve_for(n);
     EventTimer tm(ctx->stream);
     tm.start();
     SetupTrace phase(ctx);
+    // multicolour ILU0 solves P A P^T (P x) = P b on the kept permuted matrix
+    if (coloured && !m->PA) {
+      m->build_coloured();
+      phase("colour permutation");
+    }
+    HIPMatrix& A = coloured ? *m->PA : m->A;
+    std::vector<double> pb, px;
+    const double* b_in = bh;
+    double* x_io = xh;
+    if (coloured) {
+      pb.resize(n);
+      px.resize(n);
+      for (int r = 0; r < n; ++r) {
+        pb[r] = bh[m->perm[r]];
+        px[r] = xh[m->perm[r]];
+      }
+      b_in = pb.data();
+      x_io = px.data();
+    }
     if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
     else A.setSpmvLayout(p->layout);
     phase("row-bundled layout");
-    solve_on(ctx, p, A, bh, xh, res, tm, phase);
+    solve_on(ctx, p, A, b_in, x_io, res, tm, phase, coloured ? &m->cptr : nullptr);
+    if (coloured)
+      for (int r = 0; r < n; ++r) xh[m->perm[r]] = px[r];
     ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   } catch (const Error& e) {
     return fail(e.code, e.what());

@@ -351,9 +351,10 @@ typedef struct {
  * structure (host array of nnz values) and refills every layout already built;
  * alien_hip_solve runs alien_hip_solve_csr's solve on it (row-bundled layouts are
  * built at the first solve that needs them and kept). Multicolour ILU0
- * (ilu_schedule = 1) permutes the system and is refused here (EINVAL): use
- * alien_hip_solve_csr. A matrix is used with contexts of the device it was
- * created on, one solve at a time. */
+ * (ilu_schedule = 1) builds the colour-permuted matrix at its first solve and
+ * keeps it; matrix_update_values then gathers its values on the device. A
+ * matrix is used with contexts of the device it was created on, one solve at
+ * a time. */
 typedef struct alien_hip_matrix alien_hip_matrix;
 int alien_hip_matrix_create(alien_hip_ctx* ctx, int nrows, const int* row_offset_host, const int* cols_host,
                             const double* vals_host, alien_hip_matrix** out);

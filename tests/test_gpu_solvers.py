@@ -566,6 +566,13 @@ KEPT_CASES = [
                                              layout=aa.LAYOUT_BELLPACK, max_iter=3000)),
     ("bicgstab jacobi fused", dict(solver=aa.BICGS, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED)),
     ("bicgstab ilu0 reference", dict(solver=aa.BICGS, precond=aa.PREC_ILU0, mode=aa.MODE_REFERENCE)),
+    # multicolour ILU0: the kept colour-permuted matrix, its values gathered from the update
+    ("cg ilu0 multicolour graph", dict(solver=aa.CG, precond=aa.PREC_ILU0, ilu_schedule=1, mode=aa.MODE_GRAPH)),
+    ("cg ilu0 multicolour mixed", dict(solver=aa.CG, precond=aa.PREC_ILU0, ilu_schedule=1, mode=aa.MODE_MIXED)),
+    ("bicgstab ilu0 multicolour reference", dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=1,
+                                                 mode=aa.MODE_REFERENCE)),
+    ("bicgstab filu0 graph", dict(solver=aa.BICGS, precond=aa.PREC_FILU0, mode=aa.MODE_GRAPH,
+                                  filu_factor_niter=1, filu_solver_niter=2)),
 ]
 
 
@@ -605,9 +612,9 @@ def test_kept_matrix_refuses_what_it_cannot_run(ctx):
     b = np.ones(len(ro) - 1)
     with aa.Matrix(ctx, ro, cols, vals) as M:
         with pytest.raises(aa.AlienHipError):
-            M.solve(b, solver=aa.CG, precond=aa.PREC_ILU0, ilu_schedule=1)
-        with pytest.raises(aa.AlienHipError):
             M.solve(b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED, tol=-1.0)
+        with pytest.raises(aa.AlienHipError):
+            M.solve(b, solver=aa.CG, precond=aa.PREC_ILU0, ilu_schedule=0, mode=aa.MODE_FUSED)
     # a row without its diagonal: Diag-preconditioned solves are refused, unpreconditioned ones run
     ro2 = np.array([0, 1, 3], np.int32)
     cols2 = np.array([1, 0, 1], np.int32)
