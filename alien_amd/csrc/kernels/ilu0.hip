@@ -167,22 +167,26 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_split_fill(int n, const int* __
 
 // Rows [r0, r1), one wavefront per bundle from bundle b0: out_i = in_i -
 // sum_c a_ic src_c over the part's entries, then (KIND) as is (lower), divided
-// by diag_i (ILU(0) upper) or multiplied by diag_i holding the inverse diagonal
-// (FILU0 upper, pointwiseMult after addUMult). The slots are read in groups of
-// kSweepGroup (columns, then values and the gathered src[c]) before the
-// group's subtractions, which keep their order. Multicolour: src = out (rows
-// of earlier colours); FILU0: src = the previous sweep. With rdot (the last
-// upper sweep of CG) each workgroup also stores its partial of rdot . out at
-// partial[poff + blockIdx.x], so that the launches together leave the
-// partials of r . z for one finalize. Returns at entry once *done.
-constexpr int kSweepGroup = 4;
-enum SweepKind { kSweepLower = 0, kSweepUpperDiv = 1, kSweepUpperInvMul = 2 };
+// by diag_i (ILU(0) upper, and the lower sweep of the last colour, which has no
+// upper entries) or multiplied by diag_i holding the inverse diagonal (FILU0
+// upper, pointwiseMult after addUMult). Columns below split are gathered from
+// ysrc instead of src (the rows of colour 0, whose lower sweep is the identity).
+// The slots are read in groups of kSweepGroup (columns, then values and the
+// gathered values) before the group's subtractions, which keep their order.
+// Multicolour: src = the vector being swept (rows of earlier colours); FILU0:
+// src = the previous sweep. With rdot (the sweeps that produce z in CG) each
+// workgroup also stores its partial of rdot . out at partial[poff +
+// blockIdx.x], so that the launches together leave the partials of r . z for
+// one finalize. Returns at entry once *done.
+constexpr int kSweepGroup = 7;
+enum SweepKind { kSweepLower = 0, kSweepUpperDiv = 1, kSweepUpperInvMul = 2, kSweepLowerDiv = 3 };
 template <typename T, int KIND>
 __global__ void __launch_bounds__(kRowBlock) k_ilu0_split_sweep(int b0, int r0, int r1, const int* __restrict__ bo,
                                                                  const int* __restrict__ bc,
                                                                  const T* __restrict__ bv,
                                                                  const T* __restrict__ diag,
                                                                  const T* __restrict__ in, const T* src, T* out,
+                                                                 const T* __restrict__ ysrc, int split,
                                                                  const int* __restrict__ done,
                                                                  const T* __restrict__ rdot, RedWork red, int poff) {
   if (done && *done) return;
@@ -204,14 +208,16 @@ __global__ void __launch_bounds__(kRowBlock) k_ilu0_split_sweep(int b0, int r0, 
 #pragma unroll
       for (int u = 0; u < kSweepGroup; ++u) {
         v[u] = c[u] >= 0 ? bv[(long)(s0 + j0 + u) * kWave + lane] : T(0);
-        xv[u] = c[u] >= 0 ? src[c[u]] : T(0);
+        xv[u] = c[u] < 0 ? T(0) : c[u] < split ? ysrc[c[u]] : src[c[u]];
       }
 #pragma unroll
       for (int u = 0; u < kSweepGroup; ++u)
         if (c[u] >= 0) acc = acc - v[u] * xv[u];
     }
     if (act) {
-      res = KIND == kSweepUpperDiv ? acc / diag[row] : KIND == kSweepUpperInvMul ? diag[row] * acc : acc;
+      res = (KIND == kSweepUpperDiv || KIND == kSweepLowerDiv) ? acc / diag[row]
+            : KIND == kSweepUpperInvMul                         ? diag[row] * acc
+                                                                : acc;
       out[row] = res;
     }
   }
@@ -240,26 +246,47 @@ int ilu0_split_partials(int ncolours, const int* cptr_host) {
   return tot;
 }
 
+// Colour 0 comes first in the permuted order, so its rows have no strictly-
+// lower entries (t = y there: no launch, later gathers of those rows read y),
+// and the last colour has no strictly-upper entries (x = t / u_ii: its lower
+// sweep divides and writes x directly). Launches: lower sweeps of colours 1 ..
+// nc-1, then upper sweeps of colours nc-2 .. 0 (colour 0 reads t = y); one
+// colour: x = y / u_ii.
 template <typename T>
 int launch_ilu0_split_apply(hipStream_t st, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t,
                             T* x, const int* done, const T* rdot, RedWork red) {
+  if (ncolours < 1) return 0;
   int poff = 0;
-  auto sweep = [&](int c, bool upper) {
+  const int split = cptr_host[1]; // rows of colour 0
+  auto geom = [&](int c, int& b0, int& nblk) {
     const int r0 = cptr_host[c], r1 = cptr_host[c + 1];
-    if (r1 <= r0) return;
-    const int b0 = r0 / kWave, b1 = (r1 - 1) / kWave;
-    const int nblk = (b1 - b0 + 1 + kRowWaves - 1) / kRowWaves;
-    if (upper) {
-      hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepUpperDiv>), dim3(nblk), dim3(kRowBlock), 0, st, b0, r0, r1,
-                         F.ubo, F.ubc, F.ubv, F.diag, (const T*)t, (const T*)x, x, done, rdot, red, poff);
-      poff += nblk;
-    } else {
-      hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepLower>), dim3(nblk), dim3(kRowBlock), 0, st, b0, r0, r1, F.lbo,
-                         F.lbc, F.lbv, F.diag, y, (const T*)t, t, done, (const T*)nullptr, red, 0);
-    }
+    if (r1 <= r0) return false;
+    b0 = r0 / kWave;
+    nblk = ((r1 - 1) / kWave - b0 + 1 + kRowWaves - 1) / kRowWaves;
+    return true;
   };
-  for (int c = 0; c < ncolours; ++c) sweep(c, false);
-  for (int c = ncolours - 1; c >= 0; --c) sweep(c, true);
+  int b0, nblk;
+  const int last = ncolours - 1;
+  for (int c = 1; c < ncolours; ++c) {
+    if (!geom(c, b0, nblk)) continue;
+    if (c == last) {
+      hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepLowerDiv>), dim3(nblk), dim3(kRowBlock), 0, st, b0,
+                         cptr_host[c], cptr_host[c + 1], F.lbo, F.lbc, F.lbv, F.diag, y, (const T*)t, x, y, split,
+                         done, rdot, red, poff);
+      if (rdot) poff += nblk;
+    } else {
+      hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepLower>), dim3(nblk), dim3(kRowBlock), 0, st, b0,
+                         cptr_host[c], cptr_host[c + 1], F.lbo, F.lbc, F.lbv, F.diag, y, (const T*)t, t, y, split,
+                         done, (const T*)nullptr, red, 0);
+    }
+  }
+  for (int c = ncolours == 1 ? 0 : last - 1; c >= 0; --c) {
+    if (!geom(c, b0, nblk)) continue;
+    hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepUpperDiv>), dim3(nblk), dim3(kRowBlock), 0, st, b0,
+                       cptr_host[c], cptr_host[c + 1], F.ubo, F.ubc, F.ubv, F.diag, c == 0 ? y : (const T*)t,
+                       (const T*)x, x, y, 0, done, rdot, red, poff);
+    if (rdot) poff += nblk;
+  }
   return poff;
 }
 
@@ -277,7 +304,7 @@ int launch_filu0_split_apply(hipStream_t st, int n, int niter, ILUSplit<T> F, co
   for (int it = 0; it < niter; ++it) {
     T* out = ((niter - 1 - it) % 2 == 0) ? t1 : t2;
     hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepLower>), g, dim3(kRowBlock), 0, st, 0, 0, n, F.lbo, F.lbc, F.lbv,
-                       F.diag, y, src, out, done, (const T*)nullptr, red, 0);
+                       F.diag, y, src, out, (const T*)nullptr, 0, done, (const T*)nullptr, red, 0);
     src = out;
   }
   // upper: src_0 = mx = t1, outputs alternate t2 / z ending in z
@@ -286,7 +313,8 @@ int launch_filu0_split_apply(hipStream_t st, int n, int niter, ILUSplit<T> F, co
     T* out = ((niter - 1 - it) % 2 == 0) ? z : t2;
     const bool last = it == niter - 1;
     hipLaunchKernelGGL((k_ilu0_split_sweep<T, kSweepUpperInvMul>), g, dim3(kRowBlock), 0, st, 0, 0, n, F.ubo, F.ubc,
-                       F.ubv, F.diag, (const T*)t1, src, out, done, last ? rdot : (const T*)nullptr, red, 0);
+                       F.ubv, F.diag, (const T*)t1, src, out, (const T*)nullptr, 0, done,
+                       last ? rdot : (const T*)nullptr, red, 0);
     src = out;
   }
   return (rdot && niter > 0) ? (int)g.x : 0;
