@@ -1241,6 +1241,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
     const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
     EventTimer tm(ctx->stream);
+    tm.start(); // set-up time includes the colouring and the permutation
 
     // multicolour ILU0: solve the permuted system P A P^T (P x) = P b
     std::vector<int> perm;
@@ -1260,8 +1261,8 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       b_in = pb.data(); x_io = px.data();
     }
 
-    tm.start();
     SetupTrace phase(ctx);
+    if (!perm.empty()) phase("colouring + permutation");
     Alien::CSRHostView view{n, Aro, Acols, Avals};
     HIPMatrix A(ctx, view, (fused || mixed) ? ALIEN_HIP_LAYOUT_CSR : p->layout, /*borrow_host=*/true);
     phase("matrix upload");
