@@ -4,8 +4,11 @@
 
 #include "../../include/alien_hip.h"
 
+#include <algorithm>
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace alien_hip {
@@ -37,6 +40,38 @@ int host_color(int nrows, const int* ro, const int* cols, int* color);
 void group_by_level(int nrows, const int* level, int nlevels, int* ptr, int* rows);
 
 constexpr int kBundle = 64; // rows per RB-CSR bundle = one wavefront
+
+// Host array left uninitialised (no zero fill): its pages are first touched by
+// whichever threads write it.
+template <typename T>
+struct HostBuf {
+  std::unique_ptr<T[]> p;
+  size_t count = 0;
+  void alloc(size_t m) {
+    p.reset(m ? new T[m] : nullptr);
+    count = m;
+  }
+  T* data() { return p.get(); }
+  const T* data() const { return p.get(); }
+  size_t size() const { return count; }
+  T& operator[](size_t i) { return p[i]; }
+  const T& operator[](size_t i) const { return p[i]; }
+};
+
+// fn(begin, end) over contiguous ranges of [0, n) on up to 16 host threads
+// (one range, the caller's thread, below 64 Ki items)
+template <typename F>
+void parallel_ranges(long n, F fn) {
+  const long hw = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+  const long nt = std::min(hw, std::max(1L, n / 65536));
+  if (nt <= 1) {
+    fn(0L, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (long t = 0; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+  for (auto& x : th) x.join();
+}
 int rbcsr_bundle_offsets(int nrows, const int* ro, int* bundle_off);
 // RB-DIA structure of the entries [kb(r), ke(r)) of each row (NULL kb/ke: whole
 // rows): bundle offsets (nb + 1), per-slot column offsets and row masks.

@@ -3,6 +3,7 @@
 #include "../alien_hip_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <iterator>
 #include <vector>
@@ -98,9 +99,47 @@ int host_levels(int nrows, const int* ro, const int* cols, const int* dcol, int 
   return nl;
 }
 
+// True when every stored (i, j) has a stored (j, i) (rows sorted ascending;
+// an unsorted row can only make the answer false).
+static bool pattern_symmetric(int nrows, const int* ro, const int* cols) {
+  std::vector<char> ok(16, 1);
+  std::atomic<int> slot{0};
+  parallel_ranges(nrows, [&](long a, long b) {
+    const int me = slot++;
+    for (long i = a; i < b && ok[me]; ++i)
+      for (int k = ro[i]; k < ro[i + 1]; ++k) {
+        const int j = cols[k];
+        if (j == i) continue;
+        if (!std::binary_search(cols + ro[j], cols + ro[j + 1], (int)i)) {
+          ok[me] = 0;
+          break;
+        }
+      }
+  });
+  return std::all_of(ok.begin(), ok.end(), [](char c) { return c != 0; });
+}
+
 // First-fit greedy colouring, rows in natural order, over the union of the
-// stored pattern and its transpose.
+// stored pattern and its transpose. At row i only rows j < i are coloured, so
+// only those neighbours count: with a symmetric pattern they are row i's own
+// entries of column < i and the transpose is not built.
 int host_color(int nrows, const int* ro, const int* cols, int* color) {
+  if (pattern_symmetric(nrows, ro, cols)) {
+    std::vector<int> mark; // mark[c] == i + 1 <=> colour c is taken by a neighbour of i
+    int nc = 0;
+    for (int i = 0; i < nrows; ++i) {
+      if ((int)mark.size() < nc + 1) mark.resize(nc + 1, 0);
+      for (int k = ro[i]; k < ro[i + 1]; ++k) {
+        const int j = cols[k];
+        if (j < i) mark[color[j]] = i + 1;
+      }
+      int c = 0;
+      while (c < nc && mark[c] == i + 1) ++c;
+      color[i] = c;
+      nc = std::max(nc, c + 1);
+    }
+    return nc;
+  }
   // transpose pattern (CSC of the strictly off-diagonal part)
   std::vector<int> tcount(nrows + 1, 0);
   for (int i = 0; i < nrows; ++i)

@@ -80,24 +80,35 @@ void validate_csr(int n, const int* ro, const int* cols, bool need_diag) {
 // Symmetric permutation P A P^T with rows (and columns) renumbered so that
 // new row r is old row perm[r]; columns re-sorted ascending inside each row.
 void permute_csr(int n, const int* ro, const int* cols, const double* vals, const std::vector<int>& perm,
-                 std::vector<int>& pro, std::vector<int>& pcols, std::vector<double>& pvals) {
-  std::vector<int> inv(n);
-  for (int r = 0; r < n; ++r) inv[perm[r]] = r;
+                 std::vector<int>& pro, HostBuf<int>& pcols, HostBuf<double>& pvals) {
+  HostBuf<int> inv;
+  inv.alloc(n);
+  parallel_ranges(n, [&](long a, long b) {
+    for (long r = a; r < b; ++r) inv[perm[r]] = (int)r;
+  });
   pro.assign(n + 1, 0);
   for (int r = 0; r < n; ++r) pro[r + 1] = pro[r] + (ro[perm[r] + 1] - ro[perm[r]]);
-  pcols.resize(ro[n]);
-  pvals.resize(ro[n]);
-  std::vector<std::pair<int, double>> row;
-  for (int r = 0; r < n; ++r) {
-    const int o = perm[r];
-    row.clear();
-    for (int k = ro[o]; k < ro[o + 1]; ++k) row.emplace_back(inv[cols[k]], vals[k]);
-    std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
-    for (size_t q = 0; q < row.size(); ++q) {
-      pcols[pro[r] + q] = row[q].first;
-      pvals[pro[r] + q] = row[q].second;
+  pcols.alloc(ro[n]);
+  pvals.alloc(ro[n]);
+  // rows are independent: host threads over row ranges, insertion sort for short rows
+  parallel_ranges(n, [&](long a, long b) {
+    std::vector<std::pair<int, double>> row;
+    for (long r = a; r < b; ++r) {
+      const int o = perm[r];
+      row.clear();
+      for (int k = ro[o]; k < ro[o + 1]; ++k) row.emplace_back(inv[cols[k]], vals[k]);
+      if (row.size() <= 32) {
+        for (size_t q = 1; q < row.size(); ++q)
+          for (size_t t = q; t > 0 && row[t].first < row[t - 1].first; --t) std::swap(row[t], row[t - 1]);
+      } else {
+        std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      }
+      for (size_t q = 0; q < row.size(); ++q) {
+        pcols[pro[r] + q] = row[q].first;
+        pvals[pro[r] + q] = row[q].second;
+      }
     }
-  }
+  });
 }
 
 // rows sorted by colour, natural order inside a colour; cptr[c] = first permuted
@@ -1245,8 +1256,10 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
 
     // multicolour ILU0: solve the permuted system P A P^T (P x) = P b
     std::vector<int> perm;
-    std::vector<int> pro, pcols;
-    std::vector<double> pvals, pb(n), px(n);
+    std::vector<int> pro;
+    HostBuf<int> pcols;
+    HostBuf<double> pvals;
+    std::vector<double> pb(n), px(n);
     const int* Aro = ro;
     const int* Acols = cols;
     const double* Avals = vals;
@@ -1302,22 +1315,28 @@ struct alien_hip_matrix {
     const int* cols = A.cols();
     perm = alien_hip::colour_permutation(n, ro, cols, cptr);
     // permute the entry indices, then read the values through them
-    std::vector<double> idx(A.getNnz());
-    for (size_t k = 0; k < idx.size(); ++k) idx[k] = (double)k;
-    std::vector<int> pro, pcols;
-    std::vector<double> pidx;
+    const long nnz = A.getNnz();
+    alien_hip::HostBuf<double> idx, pidx, pvals;
+    alien_hip::HostBuf<int> pcols, map;
+    idx.alloc(nnz);
+    alien_hip::parallel_ranges(nnz, [&](long a, long b) {
+      for (long k = a; k < b; ++k) idx[k] = (double)k;
+    });
+    std::vector<int> pro;
     alien_hip::permute_csr(n, ro, cols, idx.data(), perm, pro, pcols, pidx);
-    std::vector<int> map(pidx.size());
-    std::vector<double> pvals(pidx.size());
+    map.alloc(nnz);
+    pvals.alloc(nnz);
     const double* v = A.hostValues();
-    for (size_t k = 0; k < map.size(); ++k) {
-      map[k] = (int)pidx[k];
-      pvals[k] = v[map[k]];
-    }
+    alien_hip::parallel_ranges(nnz, [&](long a, long b) {
+      for (long k = a; k < b; ++k) {
+        map[k] = (int)pidx[k];
+        pvals[k] = v[map[k]];
+      }
+    });
     std::unique_ptr<alien_hip::HIPMatrix> P(new alien_hip::HIPMatrix);
     Alien::CSRHostView view{n, pro.data(), pcols.data(), pvals.data()};
     P->build(ctx, view, ALIEN_HIP_LAYOUT_CSR, /*borrow_host=*/false);
-    d_map.upload(map.data(), map.size(), ctx->stream);
+    d_map.upload(map.data(), (size_t)nnz, ctx->stream);
     ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     PA = std::move(P);
   }

@@ -156,6 +156,27 @@ def test_levels_and_colors_match_oracle():
     assert nc == onc == 2
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_colouring_of_nonsymmetric_and_symmetric_patterns(seed):
+    """Greedy colouring over the pattern and its transpose: a nonsymmetric
+    pattern (transpose built) and its symmetrised version (own lower entries
+    only) both equal the oracle's colouring, and no two coupled rows share a
+    colour."""
+    rng = np.random.default_rng(seed)
+    n = 300
+    A = (rng.random((n, n)) < 0.02) | np.eye(n, dtype=bool)
+    for M in (A, A | A.T):
+        ro = np.concatenate([[0], np.cumsum(M.sum(1))]).astype(np.int32)
+        cols = np.nonzero(M)[1].astype(np.int32)
+        c, nc = aa.host_color(ro, cols)
+        oc, onc = orc.color(ro, cols)
+        np.testing.assert_array_equal(c, oc)
+        assert nc == onc
+        i, j = np.nonzero(M)
+        off = i != j
+        assert np.all(c[i[off]] != c[j[off]])
+
+
 def test_rbcsr_bundle_offsets():
     ro, cols, _ = aa.gen_csr(aa.GEN_POISSON7, 10)
     bo = aa.rbcsr_bundle_offsets(ro)
