@@ -27,9 +27,32 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
   typedef HIPVector VectorType;
 
   ILU0Preconditioner(AlgebraType& alg, const MatrixType& A, ITraceMng* = nullptr) : m_alg(alg), m_A(A) {}
+  // Colour classes of a colour-permuted matrix (colour c = rows [cptr[c],
+  // cptr[c+1]), no two coupled rows in one class): used by init() as the
+  // schedules, the lower DAG's in increasing colour order, the upper DAG's in
+  // decreasing order, in place of the level sets.
+  void setColourSchedule(const std::vector<int>& cptr) { m_colours = cptr; }
   void init() {
     const int n = m_A.getNRows();
     hipStream_t st = m_alg.stream();
+    if (!m_colours.empty()) {
+      const int nc = (int)m_colours.size() - 1;
+      m_nl = m_nu = nc;
+      m_lptr = m_colours;
+      std::vector<int> rows(n);
+      for (int i = 0; i < n; ++i) rows[i] = i;
+      d_lrows.upload(rows.data(), rows.size(), st);
+      m_uptr.assign(nc + 1, 0);
+      int pos = 0;
+      for (int l = 0; l < nc; ++l) {
+        const int c = nc - 1 - l;
+        for (int r = m_colours[c]; r < m_colours[c + 1]; ++r) rows[pos++] = r;
+        m_uptr[l + 1] = pos;
+      }
+      d_urows.upload(rows.data(), rows.size(), st);
+      factorize(st);
+      return;
+    }
     std::vector<int> level(n);
     m_nl = alien_hip::host_levels(n, m_A.kcol(), m_A.cols(), m_A.dcol(), 0, level.data());
     m_lptr.resize(m_nl + 1);
@@ -40,6 +63,9 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
     m_uptr.resize(m_nu + 1);
     alien_hip::group_by_level(n, level.data(), m_nu, m_uptr.data(), rows.data());
     d_urows.upload(rows.data(), rows.size(), st);
+    factorize(st);
+  }
+  void factorize(hipStream_t st) {
     d_lu.resize(m_A.getNnz());
     ALIEN_HIP_CHECK(hipMemcpyAsync(d_lu.data(), m_A.d_vals.data(), sizeof(double) * m_A.getNnz(),
                                    hipMemcpyDeviceToDevice, st));
@@ -66,7 +92,7 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
   AlgebraType& m_alg;
   const MatrixType& m_A;
   int m_nl = 0, m_nu = 0;
-  std::vector<int> m_lptr, m_uptr;
+  std::vector<int> m_lptr, m_uptr, m_colours;
   alien_hip::DeviceArray<int> d_lrows, d_urows;
   alien_hip::DeviceArray<double> d_lu;
   VectorType m_t;

@@ -958,6 +958,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
     }
     case ALIEN_HIP_PREC_ILU0: {
       Alien::ILU0Preconditioner<Alg> P(alg, A);
+      if (colours && !colours->empty()) P.setColourSchedule(*colours);
       P.init();
       res->setup_s = tm.stop();
       run_reference(alg, p, P, A, b, x, res, tm);
@@ -1021,6 +1022,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
     if (p->precond == ALIEN_HIP_PREC_ILU0) {
       if (!colours || colours->empty()) throw Error(ALIEN_HIP_EINVAL, "fused ILU0: no colour classes");
       ilu.reset(new Alien::ILU0Preconditioner<Alg>(alg, A));
+      ilu->setColourSchedule(*colours);
       ilu->init();
       phase("ILU0 factorisation");
       ps.kind = kPrecILU0;
@@ -1253,6 +1255,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
     EventTimer tm(ctx->stream);
     tm.start(); // set-up time includes the colouring and the permutation
+    SetupTrace phase(ctx);
 
     // multicolour ILU0: solve the permuted system P A P^T (P x) = P b
     std::vector<int> perm;
@@ -1274,7 +1277,6 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
       b_in = pb.data(); x_io = px.data();
     }
 
-    SetupTrace phase(ctx);
     if (!perm.empty()) phase("colouring + permutation");
     Alien::CSRHostView view{n, Aro, Acols, Avals};
     HIPMatrix A(ctx, view, (fused || mixed) ? ALIEN_HIP_LAYOUT_CSR : p->layout, /*borrow_host=*/true);
