@@ -101,6 +101,36 @@ __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, cons
   }
 }
 
+// One step of the Neumann polynomial apply (NeumannPolyPreconditioner.h:85-175,
+// the algebra calls of its solve): yout = (yin + (-f) (A yin)) + x, the last
+// step scaled by f; with rdot the last step also stores the partials of
+// rdot . yout. Returns at entry once *done.
+template <typename T, bool LAST>
+__global__ void __launch_bounds__(kRowBlock) k_neumann_step(int n, RBMat<T> A, const T* __restrict__ x,
+                                                          const T* __restrict__ yin, T* __restrict__ yout, T f,
+                                                          const int* __restrict__ done, const T* __restrict__ rdot,
+                                                          RedWork w) {
+  if (done && *done) return;
+  __shared__ double lds4[kRowBlock / kWave];
+  double s = 0.0;
+  const BundleWalk bw = bundle_walk(n);
+  for (int b = bw.first; b < bw.end; b += bw.stride) {
+    const T t = rb_sum<T>(A, b, [&](int c) { return yin[c]; });
+    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+    if (row < n) {
+      T v = yin[row] + (-f) * t;
+      v = v + T(1) * x[row];
+      if (LAST) v = f * v;
+      yout[row] = v;
+      if (LAST && rdot) s += (double)rdot[row] * (double)v;
+    }
+  }
+  if (LAST && rdot) {
+    double v[1] = {block_sum<kRowBlock>(s, lds4)};
+    store_partials<1>(v, w);
+  }
+}
+
 // Jacobi apply for the Diag preconditioner: z = invD .* r, rho1 = r . z
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_diag_apply(int n, const T* __restrict__ r, const T* __restrict__ invd,
@@ -368,6 +398,23 @@ void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin,
 }
 
 template <typename T>
+long launch_neumann_apply(hipStream_t s, RedWork w, int n, RBMat<T> A, int order, double f, const T* x, T* tmp,
+                          T* z, const int* done, const T* rdot) {
+  const dim3 g = bgrid<T>(n), blk(kRowBlock);
+  const T* yin = x;
+  for (int i = 0; i < order; ++i) {
+    T* out = ((order - 1 - i) % 2 == 0) ? z : tmp;
+    if (i == order - 1)
+      hipLaunchKernelGGL((k_neumann_step<T, true>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rdot, w);
+    else
+      hipLaunchKernelGGL((k_neumann_step<T, false>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, (const T*)nullptr,
+                         w);
+    yin = out;
+  }
+  return (rdot && order > 0) ? rblocks<T>(n) : 0;
+}
+
+template <typename T>
 void launch_diag_apply(hipStream_t s, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st) {
   hipLaunchKernelGGL(k_diag_apply<T>, egrid(n), dim3(kBlock), 0, s, n, r, invd, z, st, w);
   launch_finalize(s, w, ept_blocks(n), 1, kFinRho1, nullptr, st);
@@ -431,6 +478,8 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
   template void launch_cheb_step<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, T*,      \
                                     const T*, double, double, bool, bool, CGState*);                    \
   template void launch_diag_apply<T>(hipStream_t, RedWork, int, const T*, const T*, T*, CGState*);      \
+  template long launch_neumann_apply<T>(hipStream_t, RedWork, int, RBMat<T>, int, double, const T*, T*, T*,  \
+                                        const int*, const T*);                                            \
   template void launch_cg_p<T>(hipStream_t, int, const T*, T*, CGState*, int);                       \
   template void launch_factor_apply<T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,  \
                                        CGState*);                                                        \

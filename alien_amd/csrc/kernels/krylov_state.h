@@ -91,6 +91,12 @@ void launch_cheb_step(hipStream_t, RedWork w, int n, RBMat<T> A, const T* xin, c
                       const T* invd, double a, double b, bool first, bool last, CGState* st);
 template <typename T>
 void launch_diag_apply(hipStream_t, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st);
+// Neumann polynomial apply z = f sum_{k<=order} (I - f A)^k x in order >= 1 row
+// kernels (tmp: workspace); with rdot the last leaves the partials of rdot . z,
+// whose count is returned (0 without rdot). Returns early once *done.
+template <typename T>
+long launch_neumann_apply(hipStream_t, RedWork w, int n, RBMat<T> A, int order, double f, const T* x, T* tmp, T* z,
+                          const int* done, const T* rdot);
 template <typename T>
 void launch_cg_p(hipStream_t, int n, const T* z, T* p, CGState* st, int first);
 template <typename T>

@@ -157,7 +157,7 @@ void run_reference(Alg& alg, const alien_hip_solve_params* p, Precond& P, const 
 }
 
 // ---- fused CG ------------------------------------------------------------------------------
-enum PrecKind { kPrecDiag = 0, kPrecCheb = 1, kPrecFactor = 2, kPrecILU0 = 3, kPrecFILU0 = 4 };
+enum PrecKind { kPrecDiag = 0, kPrecCheb = 1, kPrecFactor = 2, kPrecILU0 = 3, kPrecFILU0 = 4, kPrecNeumann = 5 };
 struct PrecondSpec {
   PrecKind kind = kPrecDiag;
   int order = 0;
@@ -174,6 +174,7 @@ struct PrecondSpec {
   // inverse diagonal) and the Jacobi sweeps per part
   std::vector<int> cptr;
   int filu_iters = 0;
+  double neumann_factor = 0; // Neumann: f of z = f sum_{k<=order} (I - f A)^k r
   ILUSplit<double> lu64{};
   ILUSplit<float> lu32{};
   template <typename T> RBMat<T> g() const;
@@ -269,7 +270,8 @@ struct FusedCG {
     if (P.kind != kPrecFactor) z.resize(n);
     if (P.kind == kPrecCheb) cp.resize(n);
     if (P.kind == kPrecCheb && P.order > 1) { y0.resize(n); y1.resize(n); }
-    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0) tmp.resize(n);
+    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0 || P.kind == kPrecNeumann)
+      tmp.resize(n);
     if (P.kind == kPrecFILU0) tmp2.resize(n);
     if (P.kind == kPrecILU0) ctx->reserve_red(ilu0_split_partials((int)P.cptr.size() - 1, P.cptr.data()));
     st.resize(1);
@@ -301,6 +303,12 @@ struct FusedCG {
     if (P.kind == kPrecFILU0) {
       const int np = launch_filu0_split_apply<T>(s, n, P.filu_iters, P.lu<T>(), rin, tmp.data(), tmp2.data(), zout,
                                                  &st.data()->done, rin, ctx->red());
+      launch_finalize(s, ctx->red(), np, 1, kFinRho1, nullptr, st.data());
+      return;
+    }
+    if (P.kind == kPrecNeumann) {
+      const long np = launch_neumann_apply<T>(s, ctx->red(), n, A, P.order, P.neumann_factor, rin, tmp.data(), zout,
+                                              &st.data()->done, rin);
       launch_finalize(s, ctx->red(), np, 1, kFinRho1, nullptr, st.data());
       return;
     }
@@ -604,10 +612,12 @@ struct FusedBiCG {
   FusedBiCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
   : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
     if (P.kind == kPrecCheb)
-      throw Error(ALIEN_HIP_EINVAL, "fused BiCGStab implements the Diag, ILU0, FILU0 and learned preconditioners");
+      throw Error(ALIEN_HIP_EINVAL,
+                  "fused BiCGStab implements the Diag, ILU0, FILU0, Neumann and learned preconditioners");
     ctx->reserve_for(n);
     for (auto* a : {&r, &r0, &p, &phat, &v, &sv, &shat, &t}) a->resize(n);
-    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0) tmp.resize(n);
+    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0 || P.kind == kPrecNeumann)
+      tmp.resize(n);
     if (P.kind == kPrecFILU0) tmp2.resize(n);
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(BiCGState), hipHostMallocDefault));
@@ -624,6 +634,9 @@ struct FusedBiCG {
     else if (P.kind == kPrecFILU0)
       launch_filu0_split_apply<T>(ctx->stream, n, P.filu_iters, P.lu<T>(), in, tmp.data(), tmp2.data(), out,
                                   done_flag(), nullptr, ctx->red());
+    else if (P.kind == kPrecNeumann)
+      launch_neumann_apply<T>(ctx->stream, ctx->red(), n, A, P.order, P.neumann_factor, in, tmp.data(), out,
+                              done_flag(), nullptr);
     else
       launch_factor_apply_flag<T>(ctx->stream, n, P.g<T>(), P.gt<T>(), in, tmp.data(), out, done_flag());
   }
@@ -880,13 +893,16 @@ void validate_params(const alien_hip_solve_params* p) {
   const bool bicg = p->solver == ALIEN_HIP_BICGS;
   const bool ilu = p->precond == ALIEN_HIP_PREC_ILU0;
   const bool filu = p->precond == ALIEN_HIP_PREC_FILU0;
+  const bool neumann = p->precond == ALIEN_HIP_PREC_NEUMANN;
   if ((fused || mixed) && !bicg && p->precond != ALIEN_HIP_PREC_CHEB && p->precond != ALIEN_HIP_PREC_DIAG &&
-      !ilu && !filu && !learned)
-    throw Error(ALIEN_HIP_EINVAL,
-                "fused/graph/mixed modes implement CG with the Diag, Chebyshev, ILU0, FILU0 or learned preconditioner");
-  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !ilu && !filu && !learned)
-    throw Error(ALIEN_HIP_EINVAL,
-                "fused/graph/mixed modes implement BiCGStab with the Diag, ILU0, FILU0 or learned preconditioner");
+      !ilu && !filu && !neumann && !learned)
+    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement CG with the Diag, Chebyshev, ILU0, FILU0, "
+                                  "Neumann or learned preconditioner");
+  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !ilu && !filu && !neumann && !learned)
+    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement BiCGStab with the Diag, ILU0, FILU0, Neumann "
+                                  "or learned preconditioner");
+  if ((fused || mixed) && neumann && p->poly_order < 1)
+    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed Neumann needs poly_order >= 1");
   if ((fused || mixed) && filu && p->filu_solver_niter < 1)
     throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed FILU0 needs filu_solver_niter >= 1");
   if ((fused || mixed) && ilu && p->ilu_schedule != 1)
@@ -895,8 +911,6 @@ void validate_params(const alien_hip_solve_params* p) {
     throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
   if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
   if (p->stall_window < 0) throw Error(ALIEN_HIP_EINVAL, "solve_csr: stall_window < 0");
-  if (p->precond == ALIEN_HIP_PREC_NEUMANN && (fused || mixed))
-    throw Error(ALIEN_HIP_EINVAL, "Neumann runs in the REFERENCE mode");
   if (p->precond == ALIEN_HIP_PREC_FILU0 && (p->filu_factor_niter < 0 || p->filu_solver_niter < 0))
     throw Error(ALIEN_HIP_EINVAL, "FILU0: negative iteration counts");
   if (p->factor_storage != 0 && p->factor_storage != 1)
@@ -1059,6 +1073,14 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       }
       phase("FILU0 split factor");
     }
+    if (p->precond == ALIEN_HIP_PREC_NEUMANN) {
+      Alien::NeumannPolyPreconditioner<Alg> np(alg, A, p->poly_factor, p->poly_order, p->poly_max_iter);
+      np.init(); // the factor (estimated when poly_factor = 0)
+      ps.kind = kPrecNeumann;
+      ps.order = p->poly_order;
+      ps.neumann_factor = np.factor();
+      phase("Neumann factor");
+    }
     int stall_window = 0;
     if (learned) {
       gnn::SystemView v;
@@ -1208,6 +1230,8 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
           p64.kind = kPrecFILU0;
           p64.filu_iters = ps.filu_iters;
           filu_fp64(p64);
+        } else if (ps.kind == kPrecNeumann) {
+          p64 = ps;
         }
         if (bicg) {
           FusedBiCG<double> F(ctx, n, rb64, invd64, p64);

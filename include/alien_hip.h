@@ -277,13 +277,14 @@ enum { ALIEN_HIP_CG = 0, ALIEN_HIP_BICGS = 1 };
 /* Preconditioners per mode: REFERENCE and REFERENCE_FUTURE run all of them;
  * FUSED / GRAPH / MIXED run Diag, ILU0 (ilu_schedule = 1 only: the colour
  * classes of the permuted system, one launch per colour and sweep), FILU0
- * (filu_solver_niter >= 1) and LEARNED with both solvers, and Chebyshev with CG. */
+ * (filu_solver_niter >= 1), Neumann (poly_order >= 1) and LEARNED with both
+ * solvers, and Chebyshev with CG. */
 enum { ALIEN_HIP_PREC_DIAG = 0, ALIEN_HIP_PREC_CHEB = 1, ALIEN_HIP_PREC_ILU0 = 2,
        ALIEN_HIP_PREC_NONE = 3,
        ALIEN_HIP_PREC_LEARNED = 4, /* M = G G^T from the graph network (FUSED/GRAPH/MIXED modes) */
        ALIEN_HIP_PREC_FILU0 = 5, /* FILU0Preconditioner: fixed-point ILU(0) factor,
                                     Jacobi triangular sweeps */
-       ALIEN_HIP_PREC_NEUMANN = 6 /* NeumannPolyPreconditioner (REFERENCE mode): poly_factor,
+       ALIEN_HIP_PREC_NEUMANN = 6 /* NeumannPolyPreconditioner: poly_factor,
                                      poly_order, poly_max_iter */ };
 /* RB-CSR: row-bundled CSR; RB-DIA: row-bundled diagonal offsets (no column
  * stream; RB-CSR when it would store over 1.25x the slots). In the FUSED, GRAPH

@@ -559,6 +559,34 @@ def test_cg_neumann_matches_oracle(ctx, layout):
     assert r0["status"] == 1
 
 
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED])
+@pytest.mark.parametrize("solver,gen", [(aa.CG, aa.GEN_POISSON7), (aa.BICGS, aa.GEN_CONVDIFF7)])
+def test_fused_neumann_matches_reference_mode(ctx, mode, solver, gen):
+    """The Neumann polynomial inside the fused CG / BiCGStab, one row kernel per
+    term (A y by the bit-exact row sum, then the reference's two axpys and the
+    final scal): FP64 iteration counts within +-2 of the REFERENCE mode and
+    iterates equal to rounding after 3 iterations; the mixed solve reaches the
+    tolerance."""
+    ro, cols, vals = aa.gen_csr(gen, 16)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    f = 0.125 if gen == aa.GEN_POISSON7 else 0.005  # about 1 / |A|_inf (diagonal 93.5 on convection-diffusion)
+    kw = dict(solver=solver, precond=aa.PREC_NEUMANN, poly_factor=f, poly_order=3, tol=1e-8, max_iter=3000)
+    say("case fused Neumann", solver, mode)
+    _, ref = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, poll_every=4, **kw)
+    say("  ", r["iterations"], r["status"], r["true_rel_res"], "reference", ref["iterations"], ref["status"])
+    assert ref["status"] == 1 and r["status"] == 1
+    if mode == aa.MODE_MIXED:
+        assert r["true_rel_res"] < 1.05e-8 and r["iterations"] <= 2 * ref["iterations"] + 20
+    else:
+        assert abs(r["iterations"] - ref["iterations"]) <= 2 and r["true_rel_res"] < 1.5e-8
+        k3 = dict(kw, tol=1e-15, max_iter=3)
+        xr, _ = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **k3)
+        xf, _ = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, **k3)
+        np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
+
+
 KEPT_CASES = [
     ("cg jacobi graph", dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
     ("cg learned mixed", dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, learned=True)),
