@@ -68,8 +68,8 @@ template <typename T, bool FIRST, bool LAST>
 __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, const T* __restrict__ xin,
                                                        const T* __restrict__ yin, T* __restrict__ yout,
                                                        T* __restrict__ p, const T* __restrict__ invd, T a, T bcoef,
-                                                       CGState* st, RedWork w) {
-  if (st->done) return;
+                                                       const int* __restrict__ done, int rho, RedWork w) {
+  if (*done) return;
   __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
   const T* src = FIRST ? xin : yin;
@@ -92,10 +92,10 @@ __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, cons
       }
       p[row] = pn;
       yout[row] = yo;
-      if (LAST) s += (double)xi * (double)yo;
+      if (LAST && rho) s += (double)xi * (double)yo;
     }
   }
-  if (LAST) {
+  if (LAST && rho) {
     double v[1] = {block_sum<kRowBlock>(s, lds4)};
     store_partials<1>(v, w);
   }
@@ -384,17 +384,24 @@ void launch_residual(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x, co
 
 template <typename T>
 void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin, const T* yin, T* yout, T* p,
-                      const T* invd, double a, double b, bool first, bool last, CGState* st) {
+                      const T* invd, double a, double b, bool first, bool last, CGState* st,
+                      const int* done_flag) {
   dim3 g = bgrid<T>(n), blk(kRowBlock);
+  const int* done = st ? &st->done : done_flag;
+  const int rho = st ? 1 : 0;
   if (first && last)
-    hipLaunchKernelGGL((k_cheb_step<T, true, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
+    hipLaunchKernelGGL((k_cheb_step<T, true, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done,
+                       rho, w);
   else if (first)
-    hipLaunchKernelGGL((k_cheb_step<T, true, false>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
+    hipLaunchKernelGGL((k_cheb_step<T, true, false>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done,
+                       rho, w);
   else if (last)
-    hipLaunchKernelGGL((k_cheb_step<T, false, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
+    hipLaunchKernelGGL((k_cheb_step<T, false, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done,
+                       rho, w);
   else
-    hipLaunchKernelGGL((k_cheb_step<T, false, false>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, st, w);
-  if (last) launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
+    hipLaunchKernelGGL((k_cheb_step<T, false, false>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b,
+                       done, rho, w);
+  if (last && st) launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
 }
 
 template <typename T>
@@ -476,7 +483,7 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
 #define ALIEN_HIP_INST_KRYLOV(T)                                                                           \
   template void launch_residual<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, double*);  \
   template void launch_cheb_step<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, T*,      \
-                                    const T*, double, double, bool, bool, CGState*);                    \
+                                    const T*, double, double, bool, bool, CGState*, const int*);        \
   template void launch_diag_apply<T>(hipStream_t, RedWork, int, const T*, const T*, T*, CGState*);      \
   template long launch_neumann_apply<T>(hipStream_t, RedWork, int, RBMat<T>, int, double, const T*, T*, T*,  \
                                         const int*, const T*);                                            \

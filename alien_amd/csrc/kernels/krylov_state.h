@@ -86,9 +86,12 @@ void launch_factor_apply_flag(hipStream_t, int n, RBMat<T> G, RBMat<T> GT, const
 
 template <typename T>
 void launch_residual(hipStream_t, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out);
+// one Chebyshev step; with st (CG) the stop flag is st->done and the last step
+// leaves rho1 = x . y in the state, without (BiCGStab) the flag is done_flag
 template <typename T>
 void launch_cheb_step(hipStream_t, RedWork w, int n, RBMat<T> A, const T* xin, const T* yin, T* yout, T* p,
-                      const T* invd, double a, double b, bool first, bool last, CGState* st);
+                      const T* invd, double a, double b, bool first, bool last, CGState* st,
+                      const int* done_flag = nullptr);
 template <typename T>
 void launch_diag_apply(hipStream_t, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st);
 // Neumann polynomial apply z = f sum_{k<=order} (I - f A)^k x in order >= 1 row

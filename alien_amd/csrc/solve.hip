@@ -605,20 +605,19 @@ struct FusedBiCG {
   RBMat<T> A;
   const T* invd;
   PrecondSpec P;
-  DeviceArray<T> r, r0, p, phat, v, sv, shat, t, tmp, tmp2;
+  DeviceArray<T> r, r0, p, phat, v, sv, shat, t, tmp, tmp2, cp;
   DeviceArray<BiCGState> st;
   BiCGState* h_st = nullptr; // pinned
 
   FusedBiCG(alien_hip_ctx* c, int n_, RBMat<T> A_, const T* invd_, const PrecondSpec& P_)
   : ctx(c), n(n_), A(A_), invd(invd_), P(P_) {
-    if (P.kind == kPrecCheb)
-      throw Error(ALIEN_HIP_EINVAL,
-                  "fused BiCGStab implements the Diag, ILU0, FILU0, Neumann and learned preconditioners");
     ctx->reserve_for(n);
     for (auto* a : {&r, &r0, &p, &phat, &v, &sv, &shat, &t}) a->resize(n);
-    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0 || P.kind == kPrecNeumann)
+    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0 || P.kind == kPrecNeumann ||
+        P.kind == kPrecCheb)
       tmp.resize(n);
-    if (P.kind == kPrecFILU0) tmp2.resize(n);
+    if (P.kind == kPrecFILU0 || P.kind == kPrecCheb) tmp2.resize(n);
+    if (P.kind == kPrecCheb) cp.resize(n);
     st.resize(1);
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(BiCGState), hipHostMallocDefault));
   }
@@ -637,7 +636,17 @@ struct FusedBiCG {
     else if (P.kind == kPrecNeumann)
       launch_neumann_apply<T>(ctx->stream, ctx->red(), n, A, P.order, P.neumann_factor, in, tmp.data(), out,
                               done_flag(), nullptr);
-    else
+    else if (P.kind == kPrecCheb) {
+      // the steps of FusedCG::precond, stop flag from this state, no rho1
+      T* bufs[2] = {tmp.data(), tmp2.data()};
+      const T* yin = in;
+      for (int i = 0; i < P.order; ++i) {
+        T* yout = (i == P.order - 1) ? out : bufs[i % 2];
+        launch_cheb_step<T>(ctx->stream, ctx->red(), n, A, in, yin, yout, cp.data(), invd, P.a[i], P.b[i], i == 0,
+                            i == P.order - 1, nullptr, done_flag());
+        yin = yout;
+      }
+    } else
       launch_factor_apply_flag<T>(ctx->stream, n, P.g<T>(), P.gt<T>(), in, tmp.data(), out, done_flag());
   }
   // one step of the while loop (or the first block, per the state's `first`)
@@ -898,9 +907,10 @@ void validate_params(const alien_hip_solve_params* p) {
       !ilu && !filu && !neumann && !learned)
     throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement CG with the Diag, Chebyshev, ILU0, FILU0, "
                                   "Neumann or learned preconditioner");
-  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && !ilu && !filu && !neumann && !learned)
-    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement BiCGStab with the Diag, ILU0, FILU0, Neumann "
-                                  "or learned preconditioner");
+  if ((fused || mixed) && bicg && p->precond != ALIEN_HIP_PREC_DIAG && p->precond != ALIEN_HIP_PREC_CHEB && !ilu &&
+      !filu && !neumann && !learned)
+    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes implement BiCGStab with the Diag, Chebyshev, ILU0, "
+                                  "FILU0, Neumann or learned preconditioner");
   if ((fused || mixed) && neumann && p->poly_order < 1)
     throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed Neumann needs poly_order >= 1");
   if ((fused || mixed) && filu && p->filu_solver_niter < 1)
@@ -1230,8 +1240,8 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
           p64.kind = kPrecFILU0;
           p64.filu_iters = ps.filu_iters;
           filu_fp64(p64);
-        } else if (ps.kind == kPrecNeumann) {
-          p64 = ps;
+        } else if (ps.kind == kPrecNeumann || ps.kind == kPrecCheb) {
+          p64 = ps; // coefficients only: the FP64 solver applies them to A and invd in FP64
         }
         if (bicg) {
           FusedBiCG<double> F(ctx, n, rb64, invd64, p64);

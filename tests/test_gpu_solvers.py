@@ -587,6 +587,35 @@ def test_fused_neumann_matches_reference_mode(ctx, mode, solver, gen):
         np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
 
 
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED])
+def test_fused_bicgstab_chebyshev_matches_reference_mode(ctx, mode):
+    """The reference's Chebyshev preconditioner inside the fused BiCGStab (the
+    fused CG's Chebyshev steps without the r.z reduction) on the 7-point
+    Laplacian, where BiCGStab-Chebyshev converges (it diverges on the
+    convection-diffusion operator): FP64 iteration counts within +-2 of the
+    REFERENCE mode, iterates equal to rounding after 3 iterations; the mixed
+    solve reaches the tolerance."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 24)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    kw = dict(solver=aa.BICGS, precond=aa.PREC_CHEB, poly_factor=0.5, poly_order=3, poly_max_iter=10, tol=1e-8,
+              max_iter=3000)
+    say("case fused BiCGStab Chebyshev", mode)
+    _, ref = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, poll_every=4, **kw)
+    say("  ", r["iterations"], r["status"], r["true_rel_res"], "reference", ref["iterations"], ref["status"])
+    assert ref["status"] == 1 and r["status"] == 1
+    assert r["cheb_alpha"] == ref["cheb_alpha"] and r["cheb_beta"] == ref["cheb_beta"]
+    if mode == aa.MODE_MIXED:
+        assert r["true_rel_res"] < 1.05e-8 and r["iterations"] <= 2 * ref["iterations"] + 20
+    else:
+        assert abs(r["iterations"] - ref["iterations"]) <= 2 and r["true_rel_res"] < 1.5e-8
+        k3 = dict(kw, tol=1e-15, max_iter=3)
+        xr, _ = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **k3)
+        xf, _ = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, **k3)
+        np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
+
+
 KEPT_CASES = [
     ("cg jacobi graph", dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
     ("cg learned mixed", dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, learned=True)),
@@ -601,6 +630,8 @@ KEPT_CASES = [
                                                  mode=aa.MODE_REFERENCE)),
     ("bicgstab filu0 graph", dict(solver=aa.BICGS, precond=aa.PREC_FILU0, mode=aa.MODE_GRAPH,
                                   filu_factor_niter=1, filu_solver_niter=2)),
+    ("cg neumann graph", dict(solver=aa.CG, precond=aa.PREC_NEUMANN, mode=aa.MODE_GRAPH, poly_factor=0.01,
+                              poly_order=2)),
 ]
 
 
