@@ -65,16 +65,6 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
     d_urows.upload(rows.data(), rows.size(), st);
     factorize(st);
   }
-  void factorize(hipStream_t st) {
-    d_lu.resize(m_A.getNnz());
-    ALIEN_HIP_CHECK(hipMemcpyAsync(d_lu.data(), m_A.d_vals.data(), sizeof(double) * m_A.getNnz(),
-                                   hipMemcpyDeviceToDevice, st));
-    alien_hip::launch_ilu0_factor<double>(st, m_nl, m_lptr.data(), d_lrows.data(), m_A.d_ro.data(),
-                                          m_A.d_cols.data(), m_A.d_dcol.data(), d_lu.data());
-    alien_hip::check_launch("ilu0_factor");
-    m_alg.allocate(AlgebraType::resource(m_A), m_t);
-    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
-  }
   void update() {}
   void solve(AlgebraType& alg, const VectorType& y, VectorType& x) const {
     alien_hip::launch_ilu0_solve<double>(alg.stream(), m_nl, m_lptr.data(), d_lrows.data(), m_nu, m_uptr.data(),
@@ -89,6 +79,16 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
   const double* luValues() const { return d_lu.data(); }
 
  private:
+  void factorize(hipStream_t st) {
+    d_lu.resize(m_A.getNnz());
+    ALIEN_HIP_CHECK(hipMemcpyAsync(d_lu.data(), m_A.d_vals.data(), sizeof(double) * m_A.getNnz(),
+                                   hipMemcpyDeviceToDevice, st));
+    alien_hip::launch_ilu0_factor<double>(st, m_nl, m_lptr.data(), d_lrows.data(), m_A.d_ro.data(),
+                                          m_A.d_cols.data(), m_A.d_dcol.data(), d_lu.data());
+    alien_hip::check_launch("ilu0_factor");
+    m_alg.allocate(AlgebraType::resource(m_A), m_t);
+    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+  }
   AlgebraType& m_alg;
   const MatrixType& m_A;
   int m_nl = 0, m_nu = 0;
