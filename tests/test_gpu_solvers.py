@@ -616,6 +616,56 @@ def test_fused_bicgstab_chebyshev_matches_reference_mode(ctx, mode):
         np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
 
 
+X0_CASES = [
+    ("cg diag graph", aa.CG, dict(precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
+    ("cg chebyshev fused", aa.CG, dict(precond=aa.PREC_CHEB, mode=aa.MODE_FUSED)),
+    ("cg ilu0 multicolour graph", aa.CG, dict(precond=aa.PREC_ILU0, ilu_schedule=1, mode=aa.MODE_GRAPH)),
+    ("cg ilu0 multicolour mixed", aa.CG, dict(precond=aa.PREC_ILU0, ilu_schedule=1, mode=aa.MODE_MIXED)),
+    ("cg neumann graph", aa.CG, dict(precond=aa.PREC_NEUMANN, poly_factor=0.125, poly_order=3,
+                                     mode=aa.MODE_GRAPH)),
+    ("cg learned mixed", aa.CG, dict(precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED)),
+    ("bicgstab diag fused", aa.BICGS, dict(precond=aa.PREC_DIAG, mode=aa.MODE_FUSED)),
+    ("bicgstab ilu0 multicolour graph", aa.BICGS, dict(precond=aa.PREC_ILU0, ilu_schedule=1,
+                                                       mode=aa.MODE_GRAPH)),
+    ("bicgstab filu0 mixed", aa.BICGS, dict(precond=aa.PREC_FILU0, filu_factor_niter=0, filu_solver_niter=3,
+                                            mode=aa.MODE_MIXED)),
+]
+
+
+@pytest.mark.parametrize("name,solver,kw", X0_CASES, ids=[c[0] for c in X0_CASES])
+def test_nonzero_initial_guess(ctx, name, solver, kw):
+    """Solves from a nonzero initial guess (the initial residual b - A x0, the
+    colour permutation of x0, the mixed solver's FP64 start) against the
+    REFERENCE mode from the same x0: same status, iterations within +-2 in
+    FP64 (within 2x + 20 for the mixed solver), the tolerance met on the true
+    residual and the same solution to it. The learned preconditioner has no
+    REFERENCE counterpart: compared with the REFERENCE-mode Jacobi solution."""
+    from oracle import gnn_oracle as go
+    gen = aa.GEN_POISSON7 if solver == aa.CG else aa.GEN_CONVDIFF7
+    ro, cols, vals = aa.gen_csr(gen, 16)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    x0 = 0.5 * aa.uniform01(77, n) - 0.25
+    kw = dict(kw, solver=solver, tol=1e-9, max_iter=3000)
+    if kw["precond"] == aa.PREC_LEARNED:
+        kw["gnn_theta"] = go.init_params(0)
+    ref_kw = dict(kw, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR)
+    if kw["precond"] == aa.PREC_LEARNED:
+        ref_kw = dict(ref_kw, precond=aa.PREC_DIAG)
+        del ref_kw["gnn_theta"]
+    say("case nonzero x0", name)
+    xr, rr = aa.solve_csr(ctx, ro, cols, vals, b, x0=x0.copy(), **ref_kw)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, x0=x0.copy(), **kw)
+    say("  ", r["iterations"], r["status"], r["true_rel_res"], "reference", rr["iterations"], rr["status"])
+    assert rr["status"] == 1 and r["status"] == 1
+    assert r["true_rel_res"] < 1.5e-9
+    if kw["mode"] != aa.MODE_MIXED and kw["precond"] != aa.PREC_LEARNED:
+        assert abs(r["iterations"] - rr["iterations"]) <= 2
+    elif kw["precond"] != aa.PREC_LEARNED:
+        assert r["iterations"] <= 2 * rr["iterations"] + 20
+    assert np.linalg.norm(x - xr) <= 1e-6 * np.linalg.norm(xr)
+
+
 KEPT_CASES = [
     ("cg jacobi graph", dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
     ("cg learned mixed", dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, learned=True)),
