@@ -666,6 +666,22 @@ def test_nonzero_initial_guess(ctx, name, solver, kw):
     assert np.linalg.norm(x - xr) <= 1e-6 * np.linalg.norm(xr)
 
 
+@pytest.mark.parametrize("name,solver,kw", X0_CASES, ids=[c[0] for c in X0_CASES])
+def test_null_rhs_fused_preconditioners(ctx, name, solver, kw):
+    """b = 0 (Iteration's nullRhs): status 1, no iteration, x = 0, for every
+    fused preconditioner and mode."""
+    from oracle import gnn_oracle as go
+    gen = aa.GEN_POISSON7 if solver == aa.CG else aa.GEN_CONVDIFF7
+    ro, cols, vals = aa.gen_csr(gen, 8)
+    n = len(ro) - 1
+    kw = dict(kw, solver=solver, tol=1e-9, max_iter=100)
+    if kw["precond"] == aa.PREC_LEARNED:
+        kw["gnn_theta"] = go.init_params(0)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, np.zeros(n), **kw)
+    assert r["status"] == 1 and r["iterations"] == 0 and r["error"] == 0
+    assert np.all(x == 0)
+
+
 KEPT_CASES = [
     ("cg jacobi graph", dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
     ("cg learned mixed", dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, learned=True)),
