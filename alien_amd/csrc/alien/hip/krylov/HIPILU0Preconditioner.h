@@ -76,6 +76,11 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
 
   int lowerLevels() const { return m_nl; }
   int upperLevels() const { return m_nu; }
+  // the schedules: host level pointers, device row lists
+  const int* lowerPtr() const { return m_lptr.data(); }
+  const int* upperPtr() const { return m_uptr.data(); }
+  const int* lowerRows() const { return d_lrows.data(); }
+  const int* upperRows() const { return d_urows.data(); }
   const double* luValues() const { return d_lu.data(); }
 
  private:

@@ -90,7 +90,9 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_lsolve_level(int count, const i
                                                                const int* __restrict__ cols,
                                                                const int* __restrict__ dcol,
                                                                const T* __restrict__ lu,
-                                                               const T* __restrict__ y, T* __restrict__ t) {
+                                                               const T* __restrict__ y, T* __restrict__ t,
+                                                               const int* __restrict__ done) {
+  if (done && *done) return;
   const int q = blockIdx.x * kBlock + threadIdx.x;
   if (q >= count) return;
   const int i = rows[q];
@@ -107,7 +109,9 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_usolve_level(int count, const i
                                                                const int* __restrict__ cols,
                                                                const int* __restrict__ dcol,
                                                                const T* __restrict__ lu,
-                                                               const T* __restrict__ t, T* __restrict__ x) {
+                                                               const T* __restrict__ t, T* __restrict__ x,
+                                                               const int* __restrict__ done) {
+  if (done && *done) return;
   const int q = blockIdx.x * kBlock + threadIdx.x;
   if (q >= count) return;
   const int i = rows[q];
@@ -334,18 +338,18 @@ void launch_ilu0_factor(hipStream_t st, int nlevels, const int* ptr_host, const 
 template <typename T>
 void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* lrows, int nu,
                        const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,
-                       const T* lu, const T* y, T* t, T* x) {
+                       const T* lu, const T* y, T* t, T* x, const int* done) {
   for (int l = 0; l < nl; ++l) {
     const int c = lptr_host[l + 1] - lptr_host[l];
     if (c > 0)
       hipLaunchKernelGGL(k_ilu0_lsolve_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
-                         lrows + lptr_host[l], ro, cols, dcol, lu, y, t);
+                         lrows + lptr_host[l], ro, cols, dcol, lu, y, t, done);
   }
   for (int l = 0; l < nu; ++l) {
     const int c = uptr_host[l + 1] - uptr_host[l];
     if (c > 0)
       hipLaunchKernelGGL(k_ilu0_usolve_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
-                         urows + uptr_host[l], ro, cols, dcol, lu, t, x);
+                         urows + uptr_host[l], ro, cols, dcol, lu, t, x, done);
   }
 }
 
@@ -354,7 +358,7 @@ void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* 
                                       const int*, T*);                                                    \
   template void launch_ilu0_solve<T>(hipStream_t, int, const int*, const int*, int, const int*,           \
                                      const int*, const int*, const int*, const int*, const T*, const T*,  \
-                                     T*, T*);                                                             \
+                                     T*, T*, const int*);                                                 \
   template void launch_ilu0_split_fill<T>(hipStream_t, int, const int*, const int*, const int*,           \
                                           const double*, const int*, const int*, int*, int*, T*, T*, T*);  \
   template int launch_ilu0_split_apply<T>(hipStream_t, int, const int*, ILUSplit<T>, const T*, T*, T*,    \

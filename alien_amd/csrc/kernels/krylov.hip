@@ -404,6 +404,27 @@ void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin,
   if (last && st) launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
 }
 
+// rho1 = r . z into the CG state (a preconditioner applied by separate launches)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_rz(int n, const T* __restrict__ r, const T* __restrict__ z,
+                                                CGState* st, RedWork w) {
+  if (st->done) return;
+  __shared__ double lds4[4];
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kEPT; ++k) {
+    const long i = EPT_INDEX(k);
+    if (i < n) s += (double)r[i] * (double)z[i];
+  }
+  double v[1] = {block_sum(s, lds4)};
+  store_partials<1>(v, w);
+}
+template <typename T>
+void launch_rz(hipStream_t s, RedWork w, int n, const T* r, const T* z, CGState* st) {
+  hipLaunchKernelGGL(k_rz<T>, egrid(n), dim3(kBlock), 0, s, n, r, z, st, w);
+  launch_finalize(s, w, ept_blocks(n), 1, kFinRho1, nullptr, st);
+}
+
 template <typename T>
 long launch_neumann_apply(hipStream_t s, RedWork w, int n, RBMat<T> A, int order, double f, const T* x, T* tmp,
                           T* z, const int* done, const T* rdot) {
@@ -485,6 +506,7 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
   template void launch_cheb_step<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, const T*, T*, T*,      \
                                     const T*, double, double, bool, bool, CGState*, const int*);        \
   template void launch_diag_apply<T>(hipStream_t, RedWork, int, const T*, const T*, T*, CGState*);      \
+  template void launch_rz<T>(hipStream_t, RedWork, int, const T*, const T*, CGState*);                  \
   template long launch_neumann_apply<T>(hipStream_t, RedWork, int, RBMat<T>, int, double, const T*, T*, T*,  \
                                         const int*, const T*);                                            \
   template void launch_cg_p<T>(hipStream_t, int, const T*, T*, CGState*, int);                       \

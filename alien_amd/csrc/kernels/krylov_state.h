@@ -94,6 +94,9 @@ void launch_cheb_step(hipStream_t, RedWork w, int n, RBMat<T> A, const T* xin, c
                       const int* done_flag = nullptr);
 template <typename T>
 void launch_diag_apply(hipStream_t, RedWork w, int n, const T* r, const T* invd, T* z, CGState* st);
+// rho1 = r . z into the CG state (after a preconditioner applied by separate launches)
+template <typename T>
+void launch_rz(hipStream_t, RedWork w, int n, const T* r, const T* z, CGState* st);
 // Neumann polynomial apply z = f sum_{k<=order} (I - f A)^k x in order >= 1 row
 // kernels (tmp: workspace); with rdot the last leaves the partials of rdot . z,
 // whose count is returned (0 without rdot). Returns early once *done.

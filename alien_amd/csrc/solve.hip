@@ -157,7 +157,29 @@ void run_reference(Alg& alg, const alien_hip_solve_params* p, Precond& P, const 
 }
 
 // ---- fused CG ------------------------------------------------------------------------------
-enum PrecKind { kPrecDiag = 0, kPrecCheb = 1, kPrecFactor = 2, kPrecILU0 = 3, kPrecFILU0 = 4, kPrecNeumann = 5 };
+enum PrecKind {
+  kPrecDiag = 0,
+  kPrecCheb = 1,
+  kPrecFactor = 2,
+  kPrecILU0 = 3,
+  kPrecFILU0 = 4,
+  kPrecNeumann = 5,
+  kPrecILU0Level = 6
+};
+// Natural-order ILU(0) on its level-set schedules (the reference's ILU0 bit for bit):
+// one launch per level and sweep over the CSR of A and the factor in CSR order.
+struct LevelILU {
+  int nl = 0, nu = 0;
+  const int* lptr = nullptr; // host
+  const int* uptr = nullptr; // host
+  const int* lrows = nullptr;
+  const int* urows = nullptr;
+  const int* ro = nullptr;
+  const int* cols = nullptr;
+  const int* dcol = nullptr;
+  const double* lu64 = nullptr;
+  const float* lu32 = nullptr;
+};
 struct PrecondSpec {
   PrecKind kind = kPrecDiag;
   int order = 0;
@@ -173,6 +195,7 @@ struct PrecondSpec {
   // factor in the solver's precision; FILU0: the split factor (diag = its
   // inverse diagonal) and the Jacobi sweeps per part
   std::vector<int> cptr;
+  LevelILU lev;
   int filu_iters = 0;
   double neumann_factor = 0; // Neumann: f of z = f sum_{k<=order} (I - f A)^k r
   ILUSplit<double> lu64{};
@@ -247,6 +270,13 @@ template <> RBMat<double> PrecondSpec::g<double>() const { return g64; }
 template <> RBMat<double> PrecondSpec::gt<double>() const { return gt64; }
 template <> RBMat<float> PrecondSpec::g<float>() const { return g32; }
 template <> RBMat<float> PrecondSpec::gt<float>() const { return gt32; }
+template <typename T>
+void level_ilu_apply(hipStream_t s, const LevelILU& L, const T* y, T* t, T* x, const int* done) {
+  const T* lu;
+  if constexpr (std::is_same<T, float>::value) lu = L.lu32;
+  else lu = L.lu64;
+  launch_ilu0_solve<T>(s, L.nl, L.lptr, L.lrows, L.nu, L.uptr, L.urows, L.ro, L.cols, L.dcol, lu, y, t, x, done);
+}
 template <> ILUSplit<double> PrecondSpec::lu<double>() const { return lu64; }
 template <> ILUSplit<float> PrecondSpec::lu<float>() const { return lu32; }
 
@@ -270,7 +300,8 @@ struct FusedCG {
     if (P.kind != kPrecFactor) z.resize(n);
     if (P.kind == kPrecCheb) cp.resize(n);
     if (P.kind == kPrecCheb && P.order > 1) { y0.resize(n); y1.resize(n); }
-    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0 || P.kind == kPrecNeumann)
+    if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0 || P.kind == kPrecNeumann ||
+        P.kind == kPrecILU0Level)
       tmp.resize(n);
     if (P.kind == kPrecFILU0) tmp2.resize(n);
     if (P.kind == kPrecILU0) ctx->reserve_red(ilu0_split_partials((int)P.cptr.size() - 1, P.cptr.data()));
@@ -310,6 +341,11 @@ struct FusedCG {
       const long np = launch_neumann_apply<T>(s, ctx->red(), n, A, P.order, P.neumann_factor, rin, tmp.data(), zout,
                                               &st.data()->done, rin);
       launch_finalize(s, ctx->red(), np, 1, kFinRho1, nullptr, st.data());
+      return;
+    }
+    if (P.kind == kPrecILU0Level) {
+      level_ilu_apply<T>(s, P.lev, rin, tmp.data(), zout, &st.data()->done);
+      launch_rz<T>(s, ctx->red(), n, rin, zout, st.data());
       return;
     }
     const int m = P.order;
@@ -614,7 +650,7 @@ struct FusedBiCG {
     ctx->reserve_for(n);
     for (auto* a : {&r, &r0, &p, &phat, &v, &sv, &shat, &t}) a->resize(n);
     if (P.kind == kPrecFactor || P.kind == kPrecILU0 || P.kind == kPrecFILU0 || P.kind == kPrecNeumann ||
-        P.kind == kPrecCheb)
+        P.kind == kPrecCheb || P.kind == kPrecILU0Level)
       tmp.resize(n);
     if (P.kind == kPrecFILU0 || P.kind == kPrecCheb) tmp2.resize(n);
     if (P.kind == kPrecCheb) cp.resize(n);
@@ -636,6 +672,8 @@ struct FusedBiCG {
     else if (P.kind == kPrecNeumann)
       launch_neumann_apply<T>(ctx->stream, ctx->red(), n, A, P.order, P.neumann_factor, in, tmp.data(), out,
                               done_flag(), nullptr);
+    else if (P.kind == kPrecILU0Level)
+      level_ilu_apply<T>(ctx->stream, P.lev, in, tmp.data(), out, done_flag());
     else if (P.kind == kPrecCheb) {
       // the steps of FusedCG::precond, stop flag from this state, no rho1
       T* bufs[2] = {tmp.data(), tmp2.data()};
@@ -915,8 +953,8 @@ void validate_params(const alien_hip_solve_params* p) {
     throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed Neumann needs poly_order >= 1");
   if ((fused || mixed) && filu && p->filu_solver_niter < 1)
     throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed FILU0 needs filu_solver_niter >= 1");
-  if ((fused || mixed) && ilu && p->ilu_schedule != 1)
-    throw Error(ALIEN_HIP_EINVAL, "fused/graph/mixed modes run ILU0 on the multicolour schedule (ilu_schedule = 1)");
+  if (ilu && p->ilu_schedule != 0 && p->ilu_schedule != 1)
+    throw Error(ALIEN_HIP_EINVAL, "ILU0: ilu_schedule must be 0 (level sets) or 1 (multicolour)");
   if (learned && !(fused || mixed))
     throw Error(ALIEN_HIP_EINVAL, "the learned preconditioner runs in the fused, graph or mixed modes");
   if (learned && !p->gnn_theta) throw Error(ALIEN_HIP_EINVAL, "learned preconditioner: gnn_theta is NULL");
@@ -1043,8 +1081,30 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
     auto ilu_fp64 = [&](PrecondSpec& q) {
       q.lu64 = lus.fill<double>(ctx->stream, A, ilu->luValues(), lus.lbv64, lus.ubv64, lus.d64);
     };
-    if (p->precond == ALIEN_HIP_PREC_ILU0) {
-      if (!colours || colours->empty()) throw Error(ALIEN_HIP_EINVAL, "fused ILU0: no colour classes");
+    DeviceArray<float> lev_lu32;
+    if (p->precond == ALIEN_HIP_PREC_ILU0 && !(colours && !colours->empty())) {
+      // natural order: the level-set factor and sweeps of the REFERENCE mode
+      ilu.reset(new Alien::ILU0Preconditioner<Alg>(alg, A));
+      ilu->init();
+      phase("ILU0 level-set factorisation");
+      ps.kind = kPrecILU0Level;
+      LevelILU& L = ps.lev;
+      L.nl = ilu->lowerLevels();
+      L.nu = ilu->upperLevels();
+      L.lptr = ilu->lowerPtr();
+      L.uptr = ilu->upperPtr();
+      L.lrows = ilu->lowerRows();
+      L.urows = ilu->upperRows();
+      L.ro = A.d_ro.data();
+      L.cols = A.d_cols.data();
+      L.dcol = A.d_dcol.data();
+      L.lu64 = ilu->luValues();
+      if (mixed) {
+        lev_lu32.resize(A.getNnz());
+        launch_demote(ctx->stream, A.getNnz(), ilu->luValues(), lev_lu32.data());
+        L.lu32 = lev_lu32.data();
+      }
+    } else if (p->precond == ALIEN_HIP_PREC_ILU0) {
       ilu.reset(new Alien::ILU0Preconditioner<Alg>(alg, A));
       ilu->setColourSchedule(*colours);
       ilu->init();
@@ -1240,8 +1300,8 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
           p64.kind = kPrecFILU0;
           p64.filu_iters = ps.filu_iters;
           filu_fp64(p64);
-        } else if (ps.kind == kPrecNeumann || ps.kind == kPrecCheb) {
-          p64 = ps; // coefficients only: the FP64 solver applies them to A and invd in FP64
+        } else if (ps.kind == kPrecNeumann || ps.kind == kPrecCheb || ps.kind == kPrecILU0Level) {
+          p64 = ps; // the FP64 solver applies them to A, invd and the FP64 factor
         }
         if (bicg) {
           FusedBiCG<double> F(ctx, n, rb64, invd64, p64);

@@ -476,15 +476,46 @@ def test_fused_ilu_small_and_ragged_systems(ctx, n, prec, solver):
             assert r["iterations"] == 0
 
 
-def test_fused_ilu0_needs_the_multicolour_schedule(ctx):
+def test_fused_preconditioner_parameter_checks(ctx):
     ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 8)
     b = np.ones(len(ro) - 1)
     with pytest.raises(aa.AlienHipError):
-        aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, ilu_schedule=0)
-    _, r = aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, ilu_schedule=1)
-    assert r["status"] == 1
+        aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, ilu_schedule=2)
     with pytest.raises(aa.AlienHipError):
         aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_FILU0, mode=aa.MODE_FUSED, filu_solver_niter=0)
+    with pytest.raises(aa.AlienHipError):
+        aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_NEUMANN, mode=aa.MODE_FUSED, poly_order=0)
+    for sched in (0, 1):
+        _, r = aa.solve_csr(ctx, ro, cols, vals, b, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, ilu_schedule=sched)
+        assert r["status"] == 1
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED])
+@pytest.mark.parametrize("case", ["cg_poisson", "bicgstab_convdiff"])
+def test_fused_levelset_ilu0_matches_reference_mode(ctx, mode, case):
+    """Natural-order ILU(0), the reference's own preconditioner (level-set
+    factor and sweeps, bit-exact with the sequential code), inside the fused
+    CG / BiCGStab: one launch per level and sweep (graph nodes in GRAPH mode),
+    r.z by a separate pass. FP64 iteration counts within +-2 of the REFERENCE
+    mode and iterates equal to rounding after 3 iterations; the mixed solve
+    reaches the tolerance."""
+    (ro, cols, vals), solver, tol = ilu_case(case)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    kw = dict(solver=solver, precond=aa.PREC_ILU0, ilu_schedule=0, tol=tol, max_iter=3000)
+    say("case fused level-set ILU0", case, mode)
+    _, ref = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **kw)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, poll_every=4, **kw)
+    say("  ", r["iterations"], r["status"], r["true_rel_res"], "reference", ref["iterations"], ref["status"])
+    assert ref["status"] == 1 and r["status"] == 1 and r["error"] == 0
+    if mode == aa.MODE_MIXED:
+        assert r["true_rel_res"] < 1.05 * tol and r["iterations"] <= 2 * ref["iterations"] + 20
+    else:
+        assert abs(r["iterations"] - ref["iterations"]) <= 2 and r["true_rel_res"] < 1.5 * tol
+        k3 = dict(kw, tol=1e-15, max_iter=3)
+        xr, _ = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_REFERENCE, layout=aa.LAYOUT_CSR, **k3)
+        xf, _ = aa.solve_csr(ctx, ro, cols, vals, b, mode=mode, **k3)
+        np.testing.assert_allclose(xf, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
 
 
 @pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
@@ -629,6 +660,7 @@ X0_CASES = [
                                                        mode=aa.MODE_GRAPH)),
     ("bicgstab filu0 mixed", aa.BICGS, dict(precond=aa.PREC_FILU0, filu_factor_niter=0, filu_solver_niter=3,
                                             mode=aa.MODE_MIXED)),
+    ("bicgstab ilu0 level-set graph", aa.BICGS, dict(precond=aa.PREC_ILU0, ilu_schedule=0, mode=aa.MODE_GRAPH)),
 ]
 
 
