@@ -1151,6 +1151,10 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       ps.neumann_factor = np.factor();
       phase("Neumann factor");
     }
+    // Level-set ILU0 launches one kernel per level and sweep: a graph of k
+    // iterations would hold k x 2 x levels nodes and cost more to instantiate
+    // than it saves, so those solves launch directly.
+    const bool capture = ps.kind != kPrecILU0Level;
     int stall_window = 0;
     if (learned) {
       gnn::SystemView v;
@@ -1203,7 +1207,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
     }
     if (!mixed) {
       RBMat<double> rb = A.rb64();
-      const bool graph = p->mode == ALIEN_HIP_MODE_GRAPH;
+      const bool graph = p->mode == ALIEN_HIP_MODE_GRAPH && capture;
       DeviceArray<double> x0;
       if (learned) {
         x0.resize(n);
@@ -1275,13 +1279,13 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       const double delta = p->inner_tol > 0 ? p->inner_tol : 0.1;
       auto run = bicg ? mixed_reliable_bicg : mixed_reliable_cg;
       MixedResult m = run(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol, p->max_iter, delta,
-                          p->poll_every, true, stall_window);
+                          p->poll_every, capture, stall_window);
       if (m.stalled) {
         // Jacobi from the last reliable update, for the remaining iterations
         res->fallback = 1;
         const int used = m.it, ups = m.updates;
         m = run(ctx, n, rb64, rb32, invd32.data(), PrecondSpec{}, b.data(), x.data(), p->tol,
-                std::max(0, p->max_iter - used), delta, p->poll_every, true, 0);
+                std::max(0, p->max_iter - used), delta, p->poll_every, capture, 0);
         m.it += used;
         m.updates += ups;
       }
@@ -1305,14 +1309,14 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
         }
         if (bicg) {
           FusedBiCG<double> F(ctx, n, rb64, invd64, p64);
-          const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, true, p->poll_every, nullptr);
+          const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, capture, p->poll_every, nullptr);
           m.it = used + s.it;
           m.status = s.status;
           m.error = s.breakdown ? 1 : 0;
           m.rel = std::sqrt(s.rr / alg.dot(b, b));
         } else {
           FusedCG<double> F(ctx, n, rb64, invd64, p64);
-          CGState s = F.solve(b.data(), x.data(), false, p->tol, left, true, p->poll_every, nullptr);
+          CGState s = F.solve(b.data(), x.data(), false, p->tol, left, capture, p->poll_every, nullptr);
           finish_breakdown(F, s, &m.error);
           m.it = used + s.it;
           m.status = s.status;

@@ -274,10 +274,10 @@ int alien_hip_ilu0_solve_f64(alien_hip_ctx* ctx, int nrows, const int* row_offse
 /* --kernel hip): examples/test_krylov.cpp:500-665                           */
 /* ------------------------------------------------------------------------ */
 enum { ALIEN_HIP_CG = 0, ALIEN_HIP_BICGS = 1 };
-/* Preconditioners per mode: REFERENCE and REFERENCE_FUTURE run all of them;
- * FUSED / GRAPH / MIXED run Diag, Chebyshev, ILU0 (ilu_schedule = 1 only: the
- * colour classes of the permuted system), FILU0 (filu_solver_niter >= 1),
- * Neumann (poly_order >= 1) and LEARNED with both solvers. */
+/* Preconditioners per mode: REFERENCE and REFERENCE_FUTURE run all but
+ * LEARNED; FUSED / GRAPH / MIXED run all of them with both solvers (ILU0 on
+ * level sets, ilu_schedule = 0, launched directly in GRAPH mode; FILU0 with
+ * filu_solver_niter >= 1; Neumann with poly_order >= 1). */
 enum { ALIEN_HIP_PREC_DIAG = 0, ALIEN_HIP_PREC_CHEB = 1, ALIEN_HIP_PREC_ILU0 = 2,
        ALIEN_HIP_PREC_NONE = 3,
        ALIEN_HIP_PREC_LEARNED = 4, /* M = G G^T from the graph network (FUSED/GRAPH/MIXED modes) */
