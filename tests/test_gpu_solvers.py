@@ -771,7 +771,7 @@ def test_kept_matrix_refuses_what_it_cannot_run(ctx):
         with pytest.raises(aa.AlienHipError):
             M.solve(b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED, tol=-1.0)
         with pytest.raises(aa.AlienHipError):
-            M.solve(b, solver=aa.CG, precond=aa.PREC_ILU0, ilu_schedule=0, mode=aa.MODE_FUSED)
+            M.solve(b, solver=aa.CG, precond=aa.PREC_ILU0, ilu_schedule=2, mode=aa.MODE_FUSED)
     # a row without its diagonal: Diag-preconditioned solves are refused, unpreconditioned ones run
     ro2 = np.array([0, 1, 3], np.int32)
     cols2 = np.array([1, 0, 1], np.int32)
