@@ -714,6 +714,24 @@ def test_null_rhs_fused_preconditioners(ctx, name, solver, kw):
     assert np.all(x == 0)
 
 
+@pytest.mark.parametrize("name,solver,kw", X0_CASES, ids=[c[0] for c in X0_CASES])
+def test_fused_solves_are_deterministic(ctx, name, solver, kw):
+    """The same solve twice gives the same iterates bit for bit: every reduction
+    is a fixed-order two-stage sum (no atomics), whatever the preconditioner."""
+    from oracle import gnn_oracle as go
+    gen = aa.GEN_POISSON7 if solver == aa.CG else aa.GEN_CONVDIFF7
+    ro, cols, vals = aa.gen_csr(gen, 16)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(5, n))
+    kw = dict(kw, solver=solver, tol=1e-9, max_iter=3000)
+    if kw["precond"] == aa.PREC_LEARNED:
+        kw["gnn_theta"] = go.init_params(0)
+    x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, **kw)
+    x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, **kw)
+    assert r1["iterations"] == r2["iterations"] and r1["status"] == r2["status"] == 1
+    np.testing.assert_array_equal(x1, x2)
+
+
 KEPT_CASES = [
     ("cg jacobi graph", dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
     ("cg learned mixed", dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, learned=True)),
