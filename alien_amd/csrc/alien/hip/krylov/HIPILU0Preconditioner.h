@@ -69,7 +69,8 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
   void solve(AlgebraType& alg, const VectorType& y, VectorType& x) const {
     alien_hip::launch_ilu0_solve<double>(alg.stream(), m_nl, m_lptr.data(), d_lrows.data(), m_nu, m_uptr.data(),
                                          d_urows.data(), m_A.d_ro.data(), m_A.d_cols.data(), m_A.d_dcol.data(),
-                                         d_lu.data(), y.data(), const_cast<double*>(m_t.data()), x.data());
+                                         d_lu.data(), y.data(), const_cast<double*>(m_t.data()), x.data(), nullptr,
+                                         d_lptr.data(), d_uptr.data());
     alien_hip::check_launch("ilu0_solve");
   }
   void solve(const VectorType& y, VectorType& x) const { solve(m_alg, y, x); }
@@ -81,10 +82,15 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
   const int* upperPtr() const { return m_uptr.data(); }
   const int* lowerRows() const { return d_lrows.data(); }
   const int* upperRows() const { return d_urows.data(); }
+  // the level pointers on the device (the sweeps merge runs of small levels)
+  const int* lowerPtrDevice() const { return d_lptr.data(); }
+  const int* upperPtrDevice() const { return d_uptr.data(); }
   const double* luValues() const { return d_lu.data(); }
 
  private:
   void factorize(hipStream_t st) {
+    d_lptr.upload(m_lptr.data(), m_lptr.size(), st);
+    d_uptr.upload(m_uptr.data(), m_uptr.size(), st);
     d_lu.resize(m_A.getNnz());
     ALIEN_HIP_CHECK(hipMemcpyAsync(d_lu.data(), m_A.d_vals.data(), sizeof(double) * m_A.getNnz(),
                                    hipMemcpyDeviceToDevice, st));
@@ -98,7 +104,7 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
   const MatrixType& m_A;
   int m_nl = 0, m_nu = 0;
   std::vector<int> m_lptr, m_uptr, m_colours;
-  alien_hip::DeviceArray<int> d_lrows, d_urows;
+  alien_hip::DeviceArray<int> d_lrows, d_urows, d_lptr, d_uptr;
   alien_hip::DeviceArray<double> d_lu;
   VectorType m_t;
 };

@@ -122,6 +122,40 @@ __global__ void __launch_bounds__(kBlock) k_ilu0_usolve_level(int count, const i
   x[i] = v / lu[dk];
 }
 
+// Consecutive small levels [l0, l1) in one workgroup: the levels in order,
+// their rows strided over the workgroup's lanes, a barrier between levels (the
+// rows of level l read rows of earlier levels only). The per-row operations are
+// the per-level kernels'.
+constexpr int kMergeRows = kBlock; // levels of at most one row per lane are merged
+template <typename T, bool UPPER>
+__global__ void __launch_bounds__(kBlock) k_ilu0_sweep_levels(int l0, int l1, const int* __restrict__ ptr,
+                                                               const int* __restrict__ rows,
+                                                               const int* __restrict__ ro,
+                                                               const int* __restrict__ cols,
+                                                               const int* __restrict__ dcol,
+                                                               const T* __restrict__ lu, const T* in, T* out,
+                                                               const int* __restrict__ done) {
+  if (done && *done) return;
+  for (int l = l0; l < l1; ++l) {
+    for (int q = ptr[l] + (int)threadIdx.x; q < ptr[l + 1]; q += kBlock) {
+      const int i = rows[q];
+      if (UPPER) {
+        const int dk = dcol[i];
+        T v = in[i];
+        const int e = ro[i + 1];
+        for (int k = dk + 1; k < e; ++k) v = v - lu[k] * out[cols[k]];
+        out[i] = v / lu[dk];
+      } else {
+        T v = in[i];
+        const int e = dcol[i];
+        for (int k = ro[i]; k < e; ++k) v = v - lu[k] * out[cols[k]];
+        out[i] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- multicolour sweeps over the split factor --------------------------------
 // The factor LU of the colour-permuted matrix is stored as two row-bundled
 // parts (64 rows per bundle, slot (bo[b] + j) * 64 + lane, column -1 = padding)
@@ -338,19 +372,40 @@ void launch_ilu0_factor(hipStream_t st, int nlevels, const int* ptr_host, const 
 template <typename T>
 void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* lrows, int nu,
                        const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,
-                       const T* lu, const T* y, T* t, T* x, const int* done) {
-  for (int l = 0; l < nl; ++l) {
-    const int c = lptr_host[l + 1] - lptr_host[l];
-    if (c > 0)
-      hipLaunchKernelGGL(k_ilu0_lsolve_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
-                         lrows + lptr_host[l], ro, cols, dcol, lu, y, t, done);
-  }
-  for (int l = 0; l < nu; ++l) {
-    const int c = uptr_host[l + 1] - uptr_host[l];
-    if (c > 0)
-      hipLaunchKernelGGL(k_ilu0_usolve_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
-                         urows + uptr_host[l], ro, cols, dcol, lu, t, x, done);
-  }
+                       const T* lu, const T* y, T* t, T* x, const int* done, const int* lptr_dev,
+                       const int* uptr_dev) {
+  // per level one launch, or (with the level pointers on the device) one
+  // single-workgroup launch per run of at least two levels of <= kMergeRows rows
+  auto sweep = [&](bool upper, int nlev, const int* ptr, const int* ptr_dev, const int* rows) {
+    int l = 0;
+    while (l < nlev) {
+      int m = l;
+      if (ptr_dev)
+        while (m < nlev && ptr[m + 1] - ptr[m] <= kMergeRows) ++m;
+      if (m - l >= 2) {
+        if (upper)
+          hipLaunchKernelGGL((k_ilu0_sweep_levels<T, true>), dim3(1), dim3(kBlock), 0, st, l, m, ptr_dev, rows, ro,
+                             cols, dcol, lu, (const T*)t, x, done);
+        else
+          hipLaunchKernelGGL((k_ilu0_sweep_levels<T, false>), dim3(1), dim3(kBlock), 0, st, l, m, ptr_dev, rows, ro,
+                             cols, dcol, lu, y, t, done);
+        l = m;
+        continue;
+      }
+      const int c = ptr[l + 1] - ptr[l];
+      if (c > 0) {
+        if (upper)
+          hipLaunchKernelGGL(k_ilu0_usolve_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
+                             rows + ptr[l], ro, cols, dcol, lu, t, x, done);
+        else
+          hipLaunchKernelGGL(k_ilu0_lsolve_level<T>, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c,
+                             rows + ptr[l], ro, cols, dcol, lu, y, t, done);
+      }
+      ++l;
+    }
+  };
+  sweep(false, nl, lptr_host, lptr_dev, lrows);
+  sweep(true, nu, uptr_host, uptr_dev, urows);
 }
 
 #define ALIEN_HIP_INST_ILU0(T)                                                                            \
@@ -358,7 +413,7 @@ void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* 
                                       const int*, T*);                                                    \
   template void launch_ilu0_solve<T>(hipStream_t, int, const int*, const int*, int, const int*,           \
                                      const int*, const int*, const int*, const int*, const T*, const T*,  \
-                                     T*, T*, const int*);                                                 \
+                                     T*, T*, const int*, const int*, const int*);                         \
   template void launch_ilu0_split_fill<T>(hipStream_t, int, const int*, const int*, const int*,           \
                                           const double*, const int*, const int*, int*, int*, T*, T*, T*);  \
   template int launch_ilu0_split_apply<T>(hipStream_t, int, const int*, ILUSplit<T>, const T*, T*, T*,    \

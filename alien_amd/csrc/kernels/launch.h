@@ -131,6 +131,7 @@ int launch_ilu0_split_apply(hipStream_t, int ncolours, const int* cptr_host, ILU
 template <typename T>
 void launch_ilu0_solve(hipStream_t, int nl, const int* lptr_host, const int* lrows, int nu,
                        const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,
-                       const T* lu, const T* y, T* t, T* x, const int* done = nullptr);
+                       const T* lu, const T* y, T* t, T* x, const int* done = nullptr,
+                       const int* lptr_dev = nullptr, const int* uptr_dev = nullptr);
 
 } // namespace alien_hip

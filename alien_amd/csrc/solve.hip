@@ -174,6 +174,8 @@ struct LevelILU {
   const int* uptr = nullptr; // host
   const int* lrows = nullptr;
   const int* urows = nullptr;
+  const int* lptr_dev = nullptr;
+  const int* uptr_dev = nullptr;
   const int* ro = nullptr;
   const int* cols = nullptr;
   const int* dcol = nullptr;
@@ -275,7 +277,8 @@ void level_ilu_apply(hipStream_t s, const LevelILU& L, const T* y, T* t, T* x, c
   const T* lu;
   if constexpr (std::is_same<T, float>::value) lu = L.lu32;
   else lu = L.lu64;
-  launch_ilu0_solve<T>(s, L.nl, L.lptr, L.lrows, L.nu, L.uptr, L.urows, L.ro, L.cols, L.dcol, lu, y, t, x, done);
+  launch_ilu0_solve<T>(s, L.nl, L.lptr, L.lrows, L.nu, L.uptr, L.urows, L.ro, L.cols, L.dcol, lu, y, t, x, done,
+                       L.lptr_dev, L.uptr_dev);
 }
 template <> ILUSplit<double> PrecondSpec::lu<double>() const { return lu64; }
 template <> ILUSplit<float> PrecondSpec::lu<float>() const { return lu32; }
@@ -1095,6 +1098,8 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       L.uptr = ilu->upperPtr();
       L.lrows = ilu->lowerRows();
       L.urows = ilu->upperRows();
+      L.lptr_dev = ilu->lowerPtrDevice();
+      L.uptr_dev = ilu->upperPtrDevice();
       L.ro = A.d_ro.data();
       L.cols = A.d_cols.data();
       L.dcol = A.d_dcol.data();

@@ -732,6 +732,26 @@ def test_fused_solves_are_deterministic(ctx, name, solver, kw):
     np.testing.assert_array_equal(x1, x2)
 
 
+@pytest.mark.parametrize("case", ["cg_poisson2d", "cg_poisson", "bicgstab_convdiff"])
+def test_levelset_ilu0_iterates_equal_oracle(ctx, case):
+    """Natural-order ILU(0) sweeps (runs of small levels merged into one
+    single-workgroup launch: every level of the 96 x 96 2-D Laplacian, the ends
+    of the 3-D level DAG) against the oracle's sequential ILU(0): three solver
+    iterations in the REFERENCE and FUSED modes give the oracle's iterates to
+    rounding of the dots."""
+    (ro, cols, vals), solver, _ = ilu_case(case)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(11, n))
+    xo, _ = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG if solver == aa.CG else orc.ORC_BICGS,
+                      precond=orc.PREC_ILU0, tol=1e-15, max_iter=3)
+    for mode in (aa.MODE_REFERENCE, aa.MODE_FUSED):
+        x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=solver, precond=aa.PREC_ILU0, ilu_schedule=0, mode=mode,
+                            layout=aa.LAYOUT_CSR, tol=1e-15, max_iter=3)
+        say("case level-set ILU0 vs oracle", case, mode, np.abs(x - xo).max())
+        assert r["iterations"] == 3
+        np.testing.assert_allclose(x, xo, rtol=0, atol=1e-11 * np.abs(xo).max())
+
+
 KEPT_CASES = [
     ("cg jacobi graph", dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
     ("cg learned mixed", dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, learned=True)),
