@@ -11,8 +11,11 @@ HBM. `value` is the mean solve time of the learned-preconditioner,
 mixed-precision solver: M = G G^T from the graph network (trained on the GPU
 first, on small systems of the same generator), FP32 matrix / factor / Krylov
 vectors with FP64 dots, FP64 reliable updates. The plain-FP64 solvers of the
-same system (Jacobi-CG, Chebyshev-CG, multicolour ILU0-CG) and mixed Jacobi-CG
-are timed beside it.
+same system (Jacobi-CG, Chebyshev-CG, multicolour ILU0-CG in the reference's
+call sequence and fused) and the mixed Jacobi-CG and ILU0-CG are timed beside
+it. Other workloads (--workload): convdiff (BiCGStab with ILU0 on level sets and
+multicolour, FILU0, Jacobi), blocks2 (two unknowns per cell), eval (unseen
+systems) and train (the network's training configuration).
 
 The roofline object covers the dominant kernel, the SpMV, timed with HIP
 events on the stream it runs on; `cpu_baseline` times the CPU oracle (a
