@@ -768,6 +768,9 @@ KEPT_CASES = [
                                   filu_factor_niter=1, filu_solver_niter=2)),
     ("cg neumann graph", dict(solver=aa.CG, precond=aa.PREC_NEUMANN, mode=aa.MODE_GRAPH, poly_factor=0.01,
                               poly_order=2)),
+    ("bicgstab ilu0 level-set mixed", dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0,
+                                           mode=aa.MODE_MIXED)),
+    ("bicgstab chebyshev graph", dict(solver=aa.BICGS, precond=aa.PREC_CHEB, mode=aa.MODE_GRAPH, max_iter=300)),
 ]
 
 
