@@ -28,6 +28,7 @@ PREC_DIAG, PREC_CHEB, PREC_ILU0, PREC_NONE, PREC_LEARNED, PREC_FILU0, PREC_NEUMA
 LAYOUT_CSR, LAYOUT_BELLPACK, LAYOUT_RBCSR, LAYOUT_RBDIA = 0, 1, 2, 3
 MODE_REFERENCE, MODE_FUSED, MODE_GRAPH, MODE_MIXED, MODE_REFERENCE_FUTURE = 0, 1, 2, 3, 4
 EINVAL = -22
+NOALLOC_MAX_ROWS = 16777216  # ALIEN_HIP_NOALLOC_MAX_ROWS
 
 
 class SolveParams(C.Structure):
@@ -135,6 +136,8 @@ _SIGS = [
     ("alien_hip_axpy_dot_f64", C.c_int, [c_void_p, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_spmv_dot_csr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_void_p]),
+    ("alien_hip_spmv_dot_csr_f64_noalloc", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                     c_void_p, c_void_p]),
     ("alien_hip_ilu0_factor_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, C.c_int,
                                             c_int_p, c_void_p]),
     ("alien_hip_ilu0_solve_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, C.c_int,

@@ -687,6 +687,64 @@ int alien_hip_spmv_dot_csr_f64(alien_hip_ctx* ctx, int n, const int* ro, const i
   LAUNCHED("spmv_dot_csr_f64");
   API_END
 }
+} // extern "C"
+
+// ---- allocation-free fused SpMV + dot ---------------------------------------------------------
+// The per-workgroup partials of k_spmv_dot_csr (one double per workgroup of kBlock rows) live in
+// a file-scope device array instead of a context's workspace, so a call allocates nothing. Each
+// GPU holds its own copy of the array at its own address; the host-side name `scratch` is only a
+// stand-in for it, so host code uses the address hipGetSymbolAddress returns for the device
+// (taken once per device, on the first call there) and never the name itself.
+namespace {
+constexpr int kNoallocPartials = 65536;  // 512 KiB per GPU
+constexpr int kNoallocMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoallocPartials];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoallocMaxDevices] = {};
+
+double* scratch_on_device(int dev) {
+  if (dev < 0 || dev >= kNoallocMaxDevices) throw Error(ALIEN_HIP_EINVAL, "spmv_dot_csr_f64_noalloc: device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    ALIEN_HIP_CHECK(hipGetDevice(&cur));
+    if (cur != dev) ALIEN_HIP_CHECK(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    ALIEN_HIP_CHECK(e);
+    if (!ptr) throw Error(ALIEN_HIP_EINVAL, "spmv_dot_csr_f64_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+} // namespace
+
+extern "C" {
+int alien_hip_spmv_dot_csr_f64_noalloc(void* stream, int n, const int* ro, const int* cols, const double* vals,
+                                       const double* x, double* y, double* result) {
+  REQUIRE(n >= 0 && result && (n == 0 || (ro && cols && vals && x && y)),
+          "spmv_dot_csr_f64_noalloc: bad arguments");
+  REQUIRE(n <= ALIEN_HIP_NOALLOC_MAX_ROWS,
+          "spmv_dot_csr_f64_noalloc: nrows exceeds ALIEN_HIP_NOALLOC_MAX_ROWS (16777216)");
+  static_assert((long)ALIEN_HIP_NOALLOC_MAX_ROWS <= (long)kNoallocPartials * kBlock,
+                "the scratch array holds one partial per workgroup of kBlock rows");
+  API_BEGIN
+  const hipStream_t st = (hipStream_t)stream;
+  int dev = -1;
+  if (st) {
+    hipDevice_t d = -1;
+    ALIEN_HIP_CHECK(hipStreamGetDevice(st, &d));
+    dev = (int)d;
+  } else {
+    ALIEN_HIP_CHECK(hipGetDevice(&dev));
+  }
+  double* part = scratch_on_device((int)dev);
+  launch_spmv_dot_csr<double>(st, RedWork{part, kNoallocPartials}, n, ro, cols, vals, x, y, result);
+  LAUNCHED("spmv_dot_csr_f64_noalloc");
+  API_END
+}
 
 // ---- ILU0 --------------------------------------------------------------------------------------
 int alien_hip_ilu0_factor_f64(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, const int* dcol,

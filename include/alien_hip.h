@@ -244,6 +244,19 @@ int alien_hip_axpy_dot_f64(alien_hip_ctx* ctx, int n, double a, const double* x,
 /* fused: y = A x (CSR, bit-exact with _seqMult), *result = y . x */
 int alien_hip_spmv_dot_csr_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
                                const double* vals, const double* x, double* y, double* result);
+/* Allocation-free variant of alien_hip_spmv_dot_csr_f64. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). It takes
+ * a HIP stream (NULL = the null stream of the current device) instead of a
+ * context and allocates nothing: the reduction's per-workgroup partials live
+ * in a static 512 KiB device array that each GPU holds until the process ends.
+ * nrows above ALIEN_HIP_NOALLOC_MAX_ROWS is rejected with ALIEN_HIP_EINVAL.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (on
+ * different streams, or from different threads) race and must be ordered by
+ * the caller. Results equal alien_hip_spmv_dot_csr_f64's bit for bit. */
+#define ALIEN_HIP_NOALLOC_MAX_ROWS 16777216
+int alien_hip_spmv_dot_csr_f64_noalloc(void* stream, int nrows, const int* row_offset, const int* cols,
+                                       const double* vals, const double* x, double* y, double* result);
 
 /* ------------------------------------------------------------------------ */
 /* Device kernels: ILU0 on level-set schedules                               */
