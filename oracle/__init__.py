@@ -61,6 +61,8 @@ _SIGS = [
     ("orc_levels_upper", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
     ("orc_color_greedy", C.c_int, [C.c_int, _ip, _ip, _ip]),
     ("orc_solve", C.c_int, [C.POINTER(SolveParams), C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(SolveResult)]),
+    ("orc_cheb_bounds", None, [C.c_int, _ip, _ip, _dp, C.c_double, C.c_int, _dp, _dp]),
+    ("orc_cheb_apply", None, [C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]),
     ("orc_time_cg_iters", C.c_double, [C.c_int, _ip, _ip, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("orc_time_spmv", C.c_double, [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_int]),
 ]
@@ -228,6 +230,21 @@ def solve(ro, cols, vals, b, x0=None, *, solver=ORC_CG, precond=PREC_CHEB, tol=1
     lib().orc_solve(C.byref(p), n, _i(ro), _i(cols), _d(vals), _d(np.ascontiguousarray(b, np.float64)), _d(x),
                     C.byref(r))
     return x, r.as_dict()
+
+
+def cheb_bounds(ro, cols, vals, factor=0.5, max_it=10):
+    """(alpha, beta) of ChebyshevPreconditioner::init (power method, then alpha = beta / factor)."""
+    a, b = C.c_double(), C.c_double()
+    lib().orc_cheb_bounds(len(ro) - 1, _i(ro), _i(cols), _d(vals), factor, max_it, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def cheb_apply(ro, cols, vals, alpha, beta, order, x):
+    """y = M^-1 x by ChebyshevPreconditioner::_algo2 with the given bounds."""
+    y = np.empty(len(ro) - 1)
+    lib().orc_cheb_apply(len(ro) - 1, _i(ro), _i(cols), _d(vals), alpha, beta, order,
+                         _d(np.ascontiguousarray(x, np.float64)), _d(y))
+    return y
 
 
 def filu0_factor(ro, cols, vals, niter):

@@ -813,6 +813,25 @@ int orc_solve(const orc_solve_params* p, int n, const int* ro, const int* cols, 
   return res->error;
 }
 
+void orc_cheb_bounds(int n, const int* ro, const int* cols, const double* vals, double factor, int max_it,
+                     double* alpha, double* beta) {
+  Algebra alg(n, ro, cols, vals, 1);
+  ChebPrec P(alg, factor, 1, max_it);
+  *alpha = P.alpha;
+  *beta = P.beta;
+}
+
+void orc_cheb_apply(int n, const int* ro, const int* cols, const double* vals, double alpha, double beta,
+                    int order, const double* xp, double* yp) {
+  Algebra alg(n, ro, cols, vals, 1);
+  ChebPrec P(alg, 0., order, 0); /* no power iteration: the bounds are set below */
+  P.alpha = alpha;
+  P.beta = beta;
+  Vec x(xp, xp + n), y(n, 0.);
+  P.solve(alg, x, y);
+  std::memcpy(yp, y.data(), sizeof(double) * n);
+}
+
 double orc_time_cg_iters(int n, const int* ro, const int* cols, const double* vals, const double* bptr,
                          int precond, int poly_order, int n_iter, int nthreads) {
   Algebra alg(n, ro, cols, vals, nthreads);

@@ -123,6 +123,15 @@ typedef struct {
 void orc_filu0_factor(int n, const int* ro, const int* cols, const int* dcol, const double* a, int niter,
                       double* lu);
 
+/* ChebyshevPreconditioner::init (ChebyshevPreconditioner.h:73-116): inverse diagonal, power
+ * method for beta (computeBeta, :118-156) and for alpha on the shifted operator (computeAlpha,
+ * :158-202), max_it iterations each, then alpha = beta / factor when factor != 0 (:99-100) */
+void orc_cheb_bounds(int nrows, const int* row_offset, const int* cols, const double* vals, double factor,
+                     int max_it, double* alpha, double* beta);
+/* ChebyshevPreconditioner::_algo2 (:392-434) with the given bounds: y = M^-1 x, y starts at x */
+void orc_cheb_apply(int nrows, const int* row_offset, const int* cols, const double* vals, double alpha,
+                    double beta, int order, const double* x, double* y);
+
 /* x is the initial guess on input and the solution on output */
 int orc_solve(const orc_solve_params* p, int nrows, const int* row_offset, const int* cols,
               const double* vals, const double* b, double* x, orc_solve_result* res);

@@ -453,3 +453,148 @@ def test_factor_preconditioner_pins():
     GT.sort_indices()
     _, re = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_FACTOR, factor=(G, GT), max_iter=10)
     assert re["iterations"] <= 2 and re["status"] == 1
+
+
+# ---- Chebyshev: power method and _algo2 against what the mathematics fixes ------------------------
+def _dense(ro, cols, vals):
+    n = len(ro) - 1
+    return sp.csr_matrix((vals, cols, ro), shape=(n, n)).toarray()
+
+
+@pytest.mark.parametrize("n", [4, 5, 6])
+def test_power_method_reaches_laplacian_extreme_eigenvalues(n):
+    """computeBeta/computeAlpha (ChebyshevPreconditioner.h:118-202) with many iterations:
+    on the n^3 7-point Laplacian D^-1 A = I - adj/6 has lambda = 1 -+ cos(pi/(n+1))
+    at its ends. factor = 0 keeps computeAlpha's result (the shifted power method,
+    converging to lambda_min - beta, plus beta); factor 0.5 overwrites it with beta/0.5."""
+    ro, cols, vals = poisson(n)
+    a, b = orc.cheb_bounds(ro, cols, vals, factor=0.0, max_it=3000)
+    assert b == pytest.approx(1 + np.cos(np.pi / (n + 1)), abs=1e-12)
+    assert a == pytest.approx(1 - np.cos(np.pi / (n + 1)), abs=1e-12)
+    a2, b2 = orc.cheb_bounds(ro, cols, vals, factor=0.5, max_it=3000)
+    assert b2 == b and a2 == b / 0.5  # Q1: alpha = beta / factor (:99-100)
+
+
+def test_power_method_uses_the_inverse_diagonal():
+    """Non-constant diagonal (log-normal coefficients): the bounds are the extreme
+    eigenvalues of D^-1 A (= those of D^-1/2 A D^-1/2), not of A."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 5, params=[1.0, 1.0, 1.0, 1.0], seed=3)
+    A = _dense(ro, cols, vals)
+    dh = np.sqrt(np.diag(A))
+    ev = np.linalg.eigvalsh(A / np.outer(dh, dh))
+    a, b = orc.cheb_bounds(ro, cols, vals, factor=0.0, max_it=1000)
+    assert b == pytest.approx(ev[-1], abs=1e-12)
+    assert a == pytest.approx(ev[0], abs=1e-12)
+    assert abs(b - np.linalg.eigvalsh(A)[-1]) > 1.0
+
+
+def test_cheb_apply_order1_and_order2_closed_forms():
+    """_algo2 starts from y = x (:399), first step alpha = 2/d (:409):
+    order 1: y = x + (2/d) D^-1 (x - A x).
+    order 2, expanded by hand: R_2(t) = 1 - (a0 + a1 + a1 b0) t + a0 a1 t^2 with
+    a0 = 2/d, b0 = a0 c^2/4, a1 = 1/(d - b0), and y = A^-1 x - R_2(D^-1 A)(A^-1 x - x)."""
+    import pin_systems as ps
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 4, params=[1.0, 1.0, 1.0, 1.0], seed=1)
+    A = _dense(ro, cols, vals)
+    x = np.random.default_rng(5).standard_normal(len(ro) - 1)
+    for al, be in ((0.05, 1.9), (3.64, 1.82)):
+        d, c = (be + al) / 2, (be - al) / 2
+        y1 = orc.cheb_apply(ro, cols, vals, al, be, 1, x)
+        np.testing.assert_allclose(y1, x + (2 / d) * (x - A @ x) / np.diag(A), rtol=1e-13, atol=1e-13)
+        a0 = 2 / d
+        b0 = a0 * c * c / 4
+        a1 = 1 / (d - b0)
+        R2 = np.array([1.0, -(a0 + a1 + a1 * b0), a0 * a1])
+        np.testing.assert_allclose(ps.cheb_residual_poly(al, be, 2), R2, rtol=1e-15)
+        ainv = np.linalg.solve(A, x)
+        y2 = orc.cheb_apply(ro, cols, vals, al, be, 2, x)
+        pred = ainv - ps.apply_poly_of_jacobi_operator(A, R2, ainv - x)
+        assert np.abs(y2 - pred).max() <= 1e-12 * np.abs(y2).max()
+
+
+@pytest.mark.parametrize("bounds", [(0.05, 1.9), (3.6392, 1.8196)], ids=["sane", "factor0.5"])
+def test_cheb_apply_is_its_residual_polynomial(bounds):
+    """Orders 1-6 on a log-normal system: y_k = A^-1 x - R_k(D^-1 A)(A^-1 x - x), R_k
+    evaluated on the eigen-decomposition of D^-1/2 A D^-1/2 (no vector recurrence). A
+    start from y = 0, or d and c exchanged, misses by O(1) (checked, so the test can fail)."""
+    import pin_systems as ps
+    al, be = bounds
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 5, params=[1.0, 1.0, 1.0, 1.0], seed=3)
+    A = _dense(ro, cols, vals)
+    x = np.random.default_rng(1).standard_normal(len(ro) - 1)
+    for k in range(1, 7):
+        y = orc.cheb_apply(ro, cols, vals, al, be, k, x)
+        scale = np.abs(y).max()
+        assert np.abs(y - ps.cheb_apply_predicted(A, x, al, be, k)).max() <= 1e-12 * scale, k
+        wrong_start = ps.cheb_apply_predicted(A, x, al, be, k, y0=np.zeros_like(x))
+        assert np.abs(y - wrong_start).max() > 1e-2 * scale
+        # d and c exchanged (the recurrence run with -alpha for alpha)
+        wrong_dc = ps.cheb_apply_predicted(A, x, -al, be, k)
+        assert np.abs(y - wrong_dc).max() > 1e-2 * scale
+
+
+def test_cheb_algo2_is_not_the_textbook_chebyshev_polynomial():
+    """Finding: _algo2's recurrence (first step 2/d, b = a (c/2)^2, :409,:424-425) is not the
+    Chebyshev semi-iteration on [alpha, beta], whose residual polynomial is
+    T_k((d - t)/c) / T_k(d/c) (first step 1/d). Already at order 1 they differ:
+    1 - 2t/d against 1 - t/d."""
+    import pin_systems as ps
+    from numpy.polynomial import chebyshev as Ch
+    al, be = 0.05, 1.9
+    d, c = (be + al) / 2, (be - al) / 2
+    t = np.linspace(al, be, 7)
+    for k in (1, 2, 3):
+        ref = np.polynomial.polynomial.polyval(t, ps.cheb_residual_poly(al, be, k))
+        textbook = Ch.chebval((d - t) / c, [0] * k + [1]) / Ch.chebval(d / c, [0] * k + [1])
+        assert np.abs(ref - textbook).max() > 0.1, k
+
+
+# ---- Krylov exit and count rules against finite termination ------------------------------------
+@pytest.mark.parametrize("m", [1, 2, 3, 4])
+def test_krylov_counts_on_m_eigenvalue_systems(m):
+    """D^-1 A with m distinct eigenvalues: PCG needs exactly m iterations; BiCGStab exits on
+    the half-step vector s of its m-th step, which counts the iteration only in the first
+    block (BiCGStab.h:194-199, m = 1) and not inside the loop (:259-263): m - 1 otherwise."""
+    import pin_systems as ps
+    ro, cols, vals = ps.m_eig_system(m)
+    A = _dense(ro, cols, vals)
+    dh = np.sqrt(np.diag(A))
+    ev = np.linalg.eigvalsh(A / np.outer(dh, dh))
+    np.testing.assert_allclose(np.unique(np.round(ev, 12)), ps.m_eig_spectrum(m), atol=1e-12)
+    b = np.random.default_rng(7).uniform(0, 1, len(ro) - 1)
+    xs = np.linalg.solve(A, b)
+    for solver, want in ((orc.ORC_CG, ps.CG_ITERATIONS[m]), (orc.ORC_BICGS, ps.BICGSTAB_ITERATIONS[m])):
+        x, r = orc.solve(ro, cols, vals, b, solver=solver, precond=orc.PREC_DIAG, tol=1e-8, max_iter=100)
+        assert (r["iterations"], r["status"], r["error"]) == (want, 1, 0), (solver, r)
+        assert np.abs(x - xs).max() <= 1e-13 * np.abs(xs).max()
+
+
+def test_bicgstab_nullvalue_rules():
+    """The NullValueException exits, on systems worked out by hand (pin_systems)."""
+    import pin_systems as ps
+    (ro, cols, vals), b = ps.bicgstab_tt_zero_system()
+    for prec in (orc.PREC_NONE, orc.PREC_DIAG):
+        x, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=prec, max_iter=100)
+        assert (r["iterations"], r["error"]) == (0, 1)
+        np.testing.assert_array_equal(x, [0.0, 0.0])
+    (ro, cols, vals), b, x_after = ps.bicgstab_vr0_zero_system()
+    x, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_NONE, max_iter=100)
+    assert (r["iterations"], r["error"]) == (1, 1)
+    np.testing.assert_array_equal(x, x_after)
+
+
+def test_max_iter_zero_still_takes_the_first_step():
+    """CG.h:89-109 and BiCGStab.h:169-232 run their first block before any stop test, and
+    Iteration::stop returns true at it >= max_it without a dot (Iteration.h:83-90): with
+    max_iter = 0 CG takes one step, x = (r.z / z.A z) z with z = D^-1 b, and BiCGStab leaves
+    at its first half step with ++iter and x = alpha phat, alpha = (b.b) / (A phat . b)."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 4, params=[1.0, 1.0, 1.0, 1.0], seed=2)
+    A = _dense(ro, cols, vals)
+    b = aa.uniform01(42, len(ro) - 1)
+    z = b / np.diag(A)
+    x, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_DIAG, max_iter=0)
+    assert (r["iterations"], r["status"]) == (1, 0)
+    np.testing.assert_allclose(x, (b @ z) / (z @ (A @ z)) * z, rtol=1e-13)
+    x, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_DIAG, max_iter=0)
+    assert (r["iterations"], r["status"]) == (1, 0)
+    np.testing.assert_allclose(x, (b @ b) / ((A @ z) @ b) * z, rtol=1e-13)
