@@ -82,6 +82,10 @@ _SIGS = [
     ("alien_hip_spmv_csr_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_spmv_bellpack_f64", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                               c_void_p]),
+    ("alien_hip_spmv_bellpack_f32", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p]),
+    ("alien_hip_bellpack_fill_f64", C.c_int, [c_void_p, C.c_int, C.c_int] + [c_void_p] * 9),
+    ("alien_hip_bellpack_fill_f32", C.c_int, [c_void_p, C.c_int, C.c_int] + [c_void_p] * 6),
     ("alien_hip_addmult_bellpack_f64", C.c_int, [c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, c_void_p,
                                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_addmult_csr_f64", C.c_int, [c_void_p, C.c_int, C.c_double, C.c_int, c_void_p, c_void_p, c_void_p,
@@ -134,6 +138,8 @@ _SIGS = [
     ("alien_hip_dot_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_dot_f32", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_axpy_dot_f64", C.c_int, [c_void_p, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_dot_rbdia_f64", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 7),
+    ("alien_hip_spmv_dot_rbdia_f32", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 7),
     ("alien_hip_spmv_dot_csr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_void_p]),
     ("alien_hip_spmv_dot_csr_f64_noalloc", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -276,6 +282,14 @@ def vblock_to_csr(ro, cols, block_cols, block_size, block_offset, vals=None):
     return sro, scols[:nnz], (None if svals is None else svals[:nnz])
 
 
+def bellpack_bro(ro, slice_=1024):
+    """Slice offsets of the BEllPack layout (nslices + 1 entries)."""
+    n = len(ro) - 1
+    bro = np.empty((n + slice_ - 1) // slice_ + 1, np.int32)
+    check(lib().alien_hip_host_bellpack_bro(n, slice_, iptr(np.ascontiguousarray(ro, np.int32)), iptr(bro)))
+    return bro
+
+
 def bellpack_build(ro, cols, vals=None, slice_=1024, masks=False):
     n = len(ro) - 1
     ns = (n + slice_ - 1) // slice_
@@ -415,12 +429,14 @@ class Matrix:
         assert len(v) == self.nnz
         check(lib().alien_hip_matrix_update_values(self.h, dptr(v)))
 
-    def solve(self, b, x0=None, **kw):
+    def solve(self, b, x0=None, ctx_override: Context | None = None, **kw):
+        """ctx_override: solve through another context (stream) than the one the matrix was created on."""
         x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
         b = np.ascontiguousarray(b, dtype=np.float64)
         p, _theta = _solve_params(**kw)
         r = SolveResult()
-        check(lib().alien_hip_solve(self.ctx.h, C.byref(p), self.h, dptr(b), dptr(x), C.byref(r)))
+        c = ctx_override if ctx_override is not None else self.ctx
+        check(lib().alien_hip_solve(c.h, C.byref(p), self.h, dptr(b), dptr(x), C.byref(r)))
         return x, r.as_dict()
 
     def close(self):

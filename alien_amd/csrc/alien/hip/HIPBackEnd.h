@@ -8,15 +8,21 @@
 //   matrix             SYCLBEllPackMatrix (sycl/data/SYCLBEllPackMatrix.h:63-225)
 //   algebra            SYCLInternalLinearAlgebra (sycl/algebra/SYCLInternalLinearAlgebra.h:46-141)
 //
-// Outside a full Alien build there is no MultiMatrixImpl to own the matrix, so
-// HIPMatrix is built directly from a host CSR view (what a SimpleCSR -> hip
-// matrix converter registered in Alien would do with the source's profile and
-// values).
+// Inside an Alien build (ALIEN_HIP_HAVE_ALGEBRA_TRAITS defined, Alien's include
+// directories on the path) the vector and matrix derive from Alien's
+// IVectorImpl / IMatrixImpl and are constructed by MultiVectorImpl /
+// MultiMatrixImpl (`new matrix_type(this)`, MultiMatrixImpl.h:318), then filled
+// by the SimpleCSR -> hip converters through build() / setValues(); the
+// algebras derive from IInternalLinearAlgebra / IInternalLinearAlgebraExpr.
+// Standalone (this repository's library and tests) the same classes are built
+// directly from a host CSR view, and the few Alien names the traits use are
+// declared below.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -27,7 +33,33 @@
 #include "../../kernels/launch.h"
 #include "HIPContext.h"
 
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+#include <alien/core/backend/BackEnd.h>
+#include <alien/core/backend/IInternalLinearAlgebraExprT.h>
+#include <alien/core/backend/IInternalLinearAlgebraT.h>
+#include <alien/core/impl/IMatrixImpl.h>
+#include <alien/core/impl/IVectorImpl.h>
+#include <alien/core/impl/MultiMatrixImpl.h>
+#include <alien/core/impl/MultiVectorImpl.h>
+#else
+namespace Arccore {
+namespace MessagePassing {
+class IMessagePassingMng;
+} // namespace MessagePassing
+} // namespace Arccore
+#endif
+
 namespace Alien {
+
+#ifndef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+// Declared by Alien in alien/core/backend/BackEnd.h (BackEndId is Arccore's String there,
+// constructible from the same literal) and alien/utils/Precomp.h.
+typedef std::string BackEndId;
+template <typename Tag> struct AlgebraTraits;
+using Arccore::MessagePassing::IMessagePassingMng;
+class MultiMatrixImpl;
+class MultiVectorImpl;
+#endif
 
 namespace BackEnd {
 namespace tag {
@@ -38,19 +70,27 @@ struct hip {};
 template <typename T> class HIPVectorT;
 class HIPMatrix;
 class HIPInternalLinearAlgebra;
+class HIPInternalLinearAlgebraExpr;
 
-#ifndef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
-// Alien declares this primary template in alien/core/backend/BackEnd.h.
-template <typename Tag> struct AlgebraTraits;
-#endif
-
+// Backend traits (SYCLBackEnd.h:76-96, SimpleCSRBackEnd.h:62-94): inside Alien the
+// algebra types are its virtual interfaces, which the hip algebras implement.
 template <>
 struct AlgebraTraits<BackEnd::tag::hip> {
   typedef HIPMatrix matrix_type;
   typedef HIPVectorT<double> vector_type;
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+  typedef IInternalLinearAlgebra<HIPMatrix, HIPVectorT<double>> algebra_type;
+  typedef IInternalLinearAlgebraExpr<HIPMatrix, HIPVectorT<double>> algebra_expr_type;
+#else
   typedef HIPInternalLinearAlgebra algebra_type;
+  typedef HIPInternalLinearAlgebraExpr algebra_expr_type;
+#endif
   typedef double value_type;
-  static const char* name() { return "hip"; }
+  // a new algebra on the library's default context (one HIP stream per process); the
+  // caller owns it, as LinearAlgebra<tag> owns the one algebra_factory returns
+  static algebra_type* algebra_factory(IMessagePassingMng* p_mng = nullptr);
+  static algebra_expr_type* algebra_expr_factory(IMessagePassingMng* p_mng = nullptr);
+  static BackEndId name() { return "hip"; }
 };
 
 // "Resource" of a vector: its size and the context that owns its stream
@@ -61,13 +101,32 @@ struct HIPResource {
 };
 
 template <typename T>
-class HIPVectorT {
+class HIPVectorT
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+: public IVectorImpl
+#endif
+{
  public:
   typedef T ValueType;
   static const bool on_host_only = false;
 
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+  HIPVectorT() : IVectorImpl(nullptr, AlgebraTraits<BackEnd::tag::hip>::name()) {}
+  // created by MultiVectorImpl, which then calls init(distribution, true)
+  explicit HIPVectorT(const MultiVectorImpl* multi_impl)
+  : IVectorImpl(multi_impl, AlgebraTraits<BackEnd::tag::hip>::name()) {}
+  HIPVectorT(const HIPResource& r) : HIPVectorT() { init(r, true); }
+  // IVectorImpl::init: the local rows of the distribution, on the default context
+  void init(const VectorDistribution& dist, bool need_alloc) override {
+    init(HIPResource{dist.localSize(), alien_hip::default_ctx()}, need_alloc);
+  }
+  void clear() override { release(); }
+#else
   HIPVectorT() = default;
+  explicit HIPVectorT(const MultiVectorImpl*) {}
   HIPVectorT(const HIPResource& r) { init(r, true); }
+  void clear() { release(); }
+#endif
 
   // zero-filled like SimpleCSRVector::init (SimpleCSRVector.h:111-131)
   void init(const HIPResource& r, bool need_alloc) {
@@ -78,7 +137,7 @@ class HIPVectorT {
       if (m_n) ALIEN_HIP_CHECK(hipMemsetAsync(m_data.data(), 0, sizeof(T) * m_n, m_ctx->stream));
     }
   }
-  void clear() {
+  void release() {
     m_data.reset();
     m_n = 0;
   }
@@ -127,16 +186,85 @@ struct CSRHostView {
 enum HIPLayout { kLayoutCSR = ALIEN_HIP_LAYOUT_CSR, kLayoutBEllPack = ALIEN_HIP_LAYOUT_BELLPACK,
                  kLayoutRBCSR = ALIEN_HIP_LAYOUT_RBCSR, kLayoutRBDIA = ALIEN_HIP_LAYOUT_RBDIA };
 
-class HIPMatrix {
+class HIPMatrix
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+: public IMatrixImpl
+#endif
+{
  public:
   typedef double ValueType;
   typedef BackEnd::tag::hip TagType;
   static const bool on_host_only = false;
   static constexpr int kBEllPackSlice = 1024; // BEllPackStructInfo<1024,int>
 
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+  HIPMatrix() : IMatrixImpl(nullptr, AlgebraTraits<BackEnd::tag::hip>::name()) {}
+  // MultiMatrixImpl::getImpl (MultiMatrixImpl.h:318) creates the empty matrix; the
+  // SimpleCSR -> hip converter fills it with build()
+  explicit HIPMatrix(const MultiMatrixImpl* multi_impl)
+  : IMatrixImpl(multi_impl, AlgebraTraits<BackEnd::tag::hip>::name()) {}
+  HIPMatrix(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR, bool borrow_host = false)
+  : HIPMatrix() {
+    build(ctx, a, spmv_layout, borrow_host);
+  }
+  void clear() override { release(); }
+#else
   HIPMatrix() = default;
+  explicit HIPMatrix(const MultiMatrixImpl*) {}
   HIPMatrix(alien_hip_ctx* ctx, const CSRHostView& a, int spmv_layout = kLayoutRBCSR, bool borrow_host = false) {
     build(ctx, a, spmv_layout, borrow_host);
+  }
+  void clear() { release(); }
+#endif
+  HIPMatrix(const HIPMatrix&) = delete;
+  HIPMatrix& operator=(const HIPMatrix&) = delete;
+
+  // A new matrix with the same profile and values (owning its host copy), in the same
+  // SpMV layout: the cloneTo of Alien's matrix impls (SYCLBEllPackMatrix.h:123).
+  HIPMatrix* cloneTo(const MultiMatrixImpl* multi) const {
+    std::unique_ptr<HIPMatrix> c(new HIPMatrix(multi));
+    if (m_ctx) {
+      CSRHostView v{m_n, h_ro, h_cols, hostValues()};
+      c->build(m_ctx, v, m_layout, /*borrow_host=*/false);
+    }
+    return c.release();
+  }
+  // Alien's notification that the values were changed through data() on the host
+  // (SYCLBEllPackMatrix::notifyChanges): upload them and refill the layouts.
+  void notifyChanges() {
+    if (m_ctx && h_vals == m_vals.data()) updateValues(m_vals.data());
+  }
+  void endUpdate() { notifyChanges(); }
+  int getAllocSize() const { return m_n; }
+  // host values (the matrix's own copy): Alien's data() of the host-side profile
+  double* data() {
+    syncHostValues();
+    return m_vals.empty() ? nullptr : m_vals.data();
+  }
+  // frees the device arrays and the host copy (IMatrixImpl::clear)
+  void release() {
+    d_ro.reset();
+    d_cols.reset();
+    d_dcol.reset();
+    d_vals.reset();
+    freeRB();
+    d_bro.reset();
+    d_bellc.reset();
+    d_bellv.reset();
+    d_bellv32.reset();
+    d_lmask.reset();
+    d_umask.reset();
+    m_fused_ell = false;
+    m_kcol.clear();
+    m_cols.clear();
+    m_dcol.clear();
+    m_vals.clear();
+    m_bo.clear();
+    m_bro.clear();
+    h_ro = h_cols = nullptr;
+    h_vals = nullptr;
+    m_n = m_nnz = 0;
+    m_layout = kLayoutCSR;
   }
 
   // Upload a host CSR (ascending columns) and build the requested SpMV layout:
@@ -190,6 +318,12 @@ class HIPMatrix {
     syncHostValues();
     return h_vals;
   }
+  // The device CSR values were changed on `st` (by the expression algebra): the layouts
+  // already built are refilled from them and the host copy is marked stale.
+  void deviceValuesChanged(hipStream_t st) {
+    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+    valuesChanged();
+  }
   // New values on the same structure (the matrix owns its host copy): the
   // device CSR values and every value layout already built are refilled.
   // The host copy is refreshed from the device only when a host-side builder
@@ -229,19 +363,21 @@ class HIPMatrix {
                                             d_bc.data(), d_bv32.data());
     }
     alien_hip::check_launch("updateValues refill");
-    if (!d_bellv.empty()) { // BEllPack values are filled on the host: rebuild them
-      syncHostValues();
-      d_bro.reset();
-      d_bellc.reset();
-      d_bellv.reset();
-      ensureBEllPack(false);
-    }
+    if (!d_bellv.empty())
+      alien_hip::launch_bellpack_fill<double>(st, m_n, kBEllPackSlice, d_ro.data(), d_cols.data(), d_vals.data(),
+                                              nullptr, d_bro.data(), nullptr, d_bellv.data(), nullptr, nullptr);
+    if (!d_bellv32.empty())
+      alien_hip::launch_bellpack_fill<float>(st, m_n, kBEllPackSlice, d_ro.data(), d_cols.data(), d_vals.data(),
+                                             nullptr, d_bro.data(), nullptr, d_bellv32.data(), nullptr, nullptr);
+    alien_hip::check_launch("updateValues BEllPack refill");
     ALIEN_HIP_CHECK(hipStreamSynchronize(st));
   }
 
  public:
   // host bundle offsets of the row-bundled layout (after ensureRB / ensureRBCSR)
   const std::vector<int>& bundleOffsets() const { return m_bo; }
+  // host slice offsets of the BEllPack layout, nslices + 1 entries (after ensureBEllPack)
+  const std::vector<int>& bellpackRowOffsets() const { return m_bro; }
 
   // Builds the layout on first use and selects it for mult().
   void setSpmvLayout(int layout) {
@@ -293,6 +429,7 @@ class HIPMatrix {
     m_pair = pair;
     d_bv.resize((size_t)slots * alien_hip::kBundle * (pair ? 2 : 1));
     fillDIA<double>(d_bv.data());
+    settle();
   }
   template <typename T>
   void fillDIA(T* bv) {
@@ -306,6 +443,10 @@ class HIPMatrix {
     alien_hip::check_launch("rbdia_fill");
   }
   void ensureRB32() {
+    if (m_fused_ell) {
+      ensureBEllPack32();
+      return;
+    }
     if (d_bo.empty()) ensureRB(true);
     if (!d_bv32.empty()) return;
     d_bv32.resize(d_bv.size());
@@ -315,18 +456,27 @@ class HIPMatrix {
       alien_hip::launch_rbcsr_fill<float>(m_ctx->stream, m_n, d_ro.data(), d_cols.data(), d_vals.data(),
                                           d_bo.data(), d_bc.data(), d_bv32.data());
     alien_hip::check_launch("row-bundled fill<float>");
+    settle();
   }
-  bool isRBDIA() const { return m_rbdia; }
-  bool isPairDIA() const { return m_rbdia && m_pair; }
+  bool isRBDIA() const { return m_rbdia && !m_fused_ell; }
+  bool isPairDIA() const { return m_rbdia && m_pair && !m_fused_ell; }
   alien_hip::RBMat<double> rb64() const {
+    if (m_fused_ell) return ellMat<double>(d_bellv.data());
     return alien_hip::RBMat<double>{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv.data(),
                                     m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr,
                                     m_pair ? 1 : 0};
   }
   alien_hip::RBMat<float> rb32() const {
+    if (m_fused_ell) return ellMat<float>(d_bellv32.data());
     return alien_hip::RBMat<float>{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv32.data(),
                                    m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr,
                                    m_pair ? 1 : 0};
+  }
+  template <typename T>
+  alien_hip::RBMat<T> ellMat(const T* bv) const {
+    alien_hip::RBMat<T> a{d_bro.data(), d_bellc.data(), bv};
+    a.ell = kBEllPackSlice / alien_hip::kBundle;
+    return a;
   }
   void freeRB() {
     d_bo.reset();
@@ -354,36 +504,58 @@ class HIPMatrix {
     alien_hip::launch_rbcsr_fill<double>(st, m_n, d_ro.data(), d_cols.data(), d_vals.data(), d_bo.data(),
                                          d_bc.data(), d_bv.data());
     alien_hip::check_launch("rbcsr_fill");
+    settle();
   }
+  // Alien's BEllPack arrays (slice offsets from the host profile, columns, values and, on
+  // request, the L/U masks scattered on the device from the device CSR)
   void ensureBEllPack(bool masks) {
+    hipStream_t st = m_ctx->stream;
+    const int ns = alien_hip::bellpack_nslices(m_n, kBEllPackSlice);
     if (d_bro.empty()) {
-      syncHostValues();
-      const int ns = alien_hip::bellpack_nslices(m_n, kBEllPackSlice);
       m_bro.resize(ns + 1);
       alien_hip::bellpack_bro(m_n, kBEllPackSlice, h_ro, m_bro.data());
+      if ((long)m_bro[ns] * kBEllPackSlice > 0x7fffffffL)
+        throw alien_hip::Error(ALIEN_HIP_EINVAL, "BEllPack storage exceeds int32 indexing");
       const size_t total = (size_t)m_bro[ns] * kBEllPackSlice;
-      std::vector<int> bc(total);
-      std::vector<double> bv(total);
-      alien_hip::bellpack_fill(m_n, kBEllPackSlice, h_ro, h_cols, h_vals, m_bro.data(),
-                               bc.data(), bv.data(), nullptr, nullptr);
-      hipStream_t st = m_ctx->stream;
       d_bro.upload(m_bro.data(), m_bro.size(), st);
-      d_bellc.upload(bc.data(), bc.size(), st);
-      d_bellv.upload(bv.data(), bv.size(), st);
-      ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+      d_bellc.resize(total);
+      d_bellv.resize(total);
+      alien_hip::launch_bellpack_fill<double>(st, m_n, kBEllPackSlice, d_ro.data(), d_cols.data(), d_vals.data(),
+                                              nullptr, d_bro.data(), d_bellc.data(), d_bellv.data(), nullptr,
+                                              nullptr);
+      alien_hip::check_launch("bellpack_fill");
+      settle();
     }
     if (masks && d_lmask.empty()) {
-      const int ns = alien_hip::bellpack_nslices(m_n, kBEllPackSlice);
       const size_t total = (size_t)m_bro[ns] * kBEllPackSlice;
-      std::vector<int> bc(total);
-      std::vector<uint8_t> lm(total), um(total);
-      alien_hip::bellpack_fill(m_n, kBEllPackSlice, h_ro, h_cols, nullptr, m_bro.data(),
-                               bc.data(), nullptr, lm.data(), um.data());
-      d_lmask.upload(lm.data(), lm.size(), m_ctx->stream);
-      d_umask.upload(um.data(), um.size(), m_ctx->stream);
-      ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream));
+      d_lmask.resize(total);
+      d_umask.resize(total);
+      alien_hip::launch_bellpack_fill<double>(st, m_n, kBEllPackSlice, d_ro.data(), d_cols.data(), nullptr,
+                                              d_dcol.data(), d_bro.data(), nullptr, nullptr, d_lmask.data(),
+                                              d_umask.data());
+      alien_hip::check_launch("bellpack_fill masks");
+      settle();
     }
   }
+  // FP32 copy of the BEllPack values (mixed-precision solvers on that layout)
+  void ensureBEllPack32() {
+    ensureBEllPack(false);
+    if (!d_bellv32.empty() || d_bellv.empty()) return;
+    d_bellv32.resize(d_bellv.size());
+    alien_hip::launch_bellpack_fill<float>(m_ctx->stream, m_n, kBEllPackSlice, d_ro.data(), d_cols.data(),
+                                           d_vals.data(), nullptr, d_bro.data(), nullptr, d_bellv32.data(), nullptr,
+                                           nullptr);
+    alien_hip::check_launch("bellpack_fill<float>");
+    settle();
+  }
+  // Operand layout of the fused solvers: Alien's BEllPack read in place (the row kernels'
+  // slice mode) when asked for, else the row-bundled layouts (ensureRB)
+  void ensureFused(int layout) {
+    m_fused_ell = layout == kLayoutBEllPack;
+    if (m_fused_ell) ensureBEllPack(false);
+    else ensureRB(layout != kLayoutRBCSR);
+  }
+  bool fusedBEllPack() const { return m_fused_ell; }
 
   // device arrays (public for the kernels' callers inside the backend)
   alien_hip::DeviceArray<int> d_ro, d_cols, d_dcol;
@@ -395,12 +567,14 @@ class HIPMatrix {
   alien_hip::DeviceArray<unsigned long long> d_dmask;
   alien_hip::DeviceArray<int> d_bro, d_bellc;    // BEllPack
   alien_hip::DeviceArray<double> d_bellv;
+  alien_hip::DeviceArray<float> d_bellv32;
   alien_hip::DeviceArray<uint8_t> d_lmask, d_umask;
 
  private:
   alien_hip_ctx* m_ctx = nullptr;
   bool m_rbdia = false;
   bool m_pair = false; // RB-DIA in pair mode
+  bool m_fused_ell = false; // the fused solvers read the BEllPack arrays
   int m_n = 0, m_nnz = 0;
   int m_layout = kLayoutCSR;
   std::vector<int> m_kcol, m_cols, m_dcol, m_bo, m_bro;
@@ -414,6 +588,10 @@ class HIPMatrix {
     m_host_vals_stale = false;
   }
   mutable bool m_host_vals_stale = false;
+  // The layouts are built on the stream of the matrix's own context, but a
+  // kept matrix may be solved through any context on its device: each
+  // one-time layout build drains its stream before the layout is used.
+  void settle() const { ALIEN_HIP_CHECK(hipStreamSynchronize(m_ctx->stream)); }
   const int* h_ro = nullptr;   // host profile: the copies above or the caller's arrays
   const int* h_cols = nullptr;
   const double* h_vals = nullptr;
@@ -447,7 +625,11 @@ class HIPFuture {
   bool m_pending = false;
 };
 
-class HIPInternalLinearAlgebra {
+class HIPInternalLinearAlgebra
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+: public IInternalLinearAlgebra<HIPMatrix, HIPVectorT<double>>
+#endif
+{
  public:
   typedef BackEnd::tag::hip BackEndType;
   typedef HIPMatrix Matrix;
@@ -507,15 +689,23 @@ class HIPInternalLinearAlgebra {
     }
     alien_hip::check_launch("mult");
   }
-  void addLMult(double alpha, const Matrix& A, const Vector& x, Vector& y) const {
-    alien_hip::launch_addmult_csr(stream(), A.getNRows(), 0, alpha, A.d_ro.data(), A.d_cols.data(), A.d_dcol.data(),
-                                  A.d_vals.data(), x.data(), y.data());
-    alien_hip::check_launch("addLMult");
-  }
-  void addUMult(double alpha, const Matrix& A, const Vector& x, Vector& y) const {
-    alien_hip::launch_addmult_csr(stream(), A.getNRows(), 1, alpha, A.d_ro.data(), A.d_cols.data(), A.d_dcol.data(),
-                                  A.d_vals.data(), x.data(), y.data());
-    alien_hip::check_launch("addUMult");
+  // y += alpha (strictly lower part of A) x: on a BEllPack matrix through its lower mask,
+  // the reference's device path (MatrixInternal::addLMult, SYCLBEllPackMatrix.cc:769-840),
+  // else the CSR loop (SimpleCSRMatrixMultT.h:193-212)
+  void addLMult(double alpha, const Matrix& A, const Vector& x, Vector& y) const { addTriMult(0, alpha, A, x, y); }
+  // y += alpha (strictly upper part of A) x (SYCLBEllPackMatrix.cc:842-911, SimpleCSRMatrixMultT.h:214-233)
+  void addUMult(double alpha, const Matrix& A, const Vector& x, Vector& y) const { addTriMult(1, alpha, A, x, y); }
+  void addTriMult(int upper, double alpha, const Matrix& A, const Vector& x, Vector& y) const {
+    if (A.spmvLayout() == kLayoutBEllPack) {
+      const_cast<Matrix&>(A).ensureBEllPack(true); // masks built once, on the first masked product
+      alien_hip::launch_addmult_bellpack(stream(), A.getNRows(), HIPMatrix::kBEllPackSlice, alpha, A.d_bro.data(),
+                                         A.d_bellc.data(), A.d_bellv.data(),
+                                         upper ? A.d_umask.data() : A.d_lmask.data(), x.data(), y.data());
+    } else {
+      alien_hip::launch_addmult_csr(stream(), A.getNRows(), upper, alpha, A.d_ro.data(), A.d_cols.data(),
+                                    A.d_dcol.data(), A.d_vals.data(), x.data(), y.data());
+    }
+    alien_hip::check_launch(upper ? "addUMult" : "addLMult");
   }
   // d_i = 1 / a[dcol_i] (SimpleCSRMatrixMultT::computeInvDiag, :494-509)
   void computeInvDiag(const Matrix& A, Vector& d) const {
@@ -577,5 +767,114 @@ class HIPInternalLinearAlgebra {
  private:
   alien_hip_ctx* m_ctx = nullptr;
 };
+
+// The expression algebra (IInternalLinearAlgebraExprT.h:54-279; SYCLInternalLinearAlgebraExpr,
+// SYCLInternalLinearAlgebra.h:143-180): the vector operations of HIPInternalLinearAlgebra, the
+// matrix operations on the device values of matrices of one profile (the layouts already built
+// are refilled), and the same operations on host arrays.
+class HIPInternalLinearAlgebraExpr
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+: public IInternalLinearAlgebraExpr<HIPMatrix, HIPVectorT<double>>
+#endif
+{
+ public:
+  typedef HIPMatrix Matrix;
+  typedef HIPVector Vector;
+#ifdef ALIEN_HIP_HAVE_ALGEBRA_TRAITS
+  typedef UniqueArray<Real> HostArray;
+  typedef Integer SizeType;
+#else
+  typedef std::vector<double> HostArray;
+  typedef int SizeType;
+#endif
+  typedef HIPInternalLinearAlgebra::NotImplementedException NotImplementedException;
+
+  explicit HIPInternalLinearAlgebraExpr(alien_hip_ctx* ctx = nullptr) : m_alg(ctx) {}
+
+  double norm0(const Vector& x) const { return m_alg.norm0(x); }
+  double norm1(const Vector& x) const { return m_alg.norm1(x); }
+  double norm2(const Vector& x) const { return m_alg.norm2(x); }
+  // Frobenius norm: sqrt of the dot product of the stored values
+  double norm2(const Matrix& a) const {
+    const int s = m_alg.ctx()->take_slot();
+    m_alg.ctx()->reserve_for(a.getNnz());
+    alien_hip::launch_dot<double>(m_alg.stream(), m_alg.ctx()->red(), a.getNnz(), a.d_vals.data(), a.d_vals.data(),
+                                  m_alg.ctx()->dslot(s));
+    alien_hip::check_launch("norm2(matrix)");
+    return std::sqrt(m_alg.ctx()->read_slot(s));
+  }
+  void mult(const Matrix& a, const Vector& x, Vector& r) const { m_alg.mult(a, x, r); }
+  void axpy(double alpha, const Vector& x, Vector& y) const { m_alg.axpy(alpha, x, y); }
+  void aypx(double alpha, Vector& y, const Vector& x) const { m_alg.aypx(alpha, y, x); }
+  void copy(const Vector& x, Vector& r) const { m_alg.copy(x, r); }
+  // r's values = a's values (same profile)
+  void copy(const Matrix& a, Matrix& r) const {
+    sameProfile(a, r, "copy(matrix)");
+    alien_hip::launch_copy<double>(m_alg.stream(), a.getNnz(), a.d_vals.data(), r.d_vals.data());
+    alien_hip::check_launch("copy(matrix)");
+    r.deviceValuesChanged(m_alg.stream());
+  }
+  // r's values += a's values (same profile)
+  void add(const Matrix& a, Matrix& r) const {
+    sameProfile(a, r, "add(matrix)");
+    alien_hip::launch_axpy<double>(m_alg.stream(), a.getNnz(), 1.0, a.d_vals.data(), r.d_vals.data());
+    alien_hip::check_launch("add(matrix)");
+    r.deviceValuesChanged(m_alg.stream());
+  }
+  double dot(const Vector& x, const Vector& y) const { return m_alg.dot(x, y); }
+  void scal(double alpha, Vector& x) const { m_alg.scal(alpha, x); }
+  void scal(double alpha, Matrix& a) const {
+    alien_hip::launch_scal<double>(m_alg.stream(), a.getNnz(), alpha, a.d_vals.data());
+    alien_hip::check_launch("scal(matrix)");
+    a.deviceValuesChanged(m_alg.stream());
+  }
+  void diagonal(const Matrix& a, Vector& x) const { m_alg.diagonal(a, x); }
+  void reciprocal(Vector& x) const { m_alg.reciprocal(x); }
+  void pointwiseMult(const Vector& x, const Vector& y, Vector& w) const { m_alg.pointwiseMult(x, y, w); }
+
+  // host arrays: r = a x through the device (upload x, product, download r), the BLAS-1
+  // operations as plain host loops
+  void mult(const Matrix& a, const HostArray& x, HostArray& r) const {
+    Vector dx(a.resource()), dr(a.resource());
+    dx.setValues(x.data());
+    m_alg.mult(a, dx, dr);
+    dr.copyValuesTo(r.data());
+  }
+  void axpy(double alpha, const HostArray& x, HostArray& y) const {
+    for (SizeType i = 0; i < (SizeType)x.size(); ++i) y[i] = y[i] + alpha * x[i];
+  }
+  void aypx(double, HostArray&, const HostArray&) const {
+    throw NotImplementedException("HIPInternalLinearAlgebraExpr::aypx");
+  }
+  void copy(const HostArray& x, HostArray& r) const {
+    for (SizeType i = 0; i < (SizeType)x.size(); ++i) r[i] = x[i];
+  }
+  double dot(SizeType local_size, const HostArray& x, const HostArray& y) const {
+    double s = 0.0;
+    for (SizeType i = 0; i < local_size; ++i) s = s + x[i] * y[i];
+    return s;
+  }
+  void scal(double alpha, HostArray& x) const {
+    for (SizeType i = 0; i < (SizeType)x.size(); ++i) x[i] = alpha * x[i];
+  }
+
+  HIPInternalLinearAlgebra& algebra() { return m_alg; }
+
+ private:
+  static void sameProfile(const Matrix& a, const Matrix& r, const char* what) {
+    if (a.getNRows() != r.getNRows() || a.getNnz() != r.getNnz())
+      throw alien_hip::Error(ALIEN_HIP_EINVAL, std::string(what) + ": matrices of different profiles");
+  }
+  HIPInternalLinearAlgebra m_alg;
+};
+
+inline AlgebraTraits<BackEnd::tag::hip>::algebra_type*
+AlgebraTraits<BackEnd::tag::hip>::algebra_factory(IMessagePassingMng*) {
+  return new HIPInternalLinearAlgebra();
+}
+inline AlgebraTraits<BackEnd::tag::hip>::algebra_expr_type*
+AlgebraTraits<BackEnd::tag::hip>::algebra_expr_factory(IMessagePassingMng*) {
+  return new HIPInternalLinearAlgebraExpr();
+}
 
 } // namespace Alien

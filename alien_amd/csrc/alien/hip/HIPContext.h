@@ -13,6 +13,7 @@
 #include <utility>
 
 #include "../../kernels/device_common.h"
+#include "../../kernels/krylov_state.h"
 
 namespace alien_hip {
 
@@ -87,6 +88,8 @@ struct alien_hip_ctx {
   alien_hip::DeviceArray<double> partials;
   int red_cap = 0;
   alien_hip::DeviceArray<double> scalars; // device-resident scalar slots
+  // zeroed CG state (done = 0, never written): the fused iteration's kernels launched on their own
+  alien_hip::DeviceArray<alien_hip::CGState> idle_cg;
   double* host_scalars = nullptr;          // pinned mirror of `scalars`
   int next_slot = 0;
   static constexpr int kSlots = 64;

@@ -68,6 +68,18 @@ class FILU0Preconditioner<HIPInternalLinearAlgebra> {
     m_alg.assign(m_inv_diag, 1.);
     alien_hip::launch_invdiag_csr(st, n, m_A.d_dcol.data(), d_lu.data(), m_inv_diag.data());
     alien_hip::check_launch("filu0 invdiag");
+    if (m_A.spmvLayout() == kLayoutBEllPack) {
+      // the factor as a BEllPack matrix of A's profile (the reference clones A in its own
+      // layout, FILU0Preconditioner.h:106-110): LU scattered to A's slots, A's L/U masks
+      MatrixType& A = const_cast<MatrixType&>(m_A);
+      A.ensureBEllPack(true);
+      d_lu_bell.resize((size_t)A.bellpackRowOffsets().back() * HIPMatrix::kBEllPackSlice);
+      alien_hip::launch_bellpack_fill<double>(st, n, HIPMatrix::kBEllPackSlice, A.d_ro.data(), A.d_cols.data(),
+                                              d_lu.data(), nullptr, A.d_bro.data(), nullptr, d_lu_bell.data(),
+                                              nullptr, nullptr);
+      alien_hip::check_launch("filu0 bellpack factor");
+      ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+    }
   }
 
   // solve (:236-256): x -> m_x by L sweeps, m_x -> y by U sweeps
@@ -94,10 +106,17 @@ class FILU0Preconditioner<HIPInternalLinearAlgebra> {
   const VectorType& invDiag() const { return m_inv_diag; }
 
  private:
-  // algebra.addLMult / addUMult(-1, LU, xk, y): A's structure with the factor's values
+  // algebra.addLMult / addUMult(-1, LU, xk, y): A's structure with the factor's values; on a
+  // BEllPack matrix through the masks (MatrixInternal::addLMult / addUMult,
+  // SYCLBEllPackMatrix.cc:769-911), else the CSR loops (SimpleCSRMatrixMultT.h:193-233)
   void addMult(AlgebraType& alg, int upper, const VectorType& xk, VectorType& y) const {
-    alien_hip::launch_addmult_csr(alg.stream(), m_A.getNRows(), upper, -1., m_A.d_ro.data(), m_A.d_cols.data(),
-                                  m_A.d_dcol.data(), d_lu.data(), xk.data(), y.data());
+    if (!d_lu_bell.empty())
+      alien_hip::launch_addmult_bellpack(alg.stream(), m_A.getNRows(), HIPMatrix::kBEllPackSlice, -1.,
+                                         m_A.d_bro.data(), m_A.d_bellc.data(), d_lu_bell.data(),
+                                         upper ? m_A.d_umask.data() : m_A.d_lmask.data(), xk.data(), y.data());
+    else
+      alien_hip::launch_addmult_csr(alg.stream(), m_A.getNRows(), upper, -1., m_A.d_ro.data(), m_A.d_cols.data(),
+                                    m_A.d_dcol.data(), d_lu.data(), xk.data(), y.data());
     alien_hip::check_launch(upper ? "filu0 addUMult" : "filu0 addLMult");
   }
 
@@ -107,6 +126,7 @@ class FILU0Preconditioner<HIPInternalLinearAlgebra> {
   int m_nb_solver_iter = 0;
   double m_tol = 0;
   alien_hip::DeviceArray<double> d_lu;
+  alien_hip::DeviceArray<double> d_lu_bell; // the factor in A's BEllPack slots (BEllPack layout only)
   VectorType m_inv_diag, m_x, m_xk;
 };
 

@@ -38,6 +38,8 @@ alien_hip_ctx* make_ctx(void* stream) {
     c->reserve_red(4096);
     c->scalars.resize(alien_hip_ctx::kSlots);
     ALIEN_HIP_CHECK(hipMemsetAsync(c->scalars.data(), 0, sizeof(double) * alien_hip_ctx::kSlots, c->stream));
+    c->idle_cg.resize(1);
+    ALIEN_HIP_CHECK(hipMemsetAsync(c->idle_cg.data(), 0, sizeof(CGState), c->stream));
     ALIEN_HIP_CHECK(hipHostMalloc((void**)&c->host_scalars, sizeof(double) * alien_hip_ctx::kSlots,
                                   hipHostMallocDefault));
     ALIEN_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -53,6 +55,7 @@ void destroy_ctx(alien_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->partials.reset();
   c->scalars.reset();
+  c->idle_cg.reset();
   if (c->host_scalars) (void)hipHostFree(c->host_scalars);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -338,6 +341,38 @@ int alien_hip_spmv_bellpack_f64(alien_hip_ctx* ctx, int n, int slice, const int*
   API_END
 }
 
+int alien_hip_spmv_bellpack_f32(alien_hip_ctx* ctx, int n, int slice, const int* bro, const int* bc,
+                                const float* bv, const float* x, float* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && slice > 0 && (n == 0 || (bro && bc && bv && x && y)), "spmv_bellpack_f32: bad arguments");
+  API_BEGIN
+  launch_spmv_bellpack<float>(ctx->stream, n, slice, bro, bc, bv, x, y);
+  LAUNCHED("spmv_bellpack_f32");
+  API_END
+}
+
+int alien_hip_bellpack_fill_f64(alien_hip_ctx* ctx, int n, int slice, const int* ro, const int* cols,
+                                const double* vals, const int* dcol, const int* bro, int* bc, double* bv,
+                                uint8_t* lmask, uint8_t* umask) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && slice > 0 && ro && bro && (!bv || vals) && (!(lmask || umask) || dcol) && (n == 0 || cols),
+          "bellpack_fill_f64: bad arguments");
+  API_BEGIN
+  launch_bellpack_fill<double>(ctx->stream, n, slice, ro, cols, vals, dcol, bro, bc, bv, lmask, umask);
+  LAUNCHED("bellpack_fill_f64");
+  API_END
+}
+
+int alien_hip_bellpack_fill_f32(alien_hip_ctx* ctx, int n, int slice, const int* ro, const int* cols,
+                                const double* vals, const int* bro, int* bc, float* bv) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && slice > 0 && ro && bro && (!bv || vals) && (n == 0 || cols), "bellpack_fill_f32: bad arguments");
+  API_BEGIN
+  launch_bellpack_fill<float>(ctx->stream, n, slice, ro, cols, vals, nullptr, bro, bc, bv, nullptr, nullptr);
+  LAUNCHED("bellpack_fill_f32");
+  API_END
+}
+
 int alien_hip_addmult_bellpack_f64(alien_hip_ctx* ctx, int upper, double alpha, int n, int slice, const int* bro,
                                    const int* bc, const double* bv, const uint8_t* mask, const double* x,
                                    double* y) {
@@ -575,6 +610,30 @@ int alien_hip_spmv_rbdia_f32(alien_hip_ctx* ctx, int n, const int* bo, const int
   RBMat<float> A{bo, nullptr, bv, doff, reinterpret_cast<const unsigned long long*>(dmask)};
   launch_spmv_rb<float>(ctx->stream, n, A, x, y);
   LAUNCHED("spmv_rbdia_f32");
+  API_END
+}
+
+int alien_hip_spmv_dot_rbdia_f64(alien_hip_ctx* ctx, int n, const int* bo, const int* doff, const uint64_t* dmask,
+                                 const double* bv, const double* x, double* y, double* result) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && result && (n == 0 || (bo && doff && dmask && bv && x && y)), "spmv_dot_rbdia_f64: bad arguments");
+  API_BEGIN
+  ctx->reserve_for(n);
+  RBMat<double> A{bo, nullptr, bv, doff, reinterpret_cast<const unsigned long long*>(dmask)};
+  launch_spmv_dot_rb<double>(ctx->stream, ctx->red(), n, A, x, y, ctx->idle_cg.data(), result);
+  LAUNCHED("spmv_dot_rbdia_f64");
+  API_END
+}
+
+int alien_hip_spmv_dot_rbdia_f32(alien_hip_ctx* ctx, int n, const int* bo, const int* doff, const uint64_t* dmask,
+                                 const float* bv, const float* x, float* y, double* result) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && result && (n == 0 || (bo && doff && dmask && bv && x && y)), "spmv_dot_rbdia_f32: bad arguments");
+  API_BEGIN
+  ctx->reserve_for(n);
+  RBMat<float> A{bo, nullptr, bv, doff, reinterpret_cast<const unsigned long long*>(dmask)};
+  launch_spmv_dot_rb<float>(ctx->stream, ctx->red(), n, A, x, y, ctx->idle_cg.data(), result);
+  LAUNCHED("spmv_dot_rbdia_f32");
   API_END
 }
 

@@ -459,6 +459,18 @@ void launch_cg_q(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* p, T* q, 
   launch_finalize(s, w, rblocks<T>(n), 1, kFinPQ, nullptr, st);
 }
 
+// the q = A p, p.q kernel of the fused iteration on its own: y = A x, *out = x . y (idle: a CG
+// state that is never done)
+template <typename T>
+void launch_spmv_dot_rb(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x, T* y, CGState* idle, double* out) {
+  if (n <= 0) {
+    launch_finalize(s, w, 0, 1, kFinStore, out, nullptr);
+    return;
+  }
+  hipLaunchKernelGGL(k_cg_q<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, A, x, y, idle, w);
+  launch_finalize(s, w, rblocks<T>(n), 1, kFinStore, out, nullptr);
+}
+
 template <typename T>
 void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* p, const T* q, T* x, T* r, const T* invd_or_null,
                   T* z_or_null, CGState* st) {
@@ -515,6 +527,7 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
   template void launch_factor_p<T, T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,  \
                                       CGState*, int, int);                                               \
   template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*);          \
+  template void launch_spmv_dot_rb<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*, double*); \
   template void launch_cg_update<T>(hipStream_t, int, const T*, const T*, T*, T*, CGState*);            \
   template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, T*, T*, const T*, T*,     \
                                 CGState*);

@@ -25,6 +25,10 @@ struct RBMat {
   // per cell, interleaved): a slot is one pair at c = (row & ~1) + doff, its two
   // values in planes 2 s and 2 s + 1
   int pair = 0;
+  // Alien's BEllPack (sliced ELL, SYCLBEllPackMatrix.cc:69-170) when > 0: the number of
+  // 64-row bundles per slice (slice height / 64); bo then holds the slice offsets (bro),
+  // bc / bv the BEllPack columns and values (padding: column 0, value 0)
+  int ell = 0;
 };
 
 // Scalars of one CG solve; lives in device memory, read by the host only to
@@ -107,6 +111,9 @@ template <typename T>
 void launch_cg_p(hipStream_t, int n, const T* z, T* p, CGState* st, int first);
 template <typename T>
 void launch_cg_q(hipStream_t, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st);
+// y = A x and *out = x . y by the kernel of launch_cg_q (idle: a state that is never done)
+template <typename T>
+void launch_spmv_dot_rb(hipStream_t, RedWork w, int n, RBMat<T> A, const T* x, T* y, CGState* idle, double* out);
 // invd_or_null != NULL fuses the Diag preconditioner (z = invD r, rho1 = r.z)
 template <typename T>
 void launch_cg_xr(hipStream_t, RedWork w, int n, const T* p, const T* q, T* x, T* r, const T* invd_or_null,

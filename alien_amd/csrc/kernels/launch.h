@@ -66,9 +66,16 @@ void launch_rbdia2_fill(hipStream_t, int n, const int* ro, const int* cols, cons
                         const int* doff, T* bv);
 long rbdia_structure_device(hipStream_t, int n, const int* ro, const int* cols, int max_w, std::vector<int>& bo,
                             DeviceArray<int>& d_bo, DeviceArray<int>& doff, DeviceArray<unsigned long long>& dmask);
+// y = A x on Alien's BEllPack arrays: one wavefront per 64 rows of a slice when the slice
+// height is a multiple of 64, else one lane per row
 template <typename T>
 void launch_spmv_bellpack(hipStream_t, int n, int slice, const int* bro, const int* bc, const T* bv,
                           const T* x, T* y);
+// BEllPack arrays (bro[nslices] * slice entries) from a device CSR; any of bc / bv / lmask /
+// umask may be NULL; dcol is read only for the masks
+template <typename T>
+void launch_bellpack_fill(hipStream_t, int n, int slice, const int* ro, const int* cols, const double* vals,
+                          const int* dcol, const int* bro, int* bc, T* bv, uint8_t* lmask, uint8_t* umask);
 void launch_addmult_bellpack(hipStream_t, int n, int slice, double alpha, const int* bro, const int* bc,
                              const double* bv, const uint8_t* mask, const double* x, double* y);
 void launch_addmult_csr(hipStream_t, int n, int upper, double alpha, const int* ro, const int* cols,

@@ -3,6 +3,7 @@
 // (bo[b] + j) * 64 + l).
 //
 //  * RB-CSR (column mode): each slot holds its column (-1 = padding).
+//  * BEllPack (slice mode): Alien's sliced ELL read in place, below.
 //  * RB-DIA (offset mode): slot j of bundle b is one "diagonal" of the bundle,
 //    the j-th smallest distinct column offset c - r among its rows' entries
 //    (doff[bo[b] + j]); bit l of dmask[bo[b] + j] says whether row 64 b + l has
@@ -129,6 +130,44 @@ __device__ __forceinline__ T rb_sum_pair_u(const RBMat<T, V>& A, int b, int s0, 
   return acc;
 }
 
+// BEllPack mode (RBMat::ell = E bundles per slice): the 64 rows of bundle b are rows
+// 64 (b % E) .. + 63 of slice s = b / E, whose entry j sits at
+// (bro[s] E + j E + b % E) 64 + lane: the wave reads 64 consecutive values and columns
+// per slot, slots 64 E elements apart. Every slot of the slice's width is summed,
+// padding (column 0, value 0) included, in slot order: MatrixInternal::mult's
+// arithmetic (SYCLBEllPackMatrix.cc:647-685), +0 * x[0] for each padding slot.
+template <int U, typename T, typename V, typename XS>
+__device__ __forceinline__ T ell_sum_u(const RBMat<T, V>& A, long base, long stride, int w, XS xs) {
+  T acc = 0;
+  for (int j = 0; j < w; j += U) {
+    int c[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long e = base + (long)min(j + u, w - 1) * stride;
+      c[u] = __builtin_nontemporal_load(A.bc + e);
+      v[u] = widen<T>(__builtin_nontemporal_load(A.bv + e));
+    }
+    T xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = xs(c[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j + u < w) acc = acc + v[u] * xv[u];
+  }
+  return acc;
+}
+template <typename T, typename V, typename XS>
+__device__ __forceinline__ T ell_sum(const RBMat<T, V>& A, int b, XS xs) {
+  const int E = A.ell;
+  const int s = b / E;
+  const int w = A.bo[s + 1] - A.bo[s];
+  if (w <= 0) return T(0);
+  const long stride = (long)E * kWave;
+  const long base = ((long)A.bo[s] * E + (b - s * E)) * kWave + (threadIdx.x & (kWave - 1));
+  return w <= 4 ? ell_sum_u<4, T>(A, base, stride, w, xs) : ell_sum_u<7, T>(A, base, stride, w, xs);
+}
+
 // Row sum of this lane's row of bundle b over the vector given by xs(column).
 // Short bundles (factors of M = G G^T: 4 slots at a 7-point stencil) use a
 // batch of 4, longer ones a batch of 7.
@@ -144,6 +183,7 @@ __device__ __forceinline__ T rb_sum_single(const RBMat<T, V>& A, int b, XS xs) {
 }
 template <typename T, typename V, typename XS>
 __device__ __forceinline__ T rb_sum(const RBMat<T, V>& A, int b, XS xs) {
+  if (A.ell) return ell_sum<T>(A, b, xs);
   if (A.pair) {
     const int s0 = A.bo[b];
     const int w = A.bo[b + 1] - s0;

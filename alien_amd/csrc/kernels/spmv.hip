@@ -247,8 +247,49 @@ __global__ void __launch_bounds__(kBlock) k_rbcsr_fill(int n, const int* __restr
 }
 
 // ---- BEllPack (Alien's sliced ELL) ------------------------------------------------------
+// y = A x read from the BEllPack arrays in place: one wavefront per 64 rows of a slice
+// (rb_rows.h, slice mode), batched coalesced value / column loads, the row's slots
+// summed in the reference's order (padding included).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_spmv_bellpack(int n, int slice, const int* __restrict__ bro,
+__global__ void __launch_bounds__(kRowBlock) k_spmv_bellpack(int n, RBMat<T> A, const T* __restrict__ x,
+                                                              T* __restrict__ y) {
+  spmv_rb_body<T>(n, A, x, y);
+}
+
+// Device conversion CSR -> BEllPack (the scatter of SYCLBEllPackMatrix.cc:127-170 and
+// 422-460, masks of :235-315): one lane per row slot of the padded slices (rows past n
+// included), so every storage entry is written: entry j of row r at
+// bro[s] slice + j slice + r % slice, column 0 / value 0 / masks 0 past the row's end.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bellpack_fill(int n, int ns, int slice, const int* __restrict__ ro,
+                                                           const int* __restrict__ cols,
+                                                           const double* __restrict__ vals,
+                                                           const int* __restrict__ dcol,
+                                                           const int* __restrict__ bro, int* __restrict__ bc,
+                                                           T* __restrict__ bv, uint8_t* __restrict__ lmask,
+                                                           uint8_t* __restrict__ umask) {
+  const long i = blockIdx.x * (long)kBlock + threadIdx.x;
+  if (i >= (long)ns * slice) return;
+  const int s = (int)(i / slice), l = (int)(i - (long)s * slice);
+  const int w = bro[s + 1] - bro[s];
+  const long base = (long)bro[s] * slice + l;
+  const int r0 = i < n ? ro[i] : 0;
+  const int len = i < n ? ro[i + 1] - r0 : 0;
+  const int dk = (i < n && (lmask || umask)) ? dcol[i] : 0;
+  for (int j = 0; j < w; ++j) {
+    const long e = base + (long)j * slice;
+    const bool on = j < len;
+    if (bc) bc[e] = on ? cols[r0 + j] : 0;
+    if (bv) bv[e] = on ? (T)vals[r0 + j] : T(0);
+    if (lmask) lmask[e] = (on && r0 + j < dk) ? 1 : 0;
+    if (umask) umask[e] = (on && r0 + j > dk) ? 1 : 0;
+  }
+}
+
+// the reference's one-work-item-per-row loop, for slice heights that are not a multiple
+// of the wavefront
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_bellpack_rows(int n, int slice, const int* __restrict__ bro,
                                                            const int* __restrict__ bc,
                                                            const T* __restrict__ bv,
                                                            const T* __restrict__ x, T* __restrict__ y) {
@@ -578,8 +619,24 @@ void launch_rbcsr_fill(hipStream_t st, int n, const int* ro, const int* cols, co
 template <typename T>
 void launch_spmv_bellpack(hipStream_t st, int n, int slice, const int* bro, const int* bc, const T* bv,
                           const T* x, T* y) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_spmv_bellpack<T>, rows_grid(n), dim3(kBlock), 0, st, n, slice, bro, bc, bv, x, y);
+  if (n <= 0) return;
+  if (slice % kWave == 0) {
+    RBMat<T> A{bro, bc, bv};
+    A.ell = slice / kWave;
+    const dim3 g((unsigned)bundle_blocks(n, (int)sizeof(T)));
+    hipLaunchKernelGGL(k_spmv_bellpack<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
+  } else {
+    hipLaunchKernelGGL(k_spmv_bellpack_rows<T>, rows_grid(n), dim3(kBlock), 0, st, n, slice, bro, bc, bv, x, y);
+  }
+}
+template <typename T>
+void launch_bellpack_fill(hipStream_t st, int n, int slice, const int* ro, const int* cols, const double* vals,
+                          const int* dcol, const int* bro, int* bc, T* bv, uint8_t* lmask, uint8_t* umask) {
+  const int ns = (n + slice - 1) / slice;
+  const long lanes = (long)ns * slice;
+  if (lanes > 0)
+    hipLaunchKernelGGL(k_bellpack_fill<T>, dim3((unsigned)((lanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n,
+                       ns, slice, ro, cols, vals, dcol, bro, bc, bv, lmask, umask);
 }
 void launch_addmult_bellpack(hipStream_t st, int n, int slice, double alpha, const int* bro, const int* bc,
                              const double* bv, const uint8_t* mask, const double* x, double* y) {
@@ -641,6 +698,8 @@ void launch_invdiag_bellpack(hipStream_t st, int n, int slice, const int* bro, c
                                      const int*, int*, T*);                                             \
   template void launch_spmv_bellpack<T>(hipStream_t, int, int, const int*, const int*, const T*,        \
                                         const T*, T*);                                                  \
+  template void launch_bellpack_fill<T>(hipStream_t, int, int, const int*, const int*, const double*,    \
+                                        const int*, const int*, int*, T*, uint8_t*, uint8_t*);          \
   template void launch_spmv_bsr2<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*);     \
   template void launch_spmv_rbbsr2<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*);   \
   template void launch_rbbsr2_fill<T>(hipStream_t, int, const int*, const int*, const double*, const int*, \

@@ -1384,7 +1384,7 @@ extern "C" int alien_hip_solve_csr(alien_hip_ctx* ctx, const alien_hip_solve_par
     Alien::CSRHostView view{n, Aro, Acols, Avals};
     HIPMatrix A(ctx, view, (fused || mixed) ? ALIEN_HIP_LAYOUT_CSR : p->layout, /*borrow_host=*/true);
     phase("matrix upload");
-    if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
+    if (fused || mixed) A.ensureFused(p->layout);
     phase("row-bundled layout");
     solve_on(ctx, p, A, b_in, x_io, res, tm, phase, &cptr);
     if (!perm.empty())
@@ -1522,7 +1522,7 @@ extern "C" int alien_hip_solve(alien_hip_ctx* ctx, const alien_hip_solve_params*
       b_in = pb.data();
       x_io = px.data();
     }
-    if (fused || mixed) A.ensureRB(p->layout != ALIEN_HIP_LAYOUT_RBCSR);
+    if (fused || mixed) A.ensureFused(p->layout);
     else A.setSpmvLayout(p->layout);
     phase("row-bundled layout");
     solve_on(ctx, p, A, b_in, x_io, res, tm, phase, coloured ? &m->cptr : nullptr);

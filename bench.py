@@ -17,9 +17,12 @@ it. Other workloads (--workload): convdiff (BiCGStab with ILU0 on level sets and
 multicolour, FILU0, Jacobi), blocks2 (two unknowns per cell), eval (unseen
 systems) and train (the network's training configuration).
 
-The roofline object covers the dominant kernel, the SpMV, timed with HIP
-events on the stream it runs on; `cpu_baseline` times the CPU oracle (a
-C++ restatement of Alien's SimpleCSR path) on a bounded sample.
+The roofline object covers the dominant kernel of that solve, k_cg_q<float>
+(q = A p with p.q on the FP32 row-bundled matrix), timed with HIP events on the
+stream it runs on; its HBM traffic is read from the committed rocprofv3 counter
+summary under profiles/. `cpu_baseline` is a measured solve of the same system
+by the CPU oracle (a C++ restatement of Alien's SimpleCSR path): Diag-CG to
+1e-8 on the host threads.
 """
 from __future__ import annotations
 
@@ -35,15 +38,53 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# the roofline line covers the FP64 RB-DIA SpMV (the row sum of the fused FP64 solvers)
-ROOF, ROOF_KERNEL = "rbdia_fp64", "k_spmv_rbdia<double>"
-# HBM bytes per launch at 256^3 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by
-# the same counter on a copy kernel of known size (profiles/spmv_lognormal256_hbm_traffic.md)
-TRAFFIC = {"rbdia_fp64": 1389471276, "rbdia_fp32": 719750450, "rbcsr_fp64": 1813950096, "rbcsr_fp32": 1144821033}
+# The roofline line covers the dominant kernel of the headline solve (learned + mixed CG):
+# k_cg_q<float>, q = A p with p.q on the FP32 RB-DIA matrix. bench.py times it through
+# alien_hip_spmv_dot_rbdia_f32, which launches that kernel (then its one-workgroup finalize).
+ROOF, ROOF_KERNEL = "cgq_fp32", "k_cg_q<float>"
+# the CSR-family kernel whose bytes are the CSR formula's (RB-CSR stores every CSR column)
+CSR_ROOF, CSR_ROOF_KERNEL = "rbcsr_fp64", "k_spmv_rbcsr<double>"
+PROFILES = os.path.join(ROOT, "profiles")
+# HBM bytes per launch at 256^3 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by the same
+# counter on a copy kernel of known size; kernel times inside the headline solve (rocprofv3 --stats)
+TRAFFIC_PROFILE = os.path.join(PROFILES, "spmv_lognormal256_hbm_traffic.md")
+STATS_PROFILE = os.path.join(PROFILES, "cg_lognormal256_kernel_stats.md")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def _profile_rows(path):
+    """Rows of the markdown tables of a committed profile summary: {kernel name: [cells]}."""
+    rows = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 3 and cells[0].startswith("`"):
+                    rows[cells[0].strip("`").split("` (")[0].replace("alien_hip::", "")] = cells[1:]
+    except OSError:
+        return {}
+    return rows
+
+
+def profile_traffic(kernel):
+    """Calibrated HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE summary."""
+    row = _profile_rows(TRAFFIC_PROFILE).get(kernel)
+    try:
+        return int(row[2].replace(",", "")) if row else None
+    except ValueError:
+        return None
+
+
+def profile_avg_us(kernel):
+    """Average duration (us) of `kernel` inside the headline solve, from the committed --stats summary."""
+    row = _profile_rows(STATS_PROFILE).get(kernel)
+    try:
+        return float(row[1]) if row else None
+    except ValueError:
+        return None
 
 
 def spmv_bytes_fp64(n, nnz):
@@ -53,6 +94,12 @@ def spmv_bytes_fp64(n, nnz):
 
 def spmv_bytes_fp32(n, nnz):
     return 8 * nnz + 4 * (n + 1) + 8 * n
+
+
+def bellpack_bytes(n, nnz, nslices, word):
+    """BEllPack: values and columns of the unpadded entries, x once, y, slice offsets (no row
+    offsets are read; padding is not credited)."""
+    return (word + 4) * nnz + 2 * word * n + 4 * (nslices + 1)
 
 
 def rbdia_bytes(n, nnz, slots, word):
@@ -82,7 +129,17 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     ddbo, ddoff, ddmask = dev(dbo_d), dev(doff), dev(dmask.view(np.int64))
     dv64 = torch.empty(dslots * 64, dtype=torch.float64, device="cuda")
     dv32 = torch.empty(dslots * 64, dtype=torch.float32, device="cuda")
+    # Alien's BEllPack (slice 1024), scattered on the device
+    bro = aa.bellpack_bro(ro, 1024)
+    btotal = int(bro[-1]) * 1024
+    dbro = dev(bro)
+    ebc = torch.empty(btotal, dtype=torch.int32, device="cuda")
+    ebv = torch.empty(btotal, dtype=torch.float64, device="cuda")
+    ebv32 = torch.empty(btotal, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
+    aa.check(L.alien_hip_bellpack_fill_f64(ctx.h, n, 1024, P(dro), P(dc), P(dv), None, P(dbro), P(ebc), P(ebv),
+                                           None, None))
+    aa.check(L.alien_hip_bellpack_fill_f32(ctx.h, n, 1024, P(dro), P(dc), P(dv), P(dbro), None, P(ebv32)))
     aa.check(L.alien_hip_rbcsr_fill_f64(ctx.h, n, P(dro), P(dc), P(dv), P(dbo), P(bc), P(bv)))
     aa.check(L.alien_hip_rbcsr_fill_f32(ctx.h, n, P(dro), P(dc), P(dv), P(dbo), P(bc), P(bv32)))
     aa.check(L.alien_hip_rbdia_fill_f64(ctx.h, n, P(dro), P(dc), P(dv), P(ddbo), P(ddoff), P(dv64)))
@@ -93,9 +150,14 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     y2 = torch.empty(n, dtype=torch.float64, device="cuda")
     y32 = torch.empty(n, dtype=torch.float32, device="cuda")
     y32b = torch.empty(n, dtype=torch.float32, device="cuda")
+    yq32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    ye64 = torch.empty(n, dtype=torch.float64, device="cuda")
+    ye32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    dotv = torch.empty(1, dtype=torch.float64, device="cuda")
     ctx.synchronize()
     torch.cuda.synchronize()
-    res = {"rbdia_slots": dslots}
+    res = {"rbdia_slots": dslots, "bellpack_slices": len(bro) - 1, "bellpack_padding":
+           btotal / max(len(vals), 1) - 1.0}
     kernels = (
         ("rbcsr_fp64", lambda: L.alien_hip_spmv_rbcsr_f64(ctx.h, n, P(dbo), P(bc), P(bv), P(x), P(y))),
         ("rbcsr_fp32", lambda: L.alien_hip_spmv_rbcsr_f32(ctx.h, n, P(dbo), P(bc), P(bv32), P(x32), P(y32))),
@@ -103,6 +165,12 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
                                                           P(y2))),
         ("rbdia_fp32", lambda: L.alien_hip_spmv_rbdia_f32(ctx.h, n, P(ddbo), P(ddoff), P(ddmask), P(dv32), P(x32),
                                                           P(y32b))),
+        ("cgq_fp32", lambda: L.alien_hip_spmv_dot_rbdia_f32(ctx.h, n, P(ddbo), P(ddoff), P(ddmask), P(dv32), P(x32),
+                                                            P(yq32), P(dotv))),
+        ("bellpack_fp64", lambda: L.alien_hip_spmv_bellpack_f64(ctx.h, n, 1024, P(dbro), P(ebc), P(ebv), P(x),
+                                                                P(ye64))),
+        ("bellpack_fp32", lambda: L.alien_hip_spmv_bellpack_f32(ctx.h, n, 1024, P(dbro), P(ebc), P(ebv32), P(x32),
+                                                                P(ye32))),
         ("csr_fp64", lambda: L.alien_hip_spmv_csr_f64(ctx.h, n, P(dro), P(dc), P(dv), P(x), P(y))))
     for name, fn in kernels:
         for _ in range(3):
@@ -118,7 +186,10 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     # the two layouts are the same sum: check once at full size
     res["rbdia_equals_rbcsr_fp64"] = bool(torch.equal(y, y2))
     res["rbdia_equals_rbcsr_fp32"] = bool(torch.equal(y32, y32b))
-    del bc, bv, bv32, dro, dc, dv, dbo, dv64, dv32, ddbo, ddoff, ddmask
+    res["cgq_equals_rbdia_fp32"] = bool(torch.equal(yq32, y32b))
+    res["bellpack_equals_rbcsr_fp64"] = bool(torch.equal(ye64, y))
+    res["bellpack_equals_rbcsr_fp32"] = bool(torch.equal(ye32, y32))
+    del bc, bv, bv32, dro, dc, dv, dbo, dv64, dv32, ddbo, ddoff, ddmask, dbro, ebc, ebv, ebv32
     torch.cuda.empty_cache()
     return res
 
@@ -198,21 +269,39 @@ def time_spmv_pair(aa, torch, ctx, ro, cols, vals, reps):
     return out
 
 
-def cpu_baseline(aa, ro, cols, vals, b, fp64_iters, budget_s):
-    """Oracle (C++ restatement of SimpleCSR) Diag-CG iterations on host threads; time-to-1e-8 is the
-    per-iteration time times the iteration count the FP64 GPU solve needed (same algorithm)."""
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(ro, cols, vals, b, n):
+    """The CPU oracle (C++ restatement of Alien's SimpleCSR path, FP64) solving the same system with
+    Diag-CG to relative residual 1e-8 on host threads, rows in contiguous blocks as the ranks of
+    examples/test_krylov.cpp:175-184 (dot products summed per block, then in block order): a measured
+    time to tolerance, set-up (inverse diagonal) reported apart."""
     try:
         import oracle as orc
     except Exception as e:  # pragma: no cover
         log("cpu_baseline unavailable:", e)
         return None
-    threads = min(16, os.cpu_count() or 1)
-    t1 = orc.time_cg_iters(ro, cols, vals, b, precond=orc.PREC_DIAG, n_iter=1, nthreads=threads)
-    k = max(1, min(50, int(budget_s / max(t1, 1e-6))))
-    t = orc.time_cg_iters(ro, cols, vals, b, precond=orc.PREC_DIAG, n_iter=k, nthreads=threads)
-    return {"value": t * fp64_iters, "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"{k} FP64 Diag-CG iterations at 256^3 on {threads} threads ({t * 1e3:.1f} ms/iter), "
-                      f"x {fp64_iters} iterations of the FP64 GPU solve"}
+    nproc = os.cpu_count() or 1
+    threads = min(16, nproc)  # the GPU box's CPU share for one GPU
+    _, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_DIAG, tol=1e-8, max_iter=20000,
+                     nthreads=threads)
+    log(f"cpu baseline: {r['iterations']} its, solve {r['solve_s']:.1f}s on {threads} threads, "
+        f"true_rel_res {r['true_rel_res']:.3e}")
+    return {"value": r["solve_s"], "unit": "s", "cores": threads, "nproc": nproc, "host_cpu": host_cpu_model(),
+            "kind": "port", "iterations": r["iterations"], "converged": bool(r["status"] == 1),
+            "true_rel_res": r["true_rel_res"], "setup_s": r["setup_s"],
+            "sample": f"one FP64 Diag-CG solve of the same {n}^3 system to relative residual 1e-8 "
+                      f"({r['iterations']} iterations) by the CPU oracle on {threads} threads in contiguous "
+                      f"row blocks"}
 
 
 def train_network(aa, ctx, a):
@@ -620,7 +709,6 @@ def main():
     ap.add_argument("--cheb-max-iter", type=int, default=3000)
     ap.add_argument("--ilu-max-iter", type=int, default=5000)
     ap.add_argument("--spmv-reps", type=int, default=20)
-    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernels and a reference copy kernel (for rocprofv3 --pmc runs)")
@@ -670,10 +758,13 @@ def main():
         for _ in range(a.spmv_reps):
             aa.check(aa.lib().alien_hip_copy_f64(ctx.h, N, P(src), P(dst)))
         ctx.synchronize()
-        print(json.dumps({"spmv_us": {k: v * 1e6 for k, v in sp.items() if isinstance(v, float)},
+        print(json.dumps({"spmv_us": {k: v * 1e6 for k, v in sp.items() if isinstance(v, float) and k != "bellpack_padding"},
                           "copy_bytes_each_way": 8 * N,
                           "algorithmic_bytes": {
                               "rbcsr_fp64": spmv_bytes_fp64(N, nnz), "rbcsr_fp32": spmv_bytes_fp32(N, nnz),
+                              "bellpack_fp64": bellpack_bytes(N, nnz, sp["bellpack_slices"], 8),
+                              "bellpack_fp32": bellpack_bytes(N, nnz, sp["bellpack_slices"], 4),
+                              "cgq_fp32": rbdia_bytes(N, nnz, sp["rbdia_slots"], 4) + 8 * 4096,
                               "rbdia_fp64": rbdia_bytes(N, nnz, sp["rbdia_slots"], 8),
                               "rbdia_fp32": rbdia_bytes(N, nnz, sp["rbdia_slots"], 4)}}), flush=True)
         ctx.close()
@@ -765,6 +856,16 @@ def main():
         spmv[name] = {"us": sp[name] * 1e6, "bytes": nbytes, "gbs": nbytes / sp[name] / 1e9,
                       "gflops": 2 * nnz / sp[name] / 1e9}
     spmv["rbdia_fp64"]["csr_equivalent_gbs"] = csr64 / sp["rbdia_fp64"] / 1e9
+    # k_cg_q<float>: the FP32 RB-DIA bytes (p is x: read once, its p.q reuse is the row's own entry)
+    # plus the 8-byte partial of each workgroup; timed with its finalize launch
+    q_bytes = rbdia_bytes(N, nnz, sp["rbdia_slots"], 4) + 8 * 4096
+    spmv["cgq_fp32"] = {"us": sp["cgq_fp32"] * 1e6, "bytes": q_bytes, "gbs": q_bytes / sp["cgq_fp32"] / 1e9,
+                        "gflops": 2 * nnz / sp["cgq_fp32"] / 1e9, "equals_spmv_rbdia_fp32": sp["cgq_equals_rbdia_fp32"]}
+    for name, word in (("bellpack_fp64", 8), ("bellpack_fp32", 4)):
+        nb = bellpack_bytes(N, nnz, sp["bellpack_slices"], word)
+        spmv[name] = {"us": sp[name] * 1e6, "bytes": nb, "gbs": nb / sp[name] / 1e9, "gflops": 2 * nnz / sp[name] / 1e9,
+                      "padding": sp["bellpack_padding"],
+                      "equals_rbcsr": sp[f"bellpack_equals_rbcsr_{name[-4:]}"]}
 
     ceiling = copy_ceiling(aa, torch, ctx, a.spmv_reps)
     if "learned_mixed" not in results:  # partial run (profiling): no JSON line
@@ -774,9 +875,14 @@ def main():
     speedups = {f"vs_{k}": (v["time_s"] / main_t if v["converged"] else None)
                 for k, v in results.items() if k != "learned_mixed"}
     cpu = None
-    if not a.no_cpu and "fp64_jacobi" in results:
+    if not a.no_cpu:
         log("cpu baseline (oracle)")
-        cpu = cpu_baseline(aa, ro, cols, vals, b, results["fp64_jacobi"]["iterations"], a.cpu_budget_s)
+        cpu = cpu_baseline(ro, cols, vals, b, n)
+    roof_frac = spmv[ROOF]["gbs"] / HBM_PEAK_GBS
+    if roof_frac > 1.0:
+        raise RuntimeError(f"{ROOF_KERNEL}: {spmv[ROOF]['gbs']:.0f} GB/s exceeds the HBM peak: bytes or time wrong")
+    beside = {k: {"us_in_solve": profile_avg_us(k), "source": os.path.relpath(STATS_PROFILE, ROOT)}
+              for k in ("k_factor_t<float, float>", "k_factor_p<float, float>", "k_cg_update<float>")}
 
     out = {
         "metric": "time-to-1e-8 residual (s), SpMV HBM GB/s; learned+mixed vs plain FP64",
@@ -800,9 +906,14 @@ def main():
         "kept_matrix": kept,
         "spmv": spmv,
         "roofline": {"bound": "hbm", "achieved": spmv[ROOF]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv[ROOF]["gbs"] / HBM_PEAK_GBS, "traffic": TRAFFIC.get(ROOF) if n == 256 else None,
+                     "frac": roof_frac, "traffic": profile_traffic(ROOF_KERNEL) if n == 256 else None,
                      "kernel": ROOF_KERNEL, "bytes_per_launch": spmv[ROOF]["bytes"],
-                     "copy_ceiling_gbs": ceiling},
+                     "us_per_launch": spmv[ROOF]["us"], "us_in_solve_profile": profile_avg_us(ROOF_KERNEL),
+                     "copy_ceiling_gbs": ceiling, "frac_of_copy_ceiling": spmv[ROOF]["gbs"] / ceiling,
+                     "csr_family": {"kernel": CSR_ROOF_KERNEL, "bytes_per_launch": spmv[CSR_ROOF]["bytes"],
+                                    "achieved": spmv[CSR_ROOF]["gbs"], "frac": spmv[CSR_ROOF]["gbs"] / HBM_PEAK_GBS,
+                                    "traffic": profile_traffic(CSR_ROOF_KERNEL) if n == 256 else None},
+                     "beside": beside},
         "cpu_baseline": cpu,
     }
     ctx.close()

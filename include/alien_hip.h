@@ -100,9 +100,24 @@ int alien_hip_spmv_csr_f64(alien_hip_ctx* ctx, int nrows, const int* row_offset,
 int alien_hip_spmv_csr_f32(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols,
                            const float* vals, const float* x, float* y);
 /* y = A x on Alien's sliced-ELL layout (slice height `slice`, padding col 0 / value 0):
- * MatrixInternal::mult (sycl/data/SYCLBEllPackMatrix.cc:623-688) */
+ * MatrixInternal::mult (sycl/data/SYCLBEllPackMatrix.cc:623-688), its per-row sums in its
+ * order (padding slots add 0 * x[0]). One wavefront per 64 rows of a slice when slice is a
+ * multiple of 64, else one lane per row. */
 int alien_hip_spmv_bellpack_f64(alien_hip_ctx* ctx, int nrows, int slice, const int* bro,
                                 const int* bcols, const double* bvals, const double* x, double* y);
+int alien_hip_spmv_bellpack_f32(alien_hip_ctx* ctx, int nrows, int slice, const int* bro,
+                                const int* bcols, const float* bvals, const float* x, float* y);
+/* The BEllPack arrays (bro[nslices] * slice entries; bro from alien_hip_host_bellpack_bro)
+ * scattered on the device from a device CSR: the block_cols / values scatter and the L/U
+ * masks of SYCLBEllPackMatrix.cc:127-170, 235-315, 422-460, equal to
+ * alien_hip_host_bellpack_fill's. Any of bcols / bvals / lmask / umask may be NULL; dcol
+ * (diagonal positions) is read for the masks only. */
+int alien_hip_bellpack_fill_f64(alien_hip_ctx* ctx, int nrows, int slice, const int* row_offset,
+                                const int* cols, const double* vals, const int* dcol, const int* bro,
+                                int* bcols, double* bvals, uint8_t* lmask, uint8_t* umask);
+int alien_hip_bellpack_fill_f32(alien_hip_ctx* ctx, int nrows, int slice, const int* row_offset,
+                                const int* cols, const double* vals, const int* bro, int* bcols,
+                                float* bvals);
 /* y += alpha * (strict lower | strict upper) A x through the BEllPack masks:
  * MatrixInternal::addLMult / addUMult (SYCLBEllPackMatrix.cc:769-911) */
 int alien_hip_addmult_bellpack_f64(alien_hip_ctx* ctx, int upper, double alpha, int nrows, int slice,
@@ -223,6 +238,15 @@ int alien_hip_spmv_rbdia_f64(alien_hip_ctx* ctx, int nrows, const int* bundle_of
                              const uint64_t* dmask, const double* bvals, const double* x, double* y);
 int alien_hip_spmv_rbdia_f32(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* doff,
                              const uint64_t* dmask, const float* bvals, const float* x, float* y);
+/* fused on RB-DIA: y = A x (the sequential CSR sum of _seqMult) and *result = x . y (FP64
+ * accumulation), the q = A p, p.q step of CG::solve (CG.h:133-134) as the fused CG iteration
+ * runs it (the same kernel). */
+int alien_hip_spmv_dot_rbdia_f64(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* doff,
+                                 const uint64_t* dmask, const double* bvals, const double* x, double* y,
+                                 double* result);
+int alien_hip_spmv_dot_rbdia_f32(alien_hip_ctx* ctx, int nrows, const int* bundle_off, const int* doff,
+                                 const uint64_t* dmask, const float* bvals, const float* x, float* y,
+                                 double* result);
 
 /* ------------------------------------------------------------------------ */
 /* Device kernels: BLAS-1 (CBLASMPIKernel.h:37-125 / SYCLKernelInternal.h:110-246) */

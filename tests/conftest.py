@@ -27,6 +27,9 @@ def _ensure_built():
         build.build()
     if not oracle.LIB_PATH.exists():
         oracle.build()
+    if not (ROOT / "tests" / "cxx" / "alien_api_check").exists():
+        import subprocess
+        subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cxx")], check=True)
 
 
 @pytest.fixture(scope="session", autouse=True)

@@ -20,7 +20,7 @@ def dev(a: np.ndarray) -> torch.Tensor:
 
 
 def empty(n, dtype=torch.float64) -> torch.Tensor:
-    fill = float("nan") if dtype.is_floating_point else -7
+    fill = float("nan") if dtype.is_floating_point else (0x5A if dtype == torch.uint8 else -7)
     t = torch.full((max(int(n), 1),), fill, dtype=dtype, device="cuda")
     torch.cuda.synchronize()
     return t

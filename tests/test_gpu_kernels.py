@@ -110,6 +110,18 @@ def test_spmv_rbdia_bit_exact(ctx, name, A):
     yc = empty(n, torch.float32)
     call("alien_hip_spmv_rbcsr_f32", ctx.h, n, p(dcbo), p(cbc), p(cbv), p(dx32), p(yc))
     np.testing.assert_array_equal(host(y32, ctx)[:n], host(yc, ctx)[:n])
+    # the fused CG iteration's q = A p, p.q kernel: the same sums, and x . y
+    res = empty(1)
+    yq = empty(n)
+    call("alien_hip_spmv_dot_rbdia_f64", ctx.h, n, p(dbo), p(ddo), p(ddm), p(bv), p(dx), p(yq), p(res))
+    np.testing.assert_array_equal(host(yq, ctx)[:n], ref)
+    assert abs(host(res, ctx)[0] - np.dot(ref, x)) <= 1e-13 * np.abs(ref * x).sum() + 1e-300
+    yq32 = empty(n, torch.float32)
+    call("alien_hip_spmv_dot_rbdia_f32", ctx.h, n, p(dbo), p(ddo), p(ddm), p(bv32), p(dx32), p(yq32), p(res))
+    y32h = host(y32, ctx)[:n]
+    np.testing.assert_array_equal(host(yq32, ctx)[:n], y32h)
+    e32 = np.dot(y32h.astype(np.float64), x.astype(np.float32).astype(np.float64))
+    assert abs(host(res, ctx)[0] - e32) <= 1e-13 * np.abs(y32h.astype(np.float64) * x).sum() + 1e-300
 
 
 def rbdia_device(ctx, dro, dc, n, max_width, pair=0):
@@ -190,6 +202,43 @@ def test_spmv_bellpack_bit_exact(ctx, name, A):
     call("alien_hip_assign_f64", ctx.h, n, C.c_double(1.0), p(d))
     call("alien_hip_invdiag_bellpack_f64", ctx.h, n, 1024, p(dbro), p(dbc), p(dbv), p(d))
     np.testing.assert_array_equal(host(d, ctx)[:n], orc.bellpack_invdiag(n, bro, bc, bv))
+
+
+@pytest.mark.parametrize("slice_", [64, 100, 1024])
+@pytest.mark.parametrize("name,A", SYSTEMS, ids=[s[0] for s in SYSTEMS])
+def test_bellpack_device_fill_and_spmv(ctx, name, A, slice_):
+    """BEllPack built on the device equals the host builder (columns, values, masks, padding),
+    and the SpMV (one wavefront per 64 rows of a slice when the slice height allows, else one
+    lane per row) keeps the reference's per-row order: FP64 bit-exact with the oracle's
+    MatrixInternal::mult restatement, FP32 with the sequential FP32 CSR sum."""
+    ro, cols, vals = A
+    n = len(ro) - 1
+    say("case bellpack device fill + spmv", name, n, slice_)
+    bro, bc, bv, lm, um = aa.bellpack_build(ro, cols, vals, slice_, masks=True)
+    total = len(bc)
+    dro, dc, dv, dbro = dev(ro), dev(cols), dev(vals), dev(bro)
+    dd = dev(aa.host_dcol(ro, cols))
+    gbc, gbv = empty(total, torch.int32), empty(total)
+    glm, gum = empty(total, torch.uint8), empty(total, torch.uint8)
+    call("alien_hip_bellpack_fill_f64", ctx.h, n, slice_, p(dro), p(dc), p(dv), p(dd), p(dbro), p(gbc), p(gbv),
+         p(glm), p(gum))
+    np.testing.assert_array_equal(host(gbc, ctx)[:total], bc)
+    np.testing.assert_array_equal(host(gbv, ctx)[:total], bv)
+    np.testing.assert_array_equal(host(glm, ctx)[:total], lm)
+    np.testing.assert_array_equal(host(gum, ctx)[:total], um)
+    x = np.random.default_rng(2).standard_normal(n)
+    dx = dev(x)
+    y = empty(n)
+    call("alien_hip_spmv_bellpack_f64", ctx.h, n, slice_, p(dbro), p(gbc), p(gbv), p(dx), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:n], orc.bellpack_spmv(n, bro, bc, bv, x, slice_))
+    gbv32 = empty(total, torch.float32)
+    call("alien_hip_bellpack_fill_f32", ctx.h, n, slice_, p(dro), p(dc), p(dv), p(dbro), None, p(gbv32))
+    np.testing.assert_array_equal(host(gbv32, ctx)[:total], bv.astype(np.float32))
+    dx32 = dev(x.astype(np.float32))
+    y32, yc32 = empty(n, torch.float32), empty(n, torch.float32)
+    call("alien_hip_spmv_bellpack_f32", ctx.h, n, slice_, p(dbro), p(gbc), p(gbv32), p(dx32), p(y32))
+    call("alien_hip_spmv_csr_f32", ctx.h, n, p(dro), p(dc), p(dev(vals.astype(np.float32))), p(dx32), p(yc32))
+    np.testing.assert_array_equal(host(y32, ctx)[:n], host(yc32, ctx)[:n])
 
 
 @pytest.mark.parametrize("name,A", SYSTEMS, ids=[s[0] for s in SYSTEMS])

@@ -535,17 +535,20 @@ def test_learned_bicgstab_on_two_unknown_system(ctx, mode):
     assert r["status"] == 1 and r["true_rel_res"] < 1.05e-10
 
 
+@pytest.mark.parametrize("layout", [aa.LAYOUT_CSR, aa.LAYOUT_BELLPACK], ids=["csr", "bellpack"])
 @pytest.mark.parametrize("factor_niter,solver_niter", [(0, 3), (2, 3), (5, 2)])
-def test_bicgstab_filu0_matches_oracle(ctx, factor_niter, solver_niter):
+def test_bicgstab_filu0_matches_oracle(ctx, factor_niter, solver_niter, layout):
     """FILU0 (FILU0Preconditioner.h, device branch): fixed-point factor sweeps and
     Jacobi triangular sweeps through addLMult/addUMult, against the oracle's
-    restatement on the same inputs: iterations within +-2."""
+    restatement on the same inputs: iterations within +-2. On the BEllPack layout
+    the masked products run on the BEllPack copy of the factor through its L/U
+    masks, the reference's SYCL path (SYCLBEllPackMatrix.cc:769-911)."""
     ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 20)
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
-    say("case BiCGStab FILU0", factor_niter, solver_niter)
+    say("case BiCGStab FILU0", factor_niter, solver_niter, layout)
     x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_FILU0, tol=1e-8, max_iter=3000,
-                        filu_factor_niter=factor_niter, filu_solver_niter=solver_niter)
+                        filu_factor_niter=factor_niter, filu_solver_niter=solver_niter, layout=layout)
     _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_FILU0, tol=1e-8, max_iter=3000,
                      filu_factor_niter=factor_niter, filu_solver_niter=solver_niter)
     say("  gpu", r["iterations"], r["status"], "oracle", o["iterations"], o["status"])
@@ -803,6 +806,43 @@ def test_kept_matrix_solves_equal_solve_csr(ctx, name, kw):
             assert r1["status"] == r2["status"]
             assert r1["iterations"] == r2["iterations"]
             np.testing.assert_array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_MIXED, aa.MODE_GRAPH], ids=["mixed", "graph"])
+def test_kept_matrix_solved_through_a_second_context(ctx, mode):
+    """A matrix created on one context and first solved through another: the row-bundled
+    layouts (FP64 and, for MIXED, FP32) are built on the matrix's own stream at that first
+    solve and must be complete before the other context's stream reads them. The iterates
+    equal a solve of the same values on the creating context, bit for bit."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 40, params=[1.5, 1, 1, 1], seed=9)
+    b = aa.uniform01(3, len(ro) - 1)
+    kw = dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=mode, tol=1e-8, max_iter=3000)
+    say("case kept matrix, second context", mode)
+    with aa.Context() as other:
+        with aa.Matrix(ctx, ro, cols, vals) as M:
+            x1, r1 = M.solve(b, ctx_override=other, **kw)
+        x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, **kw)
+    assert r1["status"] == r2["status"] == 1
+    assert r1["iterations"] == r2["iterations"]
+    np.testing.assert_array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED], ids=["fused", "graph", "mixed"])
+@pytest.mark.parametrize("solver", [aa.CG, aa.BICGS], ids=["cg", "bicgstab"])
+def test_fused_solvers_on_bellpack(ctx, solver, mode):
+    """The fused, graph and mixed solvers reading Alien's BEllPack arrays in place (slice
+    mode of the row kernels, FP64 and FP32 values): the row sums are RB-CSR's (padding adds
+    +0), so the iterates equal the RB-CSR layout's, bit for bit."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7 if solver == aa.CG else aa.GEN_CONVDIFF7, 20,
+                                params=[1.5, 1, 1, 1] if solver == aa.CG else None, seed=4)
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, len(ro) - 1))
+    say("case fused solver on bellpack", solver, mode)
+    kw = dict(solver=solver, precond=aa.PREC_DIAG, mode=mode, tol=1e-8, max_iter=3000)
+    x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_BELLPACK, **kw)
+    x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBCSR, **kw)
+    assert r1["status"] == r2["status"] == 1 and r1["true_rel_res"] < 1.05e-8
+    assert r1["iterations"] == r2["iterations"]
+    np.testing.assert_array_equal(x1, x2)
 
 
 def test_kept_matrix_refuses_what_it_cannot_run(ctx):
