@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p(int n, const T* __restrict__ 
 }
 
 // v = A phat ; v.r0
-template <typename T>
+template <typename T, bool ELL = false>
 __global__ void __launch_bounds__(kRowBlock) k_bicg_v(int n, RBMat<T> A, const T* __restrict__ phat,
                                                        const T* __restrict__ r0, T* __restrict__ v,
                                                        const BiCGState* st, RedWork w) {
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(kRowBlock) k_bicg_v(int n, RBMat<T> A, const T
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T r0v = r0[min(row, n - 1)];
-    const T t = rb_sum<T>(A, b, [&](int c) { return phat[c]; });
+    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return phat[c]; });
     if (row < n) {
       v[row] = t;
       s += (double)t * (double)r0v;
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s(int n, const T* __restrict__ 
 }
 
 // t = A shat ; t.s, t.t
-template <typename T>
+template <typename T, bool ELL = false>
 __global__ void __launch_bounds__(kRowBlock) k_bicg_t(int n, RBMat<T> A, const T* __restrict__ shat,
                                                        const T* __restrict__ sv, T* __restrict__ t,
                                                        const BiCGState* st, RedWork w) {
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(kRowBlock) k_bicg_t(int n, RBMat<T> A, const T
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T srow = sv[min(row, n - 1)];
-    const T tv = rb_sum<T>(A, b, [&](int c) { return shat[c]; });
+    const T tv = rb_sum<T, ELL>(A, b, [&](int c) { return shat[c]; });
     if (row < n) {
       t[row] = tv;
       ts += (double)tv * (double)srow;
@@ -190,9 +190,12 @@ void launch_bicg_p(hipStream_t s, int n, const T* r, const T* v, T* p, const T* 
 }
 template <typename T>
 void launch_bicg_v(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* phat, const T* r0, T* v, BiCGState* st) {
-  hipLaunchKernelGGL(k_bicg_v<T>, dim3((unsigned)rblocks<T>(n)), dim3(kRowBlock), 0, s, n, A, phat, r0, v,
-                     (const BiCGState*)st, w);
-  launch_finalize_bicg(s, w, rblocks<T>(n), 1, kBFinAlpha, st);
+  const dim3 g((unsigned)rb_blocks(n, A));
+  if (A.ell)
+    hipLaunchKernelGGL((k_bicg_v<T, true>), g, dim3(kRowBlock), 0, s, n, A, phat, r0, v, (const BiCGState*)st, w);
+  else
+    hipLaunchKernelGGL((k_bicg_v<T, false>), g, dim3(kRowBlock), 0, s, n, A, phat, r0, v, (const BiCGState*)st, w);
+  launch_finalize_bicg(s, w, rb_blocks(n, A), 1, kBFinAlpha, st);
 }
 template <typename T>
 void launch_bicg_s(hipStream_t s, RedWork w, int n, const T* r, const T* v, T* sv, const T* invd, T* shat,
@@ -207,9 +210,12 @@ void launch_bicg_s(hipStream_t s, RedWork w, int n, const T* r, const T* v, T* s
 }
 template <typename T>
 void launch_bicg_t(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* shat, const T* sv, T* t, BiCGState* st) {
-  hipLaunchKernelGGL(k_bicg_t<T>, dim3((unsigned)rblocks<T>(n)), dim3(kRowBlock), 0, s, n, A, shat, sv, t,
-                     (const BiCGState*)st, w);
-  launch_finalize_bicg(s, w, rblocks<T>(n), 2, kBFinOmega, st);
+  const dim3 g((unsigned)rb_blocks(n, A));
+  if (A.ell)
+    hipLaunchKernelGGL((k_bicg_t<T, true>), g, dim3(kRowBlock), 0, s, n, A, shat, sv, t, (const BiCGState*)st, w);
+  else
+    hipLaunchKernelGGL((k_bicg_t<T, false>), g, dim3(kRowBlock), 0, s, n, A, shat, sv, t, (const BiCGState*)st, w);
+  launch_finalize_bicg(s, w, rb_blocks(n, A), 2, kBFinOmega, st);
 }
 template <typename T>
 void launch_bicg_xr(hipStream_t s, RedWork w, int n, const T* shat, const T* phat, const T* sv, const T* t,

@@ -10,8 +10,11 @@ namespace alien_hip {
 namespace gnn {
 
 constexpr int kHidden = 32;
-constexpr int kFeat = 4;
-// flat FP32 parameter vector: W_in[4x32] b_in[32] W1[64x32] b1[32] W2[64x32] b2[32]
+// node features f_i = [1, s_i, 1 - s_i, max_j e_ij, sum_{j>i} e_ij^2, sum_{j<i} e_ij^2]
+// (s_i = sum_j e_ij): the squared couplings to later and to earlier rows carry what the
+// factorised sparse approximate inverse on A's lower pattern needs (its column scales)
+constexpr int kFeat = 6;
+// flat FP32 parameter vector: W_in[6x32] b_in[32] W1[64x32] b1[32] W2[64x32] b2[32]
 // W3[64x32] b3[32] w_d[32] c_d w_a[32] w_b[32] c_e   (row-major matrices)
 constexpr int kOffWin = 0;
 constexpr int kOffBin = kOffWin + kFeat * kHidden;
@@ -24,7 +27,7 @@ constexpr int kOffCd = kOffWd + kHidden;
 constexpr int kOffWa = kOffCd + 1;
 constexpr int kOffWb = kOffWa + kHidden;
 constexpr int kOffCe = kOffWb + kHidden;
-constexpr int kNParam = kOffCe + 1; // 6498
+constexpr int kNParam = kOffCe + 1; // 6562
 constexpr int kHeadGrads = kNParam - kOffWd; // w_d, c_d, w_a, w_b, c_e = 98
 constexpr int kMaxProbes = 8;
 constexpr int kMaxCgSteps = 200;

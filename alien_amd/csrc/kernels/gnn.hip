@@ -52,7 +52,7 @@ __global__ void k_prep_feat(int n, const int* __restrict__ ro, const int* __rest
                             float* __restrict__ f) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float s = 0.f, mx = 0.f;
+  float s = 0.f, mx = 0.f, up = 0.f, lo = 0.f;
   bool any = false;
   for (int k = ro[i]; k < ro[i + 1]; ++k) {
     const int j = cols[k];
@@ -62,13 +62,18 @@ __global__ void k_prep_feat(int n, const int* __restrict__ ro, const int* __rest
       s += v;
       mx = any ? fmaxf(mx, v) : v;
       any = true;
+      if (j > i) up += v * v;
+      else lo += v * v;
     }
     e[k] = v;
   }
-  f[i * 4 + 0] = 1.f;
-  f[i * 4 + 1] = s;
-  f[i * 4 + 2] = 1.f - s;
-  f[i * 4 + 3] = mx;
+  float* fi = f + (long)i * kFeat;
+  fi[0] = 1.f;
+  fi[1] = s;
+  fi[2] = 1.f - s;
+  fi[3] = mx;
+  fi[4] = up;
+  fi[5] = lo;
 }
 
 // ---- dense layer on MFMA --------------------------------------------------------------------
@@ -788,9 +793,9 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, int 
     cur ^= 1;
   }
   // ---- input layer
-  hipLaunchKernelGGL(k_dense_bwd<4>, g1(n), dim3(256), 0, s, n, dh[cur].data(), d_z[0].data(), P + kOffWin, kFeat,
+  hipLaunchKernelGGL(k_dense_bwd<kFeat>, g1(n), dim3(256), 0, s, n, dh[cur].data(), d_z[0].data(), P + kOffWin, kFeat,
                      dz.data(), (float*)nullptr, (float*)nullptr);
-  hipLaunchKernelGGL(k_wgrad_partials<4>, dim3((unsigned)nb), dim3(256), 0, s, n, d_f.data(), kFeat, kFeat,
+  hipLaunchKernelGGL(k_wgrad_partials<kFeat>, dim3((unsigned)nb), dim3(256), 0, s, n, d_f.data(), kFeat, kFeat,
                      (const float*)nullptr, 0, kFeat, dz.data(), part.data(), cap);
   hipLaunchKernelGGL(k_colsum, g1(kFeat * kHidden + kHidden), dim3(256), 0, s, kFeat * kHidden + kHidden, (int)nb,
                      part.data(), cap, grad + kOffWin, 0);

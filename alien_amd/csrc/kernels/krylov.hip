@@ -33,6 +33,8 @@
 #include "launch.h"
 #include "rb_rows.h"
 
+#include <type_traits>
+
 namespace alien_hip {
 
 #define EPT_INDEX(k) (blockIdx.x * (long)(kBlock * kEPT) + (k) * kBlock + threadIdx.x)
@@ -40,7 +42,7 @@ namespace alien_hip {
 // Row kernels walk their bundles with bundle_walk (device_common.h).
 
 // r = b - A x  (copy(b,r); mult(A,x,p); axpy(-1,p,r)); out = r.r
-template <typename T>
+template <typename T, bool ELL = false>
 __global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const T* __restrict__ x,
                                                       const T* __restrict__ rhs, T* __restrict__ r,
                                                       RedWork w) {
@@ -50,7 +52,7 @@ __global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T br = rhs[min(row, n - 1)];
-    const T t = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return x[c]; });
     if (row < n) {
       const T v = br + T(-1) * t;
       r[row] = v;
@@ -64,7 +66,7 @@ __global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const
 // One Chebyshev step (ChebyshevPreconditioner::_algo2). FIRST: y_in = x,
 // p = z, y' = x + a z. Otherwise p = b p + z, y' = y + a p. LAST also
 // accumulates rho1 = x . y' into the CG state.
-template <typename T, bool FIRST, bool LAST>
+template <typename T, bool FIRST, bool LAST, bool ELL = false>
 __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, const T* __restrict__ xin,
                                                        const T* __restrict__ yin, T* __restrict__ yout,
                                                        T* __restrict__ p, const T* __restrict__ invd, T a, T bcoef,
@@ -75,7 +77,7 @@ __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, cons
   const T* src = FIRST ? xin : yin;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T>(A, b, [&](int c) { return src[c]; });
+    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return src[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       const T xi = xin[row];
@@ -105,7 +107,7 @@ __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, cons
 // the algebra calls of its solve): yout = (yin + (-f) (A yin)) + x, the last
 // step scaled by f; with rdot the last step also stores the partials of
 // rdot . yout. Returns at entry once *done.
-template <typename T, bool LAST>
+template <typename T, bool LAST, bool ELL = false>
 __global__ void __launch_bounds__(kRowBlock) k_neumann_step(int n, RBMat<T> A, const T* __restrict__ x,
                                                           const T* __restrict__ yin, T* __restrict__ yout, T f,
                                                           const int* __restrict__ done, const T* __restrict__ rdot,
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__(kRowBlock) k_neumann_step(int n, RBMat<T> A, c
   double s = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T>(A, b, [&](int c) { return yin[c]; });
+    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return yin[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       T v = yin[row] + (-f) * t;
@@ -180,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_p(int n, const T* __restrict__ z,
 }
 
 // q = A p ; pq = p . q (a zero pq raises the breakdown flag in the finalize)
-template <typename T>
+template <typename T, bool ELL = false>
 __global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* __restrict__ p, T* __restrict__ q,
                                                      CGState* st, RedWork w) {
   if (st->done) return;
@@ -190,7 +192,7 @@ __global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* 
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T prow = p[min(row, n - 1)]; // unconditional (clamped): issued with the row's loads
-    const T t = rb_sum<T>(A, b, [&](int c) { return p[c]; });
+    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return p[c]; });
     if (row < n) {
       q[row] = t;
       s += (double)prow * (double)t;
@@ -378,30 +380,32 @@ static inline dim3 egrid(long n) { return dim3((unsigned)ept_blocks(n)); }
 
 template <typename T>
 void launch_residual(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out) {
-  hipLaunchKernelGGL(k_residual<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, A, x, b, r, w);
-  launch_finalize(s, w, rblocks<T>(n), 1, kFinStore, out, nullptr);
+  const dim3 g((unsigned)rb_blocks(n, A));
+  if (A.ell) hipLaunchKernelGGL((k_residual<T, true>), g, dim3(kRowBlock), 0, s, n, A, x, b, r, w);
+  else hipLaunchKernelGGL((k_residual<T, false>), g, dim3(kRowBlock), 0, s, n, A, x, b, r, w);
+  launch_finalize(s, w, rb_blocks(n, A), 1, kFinStore, out, nullptr);
 }
 
 template <typename T>
 void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin, const T* yin, T* yout, T* p,
                       const T* invd, double a, double b, bool first, bool last, CGState* st,
                       const int* done_flag) {
-  dim3 g = bgrid<T>(n), blk(kRowBlock);
+  dim3 g((unsigned)rb_blocks(n, A)), blk(kRowBlock);
   const int* done = st ? &st->done : done_flag;
   const int rho = st ? 1 : 0;
-  if (first && last)
-    hipLaunchKernelGGL((k_cheb_step<T, true, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done,
-                       rho, w);
-  else if (first)
-    hipLaunchKernelGGL((k_cheb_step<T, true, false>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done,
-                       rho, w);
-  else if (last)
-    hipLaunchKernelGGL((k_cheb_step<T, false, true>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done,
-                       rho, w);
-  else
-    hipLaunchKernelGGL((k_cheb_step<T, false, false>), g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b,
-                       done, rho, w);
-  if (last && st) launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
+  auto go = [&](auto first_c, auto last_c, auto ell_c) {
+    hipLaunchKernelGGL((k_cheb_step<T, decltype(first_c)::value, decltype(last_c)::value, decltype(ell_c)::value>), g,
+                       blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done, rho, w);
+  };
+  auto by_ell = [&](auto first_c, auto last_c) {
+    if (A.ell) go(first_c, last_c, std::true_type{});
+    else go(first_c, last_c, std::false_type{});
+  };
+  if (first && last) by_ell(std::true_type{}, std::true_type{});
+  else if (first) by_ell(std::true_type{}, std::false_type{});
+  else if (last) by_ell(std::false_type{}, std::true_type{});
+  else by_ell(std::false_type{}, std::false_type{});
+  if (last && st) launch_finalize(s, w, rb_blocks(n, A), 1, kFinRho1, nullptr, st);
 }
 
 // rho1 = r . z into the CG state (a preconditioner applied by separate launches)
@@ -428,18 +432,23 @@ void launch_rz(hipStream_t s, RedWork w, int n, const T* r, const T* z, CGState*
 template <typename T>
 long launch_neumann_apply(hipStream_t s, RedWork w, int n, RBMat<T> A, int order, double f, const T* x, T* tmp,
                           T* z, const int* done, const T* rdot) {
-  const dim3 g = bgrid<T>(n), blk(kRowBlock);
+  const dim3 g((unsigned)rb_blocks(n, A)), blk(kRowBlock);
   const T* yin = x;
   for (int i = 0; i < order; ++i) {
     T* out = ((order - 1 - i) % 2 == 0) ? z : tmp;
-    if (i == order - 1)
-      hipLaunchKernelGGL((k_neumann_step<T, true>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rdot, w);
+    const bool last = i == order - 1;
+    const T* rd = last ? rdot : (const T*)nullptr;
+    if (last && A.ell)
+      hipLaunchKernelGGL((k_neumann_step<T, true, true>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rd, w);
+    else if (last)
+      hipLaunchKernelGGL((k_neumann_step<T, true, false>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rd, w);
+    else if (A.ell)
+      hipLaunchKernelGGL((k_neumann_step<T, false, true>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rd, w);
     else
-      hipLaunchKernelGGL((k_neumann_step<T, false>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, (const T*)nullptr,
-                         w);
+      hipLaunchKernelGGL((k_neumann_step<T, false, false>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rd, w);
     yin = out;
   }
-  return (rdot && order > 0) ? rblocks<T>(n) : 0;
+  return (rdot && order > 0) ? rb_blocks(n, A) : 0;
 }
 
 template <typename T>
@@ -455,8 +464,10 @@ void launch_cg_p(hipStream_t s, int n, const T* z, T* p, CGState* st, int first)
 
 template <typename T>
 void launch_cg_q(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st) {
-  hipLaunchKernelGGL(k_cg_q<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, A, p, q, st, w);
-  launch_finalize(s, w, rblocks<T>(n), 1, kFinPQ, nullptr, st);
+  const dim3 g((unsigned)rb_blocks(n, A));
+  if (A.ell) hipLaunchKernelGGL((k_cg_q<T, true>), g, dim3(kRowBlock), 0, s, n, A, p, q, st, w);
+  else hipLaunchKernelGGL((k_cg_q<T, false>), g, dim3(kRowBlock), 0, s, n, A, p, q, st, w);
+  launch_finalize(s, w, rb_blocks(n, A), 1, kFinPQ, nullptr, st);
 }
 
 // the q = A p, p.q kernel of the fused iteration on its own: y = A x, *out = x . y (idle: a CG
@@ -467,8 +478,10 @@ void launch_spmv_dot_rb(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x,
     launch_finalize(s, w, 0, 1, kFinStore, out, nullptr);
     return;
   }
-  hipLaunchKernelGGL(k_cg_q<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, A, x, y, idle, w);
-  launch_finalize(s, w, rblocks<T>(n), 1, kFinStore, out, nullptr);
+  const dim3 g((unsigned)rb_blocks(n, A));
+  if (A.ell) hipLaunchKernelGGL((k_cg_q<T, true>), g, dim3(kRowBlock), 0, s, n, A, x, y, idle, w);
+  else hipLaunchKernelGGL((k_cg_q<T, false>), g, dim3(kRowBlock), 0, s, n, A, x, y, idle, w);
+  launch_finalize(s, w, rb_blocks(n, A), 1, kFinStore, out, nullptr);
 }
 
 template <typename T>

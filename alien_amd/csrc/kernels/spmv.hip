@@ -69,13 +69,13 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot_csr(int n, const int* __res
 // ---- RB-CSR / RB-DIA --------------------------------------------------------------
 // One wavefront per bundle (rb_rows.h); two entry points so that profiles name
 // the layout.
-template <typename T>
+template <typename T, bool ELL = false>
 __device__ __forceinline__ void spmv_rb_body(int n, const RBMat<T>& A, const T* __restrict__ x,
                                              T* __restrict__ y) {
   const int lane = threadIdx.x & (kWave - 1);
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T acc = rb_sum<T>(A, b, [&](int c) { return x[c]; });
+    const T acc = rb_sum<T, ELL>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + lane;
     if (row < n) y[row] = acc;
   }
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(kBlock) k_rbcsr_fill(int n, const int* __restr
 template <typename T>
 __global__ void __launch_bounds__(kRowBlock) k_spmv_bellpack(int n, RBMat<T> A, const T* __restrict__ x,
                                                               T* __restrict__ y) {
-  spmv_rb_body<T>(n, A, x, y);
+  spmv_rb_body<T, true>(n, A, x, y);
 }
 
 // Device conversion CSR -> BEllPack (the scatter of SYCLBEllPackMatrix.cc:127-170 and
@@ -543,8 +543,9 @@ void launch_spmv_dot_csr(hipStream_t st, RedWork w, int n, const int* ro, const 
 template <typename T>
 void launch_spmv_rb(hipStream_t st, int n, RBMat<T> A, const T* x, T* y) {
   if (n <= 0) return;
-  const dim3 g((unsigned)bundle_blocks(n, (int)sizeof(T)));
-  if (A.doff) hipLaunchKernelGGL(k_spmv_rbdia<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
+  const dim3 g((unsigned)rb_blocks(n, A));
+  if (A.ell) hipLaunchKernelGGL(k_spmv_bellpack<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
+  else if (A.doff) hipLaunchKernelGGL(k_spmv_rbdia<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
   else hipLaunchKernelGGL(k_spmv_rbcsr<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
 }
 template <typename T>
@@ -623,7 +624,7 @@ void launch_spmv_bellpack(hipStream_t st, int n, int slice, const int* bro, cons
   if (slice % kWave == 0) {
     RBMat<T> A{bro, bc, bv};
     A.ell = slice / kWave;
-    const dim3 g((unsigned)bundle_blocks(n, (int)sizeof(T)));
+    const dim3 g((unsigned)rb_blocks(n, A));
     hipLaunchKernelGGL(k_spmv_bellpack<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
   } else {
     hipLaunchKernelGGL(k_spmv_bellpack_rows<T>, rows_grid(n), dim3(kBlock), 0, st, n, slice, bro, bc, bv, x, y);

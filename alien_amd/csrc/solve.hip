@@ -1210,6 +1210,39 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       phase("factor fill + release");
       stall_window = p->stall_window > 0 ? p->stall_window : 400;
     }
+    // ILU0 in natural order (the reference's factor and sweeps, on level-set schedules): the
+    // preconditioner a stalled learned M hands over to, set up when that happens
+    std::unique_ptr<Alien::ILU0Preconditioner<Alg>> fb_ilu;
+    DeviceArray<float> fb_lu32;
+    auto ilu_fallback = [&](bool fp32) {
+      if (!fb_ilu) {
+        fb_ilu.reset(new Alien::ILU0Preconditioner<Alg>(alg, A));
+        fb_ilu->init();
+      }
+      PrecondSpec q;
+      q.kind = kPrecILU0Level;
+      LevelILU& L = q.lev;
+      L.nl = fb_ilu->lowerLevels();
+      L.nu = fb_ilu->upperLevels();
+      L.lptr = fb_ilu->lowerPtr();
+      L.uptr = fb_ilu->upperPtr();
+      L.lrows = fb_ilu->lowerRows();
+      L.urows = fb_ilu->upperRows();
+      L.lptr_dev = fb_ilu->lowerPtrDevice();
+      L.uptr_dev = fb_ilu->upperPtrDevice();
+      L.ro = A.d_ro.data();
+      L.cols = A.d_cols.data();
+      L.dcol = A.d_dcol.data();
+      L.lu64 = fb_ilu->luValues();
+      if (fp32) {
+        if (fb_lu32.empty()) {
+          fb_lu32.resize(A.getNnz());
+          launch_demote(ctx->stream, A.getNnz(), fb_ilu->luValues(), fb_lu32.data());
+        }
+        L.lu32 = fb_lu32.data();
+      }
+      return q;
+    };
     if (!mixed) {
       RBMat<double> rb = A.rb64();
       const bool graph = p->mode == ALIEN_HIP_MODE_GRAPH && capture;
@@ -1232,8 +1265,8 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
           res->fallback = 1;
           if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
           const int used = s.it;
-          FusedBiCG<double> J(ctx, n, rb, invd64, PrecondSpec{});
-          s = J.solve(b.data(), x.data(), p->tol, std::max(0, p->max_iter - used), graph, p->poll_every,
+          FusedBiCG<double> J(ctx, n, rb, invd64, ilu_fallback(false));
+          s = J.solve(b.data(), x.data(), p->tol, std::max(0, p->max_iter - used), false, p->poll_every,
                       nullptr);
           s.it += used;
         }
@@ -1256,12 +1289,12 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
         if (!stalled) finish_breakdown(F, s, &res->error);
       }
       if (stalled) {
-        // Jacobi from the last finite iterate, for the remaining iterations
+        // ILU0 from the last finite iterate, for the remaining iterations
         res->fallback = 1;
         if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
         const int used = s.it;
-        FusedCG<double> J(ctx, n, rb, invd64, PrecondSpec{});
-        s = J.solve(b.data(), x.data(), false, p->tol, std::max(0, p->max_iter - used), graph, p->poll_every,
+        FusedCG<double> J(ctx, n, rb, invd64, ilu_fallback(false));
+        s = J.solve(b.data(), x.data(), false, p->tol, std::max(0, p->max_iter - used), false, p->poll_every,
                     nullptr);
         finish_breakdown(J, s, &res->error);
         s.it += used;
@@ -1286,17 +1319,17 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       MixedResult m = run(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol, p->max_iter, delta,
                           p->poll_every, capture, stall_window);
       if (m.stalled) {
-        // Jacobi from the last reliable update, for the remaining iterations
+        // ILU0 from the last reliable update, for the remaining iterations
         res->fallback = 1;
         const int used = m.it, ups = m.updates;
-        m = run(ctx, n, rb64, rb32, invd32.data(), PrecondSpec{}, b.data(), x.data(), p->tol,
-                std::max(0, p->max_iter - used), delta, p->poll_every, capture, 0);
+        m = run(ctx, n, rb64, rb32, invd32.data(), ilu_fallback(true), b.data(), x.data(), p->tol,
+                std::max(0, p->max_iter - used), delta, p->poll_every, false, 0);
         m.it += used;
         m.updates += ups;
       }
       if (m.fp32_stalled) {
-        // FP64 Jacobi from the last reliable update, for the remaining iterations (a run
-        // with the learned M reports a stall instead, so Jacobi is the inner preconditioner)
+        // FP64 from the last reliable update, for the remaining iterations, with the FP64 form of
+        // the inner preconditioner (ILU0 for the learned M, its stall fallback)
         res->fallback |= 2;
         const int used = m.it;
         const int left = std::max(0, p->max_iter - used);
@@ -1311,17 +1344,21 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
           filu_fp64(p64);
         } else if (ps.kind == kPrecNeumann || ps.kind == kPrecCheb || ps.kind == kPrecILU0Level) {
           p64 = ps; // the FP64 solver applies them to A, invd and the FP64 factor
+        } else if (ps.kind == kPrecFactor) {
+          p64 = ilu_fallback(false); // the learned M's fallback
         }
         if (bicg) {
           FusedBiCG<double> F(ctx, n, rb64, invd64, p64);
-          const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, capture, p->poll_every, nullptr);
+          const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, capture && p64.kind != kPrecILU0Level,
+                                      p->poll_every, nullptr);
           m.it = used + s.it;
           m.status = s.status;
           m.error = s.breakdown ? 1 : 0;
           m.rel = std::sqrt(s.rr / alg.dot(b, b));
         } else {
           FusedCG<double> F(ctx, n, rb64, invd64, p64);
-          CGState s = F.solve(b.data(), x.data(), false, p->tol, left, capture, p->poll_every, nullptr);
+          CGState s = F.solve(b.data(), x.data(), false, p->tol, left, capture && p64.kind != kPrecILU0Level,
+                              p->poll_every, nullptr);
           finish_breakdown(F, s, &m.error);
           m.it = used + s.it;
           m.status = s.status;

@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # The roofline line covers the dominant kernel of the headline solve (learned + mixed CG):
 # k_cg_q<float>, q = A p with p.q on the FP32 RB-DIA matrix. bench.py times it through
 # alien_hip_spmv_dot_rbdia_f32, which launches that kernel (then its one-workgroup finalize).
-ROOF, ROOF_KERNEL = "cgq_fp32", "k_cg_q<float>"
+ROOF, ROOF_KERNEL = "cgq_fp32", "k_cg_q<float, false>"  # false: the row-bundled (not BEllPack) instance
 # the CSR-family kernel whose bytes are the CSR formula's (RB-CSR stores every CSR column)
 CSR_ROOF, CSR_ROOF_KERNEL = "rbcsr_fp64", "k_spmv_rbcsr<double>"
 PROFILES = os.path.join(ROOT, "profiles")
@@ -304,35 +304,55 @@ def cpu_baseline(ro, cols, vals, b, n):
                       f"row blocks"}
 
 
+def c3_system_params(rng):
+    """Log-normal coefficients of the training distribution: sigma ~ U[0.5, 3], per-axis
+    anisotropy ratios log-uniform in [1, 100]."""
+    return [float(rng.uniform(0.5, 3.0))] + [float(np.exp(rng.uniform(0.0, np.log(100.0)))) for _ in range(3)]
+
+
 def train_network(aa, ctx, a):
-    """Adam on generated log-normal systems (GPU forward/backward kernels, loss = residual reduction
-    after k PCG steps + probe term), then model selection
-    between the initial and the trained parameters by FP64 learned-PCG iterations on a held-out
-    48^3 system (sigma 2, seed not used in training)."""
+    """Adam on generated log-normal systems of the training distribution (GPU forward/backward
+    kernels; loss: Kaporin's condition-number bound, whose minimiser over the factor's pattern is
+    the factorised sparse approximate inverse), then model selection between the initial and the
+    trained parameters by FP64 learned-PCG iterations on 4 held-out systems of that distribution
+    (24^3, seeds not used in training)."""
     th0 = aa.gnn_init(0)
     t0 = time.time()
-    th1, hist = aa.gnn_train(ctx, th0, steps=a.train_steps, steps_per_system=4, n_min=16, n_max=32,
-                             sigma_min=0.5, sigma_max=3.0, aniso_max=4.0, lr=1e-3, beta=0.1, seed=7,
-                             cg_steps=a.train_cg_steps)
+    # phase 1: Kaporin's bound (its minimiser over the pattern is the factorised sparse approximate
+    # inverse); phase 2: the residual reduction after 16 PCG steps, the method's own objective
+    th1, hist = aa.gnn_train(ctx, th0, steps=a.train_steps, steps_per_system=1, n_min=8, n_max=16,
+                             sigma_min=0.5, sigma_max=3.0, aniso_max=100.0, lr=3e-3, beta=0.0, seed=7, cg_steps=0)
+    th1, hist2 = aa.gnn_train(ctx, th1, steps=a.train_cg_phase, steps_per_system=1, n_min=12, n_max=20,
+                              sigma_min=0.5, sigma_max=3.0, aniso_max=100.0, lr=1e-3, beta=0.0, seed=8,
+                              cg_steps=a.train_cg_steps)
     t_train = time.time() - t0
-    hro, hcols, hvals = aa.gen_csr(aa.GEN_LOGNORMAL7, 48, params=[2.0, 1.0, 1.0, 1.0], seed=99991)
-    hb = aa.uniform01(5, len(hro) - 1)
-    its = {}
-    for name, th in (("init", th0), ("trained", th1)):
-        _, r = aa.solve_csr(ctx, hro, hcols, hvals, hb, solver=aa.CG, precond=aa.PREC_LEARNED,
-                            mode=aa.MODE_FUSED, tol=1e-8, max_iter=5000, gnn_theta=th)
-        its[name] = r["iterations"] if r["status"] == 1 and not r["fallback"] else 10 ** 9
-    _, rj = aa.solve_csr(ctx, hro, hcols, hvals, hb, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED,
-                         tol=1e-8, max_iter=5000)
-    chosen = "trained" if its["trained"] < its["init"] else "init"
-    log(f"training: {a.train_steps} steps in {t_train:.1f}s, loss {hist[:8].mean():.4f} -> {hist[-8:].mean():.4f}; "
-        f"held-out 48^3 iterations init={its['init']} trained={its['trained']} jacobi={rj['iterations']} "
-        f"-> {chosen}")
-    info = {"steps": a.train_steps, "time_s": t_train, "loss_first8": float(hist[:8].mean()),
-            "loss_last8": float(hist[-8:].mean()), "heldout_48cubed_iterations": dict(its, jacobi=rj["iterations"]),
-            "chosen": chosen, "loss": (f"log residual reduction after {a.train_cg_steps} PCG steps" if a.train_cg_steps
-                                        else "Kaporin condition number") + " + 0.1 x |I - M A| probe term",
-            "systems": "log-normal 16^3..32^3, sigma U[0.5,3], anisotropy log-U[1,4], 4 Adam steps per system"}
+    rng = np.random.default_rng(99991)
+    its = {"init": [], "trained": [], "jacobi": []}
+    for s in range(5):
+        # 4 systems of the training distribution and one isotropic sigma-2 system (the headline's kind)
+        par = c3_system_params(rng) if s < 4 else [2.0, 1.0, 1.0, 1.0]
+        hro, hcols, hvals = aa.gen_csr(aa.GEN_LOGNORMAL7, 24 if s < 4 else 32, params=par, seed=99991 + s)
+        hb = aa.uniform01(5, len(hro) - 1)
+        for name, th in (("init", th0), ("trained", th1)):
+            _, r = aa.solve_csr(ctx, hro, hcols, hvals, hb, solver=aa.CG, precond=aa.PREC_LEARNED,
+                                mode=aa.MODE_FUSED, tol=1e-8, max_iter=5000, gnn_theta=th)
+            its[name].append(r["iterations"] if r["status"] == 1 and not r["fallback"] else 10 ** 9)
+        _, rj = aa.solve_csr(ctx, hro, hcols, hvals, hb, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED,
+                             tol=1e-8, max_iter=5000)
+        its["jacobi"].append(rj["iterations"])
+    chosen = "trained" if sum(its["trained"]) < sum(its["init"]) else "init"
+    log(f"training: {a.train_steps} + {a.train_cg_phase} steps in {t_train:.1f}s, Kaporin loss {hist[:8].mean():.4f} -> "
+        f"{hist[-8:].mean():.4f}, CG loss {hist2[:8].mean():.4f} -> {hist2[-8:].mean():.4f}; "
+        f"held-out iterations init={its['init']} trained={its['trained']} jacobi={its['jacobi']} -> {chosen}")
+    info = {"steps": [a.train_steps, a.train_cg_phase], "time_s": t_train,
+            "loss_kaporin_first8_last8": [float(hist[:8].mean()), float(hist[-8:].mean())],
+            "loss_cg_first8_last8": [float(hist2[:8].mean()), float(hist2[-8:].mean())], "heldout_iterations": its,
+            "heldout": "4 unseen log-normal 24^3 systems (sigma U[0.5,3], anisotropy log-U[1,100]) and one "
+                       "isotropic sigma-2 32^3 system",
+            "chosen": chosen,
+            "loss": f"Kaporin condition-number bound, then log residual reduction after {a.train_cg_steps} PCG steps",
+            "systems": "log-normal, sigma U[0.5,3], anisotropy log-U[1,100]: 8^3..16^3 (Kaporin), 12^3..20^3 (CG), "
+                       "one Adam step per system"}
     return (th1 if chosen == "trained" else th0), info
 
 
@@ -578,9 +598,10 @@ def run_blocks2(a):
 
 def run_eval(a):
     """Learned preconditioner on unseen systems (not the default line): `--seeds` log-normal systems
-    (sigma 2, seeds 1..S, none used in training) at n^3, CG to FP64 relative residual 1e-8 with the
-    learned M (mixed and FP64), Jacobi (mixed and FP64) and multicolour ILU0 (REFERENCE mode);
-    per-seed iterations and solve times and their summary."""
+    (seeds 1..S, none used in training; --eval-dist c3: sigma U[0.5,3] and anisotropy log-U[1,100],
+    the training distribution; iso: sigma 2 isotropic) at n^3, CG to FP64 relative residual 1e-8 with
+    the learned M (trained and initial parameters, mixed; trained, FP64), Jacobi (mixed and FP64) and
+    multicolour ILU0 (REFERENCE mode and mixed); per-seed iterations and solve times and their summary."""
     import torch
 
     import alien_amd as aa
@@ -593,6 +614,7 @@ def run_eval(a):
     else:
         theta, training = aa.gnn_init(0), {"steps": 0, "chosen": "init"}
     runs = [("learned_mixed", dict(mode=aa.MODE_MIXED, precond=aa.PREC_LEARNED, gnn_theta=theta)),
+            ("learned_mixed_init", dict(mode=aa.MODE_MIXED, precond=aa.PREC_LEARNED, gnn_theta=aa.gnn_init(0))),
             ("mixed_jacobi", dict(mode=aa.MODE_MIXED, precond=aa.PREC_DIAG)),
             ("fp64_learned", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_LEARNED, gnn_theta=theta)),
             ("fp64_jacobi", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
@@ -600,8 +622,10 @@ def run_eval(a):
                                            layout=aa.LAYOUT_RBCSR)),
             ("mixed_ilu0_multicolour", dict(mode=aa.MODE_MIXED, precond=aa.PREC_ILU0, ilu_schedule=1))]
     per = {name: {"iterations": [], "time_s": [], "setup_s": [], "converged": []} for name, _ in runs}
+    prng = np.random.default_rng(4242)
     for seed in range(1, a.seeds + 1):
-        ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=[2.0, 1.0, 1.0, 1.0], seed=seed)
+        par = c3_system_params(prng) if a.eval_dist == "c3" else [2.0, 1.0, 1.0, 1.0]
+        ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=par, seed=seed)
         xt = aa.uniform01(seed, len(ro) - 1)
         b = gpu_rhs(aa, torch, ctx, ro, cols, vals, xt)
         line = []
@@ -624,7 +648,9 @@ def run_eval(a):
     print(json.dumps({"metric": "time-to-1e-8 residual (s) on unseen systems, learned+mixed vs baselines",
                       "value": summary["learned_mixed"]["mean_time_s"], "unit": "s", "n_gpus": 1,
                       "higher_is_better": False, "dtype": "f32+f64", "data": "synthetic",
-                      "config": {"workload": f"log-normal diffusion {n}^3 (sigma=2), seeds 1..{a.seeds}, CG to 1e-8",
+                      "config": {"workload": f"log-normal diffusion {n}^3, seeds 1..{a.seeds}, " + (
+                                     "sigma U[0.5,3] and anisotropy log-U[1,100] (the training distribution)"
+                                     if a.eval_dist == "c3" else "sigma 2, isotropic") + ", CG to 1e-8",
                                  "N": n ** 3},
                       "training": training, "summary": summary, "per_seed": per}), flush=True)
     ctx.close()
@@ -633,9 +659,11 @@ def run_eval(a):
 def run_train(a):
     """Training configuration (not the default line): `--systems` generated log-normal systems of
     edge n in [32, 64], sigma ~ U[0.5, 3], per-axis anisotropy log-uniform in [1, 100], one Adam step
-    each (loss: residual reduction after --train-cg-steps PCG steps + probe term), then the initial
-    and the trained parameters on `--seeds` unseen n^3 systems (sigma 2), CG to FP64 1e-8 in the
-    mixed solver, beside Jacobi (mixed, FP64), multicolour ILU0 and Chebyshev (FP64)."""
+    each on Kaporin's bound, then --train-cg-phase steps on the residual reduction after
+    --train-cg-steps PCG steps,
+    then the initial and the trained parameters on `--seeds` unseen n^3 systems (--eval-dist: of the
+    training distribution, or isotropic sigma 2), CG to FP64 1e-8 in the mixed solver, beside Jacobi
+    (mixed, FP64), multicolour ILU0 and Chebyshev (FP64)."""
     import torch
 
     import alien_amd as aa
@@ -647,8 +675,10 @@ def run_train(a):
     log(f"training on {a.systems} systems (32^3..64^3)")
     t0 = time.time()
     th1, hist = aa.gnn_train(ctx, th0, steps=a.systems, steps_per_system=1, n_min=32, n_max=64, sigma_min=0.5,
-                             sigma_max=3.0, aniso_max=100.0, lr=1e-3, beta=0.1, seed=7,
-                             cg_steps=a.train_cg_steps)
+                             sigma_max=3.0, aniso_max=100.0, lr=3e-3, beta=0.0, seed=7, cg_steps=0)
+    th1, hist2 = aa.gnn_train(ctx, th1, steps=a.train_cg_phase, steps_per_system=1, n_min=32, n_max=48,
+                              sigma_min=0.5, sigma_max=3.0, aniso_max=100.0, lr=1e-3, beta=0.0, seed=8,
+                              cg_steps=a.train_cg_steps)
     t_train = time.time() - t0
     log(f"trained in {t_train:.1f}s, loss {hist[:50].mean():.4f} -> {hist[-50:].mean():.4f}")
     runs = [("learned_mixed_init", dict(mode=aa.MODE_MIXED, precond=aa.PREC_LEARNED, gnn_theta=th0)),
@@ -659,8 +689,10 @@ def run_train(a):
                                            layout=aa.LAYOUT_RBCSR)),
             ("fp64_chebyshev", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_CHEB, max_iter=a.cheb_max_iter))]
     per = {name: {"iterations": [], "time_s": [], "converged": []} for name, _ in runs}
+    prng = np.random.default_rng(4242)
     for seed in range(1, a.seeds + 1):
-        ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=[2.0, 1.0, 1.0, 1.0], seed=seed)
+        par = c3_system_params(prng) if a.eval_dist == "c3" else [2.0, 1.0, 1.0, 1.0]
+        ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=par, seed=seed)
         xt = aa.uniform01(seed, len(ro) - 1)
         b = gpu_rhs(aa, torch, ctx, ro, cols, vals, xt)
         line = []
@@ -680,10 +712,14 @@ def run_train(a):
                       "value": summary["learned_mixed_trained"]["mean_time_s"], "unit": "s", "n_gpus": 1,
                       "higher_is_better": False, "dtype": "f32+f64", "data": "synthetic",
                       "config": {"workload": f"training on {a.systems} log-normal systems 32^3..64^3 (sigma U[0.5,3], "
-                                             f"anisotropy log-U[1,100]); evaluation on {a.seeds} unseen {n}^3 systems",
+                                             f"anisotropy log-U[1,100]); evaluation on {a.seeds} unseen {n}^3 systems "
+                                             f"({a.eval_dist})",
                                  "N": n ** 3},
-                      "training": {"systems": a.systems, "time_s": t_train, "loss_first50": float(hist[:50].mean()),
-                                   "loss_last50": float(hist[-50:].mean()), "cg_steps": a.train_cg_steps},
+                      "training": {"systems": a.systems, "cg_phase_steps": a.train_cg_phase, "time_s": t_train,
+                                   "kaporin_loss_first50": float(hist[:50].mean()),
+                                   "kaporin_loss_last50": float(hist[-50:].mean()),
+                                   "cg_loss_first8_last8": [float(hist2[:8].mean()), float(hist2[-8:].mean())],
+                                   "cg_steps": a.train_cg_steps},
                       "summary": summary, "per_seed": per}), flush=True)
     ctx.close()
 
@@ -699,7 +735,11 @@ def main():
     ap.add_argument("--systems", type=int, default=2000, help="train workload: number of training systems")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--train-cg-steps", type=int, default=16,
-                    help="training loss: residual reduction after this many PCG steps (0: Kaporin's bound)")
+                    help="second training phase: residual reduction after this many PCG steps")
+    ap.add_argument("--train-cg-phase", type=int, default=90, help="Adam steps of the second training phase")
+    ap.add_argument("--eval-dist", default="c3", choices=["c3", "iso"],
+                    help="eval / train workloads: unseen systems of the training distribution (c3: sigma U[0.5,3], "
+                         "anisotropy log-U[1,100]) or isotropic sigma-2 systems (iso)")
     ap.add_argument("--no-slow-baselines", action="store_true",
                     help="skip the FP64 Chebyshev-CG and multicolour ILU0-CG baselines")
     ap.add_argument("--solvers", default="",
@@ -786,6 +826,7 @@ def main():
         return r, err
 
     runs = [("learned_mixed", aa.MODE_MIXED, aa.PREC_LEARNED, dict(inner_tol=0.1, gnn_theta=theta), a.steps),
+            ("learned_mixed_init", aa.MODE_MIXED, aa.PREC_LEARNED, dict(inner_tol=0.1, gnn_theta=aa.gnn_init(0)), 3),
             ("mixed_jacobi", aa.MODE_MIXED, aa.PREC_DIAG, dict(inner_tol=0.1), a.steps),
             ("fp64_jacobi", aa.MODE_GRAPH, aa.PREC_DIAG, {}, a.steps),
             ("fp64_learned", aa.MODE_GRAPH, aa.PREC_LEARNED, dict(gnn_theta=theta), 1)]
@@ -908,7 +949,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": spmv[ROOF]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": roof_frac, "traffic": profile_traffic(ROOF_KERNEL) if n == 256 else None,
                      "kernel": ROOF_KERNEL, "bytes_per_launch": spmv[ROOF]["bytes"],
-                     "us_per_launch": spmv[ROOF]["us"], "us_in_solve_profile": profile_avg_us(ROOF_KERNEL),
+                     "us_per_launch": spmv[ROOF]["us"], "timed_with": "its one-workgroup finalize (6 us)",
+                     "us_in_solve_profile": profile_avg_us(ROOF_KERNEL),
+                     "achieved_in_solve_profile": (spmv[ROOF]["bytes"] / profile_avg_us(ROOF_KERNEL) / 1e3
+                                                   if profile_avg_us(ROOF_KERNEL) else None),
                      "copy_ceiling_gbs": ceiling, "frac_of_copy_ceiling": spmv[ROOF]["gbs"] / ceiling,
                      "csr_family": {"kernel": CSR_ROOF_KERNEL, "bytes_per_launch": spmv[CSR_ROOF]["bytes"],
                                     "achieved": spmv[CSR_ROOF]["gbs"], "frac": spmv[CSR_ROOF]["gbs"] / HBM_PEAK_GBS,

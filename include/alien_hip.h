@@ -354,7 +354,8 @@ typedef struct {
   int    poll_every;    /* GRAPH: iterations per graph launch / stop-flag poll */
   const float* gnn_theta; /* LEARNED: network parameters (alien_hip_gnn_nparams() floats) */
   int    stall_window;  /* LEARNED: iterations without a tenfold drop of the residual norm
-                           (MIXED: without a reliable update) before the Jacobi fallback;
+                           (MIXED: without a reliable update) before the ILU0 fallback
+                           (natural order: the reference's factor and sweeps);
                            0 = 400. A non-finite residual falls back at once. */
   int    filu_factor_niter; /* FILU0: nb-factor-iter (0 = exact ILU(0) factor) */
   int    filu_solver_niter; /* FILU0: nb-solver-iter (test_krylov default 3) */
@@ -373,7 +374,7 @@ typedef struct {
   double cheb_beta;
   double setup_s;       /* conversion + preconditioner init, hipEvent-timed */
   double solve_s;       /* the Krylov solve, hipEvent-timed */
-  int    fallback;      /* bit 0 (LEARNED): the learned M stalled and Jacobi took over;
+  int    fallback;      /* bit 0 (LEARNED): the learned M stalled and ILU0 took over;
                            bit 1 (MIXED): the FP32 inner iteration overflowed, or (BiCGStab)
                            stagnated, and the solve finished in FP64 from the last reliable x */
   int    pad_;

@@ -6,7 +6,8 @@ its mathematics that the HIP forward/backward kernels are checked against
 
 Model (parameters in one flat FP32 vector, layout `LAYOUT` below):
   d_i = a_ii,  e_ij = -a_ij / sqrt(d_i d_j)  (j != i, on A's pattern)
-  f_i = [1, s_i, 1 - s_i, max_j e_ij],  s_i = sum_j e_ij
+  f_i = [1, s_i, 1 - s_i, max_j e_ij, u_i, l_i],  s_i = sum_j e_ij,
+        u_i = sum_{j > i} e_ij^2,  l_i = sum_{j < i} e_ij^2
   h0  = relu(f W_in + b_in)                                   (N x 32)
   h_l = h_{l-1} + relu([h_{l-1}, S h_{l-1}] W_l + b_l), l=1..3, S_ij = e_ij
   Ghat_ii = exp(h3_i . w_d + c_d)
@@ -23,7 +24,7 @@ import numpy as np
 import scipy.sparse as sp
 
 H = 32
-F = 4
+F = 6
 LAYOUT = [("W_in", (F, H)), ("b_in", (H,)),
           ("W1", (2 * H, H)), ("b1", (H,)),
           ("W2", (2 * H, H)), ("b2", (H,)),
@@ -94,7 +95,11 @@ class Graph:
         for i in range(n):
             row = Ec.data[Ec.indptr[i]:Ec.indptr[i + 1]]
             mx[i] = row.max() if len(row) else 0.0
-        self.f = np.stack([np.ones(n), s, 1.0 - s, mx], axis=1)
+        EU = sp.triu(self.E, 1).tocsr()
+        ELs = sp.tril(self.E, -1).tocsr()
+        u = np.asarray(EU.multiply(EU).sum(axis=1)).ravel()
+        lo = np.asarray(ELs.multiply(ELs).sum(axis=1)).ravel()
+        self.f = np.stack([np.ones(n), s, 1.0 - s, mx, u, lo], axis=1)
         L = self.EL.tocoo()
         self.li, self.lj, self.le = L.row, L.col, L.data  # strictly lower edges (i > j)
 

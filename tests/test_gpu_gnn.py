@@ -8,6 +8,7 @@ by 1e-7 relative moves Ghat by < 1e-4 of its max and the gradient by < 1e-5 in
 norm; the bounds below leave a factor >= 10 over that.
 """
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import alien_amd as aa
@@ -43,7 +44,7 @@ def lower_mask(ro, cols):
 
 
 def test_nparams():
-    assert aa.gnn_nparams() == go.NPARAM == 6498
+    assert aa.gnn_nparams() == go.NPARAM == 6562
 
 
 @pytest.mark.parametrize("n", [6, 12])
@@ -154,6 +155,109 @@ def test_training_lowers_the_objective(ctx):
     L1, _, _ = aa.gnn_loss_grad(ctx, ro, cols, vals, th1, V, beta=0.1)
     say("  held-out objective", L0, "->", L1)
     assert L1 < L0
+
+
+def heldout_anisotropic(n=16, count=4):
+    """Unseen systems of the training distribution (sigma ~ U[0.5, 3], per-axis anisotropy
+    log-uniform in [1, 100]), seeds not drawn by the trainer."""
+    rng = np.random.default_rng(123)
+    out = []
+    for s in range(count):
+        sig = rng.uniform(0.5, 3.0)
+        an = list(np.exp(rng.uniform(0.0, np.log(100.0), 3)))
+        out.append(aa.gen_csr(aa.GEN_LOGNORMAL7, n, params=[sig] + an, seed=9000 + s))
+    return out
+
+
+def fsai_lower(ro, cols, vals):
+    """Factorised sparse approximate inverse on A's lower pattern, row by row on dense
+    local systems (Kolotilina & Yeremin): the minimiser of Kaporin's bound over that
+    pattern, M = G G^T with G = D^-1/2 Ghat lower triangular."""
+    n = len(ro) - 1
+    A = sp.csr_matrix((vals, cols, ro), shape=(n, n))
+    d = A.diagonal()
+    Dm = sp.diags(1.0 / np.sqrt(d))
+    Ah = (Dm @ A @ Dm).tocsr()
+    U = sp.triu(Ah).tocsr()
+    r, c, v = [], [], []
+    for i in range(n):
+        P = U.indices[U.indptr[i]:U.indptr[i + 1]]
+        k = int(np.where(P == i)[0][0])
+        e = np.zeros(len(P))
+        e[k] = 1.0
+        y = np.linalg.solve(Ah[P][:, P].toarray(), e)
+        y /= np.sqrt(y[k])
+        r += [i] * len(P)
+        c += list(P)
+        v += list(y)
+    Z = sp.csr_matrix((v, (r, c)), shape=(n, n))
+    return (Dm @ Z.T).tocsr()
+
+
+def factor_pcg_iterations(ro, cols, vals, G, b, tol=1e-8, maxit=5000):
+    n = len(ro) - 1
+    A = sp.csr_matrix((vals, cols, ro), shape=(n, n))
+    GT = G.T.tocsr()
+    x, r = np.zeros(n), b.copy()
+    z = G @ (GT @ r)
+    p, rz, bb = z.copy(), r @ z, b @ b
+    for k in range(1, maxit + 1):
+        q = A @ p
+        a = rz / (p @ q)
+        x += a * p
+        r -= a * q
+        if r @ r < tol * tol * bb:
+            return k
+        z = G @ (GT @ r)
+        rz1 = r @ z
+        p = z + (rz1 / rz) * p
+        rz = rz1
+    return maxit
+
+
+def test_training_gains_on_unseen_systems(ctx):
+    """What training gains: Adam on the GPU, 300 steps on Kaporin's bound (systems of edge 8-16
+    drawn from sigma U[0.5, 3] and per-axis anisotropy log-U[1, 100]), then 90 steps on the
+    residual reduction after 16 PCG steps (edge 12-20), against the initial parameters (the
+    first-order factor I + E_lower), fused FP64 learned CG to 1e-8 on unseen systems:
+    * 4 anisotropic 16^3 systems of the training distribution: at least 5% fewer iterations in
+      total, never more than 2 more on any one, and within 5% of the optimum of the factor's
+      pattern for Kaporin's bound (FSAI, computed in numpy, printed beside);
+    * an isotropic sigma-2 32^3 system (the headline's kind, where FSAI is no better than the
+      first-order factor): no more iterations than the initial parameters."""
+    th0 = aa.gnn_init(0)
+    say("gnn train 300 Kaporin steps + 90 CG-loss steps on the training distribution")
+    th1, hist = aa.gnn_train(ctx, th0, steps=300, steps_per_system=1, n_min=8, n_max=16, sigma_min=0.5,
+                             sigma_max=3.0, aniso_max=100.0, lr=3e-3, beta=0.0, seed=7, cg_steps=0)
+    th1, hist2 = aa.gnn_train(ctx, th1, steps=90, steps_per_system=1, n_min=12, n_max=20, sigma_min=0.5,
+                              sigma_max=3.0, aniso_max=100.0, lr=1e-3, beta=0.0, seed=8, cg_steps=16)
+    assert np.all(np.isfinite(hist)) and np.all(np.isfinite(hist2)) and np.all(np.isfinite(th1))
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 32, params=[2.0, 1.0, 1.0, 1.0], seed=77)
+    b = aa.uniform01(5, len(ro) - 1)
+    iso = []
+    for th in (th0, th1):
+        _, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_FUSED,
+                            tol=1e-8, max_iter=5000, gnn_theta=th)
+        assert r["status"] == 1 and r["fallback"] == 0
+        iso.append(r["iterations"])
+    say("  isotropic 32^3: init", iso[0], "trained", iso[1])
+    assert iso[1] <= iso[0]
+    tot0 = tot1 = totf = 0
+    for ro, cols, vals in heldout_anisotropic():
+        b = aa.uniform01(5, len(ro) - 1)
+        its = []
+        for th in (th0, th1):
+            _, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_FUSED,
+                                tol=1e-8, max_iter=5000, gnn_theta=th)
+            assert r["status"] == 1 and r["fallback"] == 0
+            its.append(r["iterations"])
+        f = factor_pcg_iterations(ro, cols, vals, fsai_lower(ro, cols, vals), b)
+        say("  held-out: init", its[0], "trained", its[1], "FSAI", f)
+        assert its[1] <= its[0] + 2
+        tot0, tot1, totf = tot0 + its[0], tot1 + its[1], totf + f
+    say("  total: init", tot0, "trained", tot1, "FSAI", totf)
+    assert tot1 <= 0.95 * tot0
+    assert tot1 <= 1.05 * totf
 
 
 def oracle_factor(ro, cols, vals, th):
@@ -270,18 +374,27 @@ def broken_theta(which):
 
 @pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
 @pytest.mark.parametrize("which", ["overflow", "stagnation"])
-def test_fallback_to_jacobi(ctx, mode, which):
+def test_fallback_to_ilu0(ctx, mode, which):
+    """A learned M that overflows or stagnates hands the solve over to ILU0 (natural order,
+    the reference's factor) from the last good iterate: it converges, on the FP64 true
+    residual, within the stall window plus about an ILU0 solve's iterations (Jacobi would
+    need several times more)."""
     n = 12
     ro, cols, vals = system(n)
     b = aa.uniform01(42, n ** 3)
     say("learned fallback", which, mode)
     x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_LEARNED, mode=mode, tol=1e-8,
                         max_iter=3000, gnn_theta=broken_theta(which), stall_window=50, poll_every=4)
-    say("  ", r)
+    _, ri = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, tol=1e-8,
+                         max_iter=3000)
+    _, rj = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED, tol=1e-8,
+                         max_iter=3000)
+    say("  ", r, "ilu0 from zero", ri["iterations"], "jacobi from zero", rj["iterations"])
     assert r["fallback"] == 1
     assert r["status"] == 1
     assert np.all(np.isfinite(x))
     assert r["true_rel_res"] < 1.05e-8
+    assert r["iterations"] <= 2 * 50 + 4 + 1.5 * ri["iterations"]
 
 
 def test_learned_rejects_bad_input(ctx):

@@ -197,7 +197,7 @@ def test_gnn_init_parameters():
     assert p["c_e"][0] == 1.0 and p["c_d"][0] == 0.0
     for k in ("w_d", "w_a", "w_b", "b_in", "b1", "b2", "b3"):
         assert np.all(p[k] == 0.0), k
-    assert abs(np.std(p["W_in"]) - 0.5) < 0.1
+    assert abs(np.std(p["W_in"]) - 1.0 / np.sqrt(go.F)) < 0.1
     for l in (1, 2, 3):
         assert abs(np.std(p[f"W{l}"]) - 0.1 / 8) < 0.002
     # with these heads the oracle's factor is D^-1/2 (I + E_lower) whatever the layers hold
