@@ -65,6 +65,7 @@ _SIGS = [
     ("alien_hip_ctx_destroy", C.c_int, [c_void_p]),
     ("alien_hip_ctx_synchronize", C.c_int, [c_void_p]),
     ("alien_hip_ctx_stream", c_void_p, [c_void_p]),
+    ("alien_hip_ctx_step_timing", C.c_int, [c_void_p, C.c_int, c_double_p, C.POINTER(C.c_longlong)]),
     ("alien_hip_gen_nnz", C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
     ("alien_hip_gen_csr", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, c_double_p, C.c_uint64, c_int_p, c_int_p,
                                     c_double_p]),
@@ -372,6 +373,13 @@ class Context:
     @property
     def stream(self) -> int:
         return lib().alien_hip_ctx_stream(self.h)
+
+    def step_timing(self, on: bool):
+        """Switch the fused CG solvers' step-kernel timing (q = A p, HIP events on this stream)
+        on or off; returns the (microseconds, launches) totals gathered before the switch."""
+        us, k = C.c_double(), C.c_longlong()
+        check(lib().alien_hip_ctx_step_timing(self.h, 1 if on else 0, C.byref(us), C.byref(k)))
+        return us.value, k.value
 
     def close(self):
         if self.h:

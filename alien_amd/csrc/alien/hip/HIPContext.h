@@ -84,6 +84,11 @@ struct alien_hip_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool trace = false; // ALIEN_HIP_TRACE=1 at creation: set-up phase times on stderr
+  // Step-kernel timing (alien_hip_ctx_step_timing): when on, the fused CG solvers bracket each
+  // launch of their q = A p kernel with HIP events on `stream` and add up the durations here
+  bool step_timing = false;
+  double step_us = 0.0;
+  long long step_launches = 0;
   // reduction workspace (see RedWork); grows on demand, never while a graph is captured
   alien_hip::DeviceArray<double> partials;
   int red_cap = 0;

@@ -129,6 +129,19 @@ int alien_hip_ctx_synchronize(alien_hip_ctx* ctx) {
 
 void* alien_hip_ctx_stream(alien_hip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int alien_hip_ctx_step_timing(alien_hip_ctx* ctx, int on, double* total_us, long long* launches) {
+  REQUIRE(ctx, "ctx is NULL");
+  API_BEGIN
+  if (total_us) *total_us = ctx->step_us;
+  if (launches) *launches = ctx->step_launches;
+  if (on) {
+    ctx->step_us = 0.0;
+    ctx->step_launches = 0;
+  }
+  ctx->step_timing = on != 0;
+  API_END
+}
+
 // ---- host generators / builders ------------------------------------------------
 int64_t alien_hip_gen_nnz(int kind, int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) return -1;

@@ -313,7 +313,33 @@ struct FusedCG {
   }
   ~FusedCG() {
     release_graph();
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
     if (h_st) (void)hipHostFree(h_st);
+  }
+
+  // Step-kernel timing (ctx->step_timing): iteration slot i of an advance brackets its q = A p
+  // launch with events tev[2 i], tev[2 i + 1]; the durations of the slots that ran an iteration
+  // are added to the context's totals after the advance's state read.
+  std::vector<hipEvent_t> tev;
+  void ensure_timing_events(int k) {
+    while ((int)tev.size() < 2 * k) {
+      hipEvent_t e;
+      ALIEN_HIP_CHECK(hipEventCreate(&e));
+      tev.push_back(e);
+    }
+  }
+  void timed_cg_q(hipStream_t s, int tslot) {
+    if (tslot >= 0) ALIEN_HIP_CHECK(hipEventRecord(tev[2 * tslot], s));
+    launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
+    if (tslot >= 0) ALIEN_HIP_CHECK(hipEventRecord(tev[2 * tslot + 1], s));
+  }
+  void collect_timing(int ran) {
+    for (int i = 0; i < ran; ++i) {
+      float ms = 0.f;
+      ALIEN_HIP_CHECK(hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]));
+      ctx->step_us += 1e3 * (double)ms;
+      ++ctx->step_launches;
+    }
   }
 
   // z = M r and rho1 = r.z into the state
@@ -364,21 +390,22 @@ struct FusedCG {
   // One CG update: z = M r, p = z (+ beta p), q = A p, x, r, stop test. With
   // the Diag preconditioner cg_xr already produced z and rho1 for this
   // iteration, except on the first one.
-  void iteration(T* x, bool first) {
+  // (tslot >= 0: timed as step-timing slot tslot)
+  void iteration(T* x, bool first, int tslot = -1) {
     hipStream_t s = ctx->stream;
     const bool diag = P.kind == kPrecDiag;
     if (P.kind == kPrecFactor) {
       // z = G (G^T r) never stored: rho1 = |G^T r|^2 and p = G t + beta p; the stop test of this
       // update runs in the next iteration's first finalize, so the update has no reduction
       factor_p<T>(s, ctx->red(), n, P, r.data(), tmp.data(), p.data(), st.data(), first ? 1 : 0);
-      launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
+      timed_cg_q(s, tslot);
       launch_cg_update<T>(s, n, p.data(), q.data(), x, r.data(), st.data());
       return;
     } else {
       if (!diag || first) precond(r.data(), z.data());
       launch_cg_p<T>(s, n, z.data(), p.data(), st.data(), first ? 1 : 0);
     }
-    launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
+    timed_cg_q(s, tslot);
     launch_cg_xr<T>(s, ctx->red(), n, p.data(), q.data(), x, r.data(), diag ? invd : nullptr,
                     diag ? z.data() : nullptr, st.data());
   }
@@ -433,15 +460,23 @@ struct FusedCG {
     }
     ALIEN_HIP_CHECK(hipGraphLaunch(ge, ctx->stream));
   }
-  // k more iterations (fewer if the device stop flag is raised), then the state
+  // k more iterations (fewer if the device stop flag is raised), then the state. With step
+  // timing on, the iterations are launched directly (events recorded by a graph's nodes carry
+  // no timestamps here): the same kernels in the same order on the same stream.
   CGState advance(T* x, bool graph, int k) {
-    if (graph) {
+    const bool timed = ctx->step_timing;
+    const int it0 = h_st->it; // the state last read or written
+    if (timed) ensure_timing_events(k);
+    if (graph && !timed) {
       launch_graph(x, k);
     } else {
-      for (int i = 0; i < k; ++i) iteration(x, false);
+      for (int i = 0; i < k; ++i) iteration(x, false, timed ? i : -1);
       check_launch("fused CG iteration");
     }
-    return read_state();
+    const CGState cur = read_state();
+    // slots after the stop only returned at entry: count those that ran an iteration
+    if (timed) collect_timing(std::max(0, std::min(k, cur.it - it0)));
+    return cur;
   }
 
   // Solve A x = b from the given x (x_is_zero: skip the initial residual SpMV).

@@ -827,6 +827,8 @@ def main():
 
     runs = [("learned_mixed", aa.MODE_MIXED, aa.PREC_LEARNED, dict(inner_tol=0.1, gnn_theta=theta), a.steps),
             ("learned_mixed_init", aa.MODE_MIXED, aa.PREC_LEARNED, dict(inner_tol=0.1, gnn_theta=aa.gnn_init(0)), 3),
+            ("learned_mixed_bf16_factor", aa.MODE_MIXED, aa.PREC_LEARNED,
+             dict(inner_tol=0.1, gnn_theta=theta, factor_storage=1), 3),
             ("mixed_jacobi", aa.MODE_MIXED, aa.PREC_DIAG, dict(inner_tol=0.1), a.steps),
             ("fp64_jacobi", aa.MODE_GRAPH, aa.PREC_DIAG, {}, a.steps),
             ("fp64_learned", aa.MODE_GRAPH, aa.PREC_LEARNED, dict(gnn_theta=theta), 1)]
@@ -842,6 +844,7 @@ def main():
         keep = set(a.solvers.split(","))
         runs = [r for r in runs if r[0] in keep]
     results = {}
+    step_timing = None
     for name, mode, prec, kw, steps in runs:
         for w in range(a.warmup if steps > 1 else 0):
             log(f"warmup {name} {w}")
@@ -857,6 +860,17 @@ def main():
                 f"outer={r['outer']} status={r['status']} fallback={r['fallback']} "
                 f"true_rel_res={r['true_rel_res']:.3e} err_vs_xtrue={err:.3e}")
         r = rs[-1]
+        if name == "learned_mixed":
+            # one more solve, untimed, with the step kernel's launches bracketed by HIP events on the
+            # solve stream (iterations launched directly instead of from the captured graph): the
+            # kernel's duration inside the iteration, between the kernels it runs among
+            ctx.step_timing(True)
+            r_t, _ = run(mode, prec, **kw)
+            us, k = ctx.step_timing(False)
+            step_timing = {"us_total": us, "launches": k, "iterations": r_t["iterations"],
+                           "solve_s": r_t["solve_s"]}
+            log(f"{name} step kernel in the solve: {k} launches, {us / max(k, 1):.1f} us each "
+                f"(instrumented solve {r_t['solve_s']:.3f}s)")
         results[name] = {"time_s": float(np.mean(times)), "setup_s": r["setup_s"], "iterations": r["iterations"],
                          "converged": bool(r["status"] == 1 and r["true_rel_res"] < 1.05e-8),
                          "true_rel_res": r["true_rel_res"], "rel_err_vs_x_true": err, "steps": steps}
@@ -919,9 +933,21 @@ def main():
     if not a.no_cpu:
         log("cpu baseline (oracle)")
         cpu = cpu_baseline(ro, cols, vals, b, n)
-    roof_frac = spmv[ROOF]["gbs"] / HBM_PEAK_GBS
+    # the dominant kernel's duration inside the headline solve (HIP events around each launch);
+    # the back-to-back loop of time_spmv (finalize included, Infinity Cache warm from the previous
+    # launch) is reported beside it
+    if step_timing and step_timing["launches"] > 0:
+        roof_us = step_timing["us_total"] / step_timing["launches"]
+        roof_how = (f"HIP events around each of its {step_timing['launches']} launches inside one extra "
+                    f"learned + mixed solve (on the solve stream; that solve launches its iterations directly, not "
+                    f"from the captured graph)")
+    else:
+        roof_us = spmv[ROOF]["us"]
+        roof_how = "back-to-back launches with its one-workgroup finalize (no learned + mixed solve in this run)"
+    roof_gbs = spmv[ROOF]["bytes"] / roof_us / 1e3
+    roof_frac = roof_gbs / HBM_PEAK_GBS
     if roof_frac > 1.0:
-        raise RuntimeError(f"{ROOF_KERNEL}: {spmv[ROOF]['gbs']:.0f} GB/s exceeds the HBM peak: bytes or time wrong")
+        raise RuntimeError(f"{ROOF_KERNEL}: {roof_gbs:.0f} GB/s exceeds the HBM peak: bytes or time wrong")
     beside = {k: {"us_in_solve": profile_avg_us(k), "source": os.path.relpath(STATS_PROFILE, ROOT)}
               for k in ("k_factor_t<float, float>", "k_factor_p<float, float>", "k_cg_update<float>")}
 
@@ -946,14 +972,13 @@ def main():
         "training": training,
         "kept_matrix": kept,
         "spmv": spmv,
-        "roofline": {"bound": "hbm", "achieved": spmv[ROOF]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "achieved": roof_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": roof_frac, "traffic": profile_traffic(ROOF_KERNEL) if n == 256 else None,
                      "kernel": ROOF_KERNEL, "bytes_per_launch": spmv[ROOF]["bytes"],
-                     "us_per_launch": spmv[ROOF]["us"], "timed_with": "its one-workgroup finalize (6 us)",
+                     "us_per_launch": roof_us, "measured": roof_how,
+                     "us_back_to_back": spmv[ROOF]["us"], "achieved_back_to_back": spmv[ROOF]["gbs"],
                      "us_in_solve_profile": profile_avg_us(ROOF_KERNEL),
-                     "achieved_in_solve_profile": (spmv[ROOF]["bytes"] / profile_avg_us(ROOF_KERNEL) / 1e3
-                                                   if profile_avg_us(ROOF_KERNEL) else None),
-                     "copy_ceiling_gbs": ceiling, "frac_of_copy_ceiling": spmv[ROOF]["gbs"] / ceiling,
+                     "copy_ceiling_gbs": ceiling, "frac_of_copy_ceiling": roof_gbs / ceiling,
                      "csr_family": {"kernel": CSR_ROOF_KERNEL, "bytes_per_launch": spmv[CSR_ROOF]["bytes"],
                                     "achieved": spmv[CSR_ROOF]["gbs"], "frac": spmv[CSR_ROOF]["gbs"] / HBM_PEAK_GBS,
                                     "traffic": profile_traffic(CSR_ROOF_KERNEL) if n == 256 else None},

@@ -47,6 +47,13 @@ int alien_hip_ctx_create(void* stream, alien_hip_ctx** out);
 int alien_hip_ctx_destroy(alien_hip_ctx* ctx);
 int alien_hip_ctx_synchronize(alien_hip_ctx* ctx);
 void* alien_hip_ctx_stream(alien_hip_ctx* ctx);
+/* Instrumentation for benchmarks; no counterpart in the reference. Writes the totals
+ * so far (microseconds, launches) to the non-NULL outputs, then: on = 1 clears them and
+ * makes the fused CG solvers on this context (FUSED, GRAPH and MIXED modes) bracket each
+ * launch of their q = A p kernel with HIP events on the context's stream and add up the
+ * durations (the GRAPH and MIXED iterations are then launched directly, not replayed from
+ * a captured graph); on = 0 stops. Time solves with it off. */
+int alien_hip_ctx_step_timing(alien_hip_ctx* ctx, int on, double* total_us, long long* launches);
 
 /* ------------------------------------------------------------------------ */
 /* Host-side system generators and structure builders (no GPU involved)     */

@@ -397,6 +397,37 @@ def test_fallback_to_ilu0(ctx, mode, which):
     assert r["iterations"] <= 2 * 50 + 4 + 1.5 * ri["iterations"]
 
 
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
+def test_learned_bicgstab_on_two_unknown_system_hands_over_to_ilu0(ctx, mode):
+    """The nonsymmetric 2-unknown system (interleaved 2x2 blocks, the block generator's): the SPD
+    M = G G^T fits it worse than Jacobi, its BiCGStab residual stalls and the solve hands over to
+    ILU0 from the last good iterate (fallback bit 1), reaching the FP64 tolerance 1e-10 on the true
+    residual within the stall window plus about an ILU0 solve's iterations."""
+    ro, cols, vals = bsr2_scalar_csr(8)
+    b = aa.uniform01(42, len(ro) - 1)
+    say("learned BiCGStab 2-unknown 8^3", mode)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_LEARNED, mode=mode, tol=1e-10,
+                        max_iter=4000, gnn_theta=go.init_params(0), stall_window=40, poll_every=4)
+    _, ri = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED,
+                         tol=1e-10, max_iter=4000)
+    say("  ", r, "ilu0 from zero", ri["iterations"])
+    assert ri["status"] == 1
+    assert r["fallback"] & 1
+    assert r["status"] == 1 and np.all(np.isfinite(x))
+    assert r["true_rel_res"] < 1.05e-10
+    assert r["iterations"] <= 2 * 40 + 8 + 2 * ri["iterations"]
+
+
+def bsr2_scalar_csr(n, sigma=2.0, coupling=0.3, seed=42):
+    """The 2-unknown system of the block generator as scalar CSR (rows 2c, 2c+1)."""
+    ro, cols, vals = aa.gen_bsr2(n, params=[sigma, coupling], seed=seed)
+    nb = len(ro) - 1
+    blocks = vals.reshape(-1, 2, 2).transpose(0, 2, 1)  # column-major m[ieq + 2 iu] -> [row][col]
+    A = sp.bsr_matrix((blocks, cols, ro), shape=(2 * nb, 2 * nb)).tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
 def test_learned_rejects_bad_input(ctx):
     ro, cols, vals = system(4)
     b = np.ones(64)

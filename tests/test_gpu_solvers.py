@@ -83,7 +83,44 @@ def test_convdiff_bicgstab_ilu0(ctx, schedule):
     assert r["status"] == 1 and r["true_rel_res"] < 1e-8
     if schedule == 0:  # natural-order level sets: the reference preconditioner exactly
         _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-8)
-        assert abs(r["iterations"] - o["iterations"]) <= 2
+    else:  # multicolour: the reference's ILU0 of P A P^T, P the colour-class permutation
+        pro, pcols, pvals, perm = colour_permuted(ro, cols, vals)
+        _, o = orc.solve(pro, pcols, pvals, b[perm], solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-8)
+    say("  gpu", r["iterations"], "oracle", o["iterations"])
+    assert o["status"] == 1
+    assert abs(r["iterations"] - o["iterations"]) <= 2
+
+
+def colour_permuted(ro, cols, vals):
+    """P A P^T for the library's greedy colouring (colour classes in increasing order, stable
+    inside a class), as host CSR with sorted columns, and the permutation."""
+    import scipy.sparse as sp
+    n = len(ro) - 1
+    c, _ = aa.host_color(ro, cols)
+    perm = np.argsort(c, kind="stable")
+    A = sp.csr_matrix((vals, cols, ro), shape=(n, n))[perm][:, perm].tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64), perm
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH])
+def test_convdiff_64_multicolour_ilu0_bicgstab_matches_oracle(ctx, mode):
+    """The nonsymmetric configuration at a size its iteration count settles at (64^3 upwind
+    convection-diffusion): fused multicolour ILU0-BiCGStab against the oracle's BiCGStab with the
+    reference's ILU0 of the colour-permuted matrix, iterations within 2."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 64)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case conv-diff 64^3 multicolour ILU0 BiCGStab", mode)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_ILU0, mode=mode, tol=1e-8,
+                        ilu_schedule=1, max_iter=2000)
+    pro, pcols, pvals, perm = colour_permuted(ro, cols, vals)
+    _, o = orc.solve(pro, pcols, pvals, b[perm], solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-8,
+                     max_iter=2000)
+    say("  gpu", r["iterations"], r["true_rel_res"], "oracle", o["iterations"], o["true_rel_res"])
+    assert r["status"] == o["status"] == 1
+    assert r["true_rel_res"] < 1.05e-8
+    assert abs(r["iterations"] - o["iterations"]) <= 2
 
 
 @pytest.mark.parametrize("precond", [aa.PREC_DIAG, aa.PREC_CHEB, aa.PREC_NONE])
@@ -523,7 +560,7 @@ def test_learned_bicgstab_on_two_unknown_system(ctx, mode):
     """BiCGStab with the learned M = G G^T on the 2-unknown block system, to an
     FP64 relative residual of 1e-10. M is SPD by construction and fits this
     nonsymmetric operator worse than Jacobi; the stall rule hands the solve to
-    Jacobi and the true residual still meets the tolerance."""
+    ILU0 (fallback bit 1) and the true residual still meets the tolerance."""
     from oracle import gnn_oracle as go
     ro, cols, vals = bsr2_scalar_csr(10)
     n = len(ro) - 1
@@ -533,6 +570,7 @@ def test_learned_bicgstab_on_two_unknown_system(ctx, mode):
                         max_iter=6000, gnn_theta=go.init_params(0), stall_window=60, poll_every=4)
     say("  ", r)
     assert r["status"] == 1 and r["true_rel_res"] < 1.05e-10
+    assert r["fallback"] & 1
 
 
 @pytest.mark.parametrize("layout", [aa.LAYOUT_CSR, aa.LAYOUT_BELLPACK], ids=["csr", "bellpack"])
@@ -889,3 +927,34 @@ def test_solve2_future_variant_equals_solve(ctx, solver, precond, case):
     assert r1["status"] == 1
     assert (r1["iterations"], r1["status"], r1["error"]) == (r2["iterations"], r2["status"], r2["error"])
     np.testing.assert_array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("case", ["graph_diag", "fused_diag", "mixed_learned"])
+def test_step_timing_counts_the_step_kernel_and_changes_nothing(ctx, case):
+    """alien_hip_ctx_step_timing: every iteration's q = A p launch after the first is timed (the
+    GRAPH and MIXED iterations then run without their captured graph), the durations are positive
+    and the solve's result is the untimed solve's, bit for bit."""
+    import oracle.gnn_oracle as go
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7 if case == "mixed_learned" else aa.GEN_POISSON7, 16,
+                                params=[1.5, 1.0, 1.0, 1.0] if case == "mixed_learned" else None, seed=3)
+    b = aa.uniform01(7, len(ro) - 1)
+    mode = {"graph_diag": aa.MODE_GRAPH, "fused_diag": aa.MODE_FUSED, "mixed_learned": aa.MODE_MIXED}[case]
+    kw = dict(solver=aa.CG, mode=mode, tol=1e-8, max_iter=2000, poll_every=8)
+    if case == "mixed_learned":
+        kw.update(precond=aa.PREC_LEARNED, gnn_theta=go.init_params(0), inner_tol=0.1)
+    else:
+        kw.update(precond=aa.PREC_DIAG)
+    say("step timing", case)
+    x0, r0 = aa.solve_csr(ctx, ro, cols, vals, b, **kw)
+    assert ctx.step_timing(True) == (0.0, 0)
+    x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, **kw)
+    us, k = ctx.step_timing(False)
+    say("  ", r1["iterations"], "iterations", r1["outer"], "updates;", k, "timed launches,", us / max(k, 1), "us each")
+    assert np.array_equal(x0, x1) and r0["iterations"] == r1["iterations"]
+    its = r1["iterations"]
+    assert its - 3 - r1["outer"] <= k <= its
+    assert 0 < us / k < 1e4
+    # off: nothing more is gathered, and switching on clears the totals
+    aa.solve_csr(ctx, ro, cols, vals, b, **kw)
+    assert ctx.step_timing(True) == (us, k)
+    assert ctx.step_timing(False) == (0.0, 0)
