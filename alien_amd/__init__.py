@@ -11,7 +11,7 @@ from ._capi import (  # noqa: F401
     PREC_FILU0, PREC_LEARNED, PREC_NEUMANN, PREC_NONE, AlienHipError, Context, bellpack_bro, bellpack_build, check, gen_bsr2,
     gen_csr, gen_nnz,
     Matrix, gnn_forward, gnn_init, gnn_loss_grad, gnn_nparams, gnn_train, group_by_level, host_color, host_dcol,
-    host_levels, lib, mtx_read_csr, mtx_read_vector, mtx_write_csr, mtx_write_vector,
+    host_diagonals, host_levels, lib, mtx_read_csr, mtx_read_vector, mtx_write_csr, mtx_write_vector,
     rbcsr_bundle_offsets, rbdia_build, solve_csr, uniform01, vblock_to_csr)
 
 __version__ = "0.1.0"

@@ -141,6 +141,15 @@ _SIGS = [
     ("alien_hip_axpy_dot_f64", C.c_int, [c_void_p, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p]),
     ("alien_hip_spmv_dot_rbdia_f64", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 7),
     ("alien_hip_spmv_dot_rbdia_f32", C.c_int, [c_void_p, C.c_int] + [c_void_p] * 7),
+    ("alien_hip_host_diagonals", C.c_int, [C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p]),
+    ("alien_hip_rbdia_uniform_struct", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, C.c_int, c_int_p, c_void_p,
+                                                 c_void_p, c_void_p]),
+    ("alien_hip_spmv_udia_f64", C.c_int, [c_void_p, C.c_int, C.c_int, c_int_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_udia_f32", C.c_int, [c_void_p, C.c_int, C.c_int, c_int_p, c_void_p, c_void_p, c_void_p]),
+    ("alien_hip_spmv_dot_udia_f64", C.c_int, [c_void_p, C.c_int, C.c_int, c_int_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p]),
+    ("alien_hip_spmv_dot_udia_f32", C.c_int, [c_void_p, C.c_int, C.c_int, c_int_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p]),
     ("alien_hip_spmv_dot_csr_f64", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_void_p]),
     ("alien_hip_spmv_dot_csr_f64_noalloc", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -347,6 +356,16 @@ def rbdia_build(ro, cols, max_width=64):
     check(lib().alien_hip_rbdia_struct(n, iptr(ro), iptr(cols), max_width, iptr(doff),
                                        dmask.ctypes.data_as(C.POINTER(C.c_uint64))))
     return bo, doff[:slots], dmask[:slots]
+
+
+def host_diagonals(ro, cols, max_k=8):
+    """The distinct diagonal offsets col - row of a CSR, ascending, or None if more than max_k."""
+    ro = np.ascontiguousarray(ro, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    off = np.zeros(max_k, dtype=np.int32)
+    k = C.c_int()
+    check(lib().alien_hip_host_diagonals(len(ro) - 1, iptr(ro), iptr(cols), max_k, iptr(off), C.byref(k)))
+    return None if k.value < 0 else off[:k.value].copy()
 
 
 def group_by_level(level: np.ndarray, nlevels: int):

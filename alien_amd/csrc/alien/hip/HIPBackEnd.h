@@ -427,9 +427,47 @@ class HIPMatrix
     alien_hip::check_launch("rbdia_struct");
     m_rbdia = true;
     m_pair = pair;
+    m_uni = 0;
+    if (!pair) slots = tryUniform(slots);
     d_bv.resize((size_t)slots * alien_hip::kBundle * (pair ? 2 : 1));
     fillDIA<double>(d_bv.data());
     settle();
+  }
+  // Uniform RB-DIA (RBMat::uni) when the matrix's distinct diagonal offsets number at most
+  // kUniMaxSlots and padding every bundle to all of them adds at most 1/32 of the slots (a
+  // stencil matrix: only its boundary bundles lack some offsets). Returns the slot count.
+  long tryUniform(long slots) {
+    hipStream_t st = m_ctx->stream;
+    const int nb = (m_n + alien_hip::kBundle - 1) / alien_hip::kBundle;
+    if (nb == 0 || slots <= 0) return slots;
+    std::vector<int> doff(slots);
+    ALIEN_HIP_CHECK(hipMemcpyAsync(doff.data(), d_doff.data(), slots * sizeof(int), hipMemcpyDeviceToHost, st));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<int> u;
+    for (int d : doff) {
+      if (std::find(u.begin(), u.end(), d) != u.end()) continue;
+      u.push_back(d);
+      if ((int)u.size() > alien_hip::kUniMaxSlots) return slots;
+    }
+    std::sort(u.begin(), u.end());
+    const long k = (long)u.size();
+    if (k * nb > slots + slots / 32 || k * nb * alien_hip::kBundle > 0x7fffffffL) return slots;
+    d_doff.resize(k * nb);
+    d_dmask.resize(k * nb);
+    alien_hip::launch_rbdia_uniform_struct(st, m_n, d_ro.data(), d_cols.data(), (int)k, u.data(), d_bo.data(),
+                                           d_doff.data(), d_dmask.data());
+    alien_hip::check_launch("rbdia_uniform_struct");
+    for (int b = 0; b <= nb; ++b) m_bo[b] = (int)(b * k);
+    m_uni = (int)k;
+    for (int j = 0; j < (int)k; ++j) m_uoff[j] = u[j];
+    return k * nb;
+  }
+  template <typename T>
+  void setUniform(alien_hip::RBMat<T>& a) const {
+    if (!m_uni) return;
+    a.uni = m_uni;
+    a.nrows = m_n;
+    for (int j = 0; j < m_uni; ++j) a.uoff[j] = m_uoff[j];
   }
   template <typename T>
   void fillDIA(T* bv) {
@@ -460,6 +498,10 @@ class HIPMatrix
   }
   bool isRBDIA() const { return m_rbdia && !m_fused_ell; }
   bool isPairDIA() const { return m_rbdia && m_pair && !m_fused_ell; }
+  // FP64 operand: the uniform structure read as an ordinary RB-DIA (its masks skip the padded
+  // slots). Measured at 256^3: the FP64 row kernels on RB-DIA with one bundle per wave (q = A p
+  // 197 us, a Chebyshev step 313 us) beat their uniform instances (216 us, 337 us), which pay off
+  // for FP32 only (q = A p 100 us against 125 us).
   alien_hip::RBMat<double> rb64() const {
     if (m_fused_ell) return ellMat<double>(d_bellv.data());
     return alien_hip::RBMat<double>{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv.data(),
@@ -468,10 +510,12 @@ class HIPMatrix
   }
   alien_hip::RBMat<float> rb32() const {
     if (m_fused_ell) return ellMat<float>(d_bellv32.data());
-    return alien_hip::RBMat<float>{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv32.data(),
-                                   m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr,
-                                   m_pair ? 1 : 0};
+    alien_hip::RBMat<float> a{d_bo.data(), m_rbdia ? nullptr : d_bc.data(), d_bv32.data(),
+                              m_rbdia ? d_doff.data() : nullptr, m_rbdia ? d_dmask.data() : nullptr, m_pair ? 1 : 0};
+    setUniform(a);
+    return a;
   }
+  int uniformSlots() const { return m_uni; }
   template <typename T>
   alien_hip::RBMat<T> ellMat(const T* bv) const {
     alien_hip::RBMat<T> a{d_bro.data(), d_bellc.data(), bv};
@@ -487,6 +531,7 @@ class HIPMatrix
     d_dmask.reset();
     m_pair = false;
     m_rbdia = false;
+    m_uni = 0;
   }
 
   void ensureRBCSR() {
@@ -573,6 +618,8 @@ class HIPMatrix
  private:
   alien_hip_ctx* m_ctx = nullptr;
   bool m_rbdia = false;
+  int m_uni = 0;     // uniform RB-DIA: slots per bundle (0: not uniform)
+  int m_uoff[8] = {}; // and their column offsets
   bool m_pair = false; // RB-DIA in pair mode
   bool m_fused_ell = false; // the fused solvers read the BEllPack arrays
   int m_n = 0, m_nnz = 0;

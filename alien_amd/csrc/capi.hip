@@ -93,6 +93,23 @@ using namespace alien_hip;
 
 #define LAUNCHED(what) check_launch(what)
 
+// uniform RB-DIA operands of the C ABI: k ascending offsets, given by the caller
+static bool uni_offsets_ok(int k, const int* off) {
+  if (k < 1 || k > kUniMaxSlots || !off) return false;
+  for (int j = 1; j < k; ++j)
+    if (off[j] <= off[j - 1]) return false;
+  return true;
+}
+template <typename T>
+static RBMat<T> uni_mat(int n, int k, const int* off, const T* bv) {
+  RBMat<T> A{nullptr, nullptr, bv};
+  A.uni = k;
+  A.nrows = n;
+  for (int j = 0; j < k; ++j) A.uoff[j] = off[j];
+  return A;
+}
+
+
 extern "C" {
 
 const char* alien_hip_last_error(void) { return t_last_error.c_str(); }
@@ -102,9 +119,10 @@ int alien_hip_ctx_create(void* stream, alien_hip_ctx** out) {
   REQUIRE(out, "ctx_create: out is NULL");
   API_BEGIN
   // row-kernel grid caps (device_common.h)
-  const char* names[2] = {"ALIEN_HIP_ROW_BLOCKS_F64", "ALIEN_HIP_ROW_BLOCKS_F32"};
-  long* caps[2] = {&g_row_blocks_f64, &g_row_blocks_f32};
-  for (int i = 0; i < 2; ++i)
+  const char* names[4] = {"ALIEN_HIP_ROW_BLOCKS_F64", "ALIEN_HIP_ROW_BLOCKS_F32", "ALIEN_HIP_ROW_BLOCKS_UNI",
+                          "ALIEN_HIP_ROW_BLOCKS_UNI_WIDE"};
+  long* caps[4] = {&g_row_blocks_f64, &g_row_blocks_f32, &g_row_blocks_uni, &g_row_blocks_uni_wide};
+  for (int i = 0; i < 4; ++i)
     if (const char* e = std::getenv(names[i])) {
       const long v = std::strtol(e, nullptr, 10);
       if (v >= kXcds) *caps[i] = v / kXcds * kXcds;
@@ -647,6 +665,95 @@ int alien_hip_spmv_dot_rbdia_f32(alien_hip_ctx* ctx, int n, const int* bo, const
   RBMat<float> A{bo, nullptr, bv, doff, reinterpret_cast<const unsigned long long*>(dmask)};
   launch_spmv_dot_rb<float>(ctx->stream, ctx->red(), n, A, x, y, ctx->idle_cg.data(), result);
   LAUNCHED("spmv_dot_rbdia_f32");
+  API_END
+}
+
+// ---- uniform RB-DIA -------------------------------------------------------------------
+int alien_hip_host_diagonals(int n, const int* ro, const int* cols, int max_k, int* offsets, int* k_out) {
+  REQUIRE(n >= 0 && max_k >= 1 && offsets && k_out && (n == 0 || (ro && cols)), "host_diagonals: bad arguments");
+  API_BEGIN
+  std::vector<int> u;
+  int k = 0;
+  size_t last = 0; // consecutive entries mostly repeat a recent offset
+  for (int r = 0; r < n && k >= 0; ++r)
+    for (int e = ro[r]; e < ro[r + 1]; ++e) {
+      const long d = (long)cols[e] - r;
+      if (last < u.size() && u[last] == (int)d) continue;
+      const auto it = std::find(u.begin(), u.end(), (int)d);
+      if (it != u.end()) {
+        last = (size_t)(it - u.begin());
+        continue;
+      }
+      if ((int)u.size() == max_k) {
+        k = -1;
+        break;
+      }
+      u.push_back((int)d);
+    }
+  std::sort(u.begin(), u.end());
+  if (k >= 0) {
+    k = (int)u.size();
+    for (int j = 0; j < k; ++j) offsets[j] = u[j];
+  }
+  *k_out = k;
+  API_END
+}
+
+int alien_hip_rbdia_uniform_struct(alien_hip_ctx* ctx, int n, const int* ro, const int* cols, int k,
+                                   const int* offsets, int* bo, int* doff, uint64_t* dmask) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && uni_offsets_ok(k, offsets) && (n == 0 || (ro && cols && bo && doff && dmask)),
+          "rbdia_uniform_struct: bad arguments");
+  API_BEGIN
+  launch_rbdia_uniform_struct(ctx->stream, n, ro, cols, k, offsets, bo, doff,
+                              reinterpret_cast<unsigned long long*>(dmask));
+  LAUNCHED("rbdia_uniform_struct");
+  API_END
+}
+
+int alien_hip_spmv_udia_f64(alien_hip_ctx* ctx, int n, int k, const int* offsets, const double* bv, const double* x,
+                            double* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && uni_offsets_ok(k, offsets) && (n == 0 || (bv && x && y)), "spmv_udia_f64: bad arguments");
+  API_BEGIN
+  launch_spmv_rb<double>(ctx->stream, n, uni_mat<double>(n, k, offsets, bv), x, y);
+  LAUNCHED("spmv_udia_f64");
+  API_END
+}
+
+int alien_hip_spmv_udia_f32(alien_hip_ctx* ctx, int n, int k, const int* offsets, const float* bv, const float* x,
+                            float* y) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && uni_offsets_ok(k, offsets) && (n == 0 || (bv && x && y)), "spmv_udia_f32: bad arguments");
+  API_BEGIN
+  launch_spmv_rb<float>(ctx->stream, n, uni_mat<float>(n, k, offsets, bv), x, y);
+  LAUNCHED("spmv_udia_f32");
+  API_END
+}
+
+int alien_hip_spmv_dot_udia_f64(alien_hip_ctx* ctx, int n, int k, const int* offsets, const double* bv,
+                                const double* x, double* y, double* result) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && result && uni_offsets_ok(k, offsets) && (n == 0 || (bv && x && y)),
+          "spmv_dot_udia_f64: bad arguments");
+  API_BEGIN
+  ctx->reserve_for(n);
+  launch_spmv_dot_rb<double>(ctx->stream, ctx->red(), n, uni_mat<double>(n, k, offsets, bv), x, y,
+                             ctx->idle_cg.data(), result);
+  LAUNCHED("spmv_dot_udia_f64");
+  API_END
+}
+
+int alien_hip_spmv_dot_udia_f32(alien_hip_ctx* ctx, int n, int k, const int* offsets, const float* bv,
+                                const float* x, float* y, double* result) {
+  CTX_OK(ctx);
+  REQUIRE(n >= 0 && result && uni_offsets_ok(k, offsets) && (n == 0 || (bv && x && y)),
+          "spmv_dot_udia_f32: bad arguments");
+  API_BEGIN
+  ctx->reserve_for(n);
+  launch_spmv_dot_rb<float>(ctx->stream, ctx->red(), n, uni_mat<float>(n, k, offsets, bv), x, y,
+                            ctx->idle_cg.data(), result);
+  LAUNCHED("spmv_dot_udia_f32");
   API_END
 }
 

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p(int n, const T* __restrict__ 
 }
 
 // v = A phat ; v.r0
-template <typename T, bool ELL = false>
+template <typename T, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_bicg_v(int n, RBMat<T> A, const T* __restrict__ phat,
                                                        const T* __restrict__ r0, T* __restrict__ v,
                                                        const BiCGState* st, RedWork w) {
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(kRowBlock) k_bicg_v(int n, RBMat<T> A, const T
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T r0v = r0[min(row, n - 1)];
-    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return phat[c]; });
+    const T t = rb_sum<T, LAY>(A, b, [&](int c) { return phat[c]; });
     if (row < n) {
       v[row] = t;
       s += (double)t * (double)r0v;
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s(int n, const T* __restrict__ 
 }
 
 // t = A shat ; t.s, t.t
-template <typename T, bool ELL = false>
+template <typename T, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_bicg_t(int n, RBMat<T> A, const T* __restrict__ shat,
                                                        const T* __restrict__ sv, T* __restrict__ t,
                                                        const BiCGState* st, RedWork w) {
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(kRowBlock) k_bicg_t(int n, RBMat<T> A, const T
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T srow = sv[min(row, n - 1)];
-    const T tv = rb_sum<T, ELL>(A, b, [&](int c) { return shat[c]; });
+    const T tv = rb_sum<T, LAY>(A, b, [&](int c) { return shat[c]; });
     if (row < n) {
       t[row] = tv;
       ts += (double)tv * (double)srow;
@@ -164,20 +164,19 @@ __global__ void __launch_bounds__(kBlock) k_bicg_half(int n, const T* __restrict
 }
 
 // y = A x on the row-bundled layout, skipped once *done is set
-template <typename T>
+template <typename T, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_flag(int n, RBMat<T> A, const T* __restrict__ x,
                                                              T* __restrict__ y, const int* done) {
   if (*done) return;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum_single<T>(A, b, [&](int c) { return x[c]; });
+    const T t = rb_factor_sum<T, LAY>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) y[row] = t;
   }
 }
 
 static inline dim3 egrid(long n) { return dim3((unsigned)ept_blocks(n)); }
-template <typename T> static inline long rblocks(long n) { return bundle_blocks(n, (int)sizeof(T)); }
 
 template <typename T>
 void launch_bicg_p(hipStream_t s, int n, const T* r, const T* v, T* p, const T* invd, T* phat, BiCGState* st) {
@@ -191,10 +190,10 @@ void launch_bicg_p(hipStream_t s, int n, const T* r, const T* v, T* p, const T* 
 template <typename T>
 void launch_bicg_v(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* phat, const T* r0, T* v, BiCGState* st) {
   const dim3 g((unsigned)rb_blocks(n, A));
-  if (A.ell)
-    hipLaunchKernelGGL((k_bicg_v<T, true>), g, dim3(kRowBlock), 0, s, n, A, phat, r0, v, (const BiCGState*)st, w);
-  else
-    hipLaunchKernelGGL((k_bicg_v<T, false>), g, dim3(kRowBlock), 0, s, n, A, phat, r0, v, (const BiCGState*)st, w);
+  by_layout(A, [&](auto L) {
+    hipLaunchKernelGGL((k_bicg_v<T, decltype(L)::value>), g, dim3(kRowBlock), 0, s, n, A, phat, r0, v,
+                       (const BiCGState*)st, w);
+  });
   launch_finalize_bicg(s, w, rb_blocks(n, A), 1, kBFinAlpha, st);
 }
 template <typename T>
@@ -211,10 +210,10 @@ void launch_bicg_s(hipStream_t s, RedWork w, int n, const T* r, const T* v, T* s
 template <typename T>
 void launch_bicg_t(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* shat, const T* sv, T* t, BiCGState* st) {
   const dim3 g((unsigned)rb_blocks(n, A));
-  if (A.ell)
-    hipLaunchKernelGGL((k_bicg_t<T, true>), g, dim3(kRowBlock), 0, s, n, A, shat, sv, t, (const BiCGState*)st, w);
-  else
-    hipLaunchKernelGGL((k_bicg_t<T, false>), g, dim3(kRowBlock), 0, s, n, A, shat, sv, t, (const BiCGState*)st, w);
+  by_layout(A, [&](auto L) {
+    hipLaunchKernelGGL((k_bicg_t<T, decltype(L)::value>), g, dim3(kRowBlock), 0, s, n, A, shat, sv, t,
+                       (const BiCGState*)st, w);
+  });
   launch_finalize_bicg(s, w, rb_blocks(n, A), 2, kBFinOmega, st);
 }
 template <typename T>
@@ -231,9 +230,14 @@ void launch_bicg_half(hipStream_t s, int n, const T* phat, T* x, const BiCGState
 template <typename T>
 void launch_factor_apply_flag(hipStream_t s, int n, RBMat<T> G, RBMat<T> GT, const T* x, T* tmp, T* y,
                               const int* done) {
-  const dim3 g((unsigned)rblocks<T>(n));
-  hipLaunchKernelGGL(k_rb_spmv_flag<T>, g, dim3(kRowBlock), 0, s, n, GT, x, tmp, done);
-  hipLaunchKernelGGL(k_rb_spmv_flag<T>, g, dim3(kRowBlock), 0, s, n, G, (const T*)tmp, y, done);
+  by_layout(GT, [&](auto L) {
+    hipLaunchKernelGGL((k_rb_spmv_flag<T, decltype(L)::value>), dim3((unsigned)rb_blocks(n, GT)), dim3(kRowBlock), 0,
+                       s, n, GT, x, tmp, done);
+  });
+  by_layout(G, [&](auto L) {
+    hipLaunchKernelGGL((k_rb_spmv_flag<T, decltype(L)::value>), dim3((unsigned)rb_blocks(n, G)), dim3(kRowBlock), 0,
+                       s, n, G, (const T*)tmp, y, done);
+  });
 }
 
 #define ALIEN_HIP_INST_BICG(T)                                                                              \

@@ -28,13 +28,15 @@ inline long ept_blocks(long n) {
 // and the waves of an XCD stride through its share together: the waves running
 // at any moment read neighbouring bundles and their x-gathers hit that XCD's
 // L2, and per-wave start-up and the per-workgroup reduction are paid once per
-// walk. One-bundle-per-wave grids are mapped in chunks per XCD (bundle_walk). Measured at 256^3 (7-point): the FP32 row kernels run fastest capped at
-// 4096 workgroups, the FP64 ones uncapped. The environment variables
-// ALIEN_HIP_ROW_BLOCKS_F64 / _F32, read when a context is created, override the caps.
+// walk. One-bundle-per-wave grids are mapped in chunks per XCD (bundle_walk). Measured at 256^3
+// (7-point): on RB-CSR / RB-DIA the FP32 row kernels run fastest capped at 4096 workgroups, the
+// FP64 ones uncapped; on the uniform RB-DIA (no per-bundle descriptor loads) see uni_blocks. The
+// environment variables ALIEN_HIP_ROW_BLOCKS_F64 / _F32 / _UNI / _UNI_WIDE, read when a context is
+// created, override the caps.
 constexpr int kXcds = 8;
 constexpr int kRowWaves = kRowBlock / kWave;
 constexpr long kUncapped = 1L << 30;
-extern long g_row_blocks_f64, g_row_blocks_f32;
+extern long g_row_blocks_f64, g_row_blocks_f32, g_row_blocks_uni, g_row_blocks_uni_wide;
 inline long round_xcds(long g) {
   g = (g + kXcds - 1) / kXcds * kXcds;
   return g < kXcds ? kXcds : g;
@@ -57,6 +59,17 @@ inline long bundle_blocks(long n, int elem) {
   if (g > cap) return round_xcds(cap);
   return round_chunks(g);
 }
+// workgroups of a row kernel over a uniform RB-DIA (RBMat::uni) of k slots per bundle, both
+// precisions. Measured at 256^3 inside the solves (caps 8192 .. 65536 and uncapped): the 7-slot
+// operator's q = A p step runs best at 32768 workgroups (FP32 100 us, FP64 216 us), the 4-slot
+// factors' steps at 16384 (67 and 69 us; 72 and 81 us at 32768).
+inline long uni_blocks(long n, int k) {
+  const long nb = (n + kWave - 1) / kWave;
+  const long g = (nb + kRowWaves - 1) / kRowWaves;
+  const long cap = k >= 6 ? g_row_blocks_uni_wide : g_row_blocks_uni;
+  if (g > cap) return round_xcds(cap);
+  return round_chunks(g);
+}
 // the largest grid any row kernel uses (reduction workspace)
 inline long bundle_blocks_max(long n) {
   const long nb = (n + kWave - 1) / kWave;
@@ -67,8 +80,8 @@ inline long bundle_blocks_max(long n) {
 struct BundleWalk {
   int first, end, stride;
 };
-__device__ __forceinline__ BundleWalk bundle_walk(int n) {
-  const int nb = (n + kWave - 1) / kWave;
+// Walk over nb units (bundles, or pairs of bundles) of a row kernel's grid.
+__device__ __forceinline__ BundleWalk bundle_walk_units(int nb) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
   BundleWalk w;
   if ((long)gridDim.x * kRowWaves >= nb) {
@@ -94,6 +107,8 @@ __device__ __forceinline__ BundleWalk bundle_walk(int n) {
   w.stride = per_xcd * kRowWaves;
   return w;
 }
+// Bundles of a row kernel over n rows: b = first, first + stride, ... < end.
+__device__ __forceinline__ BundleWalk bundle_walk(int n) { return bundle_walk_units((n + kWave - 1) / kWave); }
 #endif
 // Shape of the simple element-wise kernels (grid-stride).
 inline int grid_for(long n, int per_thread = 1) {

@@ -109,9 +109,18 @@ class Engine {
     size_t slots = 0;
     DeviceArray<int> bo, bc, doff;
     DeviceArray<unsigned long long> dmask;
+    // uniform RB-DIA (every bundle holds the same uni slots at offsets uoff; RBMat::uni)
+    int uni = 0, nrows = 0;
+    int uoff[kUniMaxSlots] = {};
     template <typename T, typename V = T> RBMat<T, V> mat(const V* v) const {
-      return RBMat<T, V>{bo.data(), dia ? nullptr : bc.data(), v, dia ? doff.data() : nullptr,
-                         dia ? dmask.data() : nullptr};
+      RBMat<T, V> a{bo.data(), dia ? nullptr : bc.data(), v, dia ? doff.data() : nullptr,
+                    dia ? dmask.data() : nullptr};
+      if (uni) {
+        a.uni = uni;
+        a.nrows = nrows;
+        for (int j = 0; j < uni; ++j) a.uoff[j] = uoff[j];
+      }
+      return a;
     }
   };
   Factor m_g, m_gt;

@@ -1029,6 +1029,21 @@ void Engine::build_factor_layout() {
     const size_t slots = (size_t)bo[nb] * kBundle;
     Factor& F = t == 0 ? m_g : m_gt;
     F.dia = dia;
+    F.uni = 0;
+    if (dia && nb > 0) {
+      // uniform when every bundle holds the same offsets (the split of a uniform A)
+      const int k = bo[1] - bo[0];
+      bool same = k >= 1 && k <= kUniMaxSlots;
+      for (int b = 0; same && b < nb; ++b) {
+        same = bo[b] == b * k && bo[b + 1] == (b + 1) * k;
+        for (int j = 0; same && j < k; ++j) same = doff[(size_t)b * k + j] == doff[j];
+      }
+      if (same) {
+        F.uni = k;
+        F.nrows = n;
+        for (int j = 0; j < k; ++j) F.uoff[j] = doff[j];
+      }
+    }
     F.bo.upload(bo.data(), bo.size(), s);
     if (dia) {
       F.doff.upload(doff.data(), doff.size(), s);

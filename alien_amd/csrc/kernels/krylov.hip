@@ -42,7 +42,7 @@ namespace alien_hip {
 // Row kernels walk their bundles with bundle_walk (device_common.h).
 
 // r = b - A x  (copy(b,r); mult(A,x,p); axpy(-1,p,r)); out = r.r
-template <typename T, bool ELL = false>
+template <typename T, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const T* __restrict__ x,
                                                       const T* __restrict__ rhs, T* __restrict__ r,
                                                       RedWork w) {
@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const
   for (int b = bw.first; b < bw.end; b += bw.stride) {
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     const T br = rhs[min(row, n - 1)];
-    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return x[c]; });
+    const T t = rb_sum<T, LAY>(A, b, [&](int c) { return x[c]; });
     if (row < n) {
       const T v = br + T(-1) * t;
       r[row] = v;
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(kRowBlock) k_residual(int n, RBMat<T> A, const
 // One Chebyshev step (ChebyshevPreconditioner::_algo2). FIRST: y_in = x,
 // p = z, y' = x + a z. Otherwise p = b p + z, y' = y + a p. LAST also
 // accumulates rho1 = x . y' into the CG state.
-template <typename T, bool FIRST, bool LAST, bool ELL = false>
+template <typename T, bool FIRST, bool LAST, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, const T* __restrict__ xin,
                                                        const T* __restrict__ yin, T* __restrict__ yout,
                                                        T* __restrict__ p, const T* __restrict__ invd, T a, T bcoef,
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, cons
   const T* src = FIRST ? xin : yin;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return src[c]; });
+    const T t = rb_sum<T, LAY>(A, b, [&](int c) { return src[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       const T xi = xin[row];
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(kRowBlock) k_cheb_step(int n, RBMat<T> A, cons
 // the algebra calls of its solve): yout = (yin + (-f) (A yin)) + x, the last
 // step scaled by f; with rdot the last step also stores the partials of
 // rdot . yout. Returns at entry once *done.
-template <typename T, bool LAST, bool ELL = false>
+template <typename T, bool LAST, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_neumann_step(int n, RBMat<T> A, const T* __restrict__ x,
                                                           const T* __restrict__ yin, T* __restrict__ yout, T f,
                                                           const int* __restrict__ done, const T* __restrict__ rdot,
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(kRowBlock) k_neumann_step(int n, RBMat<T> A, c
   double s = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return yin[c]; });
+    const T t = rb_sum<T, LAY>(A, b, [&](int c) { return yin[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       T v = yin[row] + (-f) * t;
@@ -182,20 +182,46 @@ __global__ void __launch_bounds__(kBlock) k_cg_p(int n, const T* __restrict__ z,
 }
 
 // q = A p ; pq = p . q (a zero pq raises the breakdown flag in the finalize)
-template <typename T, bool ELL = false>
+template <typename T, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* __restrict__ p, T* __restrict__ q,
                                                      CGState* st, RedWork w) {
   if (st->done) return;
   __shared__ double lds4[kRowBlock / kWave];
   double s = 0.0;
-  const BundleWalk bw = bundle_walk(n);
-  for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
-    const T prow = p[min(row, n - 1)]; // unconditional (clamped): issued with the row's loads
-    const T t = rb_sum<T, ELL>(A, b, [&](int c) { return p[c]; });
-    if (row < n) {
-      q[row] = t;
-      s += (double)prow * (double)t;
+  bool paired = false;
+  if constexpr (LAY == kRowUniform) {
+    if (A.uni <= kUniPairMaxSlots) {
+      // bundles in pairs (uni_sum2)
+      paired = true;
+      const int nb = (n + kWave - 1) / kWave;
+      const BundleWalk bw = bundle_walk_units((nb + 1) / 2);
+      for (int u = bw.first; u < bw.end; u += bw.stride) {
+        const int b0 = 2 * u, b1 = min(2 * u + 1, nb - 1);
+        const int row0 = b0 * kBundle + (threadIdx.x & (kWave - 1)), row1 = row0 + kBundle;
+        const T p0 = p[min(row0, n - 1)], p1 = p[min(row1, n - 1)];
+        T t0, t1;
+        uni_sum2<T>(A, b0, b1, [&](int c) { return p[c]; }, t0, t1);
+        if (row0 < n) {
+          q[row0] = t0;
+          s += (double)p0 * (double)t0;
+        }
+        if (row1 < n) {
+          q[row1] = t1;
+          s += (double)p1 * (double)t1;
+        }
+      }
+    }
+  }
+  if (!paired) {
+    const BundleWalk bw = bundle_walk(n);
+    for (int b = bw.first; b < bw.end; b += bw.stride) {
+      const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+      const T prow = p[min(row, n - 1)]; // unconditional (clamped): issued with the row's loads
+      const T t = rb_sum<T, LAY>(A, b, [&](int c) { return p[c]; });
+      if (row < n) {
+        q[row] = t;
+        s += (double)prow * (double)t;
+      }
     }
   }
   double v[1] = {block_sum<kRowBlock>(s, lds4)};
@@ -204,7 +230,7 @@ __global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* 
 
 // y = A x with rho1 = w . y into the CG state (second SpMV of a factored
 // preconditioner z = G (G^T r): x = G^T r, w = r)
-template <typename T>
+template <typename T, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_dot(int n, RBMat<T> A, const T* __restrict__ x,
                                                             const T* __restrict__ wv, T* __restrict__ y, CGState* st,
                                                             RedWork w) {
@@ -213,7 +239,7 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_dot(int n, RBMat<T> A, co
   double s = 0.0;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum_single<T>(A, b, [&](int c) { return x[c]; });
+    const T t = rb_factor_sum<T, LAY>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) {
       y[row] = t;
@@ -225,13 +251,13 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_dot(int n, RBMat<T> A, co
 }
 
 // y = A x (RB-CSR), returning early once the CG state is done
-template <typename T>
+template <typename T, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_guarded(int n, RBMat<T> A, const T* __restrict__ x,
                                                                 T* __restrict__ y, const CGState* st) {
   if (st->done) return;
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T t = rb_sum_single<T>(A, b, [&](int c) { return x[c]; });
+    const T t = rb_factor_sum<T, LAY>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + (threadIdx.x & (kWave - 1));
     if (row < n) y[row] = t;
   }
@@ -241,21 +267,44 @@ __global__ void __launch_bounds__(kRowBlock) k_rb_spmv_guarded(int n, RBMat<T> A
 // (= r.M r, the r.z of CG.h:124 with z = G t, without forming z), plus r.r:
 // the stop test of the previous update is taken here (its finalize runs
 // kFinXRDiag), so the x/r update itself needs no reduction.
-template <typename T, typename V>
+template <typename T, typename V, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_factor_t(int n, RBMat<T, V> GT, const T* __restrict__ r,
                                                          T* __restrict__ t, CGState* st, RedWork w) {
   if (st->done) return;
   __shared__ double lds4[kRowBlock / kWave];
   double srr = 0.0, stt = 0.0;
-  const BundleWalk bw = bundle_walk(n);
-  for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
-    const T rv = r[min(row, n - 1)];
-    const T v = rb_sum_single<T>(GT, b, [&](int c) { return r[c]; });
-    if (row < n) {
-      t[row] = v;
-      srr += (double)rv * (double)rv;
-      stt += (double)v * (double)v;
+  if constexpr (LAY == kRowUniform) {
+    // bundles in pairs (uni_sum2)
+    const int nb = (n + kWave - 1) / kWave;
+    const BundleWalk bw = bundle_walk_units((nb + 1) / 2);
+    for (int u = bw.first; u < bw.end; u += bw.stride) {
+      const int b0 = 2 * u, b1 = min(2 * u + 1, nb - 1);
+      const int row0 = b0 * kBundle + (threadIdx.x & (kWave - 1)), row1 = row0 + kBundle;
+      const T r0 = r[min(row0, n - 1)], r1 = r[min(row1, n - 1)];
+      T v0, v1;
+      uni_sum2<T>(GT, b0, b1, [&](int c) { return r[c]; }, v0, v1);
+      if (row0 < n) {
+        t[row0] = v0;
+        srr += (double)r0 * (double)r0;
+        stt += (double)v0 * (double)v0;
+      }
+      if (row1 < n) {
+        t[row1] = v1;
+        srr += (double)r1 * (double)r1;
+        stt += (double)v1 * (double)v1;
+      }
+    }
+  } else {
+    const BundleWalk bw = bundle_walk(n);
+    for (int b = bw.first; b < bw.end; b += bw.stride) {
+      const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+      const T rv = r[min(row, n - 1)];
+      const T v = rb_factor_sum<T, LAY>(GT, b, [&](int c) { return r[c]; });
+      if (row < n) {
+        t[row] = v;
+        srr += (double)rv * (double)rv;
+        stt += (double)v * (double)v;
+      }
     }
   }
   double v[2];
@@ -294,17 +343,32 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int n, const T* __restrict
 
 // Second half fused with the direction update: z_i = (G t)_i is formed in
 // registers and p = z + (rho1/rho) p (p = z on the first step), as k_cg_p.
-template <typename T, typename V>
+template <typename T, typename V, int LAY = kRowBundled>
 __global__ void __launch_bounds__(kRowBlock) k_factor_p(int n, RBMat<T, V> G, const T* __restrict__ t,
                                                          T* __restrict__ p, const CGState* st, int first) {
   if (st->done) return;
   const T beta = first ? T(0) : (T)(st->rho1 / st->rho);
-  const BundleWalk bw = bundle_walk(n);
-  for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const int row = b * kBundle + (threadIdx.x & (kWave - 1));
-    const T pold = p[min(row, n - 1)];
-    const T z = rb_sum_single<T>(G, b, [&](int c) { return t[c]; });
-    if (row < n) p[row] = first ? z : z + beta * pold;
+  if constexpr (LAY == kRowUniform) {
+    // bundles in pairs (uni_sum2)
+    const int nb = (n + kWave - 1) / kWave;
+    const BundleWalk bw = bundle_walk_units((nb + 1) / 2);
+    for (int u = bw.first; u < bw.end; u += bw.stride) {
+      const int b0 = 2 * u, b1 = min(2 * u + 1, nb - 1);
+      const int row0 = b0 * kBundle + (threadIdx.x & (kWave - 1)), row1 = row0 + kBundle;
+      const T p0 = p[min(row0, n - 1)], p1 = p[min(row1, n - 1)];
+      T z0, z1;
+      uni_sum2<T>(G, b0, b1, [&](int c) { return t[c]; }, z0, z1);
+      if (row0 < n) p[row0] = first ? z0 : z0 + beta * p0;
+      if (row1 < n) p[row1] = first ? z1 : z1 + beta * p1;
+    }
+  } else {
+    const BundleWalk bw = bundle_walk(n);
+    for (int b = bw.first; b < bw.end; b += bw.stride) {
+      const int row = b * kBundle + (threadIdx.x & (kWave - 1));
+      const T pold = p[min(row, n - 1)];
+      const T z = rb_factor_sum<T, LAY>(G, b, [&](int c) { return t[c]; });
+      if (row < n) p[row] = first ? z : z + beta * pold;
+    }
   }
 }
 
@@ -374,15 +438,14 @@ __global__ void __launch_bounds__(kBlock) k_demote(int n, const double* __restri
 }
 
 // ---- launchers -------------------------------------------------------------------------------
-template <typename T> static inline long rblocks(long n) { return bundle_blocks(n, (int)sizeof(T)); }
-template <typename T> static inline dim3 bgrid(long n) { return dim3((unsigned)rblocks<T>(n)); }
 static inline dim3 egrid(long n) { return dim3((unsigned)ept_blocks(n)); }
 
 template <typename T>
 void launch_residual(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x, const T* b, T* r, double* out) {
   const dim3 g((unsigned)rb_blocks(n, A));
-  if (A.ell) hipLaunchKernelGGL((k_residual<T, true>), g, dim3(kRowBlock), 0, s, n, A, x, b, r, w);
-  else hipLaunchKernelGGL((k_residual<T, false>), g, dim3(kRowBlock), 0, s, n, A, x, b, r, w);
+  by_layout(A, [&](auto L) {
+    hipLaunchKernelGGL((k_residual<T, decltype(L)::value>), g, dim3(kRowBlock), 0, s, n, A, x, b, r, w);
+  });
   launch_finalize(s, w, rb_blocks(n, A), 1, kFinStore, out, nullptr);
 }
 
@@ -393,13 +456,11 @@ void launch_cheb_step(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* xin,
   dim3 g((unsigned)rb_blocks(n, A)), blk(kRowBlock);
   const int* done = st ? &st->done : done_flag;
   const int rho = st ? 1 : 0;
-  auto go = [&](auto first_c, auto last_c, auto ell_c) {
-    hipLaunchKernelGGL((k_cheb_step<T, decltype(first_c)::value, decltype(last_c)::value, decltype(ell_c)::value>), g,
-                       blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done, rho, w);
-  };
   auto by_ell = [&](auto first_c, auto last_c) {
-    if (A.ell) go(first_c, last_c, std::true_type{});
-    else go(first_c, last_c, std::false_type{});
+    by_layout(A, [&](auto lay_c) {
+      hipLaunchKernelGGL((k_cheb_step<T, decltype(first_c)::value, decltype(last_c)::value, decltype(lay_c)::value>),
+                         g, blk, 0, s, n, A, xin, yin, yout, p, invd, (T)a, (T)b, done, rho, w);
+    });
   };
   if (first && last) by_ell(std::true_type{}, std::true_type{});
   else if (first) by_ell(std::true_type{}, std::false_type{});
@@ -438,14 +499,14 @@ long launch_neumann_apply(hipStream_t s, RedWork w, int n, RBMat<T> A, int order
     T* out = ((order - 1 - i) % 2 == 0) ? z : tmp;
     const bool last = i == order - 1;
     const T* rd = last ? rdot : (const T*)nullptr;
-    if (last && A.ell)
-      hipLaunchKernelGGL((k_neumann_step<T, true, true>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rd, w);
-    else if (last)
-      hipLaunchKernelGGL((k_neumann_step<T, true, false>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rd, w);
-    else if (A.ell)
-      hipLaunchKernelGGL((k_neumann_step<T, false, true>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rd, w);
-    else
-      hipLaunchKernelGGL((k_neumann_step<T, false, false>), g, blk, 0, s, n, A, x, yin, out, (T)f, done, rd, w);
+    by_layout(A, [&](auto L) {
+      if (last)
+        hipLaunchKernelGGL((k_neumann_step<T, true, decltype(L)::value>), g, blk, 0, s, n, A, x, yin, out, (T)f, done,
+                           rd, w);
+      else
+        hipLaunchKernelGGL((k_neumann_step<T, false, decltype(L)::value>), g, blk, 0, s, n, A, x, yin, out, (T)f, done,
+                           rd, w);
+    });
     yin = out;
   }
   return (rdot && order > 0) ? rb_blocks(n, A) : 0;
@@ -465,8 +526,9 @@ void launch_cg_p(hipStream_t s, int n, const T* z, T* p, CGState* st, int first)
 template <typename T>
 void launch_cg_q(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st) {
   const dim3 g((unsigned)rb_blocks(n, A));
-  if (A.ell) hipLaunchKernelGGL((k_cg_q<T, true>), g, dim3(kRowBlock), 0, s, n, A, p, q, st, w);
-  else hipLaunchKernelGGL((k_cg_q<T, false>), g, dim3(kRowBlock), 0, s, n, A, p, q, st, w);
+  by_layout(A, [&](auto L) {
+    hipLaunchKernelGGL((k_cg_q<T, decltype(L)::value>), g, dim3(kRowBlock), 0, s, n, A, p, q, st, w);
+  });
   launch_finalize(s, w, rb_blocks(n, A), 1, kFinPQ, nullptr, st);
 }
 
@@ -479,8 +541,9 @@ void launch_spmv_dot_rb(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* x,
     return;
   }
   const dim3 g((unsigned)rb_blocks(n, A));
-  if (A.ell) hipLaunchKernelGGL((k_cg_q<T, true>), g, dim3(kRowBlock), 0, s, n, A, x, y, idle, w);
-  else hipLaunchKernelGGL((k_cg_q<T, false>), g, dim3(kRowBlock), 0, s, n, A, x, y, idle, w);
+  by_layout(A, [&](auto L) {
+    hipLaunchKernelGGL((k_cg_q<T, decltype(L)::value>), g, dim3(kRowBlock), 0, s, n, A, x, y, idle, w);
+  });
   launch_finalize(s, w, rb_blocks(n, A), 1, kFinStore, out, nullptr);
 }
 
@@ -499,20 +562,30 @@ void launch_cg_xr(hipStream_t s, RedWork w, int n, const T* p, const T* q, T* x,
 template <typename T>
 void launch_factor_apply(hipStream_t s, RedWork w, int n, RBMat<T> G, RBMat<T> GT, const T* r, T* tmp, T* z,
                          CGState* st) {
-  hipLaunchKernelGGL(k_rb_spmv_guarded<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, GT, r, tmp, (const CGState*)st);
-  hipLaunchKernelGGL(k_rb_spmv_dot<T>, bgrid<T>(n), dim3(kRowBlock), 0, s, n, G, tmp, r, z, st, w);
-  launch_finalize(s, w, rblocks<T>(n), 1, kFinRho1, nullptr, st);
+  by_layout(GT, [&](auto L) {
+    hipLaunchKernelGGL((k_rb_spmv_guarded<T, decltype(L)::value>), dim3((unsigned)rb_blocks(n, GT)), dim3(kRowBlock), 0, s, n, GT, r, tmp,
+                       (const CGState*)st);
+  });
+  by_layout(G, [&](auto L) {
+    hipLaunchKernelGGL((k_rb_spmv_dot<T, decltype(L)::value>), dim3((unsigned)rb_blocks(n, G)), dim3(kRowBlock), 0, s, n, G, tmp, r, z, st,
+                       w);
+  });
+  launch_finalize(s, w, rb_blocks(n, G), 1, kFinRho1, nullptr, st);
 }
 
 template <typename T, typename V>
 void launch_factor_p(hipStream_t s, RedWork w, int n, RBMat<T, V> G, RBMat<T, V> GT, const T* r, T* t, T* p,
                      CGState* st, int first, int rho1_only) {
-  hipLaunchKernelGGL((k_factor_t<T, V>), bgrid<T>(n), dim3(kRowBlock), 0, s, n, GT, r, t, st, w);
+  by_layout(GT, [&](auto L) {
+    hipLaunchKernelGGL((k_factor_t<T, V, decltype(L)::value>), dim3((unsigned)rb_blocks(n, GT)), dim3(kRowBlock), 0, s, n, GT, r, t, st, w);
+  });
   // first step (or a resumed one): rho1 only; later: rho = rho1, ++iter and the stop test of the
   // previous update, rho1 = t.t
-  launch_finalize(s, w, rblocks<T>(n), 2, (first || rho1_only) ? kFinRho1B : kFinXRDiag, nullptr, st);
-  hipLaunchKernelGGL((k_factor_p<T, V>), bgrid<T>(n), dim3(kRowBlock), 0, s, n, G, t, p, (const CGState*)st,
-                     first);
+  launch_finalize(s, w, rb_blocks(n, GT), 2, (first || rho1_only) ? kFinRho1B : kFinXRDiag, nullptr, st);
+  by_layout(G, [&](auto L) {
+    hipLaunchKernelGGL((k_factor_p<T, V, decltype(L)::value>), dim3((unsigned)rb_blocks(n, G)), dim3(kRowBlock), 0, s, n, G, t, p,
+                       (const CGState*)st, first);
+  });
 }
 template <typename T>
 void launch_cg_update(hipStream_t s, int n, const T* p, const T* q, T* x, T* r, CGState* st) {

@@ -29,7 +29,20 @@ struct RBMat {
   // 64-row bundles per slice (slice height / 64); bo then holds the slice offsets (bro),
   // bc / bv the BEllPack columns and values (padding: column 0, value 0)
   int ell = 0;
+  // Uniform RB-DIA when > 0: every bundle holds the same `uni` slots, at the column offsets
+  // uoff[0..uni) (ascending), bundle b's at slots b uni .. b uni + uni - 1, absent entries
+  // stored as zeros. The row kernels then take the offsets from here (kernel arguments)
+  // instead of loading bo / doff / dmask per bundle; bo, doff and dmask still describe the
+  // same matrix as an ordinary RB-DIA. nrows: the matrix's rows (gathers are clamped to it).
+  int uni = 0;
+  int nrows = 0;
+  int uoff[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
+constexpr int kUniMaxSlots = 8;
+
+// Row-kernel instance per layout class (a template argument: each kernel's code holds only its
+// layout's row sum)
+enum RowLayout { kRowBundled = 0, kRowEll = 1, kRowUniform = 2 };
 
 // Scalars of one CG solve; lives in device memory, read by the host only to
 // poll `done`. Mirrors the locals of CG::solve (rho, rho1, alpha) and the

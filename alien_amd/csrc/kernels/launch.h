@@ -60,6 +60,10 @@ long rbdia_count_device(hipStream_t, int n, const int* ro, const int* cols, int 
                         bool pair = false);
 void launch_rbdia_struct(hipStream_t, int n, const int* ro, const int* cols, const int* bo, int* doff,
                          unsigned long long* dmask, bool pair = false);
+// uniform RB-DIA (RBMat::uni): bo[b] = b k, every bundle's slots at the k ascending offsets `off`
+// (host array, the union of the matrix's diagonals), lane masks as in RB-DIA
+void launch_rbdia_uniform_struct(hipStream_t, int n, const int* ro, const int* cols, int k, const int* off, int* bo,
+                                 int* doff, unsigned long long* dmask);
 // values of the pair mode (two planes per slot)
 template <typename T>
 void launch_rbdia2_fill(hipStream_t, int n, const int* ro, const int* cols, const double* vals, const int* bo,

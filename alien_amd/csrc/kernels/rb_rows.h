@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.h"
 #include "krylov_state.h"
 
@@ -168,6 +170,94 @@ __device__ __forceinline__ T ell_sum(const RBMat<T, V>& A, int b, XS xs) {
   return w <= 4 ? ell_sum_u<4, T>(A, base, stride, w, xs) : ell_sum_u<7, T>(A, base, stride, w, xs);
 }
 
+// Uniform RB-DIA (RBMat::uni = K): the bundle's K slots sit at b K .. b K + K - 1 with the
+// column offsets A.uoff (kernel arguments, no per-bundle descriptor loads): one independent value
+// load and one gather per slot, so a bundle costs one memory round trip. Absent entries are stored
+// as zeros and their gathers clamped into [0, nrows): each present entry is added in ascending
+// column order as in the CSR sum, the absent ones add +0 (the same sum up to the sign of a zero).
+template <int K, typename T, typename V, typename XS>
+__device__ __forceinline__ T uni_sum_k(const RBMat<T, V>& A, int b, XS xs) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const long row = (long)b * kWave + lane;
+  const long base = (long)b * K * kWave + lane;
+  T v[K], xv[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) v[u] = widen<T>(__builtin_nontemporal_load(A.bv + base + (long)u * kWave));
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    const long c = row + A.uoff[u];
+    xv[u] = xs((int)(c < 0 ? 0 : c >= A.nrows ? A.nrows - 1 : c));
+  }
+  T acc = 0;
+#pragma unroll
+  for (int u = 0; u < K; ++u) acc = acc + v[u] * xv[u];
+  return acc;
+}
+template <typename T, typename V, typename XS>
+__device__ __forceinline__ T uni_sum(const RBMat<T, V>& A, int b, XS xs) {
+  switch (A.uni) {
+  case 4: return uni_sum_k<4, T>(A, b, xs);
+  case 7: return uni_sum_k<7, T>(A, b, xs);
+  case 1: return uni_sum_k<1, T>(A, b, xs);
+  case 2: return uni_sum_k<2, T>(A, b, xs);
+  case 3: return uni_sum_k<3, T>(A, b, xs);
+  case 5: return uni_sum_k<5, T>(A, b, xs);
+  case 6: return uni_sum_k<6, T>(A, b, xs);
+  default: return uni_sum_k<8, T>(A, b, xs);
+  }
+}
+
+// Uniform operators of at most this many slots are walked two bundles at a time by the CG step
+// kernel (k_cg_q); the factor kernels always pair.
+#ifndef ALIEN_HIP_UNI_PAIR_MAX_SLOTS
+#define ALIEN_HIP_UNI_PAIR_MAX_SLOTS 8
+#endif
+constexpr int kUniPairMaxSlots = ALIEN_HIP_UNI_PAIR_MAX_SLOTS;
+
+// Two bundles of a uniform RB-DIA at once (b0 and b1, b1 may repeat b0): both bundles' value
+// loads and gathers are issued before any product, twice the memory-level parallelism of a wave
+// walking one bundle at a time. Each row's sum is uni_sum_k's.
+template <int K, typename T, typename V, typename XS>
+__device__ __forceinline__ void uni_sum2_k(const RBMat<T, V>& A, int b0, int b1, XS xs, T& t0, T& t1) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const long r0 = (long)b0 * kWave + lane, r1 = (long)b1 * kWave + lane;
+  const long base0 = (long)b0 * K * kWave + lane, base1 = (long)b1 * K * kWave + lane;
+  T v0[K], v1[K], x0[K], x1[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    v0[u] = widen<T>(__builtin_nontemporal_load(A.bv + base0 + (long)u * kWave));
+    v1[u] = widen<T>(__builtin_nontemporal_load(A.bv + base1 + (long)u * kWave));
+  }
+  const long last = A.nrows - 1;
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    const long c0 = r0 + A.uoff[u], c1 = r1 + A.uoff[u];
+    x0[u] = xs((int)(c0 < 0 ? 0 : c0 > last ? last : c0));
+    x1[u] = xs((int)(c1 < 0 ? 0 : c1 > last ? last : c1));
+  }
+  T a0 = 0, a1 = 0;
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    a0 = a0 + v0[u] * x0[u];
+    a1 = a1 + v1[u] * x1[u];
+  }
+  t0 = a0;
+  t1 = a1;
+}
+template <typename T, typename V, typename XS>
+__device__ __forceinline__ void uni_sum2(const RBMat<T, V>& A, int b0, int b1, XS xs, T& t0, T& t1) {
+  switch (A.uni) {
+  case 4: uni_sum2_k<4, T>(A, b0, b1, xs, t0, t1); return;
+  case 7: uni_sum2_k<7, T>(A, b0, b1, xs, t0, t1); return;
+  case 1: uni_sum2_k<1, T>(A, b0, b1, xs, t0, t1); return;
+  case 2: uni_sum2_k<2, T>(A, b0, b1, xs, t0, t1); return;
+  case 3: uni_sum2_k<3, T>(A, b0, b1, xs, t0, t1); return;
+  case 5: uni_sum2_k<5, T>(A, b0, b1, xs, t0, t1); return;
+  case 6: uni_sum2_k<6, T>(A, b0, b1, xs, t0, t1); return;
+  default: uni_sum2_k<8, T>(A, b0, b1, xs, t0, t1); return;
+  }
+}
+
 // Row sum of this lane's row of bundle b over the vector given by xs(column).
 // Short bundles (factors of M = G G^T: 4 slots at a 7-point stencil) use a
 // batch of 4, longer ones a batch of 7.
@@ -181,13 +271,15 @@ __device__ __forceinline__ T rb_sum_single(const RBMat<T, V>& A, int b, XS xs) {
   if (w <= 4) return rb_sum_u<4>(A, b, s0, w, xs);
   return rb_sum_u<7>(A, b, s0, w, xs);
 }
-// ELL selects the BEllPack slice mode at compile time: kernels are instantiated once per
-// layout class (the launchers dispatch on RBMat::ell), so the row-bundled kernels carry
-// no BEllPack code and compile as before.
-template <typename T, bool ELL = false, typename V, typename XS>
+// LAY selects the layout class at compile time: kernels are instantiated once per class (the
+// launchers dispatch with by_layout), so the row-bundled kernels carry no BEllPack or uniform
+// code and compile as before.
+template <typename T, int LAY = kRowBundled, typename V, typename XS>
 __device__ __forceinline__ T rb_sum(const RBMat<T, V>& A, int b, XS xs) {
-  if constexpr (ELL) {
+  if constexpr (LAY == kRowEll) {
     return ell_sum<T>(A, b, xs);
+  } else if constexpr (LAY == kRowUniform) {
+    return uni_sum<T>(A, b, xs);
   } else {
     if (A.pair) {
       const int s0 = A.bo[b];
@@ -199,12 +291,28 @@ __device__ __forceinline__ T rb_sum(const RBMat<T, V>& A, int b, XS xs) {
   }
 }
 
+// Row sum of the operands that are never in pair or BEllPack mode (the learned factors):
+// uniform or row-bundled.
+template <typename T, int LAY, typename V, typename XS>
+__device__ __forceinline__ T rb_factor_sum(const RBMat<T, V>& A, int b, XS xs) {
+  if constexpr (LAY == kRowUniform) return uni_sum<T>(A, b, xs);
+  else return rb_sum_single<T>(A, b, xs);
+}
+
 // Workgroups of a row kernel over operand A: BEllPack (slice mode) runs one bundle per wave
 // (measured at 256^3: FP32 157 us uncapped against 192 us at the 4096 cap), the row-bundled
 // layouts follow bundle_blocks' caps.
 template <typename T, typename V>
 inline long rb_blocks(long n, const RBMat<T, V>& A) {
-  return A.ell ? bundle_blocks_max(n) : bundle_blocks(n, (int)sizeof(T));
+  return A.ell ? bundle_blocks_max(n) : A.uni ? uni_blocks(n, A.uni) : bundle_blocks(n, (int)sizeof(T));
+}
+
+// f(std::integral_constant<int, L>{}) with L the layout class of A (host side, at launch)
+template <typename T, typename V, typename F>
+inline void by_layout(const RBMat<T, V>& A, F f) {
+  if (A.ell) f(std::integral_constant<int, kRowEll>{});
+  else if (A.uni) f(std::integral_constant<int, kRowUniform>{});
+  else f(std::integral_constant<int, kRowBundled>{});
 }
 
 } // namespace alien_hip

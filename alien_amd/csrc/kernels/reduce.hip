@@ -11,6 +11,8 @@ namespace alien_hip {
 
 long g_row_blocks_f64 = kUncapped;
 long g_row_blocks_f32 = 4096;
+long g_row_blocks_uni = 16384;
+long g_row_blocks_uni_wide = 32768;
 
 constexpr int kFin = 1024;
 

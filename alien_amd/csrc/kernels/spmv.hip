@@ -69,13 +69,13 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot_csr(int n, const int* __res
 // ---- RB-CSR / RB-DIA --------------------------------------------------------------
 // One wavefront per bundle (rb_rows.h); two entry points so that profiles name
 // the layout.
-template <typename T, bool ELL = false>
+template <typename T, int LAY = kRowBundled>
 __device__ __forceinline__ void spmv_rb_body(int n, const RBMat<T>& A, const T* __restrict__ x,
                                              T* __restrict__ y) {
   const int lane = threadIdx.x & (kWave - 1);
   const BundleWalk bw = bundle_walk(n);
   for (int b = bw.first; b < bw.end; b += bw.stride) {
-    const T acc = rb_sum<T, ELL>(A, b, [&](int c) { return x[c]; });
+    const T acc = rb_sum<T, LAY>(A, b, [&](int c) { return x[c]; });
     const int row = b * kBundle + lane;
     if (row < n) y[row] = acc;
   }
@@ -89,6 +89,12 @@ template <typename T>
 __global__ void __launch_bounds__(kRowBlock) k_spmv_rbdia(int n, RBMat<T> A, const T* __restrict__ x,
                                                            T* __restrict__ y) {
   spmv_rb_body<T>(n, A, x, y);
+}
+// uniform RB-DIA (RBMat::uni)
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock) k_spmv_udia(int n, RBMat<T> A, const T* __restrict__ x,
+                                                          T* __restrict__ y) {
+  spmv_rb_body<T, kRowUniform>(n, A, x, y);
 }
 
 // Device conversion CSR -> RB-DIA: lane = row; walks the row's entries and the
@@ -253,7 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_rbcsr_fill(int n, const int* __restr
 template <typename T>
 __global__ void __launch_bounds__(kRowBlock) k_spmv_bellpack(int n, RBMat<T> A, const T* __restrict__ x,
                                                               T* __restrict__ y) {
-  spmv_rb_body<T, true>(n, A, x, y);
+  spmv_rb_body<T, kRowEll>(n, A, x, y);
 }
 
 // Device conversion CSR -> BEllPack (the scatter of SYCLBEllPackMatrix.cc:127-170 and
@@ -545,6 +551,7 @@ void launch_spmv_rb(hipStream_t st, int n, RBMat<T> A, const T* x, T* y) {
   if (n <= 0) return;
   const dim3 g((unsigned)rb_blocks(n, A));
   if (A.ell) hipLaunchKernelGGL(k_spmv_bellpack<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
+  else if (A.uni) hipLaunchKernelGGL(k_spmv_udia<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
   else if (A.doff) hipLaunchKernelGGL(k_spmv_rbdia<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
   else hipLaunchKernelGGL(k_spmv_rbcsr<T>, g, dim3(kRowBlock), 0, st, n, A, x, y);
 }
@@ -594,6 +601,52 @@ void launch_rbdia_struct(hipStream_t st, int n, const int* ro, const int* cols, 
     hipLaunchKernelGGL((k_rbdia_struct<true, false>), rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, 0, bo,
                        (int*)nullptr, doff, dmask);
 }
+// Uniform RB-DIA structure (RBMat::uni): every bundle gets the K slots of the offsets U (the union of
+// the matrix's diagonal offsets, ascending), bo[b] = b K, with each slot's lane mask as an ordinary
+// RB-DIA (rows lacking that offset have no bit, and their values are filled as zeros). One wave per
+// bundle, lane = row: the row's ascending offsets are merged with U.
+struct UniOffsets {
+  int k;
+  int off[kUniMaxSlots];
+};
+__global__ void __launch_bounds__(kBlock) k_rbdia_uniform_struct(int n, const int* __restrict__ ro,
+                                                                 const int* __restrict__ cols, UniOffsets U,
+                                                                 int* __restrict__ bo, int* __restrict__ doff,
+                                                                 unsigned long long* __restrict__ dmask) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int nb = (n + kBundle - 1) / kBundle;
+  const int b = t / kBundle, lane = t % kBundle;
+  if (b >= nb) return; // the whole wave: a wave is one bundle
+  const int row = t;
+  int k = 0, e = 0;
+  if (row < n) {
+    k = ro[row];
+    e = ro[row + 1];
+  }
+  for (int j = 0; j < U.k; ++j) {
+    const bool on = k < e && cols[k] - row == U.off[j];
+    if (on) ++k;
+    const unsigned long long m = __ballot(on);
+    if (lane == 0) {
+      doff[(long)b * U.k + j] = U.off[j];
+      dmask[(long)b * U.k + j] = m;
+    }
+  }
+  if (lane == 0) {
+    bo[b] = b * U.k;
+    if (b == nb - 1) bo[nb] = nb * U.k;
+  }
+}
+void launch_rbdia_uniform_struct(hipStream_t st, int n, const int* ro, const int* cols, int k, const int* off,
+                                 int* bo, int* doff, unsigned long long* dmask) {
+  if (n <= 0) return;
+  if (k < 1 || k > kUniMaxSlots) throw Error(ALIEN_HIP_EINVAL, "uniform RB-DIA: 1 to 8 offsets");
+  UniOffsets U{};
+  U.k = k;
+  for (int j = 0; j < k; ++j) U.off[j] = off[j];
+  hipLaunchKernelGGL(k_rbdia_uniform_struct, rows_grid(n), dim3(kBlock), 0, st, n, ro, cols, U, bo, doff, dmask);
+}
+
 template <typename T>
 void launch_rbdia2_fill(hipStream_t st, int n, const int* ro, const int* cols, const double* vals, const int* bo,
                         const int* doff, T* bv) {

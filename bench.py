@@ -41,7 +41,12 @@ HBM_PEAK_GBS = 8000.0
 # The roofline line covers the dominant kernel of the headline solve (learned + mixed CG):
 # k_cg_q<float>, q = A p with p.q on the FP32 RB-DIA matrix. bench.py times it through
 # alien_hip_spmv_dot_rbdia_f32, which launches that kernel (then its one-workgroup finalize).
-ROOF, ROOF_KERNEL = "cgq_fp32", "k_cg_q<float, false>"  # false: the row-bundled (not BEllPack) instance
+# the step kernel of the headline solve: q = A p, p.q on the FP32 uniform RB-DIA (layout class 2)
+ROOF, ROOF_KERNEL = "cgq_fp32", "k_cg_q<float, 2>"
+BESIDE = ("k_factor_t<float, float, 2>", "k_factor_p<float, float, 2>", "k_cg_update<float>")
+# workgroups of the step kernel on a 7-slot uniform RB-DIA at 256^3 (the library's default cap for
+# uniform operators of 6 or more slots): one FP64 partial each
+UNI_WIDE_GRID = 32768
 # the CSR-family kernel whose bytes are the CSR formula's (RB-CSR stores every CSR column)
 CSR_ROOF, CSR_ROOF_KERNEL = "rbcsr_fp64", "k_spmv_rbcsr<double>"
 PROFILES = os.path.join(ROOT, "profiles")
@@ -96,6 +101,12 @@ def spmv_bytes_fp32(n, nnz):
     return 8 * nnz + 4 * (n + 1) + 8 * n
 
 
+def udia_bytes(n, nnz, word):
+    """Uniform RB-DIA: the values of the stored entries, x once, y (no descriptors are read; the
+    zero padding of absent entries, 0.2% at 256^3, is not credited)."""
+    return word * nnz + 2 * word * n
+
+
 def bellpack_bytes(n, nnz, nslices, word):
     """BEllPack: values and columns of the unpadded entries, x once, y, slice offsets (no row
     offsets are read; padding is not credited)."""
@@ -129,6 +140,16 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     ddbo, ddoff, ddmask = dev(dbo_d), dev(doff), dev(dmask.view(np.int64))
     dv64 = torch.empty(dslots * 64, dtype=torch.float64, device="cuda")
     dv32 = torch.empty(dslots * 64, dtype=torch.float32, device="cuda")
+    # uniform RB-DIA (every bundle padded to the matrix's diagonals): the fused solvers' layout
+    uoff = aa.host_diagonals(ro, cols)
+    nbun = (n + 63) // 64
+    uk = len(uoff)
+    ubo = torch.empty(nbun + 1, dtype=torch.int32, device="cuda")
+    udoff = torch.empty(nbun * uk, dtype=torch.int32, device="cuda")
+    udmask = torch.empty(nbun * uk, dtype=torch.int64, device="cuda")
+    uv64 = torch.empty(nbun * uk * 64, dtype=torch.float64, device="cuda")
+    uv32 = torch.empty(nbun * uk * 64, dtype=torch.float32, device="cuda")
+    uo = aa._capi.iptr(uoff)
     # Alien's BEllPack (slice 1024), scattered on the device
     bro = aa.bellpack_bro(ro, 1024)
     btotal = int(bro[-1]) * 1024
@@ -144,6 +165,9 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     aa.check(L.alien_hip_rbcsr_fill_f32(ctx.h, n, P(dro), P(dc), P(dv), P(dbo), P(bc), P(bv32)))
     aa.check(L.alien_hip_rbdia_fill_f64(ctx.h, n, P(dro), P(dc), P(dv), P(ddbo), P(ddoff), P(dv64)))
     aa.check(L.alien_hip_rbdia_fill_f32(ctx.h, n, P(dro), P(dc), P(dv), P(ddbo), P(ddoff), P(dv32)))
+    aa.check(L.alien_hip_rbdia_uniform_struct(ctx.h, n, P(dro), P(dc), uk, uo, P(ubo), P(udoff), P(udmask)))
+    aa.check(L.alien_hip_rbdia_fill_f64(ctx.h, n, P(dro), P(dc), P(dv), P(ubo), P(udoff), P(uv64)))
+    aa.check(L.alien_hip_rbdia_fill_f32(ctx.h, n, P(dro), P(dc), P(dv), P(ubo), P(udoff), P(uv32)))
     x = torch.rand(n, dtype=torch.float64, device="cuda")
     x32 = x.float()
     y = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -151,13 +175,17 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     y32 = torch.empty(n, dtype=torch.float32, device="cuda")
     y32b = torch.empty(n, dtype=torch.float32, device="cuda")
     yq32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    yqr32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    yu64 = torch.empty(n, dtype=torch.float64, device="cuda")
+    yu32 = torch.empty(n, dtype=torch.float32, device="cuda")
     ye64 = torch.empty(n, dtype=torch.float64, device="cuda")
     ye32 = torch.empty(n, dtype=torch.float32, device="cuda")
     dotv = torch.empty(1, dtype=torch.float64, device="cuda")
     ctx.synchronize()
     torch.cuda.synchronize()
     res = {"rbdia_slots": dslots, "bellpack_slices": len(bro) - 1, "bellpack_padding":
-           btotal / max(len(vals), 1) - 1.0}
+           btotal / max(len(vals), 1) - 1.0, "udia_slots_per_bundle": uk,
+           "udia_padding": nbun * uk * 64 / max(len(vals), 1) - 1.0}
     kernels = (
         ("rbcsr_fp64", lambda: L.alien_hip_spmv_rbcsr_f64(ctx.h, n, P(dbo), P(bc), P(bv), P(x), P(y))),
         ("rbcsr_fp32", lambda: L.alien_hip_spmv_rbcsr_f32(ctx.h, n, P(dbo), P(bc), P(bv32), P(x32), P(y32))),
@@ -165,8 +193,11 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
                                                           P(y2))),
         ("rbdia_fp32", lambda: L.alien_hip_spmv_rbdia_f32(ctx.h, n, P(ddbo), P(ddoff), P(ddmask), P(dv32), P(x32),
                                                           P(y32b))),
-        ("cgq_fp32", lambda: L.alien_hip_spmv_dot_rbdia_f32(ctx.h, n, P(ddbo), P(ddoff), P(ddmask), P(dv32), P(x32),
-                                                            P(yq32), P(dotv))),
+        ("udia_fp64", lambda: L.alien_hip_spmv_udia_f64(ctx.h, n, uk, uo, P(uv64), P(x), P(yu64))),
+        ("udia_fp32", lambda: L.alien_hip_spmv_udia_f32(ctx.h, n, uk, uo, P(uv32), P(x32), P(yu32))),
+        ("cgq_fp32", lambda: L.alien_hip_spmv_dot_udia_f32(ctx.h, n, uk, uo, P(uv32), P(x32), P(yq32), P(dotv))),
+        ("cgq_rbdia_fp32", lambda: L.alien_hip_spmv_dot_rbdia_f32(ctx.h, n, P(ddbo), P(ddoff), P(ddmask), P(dv32),
+                                                                  P(x32), P(yqr32), P(dotv))),
         ("bellpack_fp64", lambda: L.alien_hip_spmv_bellpack_f64(ctx.h, n, 1024, P(dbro), P(ebc), P(ebv), P(x),
                                                                 P(ye64))),
         ("bellpack_fp32", lambda: L.alien_hip_spmv_bellpack_f32(ctx.h, n, 1024, P(dbro), P(ebc), P(ebv32), P(x32),
@@ -186,10 +217,12 @@ def time_spmv(aa, torch, ctx, ro, cols, vals, reps):
     # the two layouts are the same sum: check once at full size
     res["rbdia_equals_rbcsr_fp64"] = bool(torch.equal(y, y2))
     res["rbdia_equals_rbcsr_fp32"] = bool(torch.equal(y32, y32b))
-    res["cgq_equals_rbdia_fp32"] = bool(torch.equal(yq32, y32b))
+    res["cgq_equals_rbdia_fp32"] = bool(torch.equal(yq32, y32b)) and bool(torch.equal(yqr32, y32b))
+    res["udia_equals_rbcsr"] = bool(torch.equal(yu64, y)) and bool(torch.equal(yu32, y32))
     res["bellpack_equals_rbcsr_fp64"] = bool(torch.equal(ye64, y))
     res["bellpack_equals_rbcsr_fp32"] = bool(torch.equal(ye32, y32))
-    del bc, bv, bv32, dro, dc, dv, dbo, dv64, dv32, ddbo, ddoff, ddmask, dbro, ebc, ebv, ebv32
+    del bc, bv, bv32, dro, dc, dv, dbo, dv64, dv32, ddbo, ddoff, ddmask, dbro, ebc, ebv, ebv32, ubo, udoff, udmask
+    del uv64, uv32
     torch.cuda.empty_cache()
     return res
 
@@ -798,13 +831,16 @@ def main():
         for _ in range(a.spmv_reps):
             aa.check(aa.lib().alien_hip_copy_f64(ctx.h, N, P(src), P(dst)))
         ctx.synchronize()
-        print(json.dumps({"spmv_us": {k: v * 1e6 for k, v in sp.items() if isinstance(v, float) and k != "bellpack_padding"},
+        print(json.dumps({"spmv_us": {k: v * 1e6 for k, v in sp.items() if isinstance(v, float) and
+                                      not k.endswith("padding")},
                           "copy_bytes_each_way": 8 * N,
                           "algorithmic_bytes": {
                               "rbcsr_fp64": spmv_bytes_fp64(N, nnz), "rbcsr_fp32": spmv_bytes_fp32(N, nnz),
                               "bellpack_fp64": bellpack_bytes(N, nnz, sp["bellpack_slices"], 8),
                               "bellpack_fp32": bellpack_bytes(N, nnz, sp["bellpack_slices"], 4),
-                              "cgq_fp32": rbdia_bytes(N, nnz, sp["rbdia_slots"], 4) + 8 * 4096,
+                              "cgq_fp32": udia_bytes(N, nnz, 4) + 8 * UNI_WIDE_GRID,
+                              "cgq_rbdia_fp32": rbdia_bytes(N, nnz, sp["rbdia_slots"], 4) + 8 * 4096,
+                              "udia_fp64": udia_bytes(N, nnz, 8), "udia_fp32": udia_bytes(N, nnz, 4),
                               "rbdia_fp64": rbdia_bytes(N, nnz, sp["rbdia_slots"], 8),
                               "rbdia_fp32": rbdia_bytes(N, nnz, sp["rbdia_slots"], 4)}}), flush=True)
         ctx.close()
@@ -911,11 +947,19 @@ def main():
         spmv[name] = {"us": sp[name] * 1e6, "bytes": nbytes, "gbs": nbytes / sp[name] / 1e9,
                       "gflops": 2 * nnz / sp[name] / 1e9}
     spmv["rbdia_fp64"]["csr_equivalent_gbs"] = csr64 / sp["rbdia_fp64"] / 1e9
-    # k_cg_q<float>: the FP32 RB-DIA bytes (p is x: read once, its p.q reuse is the row's own entry)
-    # plus the 8-byte partial of each workgroup; timed with its finalize launch
-    q_bytes = rbdia_bytes(N, nnz, sp["rbdia_slots"], 4) + 8 * 4096
+    for name, word in (("udia_fp64", 8), ("udia_fp32", 4)):
+        ub = udia_bytes(N, nnz, word)
+        spmv[name] = {"us": sp[name] * 1e6, "bytes": ub, "gbs": ub / sp[name] / 1e9, "gflops": 2 * nnz / sp[name] / 1e9,
+                      "padding": sp["udia_padding"], "equals_rbcsr": sp["udia_equals_rbcsr"]}
+    # k_cg_q<float, 2>: the FP32 uniform RB-DIA bytes (p is x: read once, its p.q reuse is the row's
+    # own entry) plus the 8-byte partial of each workgroup; timed with its finalize launch. Beside it
+    # the same step on RB-DIA (per-bundle descriptors, 4096 workgroups).
+    q_bytes = udia_bytes(N, nnz, 4) + 8 * UNI_WIDE_GRID
     spmv["cgq_fp32"] = {"us": sp["cgq_fp32"] * 1e6, "bytes": q_bytes, "gbs": q_bytes / sp["cgq_fp32"] / 1e9,
                         "gflops": 2 * nnz / sp["cgq_fp32"] / 1e9, "equals_spmv_rbdia_fp32": sp["cgq_equals_rbdia_fp32"]}
+    qr_bytes = rbdia_bytes(N, nnz, sp["rbdia_slots"], 4) + 8 * 4096
+    spmv["cgq_rbdia_fp32"] = {"us": sp["cgq_rbdia_fp32"] * 1e6, "bytes": qr_bytes,
+                              "gbs": qr_bytes / sp["cgq_rbdia_fp32"] / 1e9}
     for name, word in (("bellpack_fp64", 8), ("bellpack_fp32", 4)):
         nb = bellpack_bytes(N, nnz, sp["bellpack_slices"], word)
         spmv[name] = {"us": sp[name] * 1e6, "bytes": nb, "gbs": nb / sp[name] / 1e9, "gflops": 2 * nnz / sp[name] / 1e9,
@@ -949,7 +993,7 @@ def main():
     if roof_frac > 1.0:
         raise RuntimeError(f"{ROOF_KERNEL}: {roof_gbs:.0f} GB/s exceeds the HBM peak: bytes or time wrong")
     beside = {k: {"us_in_solve": profile_avg_us(k), "source": os.path.relpath(STATS_PROFILE, ROOT)}
-              for k in ("k_factor_t<float, float>", "k_factor_p<float, float>", "k_cg_update<float>")}
+              for k in BESIDE}
 
     out = {
         "metric": "time-to-1e-8 residual (s), SpMV HBM GB/s; learned+mixed vs plain FP64",

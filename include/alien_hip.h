@@ -255,6 +255,32 @@ int alien_hip_spmv_dot_rbdia_f32(alien_hip_ctx* ctx, int nrows, const int* bundl
                                  const uint64_t* dmask, const float* bvals, const float* x, float* y,
                                  double* result);
 
+/* Uniform RB-DIA ("UDIA"), the layout the fused solvers use for stencil matrices: an RB-DIA whose
+ * bundles all hold the same k <= 8 slots, at the k distinct diagonal offsets of the whole matrix
+ * (ascending), bundle b's at slots b k .. b k + k - 1, absent entries stored as zeros. The kernels
+ * take the k offsets as arguments and load no per-bundle descriptor: one value load and one gather
+ * per slot, all independent (RB-DIA's bundle offsets and slot descriptors are two dependent loads
+ * before them). Sums: the CSR sum's nonzero terms in its order, plus +0 for each absent entry
+ * (equal values; only the sign of a zero sum can differ). Same SpMV as SimpleCSRMatrix's
+ * (SimpleCSRMatrixMultT.h:42-76, _seqMult); no counterpart layout in the reference.
+ * host_diagonals (no GPU): the distinct col - row of the CSR, ascending, into offsets (room for
+ * max_k), *k_out = their number or -1 if more than max_k. rbdia_uniform_struct: bundle_off
+ * (nbundles + 1), doff and dmask (nbundles k entries) of that layout, an ordinary RB-DIA of the
+ * matrix (rbdia_fill_* then fills its values); offsets is a host array. */
+int alien_hip_host_diagonals(int nrows, const int* row_offset, const int* cols, int max_k, int* offsets,
+                             int* k_out);
+int alien_hip_rbdia_uniform_struct(alien_hip_ctx* ctx, int nrows, const int* row_offset, const int* cols, int k,
+                                   const int* offsets, int* bundle_off, int* doff, uint64_t* dmask);
+int alien_hip_spmv_udia_f64(alien_hip_ctx* ctx, int nrows, int k, const int* offsets, const double* bvals,
+                            const double* x, double* y);
+int alien_hip_spmv_udia_f32(alien_hip_ctx* ctx, int nrows, int k, const int* offsets, const float* bvals,
+                            const float* x, float* y);
+/* the fused q = A p, p.q step (CG.h:133-134) on UDIA, as the fused CG iteration runs it */
+int alien_hip_spmv_dot_udia_f64(alien_hip_ctx* ctx, int nrows, int k, const int* offsets, const double* bvals,
+                                const double* x, double* y, double* result);
+int alien_hip_spmv_dot_udia_f32(alien_hip_ctx* ctx, int nrows, int k, const int* offsets, const float* bvals,
+                                const float* x, float* y, double* result);
+
 /* ------------------------------------------------------------------------ */
 /* Device kernels: BLAS-1 (CBLASMPIKernel.h:37-125 / SYCLKernelInternal.h:110-246) */
 /* ------------------------------------------------------------------------ */

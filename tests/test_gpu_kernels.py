@@ -124,6 +124,96 @@ def test_spmv_rbdia_bit_exact(ctx, name, A):
     assert abs(host(res, ctx)[0] - e32) <= 1e-13 * np.abs(y32h.astype(np.float64) * x).sum() + 1e-300
 
 
+UDIA_SYSTEMS = SYSTEMS[:3] + [("poisson2d 37x29", None), ("one row", None), ("65 rows", None)]
+
+
+def udia_system(name, A):
+    if name.startswith("poisson2d"):
+        B = sp.diags([-1, -1, 4, -1, -1], [-37, -1, 0, 1, 37], shape=(37 * 29, 37 * 29), format="csr")
+        B.sort_indices()
+        return B.indptr.astype(np.int32), B.indices.astype(np.int32), B.data.astype(np.float64)
+    if name == "one row":
+        return np.array([0, 1], np.int32), np.array([0], np.int32), np.array([2.5])
+    if name == "65 rows":
+        B = sp.diags([-1.5, 3.0, -0.5], [-1, 0, 1], shape=(65, 65), format="csr")
+        B.sort_indices()
+        return B.indptr.astype(np.int32), B.indices.astype(np.int32), B.data.astype(np.float64)
+    return A
+
+
+@pytest.mark.parametrize("name,A", UDIA_SYSTEMS, ids=[s[0] for s in UDIA_SYSTEMS])
+def test_spmv_uniform_rbdia(ctx, name, A):
+    """Uniform RB-DIA (every bundle padded to the matrix's k diagonals, offsets as kernel
+    arguments): its structure is an ordinary RB-DIA of the matrix (the RB-DIA kernel on it is
+    bit-exact with CSR); the uniform kernels give the CSR sum's values (FP64 against the oracle,
+    FP32 against the FP32 RB-CSR kernel; absent entries add +0, so compared with ==), and the fused
+    q = A p, p.q kernel its values and the dot within the FP64 reduction bound."""
+    ro, cols, vals = udia_system(name, A)
+    n = len(ro) - 1
+    off = aa.host_diagonals(ro, cols)
+    assert off is not None and np.all(np.diff(off) > 0)
+    rows = np.repeat(np.arange(n), np.diff(ro))
+    np.testing.assert_array_equal(off, np.unique(cols.astype(np.int64) - rows))
+    k = len(off)
+    nb = (n + 63) // 64
+    say("case udia", name, n, "offsets", list(off))
+    x = np.random.default_rng(11).standard_normal(n)
+    ref = orc.spmv(ro, cols, vals, x)
+    dro, dc, dv, dx = dev(ro), dev(cols), dev(vals), dev(x)
+    dbo, ddo, ddm = empty(nb + 1, torch.int32), empty(nb * k, torch.int32), empty(nb * k, torch.int64)
+    ioff = aa._capi.iptr(off)
+    call("alien_hip_rbdia_uniform_struct", ctx.h, n, p(dro), p(dc), k, ioff, p(dbo), p(ddo), p(ddm))
+    bo_h = host(dbo, ctx)
+    np.testing.assert_array_equal(bo_h, np.arange(nb + 1) * k)
+    np.testing.assert_array_equal(host(ddo, ctx), np.tile(off, nb))
+    # the lane masks: bit l of slot j of bundle b <=> row 64 b + l has offset off[j]
+    mh = host(ddm, ctx).view(np.uint64).reshape(nb, k)
+    want = np.zeros((nb, k), np.uint64)
+    jj = np.searchsorted(off, cols.astype(np.int64) - rows)
+    np.bitwise_or.at(want, (rows // 64, jj), (np.uint64(1) << (rows % 64).astype(np.uint64)))
+    np.testing.assert_array_equal(mh, want)
+    bv = empty(nb * k * 64)
+    call("alien_hip_rbdia_fill_f64", ctx.h, n, p(dro), p(dc), p(dv), p(dbo), p(ddo), p(bv))
+    y = empty(n)
+    call("alien_hip_spmv_rbdia_f64", ctx.h, n, p(dbo), p(ddo), p(ddm), p(bv), p(dx), p(y))
+    np.testing.assert_array_equal(host(y, ctx)[:n], ref)
+    yu = empty(n)
+    call("alien_hip_spmv_udia_f64", ctx.h, n, k, ioff, p(bv), p(dx), p(yu))
+    np.testing.assert_array_equal(host(yu, ctx)[:n], ref)
+    res = empty(1)
+    yq = empty(n)
+    call("alien_hip_spmv_dot_udia_f64", ctx.h, n, k, ioff, p(bv), p(dx), p(yq), p(res))
+    np.testing.assert_array_equal(host(yq, ctx)[:n], ref)
+    assert abs(host(res, ctx)[0] - np.dot(ref, x)) <= 1e-13 * np.abs(ref * x).sum() + 1e-300
+    bv32 = empty(nb * k * 64, torch.float32)
+    call("alien_hip_rbdia_fill_f32", ctx.h, n, p(dro), p(dc), p(dv), p(dbo), p(ddo), p(bv32))
+    dx32 = dev(x.astype(np.float32))
+    y32 = empty(n, torch.float32)
+    call("alien_hip_spmv_udia_f32", ctx.h, n, k, ioff, p(bv32), p(dx32), p(y32))
+    cbo = aa.rbcsr_bundle_offsets(ro)
+    cslots = int(cbo[-1]) * 64
+    dcbo = dev(cbo)
+    cbc = empty(cslots, torch.int32)
+    cbv = empty(cslots, torch.float32)
+    call("alien_hip_rbcsr_fill_f32", ctx.h, n, p(dro), p(dc), p(dv), p(dcbo), p(cbc), p(cbv))
+    yc = empty(n, torch.float32)
+    call("alien_hip_spmv_rbcsr_f32", ctx.h, n, p(dcbo), p(cbc), p(cbv), p(dx32), p(yc))
+    np.testing.assert_array_equal(host(y32, ctx)[:n], host(yc, ctx)[:n])
+    yq32 = empty(n, torch.float32)
+    call("alien_hip_spmv_dot_udia_f32", ctx.h, n, k, ioff, p(bv32), p(dx32), p(yq32), p(res))
+    np.testing.assert_array_equal(host(yq32, ctx)[:n], host(yc, ctx)[:n])
+
+
+def test_uniform_rbdia_rejects_bad_offsets(ctx):
+    ro, cols, vals = aa.gen_csr(aa.GEN_POISSON7, 4)
+    n = len(ro) - 1
+    x, y, v = empty(n), empty(n), empty(64 * 7)
+    for bad in (np.array([1, 0], np.int32), np.arange(9, dtype=np.int32)):
+        ptr = [C.c_void_p(t.data_ptr()) for t in (v, x, y)]
+        rc = aa.lib().alien_hip_spmv_udia_f64(ctx.h, n, len(bad), aa._capi.iptr(bad), *ptr)
+        assert rc == aa._capi.EINVAL
+
+
 def rbdia_device(ctx, dro, dc, n, max_width, pair=0):
     """RB-DIA structure built on the device: (bo, doff, dmask) on the host, or None."""
     nb = (n + 63) // 64

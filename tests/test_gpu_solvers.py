@@ -169,23 +169,34 @@ def test_lognormal_small_cg_chebyshev_and_diag(ctx):
 @pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_GRAPH, aa.MODE_MIXED])
 @pytest.mark.parametrize("precond", [aa.PREC_DIAG, aa.PREC_CHEB])
 def test_rbdia_and_rbcsr_solves_identical(ctx, mode, precond):
-    """The fused solvers give bit-identical iterates on RB-DIA (default) and
-    RB-CSR: both layouts sum each row in the sequential CSR order."""
+    """The fused solvers on the default layout of a stencil matrix (uniform RB-DIA) and on
+    RB-CSR: both sum each row in the sequential CSR order (the kernels' products are equal), and
+    the q = A p step walks the uniform layout's bundles in pairs, so only the summation order of
+    p.q differs: equal iteration counts within 1 and solutions within 1e-6 relative."""
     ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 24, params=[2.0, 1, 1, 1], seed=11)
     b = aa.uniform01(3, len(ro) - 1)
     say("case rbdia vs rbcsr", mode, precond)
     kw = dict(solver=aa.CG, precond=precond, mode=mode, tol=1e-8, max_iter=3000, poll_every=4)
     x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBDIA, **kw)
     x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBCSR, **kw)
-    assert r1["iterations"] == r2["iterations"] and r1["status"] == r2["status"]
-    np.testing.assert_array_equal(x1, x2)
+    say("  ", r1["iterations"], r2["iterations"], r1["true_rel_res"], r2["true_rel_res"])
+    assert r1["status"] == r2["status"]
+    if r2["status"] == 0:
+        # the reference Chebyshev bounds (power method) do not make this system converge: both
+        # layouts run to max_iter
+        assert precond == aa.PREC_CHEB and r1["iterations"] == r2["iterations"] == 3000
+        return
+    assert abs(r1["iterations"] - r2["iterations"]) <= 1
+    assert r1["true_rel_res"] < 1.05e-8 and r2["true_rel_res"] < 1.05e-8
+    assert np.linalg.norm(x1 - x2) <= 1e-6 * np.linalg.norm(x2)
 
 
 @pytest.mark.parametrize("mode", [aa.MODE_GRAPH, aa.MODE_MIXED])
 def test_learned_factor_layout_split_and_host_built_identical(ctx, mode):
-    """With A stored as RB-DIA the factors' structures are split from A's (by
-    the sign of the offset); with A as RB-CSR they are built from the CSR on the
-    host. Both give the same iterates bit for bit."""
+    """With A stored as (uniform) RB-DIA the factors' structures are split from A's (by
+    the sign of the offset, uniform too); with A as RB-CSR they are built from the CSR on the
+    host. The factors' products are equal; the uniform kernels walk bundles in pairs, so only
+    the dot products' summation order differs: iterations within 1, solutions within 1e-6."""
     ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 24, params=[2.0, 1, 1, 1], seed=11)
     b = aa.uniform01(3, len(ro) - 1)
     say("case learned factor split vs host-built", mode)
@@ -193,9 +204,10 @@ def test_learned_factor_layout_split_and_host_built_identical(ctx, mode):
               max_iter=3000, poll_every=4)
     x1, r1 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBDIA, **kw)
     x2, r2 = aa.solve_csr(ctx, ro, cols, vals, b, layout=aa.LAYOUT_RBCSR, **kw)
+    say("  ", r1["iterations"], r2["iterations"])
     assert r1["status"] == 1 and r1["fallback"] == 0
-    assert r1["iterations"] == r2["iterations"] and r1["status"] == r2["status"]
-    np.testing.assert_array_equal(x1, x2)
+    assert abs(r1["iterations"] - r2["iterations"]) <= 1 and r1["status"] == r2["status"]
+    assert np.linalg.norm(x1 - x2) <= 1e-6 * np.linalg.norm(x2)
 
 
 @pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])

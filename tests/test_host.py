@@ -296,3 +296,19 @@ def test_vblock_to_csr_layout_and_order():
     bad[3] += 1
     with pytest.raises(aa.AlienHipError):
         aa.vblock_to_csr(ro, cols, bcols, bsize, bad, vals)
+
+
+def test_host_diagonals():
+    """The distinct diagonal offsets of a CSR (the uniform RB-DIA's slots), against numpy."""
+    ro, cols, _ = aa.gen_csr(aa.GEN_POISSON7, 5, 4, 3)
+    np.testing.assert_array_equal(aa.host_diagonals(ro, cols), [-20, -5, -1, 0, 1, 5, 20])
+    rng = np.random.default_rng(3)
+    A = (sp.random(300, 300, density=0.05, random_state=rng, format="csr") + sp.identity(300)).tocsr()
+    A.sort_indices()
+    ro, cols = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    assert aa.host_diagonals(ro, cols) is None
+    rows = np.repeat(np.arange(300), np.diff(ro))
+    want = np.unique(cols.astype(np.int64) - rows)
+    np.testing.assert_array_equal(aa.host_diagonals(ro, cols, max_k=len(want)), want)
+    assert aa.host_diagonals(ro, cols, max_k=len(want) - 1) is None
+    assert len(aa.host_diagonals(np.zeros(1, np.int32), np.zeros(0, np.int32))) == 0
