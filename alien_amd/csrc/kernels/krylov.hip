@@ -27,6 +27,8 @@
 // Row kernels process RB-CSR / RB-DIA bundles, one wavefront per 64-row bundle;
 // streaming kernels handle kEPT elements per lane. Launch shapes depend on n
 // only, so the reductions are deterministic.
+#include <hip/hip_ext.h>
+
 #include "../alien_hip_internal.h"
 #include "device_common.h"
 #include "krylov_state.h"
@@ -524,10 +526,14 @@ void launch_cg_p(hipStream_t s, int n, const T* z, T* p, CGState* st, int first)
 }
 
 template <typename T>
-void launch_cg_q(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st) {
+void launch_cg_q(hipStream_t s, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st, hipEvent_t t0,
+                 hipEvent_t t1) {
   const dim3 g((unsigned)rb_blocks(n, A));
   by_layout(A, [&](auto L) {
-    hipLaunchKernelGGL((k_cg_q<T, decltype(L)::value>), g, dim3(kRowBlock), 0, s, n, A, p, q, st, w);
+    if (t0 || t1)
+      hipExtLaunchKernelGGL((k_cg_q<T, decltype(L)::value>), g, dim3(kRowBlock), 0, s, t0, t1, 0, n, A, p, q, st, w);
+    else
+      hipLaunchKernelGGL((k_cg_q<T, decltype(L)::value>), g, dim3(kRowBlock), 0, s, n, A, p, q, st, w);
   });
   launch_finalize(s, w, rb_blocks(n, A), 1, kFinPQ, nullptr, st);
 }
@@ -612,7 +618,8 @@ void launch_demote(hipStream_t s, int n, const double* a, float* b) {
                                        CGState*);                                                        \
   template void launch_factor_p<T, T>(hipStream_t, RedWork, int, RBMat<T>, RBMat<T>, const T*, T*, T*,  \
                                       CGState*, int, int);                                               \
-  template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*);          \
+  template void launch_cg_q<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*, hipEvent_t,      \
+                               hipEvent_t);                                                             \
   template void launch_spmv_dot_rb<T>(hipStream_t, RedWork, int, RBMat<T>, const T*, T*, CGState*, double*); \
   template void launch_cg_update<T>(hipStream_t, int, const T*, const T*, T*, T*, CGState*);            \
   template void launch_cg_xr<T>(hipStream_t, RedWork, int, const T*, const T*, T*, T*, const T*, T*,     \

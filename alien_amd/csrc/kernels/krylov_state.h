@@ -122,8 +122,10 @@ long launch_neumann_apply(hipStream_t, RedWork w, int n, RBMat<T> A, int order, 
                           const int* done, const T* rdot);
 template <typename T>
 void launch_cg_p(hipStream_t, int n, const T* z, T* p, CGState* st, int first);
+// (t0, t1 non-null: the launch records the kernel's start and end in them, hipExtLaunchKernel)
 template <typename T>
-void launch_cg_q(hipStream_t, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st);
+void launch_cg_q(hipStream_t, RedWork w, int n, RBMat<T> A, const T* p, T* q, CGState* st,
+                 hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 // y = A x and *out = x . y by the kernel of launch_cg_q (idle: a state that is never done)
 template <typename T>
 void launch_spmv_dot_rb(hipStream_t, RedWork w, int n, RBMat<T> A, const T* x, T* y, CGState* idle, double* out);

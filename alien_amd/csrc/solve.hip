@@ -283,6 +283,37 @@ void level_ilu_apply(hipStream_t s, const LevelILU& L, const T* y, T* t, T* x, c
 template <> ILUSplit<double> PrecondSpec::lu<double>() const { return lu64; }
 template <> ILUSplit<float> PrecondSpec::lu<float>() const { return lu32; }
 
+// Progress watch of a solve with the learned preconditioner: stalled once the residual is not
+// finite, or its norm has not halved (rr below a quarter) since the last point of progress within
+// `window` iterations. CG with an SPD M keeps reducing its error even where its residual norm
+// wanders, so only an M that does not help leaves the residual above its earlier level for a whole
+// window; a stricter rule (a tenfold drop per 400 iterations) handed healthy solves of strongly
+// anisotropic 256^3 systems to the fallback.
+struct ProgressWatch {
+  int window = 0;
+  double best = INFINITY;
+  int it_best = 0;
+  bool started = false;
+  // feed one residual (rr = r.r) at iteration it; true when stalled
+  bool stalled(double rr, int it) {
+    if (!std::isfinite(rr)) return true;
+    if (!started || rr < 0.25 * best) {
+      started = true;
+      best = rr;
+      it_best = it;
+      return false;
+    }
+    return it - it_best >= window;
+  }
+  // a true residual measured after a stall signal: progress resets the watch
+  bool confirms_progress(double rr, int it) {
+    if (!(rr < 0.25 * best)) return false;
+    best = rr;
+    it_best = it;
+    return true;
+  }
+};
+
 template <typename T>
 struct FusedCG {
   alien_hip_ctx* ctx;
@@ -317,9 +348,10 @@ struct FusedCG {
     if (h_st) (void)hipHostFree(h_st);
   }
 
-  // Step-kernel timing (ctx->step_timing): iteration slot i of an advance brackets its q = A p
-  // launch with events tev[2 i], tev[2 i + 1]; the durations of the slots that ran an iteration
-  // are added to the context's totals after the advance's state read.
+  // Step-kernel timing (ctx->step_timing): iteration slot i of an advance launches its q = A p
+  // kernel with start and end events tev[2 i], tev[2 i + 1] (hipExtLaunchKernel: the events time
+  // the kernel itself); the durations of the slots that ran an iteration are added to the
+  // context's totals after the advance's state read.
   std::vector<hipEvent_t> tev;
   void ensure_timing_events(int k) {
     while ((int)tev.size() < 2 * k) {
@@ -329,9 +361,10 @@ struct FusedCG {
     }
   }
   void timed_cg_q(hipStream_t s, int tslot) {
-    if (tslot >= 0) ALIEN_HIP_CHECK(hipEventRecord(tev[2 * tslot], s));
-    launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
-    if (tslot >= 0) ALIEN_HIP_CHECK(hipEventRecord(tev[2 * tslot + 1], s));
+    if (tslot >= 0)
+      launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data(), tev[2 * tslot], tev[2 * tslot + 1]);
+    else
+      launch_cg_q<T>(s, ctx->red(), n, A, p.data(), q.data(), st.data());
   }
   void collect_timing(int ran) {
     for (int i = 0; i < ran; ++i) {
@@ -481,8 +514,7 @@ struct FusedCG {
 
   // Solve A x = b from the given x (x_is_zero: skip the initial residual SpMV).
   // Returns the final state; bb = (b,b). With stall_window > 0 the solve also
-  // stops (*stalled = true) once the residual is not finite, or has not dropped
-  // by a factor 10 within stall_window iterations.
+  // stops (*stalled = true) when a ProgressWatch of that window says so.
   CGState solve(const T* b, T* x, bool x_is_zero, double tol, int max_it, bool graph, int poll_every,
                 double* bb_out, int stall_window = 0, bool* stalled = nullptr) {
     hipStream_t s = ctx->stream;
@@ -505,19 +537,11 @@ struct FusedCG {
     CGState cur = read_state();
     // without a graph, poll after every iteration (the reference's host-side stop test)
     const int k = graph ? std::max(1, poll_every) : 1;
-    double rr_ref = INFINITY; // the first measured residual sets the reference
-    int it_ref = cur.it;
+    ProgressWatch watch{stall_window};
     while (!cur.done) {
       cur = advance(x, graph, k);
       if (stall_window <= 0) continue;
-      if (!std::isfinite(cur.rr)) {
-        *stalled = true;
-        break;
-      }
-      if (cur.rr < 0.01 * rr_ref) {
-        rr_ref = cur.rr;
-        it_ref = cur.it;
-      } else if (cur.it - it_ref >= stall_window && !cur.done) {
+      if (!std::isfinite(cur.rr) || (watch.stalled(cur.rr, cur.it) && !cur.done)) {
         *stalled = true;
         break;
       }
@@ -566,11 +590,12 @@ void finish_breakdown(FusedCG<T>& F, CGState& s, int* error) {
 // CG continues with its search direction: no restart, so the Krylov space
 // built so far is kept. Convergence is judged on the FP64 true residual only.
 // With stall_window > 0 the solve gives up (stalled = true, x = the last
-// reliable update's solution) once the FP32 residual is not finite, or no
-// reliable update has come within stall_window iterations.
+// reliable update's solution) once the FP32 residual is not finite, or a
+// ProgressWatch of that window signals and the true residual then shows no
+// progress either.
 struct MixedResult {
   int it = 0, updates = 0, status = 0, error = 0;
-  bool stalled = false;      // learned M: no progress within the stall window (Jacobi takes over)
+  bool stalled = false;      // learned M: no progress within the stall window (ILU0 takes over)
   bool fp32_stalled = false; // the FP32 inner iteration stopped making progress (FP64 takes over)
   double rel = 0;
 };
@@ -618,9 +643,10 @@ MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMa
   check_launch("mixed CG first iteration");
   CGState cur = F.read_state();
   double rr_last = rr64;
-  int it_last = 0;
   const double d2 = delta * delta;
   const int k = graph ? std::max(1, poll_every) : 1;
+  ProgressWatch watch{stall_window};
+  watch.stalled(rr64, 0);
   for (;;) {
     if (!std::isfinite(cur.rr)) {
       // the increment is lost; x holds the last reliable update
@@ -628,7 +654,8 @@ MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMa
       else out.fp32_stalled = true;
       break;
     }
-    const bool stall = stall_window > 0 && cur.it - it_last >= stall_window;
+    // a stall signal forces a reliable update: the true residual decides
+    const bool stall = stall_window > 0 && watch.stalled(cur.rr, cur.it);
     const bool update = cur.done || cur.rr < d2 * rr_last || stall;
     if (!update) {
       cur = F.advance(x32.data(), graph, k);
@@ -649,13 +676,12 @@ MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMa
       break;
     }
     if (cur.it >= max_it || !(rr64 == rr64)) break;
-    if (stall && !(rr64 < d2 * rr_last)) {
+    if (stall && !watch.confirms_progress(rr64, cur.it)) {
       out.stalled = true;
       break;
     }
     launch_demote(s, n, r64.data(), F.r.data());
     rr_last = rr64;
-    it_last = cur.it;
     const bool inside = cur.done && ps.kind == kPrecFactor; // FP32 stop test fired at a step's start
     cur.done = 0;
     cur.status = 0;
@@ -809,19 +835,12 @@ struct FusedBiCG {
     start(rr, tol * tol * bb, 0, max_it);
     const int k = graph ? std::max(1, poll_every) : 1;
     BiCGState cur = advance(x, graph, 1);
-    double rr_ref = rr;
-    int it_ref = 0;
+    ProgressWatch watch{stall_window};
+    watch.stalled(rr, 0);
     while (!cur.done) {
       cur = advance(x, graph, k);
       if (stall_window <= 0) continue;
-      if (!std::isfinite(cur.rr)) {
-        *stalled = true;
-        break;
-      }
-      if (cur.rr < 0.01 * rr_ref) {
-        rr_ref = cur.rr;
-        it_ref = cur.it;
-      } else if (cur.it - it_ref >= stall_window && !cur.done) {
+      if (!std::isfinite(cur.rr) || (watch.stalled(cur.rr, cur.it) && !cur.done)) {
         *stalled = true;
         break;
       }
@@ -880,6 +899,8 @@ MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RB
   int it_last = 0, restarts = 0, group_max = 0;
   const double d2 = delta * delta;
   const int k = graph ? std::max(1, poll_every) : 1;
+  ProgressWatch watch{stall_window};
+  watch.stalled(rr64, 0);
   for (;;) {
     if (!std::isfinite(cur.rr) || !std::isfinite(cur.alpha) || !std::isfinite(cur.omega)) {
       // the FP32 recursion overflowed: its increment is lost, x holds the last reliable update
@@ -903,7 +924,7 @@ MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RB
       out.fp32_stalled = true;
       break;
     }
-    const bool stall = stall_window > 0 && cur.it - it_last >= stall_window;
+    const bool stall = stall_window > 0 && watch.stalled(cur.rr, cur.it);
     const bool update = cur.done || cur.rr < d2 * rr_last || stall;
     if (!update) {
       cur = F.advance(x32.data(), graph, k);
@@ -919,7 +940,7 @@ MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RB
       break;
     }
     if (cur.it >= max_it || !(rr64 == rr64)) break;
-    if (stall && !(rr64 < d2 * rr_last)) {
+    if (stall && !watch.confirms_progress(rr64, cur.it)) {
       out.stalled = true;
       break;
     }
@@ -1243,7 +1264,10 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       }
       net->release_network_buffers();
       phase("factor fill + release");
-      stall_window = p->stall_window > 0 ? p->stall_window : 400;
+      // default window: 400 iterations, growing with the grid edge N^(1/3) (CG's iteration
+      // counts on these operators do): 2048 at 256^3
+      stall_window = p->stall_window > 0 ? p->stall_window
+                                         : std::max(400, (int)(8.0 * std::cbrt((double)n)));
     }
     // ILU0 in natural order (the reference's factor and sweeps, on level-set schedules): the
     // preconditioner a stalled learned M hands over to, set up when that happens

@@ -49,10 +49,10 @@ int alien_hip_ctx_synchronize(alien_hip_ctx* ctx);
 void* alien_hip_ctx_stream(alien_hip_ctx* ctx);
 /* Instrumentation for benchmarks; no counterpart in the reference. Writes the totals
  * so far (microseconds, launches) to the non-NULL outputs, then: on = 1 clears them and
- * makes the fused CG solvers on this context (FUSED, GRAPH and MIXED modes) bracket each
- * launch of their q = A p kernel with HIP events on the context's stream and add up the
- * durations (the GRAPH and MIXED iterations are then launched directly, not replayed from
- * a captured graph); on = 0 stops. Time solves with it off. */
+ * makes the fused CG solvers on this context (FUSED, GRAPH and MIXED modes) launch their
+ * q = A p kernel with start and end events (hipExtLaunchKernel, on the context's stream) and
+ * add up its durations (the GRAPH and MIXED iterations are then launched directly, not
+ * replayed from a captured graph); on = 0 stops. Time solves with it off. */
 int alien_hip_ctx_step_timing(alien_hip_ctx* ctx, int on, double* total_us, long long* launches);
 
 /* ------------------------------------------------------------------------ */
@@ -386,10 +386,11 @@ typedef struct {
   int    max_outer;     /* unused (kept for layout) */
   int    poll_every;    /* GRAPH: iterations per graph launch / stop-flag poll */
   const float* gnn_theta; /* LEARNED: network parameters (alien_hip_gnn_nparams() floats) */
-  int    stall_window;  /* LEARNED: iterations without a tenfold drop of the residual norm
-                           (MIXED: without a reliable update) before the ILU0 fallback
-                           (natural order: the reference's factor and sweeps);
-                           0 = 400. A non-finite residual falls back at once. */
+  int    stall_window;  /* LEARNED: iterations in which the residual norm does not halve
+                           (from its level at the last such halving) before the ILU0 fallback
+                           (natural order: the reference's factor and sweeps); MIXED: the true
+                           residual of a forced reliable update must confirm the stall.
+                           0 = max(400, 8 N^(1/3)). A non-finite residual falls back at once. */
   int    filu_factor_niter; /* FILU0: nb-factor-iter (0 = exact ILU(0) factor) */
   int    filu_solver_niter; /* FILU0: nb-solver-iter (test_krylov default 3) */
   int    factor_storage;    /* LEARNED, MIXED CG: G and G^T values stored as 0 = FP32, 1 = bfloat16
