@@ -1042,12 +1042,61 @@ struct SetupTrace {
   }
 };
 
+void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A, const double* b_in, double* x_io,
+              alien_hip_solve_result* res, EventTimer& tm, SetupTrace& phase, const std::vector<int>* colours);
+
+// The learned M's fallback: multicolour ILU0 (the fused ILU0 of the colour-permuted matrix P A P^T,
+// two launches per sweep on a 7-point operator) from the iterate x, for at most max_iter
+// iterations, in FP64 (FUSED / GRAPH) or mixed. Colouring and permutation on the host from A's
+// host CSR; x (device) receives the result. The reference's natural-order ILU0 runs one launch per
+// level (766 per sweep at 256^3, ~7 ms per CG iteration) and made a fallback slower than the solve
+// it rescued (a 192^3 two-unknown BiCGStab: 56 s).
+alien_hip_solve_result ilu0_multicolour_from(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
+                                             const double* b_dev, double* x_dev, int max_iter, bool mixed) {
+  const int n = A.getNRows();
+  std::vector<int> cptr, pro;
+  HostBuf<int> pcols;
+  HostBuf<double> pvals;
+  const std::vector<int> perm = colour_permutation(n, A.kcol(), A.cols(), cptr);
+  permute_csr(n, A.kcol(), A.cols(), A.hostValues(), perm, pro, pcols, pvals);
+  std::vector<double> bh(n), xh(n), pb(n), px(n);
+  ALIEN_HIP_CHECK(hipMemcpyAsync(bh.data(), b_dev, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  ALIEN_HIP_CHECK(hipMemcpyAsync(xh.data(), x_dev, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  for (int r = 0; r < n; ++r) {
+    pb[r] = bh[perm[r]];
+    px[r] = xh[perm[r]];
+  }
+  alien_hip_solve_result r{};
+  {
+    Alien::CSRHostView view{n, pro.data(), pcols.data(), pvals.data()};
+    HIPMatrix PA(ctx, view, ALIEN_HIP_LAYOUT_CSR, /*borrow_host=*/true);
+    PA.ensureFused(p->layout);
+    alien_hip_solve_params q = *p;
+    q.precond = ALIEN_HIP_PREC_ILU0;
+    q.ilu_schedule = 1;
+    q.max_iter = max_iter;
+    q.gnn_theta = nullptr;
+    q.stall_window = 0;
+    q.mode = mixed ? ALIEN_HIP_MODE_MIXED
+                   : (p->mode == ALIEN_HIP_MODE_MIXED ? ALIEN_HIP_MODE_GRAPH : p->mode);
+    EventTimer tm2(ctx->stream);
+    tm2.start();
+    SetupTrace phase2(ctx);
+    solve_on(ctx, &q, PA, pb.data(), px.data(), &r, tm2, phase2, &cptr);
+  }
+  for (int k = 0; k < n; ++k) xh[perm[k]] = px[k];
+  ALIEN_HIP_CHECK(hipMemcpyAsync(x_dev, xh.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return r;
+}
+
 // The solve proper on a converted matrix: preconditioner set-up, Krylov loop,
 // true residual, solution copied to x_io. tm was started at the beginning of
 // the set-up; phase marks the set-up trace.
 void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A, const double* b_in, double* x_io,
               alien_hip_solve_result* res, EventTimer& tm, SetupTrace& phase,
-              const std::vector<int>* colours = nullptr) {
+              const std::vector<int>* colours) {
   const int n = A.getNRows();
   const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
   const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
@@ -1269,39 +1318,6 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       stall_window = p->stall_window > 0 ? p->stall_window
                                          : std::max(400, (int)(8.0 * std::cbrt((double)n)));
     }
-    // ILU0 in natural order (the reference's factor and sweeps, on level-set schedules): the
-    // preconditioner a stalled learned M hands over to, set up when that happens
-    std::unique_ptr<Alien::ILU0Preconditioner<Alg>> fb_ilu;
-    DeviceArray<float> fb_lu32;
-    auto ilu_fallback = [&](bool fp32) {
-      if (!fb_ilu) {
-        fb_ilu.reset(new Alien::ILU0Preconditioner<Alg>(alg, A));
-        fb_ilu->init();
-      }
-      PrecondSpec q;
-      q.kind = kPrecILU0Level;
-      LevelILU& L = q.lev;
-      L.nl = fb_ilu->lowerLevels();
-      L.nu = fb_ilu->upperLevels();
-      L.lptr = fb_ilu->lowerPtr();
-      L.uptr = fb_ilu->upperPtr();
-      L.lrows = fb_ilu->lowerRows();
-      L.urows = fb_ilu->upperRows();
-      L.lptr_dev = fb_ilu->lowerPtrDevice();
-      L.uptr_dev = fb_ilu->upperPtrDevice();
-      L.ro = A.d_ro.data();
-      L.cols = A.d_cols.data();
-      L.dcol = A.d_dcol.data();
-      L.lu64 = fb_ilu->luValues();
-      if (fp32) {
-        if (fb_lu32.empty()) {
-          fb_lu32.resize(A.getNnz());
-          launch_demote(ctx->stream, A.getNnz(), fb_ilu->luValues(), fb_lu32.data());
-        }
-        L.lu32 = fb_lu32.data();
-      }
-      return q;
-    };
     if (!mixed) {
       RBMat<double> rb = A.rb64();
       const bool graph = p->mode == ALIEN_HIP_MODE_GRAPH && capture;
@@ -1321,20 +1337,24 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
                       &stalled);
         }
         if (stalled) {
-          res->fallback = 1;
+          // multicolour ILU0 from the last finite iterate, for the remaining iterations
           if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
-          const int used = s.it;
-          FusedBiCG<double> J(ctx, n, rb, invd64, ilu_fallback(false));
-          s = J.solve(b.data(), x.data(), p->tol, std::max(0, p->max_iter - used), false, p->poll_every,
-                      nullptr);
-          s.it += used;
+          const alien_hip_solve_result r2 =
+              ilu0_multicolour_from(ctx, p, A, b.data(), x.data(), std::max(0, p->max_iter - s.it), false);
+          res->solve_s = tm.stop();
+          res->fallback = 1 | r2.fallback;
+          res->iterations = s.it + r2.iterations;
+          res->status = r2.status;
+          res->error = r2.error;
+          res->value = r2.value;
+        } else {
+          res->solve_s = tm.stop();
+          res->iterations = s.it;
+          res->status = s.status;
+          res->error = s.breakdown ? 1 : 0; // NullValueException
+          double bb = alg.dot(b, b);
+          res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
         }
-        res->solve_s = tm.stop();
-        res->iterations = s.it;
-        res->status = s.status;
-        res->error = s.breakdown ? 1 : 0; // NullValueException
-        double bb = alg.dot(b, b);
-        res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
         res->true_rel_res = true_residual(alg, A, b, x);
         x.copyValuesTo(x_io);
         ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1348,21 +1368,23 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
         if (!stalled) finish_breakdown(F, s, &res->error);
       }
       if (stalled) {
-        // ILU0 from the last finite iterate, for the remaining iterations
-        res->fallback = 1;
+        // multicolour ILU0 from the last finite iterate, for the remaining iterations
         if (!std::isfinite(s.rr)) launch_copy<double>(ctx->stream, n, x0.data(), x.data());
-        const int used = s.it;
-        FusedCG<double> J(ctx, n, rb, invd64, ilu_fallback(false));
-        s = J.solve(b.data(), x.data(), false, p->tol, std::max(0, p->max_iter - used), false, p->poll_every,
-                    nullptr);
-        finish_breakdown(J, s, &res->error);
-        s.it += used;
+        const alien_hip_solve_result r2 =
+            ilu0_multicolour_from(ctx, p, A, b.data(), x.data(), std::max(0, p->max_iter - s.it), false);
+        res->solve_s = tm.stop();
+        res->fallback = 1 | r2.fallback;
+        res->iterations = s.it + r2.iterations;
+        res->status = r2.status;
+        res->error = r2.error;
+        res->value = r2.value;
+      } else {
+        res->solve_s = tm.stop();
+        res->iterations = s.it;
+        res->status = s.status;
+        double bb = alg.dot(b, b);
+        res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
       }
-      res->solve_s = tm.stop();
-      res->iterations = s.it;
-      res->status = s.status;
-      double bb = alg.dot(b, b);
-      res->value = bb > 0 ? std::sqrt(s.rr) / std::sqrt(bb) : 0.0;
     } else {
       // FP32 matrix, preconditioner and Krylov vectors, FP64 dots; FP64 defect correction
       A.ensureRB32();
@@ -1378,13 +1400,16 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       MixedResult m = run(ctx, n, rb64, rb32, invd32.data(), ps, b.data(), x.data(), p->tol, p->max_iter, delta,
                           p->poll_every, capture, stall_window);
       if (m.stalled) {
-        // ILU0 from the last reliable update, for the remaining iterations
-        res->fallback = 1;
-        const int used = m.it, ups = m.updates;
-        m = run(ctx, n, rb64, rb32, invd32.data(), ilu_fallback(true), b.data(), x.data(), p->tol,
-                std::max(0, p->max_iter - used), delta, p->poll_every, false, 0);
-        m.it += used;
-        m.updates += ups;
+        // mixed multicolour ILU0 from the last reliable update, for the remaining iterations
+        const alien_hip_solve_result r2 =
+            ilu0_multicolour_from(ctx, p, A, b.data(), x.data(), std::max(0, p->max_iter - m.it), true);
+        res->fallback = 1 | r2.fallback;
+        m.it += r2.iterations;
+        m.updates += r2.outer;
+        m.status = r2.status;
+        m.error = r2.error;
+        m.rel = r2.value;
+        m.stalled = false;
       }
       if (m.fp32_stalled) {
         // FP64 from the last reliable update, for the remaining iterations, with the FP64 form of
@@ -1403,10 +1428,16 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
           filu_fp64(p64);
         } else if (ps.kind == kPrecNeumann || ps.kind == kPrecCheb || ps.kind == kPrecILU0Level) {
           p64 = ps; // the FP64 solver applies them to A, invd and the FP64 factor
-        } else if (ps.kind == kPrecFactor) {
-          p64 = ilu_fallback(false); // the learned M's fallback
         }
-        if (bicg) {
+        if (ps.kind == kPrecFactor) {
+          // the learned M's fallback in FP64: multicolour ILU0
+          const alien_hip_solve_result r2 = ilu0_multicolour_from(ctx, p, A, b.data(), x.data(), left, false);
+          res->fallback |= 1 | r2.fallback;
+          m.it = used + r2.iterations;
+          m.status = r2.status;
+          m.error = r2.error;
+          m.rel = r2.value;
+        } else if (bicg) {
           FusedBiCG<double> F(ctx, n, rb64, invd64, p64);
           const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, capture && p64.kind != kPrecILU0Level,
                                       p->poll_every, nullptr);

@@ -388,8 +388,9 @@ typedef struct {
   const float* gnn_theta; /* LEARNED: network parameters (alien_hip_gnn_nparams() floats) */
   int    stall_window;  /* LEARNED: iterations in which the residual norm does not halve
                            (from its level at the last such halving) before the ILU0 fallback
-                           (natural order: the reference's factor and sweeps); MIXED: the true
-                           residual of a forced reliable update must confirm the stall.
+                           (multicolour ILU0 of the colour-permuted matrix, from the current
+                           iterate, in the solve's mode); MIXED: the true residual of a forced
+                           reliable update must confirm the stall.
                            0 = max(400, 8 N^(1/3)). A non-finite residual falls back at once. */
   int    filu_factor_niter; /* FILU0: nb-factor-iter (0 = exact ILU(0) factor) */
   int    filu_solver_niter; /* FILU0: nb-solver-iter (test_krylov default 3) */

@@ -375,10 +375,9 @@ def broken_theta(which):
 @pytest.mark.parametrize("mode", [aa.MODE_FUSED, aa.MODE_MIXED])
 @pytest.mark.parametrize("which", ["overflow", "stagnation"])
 def test_fallback_to_ilu0(ctx, mode, which):
-    """A learned M that overflows or stagnates hands the solve over to ILU0 (natural order,
-    the reference's factor) from the last good iterate: it converges, on the FP64 true
-    residual, within the stall window plus about an ILU0 solve's iterations (Jacobi would
-    need several times more)."""
+    """A learned M that overflows or stagnates hands the solve over to multicolour ILU0 from
+    the last good iterate: it converges, on the FP64 true residual, within the stall window
+    plus about a multicolour ILU0 solve's iterations (Jacobi would need several times more)."""
     n = 12
     ro, cols, vals = system(n)
     b = aa.uniform01(42, n ** 3)
@@ -386,10 +385,10 @@ def test_fallback_to_ilu0(ctx, mode, which):
     x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_LEARNED, mode=mode, tol=1e-8,
                         max_iter=3000, gnn_theta=broken_theta(which), stall_window=50, poll_every=4)
     _, ri = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED, tol=1e-8,
-                         max_iter=3000)
+                         max_iter=3000, ilu_schedule=1)
     _, rj = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_FUSED, tol=1e-8,
                          max_iter=3000)
-    say("  ", r, "ilu0 from zero", ri["iterations"], "jacobi from zero", rj["iterations"])
+    say("  ", r, "multicolour ilu0 from zero", ri["iterations"], "jacobi from zero", rj["iterations"])
     assert r["fallback"] == 1
     assert r["status"] == 1
     assert np.all(np.isfinite(x))
@@ -401,15 +400,16 @@ def test_fallback_to_ilu0(ctx, mode, which):
 def test_learned_bicgstab_on_two_unknown_system_hands_over_to_ilu0(ctx, mode):
     """The nonsymmetric 2-unknown system (interleaved 2x2 blocks, the block generator's): the SPD
     M = G G^T fits it worse than Jacobi, its BiCGStab residual stalls and the solve hands over to
-    ILU0 from the last good iterate (fallback bit 1), reaching the FP64 tolerance 1e-10 on the true
-    residual within the stall window plus about an ILU0 solve's iterations."""
+    multicolour ILU0 from the last good iterate (fallback bit 1), reaching the FP64 tolerance 1e-10
+    on the true residual within the stall window plus about a multicolour ILU0 solve's
+    iterations."""
     ro, cols, vals = bsr2_scalar_csr(8)
     b = aa.uniform01(42, len(ro) - 1)
     say("learned BiCGStab 2-unknown 8^3", mode)
     x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_LEARNED, mode=mode, tol=1e-10,
                         max_iter=4000, gnn_theta=go.init_params(0), stall_window=40, poll_every=4)
     _, ri = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_ILU0, mode=aa.MODE_FUSED,
-                         tol=1e-10, max_iter=4000)
+                         tol=1e-10, max_iter=4000, ilu_schedule=1)
     say("  ", r, "ilu0 from zero", ri["iterations"])
     assert ri["status"] == 1
     assert r["fallback"] & 1

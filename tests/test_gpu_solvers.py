@@ -572,8 +572,8 @@ def test_learned_bicgstab_on_two_unknown_system(ctx, mode):
     """BiCGStab with the learned M = G G^T on the 2-unknown block system, to an
     FP64 relative residual of 1e-10. M is SPD by construction and fits this
     nonsymmetric operator worse than Jacobi: the solve either gets there with it or,
-    when the residual stops halving within the stall window, hands over to ILU0
-    (fallback bit 1); the true residual meets the tolerance either way."""
+    when the residual stops halving within the stall window, hands over to multicolour
+    ILU0 (fallback bit 1); the true residual meets the tolerance either way."""
     from oracle import gnn_oracle as go
     ro, cols, vals = bsr2_scalar_csr(10)
     n = len(ro) - 1
