@@ -83,7 +83,7 @@ class DeviceArray {
 struct alien_hip_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  bool trace = false; // ALIEN_HIP_TRACE=1 at creation: set-up phase times on stderr
+  bool trace = false; // ALIEN_HIP_TRACE=1 at creation: set-up phase times and mixed BiCGStab updates on stderr
   // Step-kernel timing (alien_hip_ctx_step_timing): when on, the fused CG solvers bracket each
   // launch of their q = A p kernel with HIP events on `stream` and add up the durations here
   bool step_timing = false;

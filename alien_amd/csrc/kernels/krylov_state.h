@@ -70,6 +70,7 @@ struct BiCGState {
   double rr;     // last (r,r) or (s,s) of a stop test (Iteration's m_value)
   double crit;   // tol^2 (b,b)
   double rr0;    // (r,r) of the initial residual (first-block breakdown rule)
+  double rrmax;  // largest (r,r) or (s,s) of a stop test since the last (re)start
   int it, max_it;
   int done;      // stop test passed, max_it reached or breakdown
   int status;    // Iteration::getStatus()

@@ -112,6 +112,7 @@ __global__ void __launch_bounds__(kFin) k_finalize_bicg(RedWork w, int nblk, int
       st->half = 1;
     } else {
       st->rr = tot[0];
+      st->rrmax = fmax(st->rrmax, tot[0]);
       if (tot[0] < st->crit) {
         st->status = 1;
         st->done = 1;
@@ -144,6 +145,7 @@ __global__ void __launch_bounds__(kFin) k_finalize_bicg(RedWork w, int nblk, int
       st->done = 1;
     } else {
       st->rr = tot[0];
+      st->rrmax = fmax(st->rrmax, tot[0]);
       st->status = (tot[0] < st->crit) ? 1 : 0;
       st->done = st->status;
       if (!st->done && st->rho == 0) { // NullValueException("rho") at the next step (:240)

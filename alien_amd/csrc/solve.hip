@@ -600,13 +600,24 @@ struct MixedResult {
   double rel = 0;
 };
 
-// A BiCGStab group (iterations between reliable updates) longer than
-// max(kMinGroup, kGroupFactor x the longest group so far) is taken as a
-// stagnated FP32 inner iteration: FP32 storage of A carries a relative error
-// of ~6e-8 per entry, and when that times the condition number nears 1 the
-// inner iteration stops reducing the true residual.
+// Mixed BiCGStab's FP32 stagnation rule: the FP64 true residual, checked at
+// each reliable update, must fall to a quarter of its best value within
+// max(kMinGroup, kGroupFactor x the longest such interval so far) iterations.
+// FP32 storage of A carries a relative error of ~6e-8 per entry, and when that
+// times the condition number nears 1 the inner iteration stops reducing the
+// true residual: FP64 then finishes the solve.
 constexpr int kMinGroup = 250;
 constexpr int kGroupFactor = 3;
+// squared-norm thresholds of mixed BiCGStab's restart rule
+constexpr double kRestartGap = 2.25;
+constexpr double kRestartPeak = 100.0;
+// A true residual above kRiseFactor x the lowest so far (squared norms: 100x in
+// norm) at a reliable update means the FP32 recurrence diverged (its rounding
+// at a residual peak broke the Bi-CG relations): the iteration resumes from the
+// best iterate, at most kMaxRises times before FP64 takes over from it (with the
+// learned M, which may not suit a nonsymmetric operator, ILU0 takes over at once).
+constexpr double kRiseFactor = 1e4;
+constexpr int kMaxRises = 1;
 
 MixedResult mixed_reliable_cg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMat<float> A32, const float* invd32,
                               const PrecondSpec& ps, const double* b, double* x, double tol, int max_it,
@@ -810,6 +821,7 @@ struct FusedBiCG {
     h.rho1 = rr;
     h.rr = rr;
     h.rr0 = rr;
+    h.rrmax = rr;
     h.crit = crit;
     h.it = it;
     h.max_it = max_it;
@@ -852,12 +864,19 @@ struct FusedBiCG {
 
 // Mixed-precision BiCGStab with reliable updates (Sleijpen & van der Vorst 1996,
 // the reference for the scheme): the FP32 recurrence updates an increment
-// x32; when the recursive residual has dropped by delta since the last update,
+// x32; when the recursive residual has dropped by delta below the largest
+// residual since the last update (their M(r) rule: a peak of the erratic
+// BiCGStab residual leaves an FP32 rounding error of ~6e-8 x the peak in the
+// recursive residual, so it must be replaced soon after the peak has passed),
 // or the FP32 solve stops, x32 is folded into x64 and r = b - A x64 (FP64)
-// replaces the FP32 residual (rho1 = r.r0 recomputed); the shadow residual r0
-// and the search direction are kept. After a stop on s, a breakdown or an FP32
-// stop that the true residual does not confirm, the recurrence restarts from
-// the true residual. Convergence is judged on the FP64 true residual only.
+// replaces the FP32 residual. The shadow residual r0 and the search direction
+// are kept (rho1 = r.r0 recomputed) unless the replaced residual differs from
+// the recursive one by more than kRestartGap, or the group's peak exceeded
+// kRestartPeak x the last true residual (squared norms): the Bi-CG relations
+// are then too far off and the recurrence restarts from the true residual
+// (r0 = r, first block), as it does after a stop on s, a breakdown or an FP32
+// stop that the true residual does not confirm. Convergence is judged on the
+// FP64 true residual only.
 MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RBMat<float> A32,
                                 const float* invd32, const PrecondSpec& ps, const double* b, double* x,
                                 double tol, int max_it, double delta, int poll_every, bool graph,
@@ -896,36 +915,40 @@ MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RB
   restart(0);
   BiCGState cur = F.advance(x32.data(), graph, 1);
   double rr_last = rr64;
-  int it_last = 0, restarts = 0, group_max = 0;
+  int restarts = 0, it_best = 0, gap_max = 0, rises = 0;
+  double rr_best = rr64;
+  // the iterate of the lowest true residual so far: hand-overs and divergence restarts resume from it
+  double rr_min = rr64;
+  DeviceArray<double> x_min(n);
+  launch_copy<double>(s, n, x, x_min.data());
+  auto resume_min = [&]() {
+    launch_copy<double>(s, n, x_min.data(), x);
+    rr64 = true_res();
+    out.rel = std::sqrt(rr64 / bb);
+  };
   const double d2 = delta * delta;
   const int k = graph ? std::max(1, poll_every) : 1;
   ProgressWatch watch{stall_window};
   watch.stalled(rr64, 0);
   for (;;) {
     if (!std::isfinite(cur.rr) || !std::isfinite(cur.alpha) || !std::isfinite(cur.omega)) {
-      // the FP32 recursion overflowed: its increment is lost, x holds the last reliable update
+      // the FP32 recursion overflowed: its increment is lost
+      resume_min();
       if (stall_window > 0) out.stalled = true;
       else out.fp32_stalled = true;
       break;
     }
-    if (stall_window == 0 && !cur.done && cur.it - it_last >= std::max(kMinGroup, kGroupFactor * group_max) &&
+    if (stall_window == 0 && !cur.done && cur.it - it_best >= std::max(kMinGroup, kGroupFactor * gap_max) &&
         cur.it < max_it) {
-      // stagnated: keep the increment only if it lowers the true residual
-      DeviceArray<double> xb(n);
-      launch_copy<double>(s, n, x, xb.data());
+      // stagnated: keep the increment only if it lowers the true residual below the best
       launch_fold_increment(s, n, x32.data(), x);
-      const double rrc = true_res();
-      if (rrc < rr_last) {
-        ++out.updates;
-        out.rel = std::sqrt(rrc / bb);
-      } else {
-        launch_copy<double>(s, n, xb.data(), x);
-      }
+      if (true_res() < rr_min) ++out.updates;
+      else resume_min();
       out.fp32_stalled = true;
       break;
     }
     const bool stall = stall_window > 0 && watch.stalled(cur.rr, cur.it);
-    const bool update = cur.done || cur.rr < d2 * rr_last || stall;
+    const bool update = cur.done || cur.rr < d2 * std::max(rr_last, cur.rrmax) || stall;
     if (!update) {
       cur = F.advance(x32.data(), graph, k);
       continue;
@@ -941,19 +964,52 @@ MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RB
     }
     if (cur.it >= max_it || !(rr64 == rr64)) break;
     if (stall && !watch.confirms_progress(rr64, cur.it)) {
+      if (rr64 > rr_min) resume_min();
       out.stalled = true;
       break;
     }
+    if (rr64 < rr_min) {
+      rr_min = rr64;
+      launch_copy<double>(s, n, x, x_min.data());
+    } else if (rr64 > kRiseFactor * rr_min) {
+      // the FP32 recurrence diverged from its best iterate: resume from that iterate, or hand over
+      // from it (to FP64 after kMaxRises such restarts; the learned M's to ILU0 at once)
+      if (ctx->trace)
+        std::fprintf(stderr, "[alien_hip] mixed BiCGStab it %d: true %.3e, %.3e at the best iterate: %s\n", cur.it,
+                     std::sqrt(rr64 / bb), std::sqrt(rr_min / bb),
+                     stall_window > 0 ? "ILU0" : rises < kMaxRises ? "resume" : "FP64");
+      resume_min();
+      if (stall_window > 0) {
+        out.stalled = true;
+        break;
+      }
+      if (++rises > kMaxRises) {
+        out.fp32_stalled = true;
+        break;
+      }
+      rr_last = rr64;
+      restart(cur.it);
+      cur = F.advance(x32.data(), graph, k);
+      continue;
+    }
+    const bool far = rr64 > kRestartGap * cur.rr || cur.rrmax > kRestartPeak * rr_last;
+    if (ctx->trace)
+      std::fprintf(stderr, "[alien_hip] mixed BiCGStab it %d: true %.3e recursive %.3e peak %.3e%s\n", cur.it,
+                   std::sqrt(rr64 / bb), std::sqrt(cur.rr / bb), std::sqrt(cur.rrmax / bb),
+                   cur.done || far ? " restart" : "");
     rr_last = rr64;
-    group_max = std::max(group_max, cur.it - it_last);
-    it_last = cur.it;
-    if (cur.done) { // stop on s, FP32 stop not confirmed, or breakdown: restart from the true residual
+    if (rr64 < 0.25 * rr_best) {
+      gap_max = std::max(gap_max, cur.it - it_best);
+      rr_best = rr64;
+      it_best = cur.it;
+    }
+    if (cur.done || far) { // stop on s, FP32 stop not confirmed, breakdown, or Bi-CG relations lost
       if (cur.breakdown && ++restarts > 10) {
         out.error = 1;
         break;
       }
       restart(cur.it);
-      cur = F.advance(x32.data(), graph, 1);
+      cur = F.advance(x32.data(), graph, k);
       continue;
     }
     // reliable update: r <- b - A x64, rho1 <- r.r0
@@ -962,6 +1018,7 @@ MixedResult mixed_reliable_bicg(alien_hip_ctx* ctx, int n, RBMat<double> A64, RB
     launch_dot<float>(s, ctx->red(), n, F.r.data(), F.r0.data(), ctx->dslot(sl));
     cur.rho1 = ctx->read_slot(sl);
     cur.rr = rr64;
+    cur.rrmax = rr64;
     F.write_state(cur);
     cur = F.advance(x32.data(), graph, k);
   }

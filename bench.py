@@ -456,10 +456,12 @@ def run_convdiff(a):
         x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, tol=1e-8, max_iter=5000, poll_every=8, **kw)
         err = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
         log(f"{name}: solve {r['solve_s']:.3f}s setup {r['setup_s']:.3f}s its={r['iterations']} "
-            f"status={r['status']} true_rel_res={r['true_rel_res']:.3e}")
+            f"status={r['status']} fallback={r['fallback']} true_rel_res={r['true_rel_res']:.3e}")
         res[name] = {"time_s": r["solve_s"], "setup_s": r["setup_s"], "iterations": r["iterations"],
                      "converged": bool(r["status"] == 1), "true_rel_res": r["true_rel_res"],
                      "rel_err_vs_x_true": err}
+        if kw["mode"] == aa.MODE_MIXED:
+            res[name]["fallback"] = r["fallback"]
     sp = time_spmv(aa, torch, ctx, ro, cols, vals, a.spmv_reps)
     spmv = {k: {"us": sp[k] * 1e6, "gbs": nb / sp[k] / 1e9} for k, nb in
             (("csr_fp64", spmv_bytes_fp64(N, nnz)), ("rbcsr_fp64", spmv_bytes_fp64(N, nnz)),
@@ -615,7 +617,7 @@ def run_blocks2(a):
         res[name] = {"time_s": r["solve_s"], "setup_s": r["setup_s"], "iterations": r["iterations"],
                      "converged": bool(r["status"] == 1 and r["true_rel_res"] < 1.05e-10),
                      "true_rel_res": r["true_rel_res"], "rel_err_vs_x_true": err}
-        if kw["precond"] == aa.PREC_LEARNED:
+        if kw["mode"] == aa.MODE_MIXED:
             res[name]["fallback"] = r["fallback"]
     sp = time_spmv(aa, torch, ctx, ro, cols, vals, a.spmv_reps)
     spmv = {k: {"us": sp[k] * 1e6, "gbs": nbytes / sp[k] / 1e9} for k, nbytes in

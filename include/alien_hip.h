@@ -381,7 +381,8 @@ typedef struct {
   int    poly_max_iter; /* --poly-factor-max-iter */
   int    ilu_schedule;  /* 0 = level sets (natural order, bit-exact), 1 = multicolour */
   double inner_tol;     /* MIXED: reliable-update threshold delta: the FP64 true residual
-                           replaces the FP32 one once it has dropped by delta (0 = 0.1) */
+                           replaces the FP32 one once it has dropped by delta (CG: below the
+                           last update's; BiCGStab: below the largest since then) (0 = 0.1) */
   int    inner_max_iter;/* unused (kept for layout) */
   int    max_outer;     /* unused (kept for layout) */
   int    poll_every;    /* GRAPH: iterations per graph launch / stop-flag poll */
@@ -411,7 +412,8 @@ typedef struct {
   double solve_s;       /* the Krylov solve, hipEvent-timed */
   int    fallback;      /* bit 0 (LEARNED): the learned M stalled and ILU0 took over;
                            bit 1 (MIXED): the FP32 inner iteration overflowed, or (BiCGStab)
-                           stagnated, and the solve finished in FP64 from the last reliable x */
+                           diverged or stagnated, and the solve finished in FP64 from the last
+                           reliable x (BiCGStab: the iterate of the lowest true residual) */
   int    pad_;
 } alien_hip_solve_result;
 

@@ -107,7 +107,7 @@ def test_two_unknown_24_bicgstab_jacobi(ctx):
     operator amplifies rounding: the oracle's own counts with 1, 2, 3, 4 and 8 row blocks, which
     change only the summation order of its dot products, span 264-284. The fused and graph
     counts must lie in that span widened by 5% each side; the mixed solver reaches 1e-10 on
-    the true residual."""
+    the true residual in FP32 (no FP64 hand-over)."""
     ro, cols, vals = bsr2_scalar_csr(24)
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
@@ -124,5 +124,6 @@ def test_two_unknown_24_bicgstab_jacobi(ctx):
                             max_iter=5000, poll_every=4)
         say("  gpu", r["iterations"], r["fallback"], r["true_rel_res"], "oracle", its)
         assert r["status"] == 1 and r["true_rel_res"] < 1.05e-10
+        assert r["fallback"] == 0
         if mode != aa.MODE_MIXED:
             assert lo <= r["iterations"] <= hi

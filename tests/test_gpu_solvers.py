@@ -312,24 +312,44 @@ def test_mixed_bicgstab_reaches_fp64_tolerance(ctx, case):
 
 
 def test_mixed_bicgstab_hands_over_to_fp64_when_fp32_stagnates(ctx):
-    """2-unknown block system with sigma 3 on 20^3 cells, to 1e-12: stored in
-    FP32 the operator is too ill-conditioned for the FP32 inner iteration to keep
-    reducing the true residual (a CPU emulation of the mixed solver overflows
-    after 4 reliable updates). The solver detects the stagnation or the overflow, keeps
-    the last reliable x and finishes in FP64 (fallback bit 2); the true residual
-    meets the tolerance."""
-    ro, cols, vals = bsr2_scalar_csr(20, sigma=3.0)
+    """Log-normal diffusion with sigma 4 on 16^3 cells, to 1e-10 (a CPU emulation of FP64
+    Jacobi-BiCGStab: 519 iterations): stored in FP32 the operator is too ill-conditioned for the
+    FP32 inner iteration to keep reducing the true residual (the emulated mixed solver's true
+    residual stays above 1e-5). The solver detects the stagnation or an overflow, keeps the last
+    reliable x and finishes in FP64 (fallback bit 2); the true residual meets the tolerance."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 16, params=[4.0, 1, 1, 1], seed=11)
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
     say("case mixed BiCGStab FP32 stagnation")
-    kw = dict(solver=aa.BICGS, precond=aa.PREC_DIAG, tol=1e-12, max_iter=8000, poll_every=4)
+    kw = dict(solver=aa.BICGS, precond=aa.PREC_DIAG, tol=1e-10, max_iter=8000, poll_every=4)
     x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_MIXED, **kw)
     _, r64 = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_GRAPH, **kw)
     say("  mixed", r["iterations"], r["outer"], r["fallback"], r["true_rel_res"], "fp64", r64["iterations"])
     assert r64["status"] == 1
     assert r["status"] == 1 and r["fallback"] == 2
-    assert r["true_rel_res"] < 2e-12
+    assert r["true_rel_res"] < 1.05e-10
     assert np.all(np.isfinite(x))
+
+
+def test_mixed_bicgstab_fp32_holds_through_residual_peaks(ctx):
+    """2-unknown block system with sigma 3 on 20^3 cells, to 1e-12. The BiCGStab residual peaks
+    at up to 1e3-1e4 times its last reliable value here; replacing the residual only after a
+    tenfold drop below the last update let the FP32 rounding of those peaks (6e-8 x the peak)
+    overtake the true residual, and the FP32 stage stagnated. With the update taken a tenfold
+    drop below the group's peak, and a restart when the replaced residual is far from the
+    recursive one, the FP32 stage reaches 1e-12 without the FP64 hand-over (fallback 0), in at
+    most twice FP64's iterations."""
+    ro, cols, vals = bsr2_scalar_csr(20, sigma=3.0)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    say("case mixed BiCGStab residual peaks")
+    kw = dict(solver=aa.BICGS, precond=aa.PREC_DIAG, tol=1e-12, max_iter=8000, poll_every=4)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_MIXED, **kw)
+    _, r64 = aa.solve_csr(ctx, ro, cols, vals, b, mode=aa.MODE_GRAPH, **kw)
+    say("  mixed", r["iterations"], r["outer"], r["fallback"], r["true_rel_res"], "fp64", r64["iterations"])
+    assert r64["status"] == 1
+    assert r["status"] == 1 and r["fallback"] == 0 and r["true_rel_res"] < 1.05e-12
+    assert r["iterations"] <= 2 * r64["iterations"]
 
 
 def poisson2d(k):
