@@ -64,14 +64,43 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
     alien_hip::group_by_level(n, level.data(), m_nu, m_uptr.data(), rows.data());
     d_urows.upload(rows.data(), rows.size(), st);
     factorize(st);
+    buildLevelSweeps(st);
   }
   void update() {}
   void solve(AlgebraType& alg, const VectorType& y, VectorType& x) const {
-    alien_hip::launch_ilu0_solve<double>(alg.stream(), m_nl, m_lptr.data(), d_lrows.data(), m_nu, m_uptr.data(),
-                                         d_urows.data(), m_A.d_ro.data(), m_A.d_cols.data(), m_A.d_dcol.data(),
-                                         d_lu.data(), y.data(), const_cast<double*>(m_t.data()), x.data(), nullptr,
-                                         d_lptr.data(), d_uptr.data());
+    if (levelOrdered())
+      alien_hip::launch_ilu0_solve_lev<double>(alg.stream(), m_nl, m_lptr.data(), d_lptr.data(), m_lsw, m_nu,
+                                               m_uptr.data(), d_uptr.data(), m_usw, y.data(),
+                                               const_cast<double*>(m_t.data()), x.data());
+    else
+      alien_hip::launch_ilu0_solve<double>(alg.stream(), m_nl, m_lptr.data(), d_lrows.data(), m_nu, m_uptr.data(),
+                                           d_urows.data(), m_A.d_ro.data(), m_A.d_cols.data(), m_A.d_dcol.data(),
+                                           d_lu.data(), y.data(), const_cast<double*>(m_t.data()), x.data(),
+                                           nullptr, d_lptr.data(), d_uptr.data());
     alien_hip::check_launch("ilu0_solve");
+  }
+  // Natural order: the factor copied in level order (LevelSweep) for both sweeps, built when
+  // every strictly-lower and strictly-upper row length is at most kLevelMaxSlots.
+  bool levelOrdered() const { return m_lsw.k > 0 && m_usw.k > 0; }
+  const alien_hip::LevelSweep<double>& lowerSweep() const { return m_lsw; }
+  const alien_hip::LevelSweep<double>& upperSweep() const { return m_usw; }
+  // The same sweeps with FP32 values (columns shared), filled from the factor into val / diag
+  void fillSweepsF32(hipStream_t st, alien_hip::DeviceArray<float>& lval, alien_hip::DeviceArray<float>& uval,
+                     alien_hip::DeviceArray<float>& udiag, alien_hip::LevelSweep<float>& L,
+                     alien_hip::LevelSweep<float>& U) const {
+    const int n = m_A.getNRows();
+    lval.resize((size_t)m_lsw.k * n);
+    uval.resize((size_t)m_usw.k * n);
+    udiag.resize(n);
+    alien_hip::launch_level_sweep_fill<float>(st, n, m_lsw.k, false, d_lrows.data(), m_A.d_ro.data(),
+                                              m_A.d_cols.data(), m_A.d_dcol.data(), d_lu.data(), nullptr,
+                                              lval.data(), nullptr);
+    alien_hip::launch_level_sweep_fill<float>(st, n, m_usw.k, true, d_urows.data(), m_A.d_ro.data(),
+                                              m_A.d_cols.data(), m_A.d_dcol.data(), d_lu.data(), nullptr,
+                                              uval.data(), udiag.data());
+    alien_hip::check_launch("ilu0 level sweeps (FP32)");
+    L = {n, m_lsw.k, m_lsw.row, m_lsw.col, lval.data(), nullptr};
+    U = {n, m_usw.k, m_usw.row, m_usw.col, uval.data(), udiag.data()};
   }
   void solve(const VectorType& y, VectorType& x) const { solve(m_alg, y, x); }
 
@@ -100,8 +129,31 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
     m_alg.allocate(AlgebraType::resource(m_A), m_t);
     ALIEN_HIP_CHECK(hipStreamSynchronize(st));
   }
+  void buildLevelSweeps(hipStream_t st) {
+    const int n = m_A.getNRows();
+    const int kl = alien_hip::level_sweep_slots(n, m_A.kcol(), m_A.dcol(), false);
+    const int ku = alien_hip::level_sweep_slots(n, m_A.kcol(), m_A.dcol(), true);
+    if (kl == 0 || ku == 0) return; // the CSR-order kernels
+    d_lcol.resize((size_t)kl * n);
+    d_lval.resize((size_t)kl * n);
+    d_ucol.resize((size_t)ku * n);
+    d_uval.resize((size_t)ku * n);
+    d_udiag.resize(n);
+    alien_hip::launch_level_sweep_fill<double>(st, n, kl, false, d_lrows.data(), m_A.d_ro.data(), m_A.d_cols.data(),
+                                               m_A.d_dcol.data(), d_lu.data(), d_lcol.data(), d_lval.data(), nullptr);
+    alien_hip::launch_level_sweep_fill<double>(st, n, ku, true, d_urows.data(), m_A.d_ro.data(), m_A.d_cols.data(),
+                                               m_A.d_dcol.data(), d_lu.data(), d_ucol.data(), d_uval.data(),
+                                               d_udiag.data());
+    alien_hip::check_launch("ilu0 level sweeps");
+    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+    m_lsw = {n, kl, d_lrows.data(), d_lcol.data(), d_lval.data(), nullptr};
+    m_usw = {n, ku, d_urows.data(), d_ucol.data(), d_uval.data(), d_udiag.data()};
+  }
   AlgebraType& m_alg;
   const MatrixType& m_A;
+  alien_hip::LevelSweep<double> m_lsw, m_usw;
+  alien_hip::DeviceArray<int> d_lcol, d_ucol;
+  alien_hip::DeviceArray<double> d_lval, d_uval, d_udiag;
   int m_nl = 0, m_nu = 0;
   std::vector<int> m_lptr, m_uptr, m_colours;
   alien_hip::DeviceArray<int> d_lrows, d_urows, d_lptr, d_uptr;

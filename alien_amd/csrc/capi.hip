@@ -127,6 +127,7 @@ int alien_hip_ctx_create(void* stream, alien_hip_ctx** out) {
       const long v = std::strtol(e, nullptr, 10);
       if (v >= kXcds) *caps[i] = v / kXcds * kXcds;
     }
+  if (const char* e = std::getenv("ALIEN_HIP_ILU_WG_ROWS")) g_ilu_wg_rows = std::max(0L, std::strtol(e, nullptr, 10));
   *out = make_ctx(stream);
   if (const char* e = std::getenv("ALIEN_HIP_TRACE")) (*out)->trace = std::atoi(e) != 0;
   API_END

@@ -139,6 +139,39 @@ int launch_filu0_split_apply(hipStream_t, int n, int niter, ILUSplit<T> F, const
 template <typename T>
 int launch_ilu0_split_apply(hipStream_t, int ncolours, const int* cptr_host, ILUSplit<T> F, const T* y, T* t, T* x,
                             const int* done, const T* rdot, RedWork red);
+// One sweep of the natural-order ILU(0) solve in level order: position q of the
+// concatenated level sets holds factor row row[q]; its strictly-lower (lower
+// sweep) or strictly-upper (upper sweep) entries, in ascending column order,
+// sit in slots s = 0..k-1 at [s * n + q] (column -1: no entry); the upper
+// sweep also keeps u_ii per position. A level's per-row data is then read
+// contiguously; only the solution vector is gathered.
+template <typename T>
+struct LevelSweep {
+  int n = 0, k = 0;          // rows, slots (1..kLevelMaxSlots)
+  const int* row = nullptr;  // n
+  const int* col = nullptr;  // k * n
+  const T* val = nullptr;    // k * n
+  const T* diag = nullptr;   // upper sweep: n
+};
+constexpr int kLevelMaxSlots = 8;
+// levels of at most this many rows run in one workgroup (runs of two or more);
+// ALIEN_HIP_ILU_WG_ROWS, read when a context is created, overrides it
+extern long g_ilu_wg_rows;
+// Slot count of a sweep (the largest strictly-lower or strictly-upper row
+// length), from the host CSR; 0 when it exceeds kLevelMaxSlots.
+int level_sweep_slots(int n, const int* ro_host, const int* dcol_host, bool upper);
+// Fill one sweep's columns (when col is not null), values and diagonal from the
+// factor lu in CSR order (device arrays; row: the level order).
+template <typename T>
+void launch_level_sweep_fill(hipStream_t, int n, int k, bool upper, const int* row, const int* ro, const int* cols,
+                             const int* dcol, const double* lu, int* col, T* val, T* diag);
+// ILU(0) solve z = (LU)^-1 y over the level-ordered sweeps (bit-exact with
+// launch_ilu0_solve): runs of small levels in one workgroup, larger levels one
+// launch each. ptr_host / ptr_dev: the level pointers.
+template <typename T>
+void launch_ilu0_solve_lev(hipStream_t, int nl, const int* lptr_host, const int* lptr_dev, LevelSweep<T> L, int nu,
+                           const int* uptr_host, const int* uptr_dev, LevelSweep<T> U, const T* y, T* t, T* x,
+                           const int* done = nullptr);
 template <typename T>
 void launch_ilu0_solve(hipStream_t, int nl, const int* lptr_host, const int* lrows, int nu,
                        const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,

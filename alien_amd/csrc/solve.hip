@@ -181,6 +181,10 @@ struct LevelILU {
   const int* dcol = nullptr;
   const double* lu64 = nullptr;
   const float* lu32 = nullptr;
+  // the factor in level order (both sweeps), when the preconditioner built it
+  bool ordered = false;
+  LevelSweep<double> l64{}, u64{};
+  LevelSweep<float> l32{}, u32{};
 };
 struct PrecondSpec {
   PrecKind kind = kPrecDiag;
@@ -274,6 +278,14 @@ template <> RBMat<float> PrecondSpec::g<float>() const { return g32; }
 template <> RBMat<float> PrecondSpec::gt<float>() const { return gt32; }
 template <typename T>
 void level_ilu_apply(hipStream_t s, const LevelILU& L, const T* y, T* t, T* x, const int* done) {
+  if (L.ordered) {
+    if constexpr (std::is_same<T, float>::value)
+      launch_ilu0_solve_lev<float>(s, L.nl, L.lptr, L.lptr_dev, L.l32, L.nu, L.uptr, L.uptr_dev, L.u32, y, t, x, done);
+    else
+      launch_ilu0_solve_lev<double>(s, L.nl, L.lptr, L.lptr_dev, L.l64, L.nu, L.uptr, L.uptr_dev, L.u64, y, t, x,
+                                    done);
+    return;
+  }
   const T* lu;
   if constexpr (std::is_same<T, float>::value) lu = L.lu32;
   else lu = L.lu64;
@@ -1246,7 +1258,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
     auto ilu_fp64 = [&](PrecondSpec& q) {
       q.lu64 = lus.fill<double>(ctx->stream, A, ilu->luValues(), lus.lbv64, lus.ubv64, lus.d64);
     };
-    DeviceArray<float> lev_lu32;
+    DeviceArray<float> lev_lu32, lev_l32, lev_u32, lev_d32;
     if (p->precond == ALIEN_HIP_PREC_ILU0 && !(colours && !colours->empty())) {
       // natural order: the level-set factor and sweeps of the REFERENCE mode
       ilu.reset(new Alien::ILU0Preconditioner<Alg>(alg, A));
@@ -1266,7 +1278,14 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       L.cols = A.d_cols.data();
       L.dcol = A.d_dcol.data();
       L.lu64 = ilu->luValues();
-      if (mixed) {
+      L.ordered = ilu->levelOrdered();
+      if (L.ordered) {
+        L.l64 = ilu->lowerSweep();
+        L.u64 = ilu->upperSweep();
+      }
+      if (mixed && L.ordered) {
+        ilu->fillSweepsF32(ctx->stream, lev_l32, lev_u32, lev_d32, L.l32, L.u32);
+      } else if (mixed) {
         lev_lu32.resize(A.getNnz());
         launch_demote(ctx->stream, A.getNnz(), ilu->luValues(), lev_lu32.data());
         L.lu32 = lev_lu32.data();

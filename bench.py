@@ -451,6 +451,9 @@ def run_convdiff(a):
                                           filu_solver_niter=3)),
             ("fp64_bicgstab_jacobi_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_DIAG)),
             ("mixed_bicgstab_jacobi", dict(mode=aa.MODE_MIXED, precond=aa.PREC_DIAG))]
+    if a.solvers:
+        keep = set(a.solvers.split(","))
+        runs = [r for r in runs if r[0] in keep]
     res = {}
     for name, kw in runs:
         x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, tol=1e-8, max_iter=5000, poll_every=8, **kw)
