@@ -783,6 +783,9 @@ struct FusedBiCG {
     if (!diag) precond(p.data(), phat.data());
     launch_bicg_v<T>(s, ctx->red(), n, A, phat.data(), r0.data(), v.data(), st.data());
     launch_bicg_s<T>(s, ctx->red(), n, r.data(), v.data(), sv.data(), dinv, shat.data(), st.data());
+    // level-set ILU0 (never captured in a graph): a stop on s skips the second apply's hundreds of
+    // per-level launches, which would otherwise each be launched to return at once
+    if (P.kind == kPrecILU0Level && read_state().done) return;
     if (!diag) precond(sv.data(), shat.data());
     launch_bicg_t<T>(s, ctx->red(), n, A, shat.data(), sv.data(), t.data(), st.data());
     launch_bicg_xr<T>(s, ctx->red(), n, shat.data(), phat.data(), sv.data(), t.data(), r0.data(), x, r.data(),
