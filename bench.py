@@ -357,7 +357,7 @@ def train_network(aa, ctx, a):
     th0 = aa.gnn_init(0)
     t0 = time.time()
     # phase 1: Kaporin's bound (its minimiser over the pattern is the factorised sparse approximate
-    # inverse); phase 2: the residual reduction after 16 PCG steps, the method's own objective
+    # inverse); phase 2: the residual reduction after a few PCG steps, the method's own objective
     th1, hist = aa.gnn_train(ctx, th0, steps=a.train_steps, steps_per_system=1, n_min=8, n_max=16,
                              sigma_min=0.5, sigma_max=3.0, aniso_max=100.0, lr=3e-3, beta=0.0, seed=7, cg_steps=0)
     th1, hist2 = aa.gnn_train(ctx, th1, steps=a.train_cg_phase, steps_per_system=1, n_min=12, n_max=20,
@@ -777,9 +777,12 @@ def main():
     ap.add_argument("--seeds", type=int, default=20, help="eval / train workloads: number of unseen systems")
     ap.add_argument("--systems", type=int, default=2000, help="train workload: number of training systems")
     ap.add_argument("--train-steps", type=int, default=300)
-    ap.add_argument("--train-cg-steps", type=int, default=16,
-                    help="second training phase: residual reduction after this many PCG steps")
-    ap.add_argument("--train-cg-phase", type=int, default=90, help="Adam steps of the second training phase")
+    ap.add_argument("--train-cg-steps", type=int, default=None,
+                    help="second training phase: residual reduction after this many PCG steps (default 40 for "
+                         "the lognormal line, 16 for the eval and train workloads)")
+    ap.add_argument("--train-cg-phase", type=int, default=None,
+                    help="Adam steps of the second training phase (default 600 for the lognormal line, 90 for "
+                         "the eval and train workloads)")
     ap.add_argument("--eval-dist", default="c3", choices=["c3", "iso"],
                     help="eval / train workloads: unseen systems of the training distribution (c3: sigma U[0.5,3], "
                          "anisotropy log-U[1,100]) or isotropic sigma-2 systems (iso)")
@@ -802,6 +805,13 @@ def main():
     a = ap.parse_args()
     if a.n is None:
         a.n = {"lognormal": 256, "convdiff": 128, "blocks2": 192, "eval": 128, "train": 128}[a.workload]
+    # the headline line's second training phase: 600 Adam steps on the residual reduction after 40 PCG
+    # steps (measured at 256^3: 785 iterations, against 812 after 90 steps on 16; the eval and train
+    # workloads keep the 90 on 16 their recorded results used)
+    if a.train_cg_phase is None:
+        a.train_cg_phase = 600 if a.workload == "lognormal" else 90
+    if a.train_cg_steps is None:
+        a.train_cg_steps = 40 if a.workload == "lognormal" else 16
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported in this build "
               f"(the row-block sharding over several GPUs is a design, not an implementation)", file=sys.stderr)
