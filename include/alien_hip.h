@@ -391,7 +391,9 @@ typedef struct {
                            (from its level at the last such halving) before the ILU0 fallback
                            (multicolour ILU0 of the colour-permuted matrix, from the current
                            iterate, in the solve's mode); MIXED: the true residual of a forced
-                           reliable update must confirm the stall.
+                           reliable update must confirm the stall (mixed BiCGStab also falls
+                           back when the true residual rises 100x above its lowest, from the
+                           iterate of the lowest one).
                            0 = max(400, 8 N^(1/3)). A non-finite residual falls back at once. */
   int    filu_factor_niter; /* FILU0: nb-factor-iter (0 = exact ILU(0) factor) */
   int    filu_solver_niter; /* FILU0: nb-solver-iter (test_krylov default 3) */
