@@ -2,7 +2,8 @@
 (64^3 cells, 24^3 for the two-unknown system), GPU against the oracle on the same inputs:
 
 * log-normal diffusion, CG with Jacobi in the fused, graph and mixed solvers;
-* upwind convection-diffusion, BiCGStab with the reference's natural-order ILU0 (level sets);
+* upwind convection-diffusion, BiCGStab with the reference's natural-order ILU0 (level sets), also
+  at its full 128^3 size;
 * an unseen log-normal system with the learned M (FP64 against the oracle's PCG with the same
   factor, mixed to the FP64 tolerance);
 * the two-unknown system, BiCGStab with Jacobi to 1e-10.
@@ -61,6 +62,28 @@ def test_convdiff_64_bicgstab_level_set_ilu0(ctx, mode):
     assert r["status"] == o["status"] == 1
     assert r["true_rel_res"] < 1.05e-8
     assert abs(r["iterations"] - o["iterations"]) <= 2
+
+
+def test_convdiff_128_full_size_level_set_ilu0(ctx):
+    """The nonsymmetric configuration at its full size (128^3 upwind convection-diffusion,
+    N = 2,097,152, 382 level sets per sweep): BiCGStab with the reference's natural-order ILU0 to
+    1e-8 in the REFERENCE, FUSED and MIXED modes, against the oracle's sequential ILU0-BiCGStab on
+    the same system: REFERENCE and FUSED iterations within 1 of the oracle's (the factor and the
+    sweeps are the oracle's bit for bit; only dot-product order differs), MIXED to the FP64
+    tolerance on the true residual."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 128)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    _, o = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-8, max_iter=200)
+    assert o["status"] == 1
+    for mode in (aa.MODE_REFERENCE, aa.MODE_FUSED, aa.MODE_MIXED):
+        say("case conv-diff 128^3 level-set ILU0 BiCGStab", mode)
+        _, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_ILU0, mode=mode, tol=1e-8,
+                            ilu_schedule=0, max_iter=200)
+        say("  gpu", r["iterations"], r["true_rel_res"], "oracle", o["iterations"])
+        assert r["status"] == 1 and r["true_rel_res"] < 1.05e-8
+        if mode != aa.MODE_MIXED:
+            assert abs(r["iterations"] - o["iterations"]) <= 1
 
 
 def test_learned_64_unseen_system(ctx):
