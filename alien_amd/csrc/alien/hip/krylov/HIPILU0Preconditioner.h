@@ -51,6 +51,7 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
       }
       d_urows.upload(rows.data(), rows.size(), st);
       factorize(st);
+      buildLevelSweeps(st); // the colour classes as the levels
       return;
     }
     std::vector<int> level(n);
@@ -79,8 +80,9 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
                                            nullptr, d_lptr.data(), d_uptr.data());
     alien_hip::check_launch("ilu0_solve");
   }
-  // Natural order: the factor copied in level order (LevelSweep) for both sweeps, built when
-  // every strictly-lower and strictly-upper row length is at most kLevelMaxSlots.
+  // The factor copied in schedule order (LevelSweep: level sets, or colour classes) for both
+  // sweeps, built when every strictly-lower and strictly-upper row length is at most
+  // kLevelMaxSlots.
   bool levelOrdered() const { return m_lsw.k > 0 && m_usw.k > 0; }
   const alien_hip::LevelSweep<double>& lowerSweep() const { return m_lsw; }
   const alien_hip::LevelSweep<double>& upperSweep() const { return m_usw; }
