@@ -360,9 +360,9 @@ def train_network(aa, ctx, a):
     # inverse); phase 2: the residual reduction after a few PCG steps, the method's own objective
     th1, hist = aa.gnn_train(ctx, th0, steps=a.train_steps, steps_per_system=1, n_min=8, n_max=16,
                              sigma_min=0.5, sigma_max=3.0, aniso_max=100.0, lr=3e-3, beta=0.0, seed=7, cg_steps=0)
-    th1, hist2 = aa.gnn_train(ctx, th1, steps=a.train_cg_phase, steps_per_system=1, n_min=12, n_max=20,
-                              sigma_min=0.5, sigma_max=3.0, aniso_max=100.0, lr=1e-3, beta=0.0, seed=8,
-                              cg_steps=a.train_cg_steps)
+    th1, hist2 = aa.gnn_train(ctx, th1, steps=a.train_cg_phase, steps_per_system=1, n_min=a.train_cg_nmin,
+                              n_max=a.train_cg_nmax, sigma_min=0.5, sigma_max=3.0, aniso_max=100.0,
+                              lr=a.train_cg_lr, beta=0.0, seed=8, cg_steps=a.train_cg_steps)
     t_train = time.time() - t0
     rng = np.random.default_rng(99991)
     its = {"init": [], "trained": [], "jacobi": []}
@@ -783,6 +783,9 @@ def main():
     ap.add_argument("--train-cg-phase", type=int, default=None,
                     help="Adam steps of the second training phase (default 600 for the lognormal line, 90 for "
                          "the eval and train workloads)")
+    ap.add_argument("--train-cg-nmin", type=int, default=12, help="second training phase: smallest grid edge")
+    ap.add_argument("--train-cg-nmax", type=int, default=20, help="second training phase: largest grid edge")
+    ap.add_argument("--train-cg-lr", type=float, default=1e-3, help="second training phase: Adam learning rate")
     ap.add_argument("--eval-dist", default="c3", choices=["c3", "iso"],
                     help="eval / train workloads: unseen systems of the training distribution (c3: sigma U[0.5,3], "
                          "anisotropy log-U[1,100]) or isotropic sigma-2 systems (iso)")
