@@ -826,6 +826,42 @@ def test_levelset_ilu0_iterates_equal_oracle(ctx, case):
         np.testing.assert_allclose(x, xo, rtol=0, atol=1e-11 * np.abs(xo).max())
 
 
+def laplace27(k):
+    """27-point Laplacian on a k^3 grid (26 on the diagonal, -1 for each of the up to 26
+    neighbours), CSR with ascending columns: 13 strictly-lower and 13 strictly-upper entries per
+    interior row."""
+    ro, cols, vals = [0], [], []
+    for i in range(k ** 3):
+        z, rem = divmod(i, k * k)
+        y, x = divmod(rem, k)
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    X, Y, Z = x + dx, y + dy, z + dz
+                    if 0 <= X < k and 0 <= Y < k and 0 <= Z < k:
+                        cols.append(X + k * (Y + k * Z))
+                        vals.append(26.0 if (dx, dy, dz) == (0, 0, 0) else -1.0)
+        ro.append(len(cols))
+    return np.array(ro, np.int32), np.array(cols, np.int32), np.array(vals)
+
+
+@pytest.mark.parametrize("mode", [aa.MODE_REFERENCE, aa.MODE_FUSED])
+def test_levelset_ilu0_wide_rows_iterates_equal_oracle(ctx, mode):
+    """Rows with more than 8 strictly-lower or strictly-upper entries (the 27-point Laplacian:
+    13 each) keep the CSR-order level sweeps instead of the level-ordered factor copy: three CG
+    iterations with natural-order ILU(0) give the oracle's iterates to rounding of the dots."""
+    ro, cols, vals = laplace27(12)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(11, n))
+    xo, _ = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_ILU0, tol=1e-15, max_iter=3)
+    say("case level-set ILU0 27-point", mode)
+    x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.CG, precond=aa.PREC_ILU0, ilu_schedule=0, mode=mode,
+                        layout=aa.LAYOUT_CSR, tol=1e-15, max_iter=3)
+    say("  ", np.abs(x - xo).max())
+    assert r["iterations"] == 3
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-11 * np.abs(xo).max())
+
+
 def test_levelset_ilu0_level_runs_bit_identical(ctx, monkeypatch):
     """The level-ordered natural-order ILU(0) sweeps give the same solve bit for bit whether each
     level runs as its own launch (ALIEN_HIP_ILU_WG_ROWS=0), every run of levels in one workgroup
