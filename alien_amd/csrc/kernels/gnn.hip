@@ -29,12 +29,14 @@ namespace gnn {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// float -> bfloat16 bits, round to nearest even (NaN stays NaN)
+// float -> bfloat16 bits, round to nearest even (NaN stays NaN, quieted). Branch-free: both
+// results are formed and one selected (v_cndmask), where an early return for NaN made every
+// conversion a divergent branch and left the dense layers' operand conversion instruction-bound.
 __device__ __forceinline__ short to_bf16(float x) {
-  unsigned u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+  const unsigned u = __float_as_uint(x);
+  const unsigned rounded = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+  const unsigned quiet = (u >> 16) | 0x40u;
+  return (short)((u & 0x7fffffffu) > 0x7f800000u ? quiet : rounded);
 }
 __device__ __forceinline__ float bf16_round(float x) {
   return __uint_as_float(((unsigned)(unsigned short)to_bf16(x)) << 16);
@@ -82,6 +84,13 @@ __global__ void k_prep_feat(int n, const int* __restrict__ ro, const int* __rest
 // One wavefront computes 32 rows x 32 outputs: KIN/8 mfma 32x32x8 steps.
 // Lane l holds A[row = l%32][k = 4(l/32)..+3], B[k][col = l%32], and the result
 // D[8 (r/4) + 4 (l/32) + r%4][l%32] in accumulator register r.
+//
+// Message-passing layers (KIN = 64: X0 = h, X1 = m, both n x 32, Hres = h): each wave first copies
+// its 32 rows of h and m into LDS with 16-byte loads (a contiguous 4 KB block each), then reads
+// the MFMA operands and the residual term from there: h is read from memory once, not twice, and
+// the row-strided 4-byte operand loads become coalesced 16-byte ones. Same values, same MFMA
+// sequence: the results are those of the direct form below.
+constexpr int kDenseLdsPitch = 2 * kHidden + 1; // one pad float: the 32 rows of a column hit 32 banks
 template <int KIN>
 __global__ void __launch_bounds__(256) k_dense(int n, const float* __restrict__ X0, int ld0, int c0,
                                                const float* __restrict__ X1, int ld1, int kin,
@@ -91,10 +100,66 @@ __global__ void __launch_bounds__(256) k_dense(int n, const float* __restrict__ 
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
   const int r0 = wave * 32;
-  if (r0 >= n) return;
-  const int arow = r0 + (lane & 31);
   const int kb = lane >> 5;
   const int col = lane & 31;
+  if constexpr (KIN == 2 * kHidden) {
+    __shared__ float xs[4][32][kDenseLdsPitch];
+    const bool staged = ld0 == kHidden && ld1 == kHidden && c0 == kHidden && kin == KIN && (!Hres || Hres == X0);
+    if (staged) { // uniform over the grid
+      float(*x)[kDenseLdsPitch] = xs[threadIdx.x >> 6];
+      // the lane's weights for every step, loaded beside the activations (one round trip for both,
+      // where loading them inside the step loop cost a round trip per step)
+      float wv[KIN / 8][4];
+#pragma unroll
+      for (int ks = 0; ks < KIN / 8; ++ks)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) wv[ks][v] = W[(ks * 8 + kb * 4 + v) * kHidden + col];
+      const float bias = b[col];
+#pragma unroll
+      for (int part = 0; part < 2; ++part) {
+        const float* src = part ? X1 : X0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int f = j * 64 + lane; // float4 index in the wave's 32 x 32 block
+          const int rr = f >> 3, c4 = (f & 7) * 4;
+          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (r0 + rr < n) v = *reinterpret_cast<const float4*>(src + (long)(r0 + rr) * kHidden + c4);
+          float* d = &x[rr][part * kHidden + c4];
+          d[0] = v.x;
+          d[1] = v.y;
+          d[2] = v.z;
+          d[3] = v.w;
+        }
+      }
+      __syncthreads(); // every wave reaches it (no early return above)
+      if (r0 >= n) return;
+      f32x16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < KIN / 8; ++ks) {
+        bf16x4 a, bb;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          a[v] = to_bf16(x[lane & 31][ks * 8 + kb * 4 + v]);
+          bb[v] = to_bf16(wv[ks][v]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bb, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int rl = 8 * (r >> 2) + 4 * kb + (r & 3);
+        const int row = r0 + rl;
+        if (row < n) {
+          const float z = acc[r] + bias;
+          const long o = (long)row * kHidden + col;
+          if (Z) Z[o] = z;
+          Hout[o] = (Hres ? x[rl][col] : 0.f) + fmaxf(z, 0.f);
+        }
+      }
+      return;
+    }
+  }
+  if (r0 >= n) return;
+  const int arow = r0 + (lane & 31);
   f32x16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < KIN / 8; ++ks) {
