@@ -80,8 +80,10 @@ inline long bundle_blocks_max(long n) {
 struct BundleWalk {
   int first, end, stride;
 };
-// Walk over nb units (bundles, or pairs of bundles) of a row kernel's grid.
-__device__ __forceinline__ BundleWalk bundle_walk_units(int nb) {
+// Walk over nb units (bundles, or pairs of bundles) of a row kernel's grid. `chunk`: the XCD
+// chunk of the one-unit-per-wave mapping below, used when the grid is a multiple of 8 chunks
+// (kXcdChunk otherwise: every grid is rounded to that).
+__device__ __forceinline__ BundleWalk bundle_walk_units(int nb, int chunk = (int)kXcdChunk) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
   BundleWalk w;
   if ((long)gridDim.x * kRowWaves >= nb) {
@@ -90,9 +92,9 @@ __device__ __forceinline__ BundleWalk bundle_walk_units(int nb) {
     // C consecutive workgroups (the y-neighbour rows of a bundle mostly fall in
     // its own chunk, so in its XCD's L2), while the workgroups in flight still
     // cover one contiguous window of 8C. C = 64 measured best at 256^3 (8..128 swept).
+    const int C = (gridDim.x % (unsigned)(kXcds * chunk) == 0) ? chunk : (int)kXcdChunk;
     const int p = blockIdx.x, i = p / kXcds;
-    const int logical = (i / (int)kXcdChunk) * (int)(kXcds * kXcdChunk) + (p % kXcds) * (int)kXcdChunk +
-                        i % (int)kXcdChunk;
+    const int logical = (i / C) * (kXcds * C) + (p % kXcds) * C + i % C;
     w.first = logical * kRowWaves + wave;
     w.end = min(w.first + 1, nb);
     w.stride = 1;

@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(kRowBlock) k_cg_q(int n, RBMat<T> A, const T* 
       // bundles in pairs (uni_sum2)
       paired = true;
       const int nb = (n + kWave - 1) / kWave;
-      const BundleWalk bw = bundle_walk_units((nb + 1) / 2);
+      const BundleWalk bw = bundle_walk_units((nb + 1) / 2, kUniQChunk);
       for (int u = bw.first; u < bw.end; u += bw.stride) {
         const int b0 = 2 * u, b1 = min(2 * u + 1, nb - 1);
         const int row0 = b0 * kBundle + (threadIdx.x & (kWave - 1)), row1 = row0 + kBundle;

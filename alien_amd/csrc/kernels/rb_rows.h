@@ -213,6 +213,14 @@ __device__ __forceinline__ T uni_sum(const RBMat<T, V>& A, int b, XS xs) {
 #define ALIEN_HIP_UNI_PAIR_MAX_SLOTS 8
 #endif
 constexpr int kUniPairMaxSlots = ALIEN_HIP_UNI_PAIR_MAX_SLOTS;
+// XCD chunk of the CG step's pair walk (one pair per wave on its 32,768 workgroups at 256^3): with
+// 1,024-workgroup chunks each XCD's share is a contiguous run of z-planes and p is fetched into its
+// L2 1.03 times instead of 3 (measured at 256^3: FETCH_SIZE 270,151 against 328,006 KiB; the time
+// is the same, the Infinity Cache served the refetches)
+#ifndef ALIEN_HIP_UNI_Q_CHUNK
+#define ALIEN_HIP_UNI_Q_CHUNK 1024
+#endif
+constexpr int kUniQChunk = ALIEN_HIP_UNI_Q_CHUNK;
 
 // Two bundles of a uniform RB-DIA at once (b0 and b1, b1 may repeat b0): both bundles' value
 // loads and gathers are issued before any product, twice the memory-level parallelism of a wave
