@@ -70,6 +70,11 @@ class Engine {
   Engine(alien_hip_ctx* ctx, const SystemView& v);
   int n() const { return m_n; }
   int nnz() const { return m_nnz; }
+  alien_hip_ctx* context() const { return m_ctx; }
+  // A kept engine for a system with the same structure (new values, arrays possibly moved):
+  // re-reads the view, re-runs the checks of the values and re-allocates the per-system buffers;
+  // the factors' structures built earlier are kept
+  void reset_system(const SystemView& v) { init(v); }
   // activations of every layer and Ghat (per CSR entry; upper entries 0) for
   // the parameters P (device), as loss_grad needs them
   void forward(const float* P);

@@ -1115,7 +1115,8 @@ struct SetupTrace {
 };
 
 void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A, const double* b_in, double* x_io,
-              alien_hip_solve_result* res, EventTimer& tm, SetupTrace& phase, const std::vector<int>* colours);
+              alien_hip_solve_result* res, EventTimer& tm, SetupTrace& phase, const std::vector<int>* colours,
+              std::unique_ptr<gnn::Engine>* kept_engine);
 
 // The learned M's fallback: multicolour ILU0 (the fused ILU0 of the colour-permuted matrix P A P^T,
 // two launches per sweep on a 7-point operator) from the iterate x, for at most max_iter
@@ -1155,7 +1156,7 @@ alien_hip_solve_result ilu0_multicolour_from(alien_hip_ctx* ctx, const alien_hip
     EventTimer tm2(ctx->stream);
     tm2.start();
     SetupTrace phase2(ctx);
-    solve_on(ctx, &q, PA, pb.data(), px.data(), &r, tm2, phase2, &cptr);
+    solve_on(ctx, &q, PA, pb.data(), px.data(), &r, tm2, phase2, &cptr, nullptr);
   }
   for (int k = 0; k < n; ++k) xh[perm[k]] = px[k];
   ALIEN_HIP_CHECK(hipMemcpyAsync(x_dev, xh.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
@@ -1165,10 +1166,12 @@ alien_hip_solve_result ilu0_multicolour_from(alien_hip_ctx* ctx, const alien_hip
 
 // The solve proper on a converted matrix: preconditioner set-up, Krylov loop,
 // true residual, solution copied to x_io. tm was started at the beginning of
-// the set-up; phase marks the set-up trace.
+// the set-up; phase marks the set-up trace. kept_engine (a kept matrix's): the
+// learned preconditioner's network engine, reused by later solves on the same
+// structure (its factor layouts are built once) and created there when empty.
 void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A, const double* b_in, double* x_io,
               alien_hip_solve_result* res, EventTimer& tm, SetupTrace& phase,
-              const std::vector<int>* colours) {
+              const std::vector<int>* colours, std::unique_ptr<gnn::Engine>* kept_engine = nullptr) {
   const int n = A.getNRows();
   const bool fused = p->mode == ALIEN_HIP_MODE_FUSED || p->mode == ALIEN_HIP_MODE_GRAPH;
   const bool mixed = p->mode == ALIEN_HIP_MODE_MIXED;
@@ -1249,7 +1252,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
     }
     // learned preconditioner: network inference on A (read in place), then G
     // and G^T row-bundled in the solver's precision
-    std::unique_ptr<gnn::Engine> net;
+    std::unique_ptr<gnn::Engine> net_own; // the engine of a one-shot solve
     DeviceArray<float> theta, gv32, gtv32;
     DeviceArray<double> gv64, gtv64;
     DeviceArray<bf16_bits> gv16, gtv16;
@@ -1361,9 +1364,18 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
         v.d_rb_dmask = A.d_dmask.data();
       }
       phase("Jacobi set-up");
-      net.reset(new gnn::Engine(ctx, v));
+      gnn::Engine* eng = nullptr;
+      if (kept_engine && *kept_engine && (*kept_engine)->context() == ctx && (*kept_engine)->n() == n) {
+        eng = kept_engine->get();
+        eng->reset_system(v);
+      } else {
+        net_own.reset(new gnn::Engine(ctx, v));
+        eng = net_own.get();
+        if (kept_engine) *kept_engine = std::move(net_own); // owned by the kept matrix from here
+      }
       theta.upload(p->gnn_theta, gnn::kNParam, ctx->stream);
       phase("network engine");
+      gnn::Engine* net = eng;
       net->infer(theta.data());
       phase("network inference");
       net->build_factor_layout();
@@ -1614,6 +1626,8 @@ struct alien_hip_matrix {
   alien_hip_ctx* ctx = nullptr;
   alien_hip::HIPMatrix A;
   bool diag_ok = false; // every row stores its diagonal entry
+  // the learned preconditioner's network engine (factor layouts of A's structure), kept between solves
+  std::unique_ptr<alien_hip::gnn::Engine> engine;
   // multicolour ILU0: the colour-permuted matrix P A P^T, built by the first
   // such solve and kept with A (its values gathered from A's on each update)
   std::unique_ptr<alien_hip::HIPMatrix> PA;
@@ -1731,7 +1745,7 @@ extern "C" int alien_hip_solve(alien_hip_ctx* ctx, const alien_hip_solve_params*
     if (fused || mixed) A.ensureFused(p->layout);
     else A.setSpmvLayout(p->layout);
     phase("row-bundled layout");
-    solve_on(ctx, p, A, b_in, x_io, res, tm, phase, coloured ? &m->cptr : nullptr);
+    solve_on(ctx, p, A, b_in, x_io, res, tm, phase, coloured ? &m->cptr : nullptr, coloured ? nullptr : &m->engine);
     if (coloured)
       for (int r = 0; r < n; ++r) xh[m->perm[r]] = px[r];
     ALIEN_HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -951,6 +951,38 @@ def test_kept_matrix_solves_equal_solve_csr(ctx, name, kw):
             np.testing.assert_array_equal(x1, x2)
 
 
+def test_kept_matrix_learned_engine_reuse(ctx):
+    """The learned preconditioner's engine (its factor layouts of the structure) is kept by the
+    matrix between solves: a later solve re-checks the new values (a non-positive diagonal is
+    refused with EINVAL, as on a first solve), a solve through another context builds its own
+    engine, and every solve equals alien_hip_solve_csr on the same values bit for bit."""
+    from oracle import gnn_oracle as go
+    ro, cols, vals = aa.gen_csr(aa.GEN_LOGNORMAL7, 14, params=[1.5, 1, 1, 1], seed=6)
+    n = len(ro) - 1
+    b = aa.uniform01(8, n)
+    kw = dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, gnn_theta=go.init_params(0), tol=1e-8,
+              max_iter=2000)
+    bad = vals.copy()
+    diag = [k for k in range(ro[5], ro[6]) if cols[k] == 5][0]
+    bad[diag] = -1.0
+    say("case kept matrix learned engine reuse")
+    with aa.Context() as other:
+        with aa.Matrix(ctx, ro, cols, vals) as M:
+            x1, r1 = M.solve(b, **kw)
+            M.update_values(bad)
+            with pytest.raises(RuntimeError, match="diagonal"):
+                M.solve(b, **kw)
+            M.update_values(vals)
+            x2, r2 = M.solve(b, **kw)
+            x3, r3 = M.solve(b, ctx_override=other, **kw)
+            x4, r4 = M.solve(b, **kw)
+        x0, r0 = aa.solve_csr(ctx, ro, cols, vals, b, **kw)
+    say("  ", r0["iterations"], r1["iterations"], r2["iterations"], r3["iterations"], r4["iterations"])
+    for x, r in ((x1, r1), (x2, r2), (x3, r3), (x4, r4)):
+        assert r["status"] == 1 and r["iterations"] == r0["iterations"]
+        np.testing.assert_array_equal(x, x0)
+
+
 @pytest.mark.parametrize("mode", [aa.MODE_MIXED, aa.MODE_GRAPH], ids=["mixed", "graph"])
 def test_kept_matrix_solved_through_a_second_context(ctx, mode):
     """A matrix created on one context and first solved through another: the row-bundled
