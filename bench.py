@@ -329,7 +329,7 @@ def cpu_baseline(ro, cols, vals, b, n):
         log("cpu_baseline unavailable:", e)
         return None
     nproc = os.cpu_count() or 1
-    threads = min(16, nproc)  # the GPU box's CPU share for one GPU
+    threads = min(16, nproc)  # the host threads available to one GPU
     _, r = orc.solve(ro, cols, vals, b, solver=orc.ORC_CG, precond=orc.PREC_DIAG, tol=1e-8, max_iter=20000,
                      nthreads=threads)
     log(f"cpu baseline: {r['iterations']} its, solve {r['solve_s']:.1f}s on {threads} threads, "
