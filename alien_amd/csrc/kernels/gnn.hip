@@ -493,6 +493,66 @@ __global__ void k_dense_bwd(int n, const float* __restrict__ dh, const float* __
   }
 }
 
+// The same for the message-passing layers (kin = 64) on MFMA: du = dz W^T is a 32-row x 64 x 32
+// product per wave, 2 output tiles of 16 v_mfma_f32_32x32x2_f32 steps (FP32 operands: dz and the
+// bf16-rounded weights, as in the scalar form; only the summation order of the 32 terms differs).
+// Each wave stages its 32 rows of dh and dz in LDS (16-byte loads), the workgroup the weights.
+__global__ void __launch_bounds__(256) k_dense_bwd_mfma(int n, const float* __restrict__ dh,
+                                                        const float* __restrict__ Z, const float* __restrict__ W,
+                                                        float* __restrict__ dz, float* __restrict__ dh_prev,
+                                                        float* __restrict__ dmsg) {
+  __shared__ float ds[4][32][kHidden + 1];
+  __shared__ float hs[4][32][kHidden + 1];
+  __shared__ float ws[2 * kHidden][kHidden + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r0 = ((blockIdx.x * 256 + threadIdx.x) >> 6) * 32;
+  for (int t = threadIdx.x; t < 2 * kHidden * kHidden; t += 256) ws[t / kHidden][t % kHidden] = bf16_round(W[t]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int f = j * 64 + lane; // float4 index in the wave's 32 x 32 block
+    const int rr = f >> 3, c4 = (f & 7) * 4;
+    float4 h = make_float4(0.f, 0.f, 0.f, 0.f), z = make_float4(0.f, 0.f, 0.f, 0.f);
+    const long o = (long)(r0 + rr) * kHidden + c4;
+    if (r0 + rr < n) {
+      h = *reinterpret_cast<const float4*>(dh + o);
+      z = *reinterpret_cast<const float4*>(Z + o);
+    }
+    const float4 d = make_float4(z.x > 0.f ? h.x : 0.f, z.y > 0.f ? h.y : 0.f, z.z > 0.f ? h.z : 0.f,
+                                 z.w > 0.f ? h.w : 0.f);
+    if (r0 + rr < n) *reinterpret_cast<float4*>(dz + o) = d;
+    hs[w][rr][c4] = h.x;
+    hs[w][rr][c4 + 1] = h.y;
+    hs[w][rr][c4 + 2] = h.z;
+    hs[w][rr][c4 + 3] = h.w;
+    ds[w][rr][c4] = d.x;
+    ds[w][rr][c4 + 1] = d.y;
+    ds[w][rr][c4 + 2] = d.z;
+    ds[w][rr][c4 + 3] = d.w;
+  }
+  __syncthreads(); // every wave reaches it
+  if (r0 >= n) return;
+  const int half = lane >> 5, col = lane & 31;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    f32x16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int qs = 0; qs < kHidden / 2; ++qs) {
+      const int q = 2 * qs + half;
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ds[w][col][q], ws[t * kHidden + col][q], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int rl = 8 * (r >> 2) + 4 * half + (r & 3);
+      const int row = r0 + rl;
+      if (row < n) {
+        const long o = (long)row * kHidden + col;
+        if (t == 0) dh_prev[o] = hs[w][rl][col] + acc[r];
+        else dmsg[o] = acc[r];
+      }
+    }
+  }
+}
+
 // dh[i] += sum_{j != i} e_ji dmsg[j]   (E^T dmsg; e_ji read at the transpose position)
 __global__ void __launch_bounds__(256) k_aggregate_add(int n, const int* __restrict__ ro, const int* __restrict__ cols,
                                                        const int* __restrict__ tpos, const float* __restrict__ e,
@@ -847,8 +907,8 @@ double Engine::loss_grad(const float* P, const float* V, int K, float beta, int 
   int cur = 0;
   for (int l = 3; l >= 1; --l) {
     const float* Wl = P + offs[l - 1];
-    hipLaunchKernelGGL(k_dense_bwd<64>, g1(n), dim3(256), 0, s, n, dh[cur].data(), d_z[l].data(), Wl, 2 * kHidden,
-                       dz.data(), dh[cur ^ 1].data(), dmsg.data());
+    hipLaunchKernelGGL(k_dense_bwd_mfma, dim3((unsigned)(((n + 31) / 32 + 3) / 4)), dim3(256), 0, s, n, dh[cur].data(),
+                       d_z[l].data(), Wl, dz.data(), dh[cur ^ 1].data(), dmsg.data());
     hipLaunchKernelGGL(k_aggregate_add, g1((long)n * 32), dim3(256), 0, s, n, m_v.d_ro, m_v.d_cols,
                        d_tpos.data(), d_e.data(), dmsg.data(), dh[cur ^ 1].data());
     hipLaunchKernelGGL(k_wgrad_partials<64>, dim3((unsigned)nb), dim3(256), 0, s, n, d_h[l - 1].data(), kHidden, kHidden,
