@@ -51,7 +51,7 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
       }
       d_urows.upload(rows.data(), rows.size(), st);
       factorize(st);
-      buildLevelSweeps(st); // the colour classes as the levels
+      buildLevelSweeps(st, false); // the colour classes as the levels
       return;
     }
     std::vector<int> level(n);
@@ -65,14 +65,14 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
     alien_hip::group_by_level(n, level.data(), m_nu, m_uptr.data(), rows.data());
     d_urows.upload(rows.data(), rows.size(), st);
     factorize(st);
-    buildLevelSweeps(st);
+    buildLevelSweeps(st, true);
   }
   void update() {}
   void solve(AlgebraType& alg, const VectorType& y, VectorType& x) const {
     if (levelOrdered())
       alien_hip::launch_ilu0_solve_lev<double>(alg.stream(), m_nl, m_lptr.data(), d_lptr.data(), m_lsw, m_nu,
                                                m_uptr.data(), d_uptr.data(), m_usw, y.data(),
-                                               const_cast<double*>(m_t.data()), x.data());
+                                               const_cast<double*>(m_t.data()), x.data(), nullptr, sfWork());
     else
       alien_hip::launch_ilu0_solve<double>(alg.stream(), m_nl, m_lptr.data(), d_lrows.data(), m_nu, m_uptr.data(),
                                            d_urows.data(), m_A.d_ro.data(), m_A.d_cols.data(), m_A.d_dcol.data(),
@@ -86,6 +86,9 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
   bool levelOrdered() const { return m_lsw.k > 0 && m_usw.k > 0; }
   const alien_hip::LevelSweep<double>& lowerSweep() const { return m_lsw; }
   const alien_hip::LevelSweep<double>& upperSweep() const { return m_usw; }
+  // the sync-free sweeps' ticket and error flag (null when the work lists are not built); one
+  // solve at a time per preconditioner
+  int* sfWork() const { return m_lsw.nchunks > 0 ? const_cast<int*>(d_sfwork.data()) : nullptr; }
   // The same sweeps with FP32 values (columns shared), filled from the factor into val / diag
   void fillSweepsF32(hipStream_t st, alien_hip::DeviceArray<float>& lval, alien_hip::DeviceArray<float>& uval,
                      alien_hip::DeviceArray<float>& udiag, alien_hip::LevelSweep<float>& L,
@@ -101,8 +104,8 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
                                               m_A.d_cols.data(), m_A.d_dcol.data(), d_lu.data(), nullptr,
                                               uval.data(), udiag.data());
     alien_hip::check_launch("ilu0 level sweeps (FP32)");
-    L = {n, m_lsw.k, m_lsw.row, m_lsw.col, lval.data(), nullptr};
-    U = {n, m_usw.k, m_usw.row, m_usw.col, uval.data(), udiag.data()};
+    L = {n, m_lsw.k, m_lsw.row, m_lsw.col, lval.data(), nullptr, m_lsw.nchunks, m_lsw.cq, m_lsw.cn};
+    U = {n, m_usw.k, m_usw.row, m_usw.col, uval.data(), udiag.data(), m_usw.nchunks, m_usw.cq, m_usw.cn};
   }
   void solve(const VectorType& y, VectorType& x) const { solve(m_alg, y, x); }
 
@@ -131,7 +134,7 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
     m_alg.allocate(AlgebraType::resource(m_A), m_t);
     ALIEN_HIP_CHECK(hipStreamSynchronize(st));
   }
-  void buildLevelSweeps(hipStream_t st) {
+  void buildLevelSweeps(hipStream_t st, bool levels) {
     const int n = m_A.getNRows();
     const int kl = alien_hip::level_sweep_slots(n, m_A.kcol(), m_A.dcol(), false);
     const int ku = alien_hip::level_sweep_slots(n, m_A.kcol(), m_A.dcol(), true);
@@ -150,11 +153,33 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
     ALIEN_HIP_CHECK(hipStreamSynchronize(st));
     m_lsw = {n, kl, d_lrows.data(), d_lcol.data(), d_lval.data(), nullptr};
     m_usw = {n, ku, d_urows.data(), d_ucol.data(), d_uval.data(), d_udiag.data()};
+    // the sync-free work lists: level sets only (colour classes are few and large), and only
+    // where the levels are small on average
+    const long nlev = std::max(m_nl, m_nu);
+    if (levels && alien_hip::g_ilu_sync_free && nlev > 0 && n <= alien_hip::g_ilu_sf_avg_rows * nlev) {
+      std::vector<int> q, c;
+      alien_hip::level_chunks(m_nl, m_lptr.data(), q, c);
+      d_lcq.upload(q.data(), q.size(), st);
+      d_lcn.upload(c.data(), c.size(), st);
+      m_lsw.nchunks = (int)q.size();
+      alien_hip::level_chunks(m_nu, m_uptr.data(), q, c);
+      d_ucq.upload(q.data(), q.size(), st);
+      d_ucn.upload(c.data(), c.size(), st);
+      m_usw.nchunks = (int)q.size();
+      const int zero[2] = {0, 0};
+      d_sfwork.upload(zero, 2, st);
+      ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+      m_lsw.cq = d_lcq.data();
+      m_lsw.cn = d_lcn.data();
+      m_usw.cq = d_ucq.data();
+      m_usw.cn = d_ucn.data();
+    }
   }
   AlgebraType& m_alg;
   const MatrixType& m_A;
   alien_hip::LevelSweep<double> m_lsw, m_usw;
   alien_hip::DeviceArray<int> d_lcol, d_ucol;
+  alien_hip::DeviceArray<int> d_lcq, d_lcn, d_ucq, d_ucn, d_sfwork;
   alien_hip::DeviceArray<double> d_lval, d_uval, d_udiag;
   int m_nl = 0, m_nu = 0;
   std::vector<int> m_lptr, m_uptr, m_colours;

@@ -128,6 +128,13 @@ int alien_hip_ctx_create(void* stream, alien_hip_ctx** out) {
       if (v >= kXcds) *caps[i] = v / kXcds * kXcds;
     }
   if (const char* e = std::getenv("ALIEN_HIP_ILU_WG_ROWS")) g_ilu_wg_rows = std::max(0L, std::strtol(e, nullptr, 10));
+  if (const char* e = std::getenv("ALIEN_HIP_ILU_SYNC_FREE")) g_ilu_sync_free = std::atoi(e) != 0;
+  if (const char* e = std::getenv("ALIEN_HIP_ILU_SF_AVG_ROWS"))
+    g_ilu_sf_avg_rows = std::max(0L, std::strtol(e, nullptr, 10));
+  if (const char* e = std::getenv("ALIEN_HIP_ILU_SF_BLOCKS"))
+    g_ilu_sf_blocks = std::min(1024L, std::max(1L, std::strtol(e, nullptr, 10)));
+  if (const char* e = std::getenv("ALIEN_HIP_ILU_SF_SPINS"))
+    g_ilu_sf_spins = std::max(0L, std::strtol(e, nullptr, 10));
   *out = make_ctx(stream);
   if (const char* e = std::getenv("ALIEN_HIP_TRACE")) (*out)->trace = std::atoi(e) != 0;
   API_END

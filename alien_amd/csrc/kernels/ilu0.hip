@@ -324,10 +324,131 @@ void level_sweep(hipStream_t st, int nlev, const int* ptr, const int* ptr_dev, c
   }
 }
 
+// ---- sync-free sweeps ------------------------------------------------------------
+// One launch per sweep instead of one per level: persistent waves take chunks of at most 64
+// positions of one level from an atomic ticket, in level order, and each row waits for the
+// entries it reads. Those are written and polled with agent-scope relaxed atomics (global
+// loads and stores that bypass the per-XCD L2s), over a signalling-NaN sentinel that the fill
+// kernel writes first: no flag is needed, a value is ready when it is not the sentinel.
+// Progress: a row only waits on rows of earlier levels, i.e. of chunks taken earlier by waves
+// that are running, and no chunk spans two levels (the lanes of a wave never wait on each other).
+// Every wait is bounded (g_ilu_sf_spins polls); one that runs out raises the error flag, the
+// remaining waits give up at once, and the caller redoes the solve with the level launches.
+// Each row performs the level kernels' operations in their order, so the result is theirs.
+long g_ilu_sf_avg_rows = 8192, g_ilu_sf_blocks = 64, g_ilu_sf_spins = 1L << 16;
+int g_ilu_sync_free = 1;
+
+void level_chunks(int nlev, const int* ptr, std::vector<int>& cq, std::vector<int>& cn) {
+  cq.clear();
+  cn.clear();
+  for (int l = 0; l < nlev; ++l)
+    for (int q = ptr[l]; q < ptr[l + 1]; q += kWave) {
+      cq.push_back(q);
+      cn.push_back(std::min(kWave, ptr[l + 1] - q));
+    }
+}
+
+template <typename T> struct SfBits;
+template <> struct SfBits<double> {
+  typedef unsigned long long U;
+  static constexpr U kSentinel = 0x7ff4deadbeef0001ull; // a signalling NaN: never an arithmetic result
+  static __device__ __forceinline__ U bits(double v) { return (U)__double_as_longlong(v); }
+  static __device__ __forceinline__ double value(U b) { return __longlong_as_double((long long)b); }
+};
+template <> struct SfBits<float> {
+  typedef unsigned int U;
+  static constexpr U kSentinel = 0x7fa5deadu;
+  static __device__ __forceinline__ U bits(float v) { return __float_as_uint(v); }
+  static __device__ __forceinline__ float value(U b) { return __uint_as_float(b); }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_level_sf_fill(int n, T* out, int* work, const int* __restrict__ done) {
+  if (done && *done) return;
+  typedef typename SfBits<T>::U U;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
+    reinterpret_cast<U*>(out)[i] = SfBits<T>::kSentinel;
+  if (blockIdx.x == 0 && threadIdx.x == 0) work[0] = 0; // ticket
+}
+
+template <typename T>
+__device__ __forceinline__ T sf_wait(const T* p, int* err, long limit) {
+  typedef typename SfBits<T>::U U;
+  const U* q = reinterpret_cast<const U*>(p);
+  U b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  long spins = 0;
+  while (b == SfBits<T>::kSentinel) {
+    if (++spins > limit) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    __builtin_amdgcn_s_sleep(2);
+    b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return SfBits<T>::value(b);
+}
+
+template <typename T, bool UPPER, int K>
+__global__ void __launch_bounds__(kBlock) k_level_sf(LevelSweep<T> S, const T* __restrict__ in, T* out, int* work,
+                                                      long limit, const int* __restrict__ done) {
+  if (done && *done) return;
+  typedef typename SfBits<T>::U U;
+  const int lane = threadIdx.x & (kWave - 1);
+  int* ticket = work;
+  int* err = work + 1;
+  for (;;) {
+    int c = 0;
+    if (lane == 0) c = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    c = __shfl(c, 0);
+    if (c >= S.nchunks) break;
+    if (lane < S.cn[c]) {
+      const int q = S.cq[c] + lane;
+      LevelRow<T, UPPER, K> R;
+      R.fetch(S, q);
+      T v = in[R.i];
+#pragma unroll
+      for (int s = 0; s < K; ++s)
+        if (R.c[s] >= 0) v = v - R.a[s] * sf_wait<T>(out + R.c[s], err, limit);
+      if (UPPER) v = v / R.d;
+      __hip_atomic_store(reinterpret_cast<U*>(out) + R.i, SfBits<T>::bits(v), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+template <typename T, bool UPPER>
+void level_sweep_sf(hipStream_t st, const LevelSweep<T>& S, const T* in, T* out, int* work, const int* done) {
+  hipLaunchKernelGGL(k_level_sf_fill<T>, dim3((unsigned)grid_for(S.n, 4)), dim3(kBlock), 0, st, S.n, out, work, done);
+  const dim3 g((unsigned)std::max(1L, g_ilu_sf_blocks));
+  const long lim = g_ilu_sf_spins;
+  switch (S.k) {
+  case 1: hipLaunchKernelGGL((k_level_sf<T, UPPER, 1>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  case 2: hipLaunchKernelGGL((k_level_sf<T, UPPER, 2>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  case 3: hipLaunchKernelGGL((k_level_sf<T, UPPER, 3>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  case 4: hipLaunchKernelGGL((k_level_sf<T, UPPER, 4>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  case 5: hipLaunchKernelGGL((k_level_sf<T, UPPER, 5>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  case 6: hipLaunchKernelGGL((k_level_sf<T, UPPER, 6>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  case 7: hipLaunchKernelGGL((k_level_sf<T, UPPER, 7>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  case 8: hipLaunchKernelGGL((k_level_sf<T, UPPER, 8>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  default: throw Error(ALIEN_HIP_EINVAL, "sync-free ILU0 sweep: unsupported slot count");
+  }
+}
+
 template <typename T>
 void launch_ilu0_solve_lev(hipStream_t st, int nl, const int* lptr_host, const int* lptr_dev, LevelSweep<T> L, int nu,
                            const int* uptr_host, const int* uptr_dev, LevelSweep<T> U, const T* y, T* t, T* x,
-                           const int* done) {
+                           const int* done, int* sf_work) {
+  if (sf_work && g_ilu_sync_free && L.nchunks > 0 && U.nchunks > 0) {
+    level_sweep_sf<T, false>(st, L, y, t, sf_work, done);
+    level_sweep_sf<T, true>(st, U, t, x, sf_work, done);
+    int err = 0;
+    ALIEN_HIP_CHECK(hipMemcpyAsync(&err, sf_work + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
+    if (!err) return;
+    // a wait ran out of polls: clear the flag and take the level launches
+    ALIEN_HIP_CHECK(hipMemsetAsync(sf_work + 1, 0, sizeof(int), st));
+  }
   level_sweep<T, false>(st, nl, lptr_host, lptr_dev, L, y, t, done);
   level_sweep<T, true>(st, nu, uptr_host, uptr_dev, U, t, x, done);
 }
@@ -589,7 +710,7 @@ void launch_ilu0_solve(hipStream_t st, int nl, const int* lptr_host, const int* 
                                            const int*, const double*, int*, T*, T*);                       \
   template void launch_ilu0_solve_lev<T>(hipStream_t, int, const int*, const int*, LevelSweep<T>, int,      \
                                          const int*, const int*, LevelSweep<T>, const T*, T*, T*,           \
-                                         const int*);                                                       \
+                                         const int*, int*);                                                 \
   template void launch_ilu0_factor<T>(hipStream_t, int, const int*, const int*, const int*, const int*,   \
                                       const int*, T*);                                                    \
   template void launch_ilu0_solve<T>(hipStream_t, int, const int*, const int*, int, const int*,           \

@@ -152,11 +152,25 @@ struct LevelSweep {
   const int* col = nullptr;  // k * n
   const T* val = nullptr;    // k * n
   const T* diag = nullptr;   // upper sweep: n
+  // the sync-free form's work list (nchunks = 0: not built): chunk c holds positions
+  // [cq[c], cq[c] + cn[c]) of one level, cn <= 64, chunks in level order
+  int nchunks = 0;
+  const int* cq = nullptr;
+  const int* cn = nullptr;
 };
 constexpr int kLevelMaxSlots = 8;
 // levels of at most this many rows run in one workgroup (runs of two or more);
 // ALIEN_HIP_ILU_WG_ROWS, read when a context is created, overrides it
 extern long g_ilu_wg_rows;
+// Sync-free sweeps (see ilu0.hip): on / off (ALIEN_HIP_ILU_SYNC_FREE), built for level-set
+// schedules whose levels hold at most g_ilu_sf_avg_rows rows on average (ALIEN_HIP_ILU_SF_AVG_ROWS:
+// many small levels, where the launches per level dominate), run on g_ilu_sf_blocks workgroups
+// (ALIEN_HIP_ILU_SF_BLOCKS), each wait bounded by g_ilu_sf_spins polls (ALIEN_HIP_ILU_SF_SPINS);
+// all read when a context is created.
+extern int g_ilu_sync_free;
+extern long g_ilu_sf_avg_rows, g_ilu_sf_blocks, g_ilu_sf_spins;
+// level-aligned chunks of at most 64 positions over the level pointers ptr (nlev levels): cq, cn
+void level_chunks(int nlev, const int* ptr, std::vector<int>& cq, std::vector<int>& cn);
 // Slot count of a sweep (the largest strictly-lower or strictly-upper row
 // length), from the host CSR; 0 when it exceeds kLevelMaxSlots.
 int level_sweep_slots(int n, const int* ro_host, const int* dcol_host, bool upper);
@@ -168,10 +182,13 @@ void launch_level_sweep_fill(hipStream_t, int n, int k, bool upper, const int* r
 // ILU(0) solve z = (LU)^-1 y over the level-ordered sweeps (bit-exact with
 // launch_ilu0_solve): runs of small levels in one workgroup, larger levels one
 // launch each. ptr_host / ptr_dev: the level pointers.
+// sf_work: two device ints (ticket, error flag) for the sync-free form, which runs when both
+// sweeps carry work lists; a wait that runs out of polls raises the flag and the solve is redone
+// by the launch-per-level form (the result is the same either way, bit for bit).
 template <typename T>
 void launch_ilu0_solve_lev(hipStream_t, int nl, const int* lptr_host, const int* lptr_dev, LevelSweep<T> L, int nu,
                            const int* uptr_host, const int* uptr_dev, LevelSweep<T> U, const T* y, T* t, T* x,
-                           const int* done = nullptr);
+                           const int* done = nullptr, int* sf_work = nullptr);
 template <typename T>
 void launch_ilu0_solve(hipStream_t, int nl, const int* lptr_host, const int* lrows, int nu,
                        const int* uptr_host, const int* urows, const int* ro, const int* cols, const int* dcol,

@@ -185,6 +185,7 @@ struct LevelILU {
   bool ordered = false;
   LevelSweep<double> l64{}, u64{};
   LevelSweep<float> l32{}, u32{};
+  int* sf_work = nullptr; // the sync-free sweeps' ticket and error flag, when built
 };
 struct PrecondSpec {
   PrecKind kind = kPrecDiag;
@@ -280,10 +281,11 @@ template <typename T>
 void level_ilu_apply(hipStream_t s, const LevelILU& L, const T* y, T* t, T* x, const int* done) {
   if (L.ordered) {
     if constexpr (std::is_same<T, float>::value)
-      launch_ilu0_solve_lev<float>(s, L.nl, L.lptr, L.lptr_dev, L.l32, L.nu, L.uptr, L.uptr_dev, L.u32, y, t, x, done);
+      launch_ilu0_solve_lev<float>(s, L.nl, L.lptr, L.lptr_dev, L.l32, L.nu, L.uptr, L.uptr_dev, L.u32, y, t, x, done,
+                                   L.sf_work);
     else
       launch_ilu0_solve_lev<double>(s, L.nl, L.lptr, L.lptr_dev, L.l64, L.nu, L.uptr, L.uptr_dev, L.u64, y, t, x,
-                                    done);
+                                    done, L.sf_work);
     return;
   }
   const T* lu;
@@ -1288,6 +1290,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       if (L.ordered) {
         L.l64 = ilu->lowerSweep();
         L.u64 = ilu->upperSweep();
+        L.sf_work = ilu->sfWork();
       }
       if (mixed && L.ordered) {
         ilu->fillSweepsF32(ctx->stream, lev_l32, lev_u32, lev_d32, L.l32, L.u32);

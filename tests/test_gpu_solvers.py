@@ -868,12 +868,14 @@ def test_levelset_ilu0_level_runs_bit_identical(ctx, monkeypatch):
     (a large value: levels of up to 6,900 rows on the 96^3 convection-diffusion system, a first
     chunk of 2 x 1024 rows fetched before the previous level's barrier, the rest row by row), or
     the default split (levels of at most 512 rows in one workgroup), in the REFERENCE and GRAPH
-    modes; GRAPH within one iteration of REFERENCE."""
+    modes; GRAPH within one iteration of REFERENCE. The sync-free sweeps are off here
+    (ALIEN_HIP_ILU_SYNC_FREE=0)."""
     ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 96)
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
     kw = dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0, tol=1e-8, max_iter=200)
     runs = {}
+    monkeypatch.setenv("ALIEN_HIP_ILU_SYNC_FREE", "0")
     try:
         for rows in ("0", "1000000", "512"):
             monkeypatch.setenv("ALIEN_HIP_ILU_WG_ROWS", rows)
@@ -886,6 +888,7 @@ def test_levelset_ilu0_level_runs_bit_identical(ctx, monkeypatch):
                 c.close()
     finally:
         monkeypatch.setenv("ALIEN_HIP_ILU_WG_ROWS", "512")
+        monkeypatch.setenv("ALIEN_HIP_ILU_SYNC_FREE", "1")
         aa.Context().close()
     for mode in (aa.MODE_REFERENCE, aa.MODE_GRAPH):
         x0, r0 = runs["512", mode]
@@ -896,6 +899,46 @@ def test_levelset_ilu0_level_runs_bit_identical(ctx, monkeypatch):
             assert r["iterations"] == r0["iterations"]
             np.testing.assert_array_equal(x, x0)
     assert abs(runs["512", aa.MODE_GRAPH][1]["iterations"] - runs["512", aa.MODE_REFERENCE][1]["iterations"]) <= 1
+
+
+def test_levelset_ilu0_sync_free_bit_identical(ctx, monkeypatch):
+    """The sync-free level-set sweeps (one launch per sweep; rows wait on the entries they read)
+    give the launch-per-level solve bit for bit on the 96^3 convection-diffusion system (286 levels
+    of 3,100 rows on average, so the work lists are built): on 64 workgroups (the default), on 1
+    (four waves claim every chunk) and on 512; with every wait given no poll
+    (ALIEN_HIP_ILU_SF_SPINS=0: each sweep pair raises its error flag and is redone by the level
+    launches); in the REFERENCE (FP64) and MIXED (FP32 sweeps) modes."""
+    ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 96)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
+    kw = dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0, tol=1e-8, max_iter=200)
+    cases = {"levels": {"ALIEN_HIP_ILU_SYNC_FREE": "0"}, "sf64": {}, "sf1": {"ALIEN_HIP_ILU_SF_BLOCKS": "1"},
+             "sf512": {"ALIEN_HIP_ILU_SF_BLOCKS": "512"}, "redo": {"ALIEN_HIP_ILU_SF_SPINS": "0"}}
+    defaults = {"ALIEN_HIP_ILU_SYNC_FREE": "1", "ALIEN_HIP_ILU_SF_BLOCKS": "64", "ALIEN_HIP_ILU_SF_SPINS": "65536"}
+    runs = {}
+    try:
+        for name, env in cases.items():
+            for k, v in defaults.items():
+                monkeypatch.setenv(k, env.get(k, v))
+            c = aa.Context()  # the variables are read when a context is created
+            try:
+                for mode in (aa.MODE_REFERENCE, aa.MODE_MIXED):
+                    say("case level-set ILU0 sync-free", name, mode)
+                    runs[name, mode] = aa.solve_csr(c, ro, cols, vals, b, mode=mode, **kw)
+            finally:
+                c.close()
+    finally:
+        for k, v in defaults.items():
+            monkeypatch.setenv(k, v)
+        aa.Context().close()
+    for mode in (aa.MODE_REFERENCE, aa.MODE_MIXED):
+        x0, r0 = runs["levels", mode]
+        say("  ", mode, r0["iterations"], r0["true_rel_res"])
+        assert r0["status"] == 1 and r0["true_rel_res"] < 1e-8
+        for name in cases:
+            x, r = runs[name, mode]
+            assert r["iterations"] == r0["iterations"], name
+            np.testing.assert_array_equal(x, x0, err_msg=name)
 
 
 KEPT_CASES = [
