@@ -941,6 +941,61 @@ def test_levelset_ilu0_sync_free_bit_identical(ctx, monkeypatch):
             np.testing.assert_array_equal(x, x0, err_msg=name)
 
 
+def random_dag_matrix(n, seed):
+    """Unstructured nonsymmetric test matrix: each row has 0-6 random strictly-lower columns within
+    the previous 400 rows and 0-6 random strictly-upper columns within the next 400, values in
+    [-1, 0), and a diagonal 1 above the row's absolute sum (diagonally dominant). Its level sets are
+    ragged: sizes from 1 to several hundred rows, some rows without a lower or upper part."""
+    rng = np.random.default_rng(seed)
+    ro, cols, vals = [0], [], []
+    for i in range(n):
+        lo = sorted(set(rng.integers(max(0, i - 400), i, rng.integers(0, 7)).tolist())) if i else []
+        up = sorted(set(rng.integers(i + 1, min(n, i + 401), rng.integers(0, 7)).tolist())) if i < n - 1 else []
+        off = -rng.random(len(lo) + len(up)) - 1e-3
+        cols += lo + [i] + up
+        vals += list(off[:len(lo)]) + [float(np.abs(off).sum()) + 1.0] + list(off[len(lo):])
+        ro.append(len(cols))
+    return np.array(ro, np.int32), np.array(cols, np.int32), np.array(vals)
+
+
+def test_levelset_ilu0_sync_free_unstructured(ctx, monkeypatch):
+    """Sync-free level-set sweeps on an unstructured matrix (20,000 rows, ragged level sets, rows
+    without a lower or upper part, so chunks of 1 to 64 rows and slot columns of -1): with the
+    work lists forced on (ALIEN_HIP_ILU_SF_AVG_ROWS large), three BiCGStab iterations equal the
+    launch-per-level solve bit for bit and the oracle's sequential ILU(0) iterates to rounding of
+    the dots, in the REFERENCE and MIXED (FP32 sweeps: bit-identity only) modes."""
+    ro, cols, vals = random_dag_matrix(20000, 7)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(11, n))
+    xo, _ = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-15, max_iter=3)
+    kw = dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0, layout=aa.LAYOUT_CSR, tol=1e-15, max_iter=3)
+    runs = {}
+    try:
+        monkeypatch.setenv("ALIEN_HIP_ILU_SF_AVG_ROWS", "100000000")
+        for sf in ("0", "1"):
+            monkeypatch.setenv("ALIEN_HIP_ILU_SYNC_FREE", sf)
+            c = aa.Context()  # the variables are read when a context is created
+            try:
+                for mode in (aa.MODE_REFERENCE, aa.MODE_MIXED):
+                    say("case level-set ILU0 sync-free unstructured", sf, mode)
+                    runs[sf, mode] = aa.solve_csr(c, ro, cols, vals, b, mode=mode, **kw)
+            finally:
+                c.close()
+    finally:
+        monkeypatch.setenv("ALIEN_HIP_ILU_SF_AVG_ROWS", "8192")
+        monkeypatch.setenv("ALIEN_HIP_ILU_SYNC_FREE", "1")
+        aa.Context().close()
+    for mode in (aa.MODE_REFERENCE, aa.MODE_MIXED):
+        x0, r0 = runs["0", mode]
+        x1, r1 = runs["1", mode]
+        say("  ", mode, r1["iterations"], np.abs(x1 - xo).max())
+        assert r1["iterations"] == r0["iterations"]
+        np.testing.assert_array_equal(x1, x0)
+    x, r = runs["1", aa.MODE_REFERENCE]
+    assert r["iterations"] == 3
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-11 * np.abs(xo).max())
+
+
 KEPT_CASES = [
     ("cg jacobi graph", dict(solver=aa.CG, precond=aa.PREC_DIAG, mode=aa.MODE_GRAPH)),
     ("cg learned mixed", dict(solver=aa.CG, precond=aa.PREC_LEARNED, mode=aa.MODE_MIXED, learned=True)),
