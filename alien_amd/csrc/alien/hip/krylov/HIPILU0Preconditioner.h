@@ -104,8 +104,10 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
                                               m_A.d_cols.data(), m_A.d_dcol.data(), d_lu.data(), nullptr,
                                               uval.data(), udiag.data());
     alien_hip::check_launch("ilu0 level sweeps (FP32)");
-    L = {n, m_lsw.k, m_lsw.row, m_lsw.col, lval.data(), nullptr, m_lsw.nchunks, m_lsw.cq, m_lsw.cn};
-    U = {n, m_usw.k, m_usw.row, m_usw.col, uval.data(), udiag.data(), m_usw.nchunks, m_usw.cq, m_usw.cn};
+    L = {n, m_lsw.k, m_lsw.row, m_lsw.col, lval.data(), nullptr, m_lsw.nchunks, m_lsw.cq, m_lsw.cn, m_lsw.rpl,
+         m_lsw.blocks};
+    U = {n, m_usw.k, m_usw.row, m_usw.col, uval.data(), udiag.data(), m_usw.nchunks, m_usw.cq, m_usw.cn, m_usw.rpl,
+         m_usw.blocks};
   }
   void solve(const VectorType& y, VectorType& x) const { solve(m_alg, y, x); }
 
@@ -157,12 +159,16 @@ class ILU0Preconditioner<HIPInternalLinearAlgebra> {
     // where the levels are small on average
     const long nlev = std::max(m_nl, m_nu);
     if (levels && alien_hip::g_ilu_sync_free && nlev > 0 && n <= alien_hip::g_ilu_sf_avg_rows * nlev) {
+      int rpl = 1, blocks = 64;
+      alien_hip::level_sf_shape(n, nlev, rpl, blocks);
       std::vector<int> q, c;
-      alien_hip::level_chunks(m_nl, m_lptr.data(), q, c);
+      alien_hip::level_chunks(m_nl, m_lptr.data(), alien_hip::kWave * rpl, q, c);
       d_lcq.upload(q.data(), q.size(), st);
       d_lcn.upload(c.data(), c.size(), st);
       m_lsw.nchunks = (int)q.size();
-      alien_hip::level_chunks(m_nu, m_uptr.data(), q, c);
+      m_lsw.rpl = m_usw.rpl = rpl;
+      m_lsw.blocks = m_usw.blocks = blocks;
+      alien_hip::level_chunks(m_nu, m_uptr.data(), alien_hip::kWave * rpl, q, c);
       d_ucq.upload(q.data(), q.size(), st);
       d_ucn.upload(c.data(), c.size(), st);
       m_usw.nchunks = (int)q.size();

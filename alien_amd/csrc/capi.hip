@@ -131,8 +131,9 @@ int alien_hip_ctx_create(void* stream, alien_hip_ctx** out) {
   if (const char* e = std::getenv("ALIEN_HIP_ILU_SYNC_FREE")) g_ilu_sync_free = std::atoi(e) != 0;
   if (const char* e = std::getenv("ALIEN_HIP_ILU_SF_AVG_ROWS"))
     g_ilu_sf_avg_rows = std::max(0L, std::strtol(e, nullptr, 10));
-  if (const char* e = std::getenv("ALIEN_HIP_ILU_SF_BLOCKS"))
-    g_ilu_sf_blocks = std::min(1024L, std::max(1L, std::strtol(e, nullptr, 10)));
+  if (const char* e = std::getenv("ALIEN_HIP_ILU_SF_BLOCKS")) // 0: from the level sizes
+    g_ilu_sf_blocks = std::min(1024L, std::max(0L, std::strtol(e, nullptr, 10)));
+  if (const char* e = std::getenv("ALIEN_HIP_ILU_SF_RPL")) g_ilu_sf_rpl = std::strtol(e, nullptr, 10); // 0: auto
   if (const char* e = std::getenv("ALIEN_HIP_ILU_SF_SPINS"))
     g_ilu_sf_spins = std::max(0L, std::strtol(e, nullptr, 10));
   *out = make_ctx(stream);

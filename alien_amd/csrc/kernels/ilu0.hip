@@ -325,27 +325,44 @@ void level_sweep(hipStream_t st, int nlev, const int* ptr, const int* ptr_dev, c
 }
 
 // ---- sync-free sweeps ------------------------------------------------------------
-// One launch per sweep instead of one per level: persistent waves take chunks of at most 64
-// positions of one level from an atomic ticket, in level order, and each row waits for the
-// entries it reads. Those are written and polled with agent-scope relaxed atomics (global
-// loads and stores that bypass the per-XCD L2s), over a signalling-NaN sentinel that the fill
-// kernel writes first: no flag is needed, a value is ready when it is not the sentinel.
-// Progress: a row only waits on rows of earlier levels, i.e. of chunks taken earlier by waves
-// that are running, and no chunk spans two levels (the lanes of a wave never wait on each other).
+// One launch per sweep instead of one per level: persistent waves take chunks of at most 64 x rpl
+// positions of one level from an atomic ticket, in level order, each lane working on rpl rows of
+// its chunk in turn, and each row waits for the entries it reads. Those are written and polled
+// with agent-scope relaxed atomics (global loads and stores that bypass the per-XCD L2s), over a
+// signalling-NaN sentinel that the fill kernel writes first: no flag is needed, a value is ready
+// when it is not the sentinel.
+// Progress: a row only waits on rows of earlier levels, i.e. of chunks taken earlier by waves that
+// are running, and no chunk spans two levels (the rows of a chunk never wait on each other).
 // Every wait is bounded (g_ilu_sf_spins polls); one that runs out raises the error flag, the
 // remaining waits give up at once, and the caller redoes the solve with the level launches.
 // Each row performs the level kernels' operations in their order, so the result is theirs.
-long g_ilu_sf_avg_rows = 8192, g_ilu_sf_blocks = 64, g_ilu_sf_spins = 1L << 16;
+long g_ilu_sf_avg_rows = 32768, g_ilu_sf_blocks = 0, g_ilu_sf_rpl = 0, g_ilu_sf_spins = 1L << 16;
 int g_ilu_sync_free = 1;
 
-void level_chunks(int nlev, const int* ptr, std::vector<int>& cq, std::vector<int>& cn) {
+void level_chunks(int nlev, const int* ptr, int rows, std::vector<int>& cq, std::vector<int>& cn) {
   cq.clear();
   cn.clear();
   for (int l = 0; l < nlev; ++l)
-    for (int q = ptr[l]; q < ptr[l + 1]; q += kWave) {
+    for (int q = ptr[l]; q < ptr[l + 1]; q += rows) {
       cq.push_back(q);
-      cn.push_back(std::min(kWave, ptr[l + 1] - q));
+      cn.push_back(std::min(rows, ptr[l + 1] - q));
     }
+}
+
+void level_sf_shape(long n, long nlev, int& rpl, int& blocks) {
+  // Measured on 7-point systems (BiCGStab, one launch per sweep): 128^3 (5,490 rows per level)
+  // is fastest with one row per lane on 64 workgroups (2 rows: 0.039 s against 0.029 s), 256^3
+  // (21,900 rows per level) with two rows per lane on 128-192 workgroups (0.145 s; one row per
+  // lane 0.177-0.192 s, four 0.182-0.211 s). Both keep about three levels' rows in flight.
+  const long avg = nlev > 0 ? n / nlev : n;
+  rpl = g_ilu_sf_rpl == 1 || g_ilu_sf_rpl == 2 || g_ilu_sf_rpl == 4 ? (int)g_ilu_sf_rpl : (avg > 8192 ? 2 : 1);
+  if (g_ilu_sf_blocks > 0) {
+    blocks = (int)g_ilu_sf_blocks;
+  } else {
+    const long per_block = (long)kBlock * rpl; // rows one workgroup holds at once
+    const long b = (3 * avg + per_block - 1) / per_block;
+    blocks = (int)std::min(256L, std::max(64L, (b + kXcds - 1) / kXcds * kXcds));
+  }
 }
 
 template <typename T> struct SfBits;
@@ -389,7 +406,7 @@ __device__ __forceinline__ T sf_wait(const T* p, int* err, long limit) {
   return SfBits<T>::value(b);
 }
 
-template <typename T, bool UPPER, int K>
+template <typename T, bool UPPER, int K, int RPL>
 __global__ void __launch_bounds__(kBlock) k_level_sf(LevelSweep<T> S, const T* __restrict__ in, T* out, int* work,
                                                       long limit, const int* __restrict__ done) {
   if (done && *done) return;
@@ -402,35 +419,49 @@ __global__ void __launch_bounds__(kBlock) k_level_sf(LevelSweep<T> S, const T* _
     if (lane == 0) c = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     c = __shfl(c, 0);
     if (c >= S.nchunks) break;
-    if (lane < S.cn[c]) {
-      const int q = S.cq[c] + lane;
-      LevelRow<T, UPPER, K> R;
-      R.fetch(S, q);
-      T v = in[R.i];
+    const int q0 = S.cq[c], m = S.cn[c];
+    LevelRow<T, UPPER, K> R[RPL];
 #pragma unroll
-      for (int s = 0; s < K; ++s)
-        if (R.c[s] >= 0) v = v - R.a[s] * sf_wait<T>(out + R.c[s], err, limit);
-      if (UPPER) v = v / R.d;
-      __hip_atomic_store(reinterpret_cast<U*>(out) + R.i, SfBits<T>::bits(v), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
+    for (int r = 0; r < RPL; ++r)
+      if (lane + r * kWave < m) R[r].fetch(S, q0 + lane + r * kWave);
+#pragma unroll
+    for (int r = 0; r < RPL; ++r)
+      if (lane + r * kWave < m) {
+        T v = in[R[r].i];
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+          if (R[r].c[s] >= 0) v = v - R[r].a[s] * sf_wait<T>(out + R[r].c[s], err, limit);
+        if (UPPER) v = v / R[r].d;
+        __hip_atomic_store(reinterpret_cast<U*>(out) + R[r].i, SfBits<T>::bits(v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+  }
+}
+
+template <typename T, bool UPPER, int K>
+void level_sf_k(hipStream_t st, const LevelSweep<T>& S, const T* in, T* out, int* work, const int* done) {
+  const dim3 g((unsigned)std::max(1, S.blocks)), b(kBlock);
+  const long lim = g_ilu_sf_spins;
+  switch (S.rpl) {
+  case 1: hipLaunchKernelGGL((k_level_sf<T, UPPER, K, 1>), g, b, 0, st, S, in, out, work, lim, done); break;
+  case 2: hipLaunchKernelGGL((k_level_sf<T, UPPER, K, 2>), g, b, 0, st, S, in, out, work, lim, done); break;
+  case 4: hipLaunchKernelGGL((k_level_sf<T, UPPER, K, 4>), g, b, 0, st, S, in, out, work, lim, done); break;
+  default: throw Error(ALIEN_HIP_EINVAL, "sync-free ILU0 sweep: rows per lane must be 1, 2 or 4");
   }
 }
 
 template <typename T, bool UPPER>
 void level_sweep_sf(hipStream_t st, const LevelSweep<T>& S, const T* in, T* out, int* work, const int* done) {
   hipLaunchKernelGGL(k_level_sf_fill<T>, dim3((unsigned)grid_for(S.n, 4)), dim3(kBlock), 0, st, S.n, out, work, done);
-  const dim3 g((unsigned)std::max(1L, g_ilu_sf_blocks));
-  const long lim = g_ilu_sf_spins;
   switch (S.k) {
-  case 1: hipLaunchKernelGGL((k_level_sf<T, UPPER, 1>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
-  case 2: hipLaunchKernelGGL((k_level_sf<T, UPPER, 2>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
-  case 3: hipLaunchKernelGGL((k_level_sf<T, UPPER, 3>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
-  case 4: hipLaunchKernelGGL((k_level_sf<T, UPPER, 4>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
-  case 5: hipLaunchKernelGGL((k_level_sf<T, UPPER, 5>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
-  case 6: hipLaunchKernelGGL((k_level_sf<T, UPPER, 6>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
-  case 7: hipLaunchKernelGGL((k_level_sf<T, UPPER, 7>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
-  case 8: hipLaunchKernelGGL((k_level_sf<T, UPPER, 8>), g, dim3(kBlock), 0, st, S, in, out, work, lim, done); break;
+  case 1: level_sf_k<T, UPPER, 1>(st, S, in, out, work, done); break;
+  case 2: level_sf_k<T, UPPER, 2>(st, S, in, out, work, done); break;
+  case 3: level_sf_k<T, UPPER, 3>(st, S, in, out, work, done); break;
+  case 4: level_sf_k<T, UPPER, 4>(st, S, in, out, work, done); break;
+  case 5: level_sf_k<T, UPPER, 5>(st, S, in, out, work, done); break;
+  case 6: level_sf_k<T, UPPER, 6>(st, S, in, out, work, done); break;
+  case 7: level_sf_k<T, UPPER, 7>(st, S, in, out, work, done); break;
+  case 8: level_sf_k<T, UPPER, 8>(st, S, in, out, work, done); break;
   default: throw Error(ALIEN_HIP_EINVAL, "sync-free ILU0 sweep: unsupported slot count");
   }
 }

@@ -153,10 +153,12 @@ struct LevelSweep {
   const T* val = nullptr;    // k * n
   const T* diag = nullptr;   // upper sweep: n
   // the sync-free form's work list (nchunks = 0: not built): chunk c holds positions
-  // [cq[c], cq[c] + cn[c]) of one level, cn <= 64, chunks in level order
+  // [cq[c], cq[c] + cn[c]) of one level, cn <= 64 * rpl, chunks in level order; each lane works
+  // on rpl rows of a chunk; the launch has `blocks` workgroups
   int nchunks = 0;
   const int* cq = nullptr;
   const int* cn = nullptr;
+  int rpl = 1, blocks = 64;
 };
 constexpr int kLevelMaxSlots = 8;
 // levels of at most this many rows run in one workgroup (runs of two or more);
@@ -164,13 +166,15 @@ constexpr int kLevelMaxSlots = 8;
 extern long g_ilu_wg_rows;
 // Sync-free sweeps (see ilu0.hip): on / off (ALIEN_HIP_ILU_SYNC_FREE), built for level-set
 // schedules whose levels hold at most g_ilu_sf_avg_rows rows on average (ALIEN_HIP_ILU_SF_AVG_ROWS:
-// many small levels, where the launches per level dominate), run on g_ilu_sf_blocks workgroups
-// (ALIEN_HIP_ILU_SF_BLOCKS), each wait bounded by g_ilu_sf_spins polls (ALIEN_HIP_ILU_SF_SPINS);
-// all read when a context is created.
+// many small levels, where the launches per level dominate); rows per lane and workgroups from
+// level_sf_shape unless ALIEN_HIP_ILU_SF_RPL (1, 2 or 4) or ALIEN_HIP_ILU_SF_BLOCKS set them; each
+// wait bounded by g_ilu_sf_spins polls (ALIEN_HIP_ILU_SF_SPINS); all read when a context is created.
 extern int g_ilu_sync_free;
-extern long g_ilu_sf_avg_rows, g_ilu_sf_blocks, g_ilu_sf_spins;
-// level-aligned chunks of at most 64 positions over the level pointers ptr (nlev levels): cq, cn
-void level_chunks(int nlev, const int* ptr, std::vector<int>& cq, std::vector<int>& cn);
+extern long g_ilu_sf_avg_rows, g_ilu_sf_blocks, g_ilu_sf_rpl, g_ilu_sf_spins;
+// level-aligned chunks of at most `rows` positions over the level pointers ptr (nlev levels): cq, cn
+void level_chunks(int nlev, const int* ptr, int rows, std::vector<int>& cq, std::vector<int>& cn);
+// rows per lane and workgroups of the sync-free sweeps of an n-row schedule of nlev levels
+void level_sf_shape(long n, long nlev, int& rpl, int& blocks);
 // Slot count of a sweep (the largest strictly-lower or strictly-upper row
 // length), from the host CSR; 0 when it exceeds kLevelMaxSlots.
 int level_sweep_slots(int n, const int* ro_host, const int* dcol_host, bool upper);

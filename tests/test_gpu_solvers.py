@@ -904,17 +904,21 @@ def test_levelset_ilu0_level_runs_bit_identical(ctx, monkeypatch):
 def test_levelset_ilu0_sync_free_bit_identical(ctx, monkeypatch):
     """The sync-free level-set sweeps (one launch per sweep; rows wait on the entries they read)
     give the launch-per-level solve bit for bit on the 96^3 convection-diffusion system (286 levels
-    of 3,100 rows on average, so the work lists are built): on 64 workgroups (the default), on 1
-    (four waves claim every chunk) and on 512; with every wait given no poll
+    of 3,100 rows on average, so the work lists are built): with the default shape (one row per
+    lane, 64 workgroups), on 1 workgroup (four waves claim every chunk) and on 512; with 2 and 4 rows
+    per lane (chunks of up to 128 and 256 rows; 4 on 8 workgroups); with every wait given no poll
     (ALIEN_HIP_ILU_SF_SPINS=0: each sweep pair raises its error flag and is redone by the level
     launches); in the REFERENCE (FP64) and MIXED (FP32 sweeps) modes."""
     ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 96)
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
     kw = dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0, tol=1e-8, max_iter=200)
-    cases = {"levels": {"ALIEN_HIP_ILU_SYNC_FREE": "0"}, "sf64": {}, "sf1": {"ALIEN_HIP_ILU_SF_BLOCKS": "1"},
-             "sf512": {"ALIEN_HIP_ILU_SF_BLOCKS": "512"}, "redo": {"ALIEN_HIP_ILU_SF_SPINS": "0"}}
-    defaults = {"ALIEN_HIP_ILU_SYNC_FREE": "1", "ALIEN_HIP_ILU_SF_BLOCKS": "64", "ALIEN_HIP_ILU_SF_SPINS": "65536"}
+    cases = {"levels": {"ALIEN_HIP_ILU_SYNC_FREE": "0"}, "sf": {}, "sf1": {"ALIEN_HIP_ILU_SF_BLOCKS": "1"},
+             "sf512": {"ALIEN_HIP_ILU_SF_BLOCKS": "512"}, "rpl2": {"ALIEN_HIP_ILU_SF_RPL": "2"},
+             "rpl4": {"ALIEN_HIP_ILU_SF_RPL": "4", "ALIEN_HIP_ILU_SF_BLOCKS": "8"},
+             "redo": {"ALIEN_HIP_ILU_SF_SPINS": "0"}}
+    defaults = {"ALIEN_HIP_ILU_SYNC_FREE": "1", "ALIEN_HIP_ILU_SF_BLOCKS": "0", "ALIEN_HIP_ILU_SF_RPL": "0",
+                "ALIEN_HIP_ILU_SF_SPINS": "65536"}
     runs = {}
     try:
         for name, env in cases.items():
@@ -962,8 +966,9 @@ def test_levelset_ilu0_sync_free_unstructured(ctx, monkeypatch):
     """Sync-free level-set sweeps on an unstructured matrix (20,000 rows, ragged level sets, rows
     without a lower or upper part, so chunks of 1 to 64 rows and slot columns of -1): with the
     work lists forced on (ALIEN_HIP_ILU_SF_AVG_ROWS large), three BiCGStab iterations equal the
-    launch-per-level solve bit for bit and the oracle's sequential ILU(0) iterates to rounding of
-    the dots, in the REFERENCE and MIXED (FP32 sweeps: bit-identity only) modes."""
+    launch-per-level solve bit for bit (one and two rows per lane) and the oracle's sequential
+    ILU(0) iterates to rounding of the dots, in the REFERENCE and MIXED (FP32 sweeps: bit-identity
+    only) modes."""
     ro, cols, vals = random_dag_matrix(20000, 7)
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(11, n))
@@ -972,8 +977,9 @@ def test_levelset_ilu0_sync_free_unstructured(ctx, monkeypatch):
     runs = {}
     try:
         monkeypatch.setenv("ALIEN_HIP_ILU_SF_AVG_ROWS", "100000000")
-        for sf in ("0", "1"):
-            monkeypatch.setenv("ALIEN_HIP_ILU_SYNC_FREE", sf)
+        for sf in ("0", "1", "2"):
+            monkeypatch.setenv("ALIEN_HIP_ILU_SYNC_FREE", "1" if sf != "0" else "0")
+            monkeypatch.setenv("ALIEN_HIP_ILU_SF_RPL", "2" if sf == "2" else "1")
             c = aa.Context()  # the variables are read when a context is created
             try:
                 for mode in (aa.MODE_REFERENCE, aa.MODE_MIXED):
@@ -982,15 +988,17 @@ def test_levelset_ilu0_sync_free_unstructured(ctx, monkeypatch):
             finally:
                 c.close()
     finally:
-        monkeypatch.setenv("ALIEN_HIP_ILU_SF_AVG_ROWS", "8192")
+        monkeypatch.setenv("ALIEN_HIP_ILU_SF_AVG_ROWS", "32768")
         monkeypatch.setenv("ALIEN_HIP_ILU_SYNC_FREE", "1")
+        monkeypatch.setenv("ALIEN_HIP_ILU_SF_RPL", "0")
         aa.Context().close()
     for mode in (aa.MODE_REFERENCE, aa.MODE_MIXED):
         x0, r0 = runs["0", mode]
-        x1, r1 = runs["1", mode]
-        say("  ", mode, r1["iterations"], np.abs(x1 - xo).max())
-        assert r1["iterations"] == r0["iterations"]
-        np.testing.assert_array_equal(x1, x0)
+        for sf in ("1", "2"):
+            x1, r1 = runs[sf, mode]
+            say("  ", mode, sf, r1["iterations"], np.abs(x1 - xo).max())
+            assert r1["iterations"] == r0["iterations"]
+            np.testing.assert_array_equal(x1, x0)
     x, r = runs["1", aa.MODE_REFERENCE]
     assert r["iterations"] == 3
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-11 * np.abs(xo).max())
