@@ -417,7 +417,9 @@ __global__ void __launch_bounds__(kBlock) k_level_sf(LevelSweep<T> S, const T* _
   for (;;) {
     int c = 0;
     if (lane == 0) c = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    c = __shfl(c, 0);
+    // every lane is active here, so the first is lane 0: the chunk index becomes wave-uniform
+    // (a scalar register) and its bounds come through the scalar cache
+    c = __builtin_amdgcn_readfirstlane(c);
     if (c >= S.nchunks) break;
     const int q0 = S.cq[c], m = S.cn[c];
     LevelRow<T, UPPER, K> R[RPL];
