@@ -440,8 +440,31 @@ __global__ void __launch_bounds__(kBlock) k_level_sf(LevelSweep<T> S, const T* _
   }
 }
 
+// The redo of a sync-free sweep whose error flag is set (a wait ran out of polls), as a kernel so
+// that no host round trip follows each sweep: one workgroup takes the levels in order, a barrier
+// between them, with the level kernels' per-row operations; it then clears the flag. Without the
+// flag it returns at once.
 template <typename T, bool UPPER, int K>
-void level_sf_k(hipStream_t st, const LevelSweep<T>& S, const T* in, T* out, int* work, const int* done) {
+__global__ void __launch_bounds__(kLevelWgBlock) k_level_sf_redo(int nlev, const int* __restrict__ ptr,
+                                                                  LevelSweep<T> S, const T* __restrict__ in, T* out,
+                                                                  int* work, const int* __restrict__ done) {
+  if (done && *done) return;
+  if (__hip_atomic_load(work + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+  for (int l = 0; l < nlev; ++l) {
+    for (int q = ptr[l] + (int)threadIdx.x; q < ptr[l + 1]; q += kLevelWgBlock) {
+      LevelRow<T, UPPER, K> X;
+      X.fetch(S, q);
+      out[X.i] = X.value(in, out);
+    }
+    __syncthreads();
+  }
+  __syncthreads(); // every lane has read the flag
+  if (threadIdx.x == 0) __hip_atomic_store(work + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T, bool UPPER, int K>
+void level_sf_k(hipStream_t st, int nlev, const int* ptr_dev, const LevelSweep<T>& S, const T* in, T* out, int* work,
+                const int* done) {
   const dim3 g((unsigned)std::max(1, S.blocks)), b(kBlock);
   const long lim = g_ilu_sf_spins;
   switch (S.rpl) {
@@ -450,20 +473,23 @@ void level_sf_k(hipStream_t st, const LevelSweep<T>& S, const T* in, T* out, int
   case 4: hipLaunchKernelGGL((k_level_sf<T, UPPER, K, 4>), g, b, 0, st, S, in, out, work, lim, done); break;
   default: throw Error(ALIEN_HIP_EINVAL, "sync-free ILU0 sweep: rows per lane must be 1, 2 or 4");
   }
+  hipLaunchKernelGGL((k_level_sf_redo<T, UPPER, K>), dim3(1), dim3(kLevelWgBlock), 0, st, nlev, ptr_dev, S, in, out,
+                     work, done);
 }
 
 template <typename T, bool UPPER>
-void level_sweep_sf(hipStream_t st, const LevelSweep<T>& S, const T* in, T* out, int* work, const int* done) {
+void level_sweep_sf(hipStream_t st, int nlev, const int* ptr_dev, const LevelSweep<T>& S, const T* in, T* out,
+                    int* work, const int* done) {
   hipLaunchKernelGGL(k_level_sf_fill<T>, dim3((unsigned)grid_for(S.n, 4)), dim3(kBlock), 0, st, S.n, out, work, done);
   switch (S.k) {
-  case 1: level_sf_k<T, UPPER, 1>(st, S, in, out, work, done); break;
-  case 2: level_sf_k<T, UPPER, 2>(st, S, in, out, work, done); break;
-  case 3: level_sf_k<T, UPPER, 3>(st, S, in, out, work, done); break;
-  case 4: level_sf_k<T, UPPER, 4>(st, S, in, out, work, done); break;
-  case 5: level_sf_k<T, UPPER, 5>(st, S, in, out, work, done); break;
-  case 6: level_sf_k<T, UPPER, 6>(st, S, in, out, work, done); break;
-  case 7: level_sf_k<T, UPPER, 7>(st, S, in, out, work, done); break;
-  case 8: level_sf_k<T, UPPER, 8>(st, S, in, out, work, done); break;
+  case 1: level_sf_k<T, UPPER, 1>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 2: level_sf_k<T, UPPER, 2>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 3: level_sf_k<T, UPPER, 3>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 4: level_sf_k<T, UPPER, 4>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 5: level_sf_k<T, UPPER, 5>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 6: level_sf_k<T, UPPER, 6>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 7: level_sf_k<T, UPPER, 7>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 8: level_sf_k<T, UPPER, 8>(st, nlev, ptr_dev, S, in, out, work, done); break;
   default: throw Error(ALIEN_HIP_EINVAL, "sync-free ILU0 sweep: unsupported slot count");
   }
 }
@@ -473,14 +499,11 @@ void launch_ilu0_solve_lev(hipStream_t st, int nl, const int* lptr_host, const i
                            const int* uptr_host, const int* uptr_dev, LevelSweep<T> U, const T* y, T* t, T* x,
                            const int* done, int* sf_work) {
   if (sf_work && g_ilu_sync_free && L.nchunks > 0 && U.nchunks > 0) {
-    level_sweep_sf<T, false>(st, L, y, t, sf_work, done);
-    level_sweep_sf<T, true>(st, U, t, x, sf_work, done);
-    int err = 0;
-    ALIEN_HIP_CHECK(hipMemcpyAsync(&err, sf_work + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    ALIEN_HIP_CHECK(hipStreamSynchronize(st));
-    if (!err) return;
-    // a wait ran out of polls: clear the flag and take the level launches
-    ALIEN_HIP_CHECK(hipMemsetAsync(sf_work + 1, 0, sizeof(int), st));
+    // per sweep: sentinel fill, the persistent sweep, the redo kernel (a no-op unless a wait ran
+    // out of polls); no host round trip, so a solve can capture it in a graph
+    level_sweep_sf<T, false>(st, nl, lptr_dev, L, y, t, sf_work, done);
+    level_sweep_sf<T, true>(st, nu, uptr_dev, U, t, x, sf_work, done);
+    return;
   }
   level_sweep<T, false>(st, nl, lptr_host, lptr_dev, L, y, t, done);
   level_sweep<T, true>(st, nu, uptr_host, uptr_dev, U, t, x, done);

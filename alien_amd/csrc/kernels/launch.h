@@ -187,8 +187,9 @@ void launch_level_sweep_fill(hipStream_t, int n, int k, bool upper, const int* r
 // launch_ilu0_solve): runs of small levels in one workgroup, larger levels one
 // launch each. ptr_host / ptr_dev: the level pointers.
 // sf_work: two device ints (ticket, error flag) for the sync-free form, which runs when both
-// sweeps carry work lists; a wait that runs out of polls raises the flag and the solve is redone
-// by the launch-per-level form (the result is the same either way, bit for bit).
+// sweeps carry work lists (and lptr_dev / uptr_dev are given); a wait that runs out of polls
+// raises the flag and a one-workgroup kernel redoes that sweep level by level (the result is the
+// same either way, bit for bit). The sync-free form makes no host call: it may be captured.
 template <typename T>
 void launch_ilu0_solve_lev(hipStream_t, int nl, const int* lptr_host, const int* lptr_dev, LevelSweep<T> L, int nu,
                            const int* uptr_host, const int* uptr_dev, LevelSweep<T> U, const T* y, T* t, T* x,

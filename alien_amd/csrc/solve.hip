@@ -187,6 +187,10 @@ struct LevelILU {
   LevelSweep<float> l32{}, u32{};
   int* sf_work = nullptr; // the sync-free sweeps' ticket and error flag, when built
 };
+// The sweeps run sync-free (a few launches per apply, no host call): a solve may capture them.
+inline bool level_sync_free(const LevelILU& L) {
+  return L.ordered && L.sf_work && g_ilu_sync_free && L.l64.nchunks > 0 && L.u64.nchunks > 0;
+}
 struct PrecondSpec {
   PrecKind kind = kPrecDiag;
   int order = 0;
@@ -785,9 +789,9 @@ struct FusedBiCG {
     if (!diag) precond(p.data(), phat.data());
     launch_bicg_v<T>(s, ctx->red(), n, A, phat.data(), r0.data(), v.data(), st.data());
     launch_bicg_s<T>(s, ctx->red(), n, r.data(), v.data(), sv.data(), dinv, shat.data(), st.data());
-    // level-set ILU0 (never captured in a graph): a stop on s skips the second apply's hundreds of
-    // per-level launches, which would otherwise each be launched to return at once
-    if (P.kind == kPrecILU0Level && read_state().done) return;
+    // level-set ILU0 with one launch per level (never captured in a graph): a stop on s skips the
+    // second apply's hundreds of per-level launches, which would otherwise each return at once
+    if (P.kind == kPrecILU0Level && !level_sync_free(P.lev) && read_state().done) return;
     if (!diag) precond(sv.data(), shat.data());
     launch_bicg_t<T>(s, ctx->red(), n, A, shat.data(), sv.data(), t.data(), st.data());
     launch_bicg_xr<T>(s, ctx->red(), n, shat.data(), phat.data(), sv.data(), t.data(), r0.data(), x, r.data(),
@@ -1346,10 +1350,10 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
       ps.neumann_factor = np.factor();
       phase("Neumann factor");
     }
-    // Level-set ILU0 launches one kernel per level and sweep: a graph of k
-    // iterations would hold k x 2 x levels nodes and cost more to instantiate
-    // than it saves, so those solves launch directly.
-    const bool capture = ps.kind != kPrecILU0Level;
+    // Level-set ILU0 with one launch per level and sweep: a graph of k iterations
+    // would hold k x 2 x levels nodes and cost more to instantiate than it saves,
+    // so those solves launch directly; the sync-free sweeps are captured.
+    const bool capture = ps.kind != kPrecILU0Level || level_sync_free(ps.lev);
     int stall_window = 0;
     if (learned) {
       gnn::SystemView v;
@@ -1533,7 +1537,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
           m.rel = r2.value;
         } else if (bicg) {
           FusedBiCG<double> F(ctx, n, rb64, invd64, p64);
-          const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, capture && p64.kind != kPrecILU0Level,
+          const BiCGState s = F.solve(b.data(), x.data(), p->tol, left, capture,
                                       p->poll_every, nullptr);
           m.it = used + s.it;
           m.status = s.status;
@@ -1541,7 +1545,7 @@ void solve_on(alien_hip_ctx* ctx, const alien_hip_solve_params* p, HIPMatrix& A,
           m.rel = std::sqrt(s.rr / alg.dot(b, b));
         } else {
           FusedCG<double> F(ctx, n, rb64, invd64, p64);
-          CGState s = F.solve(b.data(), x.data(), false, p->tol, left, capture && p64.kind != kPrecILU0Level,
+          CGState s = F.solve(b.data(), x.data(), false, p->tol, left, capture,
                               p->poll_every, nullptr);
           finish_breakdown(F, s, &m.error);
           m.it = used + s.it;

@@ -908,11 +908,15 @@ def test_levelset_ilu0_sync_free_bit_identical(ctx, monkeypatch):
     lane, 64 workgroups), on 1 workgroup (four waves claim every chunk) and on 512; with 2 and 4 rows
     per lane (chunks of up to 128 and 256 rows; 4 on 8 workgroups); with every wait given no poll
     (ALIEN_HIP_ILU_SF_SPINS=0: each sweep pair raises its error flag and is redone by the level
-    launches); in the REFERENCE (FP64) and MIXED (FP32 sweeps) modes."""
+    launches, now by a one-workgroup kernel inside the stream); in the REFERENCE (FP64), MIXED (FP32
+    sweeps) and GRAPH modes. MIXED polls every iteration (its reliable updates are decided at the
+    polls), GRAPH every 8: the sync-free sweeps are captured in its graphs, the level launches are
+    not."""
     ro, cols, vals = aa.gen_csr(aa.GEN_CONVDIFF7, 96)
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(42, n))
     kw = dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0, tol=1e-8, max_iter=200)
+    modes = {aa.MODE_REFERENCE: 1, aa.MODE_MIXED: 1, aa.MODE_GRAPH: 8}  # mode: poll_every
     cases = {"levels": {"ALIEN_HIP_ILU_SYNC_FREE": "0"}, "sf": {}, "sf1": {"ALIEN_HIP_ILU_SF_BLOCKS": "1"},
              "sf512": {"ALIEN_HIP_ILU_SF_BLOCKS": "512"}, "rpl2": {"ALIEN_HIP_ILU_SF_RPL": "2"},
              "rpl4": {"ALIEN_HIP_ILU_SF_RPL": "4", "ALIEN_HIP_ILU_SF_BLOCKS": "8"},
@@ -926,16 +930,16 @@ def test_levelset_ilu0_sync_free_bit_identical(ctx, monkeypatch):
                 monkeypatch.setenv(k, env.get(k, v))
             c = aa.Context()  # the variables are read when a context is created
             try:
-                for mode in (aa.MODE_REFERENCE, aa.MODE_MIXED):
+                for mode, poll in modes.items():
                     say("case level-set ILU0 sync-free", name, mode)
-                    runs[name, mode] = aa.solve_csr(c, ro, cols, vals, b, mode=mode, **kw)
+                    runs[name, mode] = aa.solve_csr(c, ro, cols, vals, b, mode=mode, poll_every=poll, **kw)
             finally:
                 c.close()
     finally:
         for k, v in defaults.items():
             monkeypatch.setenv(k, v)
         aa.Context().close()
-    for mode in (aa.MODE_REFERENCE, aa.MODE_MIXED):
+    for mode in modes:
         x0, r0 = runs["levels", mode]
         say("  ", mode, r0["iterations"], r0["true_rel_res"])
         assert r0["status"] == 1 and r0["true_rel_res"] < 1e-8
@@ -973,7 +977,8 @@ def test_levelset_ilu0_sync_free_unstructured(ctx, monkeypatch):
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(11, n))
     xo, _ = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-15, max_iter=3)
-    kw = dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0, layout=aa.LAYOUT_CSR, tol=1e-15, max_iter=3)
+    kw = dict(solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0, layout=aa.LAYOUT_CSR, tol=1e-15, max_iter=3,
+              poll_every=1)
     runs = {}
     try:
         monkeypatch.setenv("ALIEN_HIP_ILU_SF_AVG_ROWS", "100000000")
