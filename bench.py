@@ -439,6 +439,7 @@ def run_convdiff(a):
             ("fp64_bicgstab_ilu0_levelset", dict(mode=aa.MODE_REFERENCE, precond=aa.PREC_ILU0, ilu_schedule=0,
                                                  layout=aa.LAYOUT_RBDIA)),
             ("fp64_bicgstab_ilu0_levelset_fused", dict(mode=aa.MODE_FUSED, precond=aa.PREC_ILU0, ilu_schedule=0)),
+            ("fp64_bicgstab_ilu0_levelset_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_ILU0, ilu_schedule=0)),
             ("mixed_bicgstab_ilu0_levelset", dict(mode=aa.MODE_MIXED, precond=aa.PREC_ILU0, ilu_schedule=0)),
             ("fp64_bicgstab_ilu0_multicolour_graph", dict(mode=aa.MODE_GRAPH, precond=aa.PREC_ILU0, ilu_schedule=1)),
             ("mixed_bicgstab_ilu0_multicolour", dict(mode=aa.MODE_MIXED, precond=aa.PREC_ILU0, ilu_schedule=1)),
