@@ -735,6 +735,9 @@ X0_CASES = [
     ("bicgstab filu0 mixed", aa.BICGS, dict(precond=aa.PREC_FILU0, filu_factor_niter=0, filu_solver_niter=3,
                                             mode=aa.MODE_MIXED)),
     ("bicgstab ilu0 level-set graph", aa.BICGS, dict(precond=aa.PREC_ILU0, ilu_schedule=0, mode=aa.MODE_GRAPH)),
+    # natural-order ILU0 with the sync-free sweeps captured in the CG graphs (GRAPH and MIXED)
+    ("cg ilu0 level-set graph", aa.CG, dict(precond=aa.PREC_ILU0, ilu_schedule=0, mode=aa.MODE_GRAPH)),
+    ("cg ilu0 level-set mixed", aa.CG, dict(precond=aa.PREC_ILU0, ilu_schedule=0, mode=aa.MODE_MIXED)),
 ]
 
 
