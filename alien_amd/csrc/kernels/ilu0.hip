@@ -328,13 +328,14 @@ void level_sweep(hipStream_t st, int nlev, const int* ptr, const int* ptr_dev, c
 // One launch per sweep instead of one per level: persistent waves take chunks of at most 64 x rpl
 // positions of one level from an atomic ticket, in level order, each lane working on rpl rows of
 // its chunk in turn, and each row waits for the entries it reads. Those are written and polled
-// with agent-scope relaxed atomics (global loads and stores that bypass the per-XCD L2s), over a
+// with agent-scope relaxed atomics (coherent across the XCDs, whose L2s are not), over a
 // signalling-NaN sentinel that the fill kernel writes first: no flag is needed, a value is ready
 // when it is not the sentinel.
 // Progress: a row only waits on rows of earlier levels, i.e. of chunks taken earlier by waves that
 // are running, and no chunk spans two levels (the rows of a chunk never wait on each other).
 // Every wait is bounded (g_ilu_sf_spins polls); one that runs out raises the error flag, the
-// remaining waits give up at once, and the caller redoes the solve with the level launches.
+// remaining waits give up within 256 polls, and the redo kernel queued after the sweep computes
+// it again level by level.
 // Each row performs the level kernels' operations in their order, so the result is theirs.
 long g_ilu_sf_avg_rows = 32768, g_ilu_sf_blocks = 0, g_ilu_sf_rpl = 0, g_ilu_sf_spins = 1L << 16;
 int g_ilu_sync_free = 1;
