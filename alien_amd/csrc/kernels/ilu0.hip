@@ -173,7 +173,12 @@ long g_ilu_wg_rows = 512;
 int level_sweep_slots(int n, const int* ro, const int* dcol, bool upper) {
   int k = 0;
   for (int i = 0; i < n; ++i) k = std::max(k, upper ? ro[i + 1] - dcol[i] - 1 : dcol[i] - ro[i]);
-  return (k >= 1 && k <= kLevelMaxSlots) ? k : 0;
+  // 1..8 as they are; 9..16 rounded up to 12 or 16 (the extra slots hold column -1), so that
+  // the 27-point stencil's 13 takes the level-ordered copy with two more instantiations, not eight
+  if (k >= 1 && k <= 8) return k;
+  if (k > 8 && k <= 12) return 12;
+  if (k > 12 && k <= kLevelMaxSlots) return 16;
+  return 0;
 }
 
 template <typename T>
@@ -260,16 +265,18 @@ __global__ void __launch_bounds__(kLevelWgBlock) k_level_sweep_run(int l0, int l
                                                                     LevelSweep<T> S, const T* __restrict__ in, T* out,
                                                                     const int* __restrict__ done) {
   if (done && *done) return;
-  constexpr int kChunk = kLevelWgBlock * kLevelBatch;
+  // rows fetched ahead per lane: kLevelBatch, one for the wide sweeps (the registers of 1024 lanes)
+  constexpr int kB = K > 8 ? 1 : kLevelBatch;
+  constexpr int kChunk = kLevelWgBlock * kB;
   const int tid = (int)threadIdx.x;
-  LevelRow<T, UPPER, K> R[kLevelBatch];
+  LevelRow<T, UPPER, K> R[kB];
   int q0 = ptr[l0], q1 = ptr[l0 + 1];
 #pragma unroll
-  for (int j = 0; j < kLevelBatch; ++j)
+  for (int j = 0; j < kB; ++j)
     if (q0 + j * kLevelWgBlock + tid < q1) R[j].fetch(S, q0 + j * kLevelWgBlock + tid);
   for (int l = l0; l < l1; ++l) {
 #pragma unroll
-    for (int j = 0; j < kLevelBatch; ++j)
+    for (int j = 0; j < kB; ++j)
       if (q0 + j * kLevelWgBlock + tid < q1) out[R[j].i] = R[j].value(in, out);
     for (int q = q0 + kChunk + tid; q < q1; q += kLevelWgBlock) { // rows beyond the first chunk
       LevelRow<T, UPPER, K> X;
@@ -280,7 +287,7 @@ __global__ void __launch_bounds__(kLevelWgBlock) k_level_sweep_run(int l0, int l
       q0 = q1;
       q1 = ptr[l + 2];
 #pragma unroll
-      for (int j = 0; j < kLevelBatch; ++j)
+      for (int j = 0; j < kB; ++j)
         if (q0 + j * kLevelWgBlock + tid < q1) R[j].fetch(S, q0 + j * kLevelWgBlock + tid);
     }
     __syncthreads();
@@ -320,6 +327,8 @@ void level_sweep(hipStream_t st, int nlev, const int* ptr, const int* ptr_dev, c
   case 6: level_sweep_k<T, UPPER, 6>(st, nlev, ptr, ptr_dev, S, in, out, done); break;
   case 7: level_sweep_k<T, UPPER, 7>(st, nlev, ptr, ptr_dev, S, in, out, done); break;
   case 8: level_sweep_k<T, UPPER, 8>(st, nlev, ptr, ptr_dev, S, in, out, done); break;
+  case 12: level_sweep_k<T, UPPER, 12>(st, nlev, ptr, ptr_dev, S, in, out, done); break;
+  case 16: level_sweep_k<T, UPPER, 16>(st, nlev, ptr, ptr_dev, S, in, out, done); break;
   default: throw Error(ALIEN_HIP_EINVAL, "level-ordered ILU0 sweep: unsupported slot count");
   }
 }
@@ -491,6 +500,8 @@ void level_sweep_sf(hipStream_t st, int nlev, const int* ptr_dev, const LevelSwe
   case 6: level_sf_k<T, UPPER, 6>(st, nlev, ptr_dev, S, in, out, work, done); break;
   case 7: level_sf_k<T, UPPER, 7>(st, nlev, ptr_dev, S, in, out, work, done); break;
   case 8: level_sf_k<T, UPPER, 8>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 12: level_sf_k<T, UPPER, 12>(st, nlev, ptr_dev, S, in, out, work, done); break;
+  case 16: level_sf_k<T, UPPER, 16>(st, nlev, ptr_dev, S, in, out, work, done); break;
   default: throw Error(ALIEN_HIP_EINVAL, "sync-free ILU0 sweep: unsupported slot count");
   }
 }

@@ -160,7 +160,7 @@ struct LevelSweep {
   const int* cn = nullptr;
   int rpl = 1, blocks = 64;
 };
-constexpr int kLevelMaxSlots = 8;
+constexpr int kLevelMaxSlots = 16;
 // levels of at most this many rows run in one workgroup (runs of two or more);
 // ALIEN_HIP_ILU_WG_ROWS, read when a context is created, overrides it
 extern long g_ilu_wg_rows;
@@ -176,7 +176,8 @@ void level_chunks(int nlev, const int* ptr, int rows, std::vector<int>& cq, std:
 // rows per lane and workgroups of the sync-free sweeps of an n-row schedule of nlev levels
 void level_sf_shape(long n, long nlev, int& rpl, int& blocks);
 // Slot count of a sweep (the largest strictly-lower or strictly-upper row
-// length), from the host CSR; 0 when it exceeds kLevelMaxSlots.
+// length, 9..16 rounded up to 12 or 16), from the host CSR; 0 when it exceeds
+// kLevelMaxSlots.
 int level_sweep_slots(int n, const int* ro_host, const int* dcol_host, bool upper);
 // Fill one sweep's columns (when col is not null), values and diagonal from the
 // factor lu in CSR order (device arrays; row: the level order).

@@ -851,8 +851,9 @@ def laplace27(k):
 @pytest.mark.parametrize("mode", [aa.MODE_REFERENCE, aa.MODE_FUSED])
 def test_levelset_ilu0_wide_rows_iterates_equal_oracle(ctx, mode):
     """Rows with more than 8 strictly-lower or strictly-upper entries (the 27-point Laplacian:
-    13 each) keep the CSR-order level sweeps instead of the level-ordered factor copy: three CG
-    iterations with natural-order ILU(0) give the oracle's iterates to rounding of the dots."""
+    13 each, 78 levels at 12^3) take the level-ordered factor copy with 16 slots (3 of them column
+    -1) and the sync-free sweeps: three CG iterations with natural-order ILU(0) give the oracle's
+    iterates to rounding of the dots."""
     ro, cols, vals = laplace27(12)
     n = len(ro) - 1
     b = orc.spmv(ro, cols, vals, aa.uniform01(11, n))
@@ -952,16 +953,18 @@ def test_levelset_ilu0_sync_free_bit_identical(ctx, monkeypatch):
             np.testing.assert_array_equal(x, x0, err_msg=name)
 
 
-def random_dag_matrix(n, seed):
-    """Unstructured nonsymmetric test matrix: each row has 0-6 random strictly-lower columns within
-    the previous 400 rows and 0-6 random strictly-upper columns within the next 400, values in
-    [-1, 0), and a diagonal 1 above the row's absolute sum (diagonally dominant). Its level sets are
-    ragged: sizes from 1 to several hundred rows, some rows without a lower or upper part."""
+def random_dag_matrix(n, seed, width=6):
+    """Unstructured nonsymmetric test matrix: each row has 0-`width` random strictly-lower columns
+    within the previous 400 rows and 0-`width` random strictly-upper columns within the next 400,
+    values in [-1, 0), and a diagonal 1 above the row's absolute sum (diagonally dominant). Its
+    level sets are ragged: sizes from 1 to several hundred rows, some rows without a lower or upper
+    part."""
     rng = np.random.default_rng(seed)
     ro, cols, vals = [0], [], []
     for i in range(n):
-        lo = sorted(set(rng.integers(max(0, i - 400), i, rng.integers(0, 7)).tolist())) if i else []
-        up = sorted(set(rng.integers(i + 1, min(n, i + 401), rng.integers(0, 7)).tolist())) if i < n - 1 else []
+        lo = sorted(set(rng.integers(max(0, i - 400), i, rng.integers(0, width + 1)).tolist())) if i else []
+        up = (sorted(set(rng.integers(i + 1, min(n, i + 401), rng.integers(0, width + 1)).tolist()))
+              if i < n - 1 else [])
         off = -rng.random(len(lo) + len(up)) - 1e-3
         cols += lo + [i] + up
         vals += list(off[:len(lo)]) + [float(np.abs(off).sum()) + 1.0] + list(off[len(lo):])
@@ -1010,6 +1013,25 @@ def test_levelset_ilu0_sync_free_unstructured(ctx, monkeypatch):
     x, r = runs["1", aa.MODE_REFERENCE]
     assert r["iterations"] == 3
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-11 * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("width", [11, 24])
+def test_levelset_ilu0_slot_widths_iterates_equal_oracle(ctx, width):
+    """Unstructured matrices whose longest strictly-lower / strictly-upper row part is up to 11
+    (the level-ordered copy with 12 slots, sync-free sweeps) or up to 24 (more than 16: the
+    CSR-order per-level kernels): three BiCGStab iterations with natural-order ILU(0) in the
+    REFERENCE and FUSED modes give the oracle's iterates to rounding of the dots."""
+    ro, cols, vals = random_dag_matrix(6000, 3, width)
+    n = len(ro) - 1
+    b = orc.spmv(ro, cols, vals, aa.uniform01(11, n))
+    xo, _ = orc.solve(ro, cols, vals, b, solver=orc.ORC_BICGS, precond=orc.PREC_ILU0, tol=1e-15, max_iter=3)
+    for mode in (aa.MODE_REFERENCE, aa.MODE_FUSED):
+        say("case level-set ILU0 slot width", width, mode)
+        x, r = aa.solve_csr(ctx, ro, cols, vals, b, solver=aa.BICGS, precond=aa.PREC_ILU0, ilu_schedule=0,
+                            mode=mode, layout=aa.LAYOUT_CSR, tol=1e-15, max_iter=3)
+        say("  ", r["iterations"], np.abs(x - xo).max())
+        assert r["iterations"] == 3
+        np.testing.assert_allclose(x, xo, rtol=0, atol=1e-11 * np.abs(xo).max())
 
 
 KEPT_CASES = [
