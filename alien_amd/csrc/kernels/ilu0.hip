@@ -440,9 +440,34 @@ __global__ void __launch_bounds__(kBlock) k_level_sf(LevelSweep<T> S, const T* _
     for (int r = 0; r < RPL; ++r)
       if (lane + r * kWave < m) {
         T v = in[R[r].i];
+        if constexpr (K > 4) {
+          // Many entries per row: wait first for the nearest one (the highest column of the lower
+          // part, the lowest of the upper part: the latest level it reads), after which the others
+          // are almost always there, then read those together; the sum keeps the slot order.
+          int sn = -1;
 #pragma unroll
-        for (int s = 0; s < K; ++s)
-          if (R[r].c[s] >= 0) v = v - R[r].a[s] * sf_wait<T>(out + R[r].c[s], err, limit);
+          for (int s = 0; s < K; ++s)
+            if (R[r].c[s] >= 0 && (!UPPER || sn < 0)) sn = s;
+          U b[K];
+#pragma unroll
+          for (int s = 0; s < K; ++s)
+            if (s == sn) b[s] = SfBits<T>::bits(sf_wait<T>(out + R[r].c[s], err, limit));
+#pragma unroll
+          for (int s = 0; s < K; ++s)
+            if (s != sn && R[r].c[s] >= 0)
+              b[s] = __hip_atomic_load(reinterpret_cast<const U*>(out) + R[r].c[s], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+          for (int s = 0; s < K; ++s)
+            if (R[r].c[s] >= 0) {
+              if (b[s] == SfBits<T>::kSentinel) b[s] = SfBits<T>::bits(sf_wait<T>(out + R[r].c[s], err, limit));
+              v = v - R[r].a[s] * SfBits<T>::value(b[s]);
+            }
+        } else {
+#pragma unroll
+          for (int s = 0; s < K; ++s)
+            if (R[r].c[s] >= 0) v = v - R[r].a[s] * sf_wait<T>(out + R[r].c[s], err, limit);
+        }
         if (UPPER) v = v / R[r].d;
         __hip_atomic_store(reinterpret_cast<U*>(out) + R[r].i, SfBits<T>::bits(v), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
