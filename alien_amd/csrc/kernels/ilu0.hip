@@ -440,10 +440,10 @@ __global__ void __launch_bounds__(kBlock) k_level_sf(LevelSweep<T> S, const T* _
     for (int r = 0; r < RPL; ++r)
       if (lane + r * kWave < m) {
         T v = in[R[r].i];
-        if constexpr (K > 4) {
-          // Many entries per row: wait first for the nearest one (the highest column of the lower
-          // part, the lowest of the upper part: the latest level it reads), after which the others
-          // are almost always there, then read those together; the sum keeps the slot order.
+        if constexpr (K > 1) {
+          // Wait first for the nearest entry (the highest column of the lower part, the lowest of
+          // the upper part: the latest level the row reads), after which the others are almost
+          // always there, then read those together; the sum keeps the slot order.
           int sn = -1;
 #pragma unroll
           for (int s = 0; s < K; ++s)
