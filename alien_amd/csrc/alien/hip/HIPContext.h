@@ -8,9 +8,12 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../kernels/device_common.h"
 #include "../../kernels/krylov_state.h"
@@ -35,7 +38,63 @@ inline void check_launch(const char* what) {
   if (e != hipSuccess) throw Error((int)e, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Owning device array.
+// Freed device blocks of 1 MiB or more, kept by device and size for the next allocation of that
+// size: the solvers' work vectors are created and destroyed with each solve, and mapping and
+// unmapping device memory each time costs milliseconds. A block is kept only after a device-wide
+// synchronisation (what hipFree does), so no queued work can still use it when it is handed out
+// again; at most 16 GiB are kept per process, and a failed hipMalloc empties the cache and tries
+// once more. The cache is never destroyed (blocks are released with the process).
+class BlockCache {
+ public:
+  static BlockCache& get() {
+    static BlockCache* c = new BlockCache();
+    return *c;
+  }
+  void* take(int dev, size_t bytes) {
+    std::lock_guard<std::mutex> g(m_);
+    auto it = free_.find({dev, bytes});
+    if (it == free_.end() || it->second.empty()) return nullptr;
+    void* p = it->second.back();
+    it->second.pop_back();
+    kept_ -= bytes;
+    return p;
+  }
+  // true when the block was kept (else the caller frees it)
+  bool give(int dev, void* p, size_t bytes) {
+    if (bytes < kMin) return false;
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != dev) return false;
+    {
+      std::lock_guard<std::mutex> g(m_);
+      if (kept_ + bytes > kMax) return false;
+    }
+    if (hipDeviceSynchronize() != hipSuccess) return false;
+    std::lock_guard<std::mutex> g(m_);
+    if (kept_ + bytes > kMax) return false;
+    free_[{dev, bytes}].push_back(p);
+    kept_ += bytes;
+    return true;
+  }
+  void release(int dev) {
+    std::lock_guard<std::mutex> g(m_);
+    for (auto& e : free_)
+      if (e.first.first == dev) {
+        for (void* p : e.second) {
+          (void)hipFree(p);
+          kept_ -= e.first.second;
+        }
+        e.second.clear();
+      }
+  }
+
+ private:
+  static constexpr size_t kMin = size_t(1) << 20, kMax = size_t(16) << 30;
+  std::mutex m_;
+  std::map<std::pair<int, size_t>, std::vector<void*>> free_;
+  size_t kept_ = 0;
+};
+
+// Owning device array (its blocks of 1 MiB or more go through BlockCache).
 template <typename T>
 class DeviceArray {
  public:
@@ -44,18 +103,34 @@ class DeviceArray {
   ~DeviceArray() { reset(); }
   DeviceArray(const DeviceArray&) = delete;
   DeviceArray& operator=(const DeviceArray&) = delete;
-  DeviceArray(DeviceArray&& o) noexcept : m_p(o.m_p), m_n(o.m_n) { o.m_p = nullptr; o.m_n = 0; }
+  DeviceArray(DeviceArray&& o) noexcept : m_p(o.m_p), m_n(o.m_n), m_dev(o.m_dev) { o.m_p = nullptr; o.m_n = 0; }
   DeviceArray& operator=(DeviceArray&& o) noexcept {
-    if (this != &o) { reset(); m_p = o.m_p; m_n = o.m_n; o.m_p = nullptr; o.m_n = 0; }
+    if (this != &o) { reset(); m_p = o.m_p; m_n = o.m_n; m_dev = o.m_dev; o.m_p = nullptr; o.m_n = 0; }
     return *this;
   }
   void resize(size_t n) {
     reset();
-    if (n) ALIEN_HIP_CHECK(hipMalloc((void**)&m_p, n * sizeof(T)));
+    if (n) {
+      const size_t bytes = n * sizeof(T);
+      ALIEN_HIP_CHECK(hipGetDevice(&m_dev));
+      m_p = static_cast<T*>(BlockCache::get().take(m_dev, bytes));
+      if (!m_p) {
+        hipError_t e = hipMalloc((void**)&m_p, bytes);
+        if (e == hipErrorOutOfMemory) {
+          (void)hipGetLastError();
+          BlockCache::get().release(m_dev);
+          e = hipMalloc((void**)&m_p, bytes);
+        }
+        if (e != hipSuccess) {
+          m_p = nullptr;
+          ALIEN_HIP_CHECK(e);
+        }
+      }
+    }
     m_n = n;
   }
   void reset() {
-    if (m_p) (void)hipFree(m_p);
+    if (m_p && !BlockCache::get().give(m_dev, m_p, m_n * sizeof(T))) (void)hipFree(m_p);
     m_p = nullptr;
     m_n = 0;
   }
@@ -75,6 +150,7 @@ class DeviceArray {
  private:
   T* m_p = nullptr;
   size_t m_n = 0;
+  int m_dev = 0;
 };
 
 } // namespace alien_hip
