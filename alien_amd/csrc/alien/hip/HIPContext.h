@@ -172,6 +172,32 @@ struct alien_hip_ctx {
   // zeroed CG state (done = 0, never written): the fused iteration's kernels launched on their own
   alien_hip::DeviceArray<alien_hip::CGState> idle_cg;
   double* host_scalars = nullptr;          // pinned mirror of `scalars`
+  // Pinned 512-byte blocks for the fused solvers' host state mirrors, allocated once per context
+  // (pinned allocation and release cost far more than a solve's own host work); a solver takes a
+  // free block and gives it back, or allocates its own when all are taken.
+  static constexpr int kPinnedBlocks = 16, kPinnedBytes = 512;
+  char* pinned_blocks = nullptr;
+  unsigned pinned_used = 0;
+  void* take_pinned() {
+    if (!pinned_blocks &&
+        hipHostMalloc((void**)&pinned_blocks, (size_t)kPinnedBlocks * kPinnedBytes, hipHostMallocDefault) != hipSuccess) {
+      pinned_blocks = nullptr;
+      return nullptr;
+    }
+    for (int i = 0; i < kPinnedBlocks; ++i)
+      if (!(pinned_used & (1u << i))) {
+        pinned_used |= 1u << i;
+        return pinned_blocks + (size_t)i * kPinnedBytes;
+      }
+    return nullptr;
+  }
+  // false when p is not one of the blocks (the caller frees it)
+  bool give_pinned(void* p) {
+    if (!pinned_blocks || (char*)p < pinned_blocks || (char*)p >= pinned_blocks + kPinnedBlocks * kPinnedBytes)
+      return false;
+    pinned_used &= ~(1u << (int)(((char*)p - pinned_blocks) / kPinnedBytes));
+    return true;
+  }
   int next_slot = 0;
   static constexpr int kSlots = 64;
 

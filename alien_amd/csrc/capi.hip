@@ -57,6 +57,7 @@ void destroy_ctx(alien_hip_ctx* c) {
   c->scalars.reset();
   c->idle_cg.reset();
   if (c->host_scalars) (void)hipHostFree(c->host_scalars);
+  if (c->pinned_blocks) (void)hipHostFree(c->pinned_blocks);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

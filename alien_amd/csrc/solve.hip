@@ -358,12 +358,14 @@ struct FusedCG {
     if (P.kind == kPrecFILU0) tmp2.resize(n);
     if (P.kind == kPrecILU0) ctx->reserve_red(ilu0_split_partials((int)P.cptr.size() - 1, P.cptr.data()));
     st.resize(1);
-    ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
+    static_assert(sizeof(CGState) <= alien_hip_ctx::kPinnedBytes, "state mirror exceeds a pinned block");
+    h_st = static_cast<CGState*>(ctx->take_pinned());
+    if (!h_st) ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
   }
   ~FusedCG() {
     release_graph();
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
-    if (h_st) (void)hipHostFree(h_st);
+    if (h_st && !ctx->give_pinned(h_st)) (void)hipHostFree(h_st);
   }
 
   // Step-kernel timing (ctx->step_timing): iteration slot i of an advance launches its q = A p
@@ -748,11 +750,13 @@ struct FusedBiCG {
     if (P.kind == kPrecFILU0 || P.kind == kPrecCheb) tmp2.resize(n);
     if (P.kind == kPrecCheb) cp.resize(n);
     st.resize(1);
-    ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(BiCGState), hipHostMallocDefault));
+    static_assert(sizeof(BiCGState) <= alien_hip_ctx::kPinnedBytes, "state mirror exceeds a pinned block");
+    h_st = static_cast<BiCGState*>(ctx->take_pinned());
+    if (!h_st) ALIEN_HIP_CHECK(hipHostMalloc((void**)&h_st, sizeof(BiCGState), hipHostMallocDefault));
   }
   ~FusedBiCG() {
     release_graph();
-    if (h_st) (void)hipHostFree(h_st);
+    if (h_st && !ctx->give_pinned(h_st)) (void)hipHostFree(h_st);
   }
   const int* done_flag() { return &st.data()->done; }
   void precond(const T* in, T* out) {
